@@ -1,0 +1,140 @@
+/*
+ * vsmc_oracle.h -- CPU restatement of vSMC's per-iteration particle step.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker for the HIP product
+ * path: only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg
+ * may load it.  Nothing in vsmc_amd/ links or calls it.
+ *
+ * Every function restates the reference algorithm it names (file:line in the
+ * vSMC tree) in plain C, FP64, sequential where vSMC is sequential, with the
+ * move loop parallelised by OpenMP the way MoveOMP does
+ * (include/vsmc/smp/backend_omp.hpp:104-117).  Build flags must keep
+ * -ffp-contract=off so that the uniform grids u0 + n*delta round as vSMC's
+ * x86-64 build does.
+ *
+ * Parity pins (see tests/test_oracle.py):
+ *   - engine blocks / streams / discard  : golden vectors written by the
+ *     compiled vSMC headers (tests/golden/ref_vectors.json) + Random123 KATs
+ *   - generate_canonical, U01Sequence*    : golden vectors from the compiled
+ *     headers (libstdc++ 11 uniform_real_distribution)
+ *   - inversion / cfrp_trans / 6 schemes  : golden vectors (replication,
+ *     copy_from, engine state after) + brute-force invariants
+ *   - WeightSet normalisation / ESS       : golden vectors (bit-exact)
+ *   - fold_prefix                         : definition (left fold) + exact
+ *     rational check on tiny inputs
+ *   - CV particle filter run              : golden run driven through
+ *     vsmc::Particle<T>, WeightSet and Resample<Systematic>
+ *   - GMM tempered SMC                    : parity unpinned beyond its pieces
+ *     (log-likelihood closed form, CESS definition); see DESIGN.md
+ *   - Kalman filter                       : closed form, checked against a
+ *     brute-force Gaussian marginal on tiny T
+ */
+#ifndef VSMC_ORACLE_H
+#define VSMC_ORACLE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------- counter-based RNG blocks ---------------- */
+void orc_philox4x32_block(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+void orc_threefry4x64_block(const uint64_t ctr[4], const uint64_t key[4], uint64_t out[4]);
+void orc_threefry4x32_block(const uint32_t ctr[4], const uint32_t key[4], uint32_t out[4]);
+
+/* ---------------- engines (ctr, key, buffer, index) ---------------- */
+typedef struct { uint64_t ctr[4]; uint64_t key[4]; uint64_t buf[4]; uint32_t index; } orc_tf64_engine;
+typedef struct { uint32_t ctr[4]; uint32_t key[4]; uint32_t buf[4]; uint32_t index; } orc_tf32_engine;
+typedef struct { uint32_t ctr[4]; uint32_t key[2]; uint32_t buf[4]; uint32_t index; } orc_ph32_engine;
+
+void orc_tf64_seed(orc_tf64_engine *e, uint64_t s);
+uint64_t orc_tf64_next(orc_tf64_engine *e);
+void orc_tf64_discard(orc_tf64_engine *e, uint64_t n);
+void orc_tf32_seed(orc_tf32_engine *e, uint32_t s);
+uint32_t orc_tf32_next(orc_tf32_engine *e);
+void orc_ph32_seed(orc_ph32_engine *e, uint32_t s);
+uint32_t orc_ph32_next(orc_ph32_engine *e);
+void orc_ph32_discard(orc_ph32_engine *e, uint64_t n);
+
+/* engine-stream helpers for tests: n successive draws */
+void orc_tf64_stream(orc_tf64_engine *e, size_t n, uint64_t *out);
+void orc_ph32_stream(orc_ph32_engine *e, size_t n, uint32_t *out);
+void orc_tf32_stream(orc_tf32_engine *e, size_t n, uint32_t *out);
+
+/* ---------------- u01 ---------------- */
+float orc_u01_open_closed_32_24(uint32_t u);
+double orc_u01_open_closed_32_53(uint32_t u);
+double orc_u01_closed_open_64_53(uint64_t u);
+/* std::uniform_real_distribution<double>(0,1) on a 64-bit / 32-bit engine */
+double orc_canonical_tf64(orc_tf64_engine *e);
+double orc_canonical_ph32(orc_ph32_engine *e);
+
+/* ---------------- resampling ---------------- */
+enum {
+    ORC_MULTINOMIAL = 0,
+    ORC_RESIDUAL = 1,
+    ORC_STRATIFIED = 2,
+    ORC_SYSTEMATIC = 3,
+    ORC_RESIDUAL_STRATIFIED = 4,
+    ORC_RESIDUAL_SYSTEMATIC = 5
+};
+
+/* left fold a_i = fl(a_{i-1} + x_i), a_{-1} = 0 */
+void orc_fold_prefix(size_t n, const double *x, double *out);
+double orc_asum(size_t n, const double *x);
+double orc_dot(size_t n, const double *x, const double *y);
+
+/* inversion over an explicit uniform array; returns the number of uniforms
+ * examined (the lazy-sequence draw count) */
+size_t orc_inversion_u(size_t M, size_t N, const double *w, const double *u, uint64_t *rep);
+void orc_cfrp_trans(size_t M, size_t N, const uint64_t *rep, uint64_t *copy_from);
+
+/* Resample<Scheme>::operator()(M, N, rng, w, rep); returns draws made */
+size_t orc_resample(int scheme, size_t M, size_t N, orc_tf64_engine *eng,
+                    const double *w, uint64_t *rep);
+/* the uniform sequence a scheme would produce, fully drawn (N values) */
+void orc_u01seq(int kind /*0 sorted,1 stratified,2 systematic*/, size_t N,
+                orc_tf64_engine *eng, double *u);
+/* residual split: integral parts, scaled residuals (rr), and NN = N - R */
+size_t orc_residual_split(size_t M, size_t N, const double *w, uint64_t *integral, double *rr);
+
+/* ---------------- WeightSet ---------------- */
+/* set_log_weight: lw normalised in place (max subtracted), w out, ESS returned */
+double orc_weightset_set_log_weight(size_t N, double *lw, double *w);
+double orc_weightset_add_log_weight(size_t N, double *lw, const double *inc, double *w);
+
+/* ---------------- CV particle filter (config 0/1 model) ---------------- */
+typedef struct {
+    double dt, sd_pos, sd_vel, prior_sd_pos, prior_sd_vel;
+    double obs_scale, nu, llh_const; /* llh_const = per-coordinate log normaliser */
+} orc_cv_param;
+
+typedef struct {
+    double ess, log_z, mean_x, mean_y;
+    int resampled;
+    uint64_t draws;
+} orc_step_record;
+
+/* one FP64 Box-Muller block: 4 normals from philox(ctr={t+1,id_lo,id_hi,0}, key=seed) */
+void orc_cv_normals(uint64_t seed, uint64_t id, uint64_t t, double z[4]);
+double orc_cv_llh(const orc_cv_param *p, double x, double y, double ox, double oy);
+
+/* Full filter run, FP64, vSMC step order (core/sampler.hpp:773-834).
+ * state: N x 4 row-major (x, y, vx, vy) out; lw: N out; rec: T records.
+ * obs: T x 2.  threads <= 0 -> OpenMP default. */
+int orc_cv_run(const orc_cv_param *p, size_t N, size_t T, const double *obs,
+               uint64_t seed, uint64_t resample_seed, int scheme, double threshold,
+               double *state, double *lw, orc_step_record *rec, int threads);
+
+/* timing-oriented variant of orc_cv_run: same arithmetic, returns per-phase
+ * seconds in phase[4] = {move, normalise, resample, copy} */
+int orc_cv_run_timed(const orc_cv_param *p, size_t N, size_t T, const double *obs,
+                     uint64_t seed, uint64_t resample_seed, int scheme, double threshold,
+                     orc_step_record *rec, int threads, double phase[4]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
