@@ -1,0 +1,167 @@
+/*
+ * vsmcx.h -- C ABI of the MI355X-native vSMC particle hot path.
+ *
+ * Plain C types only (pointers, sizes, fixed-width integers); every function
+ * returns VSMCX_OK (0) or a negative error code and leaves a message for
+ * vsmcx_last_error() (thread-local).  No exception crosses this boundary.
+ * A context owns one HIP stream on one device and is not thread-safe: use
+ * one context per host thread.
+ *
+ * Each entry names the vSMC interface it replaces (paths in the vSMC tree).
+ */
+#ifndef VSMCX_H
+#define VSMCX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- errors */
+#define VSMCX_OK 0
+#define VSMCX_ERR_ARG -1          /* invalid argument */
+#define VSMCX_ERR_HIP -2          /* HIP runtime failure */
+#define VSMCX_ERR_DEGENERATE -3   /* weights sum to zero or are not finite */
+#define VSMCX_ERR_BUDGET -4       /* device-memory budget exceeded */
+#define VSMCX_ERR_INTERNAL -5     /* device-side consistency check failed */
+#define VSMCX_ERR_STATE -6        /* call out of sequence */
+
+/* Resampling schemes: same values as vsmc::ResampleScheme
+ * (include/vsmc/resample/common.hpp:124-131). */
+#define VSMCX_MULTINOMIAL 0
+#define VSMCX_RESIDUAL 1
+#define VSMCX_STRATIFIED 2
+#define VSMCX_SYSTEMATIC 3
+#define VSMCX_RESIDUAL_STRATIFIED 4
+#define VSMCX_RESIDUAL_SYSTEMATIC 5
+
+/* Precision / pipeline of a filter. */
+#define VSMCX_MIXED 0       /* FP32 state + likelihood, FP64 normaliser/scan/log Z, fused */
+#define VSMCX_FP64_FUSED 1  /* FP64 everywhere, fused kernels */
+#define VSMCX_FP64_PLAIN 2  /* FP64, one kernel per vSMC operation (baseline) */
+
+const char *vsmcx_last_error(void);
+const char *vsmcx_version(void);
+
+/* ---------------------------------------------------------------- context */
+typedef struct vsmcx_ctx vsmcx_ctx;
+/* device ordinal; hbm_budget bytes this context may allocate (0 = no limit) */
+int vsmcx_ctx_create(int device, uint64_t hbm_budget, vsmcx_ctx **out);
+int vsmcx_ctx_destroy(vsmcx_ctx *ctx);
+int vsmcx_ctx_sync(vsmcx_ctx *ctx);
+/* bytes currently allocated by the context */
+int vsmcx_ctx_allocated(vsmcx_ctx *ctx, uint64_t *bytes);
+
+/* ---------------------------------------------------------------- RNG
+ * Bulk block generation: block k (k = 0 .. nblocks-1) is the engine's
+ * block at counter ctr0 + 1 + k, i.e. what nblocks*4 successive
+ * operator() calls return after eng.ctr(ctr0) (include/vsmc/rng/philox.hpp:
+ * 374-395, include/vsmc/rng/threefry.hpp:343-364).  `out` is host memory of
+ * 4*nblocks words.  Keys are the engine key() words (Threefry: key(), not par). */
+int vsmcx_philox4x32_fill(vsmcx_ctx *ctx, const uint32_t key[2], const uint32_t ctr0[4],
+                          uint64_t nblocks, uint32_t *out);
+int vsmcx_threefry4x64_fill(vsmcx_ctx *ctx, const uint64_t key[4], const uint64_t ctr0[4],
+                            uint64_t nblocks, uint64_t *out);
+int vsmcx_threefry4x32_fill(vsmcx_ctx *ctx, const uint32_t key[4], const uint32_t ctr0[4],
+                            uint64_t nblocks, uint32_t *out);
+
+/* Threefry4x64 engine state as the resample engine holds it
+ * (include/vsmc/rng/threefry.hpp:512-516: ctr_, par_ (first 4 words = key),
+ * buffer_, index_). */
+typedef struct {
+    uint64_t ctr[4];
+    uint64_t key[4];
+    uint64_t buffer[4];
+    uint64_t index;
+} vsmcx_threefry4x64_state;
+
+/* ---------------------------------------------------------------- resample
+ * vsmcx_resample replaces the resample operator vSMC stores as
+ *   std::function<void(size_t M, size_t N, Threefry4x64 &rng,
+ *                      const double *weight, size_t *replication)>
+ * (include/vsmc/core/particle.hpp:61-62; Resample<Scheme>::operator() in
+ * include/vsmc/resample/*.hpp:53-77).  Host pointers, synchronous.  `eng`
+ * is advanced by exactly the draws vSMC makes.  `replication` receives M
+ * counts summing to N.  When copy_from is not NULL (requires M == N) it also
+ * receives internal::cfrp_trans's layout (include/vsmc/resample/common.hpp:
+ * 87-118), i.e. what Particle::resample (core/particle.hpp:230-250) uses. */
+int vsmcx_resample(vsmcx_ctx *ctx, int scheme, uint64_t M, uint64_t N,
+                   vsmcx_threefry4x64_state *eng, const double *weight,
+                   uint64_t *replication, uint64_t *copy_from);
+
+/* Exact left fold out[i] = fl(out[i-1] + x[i]) of non-negative x, computed on
+ * the device (the running sum inside include/vsmc/resample/common.hpp:76-77
+ * and math::asum, include/vsmc/math/cblas.hpp:53-63).  Host pointers. */
+int vsmcx_fold_prefix(vsmcx_ctx *ctx, uint64_t n, const double *x, double *out);
+
+/* ---------------------------------------------------------------- filter
+ * Device-resident bootstrap particle filter for the constant-velocity model
+ * of example/pf (state x, y, vx, vy; Student-t position likelihood).  One
+ * step replays Sampler::do_iter (include/vsmc/core/sampler.hpp:783-834):
+ * move + log-likelihood -> WeightSet::add_log_weight (core/weight_set.hpp:
+ * 324-331) -> ESS test and Particle::resample (core/particle.hpp:230-250)
+ * -> monitor at MonitorMCMC stage.  Step 0 is the init step
+ * (core/sampler.hpp:773-781). */
+typedef struct vsmcx_filter vsmcx_filter;
+
+typedef struct {
+    uint64_t n;            /* particles */
+    uint64_t horizon;      /* observation capacity (steps) */
+    int precision;         /* VSMCX_MIXED / VSMCX_FP64_FUSED / VSMCX_FP64_PLAIN */
+    int scheme;            /* VSMCX_* resampling scheme */
+    double threshold;      /* resample when ESS < threshold * n */
+    uint64_t seed;         /* Philox4x32 key of the move stream */
+    uint64_t resample_seed;/* Threefry4x64 seed of the resample engine */
+    /* model */
+    double dt, sd_pos, sd_vel, prior_sd_pos, prior_sd_vel;
+    double obs_scale, nu, llh_const; /* llh_const: per-coordinate log normaliser */
+} vsmcx_filter_desc;
+
+typedef struct {
+    double ess;       /* ESS before the resampling decision */
+    double log_z;     /* accumulated log-evidence */
+    double mean_x;    /* filter mean at MonitorMCMC stage */
+    double mean_y;
+    int32_t resampled;
+    int32_t pad;
+    uint64_t draws;   /* resample-engine draws this step */
+} vsmcx_step_stats;
+
+int vsmcx_filter_create(vsmcx_ctx *ctx, const vsmcx_filter_desc *desc, vsmcx_filter **out);
+int vsmcx_filter_destroy(vsmcx_filter *f);
+/* observations: T x 2 doubles (x, y), T <= horizon */
+int vsmcx_filter_set_obs(vsmcx_filter *f, uint64_t T, const double *obs);
+/* run steps [t0, t0 + nsteps): enqueued on the context stream, no host sync */
+int vsmcx_filter_run(vsmcx_filter *f, uint64_t nsteps);
+/* finish: evaluate the monitor of the last step, synchronise, check errors */
+int vsmcx_filter_finish(vsmcx_filter *f);
+/* steps completed so far */
+int vsmcx_filter_steps(vsmcx_filter *f, uint64_t *t);
+/* history rows [0, count) (after finish) */
+int vsmcx_filter_history(vsmcx_filter *f, uint64_t count, vsmcx_step_stats *out);
+/* current state as n x 4 doubles (x, y, vx, vy), log-weights (n), resample
+ * engine; the state is the post-resample state of the last completed step */
+int vsmcx_filter_state(vsmcx_filter *f, double *state, double *log_weight);
+int vsmcx_filter_engine(vsmcx_filter *f, vsmcx_threefry4x64_state *eng);
+
+/* Timing of the run's device work with HIP events on the context stream:
+ * runs nsteps steps and reports elapsed milliseconds, and the average
+ * duration of the fused move kernel (or the plain move kernel). */
+int vsmcx_filter_time(vsmcx_filter *f, uint64_t nsteps, double *elapsed_ms,
+                      double *move_kernel_ms);
+
+/* ---------------------------------------------------------------- debug
+ * Hooks for parity tests: run one step but stop after the resampling
+ * decision and expose the normalised weights W (n doubles) the resampler
+ * reads; then resample and expose replication / copy_from.  Sorted-uniform
+ * schemes also expose the uniform array the device inverted (nn values). */
+int vsmcx_filter_debug_step(vsmcx_filter *f, double *weight, uint64_t *replication,
+                            uint64_t *copy_from, double *uniforms, uint64_t *nn,
+                            vsmcx_threefry4x64_state *eng_before, int *resampled);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* VSMCX_H */

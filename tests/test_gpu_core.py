@@ -1,0 +1,256 @@
+"""GPU parity of the HIP path against the CPU oracle, through the C ABI.
+
+Bit-exact: RNG blocks, the exact left fold, replication counts, cfrp_trans
+ancestor layout and resample-engine state for all six schemes.
+Tolerance (stated per assertion): filter log-weights, ESS, log-evidence, means.
+"""
+import ctypes as C
+import sys
+
+import numpy as np
+import pytest
+
+import vsmc_amd as V
+from oracle_ctypes import cv_param
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = V.Context(0)
+    yield c
+    c.close()
+
+
+# ---------------------------------------------------------------- RNG
+def test_philox_fill(ctx, oracle):
+    say("philox fill")
+    rng = np.random.default_rng(0)
+    for _ in range(8):
+        key = rng.integers(0, 2**32, 2, dtype=np.uint64).astype(np.uint32)
+        ctr = rng.integers(0, 2**32, 4, dtype=np.uint64).astype(np.uint32)
+        ctr[0] = 2**32 - 3  # carry into word 1 inside the fill
+        out = ctx.philox4x32_fill(key, ctr, 257)
+        for k in (0, 1, 2, 3, 100, 256):
+            c = ctr.astype(np.uint64).copy()
+            v = int(c[0]) + k + 1
+            c0, carry = v % 2**32, v // 2**32
+            want = oracle.philox4x32_block([c0, int(c[1]) + carry, c[2], c[3]], key)
+            assert (out[k] == want).all(), k
+
+
+def test_philox_fill_matches_engine_stream(ctx, oracle):
+    out = ctx.philox4x32_fill([1234, 0], [0, 0, 0, 0], 1000)
+    assert (out.reshape(-1) == oracle.ph32_stream(1234, 4000)).all()
+
+
+def test_threefry_fills(ctx, oracle):
+    say("threefry fills")
+    out = ctx.threefry4x64_fill([77, 0, 0, 0], [0, 0, 0, 0], 1000)
+    assert (out.reshape(-1) == oracle.tf64_stream(77, 4000)).all()
+    out = ctx.threefry4x32_fill([77, 0, 0, 0], [0, 0, 0, 0], 1000)
+    assert (out.reshape(-1) == oracle.tf32_stream(77, 4000)).all()
+    ctr = [2**64 - 2, 2**64 - 1, 3, 0]
+    out = ctx.threefry4x64_fill([5, 6, 7, 8], ctr, 5)
+    for k in range(5):
+        v = ctr[0] + k + 1
+        c = [v % 2**64, (ctr[1] + v // 2**64) % 2**64, ctr[2] + ((ctr[1] + v // 2**64) // 2**64),
+             0]
+        assert (out[k] == oracle.threefry4x64_block(c, [5, 6, 7, 8])).all()
+
+
+# ---------------------------------------------------------------- fold
+def _weights(kind, n, rng):
+    if kind == "flat":
+        w = rng.random(n)
+    elif kind == "heavy":
+        w = np.exp(3 * rng.standard_normal(n))
+    elif kind == "zeros":
+        w = rng.random(n) * (rng.random(n) < 0.3)
+        w[0] = 0.0
+    elif kind == "degenerate":  # a few heavy particles, a sea of ~1e-16 weights
+        w = np.full(n, 1e-16) * rng.random(n)
+        w[rng.integers(0, n // 3, 5)] = 1.0
+    elif kind == "ties":
+        w = np.ldexp(1.0, -rng.integers(0, 60, n).astype(np.int64))
+    elif kind == "range":
+        w = np.exp(20 * rng.standard_normal(n))
+    else:
+        raise ValueError(kind)
+    if w.sum() == 0:
+        w[-1] = 1.0
+    return w / w.sum() if kind != "ties" else w
+
+
+@pytest.mark.parametrize("kind", ["flat", "heavy", "zeros", "degenerate", "ties", "range"])
+def test_fold_prefix_exact(ctx, oracle, kind):
+    rng = np.random.default_rng(hash(kind) % 2**32)
+    for n in (1, 2, 3, 4095, 4096, 4097, 50000, 1 << 20):
+        say("fold", kind, n)
+        x = _weights(kind, n, rng)
+        got = ctx.fold_prefix(x)
+        want = oracle.fold_prefix(x)
+        bad = np.flatnonzero(got != want)
+        assert bad.size == 0, (kind, n, bad[:5], got[bad[:5]], want[bad[:5]])
+
+
+# ---------------------------------------------------------------- resample
+SIZES = (1, 2, 3, 17, 4096, 4097, 30000)
+
+
+def _engine_equal(eng, oeng):
+    """Device engine state (ctr, key, index, buffer) equals the oracle engine's."""
+    assert list(eng.ctr) == list(oeng.ctr)
+    assert list(eng.key) == list(oeng.key)
+    assert int(eng.index) == int(oeng.index)
+    if eng.index < 4:
+        assert list(eng.buffer) == list(oeng.buf)
+
+
+@pytest.mark.parametrize("scheme", range(6))
+def test_resample_compat_bitexact(ctx, oracle, scheme):
+    rng = np.random.default_rng(100 + scheme)
+    for n in SIZES:
+        for kind in ("flat", "heavy", "zeros", "degenerate"):
+            say("resample", V.SCHEMES and scheme, n, kind)
+            w = _weights(kind, n, rng)
+            seed = int(rng.integers(1, 2**31))
+            eng = V.ThreefryState.seeded(seed)
+            rep, cf = ctx.resample(scheme, w, eng)
+            orep, draws, oeng = oracle.resample(scheme, w, seed)
+            assert (rep == orep).all(), (scheme, n, kind)
+            assert (cf == oracle.cfrp_trans(orep)).all(), (scheme, n, kind)
+            _engine_equal(eng, oeng)
+
+
+@pytest.mark.parametrize("scheme", range(6))
+def test_resample_engine_continues(ctx, oracle, scheme):
+    """Several resamples in a row on one engine (buffer index carried across)."""
+    rng = np.random.default_rng(7 + scheme)
+    eng = V.ThreefryState.seeded(99)
+    oeng = oracle.tf64_engine(99)
+    for r in range(6):
+        n = int(rng.integers(1, 3000))
+        w = _weights("heavy", n, rng)
+        rep, cf = ctx.resample(scheme, w, eng)
+        orep, _, oeng = oracle.resample(scheme, w, engine=oeng)
+        assert (rep == orep).all(), (scheme, r, n)
+        _engine_equal(eng, oeng)
+
+
+@pytest.mark.parametrize("M,N", [(10, 3), (3, 10), (1000, 4100), (5000, 17)])
+def test_resample_m_ne_n(ctx, oracle, M, N):
+    rng = np.random.default_rng(M * 7 + N)
+    for scheme in range(6):
+        w = _weights("flat", M, rng)
+        eng = V.ThreefryState.seeded(5)
+        rep, cf = ctx.resample(scheme, w, eng, N=N)
+        oe = oracle.tf64_engine(5)
+        orep = np.zeros(M, np.uint64)
+        oracle.lib.orc_resample(scheme, M, N, C.byref(oe), w, orep)
+        assert (rep == orep).all(), (scheme, M, N)
+        assert int(rep.sum()) == N
+        assert (cf == oracle.cfrp_trans(orep, N)).all()
+        _engine_equal(eng, oe)
+
+
+# ---------------------------------------------------------------- filter
+def _obs(T, seed=3):
+    return V.cv_simulate(T, seed)
+
+
+@pytest.mark.parametrize("scheme", range(6))
+def test_filter_fp64_matches_oracle(ctx, oracle, scheme):
+    """FP64 fused filter vs the FP64 oracle run: identical resampling decisions
+    and engine draws; ESS, log-evidence and means to rtol 1e-9 (GPU and glibc
+    transcendentals differ by ulps)."""
+    N, T = 5000, 40
+    obs = _obs(T)
+    say("filter fp64", scheme)
+    f = V.Filter(ctx, V.cv_desc(N, T, precision=V.FP64_FUSED, scheme=scheme, seed=42,
+                                resample_seed=4242))
+    f.set_obs(obs)
+    f.run(T)
+    f.finish()
+    h = f.history()
+    st, lw = f.state()
+    _, olw, rec = oracle.cv_run(N, obs, 42, 4242, scheme, 0.5)
+    assert (h["resampled"] == rec["resampled"]).all()
+    assert (h["draws"] == rec["draws"]).all()
+    np.testing.assert_allclose(h["ess"], rec["ess"], rtol=1e-9)
+    np.testing.assert_allclose(h["log_z"], rec["log_z"], rtol=1e-9, atol=1e-9)
+    np.testing.assert_allclose(h["mean_x"], rec["mean_x"], rtol=1e-9, atol=1e-9)
+    np.testing.assert_allclose(h["mean_y"], rec["mean_y"], rtol=1e-9, atol=1e-9)
+    np.testing.assert_allclose(lw, olw, rtol=1e-9, atol=1e-9)
+    f.close()
+
+
+@pytest.mark.parametrize("scheme", range(6))
+def test_filter_debug_step_ancestors(ctx, oracle, scheme):
+    """Given the weights the device resampled from (and, for the sorted-uniform
+    schemes, the uniforms it inverted), vSMC's sequential inversion and
+    cfrp_trans give exactly the device's replication and ancestors."""
+    N, T = 20000, 6
+    obs = _obs(T, 11)
+    f = V.Filter(ctx, V.cv_desc(N, T, precision=V.MIXED, scheme=scheme, threshold=1.01))
+    f.set_obs(obs)
+    for t in range(T):
+        say("debug step", scheme, t)
+        d = f.debug_step()
+        assert d["resampled"]
+        w = d["weight"]
+        e0 = d["engine"]
+        oe = V.ThreefryState  # noqa: F841
+        from oracle_ctypes import TF64Engine
+        oeng = TF64Engine()
+        for j in range(4):
+            oeng.ctr[j] = e0.ctr[j]
+            oeng.key[j] = e0.key[j]
+            oeng.buf[j] = e0.buffer[j]
+        oeng.index = e0.index
+        if scheme in (V.MULTINOMIAL, V.RESIDUAL):
+            if scheme == V.MULTINOMIAL:
+                rep, _ = oracle.inversion_u(w, d["uniforms"])
+            else:
+                ip, rr, nn = oracle.residual_split(w)
+                assert nn == d["nn"]
+                rep, _ = oracle.inversion_u(rr, d["uniforms"])
+                rep = rep + ip
+        else:
+            rep, _, _ = oracle.resample(scheme, w, engine=oeng)
+        assert (d["rep"] == rep).all(), (scheme, t)
+        assert (d["copy_from"] == oracle.cfrp_trans(rep)).all(), (scheme, t)
+    f.close()
+
+
+def test_filter_mixed_vs_fp64_oracle(ctx, oracle):
+    """Mixed precision (FP32 state/likelihood, FP64 normaliser and scan) against
+    the FP64 oracle at the config-0 size: log-evidence within 1e-6 relative,
+    ESS within 1e-3 relative, means within 1e-3 absolute."""
+    N, T = 10000, 100
+    obs = _obs(T, 5)
+    f = V.Filter(ctx, V.cv_desc(N, T, precision=V.MIXED, scheme=V.SYSTEMATIC))
+    f.set_obs(obs)
+    f.run(T)
+    f.finish()
+    h = f.history()
+    _, _, rec = oracle.cv_run(N, obs, 42, 4242, V.SYSTEMATIC, 0.5)
+    say("mixed logZ", h["log_z"][-1], rec["log_z"][-1])
+    assert abs(h["log_z"][-1] - rec["log_z"][-1]) <= 1e-6 * abs(rec["log_z"][-1])
+    np.testing.assert_allclose(h["ess"], rec["ess"], rtol=1e-3)
+    np.testing.assert_allclose(h["mean_x"], rec["mean_x"], atol=1e-3)
+    np.testing.assert_allclose(h["mean_y"], rec["mean_y"], atol=1e-3)
+    f.close()
+
+
+def test_degenerate_weights_error(ctx):
+    """All-zero weights are rejected (vSMC would produce NaN silently)."""
+    eng = V.ThreefryState.seeded(1)
+    with pytest.raises(V.VsmcxError):
+        ctx.resample(V.SYSTEMATIC, np.array([1.0, -1.0]), eng)

@@ -1,0 +1,295 @@
+"""Python bindings (ctypes) over the C ABI in include/vsmcx.h.
+
+The compute path is the HIP library vsmc_amd/libvsmcx.so (gfx950).  There is
+no CPU fallback: if the library is missing every entry point raises.
+"""
+import ctypes as C
+import math
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libvsmcx.so")
+
+MULTINOMIAL, RESIDUAL, STRATIFIED, SYSTEMATIC, RESIDUAL_STRATIFIED, RESIDUAL_SYSTEMATIC = range(6)
+SCHEMES = {"multinomial": 0, "residual": 1, "stratified": 2, "systematic": 3,
+           "residual_stratified": 4, "residual_systematic": 5}
+MIXED, FP64_FUSED, FP64_PLAIN = 0, 1, 2
+
+ERRORS = {-1: "VSMCX_ERR_ARG", -2: "VSMCX_ERR_HIP", -3: "VSMCX_ERR_DEGENERATE",
+          -4: "VSMCX_ERR_BUDGET", -5: "VSMCX_ERR_INTERNAL", -6: "VSMCX_ERR_STATE"}
+
+
+class VsmcxError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{ERRORS.get(code, code)}: {msg}")
+        self.code = code
+
+
+class ThreefryState(C.Structure):
+    """vsmcx_threefry4x64_state: ctr, key, buffer, index of a Threefry4x64 engine."""
+    _fields_ = [("ctr", C.c_uint64 * 4), ("key", C.c_uint64 * 4),
+                ("buffer", C.c_uint64 * 4), ("index", C.c_uint64)]
+
+    @classmethod
+    def seeded(cls, seed):
+        """Threefry4x64(seed): ctr 0, key {seed, 0, 0, 0}, index 4."""
+        s = cls()
+        s.key[0] = seed
+        s.index = 4
+        return s
+
+    def as_tuple(self):
+        return (tuple(self.ctr), tuple(self.key), tuple(self.buffer), int(self.index))
+
+
+class FilterDesc(C.Structure):
+    _fields_ = [("n", C.c_uint64), ("horizon", C.c_uint64), ("precision", C.c_int),
+                ("scheme", C.c_int), ("threshold", C.c_double), ("seed", C.c_uint64),
+                ("resample_seed", C.c_uint64),
+                ("dt", C.c_double), ("sd_pos", C.c_double), ("sd_vel", C.c_double),
+                ("prior_sd_pos", C.c_double), ("prior_sd_vel", C.c_double),
+                ("obs_scale", C.c_double), ("nu", C.c_double), ("llh_const", C.c_double)]
+
+
+STEP_DTYPE = np.dtype([("ess", "f8"), ("log_z", "f8"), ("mean_x", "f8"), ("mean_y", "f8"),
+                       ("resampled", "i4"), ("pad", "i4"), ("draws", "u8")])
+
+_lib = None
+
+
+def lib():
+    """Load libvsmcx.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is missing: run `python vsmc_amd/build.py` "
+                           "(there is no CPU fallback)")
+    L = C.CDLL(LIB_PATH)
+    vp, u64, i32, f64 = C.c_void_p, C.c_uint64, C.c_int, C.c_double
+    P = C.POINTER
+    sig = {
+        "vsmcx_last_error": ([], C.c_char_p),
+        "vsmcx_version": ([], C.c_char_p),
+        "vsmcx_ctx_create": ([i32, u64, P(vp)], i32),
+        "vsmcx_ctx_destroy": ([vp], i32),
+        "vsmcx_ctx_sync": ([vp], i32),
+        "vsmcx_ctx_allocated": ([vp, P(u64)], i32),
+        "vsmcx_philox4x32_fill": ([vp, vp, vp, u64, vp], i32),
+        "vsmcx_threefry4x64_fill": ([vp, vp, vp, u64, vp], i32),
+        "vsmcx_threefry4x32_fill": ([vp, vp, vp, u64, vp], i32),
+        "vsmcx_resample": ([vp, i32, u64, u64, P(ThreefryState), vp, vp, vp], i32),
+        "vsmcx_fold_prefix": ([vp, u64, vp, vp], i32),
+        "vsmcx_filter_create": ([vp, P(FilterDesc), P(vp)], i32),
+        "vsmcx_filter_destroy": ([vp], i32),
+        "vsmcx_filter_set_obs": ([vp, u64, vp], i32),
+        "vsmcx_filter_run": ([vp, u64], i32),
+        "vsmcx_filter_finish": ([vp], i32),
+        "vsmcx_filter_steps": ([vp, P(u64)], i32),
+        "vsmcx_filter_history": ([vp, u64, vp], i32),
+        "vsmcx_filter_state": ([vp, vp, vp], i32),
+        "vsmcx_filter_engine": ([vp, P(ThreefryState)], i32),
+        "vsmcx_filter_time": ([vp, u64, P(f64), P(f64)], i32),
+        "vsmcx_filter_debug_step": ([vp, vp, vp, vp, vp, P(u64), P(ThreefryState), P(i32)], i32),
+    }
+    for name, (args, res) in sig.items():
+        fn = getattr(L, name)
+        fn.argtypes = args
+        fn.restype = res
+    _lib = L
+    return L
+
+
+def _check(rc):
+    if rc != 0:
+        raise VsmcxError(rc, lib().vsmcx_last_error().decode())
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data
+
+
+class Context:
+    """One HIP stream on one device (vsmcx_ctx)."""
+
+    def __init__(self, device=0, budget=0):
+        self.h = C.c_void_p()
+        _check(lib().vsmcx_ctx_create(device, budget, C.byref(self.h)))
+
+    def close(self):
+        if self.h:
+            lib().vsmcx_ctx_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def allocated(self):
+        v = C.c_uint64()
+        _check(lib().vsmcx_ctx_allocated(self.h, C.byref(v)))
+        return v.value
+
+    # ---- RNG ----
+    def philox4x32_fill(self, key, ctr0, nblocks):
+        k = np.asarray(key, np.uint32)
+        c = np.asarray(ctr0, np.uint32)
+        out = np.zeros((nblocks, 4), np.uint32)
+        _check(lib().vsmcx_philox4x32_fill(self.h, k.ctypes.data, c.ctypes.data, nblocks,
+                                           out.ctypes.data))
+        return out
+
+    def threefry4x64_fill(self, key, ctr0, nblocks):
+        k = np.asarray(key, np.uint64)
+        c = np.asarray(ctr0, np.uint64)
+        out = np.zeros((nblocks, 4), np.uint64)
+        _check(lib().vsmcx_threefry4x64_fill(self.h, k.ctypes.data, c.ctypes.data, nblocks,
+                                             out.ctypes.data))
+        return out
+
+    def threefry4x32_fill(self, key, ctr0, nblocks):
+        k = np.asarray(key, np.uint32)
+        c = np.asarray(ctr0, np.uint32)
+        out = np.zeros((nblocks, 4), np.uint32)
+        _check(lib().vsmcx_threefry4x32_fill(self.h, k.ctypes.data, c.ctypes.data, nblocks,
+                                             out.ctypes.data))
+        return out
+
+    # ---- resampling ----
+    def resample(self, scheme, weight, engine, N=None, copy_from=True):
+        """Resample<Scheme>(M, N, rng, w, rep) (+ cfrp_trans); engine advanced in place."""
+        w = np.ascontiguousarray(weight, np.float64)
+        M = w.size
+        N = M if N is None else N
+        rep = np.zeros(M, np.uint64)
+        cf = np.zeros(N, np.uint64) if copy_from else None
+        _check(lib().vsmcx_resample(self.h, scheme, M, N, C.byref(engine), w.ctypes.data,
+                                    rep.ctypes.data, _ptr(cf)))
+        return rep, cf
+
+    def fold_prefix(self, x):
+        x = np.ascontiguousarray(x, np.float64)
+        out = np.zeros_like(x)
+        _check(lib().vsmcx_fold_prefix(self.h, x.size, x.ctypes.data, out.ctypes.data))
+        return out
+
+
+def cv_llh_const(nu=10.0, scale=0.1):
+    """log normaliser of one Student-t(nu, scale) coordinate."""
+    return (math.lgamma((nu + 1) / 2) - math.lgamma(nu / 2) - 0.5 * math.log(nu * math.pi)
+            - math.log(scale))
+
+
+def cv_desc(n, horizon, precision=MIXED, scheme=SYSTEMATIC, threshold=0.5, seed=42,
+            resample_seed=4242, dt=0.1, sd_pos=0.1, sd_vel=0.5, prior_sd_pos=2.0,
+            prior_sd_vel=1.0, obs_scale=0.1, nu=10.0):
+    """Descriptor of the constant-velocity filter (dt 0.1, noise sd 0.1 / 0.5,
+    Student-t(nu=10, scale 0.1) position observations)."""
+    return FilterDesc(n=n, horizon=horizon, precision=precision, scheme=scheme,
+                      threshold=threshold, seed=seed, resample_seed=resample_seed, dt=dt,
+                      sd_pos=sd_pos, sd_vel=sd_vel, prior_sd_pos=prior_sd_pos,
+                      prior_sd_vel=prior_sd_vel, obs_scale=obs_scale, nu=nu,
+                      llh_const=cv_llh_const(nu, obs_scale))
+
+
+class Filter:
+    """Device-resident CV particle filter (vsmcx_filter)."""
+
+    def __init__(self, ctx, desc):
+        self.ctx = ctx
+        self.desc = desc
+        self.h = C.c_void_p()
+        _check(lib().vsmcx_filter_create(ctx.h, C.byref(desc), C.byref(self.h)))
+
+    def close(self):
+        if self.h:
+            lib().vsmcx_filter_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_obs(self, obs):
+        o = np.ascontiguousarray(obs, np.float64).reshape(-1, 2)
+        _check(lib().vsmcx_filter_set_obs(self.h, o.shape[0], o.ctypes.data))
+
+    def run(self, nsteps):
+        _check(lib().vsmcx_filter_run(self.h, nsteps))
+
+    def finish(self):
+        _check(lib().vsmcx_filter_finish(self.h))
+
+    def steps(self):
+        v = C.c_uint64()
+        _check(lib().vsmcx_filter_steps(self.h, C.byref(v)))
+        return v.value
+
+    def history(self, count=None):
+        count = self.steps() if count is None else count
+        out = np.zeros(count, STEP_DTYPE)
+        _check(lib().vsmcx_filter_history(self.h, count, out.ctypes.data))
+        return out
+
+    def state(self):
+        n = self.desc.n
+        st = np.zeros((n, 4))
+        lw = np.zeros(n)
+        _check(lib().vsmcx_filter_state(self.h, st.ctypes.data, lw.ctypes.data))
+        return st, lw
+
+    def engine(self):
+        e = ThreefryState()
+        _check(lib().vsmcx_filter_engine(self.h, C.byref(e)))
+        return e
+
+    def time(self, nsteps):
+        el, mv = C.c_double(), C.c_double()
+        _check(lib().vsmcx_filter_time(self.h, nsteps, C.byref(el), C.byref(mv)))
+        return el.value, mv.value
+
+    def debug_step(self):
+        n = self.desc.n
+        w = np.zeros(n)
+        rep = np.zeros(n, np.uint64)
+        cf = np.zeros(n, np.uint64)
+        u = np.zeros(n)
+        nn = C.c_uint64()
+        eng = ThreefryState()
+        res = C.c_int()
+        _check(lib().vsmcx_filter_debug_step(self.h, w.ctypes.data, rep.ctypes.data,
+                                             cf.ctypes.data, u.ctypes.data, C.byref(nn),
+                                             C.byref(eng), C.byref(res)))
+        return dict(weight=w, rep=rep, copy_from=cf, uniforms=u[:nn.value], nn=nn.value,
+                    engine=eng, resampled=bool(res.value))
+
+
+def cv_simulate(T, seed, dt=0.1, sd_pos=0.1, sd_vel=0.5, prior_sd_pos=2.0, prior_sd_vel=1.0,
+                obs_scale=0.1, nu=10.0):
+    """Synthetic observations of the CV model: a trajectory from the prior and
+    dynamics, position observed with Student-t(nu) noise of scale obs_scale."""
+    g = np.random.default_rng(seed)
+    x = np.array([prior_sd_pos * g.standard_normal(), prior_sd_pos * g.standard_normal(),
+                  prior_sd_vel * g.standard_normal(), prior_sd_vel * g.standard_normal()])
+    obs = np.zeros((T, 2))
+    for t in range(T):
+        if t > 0:
+            z = g.standard_normal(4)
+            x[0] += sd_pos * z[0] + dt * x[2]
+            x[1] += sd_pos * z[1] + dt * x[3]
+            x[2] += sd_vel * z[2]
+            x[3] += sd_vel * z[3]
+        obs[t] = x[:2] + obs_scale * g.standard_t(nu, 2)
+    return obs

@@ -1,0 +1,680 @@
+// capi.cpp -- the C ABI (include/vsmcx.h) over the HIP kernels.
+//
+// Host code only: contexts with a device-memory budget, the device-resident
+// filter driver (one stream, no host synchronisation inside a step), the
+// synchronous compatibility entries, and error translation at the boundary.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/vsmcx.h"
+#include "internal.hpp"
+#include "launch.hpp"
+#include "rng_core.hpp"
+
+using namespace vsmcx;
+
+namespace {
+
+thread_local std::string g_err;
+
+struct BudgetError : std::runtime_error {
+    explicit BudgetError(const std::string &s) : std::runtime_error(s) {}
+};
+struct ArgError : std::runtime_error {
+    explicit ArgError(const std::string &s) : std::runtime_error(s) {}
+};
+struct DevError : std::runtime_error {
+    int code;
+    DevError(int c, const std::string &s) : std::runtime_error(s), code(c) {}
+};
+
+int fail(int code, const std::string &msg)
+{
+    g_err = msg;
+    return code;
+}
+
+}  // namespace
+
+#define API_BEGIN try {
+#define API_END                                                   \
+    }                                                             \
+    catch (const ArgError &e) { return fail(VSMCX_ERR_ARG, e.what()); }          \
+    catch (const BudgetError &e) { return fail(VSMCX_ERR_BUDGET, e.what()); }    \
+    catch (const DevError &e) { return fail(e.code, e.what()); }                 \
+    catch (const HipError &e) { return fail(VSMCX_ERR_HIP, e.what()); }          \
+    catch (const std::exception &e) { return fail(VSMCX_ERR_INTERNAL, e.what()); } \
+    return VSMCX_OK;
+
+#define REQUIRE(cond, msg) \
+    do {                   \
+        if (!(cond)) throw ArgError(msg); \
+    } while (0)
+
+struct vsmcx_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    uint64_t budget = 0;
+    uint64_t allocated = 0;
+    std::map<void *, size_t> sizes;
+
+    void *alloc(size_t bytes)
+    {
+        if (bytes == 0) bytes = 1;
+        if (budget && allocated + bytes > budget)
+            throw BudgetError("device-memory budget exceeded: need " + std::to_string(bytes) +
+                              " B, " + std::to_string(budget - allocated) + " B left");
+        void *p = nullptr;
+        VSMCX_HIP(hipSetDevice(device));
+        VSMCX_HIP(hipMalloc(&p, bytes));
+        allocated += bytes;
+        sizes[p] = bytes;
+        return p;
+    }
+    template <typename T>
+    T *alloc_n(size_t n)
+    {
+        return static_cast<T *>(alloc(n * sizeof(T)));
+    }
+    template <typename T>
+    T *alloc_zero(size_t n)
+    {
+        T *p = alloc_n<T>(n);
+        VSMCX_HIP(hipMemsetAsync(p, 0, n * sizeof(T), stream));
+        return p;
+    }
+    void release(void *p)
+    {
+        if (!p) return;
+        auto it = sizes.find(p);
+        if (it != sizes.end()) {
+            allocated -= it->second;
+            sizes.erase(it);
+        }
+        (void)hipFree(p);
+    }
+    void sync() { VSMCX_HIP(hipStreamSynchronize(stream)); }
+};
+
+namespace {
+
+// Fold / resample scratch for up to M particles (and nuni uniforms).
+void alloc_resample(vsmcx_ctx *c, ResampleBuffers &rb, uint64_t M, uint64_t nuni, bool lists)
+{
+    const uint64_t tiles = (M + kTile - 1) / kTile;
+    const uint64_t utiles = (nuni + kTile - 1) / kTile + 1;
+    if (lists) {
+        rb.anc = c->alloc_n<uint32_t>(M);
+        rb.zl = c->alloc_n<uint32_t>(M);
+        rb.sl = c->alloc_n<uint32_t>(M);
+    }
+    rb.u_tsum = c->alloc_n<double>(utiles);
+    rb.u_texcl = c->alloc_n<double>(utiles);
+    rb.u_tmax = c->alloc_n<double>(utiles);
+    rb.u_tmaxex = c->alloc_n<double>(utiles);
+    rb.uscan = c->alloc_n<double>(nuni ? nuni : 1);
+    rb.nuni = nuni;
+    FoldScratch &fs = rb.fs;
+    fs.flags_sum = c->alloc_zero<uint64_t>(tiles);
+    fs.val_sum = c->alloc_n<double>(tiles);
+    fs.flags_fold = c->alloc_zero<uint64_t>(tiles);
+    fs.val_fold = c->alloc_n<double>(tiles);
+    fs.agg_d0 = c->alloc_n<uint64_t>(tiles);
+    fs.agg_d1 = c->alloc_n<uint64_t>(tiles);
+    fs.agg_pp = c->alloc_n<uint32_t>(tiles);
+    fs.agg_b = c->alloc_n<int32_t>(tiles);
+    fs.flags_cnt = c->alloc_zero<uint64_t>(tiles);
+    fs.val_cnt = c->alloc_n<uint64_t>(tiles);
+    fs.tile_counter = c->alloc_zero<unsigned long long>(1);
+    fs.epoch = c->alloc_zero<uint64_t>(1);
+    fs.tiles = tiles;
+}
+
+void free_resample(vsmcx_ctx *c, ResampleBuffers &rb)
+{
+    void *ps[] = {rb.anc, rb.zl, rb.sl, rb.rep_out, rb.a_out, rb.u_tsum, rb.u_texcl, rb.uscan,
+                  rb.u_tmax, rb.u_tmaxex, rb.fs.flags_sum, rb.fs.val_sum, rb.fs.flags_fold,
+                  rb.fs.val_fold, rb.fs.agg_d0, rb.fs.agg_d1, rb.fs.agg_pp, rb.fs.agg_b,
+                  rb.fs.flags_cnt, rb.fs.val_cnt, rb.fs.tile_counter, rb.fs.epoch};
+    for (void *p : ps) c->release(p);
+    rb = ResampleBuffers();
+}
+
+Tf64State engine_in(const vsmcx_threefry4x64_state &e)
+{
+    Tf64State s{};
+    for (int j = 0; j < 4; ++j) {
+        s.ctr[j] = e.ctr[j];
+        s.key[j] = e.key[j];
+    }
+    s.index = (uint32_t)(e.index > 4 ? 4 : e.index);
+    return s;
+}
+
+void engine_out(const Tf64State &s, vsmcx_threefry4x64_state &e)
+{
+    for (int j = 0; j < 4; ++j) {
+        e.ctr[j] = s.ctr[j];
+        e.key[j] = s.key[j];
+    }
+    e.index = s.index;
+    if (s.index < 4) {
+        u64x4 c;
+        for (int j = 0; j < 4; ++j) c.v[j] = s.ctr[j];
+        const u64x4 b = threefry4x64_20(c, s.key);
+        for (int j = 0; j < 4; ++j) e.buffer[j] = b.v[j];
+    } else {
+        for (int j = 0; j < 4; ++j) e.buffer[j] = 0;
+    }
+}
+
+void check_dev_err(const FilterCtl &h)
+{
+    if (h.err & kErrDegenerate) throw DevError(VSMCX_ERR_DEGENERATE, "weights sum to zero or are not finite");
+    if (h.err) throw DevError(VSMCX_ERR_INTERNAL, "device consistency check failed (bits " + std::to_string(h.err) + ")");
+}
+
+bool scheme_ok(int s) { return s >= 0 && s <= 5; }
+
+}  // namespace
+
+struct vsmcx_filter {
+    vsmcx_ctx *ctx = nullptr;
+    vsmcx_filter_desc d{};
+    CvParams p{};
+    bool fp64 = false;
+    uint64_t N = 0, T = 0, tiles = 0, nobs = 0, t = 0;
+    void *st[2] = {nullptr, nullptr};
+    int cur = 0;
+    double *lw = nullptr;
+    double *partials = nullptr;
+    double *obs = nullptr;
+    FilterCtl *ctl = nullptr;
+    StepRow *hist = nullptr;
+    ResampleBuffers rb;
+    std::vector<hipEvent_t> events;
+
+    size_t state_bytes() const { return fp64 ? 32 : 16; }
+
+    void step()
+    {
+        if (t >= nobs) throw DevError(VSMCX_ERR_STATE, "no observation for step " + std::to_string(t));
+        hipStream_t s = ctx->stream;
+        launch_cv_move(fp64, ctl, st[cur], st[1 - cur], rb.anc, lw, obs, p, d.seed, t, N, partials,
+                       s);
+        launch_cv_reduce(ctl, partials, tiles, hist, t, s);
+        launch_resample_from_log_weights(ctl, lw, N, d.scheme, rb, hist, s);
+        cur ^= 1;
+        ++t;
+    }
+};
+
+extern "C" {
+
+const char *vsmcx_last_error(void) { return g_err.c_str(); }
+const char *vsmcx_version(void) { return "vsmcx 0.1 (gfx950)"; }
+
+int vsmcx_ctx_create(int device, uint64_t hbm_budget, vsmcx_ctx **out)
+{
+    API_BEGIN
+    REQUIRE(out, "out is NULL");
+    int n = 0;
+    VSMCX_HIP(hipGetDeviceCount(&n));
+    REQUIRE(device >= 0 && device < n, "no such device");
+    auto c = std::make_unique<vsmcx_ctx>();
+    c->device = device;
+    c->budget = hbm_budget;
+    VSMCX_HIP(hipSetDevice(device));
+    VSMCX_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    *out = c.release();
+    API_END
+}
+
+int vsmcx_ctx_destroy(vsmcx_ctx *ctx)
+{
+    API_BEGIN
+    if (!ctx) return VSMCX_OK;
+    (void)hipStreamSynchronize(ctx->stream);
+    for (auto &kv : ctx->sizes) (void)hipFree(kv.first);
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    API_END
+}
+
+int vsmcx_ctx_sync(vsmcx_ctx *ctx)
+{
+    API_BEGIN
+    REQUIRE(ctx, "ctx is NULL");
+    ctx->sync();
+    API_END
+}
+
+int vsmcx_ctx_allocated(vsmcx_ctx *ctx, uint64_t *bytes)
+{
+    API_BEGIN
+    REQUIRE(ctx && bytes, "NULL argument");
+    *bytes = ctx->allocated;
+    API_END
+}
+
+// ---------------------------------------------------------------- RNG
+int vsmcx_philox4x32_fill(vsmcx_ctx *ctx, const uint32_t key[2], const uint32_t ctr0[4],
+                          uint64_t nblocks, uint32_t *out)
+{
+    API_BEGIN
+    REQUIRE(ctx && key && ctr0 && (out || !nblocks), "NULL argument");
+    if (!nblocks) return VSMCX_OK;
+    uint32_t *d = ctx->alloc_n<uint32_t>(4 * nblocks);
+    launch_philox4x32_fill(key[0], key[1], ctr0, nblocks, d, ctx->stream);
+    VSMCX_HIP(hipMemcpyAsync(out, d, 16 * nblocks, hipMemcpyDeviceToHost, ctx->stream));
+    ctx->sync();
+    ctx->release(d);
+    API_END
+}
+
+int vsmcx_threefry4x64_fill(vsmcx_ctx *ctx, const uint64_t key[4], const uint64_t ctr0[4],
+                            uint64_t nblocks, uint64_t *out)
+{
+    API_BEGIN
+    REQUIRE(ctx && key && ctr0 && (out || !nblocks), "NULL argument");
+    if (!nblocks) return VSMCX_OK;
+    uint64_t *d = ctx->alloc_n<uint64_t>(4 * nblocks);
+    launch_threefry4x64_fill(key, ctr0, nblocks, d, ctx->stream);
+    VSMCX_HIP(hipMemcpyAsync(out, d, 32 * nblocks, hipMemcpyDeviceToHost, ctx->stream));
+    ctx->sync();
+    ctx->release(d);
+    API_END
+}
+
+int vsmcx_threefry4x32_fill(vsmcx_ctx *ctx, const uint32_t key[4], const uint32_t ctr0[4],
+                            uint64_t nblocks, uint32_t *out)
+{
+    API_BEGIN
+    REQUIRE(ctx && key && ctr0 && (out || !nblocks), "NULL argument");
+    if (!nblocks) return VSMCX_OK;
+    uint32_t *d = ctx->alloc_n<uint32_t>(4 * nblocks);
+    launch_threefry4x32_fill(key, ctr0, nblocks, d, ctx->stream);
+    VSMCX_HIP(hipMemcpyAsync(out, d, 16 * nblocks, hipMemcpyDeviceToHost, ctx->stream));
+    ctx->sync();
+    ctx->release(d);
+    API_END
+}
+
+// ---------------------------------------------------------------- resample
+int vsmcx_resample(vsmcx_ctx *ctx, int scheme, uint64_t M, uint64_t N,
+                   vsmcx_threefry4x64_state *eng, const double *weight, uint64_t *replication,
+                   uint64_t *copy_from)
+{
+    API_BEGIN
+    REQUIRE(ctx && eng, "NULL argument");
+    REQUIRE(scheme_ok(scheme), "unknown resampling scheme");
+    if (M == 0) return VSMCX_OK;
+    REQUIRE(weight && replication, "NULL weight/replication");
+    REQUIRE(M < (1ull << 31) && N < (1ull << 31), "sizes must be below 2^31");
+    for (uint64_t i = 0; i < M; ++i)
+        REQUIRE(weight[i] >= 0 && std::isfinite(weight[i]), "weights must be finite and >= 0");
+    const bool lists = copy_from && M == N;
+    ResampleBuffers rb;
+    FilterCtl *ctl = nullptr;
+    double *w = nullptr;
+    try {
+        alloc_resample(ctx, rb, M, N, lists);
+        rb.rep_out = ctx->alloc_n<uint64_t>(M);
+        ctl = ctx->alloc_n<FilterCtl>(1);
+        w = ctx->alloc_n<double>(M);
+        FilterCtl h{};
+        h.n = N;
+        h.resample = 1;
+        h.scheme = scheme;
+        h.eng = engine_in(*eng);
+        VSMCX_HIP(hipMemcpyAsync(ctl, &h, sizeof(h), hipMemcpyHostToDevice, ctx->stream));
+        VSMCX_HIP(hipMemcpyAsync(w, weight, 8 * M, hipMemcpyHostToDevice, ctx->stream));
+        launch_resample_from_weights(ctl, w, M, scheme, rb, nullptr, ctx->stream);
+        VSMCX_HIP(hipGetLastError());
+        VSMCX_HIP(hipMemcpyAsync(&h, ctl, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
+        VSMCX_HIP(hipMemcpyAsync(replication, rb.rep_out, 8 * M, hipMemcpyDeviceToHost,
+                                 ctx->stream));
+        std::vector<uint32_t> anc;
+        if (lists) {
+            anc.resize(M);
+            VSMCX_HIP(hipMemcpyAsync(anc.data(), rb.anc, 4 * M, hipMemcpyDeviceToHost,
+                                     ctx->stream));
+        }
+        ctx->sync();
+        check_dev_err(h);
+        if (copy_from) {
+            if (lists) {
+                for (uint64_t i = 0; i < M; ++i) copy_from[i] = anc[i];
+            } else {  // cfrp_trans M != N branch: expansion (common.hpp:110-117)
+                uint64_t to = 0;
+                for (uint64_t from = 0; from < M; ++from)
+                    for (uint64_t r = 0; r < replication[from] && to < N; ++r)
+                        copy_from[to++] = from;
+            }
+        }
+        engine_out(h.eng, *eng);
+    } catch (...) {
+        free_resample(ctx, rb);
+        ctx->release(ctl);
+        ctx->release(w);
+        throw;
+    }
+    free_resample(ctx, rb);
+    ctx->release(ctl);
+    ctx->release(w);
+    API_END
+}
+
+int vsmcx_fold_prefix(vsmcx_ctx *ctx, uint64_t n, const double *x, double *out)
+{
+    API_BEGIN
+    REQUIRE(ctx, "NULL ctx");
+    if (n == 0) return VSMCX_OK;
+    REQUIRE(x && out, "NULL argument");
+    for (uint64_t i = 0; i < n; ++i)
+        REQUIRE(x[i] >= 0 && std::isfinite(x[i]), "fold terms must be finite and >= 0");
+    ResampleBuffers rb;
+    FilterCtl *ctl = nullptr;
+    double *dx = nullptr;
+    try {
+        alloc_resample(ctx, rb, n, 1, false);
+        rb.a_out = ctx->alloc_n<double>(n);
+        ctl = ctx->alloc_n<FilterCtl>(1);
+        dx = ctx->alloc_n<double>(n);
+        FilterCtl h{};
+        h.n = n;
+        h.resample = 1;
+        VSMCX_HIP(hipMemcpyAsync(ctl, &h, sizeof(h), hipMemcpyHostToDevice, ctx->stream));
+        VSMCX_HIP(hipMemcpyAsync(dx, x, 8 * n, hipMemcpyHostToDevice, ctx->stream));
+        launch_fold_values(ctl, dx, n, rb, ctx->stream);
+        VSMCX_HIP(hipGetLastError());
+        VSMCX_HIP(hipMemcpyAsync(out, rb.a_out, 8 * n, hipMemcpyDeviceToHost, ctx->stream));
+        VSMCX_HIP(hipMemcpyAsync(&h, ctl, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
+        ctx->sync();
+        check_dev_err(h);
+    } catch (...) {
+        free_resample(ctx, rb);
+        ctx->release(ctl);
+        ctx->release(dx);
+        throw;
+    }
+    free_resample(ctx, rb);
+    ctx->release(ctl);
+    ctx->release(dx);
+    API_END
+}
+
+// ---------------------------------------------------------------- filter
+int vsmcx_filter_create(vsmcx_ctx *ctx, const vsmcx_filter_desc *desc, vsmcx_filter **out)
+{
+    API_BEGIN
+    REQUIRE(ctx && desc && out, "NULL argument");
+    REQUIRE(desc->n >= 1 && desc->n < (1ull << 31), "n must be in [1, 2^31)");
+    REQUIRE(desc->horizon >= 1, "horizon must be >= 1");
+    REQUIRE(scheme_ok(desc->scheme), "unknown resampling scheme");
+    REQUIRE(desc->precision == VSMCX_MIXED || desc->precision == VSMCX_FP64_FUSED,
+            "precision must be VSMCX_MIXED or VSMCX_FP64_FUSED");
+    auto f = std::make_unique<vsmcx_filter>();
+    f->ctx = ctx;
+    f->d = *desc;
+    f->fp64 = desc->precision != VSMCX_MIXED;
+    f->p = CvParams{desc->dt, desc->sd_pos, desc->sd_vel, desc->prior_sd_pos,
+                    desc->prior_sd_vel, desc->obs_scale, desc->nu, desc->llh_const};
+    f->N = desc->n;
+    f->T = desc->horizon;
+    f->tiles = (f->N + kTile - 1) / kTile;
+    try {
+        f->st[0] = ctx->alloc(f->N * f->state_bytes());
+        f->st[1] = ctx->alloc(f->N * f->state_bytes());
+        f->lw = ctx->alloc_n<double>(f->N);
+        f->partials = ctx->alloc_n<double>(5 * f->tiles);
+        f->obs = ctx->alloc_n<double>(2 * f->T);
+        f->ctl = ctx->alloc_n<FilterCtl>(1);
+        f->hist = ctx->alloc_zero<StepRow>(f->T);
+        alloc_resample(ctx, f->rb, f->N, f->N, true);
+    } catch (...) {
+        for (void *p : {f->st[0], f->st[1], (void *)f->lw, (void *)f->partials, (void *)f->obs,
+                        (void *)f->ctl, (void *)f->hist})
+            ctx->release(p);
+        free_resample(ctx, f->rb);
+        throw;
+    }
+    FilterCtl h{};
+    h.n = f->N;
+    h.threshold = desc->threshold;
+    h.scheme = desc->scheme;
+    h.eng.key[0] = desc->resample_seed;
+    h.eng.index = 4;
+    VSMCX_HIP(hipMemcpyAsync(f->ctl, &h, sizeof(h), hipMemcpyHostToDevice, ctx->stream));
+    ctx->sync();
+    *out = f.release();
+    API_END
+}
+
+int vsmcx_filter_destroy(vsmcx_filter *f)
+{
+    API_BEGIN
+    if (!f) return VSMCX_OK;
+    vsmcx_ctx *c = f->ctx;
+    (void)hipStreamSynchronize(c->stream);
+    for (void *p : {f->st[0], f->st[1], (void *)f->lw, (void *)f->partials, (void *)f->obs,
+                    (void *)f->ctl, (void *)f->hist})
+        c->release(p);
+    free_resample(c, f->rb);
+    for (hipEvent_t e : f->events) (void)hipEventDestroy(e);
+    delete f;
+    API_END
+}
+
+int vsmcx_filter_set_obs(vsmcx_filter *f, uint64_t T, const double *obs)
+{
+    API_BEGIN
+    REQUIRE(f && obs, "NULL argument");
+    REQUIRE(T <= f->T, "more observations than the horizon");
+    VSMCX_HIP(hipMemcpyAsync(f->obs, obs, 16 * T, hipMemcpyHostToDevice, f->ctx->stream));
+    f->nobs = T;
+    f->ctx->sync();
+    API_END
+}
+
+int vsmcx_filter_run(vsmcx_filter *f, uint64_t nsteps)
+{
+    API_BEGIN
+    REQUIRE(f, "NULL filter");
+    REQUIRE(f->t + nsteps <= f->nobs, "not enough observations for the requested steps");
+    for (uint64_t k = 0; k < nsteps; ++k) f->step();
+    VSMCX_HIP(hipGetLastError());
+    API_END
+}
+
+int vsmcx_filter_finish(vsmcx_filter *f)
+{
+    API_BEGIN
+    REQUIRE(f, "NULL filter");
+    if (f->t > 0)
+        launch_cv_monitor(f->fp64, f->ctl, f->st[f->cur], f->rb.anc, f->lw, f->N, f->partials,
+                          f->hist, f->ctx->stream);
+    VSMCX_HIP(hipGetLastError());
+    FilterCtl h;
+    VSMCX_HIP(hipMemcpyAsync(&h, f->ctl, sizeof(h), hipMemcpyDeviceToHost, f->ctx->stream));
+    f->ctx->sync();
+    check_dev_err(h);
+    API_END
+}
+
+int vsmcx_filter_steps(vsmcx_filter *f, uint64_t *t)
+{
+    API_BEGIN
+    REQUIRE(f && t, "NULL argument");
+    *t = f->t;
+    API_END
+}
+
+int vsmcx_filter_history(vsmcx_filter *f, uint64_t count, vsmcx_step_stats *out)
+{
+    API_BEGIN
+    static_assert(sizeof(StepRow) == sizeof(vsmcx_step_stats), "history row layout");
+    REQUIRE(f && out, "NULL argument");
+    REQUIRE(count <= f->t, "count beyond completed steps");
+    VSMCX_HIP(hipMemcpyAsync(out, f->hist, sizeof(StepRow) * count, hipMemcpyDeviceToHost,
+                             f->ctx->stream));
+    f->ctx->sync();
+    API_END
+}
+
+int vsmcx_filter_state(vsmcx_filter *f, double *state, double *log_weight)
+{
+    API_BEGIN
+    REQUIRE(f, "NULL filter");
+    const uint64_t N = f->N;
+    if (state) {
+        void *tmp = f->ctx->alloc(N * f->state_bytes());
+        launch_cv_gather(f->fp64, f->ctl, f->st[f->cur], tmp, f->rb.anc, N, f->ctx->stream);
+        if (f->fp64) {
+            VSMCX_HIP(hipMemcpyAsync(state, tmp, 32 * N, hipMemcpyDeviceToHost, f->ctx->stream));
+            f->ctx->sync();
+        } else {
+            std::vector<float> h(4 * N);
+            VSMCX_HIP(hipMemcpyAsync(h.data(), tmp, 16 * N, hipMemcpyDeviceToHost, f->ctx->stream));
+            f->ctx->sync();
+            for (uint64_t i = 0; i < 4 * N; ++i) state[i] = h[i];
+        }
+        f->ctx->release(tmp);
+    }
+    if (log_weight) {
+        FilterCtl h;
+        VSMCX_HIP(hipMemcpyAsync(&h, f->ctl, sizeof(h), hipMemcpyDeviceToHost, f->ctx->stream));
+        VSMCX_HIP(hipMemcpyAsync(log_weight, f->lw, 8 * N, hipMemcpyDeviceToHost, f->ctx->stream));
+        f->ctx->sync();
+        // vSMC keeps log-weights normalised to max 0 (core/weight_set.hpp:108-117);
+        // after a resample they are all zero (set_equal_weight)
+        for (uint64_t i = 0; i < N; ++i)
+            log_weight[i] = h.prev_resampled ? 0.0 : log_weight[i] - h.m;
+    }
+    API_END
+}
+
+int vsmcx_filter_engine(vsmcx_filter *f, vsmcx_threefry4x64_state *eng)
+{
+    API_BEGIN
+    REQUIRE(f && eng, "NULL argument");
+    FilterCtl h;
+    VSMCX_HIP(hipMemcpyAsync(&h, f->ctl, sizeof(h), hipMemcpyDeviceToHost, f->ctx->stream));
+    f->ctx->sync();
+    engine_out(h.eng, *eng);
+    API_END
+}
+
+int vsmcx_filter_time(vsmcx_filter *f, uint64_t nsteps, double *elapsed_ms, double *move_kernel_ms)
+{
+    API_BEGIN
+    REQUIRE(f && elapsed_ms, "NULL argument");
+    REQUIRE(f->t + nsteps <= f->nobs, "not enough observations for the requested steps");
+    const size_t need = 2 * nsteps + 2;
+    while (f->events.size() < need) {
+        hipEvent_t e;
+        VSMCX_HIP(hipEventCreate(&e));
+        f->events.push_back(e);
+    }
+    hipStream_t s = f->ctx->stream;
+    VSMCX_HIP(hipEventRecord(f->events[0], s));
+    for (uint64_t k = 0; k < nsteps; ++k) {
+        if (f->t >= f->nobs) throw DevError(VSMCX_ERR_STATE, "out of observations");
+        launch_cv_move(f->fp64, f->ctl, f->st[f->cur], f->st[1 - f->cur], f->rb.anc, f->lw,
+                       f->obs, f->p, f->d.seed, f->t, f->N, f->partials, s);
+        VSMCX_HIP(hipEventRecord(f->events[2 + 2 * k], s));
+        launch_cv_reduce(f->ctl, f->partials, f->tiles, f->hist, f->t, s);
+        launch_resample_from_log_weights(f->ctl, f->lw, f->N, f->d.scheme, f->rb, f->hist, s);
+        VSMCX_HIP(hipEventRecord(f->events[3 + 2 * k], s));
+        f->cur ^= 1;
+        ++f->t;
+    }
+    VSMCX_HIP(hipEventRecord(f->events[1], s));
+    VSMCX_HIP(hipGetLastError());
+    VSMCX_HIP(hipEventSynchronize(f->events[1]));
+    float ms = 0;
+    VSMCX_HIP(hipEventElapsedTime(&ms, f->events[0], f->events[1]));
+    *elapsed_ms = ms;
+    if (move_kernel_ms) {
+        // move k runs between the end event of step k-1 (or the start) and events[2+2k]
+        double tot = 0;
+        for (uint64_t k = 0; k < nsteps; ++k) {
+            float a = 0;
+            hipEvent_t prev = k == 0 ? f->events[0] : f->events[3 + 2 * (k - 1)];
+            VSMCX_HIP(hipEventElapsedTime(&a, prev, f->events[2 + 2 * k]));
+            tot += a;
+        }
+        *move_kernel_ms = nsteps ? tot / nsteps : 0;
+    }
+    FilterCtl h;
+    VSMCX_HIP(hipMemcpy(&h, f->ctl, sizeof(h), hipMemcpyDeviceToHost));
+    check_dev_err(h);
+    API_END
+}
+
+int vsmcx_filter_debug_step(vsmcx_filter *f, double *weight, uint64_t *replication,
+                            uint64_t *copy_from, double *uniforms, uint64_t *nn,
+                            vsmcx_threefry4x64_state *eng_before, int *resampled)
+{
+    API_BEGIN
+    REQUIRE(f, "NULL filter");
+    REQUIRE(f->t < f->nobs, "out of observations");
+    vsmcx_ctx *c = f->ctx;
+    hipStream_t s = c->stream;
+    const uint64_t N = f->N;
+    launch_cv_move(f->fp64, f->ctl, f->st[f->cur], f->st[1 - f->cur], f->rb.anc, f->lw, f->obs,
+                   f->p, f->d.seed, f->t, N, f->partials, s);
+    launch_cv_reduce(f->ctl, f->partials, f->tiles, f->hist, f->t, s);
+    FilterCtl h;
+    VSMCX_HIP(hipMemcpyAsync(&h, f->ctl, sizeof(h), hipMemcpyDeviceToHost, s));
+    double *w = c->alloc_n<double>(N);
+    launch_cv_weights(f->ctl, f->lw, N, w, s);
+    if (weight) VSMCX_HIP(hipMemcpyAsync(weight, w, 8 * N, hipMemcpyDeviceToHost, s));
+    c->sync();
+    c->release(w);
+    if (eng_before) engine_out(h.eng, *eng_before);
+    if (resampled) *resampled = h.resample;
+    f->rb.rep_out = c->alloc_n<uint64_t>(N);
+    launch_resample_from_log_weights(f->ctl, f->lw, N, f->d.scheme, f->rb, f->hist, s);
+    VSMCX_HIP(hipGetLastError());
+    if (h.resample) {
+        if (replication)
+            VSMCX_HIP(hipMemcpyAsync(replication, f->rb.rep_out, 8 * N, hipMemcpyDeviceToHost, s));
+        std::vector<uint32_t> anc(N);
+        VSMCX_HIP(hipMemcpyAsync(anc.data(), f->rb.anc, 4 * N, hipMemcpyDeviceToHost, s));
+        FilterCtl h2;
+        VSMCX_HIP(hipMemcpyAsync(&h2, f->ctl, sizeof(h2), hipMemcpyDeviceToHost, s));
+        std::vector<double> us, tm;
+        const bool sorted = f->d.scheme == kMultinomial || f->d.scheme == kResidual;
+        if (uniforms && sorted) {
+            us.resize(N);
+            tm.resize((N + kTile - 1) / kTile + 1);
+            VSMCX_HIP(hipMemcpyAsync(us.data(), f->rb.uscan, 8 * N, hipMemcpyDeviceToHost, s));
+            VSMCX_HIP(hipMemcpyAsync(tm.data(), f->rb.u_tmaxex, 8 * tm.size(),
+                                     hipMemcpyDeviceToHost, s));
+        }
+        c->sync();
+        check_dev_err(h2);
+        if (copy_from)
+            for (uint64_t i = 0; i < N; ++i) copy_from[i] = anc[i];
+        if (nn) *nn = h2.nn;
+        if (uniforms && sorted)
+            for (uint64_t k = 0; k < h2.nn; ++k)
+                uniforms[k] = us[k] < tm[k / kTile] ? tm[k / kTile] : us[k];
+    } else {
+        c->sync();
+    }
+    c->release(f->rb.rep_out);
+    f->rb.rep_out = nullptr;
+    f->cur ^= 1;
+    ++f->t;
+    API_END
+}
+
+}  // extern "C"

@@ -1,0 +1,100 @@
+// internal.hpp -- device-side control structures and launcher declarations
+// shared by the HIP translation units and the host driver.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include <stdexcept>
+#include <string>
+
+#include "rng_core.hpp"
+
+namespace vsmcx {
+
+constexpr int kBlock = 256;                  // threads per workgroup
+constexpr int kItems = 16;                   // particles per thread
+constexpr int kTile = kBlock * kItems;       // particles per workgroup tile
+
+// Resampling schemes, same order as vsmc::ResampleScheme
+// (include/vsmc/resample/common.hpp:124-131).
+enum Scheme : int {
+    kMultinomial = 0,
+    kResidual = 1,
+    kStratified = 2,
+    kSystematic = 3,
+    kResidualStratified = 4,
+    kResidualSystematic = 5
+};
+
+// error bits raised on the device
+enum DevErr : int {
+    kErrFoldClass = 1,       // fold classification bound violated
+    kErrDegenerate = 2,      // weights sum to zero / NaN
+    kErrWindow = 4,          // inversion window inconsistency
+    kErrTimeout = 8,         // a look-back wait exceeded its spin limit
+};
+
+// Per-filter control block, resident in device memory.  Kernels of one step
+// communicate through it so the host never synchronises inside a step.
+struct FilterCtl {
+    double m;          // max log-weight
+    double coeff;      // 1 / sum exp(lw - m)
+    double lse;        // log sum exp(lw)
+    double logz;       // accumulated log-evidence
+    double ess;        // effective sample size of the current weights
+    double rcoeff;     // residual schemes: 1 / fold(r)
+    double u0;         // systematic: first uniform
+    double delta;      // 1.0 / nn
+    double fold_total; // last exact fold total
+    double threshold;  // resample when ess < threshold * n
+    uint64_t n;        // particles
+    uint64_t nn;       // uniforms this resample (n, or n - R for residual)
+    uint64_t r_int;    // residual integral total R
+    uint64_t n_final;  // uniforms placed before the last bin
+    uint64_t zeros;    // zero-replication slots this resample
+    uint64_t t;        // iteration number of the step in flight
+    uint64_t draws;    // engine draws of the last resample
+    Tf64State eng;     // resample engine (Threefry4x64)
+    int resample;      // this step resamples
+    int prev_resampled;// the previous step resampled (move reads ancestors)
+    int scheme;
+    int err;
+};
+
+// One row of per-step history.
+struct StepRow {
+    double ess, logz, mean_x, mean_y;
+    int32_t resampled, pad;
+    uint64_t draws;
+};
+
+// Look-back bookkeeping for the exact-fold inversion kernel.
+struct FoldScratch {
+    uint64_t *flags_sum;   // approximate-sum look-back  [tiles]
+    double *val_sum;       //                            [tiles]
+    uint64_t *flags_fold;  // exact-fold look-back       [tiles]
+    double *val_fold;      // inclusive exact value      [tiles]
+    uint64_t *agg_d0;      // clean-tile transducer      [tiles]
+    uint64_t *agg_d1;
+    uint32_t *agg_pp;
+    int32_t *agg_b;
+    uint64_t *flags_cnt;   // zeros/surplus look-back    [tiles]
+    uint64_t *val_cnt;
+    unsigned long long *tile_counter;  // device: tiles claimed in this launch
+    uint64_t *epoch;                   // device: look-back epoch of this launch
+    size_t tiles;
+};
+
+struct HipError : std::runtime_error {
+    explicit HipError(const std::string &s) : std::runtime_error(s) {}
+};
+
+#define VSMCX_HIP(call)                                                               \
+    do {                                                                              \
+        hipError_t e_ = (call);                                                       \
+        if (e_ != hipSuccess)                                                         \
+            throw ::vsmcx::HipError(std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+}  // namespace vsmcx

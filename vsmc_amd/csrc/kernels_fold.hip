@@ -1,0 +1,879 @@
+// kernels_fold.hip -- exact-fold resampling on the device.
+//
+// One kernel (k_fold_inv) per resample replaces vSMC's sequential
+//   inversion  (include/vsmc/resample/common.hpp:57-84)  and
+//   cfrp_trans (include/vsmc/resample/common.hpp:87-118),
+// producing the replication counts, the keep-in-place ancestor layout and the
+// lists the zero-slot fill (k_zero_fill) consumes.  The running weight sum is
+// the exact left fold vSMC computes (see fold_core.hpp), so given the same
+// weights and uniforms the ancestors are the reference's, at any N.
+//
+// Work decomposition: one 256-thread workgroup per 4096-particle tile, tiles
+// claimed in dispatch order from an atomic counter, three decoupled
+// look-backs (approximate FP64 prefix, exact fold value, zero/surplus counts).
+#include "device_util.hpp"
+#include "fold_core.hpp"
+#include "internal.hpp"
+#include "launch.hpp"
+
+namespace vsmcx {
+
+// ------------------------------------------------------------------------
+// x sources: the sequence being folded
+// ------------------------------------------------------------------------
+struct XArray {  // given weights (compat entry, plain FP64 path)
+    const double *w;
+    __device__ void bind(const FilterCtl *) {}
+    __device__ double x(uint64_t i) const { return w[i]; }
+    __device__ uint64_t ipart(uint64_t) const { return 0; }
+};
+
+struct XWeight {  // W_i = exp(lw_i - m) * coeff, computed from the log-weights
+    const double *lw;
+    double m, coeff;
+    __device__ void bind(const FilterCtl *c) { m = c->m; coeff = c->coeff; }
+    __device__ double x(uint64_t i) const { return exp(lw[i] - m) * coeff; }
+    __device__ uint64_t ipart(uint64_t) const { return 0; }
+};
+
+// residual.hpp:57-61: r_i = modf(N * W_i, &i_i)
+template <typename W>
+struct XResid {
+    W w;
+    double nd;
+    __device__ void bind(const FilterCtl *c) { w.bind(c); nd = (double)c->n; }
+    __device__ double x(uint64_t i) const
+    {
+        double ip;
+        return modf(nd * w.x(i), &ip);
+    }
+    __device__ uint64_t ipart(uint64_t i) const
+    {
+        double ip;
+        modf(nd * w.x(i), &ip);
+        return (uint64_t)ip;
+    }
+};
+
+// residual.hpp:62-63: r *= 1 / asum(r)
+template <typename W>
+struct XResidScaled {
+    XResid<W> r;
+    double rc;
+    __device__ void bind(const FilterCtl *c) { r.bind(c); rc = c->rcoeff; }
+    __device__ double x(uint64_t i) const { return r.x(i) * rc; }
+    __device__ uint64_t ipart(uint64_t i) const { return r.ipart(i); }
+};
+
+// ------------------------------------------------------------------------
+// uniform sources: F(c) = #{n < nn : v_n <= c} for the sorted sequence v
+// ------------------------------------------------------------------------
+struct InvNone {
+    static constexpr bool kInv = false;
+    static constexpr bool kWindow = false;
+    __device__ void bind(const FilterCtl *) {}
+};
+
+// U01SequenceSystematic (include/vsmc/rng/u01.hpp:253-287): u_n = u0 + n*delta
+struct InvSystematic {
+    static constexpr bool kInv = true;
+    static constexpr bool kWindow = false;
+    double u0, delta;
+    uint64_t nn;
+    __device__ void bind(const FilterCtl *c) { u0 = c->u0; delta = c->delta; nn = c->nn; }
+    __device__ double v(uint64_t n) const { return u0 + (double)n * delta; }
+    __device__ uint64_t F(double c) const
+    {
+        if (nn == 0) return 0;
+        const double est = (c - u0) * (double)nn;
+        uint64_t n = !(est > 0) ? 0 : (est >= (double)nn ? nn : (uint64_t)est);
+        while (n < nn && v(n) <= c) ++n;
+        while (n > 0 && v(n - 1) > c) --n;
+        return n;
+    }
+};
+
+// U01SequenceStratified (u01.hpp:212-243): u_n = U_n*delta + n*delta, with
+// U_n the n-th draw of the resample engine; v_n = max(u_n, u_{n-1}) gives
+// inversion's running-max semantics (u_{n-2} < u_n always).
+struct InvStratified {
+    static constexpr bool kInv = true;
+    static constexpr bool kWindow = true;
+    Tf64State eng;
+    double delta;
+    uint64_t nn;
+    __device__ void bind(const FilterCtl *c) { eng = c->eng; delta = c->delta; nn = c->nn; }
+    __device__ double u(uint64_t n) const
+    {
+        return canonical64(tf64_stream_word(eng, n)) * delta + (double)n * delta;
+    }
+    __device__ double v(uint64_t n) const
+    {
+        const double a = u(n);
+        if (n == 0) return a;
+        const double b = u(n - 1);
+        return a < b ? b : a;
+    }
+    __device__ uint64_t point_F(double c) const
+    {
+        if (nn == 0) return 0;
+        const double est = c * (double)nn;
+        uint64_t n = !(est > 0) ? 0 : (est >= (double)nn ? nn : (uint64_t)est);
+        while (n < nn && v(n) <= c) ++n;
+        while (n > 0 && v(n - 1) > c) --n;
+        return n;
+    }
+};
+
+// A sorted uniform array held as a tile-wise running max: v_n =
+// max(tile_prefix_max[n / kTile], uscan[n]).
+struct InvArray {
+    static constexpr bool kInv = true;
+    static constexpr bool kWindow = true;
+    const double *uscan;
+    const double *tmax;  // exclusive running max of earlier tiles
+    uint64_t nn;
+    __device__ void bind(const FilterCtl *c) { nn = c->nn; }
+    __device__ double v(uint64_t n) const
+    {
+        const double a = uscan[n], b = tmax[n / kTile];
+        return a < b ? b : a;
+    }
+    __device__ uint64_t point_F(double c) const
+    {
+        // binary search for the first n with v_n > c
+        uint64_t lo = 0, hi = nn;
+        while (lo < hi) {
+            const uint64_t mid = lo + (hi - lo) / 2;
+            if (v(mid) <= c)
+                lo = mid + 1;
+            else
+                hi = mid;
+        }
+        return lo;
+    }
+};
+
+// ------------------------------------------------------------------------
+// decoupled look-back
+// ------------------------------------------------------------------------
+enum : uint64_t { kStNone = 0, kStAgg = 1, kStIncl = 2, kStNC = 3 };
+
+__device__ __forceinline__ uint32_t lb_state(const uint64_t *flags, int64_t q, uint64_t epoch)
+{
+    const uint64_t f = atomic_load_acq(flags + q);
+    return (f >> 2) == epoch ? (uint32_t)(f & 3) : (uint32_t)kStNone;
+}
+
+__device__ __forceinline__ void lb_publish(uint64_t *flags, uint64_t q, uint64_t epoch,
+                                           uint64_t st)
+{
+    atomic_store_rel(flags + q, (epoch << 2) | st);
+}
+
+// Exclusive prefix for tile t.  Called by all 64 lanes of one wave; every
+// lane returns the result.  Tiles before 0 count as an inclusive `start`.
+constexpr uint32_t kSpinLimit = 1u << 24;
+
+template <typename A, typename V, typename LoadAgg, typename LoadIncl, typename Compose,
+          typename Apply>
+__device__ V lookback(uint64_t t, const uint64_t *flags, uint64_t epoch, V start, A ident,
+                      LoadAgg load_agg, LoadIncl load_incl, Compose compose, Apply apply,
+                      int *timed_out)
+{
+    const int lane = threadIdx.x & 63;
+    A acc = ident;
+    int64_t hi = (int64_t)t - 1;
+    uint32_t spins = 0;
+    while (true) {
+        const int64_t q = hi - 63 + lane;
+        uint32_t st = q < 0 ? (uint32_t)kStIncl : lb_state(flags, q, epoch);
+        const uint64_t incl = __ballot(st == kStIncl);
+        const int l_incl = incl ? 63 - __clzll(incl) : -1;
+        const uint64_t notready = __ballot(lane > l_incl && st != kStAgg);
+        if (notready) {
+            // bounded wait: a predecessor that never publishes is a bug; report
+            // it and leave instead of hanging the device
+            if (++spins > kSpinLimit) {
+                *timed_out = 1;
+                return start;
+            }
+            __builtin_amdgcn_s_sleep(2);
+            continue;
+        }
+        A mine = (lane > l_incl) ? load_agg(q) : ident;
+        A comp = wave_incl_scan(mine, compose);
+        comp = shfl_t(comp, 63);
+        acc = compose(comp, acc);
+        if (l_incl >= 0) {
+            V base = start;
+            const int64_t qi = hi - 63 + l_incl;
+            if (qi >= 0) base = load_incl(qi);
+            base = shfl_t(base, l_incl);
+            return apply(acc, base);
+        }
+        hi -= 64;
+    }
+}
+
+// ------------------------------------------------------------------------
+// the kernel
+// ------------------------------------------------------------------------
+struct FoldOut {
+    uint32_t *anc;       // [M] ancestor index (zero slots filled by k_zero_fill)
+    uint32_t *zl;        // [M] zero-slot list
+    uint32_t *sl;        // [M] surplus-copy list
+    uint64_t *rep;       // [M] optional replication counts
+    double *a_out;       // [M] optional exact fold values
+};
+
+constexpr int kPad = kItems + 1;  // LDS padding: one double per 16
+__device__ __forceinline__ int lidx(int e) { return e + e / kItems; }
+
+template <typename XSrc, typename Inv>
+__global__ __launch_bounds__(kBlock) void k_fold_inv(FilterCtl *ctl, XSrc xs, Inv inv, uint64_t M,
+                                                     FoldScratch fs, FoldOut out)
+{
+    if (!ctl->resample) return;
+    __shared__ double s_a[kBlock * kPad];
+    __shared__ double s_red[kBlock / kWave];
+    __shared__ Tr s_tr[kBlock / kWave];
+    __shared__ uint64_t s_u64[kBlock / kWave];
+    __shared__ uint32_t s_u32[kBlock / kWave];
+    __shared__ unsigned long long s_tile;
+    __shared__ double s_bcast_d[2];
+    __shared__ uint64_t s_bcast_u[4];
+    __shared__ int s_timeout;
+
+    const int tid = threadIdx.x;
+    if (tid == 0) s_timeout = 0;
+    if (tid == 0) s_tile = atomicAdd(fs.tile_counter, 1ull);
+    const uint64_t epoch = *fs.epoch;
+    xs.bind(ctl);
+    inv.bind(ctl);
+    __syncthreads();
+    const uint64_t t = s_tile;
+    const uint64_t base = t * (uint64_t)kTile;
+
+    // ---- load x (striped, coalesced) and transpose through LDS -----------
+#pragma unroll
+    for (int k = 0; k < kItems; ++k) {
+        const int e = k * kBlock + tid;
+        const uint64_t i = base + e;
+        s_a[lidx(e)] = i < M ? xs.x(i) : 0.0;
+    }
+    __syncthreads();
+    double xv[kItems];
+#pragma unroll
+    for (int k = 0; k < kItems; ++k) xv[k] = s_a[lidx(tid * kItems + k)];
+    __syncthreads();
+
+    // ---- approximate prefix: tile sum, look-back ----------------------------
+    double tsum = 0;
+#pragma unroll
+    for (int k = 0; k < kItems; ++k) tsum += xv[k];
+    double tile_sum;
+    block_incl_scan<kBlock>(tsum, AddOp(), s_red, &tile_sum);
+    if (tid == 0) {
+        fs.val_sum[t] = tile_sum;
+        lb_publish(fs.flags_sum, t, epoch, t == 0 ? kStIncl : kStAgg);
+    }
+    if (tid < kWave) {
+        double S = 0.0;
+        if (t > 0)
+            S = lookback<double, double>(
+                t, fs.flags_sum, epoch, 0.0, 0.0,
+                [&](int64_t q) { return fs.val_sum[q]; },
+                [&](int64_t q) { return fs.val_sum[q]; },
+                [](double a, double b) { return a + b; },
+                [](double a, double b) { return a + b; }, &s_timeout);
+        if (tid == 0) {
+            s_bcast_d[0] = S;
+            if (t > 0) {
+                fs.val_sum[t] = S + tile_sum;
+                lb_publish(fs.flags_sum, t, epoch, kStIncl);
+            }
+        }
+    }
+    __syncthreads();
+    if (tid == 0 && s_timeout) atomicOr(&ctl->err, (int)kErrTimeout);
+    const double S_t = s_bcast_d[0];
+
+    // ---- classify the tile --------------------------------------------------
+    double eps_rel, eps_abs;
+    fold_eps(M, &eps_rel, &eps_abs);
+    const int b0 = certain_binade(S_t, eps_rel, eps_abs);
+    const int b1 = certain_binade(S_t + tile_sum, eps_rel, eps_abs);
+    const bool clean = t > 0 && b0 == b1 && b0 != kNoBinade;
+
+    // ---- publish the clean-tile transducer ------------------------------------
+    if (clean) {
+        Tr thr = tr_identity();
+#pragma unroll
+        for (int k = 0; k < kItems; ++k) thr = tr_compose(thr, tr_element(xv[k], b0));
+        Tr T;
+        block_incl_scan<kBlock>(thr, [](const Tr &a, const Tr &b) { return tr_compose(a, b); },
+                                s_tr, &T);
+        if (tid == 0) {
+            fs.agg_d0[t] = T.d0;
+            fs.agg_d1[t] = T.d1;
+            fs.agg_pp[t] = T.pp;
+            fs.agg_b[t] = T.b;
+            lb_publish(fs.flags_fold, t, epoch, kStAgg);
+        }
+        if (tid == 0) s_tr[0] = T;
+    } else if (tid == 0 && t > 0) {
+        lb_publish(fs.flags_fold, t, epoch, kStNC);
+    }
+
+    // ---- exact start value of the tile ---------------------------------------
+    if (tid < kWave) {
+        double v0 = 0.0;
+        if (t > 0)
+            v0 = lookback<Tr, double>(
+                t, fs.flags_fold, epoch, 0.0, tr_identity(),
+                [&](int64_t q) {
+                    Tr a;
+                    a.d0 = fs.agg_d0[q];
+                    a.d1 = fs.agg_d1[q];
+                    a.pp = fs.agg_pp[q];
+                    a.b = fs.agg_b[q];
+                    return a;
+                },
+                [&](int64_t q) { return fs.val_fold[q]; },
+                [](const Tr &a, const Tr &b) { return tr_compose(a, b); },
+                [&](const Tr &a, double v) {
+                    double r;
+                    if (!tr_apply(a, v, &r)) {
+                        atomicOr(&ctl->err, (int)kErrFoldClass);
+                        r = v;
+                    }
+                    return r;
+                },
+                &s_timeout);
+        if (tid == 0) {
+            s_bcast_d[1] = v0;
+            if (clean) {  // publish the inclusive value right away
+                double e;
+                if (!tr_apply(s_tr[0], v0, &e)) atomicOr(&ctl->err, (int)kErrFoldClass);
+                fs.val_fold[t] = e;
+                lb_publish(fs.flags_fold, t, epoch, kStIncl);
+            }
+        }
+    }
+    __syncthreads();
+    if (tid == 0 && s_timeout) atomicOr(&ctl->err, (int)kErrTimeout);
+
+    // ---- exact values of every element: carry the fold through the tile ------
+    double cur = s_bcast_d[1];
+    int pos = 0;
+    while (pos < kTile) {
+        const int b = binade_of(cur);
+        Tr thr = tr_identity();
+#pragma unroll
+        for (int k = 0; k < kItems; ++k) {
+            const int e = tid * kItems + k;
+            if (e >= pos) thr = tr_compose(thr, tr_element(xv[k], b));
+        }
+        Tr excl;
+        {
+            Tr incl = block_incl_scan<kBlock>(
+                thr, [](const Tr &a, const Tr &c) { return tr_compose(a, c); }, s_tr);
+            excl = shfl_up_t(incl, 1);
+            const int lane = tid & 63, wid = tid / 64;
+            if (lane == 0) excl = wid > 0 ? s_tr[wid - 1] : tr_identity();
+            // s_tr still holds the per-wave inclusive totals after the scan
+        }
+        int first_fail = kTile;
+        double v;
+        if (!tr_apply(excl, cur, &v)) {
+            first_fail = tid * kItems;
+        } else {
+#pragma unroll
+            for (int k = 0; k < kItems; ++k) {
+                const int e = tid * kItems + k;
+                if (e < pos || first_fail < kTile) continue;
+                double v2;
+                if (!tr_apply(tr_element(xv[k], b), v, &v2)) {
+                    first_fail = e;
+                } else {
+                    s_a[lidx(e)] = v2;
+                    v = v2;
+                }
+            }
+        }
+        const int j = block_reduce<kBlock>(first_fail, [](int a, int c) { return a < c ? a : c; },
+                                           (int *)s_u32);
+        if (j >= kTile) break;
+        if (tid == j / kItems) {
+            const double prev = (j == pos) ? cur : s_a[lidx(j - 1)];
+            s_a[lidx(j)] = prev + xv[j % kItems];
+        }
+        __syncthreads();
+        cur = s_a[lidx(j)];
+        pos = j + 1;
+        __syncthreads();
+    }
+    __syncthreads();
+    if (!clean && tid == 0) {
+        fs.val_fold[t] = s_a[lidx(kTile - 1)];
+        lb_publish(fs.flags_fold, t, epoch, kStIncl);
+    }
+    const double v_start = s_bcast_d[1];
+
+    double av[kItems];
+#pragma unroll
+    for (int k = 0; k < kItems; ++k) av[k] = s_a[lidx(tid * kItems + k)];
+    if (out.a_out) {
+#pragma unroll
+        for (int k = 0; k < kItems; ++k) {
+            const uint64_t i = base + tid * kItems + k;
+            if (i < M) out.a_out[i] = av[k];
+        }
+    }
+    if (t == fs.tiles - 1 && tid == 0) ctl->fold_total = s_a[lidx((int)((M - 1) - base))];
+
+    if constexpr (!Inv::kInv) {
+        return;
+    } else {
+        // ---- inversion: c_i = F(a_i) for i <= M-2 ---------------------------
+        const uint64_t nn = inv.nn;
+        uint64_t c[kItems];
+        const uint64_t last_cmp = M >= 2 ? M - 2 : 0;  // highest index compared
+        if constexpr (!Inv::kWindow) {
+#pragma unroll
+            for (int k = 0; k < kItems; ++k) {
+                const uint64_t i = base + tid * kItems + k;
+                c[k] = (M >= 2 && i <= last_cmp) ? inv.F(av[k]) : 0;
+            }
+        } else {
+            // window of uniforms this tile can hold: [F(a_{start-1}), F(a_lastcmp)]
+            if (tid == 0) {
+                uint64_t lo = 0, hi = 0;
+                if (M >= 2) {
+                    lo = t == 0 ? 0 : inv.point_F(v_start);
+                    hi = lo;
+                    if (base <= last_cmp) {
+                        const uint64_t il =
+                            (base + kTile - 1 < last_cmp) ? base + kTile - 1 : last_cmp;
+                        hi = inv.point_F(s_a[lidx((int)(il - base))]);
+                    }
+                    if (hi < lo) {
+                        atomicOr(&ctl->err, (int)kErrWindow);
+                        hi = lo;
+                    }
+                }
+                s_bcast_u[0] = lo;
+                s_bcast_u[1] = hi;
+            }
+            __syncthreads();
+            const uint64_t wlo = s_bcast_u[0], whi = s_bcast_u[1];
+#pragma unroll
+            for (int k = 0; k < kItems; ++k) c[k] = wlo;
+            // reuse s_a as the window buffer (exact values are in av[])
+            constexpr int kCh = kBlock * kPad;
+            for (uint64_t w0 = wlo; w0 < whi; w0 += kCh) {
+                const int len = (int)((whi - w0) < (uint64_t)kCh ? (whi - w0) : (uint64_t)kCh);
+                __syncthreads();
+                for (int k = tid; k < len; k += kBlock) s_a[k] = inv.v(w0 + k);
+                __syncthreads();
+                const double first = s_a[0], lastv = s_a[len - 1];
+#pragma unroll
+                for (int k = 0; k < kItems; ++k) {
+                    const double a = av[k];
+                    if (a < first) continue;
+                    if (a >= lastv) {
+                        c[k] += len;
+                        continue;
+                    }
+                    int lo = 0, hi = len;  // first index with value > a
+                    while (lo < hi) {
+                        const int mid = (lo + hi) >> 1;
+                        if (s_a[mid] <= a)
+                            lo = mid + 1;
+                        else
+                            hi = mid;
+                    }
+                    c[k] += lo;
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < kItems; ++k) {
+                const uint64_t i = base + tid * kItems + k;
+                if (!(M >= 2 && i <= last_cmp)) c[k] = 0;
+            }
+        }
+        // c of the element before this thread's first item
+        uint64_t cprev;
+        {
+            const int lane = tid & 63, wid = tid / 64;
+            uint64_t up = shfl_up_t(c[kItems - 1], 1);
+            if (lane == 63) s_u64[wid] = c[kItems - 1];
+            __syncthreads();
+            if (lane == 0) up = wid > 0 ? s_u64[wid - 1] : 0;
+            cprev = up;
+            __syncthreads();
+            if (tid == 0) {
+                if constexpr (Inv::kWindow)
+                    cprev = (t == 0) ? 0 : s_bcast_u[0];
+                else
+                    cprev = (t == 0) ? 0 : inv.F(v_start);
+            }
+        }
+        // replication counts
+        uint32_t rep[kItems];
+        uint64_t zs_thr = 0;  // zeros << 32 | surplus
+#pragma unroll
+        for (int k = 0; k < kItems; ++k) {
+            const uint64_t i = base + tid * kItems + k;
+            uint64_t r = 0;
+            if (i < M) {
+                if (M == 1)
+                    r = nn;
+                else if (i <= last_cmp)
+                    r = c[k] - cprev;
+                else  // i == M-1
+                    r = nn - cprev;
+                r += xs.ipart(i);
+            }
+            cprev = c[k];
+            rep[k] = (uint32_t)r;
+            if (i < M) {
+                if (r == 0)
+                    zs_thr += 1ull << 32;
+                else if (r >= 2)
+                    zs_thr += r - 1;
+            }
+            if (out.rep && i < M) out.rep[i] = r;
+            if (M >= 2 && i == last_cmp) ctl->n_final = c[k];
+        }
+        if (M == 1 && t == 0 && tid == 0) ctl->n_final = 0;
+        if (!out.anc) return;  // replication only (M != N): no cfrp layout
+
+        // ---- zero / surplus ranks: block scan + look-back --------------------
+        uint64_t zs_tile;
+        const uint64_t zs_incl = block_incl_scan<kBlock>(zs_thr, AddOp(), s_u64, &zs_tile);
+        const uint64_t zs_excl = zs_incl - zs_thr;
+        if (tid == 0) {
+            fs.val_cnt[t] = zs_tile;
+            lb_publish(fs.flags_cnt, t, epoch, t == 0 ? kStIncl : kStAgg);
+        }
+        if (tid < kWave) {
+            uint64_t pre = 0;
+            if (t > 0)
+                pre = lookback<uint64_t, uint64_t>(
+                    t, fs.flags_cnt, epoch, 0ull, 0ull,
+                    [&](int64_t q) { return fs.val_cnt[q]; },
+                    [&](int64_t q) { return fs.val_cnt[q]; },
+                    [](uint64_t a, uint64_t b) { return a + b; },
+                    [](uint64_t a, uint64_t b) { return a + b; }, &s_timeout);
+            if (tid == 0) {
+                s_bcast_u[2] = pre;
+                if (t > 0) {
+                    fs.val_cnt[t] = pre + zs_tile;
+                    lb_publish(fs.flags_cnt, t, epoch, kStIncl);
+                }
+                if (t == fs.tiles - 1) ctl->zeros = (pre + zs_tile) >> 32;
+            }
+        }
+        __syncthreads();
+        if (tid == 0 && s_timeout) atomicOr(&ctl->err, (int)kErrTimeout);
+        const uint64_t pre = s_bcast_u[2];
+        uint64_t zrank = (pre >> 32) + (zs_excl >> 32);
+        uint64_t srank_local = zs_excl & 0xffffffffull;  // surplus before this thread (tile)
+        const uint64_t s_base = pre & 0xffffffffull;
+
+        // anc identity, zero list, per-item inclusive surplus prefix into LDS
+        uint32_t *s_sp = reinterpret_cast<uint32_t *>(s_a);  // kTile entries
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < kItems; ++k) {
+            const int e = tid * kItems + k;
+            const uint64_t i = base + e;
+            if (i < M) {
+                out.anc[i] = (uint32_t)i;
+                if (rep[k] == 0) out.zl[zrank++] = (uint32_t)i;
+                if (rep[k] >= 2) srank_local += rep[k] - 1;
+            }
+            s_sp[e] = (uint32_t)srank_local;
+        }
+        __syncthreads();
+        // expand surplus copies: positions [s_base, s_base + S_tile)
+        const uint32_t S_tile = (uint32_t)(zs_tile & 0xffffffffull);
+        for (uint32_t k = tid; k < S_tile; k += kBlock) {
+            int lo = 0, hi = kTile - 1;  // first e with s_sp[e] > k
+            while (lo < hi) {
+                const int mid = (lo + hi) >> 1;
+                if (s_sp[mid] > k)
+                    hi = mid;
+                else
+                    lo = mid + 1;
+            }
+            out.sl[s_base + k] = (uint32_t)(base + lo);
+        }
+    }
+}
+
+// ------------------------------------------------------------------------
+// zero-slot fill + engine bookkeeping
+// ------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_zero_fill(FilterCtl *ctl, const uint32_t *zl,
+                                                      const uint32_t *sl, uint32_t *anc)
+{
+    if (!ctl->resample) return;
+    const uint64_t z = ctl->zeros;
+    for (uint64_t k = blockIdx.x * (uint64_t)kBlock + threadIdx.x; k < z;
+         k += (uint64_t)gridDim.x * kBlock)
+        anc[zl[k]] = sl[k];
+}
+
+// engine bookkeeping: vSMC's lazy sequences draw min(nn, n_final + 1)
+// uniforms (u01.hpp:38-40, common.hpp:76-83); the systematic sequence draws
+// one in its constructor (u01.hpp:258-263)
+__global__ void k_resample_epilogue(FilterCtl *ctl, StepRow *hist, int systematic_like,
+                                    uint64_t M)
+{
+    if (!ctl->resample || threadIdx.x != 0) return;
+    uint64_t draws;
+    if (systematic_like)
+        draws = 1;
+    else
+        draws = ctl->nn == 0 || M <= 1 ? 0
+                : (ctl->n_final + 1 < ctl->nn ? ctl->n_final + 1 : ctl->nn);
+    tf64_advance(ctl->eng, draws);
+    ctl->draws = draws;
+    ctl->prev_resampled = 1;
+    if (hist) hist[ctl->t].draws = draws;
+}
+
+// residual: integral-part total R (exact integer sum)
+template <typename W>
+__global__ __launch_bounds__(kBlock) void k_residual_int(FilterCtl *ctl, W w, uint64_t M)
+{
+    if (!ctl->resample) return;
+    __shared__ uint64_t s_u64[kBlock / kWave];
+    XResid<W> r{w, 0.0};
+    r.bind(ctl);
+    uint64_t s = 0;
+    for (uint64_t i = blockIdx.x * (uint64_t)kBlock + threadIdx.x; i < M;
+         i += (uint64_t)gridDim.x * kBlock)
+        s += r.ipart(i);
+    s = block_reduce<kBlock>(s, AddOp(), s_u64);
+    if (threadIdx.x == 0 && s) atomicAdd((unsigned long long *)&ctl->r_int, (unsigned long long)s);
+}
+
+// after the residual fold total: rcoeff, nn, delta and (systematic) u0
+__global__ void k_residual_prologue(FilterCtl *ctl, int scheme)
+{
+    if (!ctl->resample || threadIdx.x != 0) return;
+    ctl->rcoeff = 1 / ctl->fold_total;
+    ctl->nn = ctl->n - ctl->r_int;
+    ctl->delta = 1.0 / (double)ctl->nn;
+    if (scheme == kResidualSystematic) ctl->u0 = canonical64(tf64_stream_word(ctl->eng, 0)) * ctl->delta;
+}
+
+// non-residual prologue: nn = n, delta, u0 (systematic ctor draw)
+__global__ void k_plain_prologue(FilterCtl *ctl, int scheme)
+{
+    if (!ctl->resample || threadIdx.x != 0) return;
+    ctl->nn = ctl->n;
+    ctl->r_int = 0;
+    ctl->delta = 1.0 / (double)ctl->nn;
+    if (scheme == kSystematic) ctl->u0 = canonical64(tf64_stream_word(ctl->eng, 0)) * ctl->delta;
+}
+
+// ------------------------------------------------------------------------
+// sorted uniforms (U01SequenceSorted, u01.hpp:165-202):
+//   lmax_n = sum_{k<=n} log(1 - U_k) / (nn - k);  u_n = 1 - exp(lmax_n)
+// computed with a fixed reduce-then-scan order (deterministic), then held
+// as a tile-wise running max so the sequence is sorted by construction.
+// ------------------------------------------------------------------------
+__device__ __forceinline__ double sorted_term(const Tf64State &e, uint64_t n, uint64_t nn)
+{
+    return log(1 - canonical64(tf64_stream_word(e, n))) / (double)(nn - n);
+}
+
+__global__ __launch_bounds__(kBlock) void k_sorted_tilesum(FilterCtl *ctl, double *tsum)
+{
+    if (!ctl->resample) return;
+    __shared__ double s_red[kBlock / kWave];
+    const uint64_t nn = ctl->nn;
+    const uint64_t base = blockIdx.x * (uint64_t)kTile;
+    if (base >= nn) return;
+    const Tf64State e = ctl->eng;
+    double s = 0;
+    for (int k = 0; k < kItems; ++k) {
+        const uint64_t n = base + threadIdx.x * kItems + k;
+        if (n < nn) s += sorted_term(e, n, nn);
+    }
+    s = block_reduce<kBlock>(s, AddOp(), s_red);
+    if (threadIdx.x == 0) tsum[blockIdx.x] = s;
+}
+
+// single workgroup: exclusive scan of tile sums (fixed order)
+__global__ __launch_bounds__(1024) void k_scan_tiles(FilterCtl *ctl, const double *tsum,
+                                                     double *texcl, uint64_t ntiles, int is_max)
+{
+    if (!ctl->resample) return;
+    __shared__ double s_red[1024 / kWave];
+    double carry = is_max ? -1.0 : 0.0;
+    for (uint64_t b = 0; b < ntiles; b += 1024) {
+        const uint64_t i = b + threadIdx.x;
+        double v = i < ntiles ? tsum[i] : (is_max ? -1.0 : 0.0);
+        double tot;
+        double incl = is_max ? block_incl_scan<1024>(v, MaxOp(), s_red, &tot)
+                             : block_incl_scan<1024>(v, AddOp(), s_red, &tot);
+        double ex = is_max ? (threadIdx.x == 0 ? -1.0 : 0.0) : incl - v;
+        if (is_max) {
+            double up = shfl_up_t(incl, 1);
+            const int lane = threadIdx.x & 63, wid = threadIdx.x / 64;
+            // s_red holds per-wave inclusive maxima after the scan
+            ex = lane == 0 ? (wid > 0 ? s_red[wid - 1] : -1.0) : up;
+            ex = ex < carry ? carry : ex;
+        } else {
+            ex = carry + ex;
+        }
+        if (i < ntiles) texcl[i] = ex;
+        carry = is_max ? (carry < tot ? tot : carry) : carry + tot;
+        __syncthreads();
+    }
+}
+
+// per tile: lmax = prefix + local inclusive prefix; u = 1 - exp(lmax);
+// local inclusive running max of u -> uscan; tile max -> tmaxv
+__global__ __launch_bounds__(kBlock) void k_sorted_values(FilterCtl *ctl, const double *texcl,
+                                                          double *uscan, double *tmaxv)
+{
+    if (!ctl->resample) return;
+    __shared__ double s_red[kBlock / kWave];
+    const uint64_t nn = ctl->nn;
+    const uint64_t base = blockIdx.x * (uint64_t)kTile;
+    if (base >= nn) return;
+    const Tf64State e = ctl->eng;
+    double term[kItems];
+    double s = 0;
+    for (int k = 0; k < kItems; ++k) {
+        const uint64_t n = base + threadIdx.x * kItems + k;
+        term[k] = n < nn ? sorted_term(e, n, nn) : 0.0;
+        s += term[k];
+    }
+    const double incl = block_incl_scan<kBlock>(s, AddOp(), s_red);
+    double lm = texcl[blockIdx.x] + (incl - s);
+    double mx = -1.0;
+    double uu[kItems];
+    for (int k = 0; k < kItems; ++k) {
+        lm += term[k];
+        const double u = 1 - exp(lm);
+        mx = mx < u ? u : mx;
+        uu[k] = mx;
+    }
+    // thread-exclusive running max across the tile
+    const double minc = block_incl_scan<kBlock>(mx, MaxOp(), s_red);
+    double mex = shfl_up_t(minc, 1);
+    {
+        const int lane = threadIdx.x & 63, wid = threadIdx.x / 64;
+        if (lane == 0) mex = wid > 0 ? s_red[wid - 1] : -1.0;
+    }
+    for (int k = 0; k < kItems; ++k) {
+        const uint64_t n = base + threadIdx.x * kItems + k;
+        if (n < nn) uscan[n] = uu[k] < mex ? mex : uu[k];
+    }
+    if (threadIdx.x == kBlock - 1) tmaxv[blockIdx.x] = minc;
+}
+
+// ------------------------------------------------------------------------
+// host launchers
+// ------------------------------------------------------------------------
+// new look-back epoch and a zeroed tile counter for the next k_fold_inv
+__global__ void k_fold_begin(FilterCtl *ctl, uint64_t *epoch, unsigned long long *counter)
+{
+    if (!ctl->resample || threadIdx.x != 0) return;
+    *epoch += 1;
+    *counter = 0;
+}
+
+template <typename XSrc, typename Inv>
+static void launch_fold(FilterCtl *ctl, XSrc xs, Inv inv, uint64_t M, FoldScratch &fs,
+                        const FoldOut &out, hipStream_t st)
+{
+    fs.tiles = (M + kTile - 1) / kTile;
+    hipLaunchKernelGGL(k_fold_begin, dim3(1), dim3(64), 0, st, ctl, fs.epoch, fs.tile_counter);
+    hipLaunchKernelGGL((k_fold_inv<XSrc, Inv>), dim3((unsigned)fs.tiles), dim3(kBlock), 0, st, ctl,
+                       xs, inv, M, fs, out);
+}
+
+template <typename W>
+static void resample_pipeline(FilterCtl *ctl, W w, uint64_t M, int scheme, ResampleBuffers &rb,
+                              StepRow *hist, hipStream_t st)
+{
+    FoldOut out{rb.anc, rb.zl, rb.sl, rb.rep_out, rb.a_out};
+    const bool residual = scheme == kResidual || scheme == kResidualStratified ||
+                          scheme == kResidualSystematic;
+    const unsigned grid = (unsigned)((M + kTile - 1) / kTile);
+    if (residual) {
+        hipLaunchKernelGGL((k_residual_int<W>), dim3(grid), dim3(kBlock), 0, st, ctl, w, M);
+        XResid<W> xr{w, 0.0};
+        launch_fold(ctl, xr, InvNone(), M, rb.fs, FoldOut{nullptr, nullptr, nullptr, nullptr, nullptr},
+                    st);
+        hipLaunchKernelGGL(k_residual_prologue, dim3(1), dim3(64), 0, st, ctl, scheme);
+    } else {
+        hipLaunchKernelGGL(k_plain_prologue, dim3(1), dim3(64), 0, st, ctl, scheme);
+    }
+    const bool sorted = scheme == kMultinomial || scheme == kResidual;
+    if (sorted) {
+        // uniforms: nn <= rb.nuni of them
+        const unsigned ut = (unsigned)((rb.nuni + kTile - 1) / kTile);
+        hipLaunchKernelGGL(k_sorted_tilesum, dim3(ut), dim3(kBlock), 0, st, ctl, rb.u_tsum);
+        hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(1024), 0, st, ctl, rb.u_tsum, rb.u_texcl,
+                           (uint64_t)ut, 0);
+        hipLaunchKernelGGL(k_sorted_values, dim3(ut), dim3(kBlock), 0, st, ctl, rb.u_texcl,
+                           rb.uscan, rb.u_tmax);
+        hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(1024), 0, st, ctl, rb.u_tmax, rb.u_tmaxex,
+                           (uint64_t)ut, 1);
+    }
+    InvArray ia{rb.uscan, rb.u_tmaxex, 0};
+    if (residual) {
+        XResidScaled<W> xs{XResid<W>{w, 0.0}, 0.0};
+        if (scheme == kResidual)
+            launch_fold(ctl, xs, ia, M, rb.fs, out, st);
+        else if (scheme == kResidualStratified)
+            launch_fold(ctl, xs, InvStratified(), M, rb.fs, out, st);
+        else
+            launch_fold(ctl, xs, InvSystematic(), M, rb.fs, out, st);
+    } else {
+        if (scheme == kMultinomial)
+            launch_fold(ctl, w, ia, M, rb.fs, out, st);
+        else if (scheme == kStratified)
+            launch_fold(ctl, w, InvStratified(), M, rb.fs, out, st);
+        else
+            launch_fold(ctl, w, InvSystematic(), M, rb.fs, out, st);
+    }
+    if (rb.anc) {
+        const unsigned zg = (unsigned)((M + kTile - 1) / kTile) * 4;
+        hipLaunchKernelGGL(k_zero_fill, dim3(zg ? zg : 1), dim3(kBlock), 0, st, ctl, rb.zl, rb.sl,
+                           rb.anc);
+    }
+    hipLaunchKernelGGL(k_resample_epilogue, dim3(1), dim3(64), 0, st, ctl, hist,
+                       (scheme == kSystematic || scheme == kResidualSystematic) ? 1 : 0, M);
+}
+
+void launch_resample_from_log_weights(FilterCtl *ctl, const double *lw, uint64_t M, int scheme,
+                                      ResampleBuffers &rb, StepRow *hist, hipStream_t st)
+{
+    resample_pipeline(ctl, XWeight{lw, 0.0, 0.0}, M, scheme, rb, hist, st);
+}
+
+void launch_resample_from_weights(FilterCtl *ctl, const double *w, uint64_t M, int scheme,
+                                  ResampleBuffers &rb, StepRow *hist, hipStream_t st)
+{
+    resample_pipeline(ctl, XArray{w}, M, scheme, rb, hist, st);
+}
+
+void launch_fold_values(FilterCtl *ctl, const double *x, uint64_t M, ResampleBuffers &rb,
+                        hipStream_t st)
+{
+    launch_fold(ctl, XArray{x}, InvNone(), M, rb.fs,
+                FoldOut{nullptr, nullptr, nullptr, nullptr, rb.a_out}, st);
+}
+
+}  // namespace vsmcx

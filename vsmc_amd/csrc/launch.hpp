@@ -1,0 +1,62 @@
+// launch.hpp -- host-callable launchers of the device kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "internal.hpp"
+
+namespace vsmcx {
+
+// Buffers of one resampling pipeline sized for M particles.
+struct ResampleBuffers {
+    uint32_t *anc = nullptr;     // [M] ancestors (read by the next move)
+    uint32_t *zl = nullptr;      // [M] zero-slot list
+    uint32_t *sl = nullptr;      // [M] surplus-copy list
+    uint64_t *rep_out = nullptr; // [M] optional replication output
+    double *a_out = nullptr;     // [M] optional exact fold output
+    double *u_tsum = nullptr;    // [tiles] sorted uniforms: tile sums
+    double *u_texcl = nullptr;   // [tiles] exclusive prefix
+    double *uscan = nullptr;     // [M] running-max uniforms within tile
+    double *u_tmax = nullptr;    // [tiles] tile maxima
+    double *u_tmaxex = nullptr;  // [tiles] exclusive running max of tiles
+    uint64_t nuni = 0;           // capacity of the uniform arrays
+    FoldScratch fs{};
+};
+
+// Parameters of the constant-velocity model (example/pf).
+struct CvParams {
+    double dt, sd_pos, sd_vel, prior_sd_pos, prior_sd_vel;
+    double obs_scale, nu, llh_const;
+};
+
+// kernels_fold.hip
+void launch_resample_from_log_weights(FilterCtl *ctl, const double *lw, uint64_t M, int scheme,
+                                      ResampleBuffers &rb, StepRow *hist, hipStream_t st);
+void launch_resample_from_weights(FilterCtl *ctl, const double *w, uint64_t M, int scheme,
+                                  ResampleBuffers &rb, StepRow *hist, hipStream_t st);
+void launch_fold_values(FilterCtl *ctl, const double *x, uint64_t M, ResampleBuffers &rb,
+                        hipStream_t st);
+
+// kernels_rng.hip
+void launch_philox4x32_fill(uint32_t k0, uint32_t k1, const uint32_t ctr0[4], uint64_t nblocks,
+                            uint32_t *out, hipStream_t st);
+void launch_threefry4x64_fill(const uint64_t key[4], const uint64_t ctr0[4], uint64_t nblocks,
+                              uint64_t *out, hipStream_t st);
+void launch_threefry4x32_fill(const uint32_t key[4], const uint32_t ctr0[4], uint64_t nblocks,
+                              uint32_t *out, hipStream_t st);
+
+// kernels_cv.hip
+// fused move (+ init when t == 0), partials per tile: {max, sum e, sum e^2, mean x, mean y}
+void launch_cv_move(bool fp64, FilterCtl *ctl, const void *st_in, void *st_out,
+                    const uint32_t *anc, double *lw, const double *obs, const CvParams &p,
+                    uint64_t seed, uint64_t t, uint64_t N, double *partials, hipStream_t st);
+void launch_cv_reduce(FilterCtl *ctl, const double *partials, uint64_t ntiles, StepRow *hist,
+                      uint64_t t, hipStream_t st);
+void launch_cv_monitor(bool fp64, FilterCtl *ctl, const void *state, const uint32_t *anc,
+                       const double *lw, uint64_t N, double *partials, StepRow *hist,
+                       hipStream_t st);
+void launch_cv_weights(FilterCtl *ctl, const double *lw, uint64_t N, double *w, hipStream_t st);
+void launch_cv_gather(bool fp64, FilterCtl *ctl, const void *st_in, void *st_out,
+                      const uint32_t *anc, uint64_t N, hipStream_t st);
+
+}  // namespace vsmcx
