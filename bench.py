@@ -1,0 +1,200 @@
+"""Benchmark of the particle step at 2^24 particles on one MI355X.
+
+Workload (one bench "step"): one filter time step of six constant-velocity
+bootstrap particle filters, one per vSMC resampling scheme (multinomial,
+residual, stratified, systematic, residual-stratified, residual-systematic),
+each with N = 2^24 particles, FP32 state and likelihood, FP64 log-sum-exp,
+exact FP64 weight fold, resampling when ESS < N/2.  W warmup steps (the
+first is the init step) then K timed steps; W + K = 200 = T by default.
+
+value = 6 * N * K / (device time of the K steps), inputs resident in HBM.
+The same workload is also timed for the fused FP64 pipeline and for the
+un-fused FP64 pipeline (one kernel per vSMC operation).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import vsmc_amd as V  # noqa: E402
+
+METRIC = "particle-updates/s at 2^24 particles, mixed vs FP64 plain; p50/p99 step latency"
+HBM_PEAK_GBS = 8000.0
+SCHEME_NAMES = ["multinomial", "residual", "stratified", "systematic", "residual_stratified",
+                "residual_systematic"]
+
+
+def move_bytes(prec, prev_resampled):
+    """Algorithmic HBM bytes per particle of one move-kernel launch.
+    mixed:  state 16 B read + 16 B write, log-weight 8 B write, plus 4 B ancestor
+            (after a resample) or 8 B log-weight read (otherwise);
+    fp64:   same with 32 B states."""
+    sb = 16 if prec == V.MIXED else 32
+    return 2 * sb + 8 + (4 if prev_resampled else 8)
+
+
+def run_precision(ctx, prec, N, W, K, obs, schemes, profile=True):
+    T = W + K
+    filters = [V.Filter(ctx, V.cv_desc(N, T, precision=prec, scheme=s, seed=2024 + s,
+                                       resample_seed=4242 + s)) for s in schemes]
+    for f in filters:
+        f.set_obs(obs)
+    for _ in range(W):
+        for f in filters:
+            f.run(1)
+    V.lib().vsmcx_ctx_sync(ctx.h)
+    nmark = K * len(filters) + 1
+    tm = V.Timer(ctx, nmark)
+    for f in filters:
+        f.profile(profile)
+    t0 = time.perf_counter()
+    m = 0
+    for k in range(K):
+        for f in filters:
+            tm.record(m)
+            m += 1
+            f.run(1)
+    tm.record(m)
+    V.lib().vsmcx_ctx_sync(ctx.h)
+    wall = time.perf_counter() - t0
+    per = np.array([tm.elapsed(i, i + 1) for i in range(nmark - 1)]).reshape(K, len(filters))
+    total_ms = tm.elapsed(0, nmark - 1)
+    tm.close()
+    out = {"total_ms": total_ms, "wall_s": wall, "per_filter_step_ms": per}
+    moves = []
+    hist = []
+    for f in filters:
+        mt = f.move_times() if profile else np.zeros(0)
+        f.finish()
+        h = f.history()
+        moves.append(mt)
+        hist.append(h)
+        f.close()
+    out["move_ms"] = moves
+    out["hist"] = hist
+    return out
+
+
+def roofline(prec, res, N, W):
+    """Dominant kernel (the move): algorithmic bytes / measured duration."""
+    tot_b = 0.0
+    tot_s = 0.0
+    for mt, h in zip(res["move_ms"], res["hist"]):
+        for k, ms in enumerate(mt):
+            t = W + k
+            prev = bool(h["resampled"][t - 1]) if t > 0 else False
+            tot_b += N * move_bytes(prec, prev)
+            tot_s += ms * 1e-3
+    if tot_s == 0:
+        return None
+    gbs = tot_b / tot_s / 1e9
+    nl = sum(len(m) for m in res["move_ms"])
+    return {"kernel": "k_cv_move<float>" if prec == V.MIXED else
+            ("k_cv_move<double>" if prec == V.FP64_FUSED else "k_plain_move"),
+            "bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None,
+            "bytes_per_launch": round(tot_b / nl), "avg_launch_ms": round(tot_s * 1e3 / nl, 5)}
+
+
+def cpu_baseline(N, steps, threads):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    from oracle_ctypes import Oracle
+    o = Oracle()
+    obs = V.cv_simulate(steps, 7)
+    t0 = time.perf_counter()
+    rec, phase = o.cv_run_timed(N, obs, 42, 4242, V.SYSTEMATIC, 0.5, threads=threads)
+    dt = time.perf_counter() - t0
+    return {"value": N * steps / dt, "unit": "particle-updates/s", "cores": threads,
+            "kind": "port",
+            "sample": f"vSMC-semantics CPU path (oracle/vsmc_oracle.c: OpenMP move, sequential "
+                      f"normalise/inversion/cfrp_trans/copy), N={N}, {steps} steps, systematic",
+            "seconds": round(dt, 3),
+            "phase_s": {"move": round(phase[0], 3), "normalise": round(phase[1], 3),
+                        "resample": round(phase[2], 3), "copy": round(phase[3], 3)}}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=190)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--n", type=int, default=1 << 24)
+    ap.add_argument("--schemes", default="all")
+    ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-steps", type=int, default=12)
+    ap.add_argument("--cpu-n", type=int, default=1 << 24)
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(it would shard particles with one all-reduce of three FP64 scalars per step); "
+              f"--gpus {a.gpus} is not supported", file=sys.stderr)
+        sys.exit(2)
+    W, K, N = max(1, a.warmup), a.steps, a.n
+    schemes = list(range(6)) if a.schemes == "all" else [SCHEME_NAMES.index(s) for s in
+                                                         a.schemes.split(",")]
+    obs = V.cv_simulate(W + K, 11)
+    ctx = V.Context(0)
+    res = {}
+    precs = [V.MIXED] + ([] if a.no_fp64 else [V.FP64_FUSED, V.FP64_PLAIN])
+    for prec in precs:
+        print(f"[bench] precision {prec}: {len(schemes)} filters x N={N}, W={W} K={K}",
+              file=sys.stderr, flush=True)
+        res[prec] = run_precision(ctx, prec, N, W, K, obs, schemes)
+    ctx.close()
+
+    nf = len(schemes)
+    mixed = res[V.MIXED]
+    value = nf * N * K / (mixed["total_ms"] * 1e-3)
+    steps_ms = mixed["per_filter_step_ms"].sum(axis=1)
+    out = {
+        "metric": METRIC, "value": value, "unit": "particle-updates/s", "n_gpus": 1,
+        "steps": K, "warmup": W, "ms_per_step": mixed["total_ms"] / K,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp32+fp64",
+        "data": "synthetic (observations simulated from the CV model, seed 11)",
+        "config": {"workload": "config-1: CV bootstrap filter, N=2^24 per filter, six filters "
+                               "(one per resampling scheme) advanced in turn each step, "
+                               "ESS<N/2 resampling, T=W+K",
+                   "n": N, "filters_per_step": nf,
+                   "schemes": [SCHEME_NAMES[s] for s in schemes], "horizon": W + K},
+        "p50_step_ms": float(np.percentile(steps_ms, 50)),
+        "p99_step_ms": float(np.percentile(steps_ms, 99)),
+        "roofline": roofline(V.MIXED, mixed, N, W),
+    }
+    per_scheme = {}
+    for j, s in enumerate(schemes):
+        ms = mixed["per_filter_step_ms"][:, j]
+        h = mixed["hist"][j]
+        per_scheme[SCHEME_NAMES[s]] = {
+            "updates_per_s": N * K / (ms.sum() * 1e-3), "p50_ms": float(np.percentile(ms, 50)),
+            "p99_ms": float(np.percentile(ms, 99)),
+            "resample_frac": float(h["resampled"][W:].mean()), "log_z": float(h["log_z"][-1])}
+    out["per_scheme"] = per_scheme
+    for prec, name in ((V.FP64_FUSED, "fp64_fused"), (V.FP64_PLAIN, "fp64_plain")):
+        if prec in res:
+            r = res[prec]
+            v = nf * N * K / (r["total_ms"] * 1e-3)
+            out[name] = {"value": v, "ms_per_step": r["total_ms"] / K,
+                         "roofline": roofline(prec, r, N, W),
+                         "log_z": {SCHEME_NAMES[s]: float(r["hist"][j]["log_z"][-1])
+                                   for j, s in enumerate(schemes)}}
+    if V.FP64_PLAIN in res:
+        out["speedup_mixed_vs_fp64_plain"] = value / out["fp64_plain"]["value"]
+    if V.FP64_FUSED in res:
+        out["speedup_mixed_vs_fp64_fused"] = value / out["fp64_fused"]["value"]
+    if not a.no_cpu:
+        threads = min(16, os.cpu_count() or 1)
+        print(f"[bench] cpu baseline: N={a.cpu_n}, {a.cpu_steps} steps, {threads} threads",
+              file=sys.stderr, flush=True)
+        out["cpu_baseline"] = cpu_baseline(a.cpu_n, a.cpu_steps, threads)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

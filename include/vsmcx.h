@@ -152,6 +152,20 @@ int vsmcx_filter_engine(vsmcx_filter *f, vsmcx_threefry4x64_state *eng);
 int vsmcx_filter_time(vsmcx_filter *f, uint64_t nsteps, double *elapsed_ms,
                       double *move_kernel_ms);
 
+/* ---------------------------------------------------------------- timing
+ * HIP events on the context stream.  A timer holds n marks; elapsed()
+ * synchronises on the later mark.  Profiling a filter brackets every move
+ * kernel launch with an event pair; move_times() returns their durations in
+ * launch order and forgets them.  No vSMC counterpart (vSMC times whole runs
+ * with StopWatch, include/vsmc/utility/stop_watch.hpp:117-221). */
+typedef struct vsmcx_timer vsmcx_timer;
+int vsmcx_timer_create(vsmcx_ctx *ctx, uint64_t n, vsmcx_timer **out);
+int vsmcx_timer_record(vsmcx_timer *tm, uint64_t i);
+int vsmcx_timer_elapsed(vsmcx_timer *tm, uint64_t i, uint64_t j, double *ms);
+int vsmcx_timer_destroy(vsmcx_timer *tm);
+int vsmcx_filter_profile(vsmcx_filter *f, int enable);
+int vsmcx_filter_move_times(vsmcx_filter *f, uint64_t max, uint64_t *count, double *ms);
+
 /* ---------------------------------------------------------------- debug
  * Hooks for parity tests: run one step but stop after the resampling
  * decision and expose the normalised weights W (n doubles) the resampler

@@ -93,6 +93,12 @@ def lib():
         "vsmcx_filter_engine": ([vp, P(ThreefryState)], i32),
         "vsmcx_filter_time": ([vp, u64, P(f64), P(f64)], i32),
         "vsmcx_filter_debug_step": ([vp, vp, vp, vp, vp, P(u64), P(ThreefryState), P(i32)], i32),
+        "vsmcx_timer_create": ([vp, u64, P(vp)], i32),
+        "vsmcx_timer_record": ([vp, u64], i32),
+        "vsmcx_timer_elapsed": ([vp, u64, u64, P(f64)], i32),
+        "vsmcx_timer_destroy": ([vp], i32),
+        "vsmcx_filter_profile": ([vp, i32], i32),
+        "vsmcx_filter_move_times": ([vp, u64, P(u64), vp], i32),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)
@@ -184,6 +190,33 @@ class Context:
         return out
 
 
+class Timer:
+    """n HIP-event marks on a context stream."""
+
+    def __init__(self, ctx, n):
+        self.h = C.c_void_p()
+        _check(lib().vsmcx_timer_create(ctx.h, n, C.byref(self.h)))
+
+    def record(self, i):
+        _check(lib().vsmcx_timer_record(self.h, i))
+
+    def elapsed(self, i, j):
+        v = C.c_double()
+        _check(lib().vsmcx_timer_elapsed(self.h, i, j, C.byref(v)))
+        return v.value
+
+    def close(self):
+        if self.h:
+            lib().vsmcx_timer_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def cv_llh_const(nu=10.0, scale=0.1):
     """log normaliser of one Student-t(nu, scale) coordinate."""
     return (math.lgamma((nu + 1) / 2) - math.lgamma(nu / 2) - 0.5 * math.log(nu * math.pi)
@@ -259,6 +292,15 @@ class Filter:
         el, mv = C.c_double(), C.c_double()
         _check(lib().vsmcx_filter_time(self.h, nsteps, C.byref(el), C.byref(mv)))
         return el.value, mv.value
+
+    def profile(self, enable=True):
+        _check(lib().vsmcx_filter_profile(self.h, 1 if enable else 0))
+
+    def move_times(self, max_n=1 << 20):
+        out = np.zeros(max_n)
+        n = C.c_uint64()
+        _check(lib().vsmcx_filter_move_times(self.h, max_n, C.byref(n), out.ctypes.data))
+        return out[:n.value].copy()
 
     def debug_step(self):
         n = self.desc.n

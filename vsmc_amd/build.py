@@ -16,7 +16,8 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17", "-ffp-contract=off",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
          "-Wno-unused-command-line-argument"]
-SOURCES = ["kernels_rng.hip", "kernels_fold.hip", "kernels_cv.hip", "capi.cpp"]
+SOURCES = ["kernels_rng.hip", "kernels_fold.hip", "kernels_cv.hip", "kernels_plain.hip",
+           "capi.cpp"]
 
 
 def _compile(src):

@@ -122,7 +122,8 @@ void alloc_resample(vsmcx_ctx *c, ResampleBuffers &rb, uint64_t M, uint64_t nuni
     rb.nuni = nuni;
     FoldScratch &fs = rb.fs;
     fs.flags_sum = c->alloc_zero<uint64_t>(tiles);
-    fs.val_sum = c->alloc_n<double>(tiles);
+    fs.agg_sum = c->alloc_n<double>(tiles);
+    fs.incl_sum = c->alloc_n<double>(tiles);
     fs.flags_fold = c->alloc_zero<uint64_t>(tiles);
     fs.val_fold = c->alloc_n<double>(tiles);
     fs.agg_d0 = c->alloc_n<uint64_t>(tiles);
@@ -130,7 +131,8 @@ void alloc_resample(vsmcx_ctx *c, ResampleBuffers &rb, uint64_t M, uint64_t nuni
     fs.agg_pp = c->alloc_n<uint32_t>(tiles);
     fs.agg_b = c->alloc_n<int32_t>(tiles);
     fs.flags_cnt = c->alloc_zero<uint64_t>(tiles);
-    fs.val_cnt = c->alloc_n<uint64_t>(tiles);
+    fs.agg_cnt = c->alloc_n<uint64_t>(tiles);
+    fs.incl_cnt = c->alloc_n<uint64_t>(tiles);
     fs.tile_counter = c->alloc_zero<unsigned long long>(1);
     fs.epoch = c->alloc_zero<uint64_t>(1);
     fs.tiles = tiles;
@@ -139,9 +141,10 @@ void alloc_resample(vsmcx_ctx *c, ResampleBuffers &rb, uint64_t M, uint64_t nuni
 void free_resample(vsmcx_ctx *c, ResampleBuffers &rb)
 {
     void *ps[] = {rb.anc, rb.zl, rb.sl, rb.rep_out, rb.a_out, rb.u_tsum, rb.u_texcl, rb.uscan,
-                  rb.u_tmax, rb.u_tmaxex, rb.fs.flags_sum, rb.fs.val_sum, rb.fs.flags_fold,
-                  rb.fs.val_fold, rb.fs.agg_d0, rb.fs.agg_d1, rb.fs.agg_pp, rb.fs.agg_b,
-                  rb.fs.flags_cnt, rb.fs.val_cnt, rb.fs.tile_counter, rb.fs.epoch};
+                  rb.u_tmax, rb.u_tmaxex, rb.fs.flags_sum, rb.fs.agg_sum, rb.fs.incl_sum,
+                  rb.fs.flags_fold, rb.fs.val_fold, rb.fs.agg_d0, rb.fs.agg_d1, rb.fs.agg_pp,
+                  rb.fs.agg_b, rb.fs.flags_cnt, rb.fs.agg_cnt, rb.fs.incl_cnt,
+                  rb.fs.tile_counter, rb.fs.epoch};
     for (void *p : ps) c->release(p);
     rb = ResampleBuffers();
 }
@@ -199,20 +202,63 @@ struct vsmcx_filter {
     StepRow *hist = nullptr;
     ResampleBuffers rb;
     std::vector<hipEvent_t> events;
+    // move-kernel profiling: event pairs around each move launch
+    bool profiling = false;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> prof_pool;
+    size_t prof_used = 0;
+    // plain FP64 pipeline only
+    bool plain = false;
+    double *w = nullptr, *inc = nullptr, *ppart = nullptr, *scal = nullptr;
 
     size_t state_bytes() const { return fp64 ? 32 : 16; }
 
+    // the move kernel of step t (the dominant kernel of the pipeline)
+    void launch_move()
+    {
+        hipStream_t s = ctx->stream;
+        if (plain)
+            launch_plain_move(st[0], inc, obs, p, d.seed, t, N, s);
+        else
+            launch_cv_move(fp64, ctl, st[cur], st[1 - cur], rb.anc, lw, obs, p, d.seed, t, N,
+                           partials, s);
+    }
+    // everything after the move: weights, decision, resampling, monitor
+    void launch_rest()
+    {
+        hipStream_t s = ctx->stream;
+        if (plain) {
+            launch_plain_rest(ctl, st[0], lw, w, inc, ppart, scal, t, N, d.scheme, rb, hist, s);
+        } else {
+            launch_cv_reduce(ctl, partials, tiles, hist, t, s);
+            launch_resample_from_log_weights(ctl, lw, N, d.scheme, rb, hist, s);
+            cur ^= 1;
+        }
+        ++t;
+    }
     void step()
     {
         if (t >= nobs) throw DevError(VSMCX_ERR_STATE, "no observation for step " + std::to_string(t));
-        hipStream_t s = ctx->stream;
-        launch_cv_move(fp64, ctl, st[cur], st[1 - cur], rb.anc, lw, obs, p, d.seed, t, N, partials,
-                       s);
-        launch_cv_reduce(ctl, partials, tiles, hist, t, s);
-        launch_resample_from_log_weights(ctl, lw, N, d.scheme, rb, hist, s);
-        cur ^= 1;
-        ++t;
+        if (profiling) {
+            if (prof_used == prof_pool.size()) {
+                hipEvent_t a, b;
+                VSMCX_HIP(hipEventCreate(&a));
+                VSMCX_HIP(hipEventCreate(&b));
+                prof_pool.emplace_back(a, b);
+            }
+            auto &ev = prof_pool[prof_used++];
+            VSMCX_HIP(hipEventRecord(ev.first, ctx->stream));
+            launch_move();
+            VSMCX_HIP(hipEventRecord(ev.second, ctx->stream));
+        } else {
+            launch_move();
+        }
+        launch_rest();
     }
+};
+
+struct vsmcx_timer {
+    vsmcx_ctx *ctx = nullptr;
+    std::vector<hipEvent_t> ev;
 };
 
 extern "C" {
@@ -418,12 +464,14 @@ int vsmcx_filter_create(vsmcx_ctx *ctx, const vsmcx_filter_desc *desc, vsmcx_fil
     REQUIRE(desc->n >= 1 && desc->n < (1ull << 31), "n must be in [1, 2^31)");
     REQUIRE(desc->horizon >= 1, "horizon must be >= 1");
     REQUIRE(scheme_ok(desc->scheme), "unknown resampling scheme");
-    REQUIRE(desc->precision == VSMCX_MIXED || desc->precision == VSMCX_FP64_FUSED,
-            "precision must be VSMCX_MIXED or VSMCX_FP64_FUSED");
+    REQUIRE(desc->precision == VSMCX_MIXED || desc->precision == VSMCX_FP64_FUSED ||
+                desc->precision == VSMCX_FP64_PLAIN,
+            "precision must be VSMCX_MIXED, VSMCX_FP64_FUSED or VSMCX_FP64_PLAIN");
     auto f = std::make_unique<vsmcx_filter>();
     f->ctx = ctx;
     f->d = *desc;
     f->fp64 = desc->precision != VSMCX_MIXED;
+    f->plain = desc->precision == VSMCX_FP64_PLAIN;
     f->p = CvParams{desc->dt, desc->sd_pos, desc->sd_vel, desc->prior_sd_pos,
                     desc->prior_sd_vel, desc->obs_scale, desc->nu, desc->llh_const};
     f->N = desc->n;
@@ -431,7 +479,14 @@ int vsmcx_filter_create(vsmcx_ctx *ctx, const vsmcx_filter_desc *desc, vsmcx_fil
     f->tiles = (f->N + kTile - 1) / kTile;
     try {
         f->st[0] = ctx->alloc(f->N * f->state_bytes());
-        f->st[1] = ctx->alloc(f->N * f->state_bytes());
+        if (!f->plain) f->st[1] = ctx->alloc(f->N * f->state_bytes());
+        if (f->plain) {
+            f->w = ctx->alloc_n<double>(f->N);
+            f->inc = ctx->alloc_n<double>(f->N);
+            f->ppart = ctx->alloc_n<double>(plain_partials(f->N));
+            f->scal = ctx->alloc_zero<double>(8);
+            launch_plain_init_weights(f->w, f->N, ctx->stream);
+        }
         f->lw = ctx->alloc_n<double>(f->N);
         f->partials = ctx->alloc_n<double>(5 * f->tiles);
         f->obs = ctx->alloc_n<double>(2 * f->T);
@@ -440,7 +495,8 @@ int vsmcx_filter_create(vsmcx_ctx *ctx, const vsmcx_filter_desc *desc, vsmcx_fil
         alloc_resample(ctx, f->rb, f->N, f->N, true);
     } catch (...) {
         for (void *p : {f->st[0], f->st[1], (void *)f->lw, (void *)f->partials, (void *)f->obs,
-                        (void *)f->ctl, (void *)f->hist})
+                        (void *)f->ctl, (void *)f->hist, (void *)f->w, (void *)f->inc,
+                        (void *)f->ppart, (void *)f->scal})
             ctx->release(p);
         free_resample(ctx, f->rb);
         throw;
@@ -464,10 +520,15 @@ int vsmcx_filter_destroy(vsmcx_filter *f)
     vsmcx_ctx *c = f->ctx;
     (void)hipStreamSynchronize(c->stream);
     for (void *p : {f->st[0], f->st[1], (void *)f->lw, (void *)f->partials, (void *)f->obs,
-                    (void *)f->ctl, (void *)f->hist})
+                    (void *)f->ctl, (void *)f->hist, (void *)f->w, (void *)f->inc,
+                    (void *)f->ppart, (void *)f->scal})
         c->release(p);
     free_resample(c, f->rb);
     for (hipEvent_t e : f->events) (void)hipEventDestroy(e);
+    for (auto &pr : f->prof_pool) {
+        (void)hipEventDestroy(pr.first);
+        (void)hipEventDestroy(pr.second);
+    }
     delete f;
     API_END
 }
@@ -497,7 +558,7 @@ int vsmcx_filter_finish(vsmcx_filter *f)
 {
     API_BEGIN
     REQUIRE(f, "NULL filter");
-    if (f->t > 0)
+    if (f->t > 0 && !f->plain)
         launch_cv_monitor(f->fp64, f->ctl, f->st[f->cur], f->rb.anc, f->lw, f->N, f->partials,
                           f->hist, f->ctx->stream);
     VSMCX_HIP(hipGetLastError());
@@ -533,7 +594,10 @@ int vsmcx_filter_state(vsmcx_filter *f, double *state, double *log_weight)
     API_BEGIN
     REQUIRE(f, "NULL filter");
     const uint64_t N = f->N;
-    if (state) {
+    if (state && f->plain) {
+        VSMCX_HIP(hipMemcpyAsync(state, f->st[0], 32 * N, hipMemcpyDeviceToHost, f->ctx->stream));
+        f->ctx->sync();
+    } else if (state) {
         void *tmp = f->ctx->alloc(N * f->state_bytes());
         launch_cv_gather(f->fp64, f->ctl, f->st[f->cur], tmp, f->rb.anc, N, f->ctx->stream);
         if (f->fp64) {
@@ -553,9 +617,11 @@ int vsmcx_filter_state(vsmcx_filter *f, double *state, double *log_weight)
         VSMCX_HIP(hipMemcpyAsync(log_weight, f->lw, 8 * N, hipMemcpyDeviceToHost, f->ctx->stream));
         f->ctx->sync();
         // vSMC keeps log-weights normalised to max 0 (core/weight_set.hpp:108-117);
-        // after a resample they are all zero (set_equal_weight)
-        for (uint64_t i = 0; i < N; ++i)
-            log_weight[i] = h.prev_resampled ? 0.0 : log_weight[i] - h.m;
+        // after a resample they are all zero (set_equal_weight).  The plain
+        // pipeline stores them that way already.
+        if (!f->plain)
+            for (uint64_t i = 0; i < N; ++i)
+                log_weight[i] = h.prev_resampled ? 0.0 : log_weight[i] - h.m;
     }
     API_END
 }
@@ -586,14 +652,10 @@ int vsmcx_filter_time(vsmcx_filter *f, uint64_t nsteps, double *elapsed_ms, doub
     VSMCX_HIP(hipEventRecord(f->events[0], s));
     for (uint64_t k = 0; k < nsteps; ++k) {
         if (f->t >= f->nobs) throw DevError(VSMCX_ERR_STATE, "out of observations");
-        launch_cv_move(f->fp64, f->ctl, f->st[f->cur], f->st[1 - f->cur], f->rb.anc, f->lw,
-                       f->obs, f->p, f->d.seed, f->t, f->N, f->partials, s);
+        f->launch_move();
         VSMCX_HIP(hipEventRecord(f->events[2 + 2 * k], s));
-        launch_cv_reduce(f->ctl, f->partials, f->tiles, f->hist, f->t, s);
-        launch_resample_from_log_weights(f->ctl, f->lw, f->N, f->d.scheme, f->rb, f->hist, s);
+        f->launch_rest();
         VSMCX_HIP(hipEventRecord(f->events[3 + 2 * k], s));
-        f->cur ^= 1;
-        ++f->t;
     }
     VSMCX_HIP(hipEventRecord(f->events[1], s));
     VSMCX_HIP(hipGetLastError());
@@ -618,12 +680,78 @@ int vsmcx_filter_time(vsmcx_filter *f, uint64_t nsteps, double *elapsed_ms, doub
     API_END
 }
 
+int vsmcx_timer_create(vsmcx_ctx *ctx, uint64_t n, vsmcx_timer **out)
+{
+    API_BEGIN
+    REQUIRE(ctx && out && n >= 1, "bad argument");
+    auto tm = std::make_unique<vsmcx_timer>();
+    tm->ctx = ctx;
+    tm->ev.resize(n);
+    for (auto &e : tm->ev) VSMCX_HIP(hipEventCreate(&e));
+    *out = tm.release();
+    API_END
+}
+
+int vsmcx_timer_record(vsmcx_timer *tm, uint64_t i)
+{
+    API_BEGIN
+    REQUIRE(tm && i < tm->ev.size(), "bad timer mark");
+    VSMCX_HIP(hipEventRecord(tm->ev[i], tm->ctx->stream));
+    API_END
+}
+
+int vsmcx_timer_elapsed(vsmcx_timer *tm, uint64_t i, uint64_t j, double *ms)
+{
+    API_BEGIN
+    REQUIRE(tm && ms && i < tm->ev.size() && j < tm->ev.size(), "bad timer mark");
+    VSMCX_HIP(hipEventSynchronize(tm->ev[j]));
+    float v = 0;
+    VSMCX_HIP(hipEventElapsedTime(&v, tm->ev[i], tm->ev[j]));
+    *ms = v;
+    API_END
+}
+
+int vsmcx_timer_destroy(vsmcx_timer *tm)
+{
+    API_BEGIN
+    if (!tm) return VSMCX_OK;
+    for (auto &e : tm->ev) (void)hipEventDestroy(e);
+    delete tm;
+    API_END
+}
+
+int vsmcx_filter_profile(vsmcx_filter *f, int enable)
+{
+    API_BEGIN
+    REQUIRE(f, "NULL filter");
+    f->profiling = enable != 0;
+    f->prof_used = 0;
+    API_END
+}
+
+int vsmcx_filter_move_times(vsmcx_filter *f, uint64_t max, uint64_t *count, double *ms)
+{
+    API_BEGIN
+    REQUIRE(f && count, "NULL argument");
+    f->ctx->sync();
+    const uint64_t n = f->prof_used < max ? f->prof_used : max;
+    for (uint64_t k = 0; k < n; ++k) {
+        float v = 0;
+        VSMCX_HIP(hipEventElapsedTime(&v, f->prof_pool[k].first, f->prof_pool[k].second));
+        if (ms) ms[k] = v;
+    }
+    *count = n;
+    f->prof_used = 0;
+    API_END
+}
+
 int vsmcx_filter_debug_step(vsmcx_filter *f, double *weight, uint64_t *replication,
                             uint64_t *copy_from, double *uniforms, uint64_t *nn,
                             vsmcx_threefry4x64_state *eng_before, int *resampled)
 {
     API_BEGIN
     REQUIRE(f, "NULL filter");
+    REQUIRE(!f->plain, "debug_step needs a fused pipeline");
     REQUIRE(f->t < f->nobs, "out of observations");
     vsmcx_ctx *c = f->ctx;
     hipStream_t s = c->stream;

@@ -33,6 +33,7 @@ enum DevErr : int {
     kErrDegenerate = 2,      // weights sum to zero / NaN
     kErrWindow = 4,          // inversion window inconsistency
     kErrTimeout = 8,         // a look-back wait exceeded its spin limit
+    kErrBounds = 16,         // a computed rank fell outside its buffer
 };
 
 // Per-filter control block, resident in device memory.  Kernels of one step
@@ -72,7 +73,8 @@ struct StepRow {
 // Look-back bookkeeping for the exact-fold inversion kernel.
 struct FoldScratch {
     uint64_t *flags_sum;   // approximate-sum look-back  [tiles]
-    double *val_sum;       //                            [tiles]
+    double *agg_sum;       // tile sum (published with kStAgg)
+    double *incl_sum;      // inclusive prefix (published with kStIncl)
     uint64_t *flags_fold;  // exact-fold look-back       [tiles]
     double *val_fold;      // inclusive exact value      [tiles]
     uint64_t *agg_d0;      // clean-tile transducer      [tiles]
@@ -80,7 +82,8 @@ struct FoldScratch {
     uint32_t *agg_pp;
     int32_t *agg_b;
     uint64_t *flags_cnt;   // zeros/surplus look-back    [tiles]
-    uint64_t *val_cnt;
+    uint64_t *agg_cnt;     // tile counts
+    uint64_t *incl_cnt;    // inclusive counts
     unsigned long long *tile_counter;  // device: tiles claimed in this launch
     uint64_t *epoch;                   // device: look-back epoch of this launch
     size_t tiles;

@@ -137,7 +137,9 @@ __global__ __launch_bounds__(kBlock) void k_cv_move(
         S4 s;
         double lwo = 0;
         if (t > 0) {
-            s = st_in[prev ? (uint64_t)anc[i] : i];
+            uint64_t src = prev ? (uint64_t)anc[i] : i;
+            src = src < N ? src : i;  // never read outside the state buffer
+            s = st_in[src];
             if (!prev) lwo = lw[i];
             const double w = prev ? invN : exp(lwo - m_old) * c_old;
             mx += w * (double)s.x;

@@ -275,7 +275,10 @@ __global__ __launch_bounds__(kBlock) void k_fold_inv(FilterCtl *ctl, XSrc xs, In
     double tile_sum;
     block_incl_scan<kBlock>(tsum, AddOp(), s_red, &tile_sum);
     if (tid == 0) {
-        fs.val_sum[t] = tile_sum;
+        if (t == 0)
+            fs.incl_sum[t] = tile_sum;
+        else
+            fs.agg_sum[t] = tile_sum;
         lb_publish(fs.flags_sum, t, epoch, t == 0 ? kStIncl : kStAgg);
     }
     if (tid < kWave) {
@@ -283,14 +286,14 @@ __global__ __launch_bounds__(kBlock) void k_fold_inv(FilterCtl *ctl, XSrc xs, In
         if (t > 0)
             S = lookback<double, double>(
                 t, fs.flags_sum, epoch, 0.0, 0.0,
-                [&](int64_t q) { return fs.val_sum[q]; },
-                [&](int64_t q) { return fs.val_sum[q]; },
+                [&](int64_t q) { return fs.agg_sum[q]; },
+                [&](int64_t q) { return fs.incl_sum[q]; },
                 [](double a, double b) { return a + b; },
                 [](double a, double b) { return a + b; }, &s_timeout);
         if (tid == 0) {
             s_bcast_d[0] = S;
             if (t > 0) {
-                fs.val_sum[t] = S + tile_sum;
+                fs.incl_sum[t] = S + tile_sum;
                 lb_publish(fs.flags_sum, t, epoch, kStIncl);
             }
         }
@@ -555,7 +558,10 @@ __global__ __launch_bounds__(kBlock) void k_fold_inv(FilterCtl *ctl, XSrc xs, In
         const uint64_t zs_incl = block_incl_scan<kBlock>(zs_thr, AddOp(), s_u64, &zs_tile);
         const uint64_t zs_excl = zs_incl - zs_thr;
         if (tid == 0) {
-            fs.val_cnt[t] = zs_tile;
+            if (t == 0)
+                fs.incl_cnt[t] = zs_tile;
+            else
+                fs.agg_cnt[t] = zs_tile;
             lb_publish(fs.flags_cnt, t, epoch, t == 0 ? kStIncl : kStAgg);
         }
         if (tid < kWave) {
@@ -563,14 +569,14 @@ __global__ __launch_bounds__(kBlock) void k_fold_inv(FilterCtl *ctl, XSrc xs, In
             if (t > 0)
                 pre = lookback<uint64_t, uint64_t>(
                     t, fs.flags_cnt, epoch, 0ull, 0ull,
-                    [&](int64_t q) { return fs.val_cnt[q]; },
-                    [&](int64_t q) { return fs.val_cnt[q]; },
+                    [&](int64_t q) { return fs.agg_cnt[q]; },
+                    [&](int64_t q) { return fs.incl_cnt[q]; },
                     [](uint64_t a, uint64_t b) { return a + b; },
                     [](uint64_t a, uint64_t b) { return a + b; }, &s_timeout);
             if (tid == 0) {
                 s_bcast_u[2] = pre;
                 if (t > 0) {
-                    fs.val_cnt[t] = pre + zs_tile;
+                    fs.incl_cnt[t] = pre + zs_tile;
                     lb_publish(fs.flags_cnt, t, epoch, kStIncl);
                 }
                 if (t == fs.tiles - 1) ctl->zeros = (pre + zs_tile) >> 32;
@@ -592,7 +598,13 @@ __global__ __launch_bounds__(kBlock) void k_fold_inv(FilterCtl *ctl, XSrc xs, In
             const uint64_t i = base + e;
             if (i < M) {
                 out.anc[i] = (uint32_t)i;
-                if (rep[k] == 0) out.zl[zrank++] = (uint32_t)i;
+                if (rep[k] == 0) {
+                    if (zrank < M)
+                        out.zl[zrank] = (uint32_t)i;
+                    else
+                        atomicOr(&ctl->err, (int)kErrBounds);
+                    ++zrank;
+                }
                 if (rep[k] >= 2) srank_local += rep[k] - 1;
             }
             s_sp[e] = (uint32_t)srank_local;
@@ -609,7 +621,10 @@ __global__ __launch_bounds__(kBlock) void k_fold_inv(FilterCtl *ctl, XSrc xs, In
                 else
                     lo = mid + 1;
             }
-            out.sl[s_base + k] = (uint32_t)(base + lo);
+            if (s_base + k < M)
+                out.sl[s_base + k] = (uint32_t)(base + lo);
+            else
+                atomicOr(&ctl->err, (int)kErrBounds);
         }
     }
 }
@@ -618,13 +633,19 @@ __global__ __launch_bounds__(kBlock) void k_fold_inv(FilterCtl *ctl, XSrc xs, In
 // zero-slot fill + engine bookkeeping
 // ------------------------------------------------------------------------
 __global__ __launch_bounds__(kBlock) void k_zero_fill(FilterCtl *ctl, const uint32_t *zl,
-                                                      const uint32_t *sl, uint32_t *anc)
+                                                      const uint32_t *sl, uint32_t *anc,
+                                                      uint64_t M)
 {
     if (!ctl->resample) return;
-    const uint64_t z = ctl->zeros;
+    const uint64_t z = ctl->zeros < M ? ctl->zeros : M;
     for (uint64_t k = blockIdx.x * (uint64_t)kBlock + threadIdx.x; k < z;
-         k += (uint64_t)gridDim.x * kBlock)
-        anc[zl[k]] = sl[k];
+         k += (uint64_t)gridDim.x * kBlock) {
+        const uint32_t to = zl[k], from = sl[k];
+        if (to < M && from < M)
+            anc[to] = from;
+        else
+            atomicOr(&ctl->err, (int)kErrBounds);
+    }
 }
 
 // engine bookkeeping: vSMC's lazy sequences draw min(nn, n_final + 1)
@@ -851,7 +872,7 @@ static void resample_pipeline(FilterCtl *ctl, W w, uint64_t M, int scheme, Resam
     if (rb.anc) {
         const unsigned zg = (unsigned)((M + kTile - 1) / kTile) * 4;
         hipLaunchKernelGGL(k_zero_fill, dim3(zg ? zg : 1), dim3(kBlock), 0, st, ctl, rb.zl, rb.sl,
-                           rb.anc);
+                           rb.anc, M);
     }
     hipLaunchKernelGGL(k_resample_epilogue, dim3(1), dim3(64), 0, st, ctl, hist,
                        (scheme == kSystematic || scheme == kResidualSystematic) ? 1 : 0, M);

@@ -59,4 +59,13 @@ void launch_cv_weights(FilterCtl *ctl, const double *lw, uint64_t N, double *w, 
 void launch_cv_gather(bool fp64, FilterCtl *ctl, const void *st_in, void *st_out,
                       const uint32_t *anc, uint64_t N, hipStream_t st);
 
+// kernels_plain.hip: the un-fused FP64 baseline
+void launch_plain_init_weights(double *w, uint64_t N, hipStream_t s);
+void launch_plain_move(void *st, double *inc, const double *obs, const CvParams &p, uint64_t seed,
+                       uint64_t t, uint64_t N, hipStream_t s);
+void launch_plain_rest(FilterCtl *ctl, void *st, double *lw, double *w, double *inc,
+                       double *part, double *scal, uint64_t t, uint64_t N, int scheme,
+                       ResampleBuffers &rb, StepRow *hist, hipStream_t s);
+uint64_t plain_partials(uint64_t N);
+
 }  // namespace vsmcx
