@@ -82,7 +82,7 @@ typedef struct {
  *   std::function<void(size_t M, size_t N, Threefry4x64 &rng,
  *                      const double *weight, size_t *replication)>
  * (include/vsmc/core/particle.hpp:61-62; Resample<Scheme>::operator() in
- * include/vsmc/resample/*.hpp:53-77).  Host pointers, synchronous.  `eng`
+ * include/vsmc/resample/{multinomial,residual,...}.hpp:53-77).  Host pointers, synchronous.  `eng`
  * is advanced by exactly the draws vSMC makes.  `replication` receives M
  * counts summing to N.  When copy_from is not NULL (requires M == N) it also
  * receives internal::cfrp_trans's layout (include/vsmc/resample/common.hpp:

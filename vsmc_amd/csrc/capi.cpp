@@ -107,8 +107,8 @@ namespace {
 // Fold / resample scratch for up to M particles (and nuni uniforms).
 void alloc_resample(vsmcx_ctx *c, ResampleBuffers &rb, uint64_t M, uint64_t nuni, bool lists)
 {
-    const uint64_t tiles = (M + kTile - 1) / kTile;
-    const uint64_t utiles = (nuni + kTile - 1) / kTile + 1;
+    const uint64_t tiles = (M + kFoldTile - 1) / kFoldTile;  // k_fold_inv tiles
+    const uint64_t utiles = (nuni + kTile - 1) / kTile + 1;   // sorted-uniform tiles
     if (lists) {
         rb.anc = c->alloc_n<uint32_t>(M);
         rb.zl = c->alloc_n<uint32_t>(M);

@@ -90,14 +90,20 @@ VSMCX_HD uint64_t sat_add(uint64_t a, uint64_t b)
     return s > kSat ? kSat : s;
 }
 
+// branch-free (p ? b : a) for p in {0, 1}; keeps the pair in registers
+VSMCX_HD uint64_t pick(uint64_t a, uint64_t b, uint64_t p)
+{
+    return a ^ ((a ^ b) & (0ull - p));
+}
+
 // f then g.  Binade tags must agree when both are non-empty; a mismatch is
 // reported by tagging the result with kNoBinade - 1.
 VSMCX_HD Tr tr_compose(const Tr &f, const Tr &g)
 {
     Tr r;
     const uint32_t f0 = f.pp & 1u, f1 = (f.pp >> 1) & 1u;
-    r.d0 = sat_add(f.d0, f0 ? g.d1 : g.d0);
-    r.d1 = sat_add(f.d1, f1 ? g.d1 : g.d0);
+    r.d0 = sat_add(f.d0, pick(g.d0, g.d1, f0));
+    r.d1 = sat_add(f.d1, pick(g.d0, g.d1, f1));
     const uint32_t p0 = (g.pp >> f0) & 1u, p1 = (g.pp >> f1) & 1u;
     r.pp = p0 | (p1 << 1);
     if (f.b == kNoBinade)
@@ -151,6 +157,28 @@ VSMCX_HD Tr tr_element(double x, int b)
     return t;
 }
 
+#if defined(__HIPCC__)
+// field-wise cross-lane moves (see device_util.hpp)
+__device__ __forceinline__ Tr lane_up(const Tr &v, unsigned d)
+{
+    Tr r;
+    r.d0 = (uint64_t)__shfl_up((unsigned long long)v.d0, d, 64);
+    r.d1 = (uint64_t)__shfl_up((unsigned long long)v.d1, d, 64);
+    r.pp = (uint32_t)__shfl_up((int)v.pp, d, 64);
+    r.b = __shfl_up(v.b, d, 64);
+    return r;
+}
+__device__ __forceinline__ Tr lane_get(const Tr &v, int l)
+{
+    Tr r;
+    r.d0 = (uint64_t)__shfl((unsigned long long)v.d0, l, 64);
+    r.d1 = (uint64_t)__shfl((unsigned long long)v.d1, l, 64);
+    r.pp = (uint32_t)__shfl((int)v.pp, l, 64);
+    r.b = __shfl(v.b, l, 64);
+    return r;
+}
+#endif
+
 // Apply a transducer to an exact running value v.  Returns false if v is not
 // in the segment's binade or the result would leave it (never happens when
 // the classification bound holds).
@@ -162,7 +190,7 @@ VSMCX_HD bool tr_apply(const Tr &t, double v, double *out)
     }
     if (t.b == kNoBinade - 1 || binade_of(v) != t.b) return false;
     const uint64_t A = units_of(v);
-    const uint64_t A2 = A + ((A & 1) ? t.d1 : t.d0);
+    const uint64_t A2 = A + pick(t.d0, t.d1, A & 1);
     if (A2 >= kTwo53) return false;
     *out = value_of(t.b, A2);
     return true;

@@ -15,6 +15,8 @@ namespace vsmcx {
 constexpr int kBlock = 256;                  // threads per workgroup
 constexpr int kItems = 16;                   // particles per thread
 constexpr int kTile = kBlock * kItems;       // particles per workgroup tile
+constexpr int kFoldItems = 8;                // elements per thread in k_fold_inv
+constexpr int kFoldTile = kBlock * kFoldItems;  // elements per k_fold_inv tile
 
 // Resampling schemes, same order as vsmc::ResampleScheme
 // (include/vsmc/resample/common.hpp:124-131).

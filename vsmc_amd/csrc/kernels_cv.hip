@@ -104,6 +104,10 @@ struct CvModel {
 };
 
 struct Part5 { double se, se2, mx, my; };
+__device__ __forceinline__ Part5 lane_xor(const Part5 &v, int m)
+{
+    return Part5{lane_xor(v.se, m), lane_xor(v.se2, m), lane_xor(v.mx, m), lane_xor(v.my, m)};
+}
 struct Part5Add {
     __device__ Part5 operator()(const Part5 &a, const Part5 &b) const
     {

@@ -93,6 +93,12 @@ struct InvSystematic {
     }
 };
 
+// u_n of the stratified sequence, kept out of line (20 Threefry rounds)
+__device__ __noinline__ double strat_u(const Tf64State &eng, double delta, uint64_t n)
+{
+    return canonical64(tf64_stream_word(eng, n)) * delta + (double)n * delta;
+}
+
 // U01SequenceStratified (u01.hpp:212-243): u_n = U_n*delta + n*delta, with
 // U_n the n-th draw of the resample engine; v_n = max(u_n, u_{n-1}) gives
 // inversion's running-max semantics (u_{n-2} < u_n always).
@@ -103,10 +109,7 @@ struct InvStratified {
     double delta;
     uint64_t nn;
     __device__ void bind(const FilterCtl *c) { eng = c->eng; delta = c->delta; nn = c->nn; }
-    __device__ double u(uint64_t n) const
-    {
-        return canonical64(tf64_stream_word(eng, n)) * delta + (double)n * delta;
-    }
+    __device__ double u(uint64_t n) const { return strat_u(eng, delta, n); }
     __device__ double v(uint64_t n) const
     {
         const double a = u(n);
@@ -161,7 +164,7 @@ enum : uint64_t { kStNone = 0, kStAgg = 1, kStIncl = 2, kStNC = 3 };
 
 __device__ __forceinline__ uint32_t lb_state(const uint64_t *flags, int64_t q, uint64_t epoch)
 {
-    const uint64_t f = atomic_load_acq(flags + q);
+    const uint64_t f = atomic_load_rlx(flags + q);
     return (f >> 2) == epoch ? (uint32_t)(f & 3) : (uint32_t)kStNone;
 }
 
@@ -192,24 +195,36 @@ __device__ V lookback(uint64_t t, const uint64_t *flags, uint64_t epoch, V start
         const int l_incl = incl ? 63 - __clzll(incl) : -1;
         const uint64_t notready = __ballot(lane > l_incl && st != kStAgg);
         if (notready) {
-            // bounded wait: a predecessor that never publishes is a bug; report
-            // it and leave instead of hanging the device
-            if (++spins > kSpinLimit) {
-                *timed_out = 1;
-                return start;
+            // Wait on the nearest blocking predecessor alone (one broadcast load
+            // per poll instead of 64), backing off, then rescan the window.
+            // Bounded: a predecessor that never publishes is a bug; report it
+            // and leave instead of hanging the device.
+            const int wl = 63 - __clzll(notready);
+            const int64_t qw = hi - 63 + wl;
+            const uint32_t st_w = (uint32_t)__shfl((int)st, wl, 64);
+            uint32_t backoff = 1;
+            while (true) {
+                const uint32_t s2 = lb_state(flags, qw, epoch);
+                if (s2 != st_w) break;
+                if (++spins > kSpinLimit) {
+                    *timed_out = 1;
+                    return start;
+                }
+                for (uint32_t k = 0; k < backoff; ++k) __builtin_amdgcn_s_sleep(1);
+                backoff = backoff < 8 ? backoff * 2 : 8;
             }
-            __builtin_amdgcn_s_sleep(2);
             continue;
         }
+        fence_acq_agent();  // payloads published before the flags we saw
         A mine = (lane > l_incl) ? load_agg(q) : ident;
         A comp = wave_incl_scan(mine, compose);
-        comp = shfl_t(comp, 63);
+        comp = lane_get(comp, 63);
         acc = compose(comp, acc);
         if (l_incl >= 0) {
             V base = start;
             const int64_t qi = hi - 63 + l_incl;
             if (qi >= 0) base = load_incl(qi);
-            base = shfl_t(base, l_incl);
+            base = lane_get(base, l_incl);
             return apply(acc, base);
         }
         hi -= 64;
@@ -227,51 +242,83 @@ struct FoldOut {
     double *a_out;       // [M] optional exact fold values
 };
 
-constexpr int kPad = kItems + 1;  // LDS padding: one double per 16
-__device__ __forceinline__ int lidx(int e) { return e + e / kItems; }
+// Tiles of 2048 elements, 8 per thread.  Per-element data lives in LDS (one
+// padding double per 8 keeps the thread-consecutive reads conflict-free) and
+// the per-element loops are not unrolled, which keeps every instantiation
+// small enough for the instruction cache.
+constexpr int kFI = kFoldItems;
+constexpr int kFT = kFoldTile;
+constexpr int kFPad = kFT + kFT / kFI;
+__device__ __forceinline__ int fidx(int e) { return e + e / kFI; }
+
+// First n in [lo, hi) with v(n) > c (hi if none), v non-decreasing.  All 64
+// lanes of a wave probe 64 evenly spaced points per round (64-ary search).
+template <typename Inv>
+__device__ uint64_t wave_first_greater(const Inv &inv, double c, uint64_t lo, uint64_t hi)
+{
+    const int lane = threadIdx.x & 63;
+    while (hi - lo > 64) {
+        const uint64_t step = (hi - lo + 63) / 64;
+        const uint64_t n = lo + (uint64_t)lane * step;
+        const bool gt = n < hi ? inv.v(n) > c : true;
+        const uint64_t mask = __ballot(gt);
+        const int g = mask ? __ffsll((unsigned long long)mask) - 1 : 64;
+        if (g == 0) return lo;
+        const uint64_t nlo = lo + (uint64_t)(g - 1) * step + 1;
+        const uint64_t nhi = g < 64 ? lo + (uint64_t)g * step : hi;
+        lo = nlo;
+        hi = nhi < hi ? nhi : hi;
+    }
+    const uint64_t n = lo + (uint64_t)lane;
+    const bool gt = n < hi ? inv.v(n) > c : true;
+    const uint64_t mask = __ballot(gt);
+    if (mask == 0) return hi;  // all 64 remaining candidates are <= c
+    const uint64_t r = lo + (uint64_t)(__ffsll((unsigned long long)mask) - 1);
+    return r < hi ? r : hi;
+}
 
 template <typename XSrc, typename Inv>
 __global__ __launch_bounds__(kBlock) void k_fold_inv(FilterCtl *ctl, XSrc xs, Inv inv, uint64_t M,
                                                      FoldScratch fs, FoldOut out)
 {
     if (!ctl->resample) return;
-    __shared__ double s_a[kBlock * kPad];
+    __shared__ double s_x[kFPad];  // x; later the uniform window / surplus prefix
+    __shared__ double s_a[kFPad];  // exact fold values
     __shared__ double s_red[kBlock / kWave];
     __shared__ Tr s_tr[kBlock / kWave];
     __shared__ uint64_t s_u64[kBlock / kWave];
-    __shared__ uint32_t s_u32[kBlock / kWave];
+    __shared__ int s_int[kBlock / kWave];
     __shared__ unsigned long long s_tile;
-    __shared__ double s_bcast_d[2];
-    __shared__ uint64_t s_bcast_u[4];
+    __shared__ double s_bd[2];
+    __shared__ uint64_t s_bu[4];
     __shared__ int s_timeout;
 
     const int tid = threadIdx.x;
-    if (tid == 0) s_timeout = 0;
-    if (tid == 0) s_tile = atomicAdd(fs.tile_counter, 1ull);
+    if (tid == 0) {
+        s_tile = atomicAdd(fs.tile_counter, 1ull);
+        s_timeout = 0;
+    }
     const uint64_t epoch = *fs.epoch;
     xs.bind(ctl);
     inv.bind(ctl);
     __syncthreads();
     const uint64_t t = s_tile;
-    const uint64_t base = t * (uint64_t)kTile;
+    const uint64_t base = t * (uint64_t)kFT;
+    const int e0 = tid * kFI;  // this thread's elements: e0 .. e0 + kFI - 1
 
-    // ---- load x (striped, coalesced) and transpose through LDS -----------
+    // ---- load x (striped, coalesced) into LDS --------------------------------
 #pragma unroll
-    for (int k = 0; k < kItems; ++k) {
+    for (int k = 0; k < kFI; ++k) {
         const int e = k * kBlock + tid;
         const uint64_t i = base + e;
-        s_a[lidx(e)] = i < M ? xs.x(i) : 0.0;
+        s_x[fidx(e)] = i < M ? xs.x(i) : 0.0;
     }
     __syncthreads();
-    double xv[kItems];
-#pragma unroll
-    for (int k = 0; k < kItems; ++k) xv[k] = s_a[lidx(tid * kItems + k)];
-    __syncthreads();
 
-    // ---- approximate prefix: tile sum, look-back ----------------------------
+    // ---- approximate prefix: tile sum, look-back ------------------------------
     double tsum = 0;
-#pragma unroll
-    for (int k = 0; k < kItems; ++k) tsum += xv[k];
+#pragma unroll 1
+    for (int k = 0; k < kFI; ++k) tsum += s_x[fidx(e0 + k)];
     double tile_sum;
     block_incl_scan<kBlock>(tsum, AddOp(), s_red, &tile_sum);
     if (tid == 0) {
@@ -283,15 +330,15 @@ __global__ __launch_bounds__(kBlock) void k_fold_inv(FilterCtl *ctl, XSrc xs, In
     }
     if (tid < kWave) {
         double S = 0.0;
-        if (t > 0)
+        if (t > 0) {
+            const double *agg = fs.agg_sum, *inc = fs.incl_sum;
             S = lookback<double, double>(
-                t, fs.flags_sum, epoch, 0.0, 0.0,
-                [&](int64_t q) { return fs.agg_sum[q]; },
-                [&](int64_t q) { return fs.incl_sum[q]; },
-                [](double a, double b) { return a + b; },
+                t, fs.flags_sum, epoch, 0.0, 0.0, [=](int64_t q) { return agg[q]; },
+                [=](int64_t q) { return inc[q]; }, [](double a, double b) { return a + b; },
                 [](double a, double b) { return a + b; }, &s_timeout);
+        }
         if (tid == 0) {
-            s_bcast_d[0] = S;
+            s_bd[0] = S;
             if (t > 0) {
                 fs.incl_sum[t] = S + tile_sum;
                 lb_publish(fs.flags_sum, t, epoch, kStIncl);
@@ -300,7 +347,7 @@ __global__ __launch_bounds__(kBlock) void k_fold_inv(FilterCtl *ctl, XSrc xs, In
     }
     __syncthreads();
     if (tid == 0 && s_timeout) atomicOr(&ctl->err, (int)kErrTimeout);
-    const double S_t = s_bcast_d[0];
+    const double S_t = s_bd[0];
 
     // ---- classify the tile --------------------------------------------------
     double eps_rel, eps_abs;
@@ -308,57 +355,85 @@ __global__ __launch_bounds__(kBlock) void k_fold_inv(FilterCtl *ctl, XSrc xs, In
     const int b0 = certain_binade(S_t, eps_rel, eps_abs);
     const int b1 = certain_binade(S_t + tile_sum, eps_rel, eps_abs);
     const bool clean = t > 0 && b0 == b1 && b0 != kNoBinade;
+    const auto compose = [](const Tr &a, const Tr &c) { return tr_compose(a, c); };
 
-    // ---- publish the clean-tile transducer ------------------------------------
-    if (clean) {
+    // ---- per-thread transducers, computed before the exact start is known ------
+    // Clean tiles: binade bA = b0; publish the tile transducer.  Other tiles:
+    // bA = lowest binade the start can be in, bB = bA + 1; prefix scan in bA and
+    // reverse (suffix) scan in bB let the end value follow from the exact start
+    // with one walk and one real add when the tile holds a single crossing.
+    const int bA = clean ? b0 : binade_of(S_t - (S_t * eps_rel + eps_abs) > 0
+                                              ? S_t - (S_t * eps_rel + eps_abs) : 0.0);
+    const int bB = bA + 1;
+    Tr PA = tr_identity(), TA = tr_identity(), RB = tr_identity(), TB = tr_identity();
+    if (t > 0) {
         Tr thr = tr_identity();
-#pragma unroll
-        for (int k = 0; k < kItems; ++k) thr = tr_compose(thr, tr_element(xv[k], b0));
-        Tr T;
-        block_incl_scan<kBlock>(thr, [](const Tr &a, const Tr &b) { return tr_compose(a, b); },
-                                s_tr, &T);
-        if (tid == 0) {
-            fs.agg_d0[t] = T.d0;
-            fs.agg_d1[t] = T.d1;
-            fs.agg_pp[t] = T.pp;
-            fs.agg_b[t] = T.b;
-            lb_publish(fs.flags_fold, t, epoch, kStAgg);
+#pragma unroll 1
+        for (int k = 0; k < kFI; ++k) thr = tr_compose(thr, tr_element(s_x[fidx(e0 + k)], bA));
+        PA = block_excl_scan<kBlock>(thr, compose, tr_identity(), s_tr, &TA);
+        if (clean) {
+            if (tid == 0) {
+                fs.agg_d0[t] = TA.d0;
+                fs.agg_d1[t] = TA.d1;
+                fs.agg_pp[t] = TA.pp;
+                fs.agg_b[t] = TA.b;
+                lb_publish(fs.flags_fold, t, epoch, kStAgg);
+            }
+        } else {
+            if (tid == 0) lb_publish(fs.flags_fold, t, epoch, kStNC);
+            Tr thb = tr_identity();
+#pragma unroll 1
+            for (int k = 0; k < kFI; ++k) thb = tr_compose(thb, tr_element(s_x[fidx(e0 + k)], bB));
+            // reverse inclusive scan: z_k = T_{255-k} o ... o T_255, via reversed order
+            Tr *s_rev = reinterpret_cast<Tr *>(s_a);  // kBlock entries, s_a not yet in use
+            s_rev[kBlock - 1 - tid] = thb;
+            __syncthreads();
+            const Tr y = s_rev[tid];
+            __syncthreads();
+            const Tr z = block_incl_scan<kBlock>(
+                y, [](const Tr &p, const Tr &q) { return tr_compose(q, p); }, s_tr, &TB);
+            s_rev[tid] = z;
+            __syncthreads();
+            RB = tid < kBlock - 1 ? s_rev[kBlock - 2 - tid] : tr_identity();  // threads > tid
+            __syncthreads();
         }
-        if (tid == 0) s_tr[0] = T;
-    } else if (tid == 0 && t > 0) {
-        lb_publish(fs.flags_fold, t, epoch, kStNC);
     }
 
-    // ---- exact start value of the tile ---------------------------------------
+    // ---- exact start value of the tile -----------------------------------------
     if (tid < kWave) {
         double v0 = 0.0;
-        if (t > 0)
+        if (t > 0) {
+            const uint64_t *ad0 = fs.agg_d0, *ad1 = fs.agg_d1;
+            const uint32_t *app = fs.agg_pp;
+            const int32_t *ab = fs.agg_b;
+            const double *vf = fs.val_fold;
+            int *errp = &ctl->err;
             v0 = lookback<Tr, double>(
                 t, fs.flags_fold, epoch, 0.0, tr_identity(),
-                [&](int64_t q) {
+                [=](int64_t q) {
                     Tr a;
-                    a.d0 = fs.agg_d0[q];
-                    a.d1 = fs.agg_d1[q];
-                    a.pp = fs.agg_pp[q];
-                    a.b = fs.agg_b[q];
+                    a.d0 = ad0[q];
+                    a.d1 = ad1[q];
+                    a.pp = app[q];
+                    a.b = ab[q];
                     return a;
                 },
-                [&](int64_t q) { return fs.val_fold[q]; },
-                [](const Tr &a, const Tr &b) { return tr_compose(a, b); },
-                [&](const Tr &a, double v) {
+                [=](int64_t q) { return vf[q]; }, compose,
+                [=](const Tr &a, double v) {
                     double r;
                     if (!tr_apply(a, v, &r)) {
-                        atomicOr(&ctl->err, (int)kErrFoldClass);
+                        atomicOr(errp, (int)kErrFoldClass);
                         r = v;
                     }
                     return r;
                 },
                 &s_timeout);
+        }
         if (tid == 0) {
-            s_bcast_d[1] = v0;
+            s_bd[1] = v0;
             if (clean) {  // publish the inclusive value right away
                 double e;
-                if (!tr_apply(s_tr[0], v0, &e)) atomicOr(&ctl->err, (int)kErrFoldClass);
+                if (!tr_apply(TA, v0, &e)) atomicOr(&ctl->err, (int)kErrFoldClass);
                 fs.val_fold[t] = e;
                 lb_publish(fs.flags_fold, t, epoch, kStIncl);
             }
@@ -366,123 +441,216 @@ __global__ __launch_bounds__(kBlock) void k_fold_inv(FilterCtl *ctl, XSrc xs, In
     }
     __syncthreads();
     if (tid == 0 && s_timeout) atomicOr(&ctl->err, (int)kErrTimeout);
+    const double v0 = s_bd[1];
+    bool published = clean;
 
-    // ---- exact values of every element: carry the fold through the tile ------
-    double cur = s_bcast_d[1];
-    int pos = 0;
-    while (pos < kTile) {
-        const int b = binade_of(cur);
-        Tr thr = tr_identity();
-#pragma unroll
-        for (int k = 0; k < kItems; ++k) {
-            const int e = tid * kItems + k;
-            if (e >= pos) thr = tr_compose(thr, tr_element(xv[k], b));
-        }
-        Tr excl;
-        {
-            Tr incl = block_incl_scan<kBlock>(
-                thr, [](const Tr &a, const Tr &c) { return tr_compose(a, c); }, s_tr);
-            excl = shfl_up_t(incl, 1);
-            const int lane = tid & 63, wid = tid / 64;
-            if (lane == 0) excl = wid > 0 ? s_tr[wid - 1] : tr_identity();
-            // s_tr still holds the per-wave inclusive totals after the scan
-        }
-        int first_fail = kTile;
+    if (clean) {
+        // every element: the thread prefix applied to the start, then a walk
         double v;
-        if (!tr_apply(excl, cur, &v)) {
-            first_fail = tid * kItems;
-        } else {
-#pragma unroll
-            for (int k = 0; k < kItems; ++k) {
-                const int e = tid * kItems + k;
-                if (e < pos || first_fail < kTile) continue;
-                double v2;
-                if (!tr_apply(tr_element(xv[k], b), v, &v2)) {
-                    first_fail = e;
-                } else {
-                    s_a[lidx(e)] = v2;
+        if (!tr_apply(PA, v0, &v)) {
+            atomicOr(&ctl->err, (int)kErrFoldClass);
+            v = v0;
+        }
+#pragma unroll 1
+        for (int k = 0; k < kFI; ++k) {
+            const int e = e0 + k;
+            double v2;
+            if (!tr_apply(tr_element(s_x[fidx(e)], bA), v, &v2)) {
+                atomicOr(&ctl->err, (int)kErrFoldClass);
+                v2 = v;
+            }
+            s_a[fidx(e)] = v2;
+            v = v2;
+        }
+        __syncthreads();
+    } else if (t > 0) {
+        // ---- end value first (the next tiles wait for it) ---------------------
+        const int b = binade_of(v0);
+        int ok = 0;
+        double endv = 0;
+        if (b == bB) {
+            ok = tr_apply(TB, v0, &endv);
+        } else if (b == bA) {
+            int first_fail = kFT;
+            double vprev = 0, vt;
+            if (!tr_apply(PA, v0, &vt)) {
+                first_fail = e0;
+            } else {
+#pragma unroll 1
+                for (int k = 0; k < kFI; ++k) {
+                    double v2;
+                    if (!tr_apply(tr_element(s_x[fidx(e0 + k)], bA), vt, &v2)) {
+                        first_fail = e0 + k;
+                        vprev = vt;
+                        break;
+                    }
+                    vt = v2;
+                }
+            }
+            const int j = block_reduce<kBlock>(first_fail, MinOp(), s_int);
+            if (j >= kFT) {
+                ok = tr_apply(TA, v0, &endv);
+            } else {
+                if (tid == j / kFI) {
+                    // owner: one real add at the crossing, then the rest in bB
+                    const double aj = vprev + s_x[fidx(j)];
+                    Tr part = tr_identity();
+#pragma unroll 1
+                    for (int e = j + 1; e < e0 + kFI; ++e)
+                        part = tr_compose(part, tr_element(s_x[fidx(e)], bB));
+                    double w;
+                    int okk = binade_of(aj) == bB && tr_apply(part, aj, &w);
+                    double en = 0;
+                    if (okk) okk = tr_apply(RB, w, &en);
+                    s_bd[0] = en;
+                    s_int[0] = okk;
+                }
+                __syncthreads();
+                ok = s_int[0];
+                endv = s_bd[0];
+                __syncthreads();
+            }
+        }
+        if (ok && tid == 0) {
+            fs.val_fold[t] = endv;
+            lb_publish(fs.flags_fold, t, epoch, kStIncl);
+        }
+        published = ok;
+        __syncthreads();
+    }
+
+    if (!clean) {
+        // ---- exact values of every element: carry the fold through the tile ----
+        // Each pass composes the elements from `pos` in the binade of the running
+        // value; the first element whose result would leave that binade gets one
+        // real FP64 add, and the next pass starts after it.  The first tile
+        // starts at 0 and crosses a binade every doubling, so its first
+        // elements are folded one by one.
+        double cur = v0;
+        int pos = 0;
+        if (t == 0) {
+            constexpr int kSerial = 256;
+            if (tid == 0) {
+                double a = 0;
+                for (int e = 0; e < kSerial; ++e) {
+                    a += s_x[fidx(e)];
+                    s_a[fidx(e)] = a;
+                }
+            }
+            __syncthreads();
+            cur = s_a[fidx(kSerial - 1)];
+            pos = kSerial;
+        }
+        while (pos < kFT) {
+            const int b = binade_of(cur);
+            Tr thr = tr_identity();
+#pragma unroll 1
+            for (int k = 0; k < kFI; ++k) {
+                const int e = e0 + k;
+                if (e >= pos) thr = tr_compose(thr, tr_element(s_x[fidx(e)], b));
+            }
+            const Tr excl =
+                block_excl_scan<kBlock>(thr, compose, tr_identity(), s_tr, (Tr *)nullptr);
+            int first_fail = kFT;
+            double v;
+            if (!tr_apply(excl, cur, &v)) {
+                first_fail = e0;
+            } else {
+#pragma unroll 1
+                for (int k = 0; k < kFI; ++k) {
+                    const int e = e0 + k;
+                    if (e < pos) continue;
+                    double v2;
+                    if (!tr_apply(tr_element(s_x[fidx(e)], b), v, &v2)) {
+                        first_fail = e;
+                        break;
+                    }
+                    s_a[fidx(e)] = v2;
                     v = v2;
                 }
             }
-        }
-        const int j = block_reduce<kBlock>(first_fail, [](int a, int c) { return a < c ? a : c; },
-                                           (int *)s_u32);
-        if (j >= kTile) break;
-        if (tid == j / kItems) {
-            const double prev = (j == pos) ? cur : s_a[lidx(j - 1)];
-            s_a[lidx(j)] = prev + xv[j % kItems];
+            const int j = block_reduce<kBlock>(first_fail, MinOp(), s_int);
+            if (j >= kFT) break;
+            if (tid == j / kFI) {
+                const double prev = (j == pos) ? cur : s_a[fidx(j - 1)];
+                s_a[fidx(j)] = prev + s_x[fidx(j)];
+            }
+            __syncthreads();
+            cur = s_a[fidx(j)];
+            pos = j + 1;
+            __syncthreads();
         }
         __syncthreads();
-        cur = s_a[lidx(j)];
-        pos = j + 1;
-        __syncthreads();
+        if (tid == 0) {
+            if (!published) {
+                fs.val_fold[t] = s_a[fidx(kFT - 1)];
+                lb_publish(fs.flags_fold, t, epoch, kStIncl);
+            } else if (fs.val_fold[t] != s_a[fidx(kFT - 1)]) {
+                atomicOr(&ctl->err, (int)kErrFoldClass);
+            }
+        }
     }
-    __syncthreads();
-    if (!clean && tid == 0) {
-        fs.val_fold[t] = s_a[lidx(kTile - 1)];
-        lb_publish(fs.flags_fold, t, epoch, kStIncl);
-    }
-    const double v_start = s_bcast_d[1];
-
-    double av[kItems];
-#pragma unroll
-    for (int k = 0; k < kItems; ++k) av[k] = s_a[lidx(tid * kItems + k)];
+    const double v_start = v0;
     if (out.a_out) {
-#pragma unroll
-        for (int k = 0; k < kItems; ++k) {
-            const uint64_t i = base + tid * kItems + k;
-            if (i < M) out.a_out[i] = av[k];
+#pragma unroll 1
+        for (int k = 0; k < kFI; ++k) {
+            const int e = k * kBlock + tid;
+            const uint64_t i = base + e;
+            if (i < M) out.a_out[i] = s_a[fidx(e)];
         }
     }
-    if (t == fs.tiles - 1 && tid == 0) ctl->fold_total = s_a[lidx((int)((M - 1) - base))];
+    if (t == fs.tiles - 1 && tid == 0) ctl->fold_total = s_a[fidx((int)((M - 1) - base))];
 
     if constexpr (!Inv::kInv) {
         return;
     } else {
-        // ---- inversion: c_i = F(a_i) for i <= M-2 ---------------------------
+        // ---- inversion: c_i = F(a_i) for i <= M-2 -------------------------------
         const uint64_t nn = inv.nn;
-        uint64_t c[kItems];
         const uint64_t last_cmp = M >= 2 ? M - 2 : 0;  // highest index compared
+        uint64_t c[kFI];
+        double av[kFI];
+#pragma unroll
+        for (int k = 0; k < kFI; ++k) av[k] = s_a[fidx(e0 + k)];
+        uint64_t cstart = 0;  // F(a_{base-1}); 0 for the first tile
         if constexpr (!Inv::kWindow) {
 #pragma unroll
-            for (int k = 0; k < kItems; ++k) {
-                const uint64_t i = base + tid * kItems + k;
+            for (int k = 0; k < kFI; ++k) {
+                const uint64_t i = base + e0 + k;
                 c[k] = (M >= 2 && i <= last_cmp) ? inv.F(av[k]) : 0;
             }
+            if (t > 0) cstart = inv.F(v_start);
         } else {
-            // window of uniforms this tile can hold: [F(a_{start-1}), F(a_lastcmp)]
-            if (tid == 0) {
+            // the tile holds uniforms [F(a_{base-1}), F(a_{last compared}))
+            if (tid < kWave) {
                 uint64_t lo = 0, hi = 0;
                 if (M >= 2) {
-                    lo = t == 0 ? 0 : inv.point_F(v_start);
+                    lo = t == 0 ? 0 : wave_first_greater(inv, v_start, 0, nn);
                     hi = lo;
                     if (base <= last_cmp) {
                         const uint64_t il =
-                            (base + kTile - 1 < last_cmp) ? base + kTile - 1 : last_cmp;
-                        hi = inv.point_F(s_a[lidx((int)(il - base))]);
-                    }
-                    if (hi < lo) {
-                        atomicOr(&ctl->err, (int)kErrWindow);
-                        hi = lo;
+                            (base + kFT - 1 < last_cmp) ? base + kFT - 1 : last_cmp;
+                        hi = wave_first_greater(inv, s_a[fidx((int)(il - base))], lo, nn);
                     }
                 }
-                s_bcast_u[0] = lo;
-                s_bcast_u[1] = hi;
+                if (tid == 0) {
+                    s_bu[0] = lo;
+                    s_bu[1] = hi;
+                }
             }
             __syncthreads();
-            const uint64_t wlo = s_bcast_u[0], whi = s_bcast_u[1];
+            const uint64_t wlo = s_bu[0], whi = s_bu[1];
+            cstart = wlo;
 #pragma unroll
-            for (int k = 0; k < kItems; ++k) c[k] = wlo;
-            // reuse s_a as the window buffer (exact values are in av[])
-            constexpr int kCh = kBlock * kPad;
-            for (uint64_t w0 = wlo; w0 < whi; w0 += kCh) {
-                const int len = (int)((whi - w0) < (uint64_t)kCh ? (whi - w0) : (uint64_t)kCh);
+            for (int k = 0; k < kFI; ++k) c[k] = wlo;
+            for (uint64_t w0 = wlo; w0 < whi; w0 += kFPad) {
+                const int len = (int)((whi - w0) < (uint64_t)kFPad ? (whi - w0) : (uint64_t)kFPad);
                 __syncthreads();
-                for (int k = tid; k < len; k += kBlock) s_a[k] = inv.v(w0 + k);
+#pragma unroll 1
+                for (int k = tid; k < len; k += kBlock) s_x[k] = inv.v(w0 + k);
                 __syncthreads();
-                const double first = s_a[0], lastv = s_a[len - 1];
+                const double first = s_x[0], lastv = s_x[len - 1];
 #pragma unroll
-                for (int k = 0; k < kItems; ++k) {
+                for (int k = 0; k < kFI; ++k) {
                     const double a = av[k];
                     if (a < first) continue;
                     if (a >= lastv) {
@@ -492,7 +660,7 @@ __global__ __launch_bounds__(kBlock) void k_fold_inv(FilterCtl *ctl, XSrc xs, In
                     int lo = 0, hi = len;  // first index with value > a
                     while (lo < hi) {
                         const int mid = (lo + hi) >> 1;
-                        if (s_a[mid] <= a)
+                        if (s_x[mid] <= a)
                             lo = mid + 1;
                         else
                             hi = mid;
@@ -501,8 +669,8 @@ __global__ __launch_bounds__(kBlock) void k_fold_inv(FilterCtl *ctl, XSrc xs, In
                 }
             }
 #pragma unroll
-            for (int k = 0; k < kItems; ++k) {
-                const uint64_t i = base + tid * kItems + k;
+            for (int k = 0; k < kFI; ++k) {
+                const uint64_t i = base + e0 + k;
                 if (!(M >= 2 && i <= last_cmp)) c[k] = 0;
             }
         }
@@ -510,25 +678,19 @@ __global__ __launch_bounds__(kBlock) void k_fold_inv(FilterCtl *ctl, XSrc xs, In
         uint64_t cprev;
         {
             const int lane = tid & 63, wid = tid / 64;
-            uint64_t up = shfl_up_t(c[kItems - 1], 1);
-            if (lane == 63) s_u64[wid] = c[kItems - 1];
+            uint64_t up = lane_up(c[kFI - 1], 1);
+            if (lane == 63) s_u64[wid] = c[kFI - 1];
             __syncthreads();
             if (lane == 0) up = wid > 0 ? s_u64[wid - 1] : 0;
-            cprev = up;
+            cprev = tid == 0 ? cstart : up;
             __syncthreads();
-            if (tid == 0) {
-                if constexpr (Inv::kWindow)
-                    cprev = (t == 0) ? 0 : s_bcast_u[0];
-                else
-                    cprev = (t == 0) ? 0 : inv.F(v_start);
-            }
         }
         // replication counts
-        uint32_t rep[kItems];
+        uint32_t rep[kFI];
         uint64_t zs_thr = 0;  // zeros << 32 | surplus
 #pragma unroll
-        for (int k = 0; k < kItems; ++k) {
-            const uint64_t i = base + tid * kItems + k;
+        for (int k = 0; k < kFI; ++k) {
+            const uint64_t i = base + e0 + k;
             uint64_t r = 0;
             if (i < M) {
                 if (M == 1)
@@ -555,8 +717,8 @@ __global__ __launch_bounds__(kBlock) void k_fold_inv(FilterCtl *ctl, XSrc xs, In
 
         // ---- zero / surplus ranks: block scan + look-back --------------------
         uint64_t zs_tile;
-        const uint64_t zs_incl = block_incl_scan<kBlock>(zs_thr, AddOp(), s_u64, &zs_tile);
-        const uint64_t zs_excl = zs_incl - zs_thr;
+        const uint64_t zs_excl =
+            block_excl_scan<kBlock>(zs_thr, AddOp(), (uint64_t)0, s_u64, &zs_tile);
         if (tid == 0) {
             if (t == 0)
                 fs.incl_cnt[t] = zs_tile;
@@ -566,15 +728,16 @@ __global__ __launch_bounds__(kBlock) void k_fold_inv(FilterCtl *ctl, XSrc xs, In
         }
         if (tid < kWave) {
             uint64_t pre = 0;
-            if (t > 0)
+            if (t > 0) {
+                const uint64_t *agg = fs.agg_cnt, *inc = fs.incl_cnt;
                 pre = lookback<uint64_t, uint64_t>(
-                    t, fs.flags_cnt, epoch, 0ull, 0ull,
-                    [&](int64_t q) { return fs.agg_cnt[q]; },
-                    [&](int64_t q) { return fs.incl_cnt[q]; },
+                    t, fs.flags_cnt, epoch, 0ull, 0ull, [=](int64_t q) { return agg[q]; },
+                    [=](int64_t q) { return inc[q]; },
                     [](uint64_t a, uint64_t b) { return a + b; },
                     [](uint64_t a, uint64_t b) { return a + b; }, &s_timeout);
+            }
             if (tid == 0) {
-                s_bcast_u[2] = pre;
+                s_bu[2] = pre;
                 if (t > 0) {
                     fs.incl_cnt[t] = pre + zs_tile;
                     lb_publish(fs.flags_cnt, t, epoch, kStIncl);
@@ -584,17 +747,17 @@ __global__ __launch_bounds__(kBlock) void k_fold_inv(FilterCtl *ctl, XSrc xs, In
         }
         __syncthreads();
         if (tid == 0 && s_timeout) atomicOr(&ctl->err, (int)kErrTimeout);
-        const uint64_t pre = s_bcast_u[2];
+        const uint64_t pre = s_bu[2];
         uint64_t zrank = (pre >> 32) + (zs_excl >> 32);
         uint64_t srank_local = zs_excl & 0xffffffffull;  // surplus before this thread (tile)
         const uint64_t s_base = pre & 0xffffffffull;
 
         // anc identity, zero list, per-item inclusive surplus prefix into LDS
-        uint32_t *s_sp = reinterpret_cast<uint32_t *>(s_a);  // kTile entries
+        uint32_t *s_sp = reinterpret_cast<uint32_t *>(s_x);  // kFT entries
         __syncthreads();
 #pragma unroll
-        for (int k = 0; k < kItems; ++k) {
-            const int e = tid * kItems + k;
+        for (int k = 0; k < kFI; ++k) {
+            const int e = e0 + k;
             const uint64_t i = base + e;
             if (i < M) {
                 out.anc[i] = (uint32_t)i;
@@ -613,7 +776,7 @@ __global__ __launch_bounds__(kBlock) void k_fold_inv(FilterCtl *ctl, XSrc xs, In
         // expand surplus copies: positions [s_base, s_base + S_tile)
         const uint32_t S_tile = (uint32_t)(zs_tile & 0xffffffffull);
         for (uint32_t k = tid; k < S_tile; k += kBlock) {
-            int lo = 0, hi = kTile - 1;  // first e with s_sp[e] > k
+            int lo = 0, hi = kFT - 1;  // first e with s_sp[e] > k
             while (lo < hi) {
                 const int mid = (lo + hi) >> 1;
                 if (s_sp[mid] > k)
@@ -746,7 +909,7 @@ __global__ __launch_bounds__(1024) void k_scan_tiles(FilterCtl *ctl, const doubl
                              : block_incl_scan<1024>(v, AddOp(), s_red, &tot);
         double ex = is_max ? (threadIdx.x == 0 ? -1.0 : 0.0) : incl - v;
         if (is_max) {
-            double up = shfl_up_t(incl, 1);
+            double up = lane_up(incl, 1);
             const int lane = threadIdx.x & 63, wid = threadIdx.x / 64;
             // s_red holds per-wave inclusive maxima after the scan
             ex = lane == 0 ? (wid > 0 ? s_red[wid - 1] : -1.0) : up;
@@ -790,7 +953,7 @@ __global__ __launch_bounds__(kBlock) void k_sorted_values(FilterCtl *ctl, const 
     }
     // thread-exclusive running max across the tile
     const double minc = block_incl_scan<kBlock>(mx, MaxOp(), s_red);
-    double mex = shfl_up_t(minc, 1);
+    double mex = lane_up(minc, 1);
     {
         const int lane = threadIdx.x & 63, wid = threadIdx.x / 64;
         if (lane == 0) mex = wid > 0 ? s_red[wid - 1] : -1.0;
@@ -817,7 +980,7 @@ template <typename XSrc, typename Inv>
 static void launch_fold(FilterCtl *ctl, XSrc xs, Inv inv, uint64_t M, FoldScratch &fs,
                         const FoldOut &out, hipStream_t st)
 {
-    fs.tiles = (M + kTile - 1) / kTile;
+    fs.tiles = (M + kFT - 1) / kFT;
     hipLaunchKernelGGL(k_fold_begin, dim3(1), dim3(64), 0, st, ctl, fs.epoch, fs.tile_counter);
     hipLaunchKernelGGL((k_fold_inv<XSrc, Inv>), dim3((unsigned)fs.tiles), dim3(kBlock), 0, st, ctl,
                        xs, inv, M, fs, out);
