@@ -1,7 +1,11 @@
-// Host-side check of vsmc_amd/csrc/fold_core.hpp: the tiled transducer
-// evaluation (classify with an approximate prefix, compose regular segments,
-// chain specials with real FP64 adds) must equal the sequential left fold
-// bit for bit.  Built and run by tests/test_fold_core.py.
+// Host-side check of vsmc_amd/csrc/fold_core.hpp: the segmented evaluation
+// the device runs (classify every element against an approximate prefix,
+// per-thread runs with the single-binade fast path, segment summaries combined
+// in tile/thread groupings, one real FP64 add per special element, then every
+// value from its run) must equal the sequential left fold
+// a_i = fl(a_{i-1} + x_i) bit for bit.
+// Built and run by tests/test_fold_core.py.
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -12,74 +16,136 @@
 
 using namespace vsmcx;
 
-static int check_case(const std::vector<double> &x, int tile, bool perturb_approx, int *nspec_out)
+// Compose a range of Seg values in a random binary grouping (associativity is
+// what lets tiles and threads combine in any order).
+static Seg fold_random(const std::vector<Seg> &v, size_t lo, size_t hi, std::mt19937_64 &g)
+{
+    if (hi - lo == 0) return seg_identity();
+    if (hi - lo == 1) return v[lo];
+    const size_t mid = lo + 1 + g() % (hi - lo - 1);
+    return seg_op(fold_random(v, lo, mid, g), fold_random(v, mid, hi, g));
+}
+
+static bool same(const Seg &a, const Seg &b)
+{
+    return a.cnt == b.cnt && a.tr.d0 == b.tr.d0 && a.tr.d1 == b.tr.d1 && a.tr.pp == b.tr.pp &&
+           a.tr.b == b.tr.b;
+}
+
+static int check_case(const std::vector<double> &x, int items, int tile, double perturb,
+                      std::mt19937_64 &g, long *nspec_out)
 {
     const size_t n = x.size();
-    // sequential left fold (the definition)
     std::vector<double> ref(n);
     double a = 0;
     for (size_t i = 0; i < n; ++i) {
         a += x[i];
         ref[i] = a;
     }
-    double eps_rel, eps_abs;
-    fold_eps(n, &eps_rel, &eps_abs);
-    // approximate prefix: tile sums in pairwise order, then within-tile running sums
+    double er, ea;
+    fold_eps(n, &er, &ea);
+#ifdef VSMCX_TEST_ZERO_EPS
+    er = 0;  // a broken bound, to show that it is caught
+    ea = 0;
+#endif
+    // The passes of kernels_fold.hip on the host: tiles of `tile` elements,
+    // threads of `items` consecutive elements (tail padded with zeros).
     const size_t ntiles = (n + tile - 1) / tile;
-    std::vector<double> tsum(ntiles, 0.0), tstart(ntiles, 0.0);
+    const int nthr = tile / items;
+    auto X = [&](size_t i) { return i < n ? x[i] : 0.0; };
+    // pass 1-2: tile sums and approximate tile starts (perturbed within budget)
+    std::vector<double> ts(ntiles);
+    double start = 0;
     for (size_t t = 0; t < ntiles; ++t) {
-        std::vector<double> v(x.begin() + t * tile, x.begin() + std::min(n, (t + 1) * tile));
-        while (v.size() > 1) {
-            std::vector<double> w;
-            for (size_t k = 0; k + 1 < v.size(); k += 2) w.push_back(v[k] + v[k + 1]);
-            if (v.size() % 2) w.push_back(v.back());
-            v = w;
-        }
-        tsum[t] = v.empty() ? 0 : v[0];
-        if (perturb_approx) tsum[t] *= 1 + 1e-13;
+        ts[t] = start;
+        double tsum = 0;
+        for (int e = 0; e < tile; ++e) tsum += X(t * tile + e);
+        start = (start + tsum) * (1 + perturb);
     }
-    for (size_t t = 1; t < ntiles; ++t) tstart[t] = tstart[t - 1] + tsum[t - 1];
-    // classification + per-element transducers
-    std::vector<int> special(n), bin(n);
-    std::vector<double> S(n);
+    // pass 3: per-thread runs, tile summaries, tile-local special records
+    std::vector<double> s_thr(ntiles * nthr);
+    std::vector<int> fast(ntiles * nthr);
+    std::vector<Seg> gthr(ntiles * nthr), agg(ntiles);
+    std::vector<SpecRec> spec(ntiles * tile);
     for (size_t t = 0; t < ntiles; ++t) {
-        double s = tstart[t];
-        for (size_t i = t * tile; i < std::min(n, (t + 1) * tile); ++i) {
-            const double sp = s;
-            s += x[i];
-            S[i] = s;
-            const int b0 = certain_binade(sp, eps_rel, eps_abs);
-            const int b1 = certain_binade(s, eps_rel, eps_abs);
-            special[i] = (i == 0) || b0 != b1 || b1 == kNoBinade;
-            bin[i] = b1;
+        double run = 0;
+        Seg tl = seg_identity();
+        for (int j = 0; j < nthr; ++j) {
+            const size_t i0 = t * tile + (size_t)j * items;
+            const size_t id = t * nthr + j;
+            s_thr[id] = ts[t] + run;
+            double p = 0;
+            for (int k = 0; k < items; ++k) p += X(i0 + k);
+            run += p;
+            auto xat = [&](int k) { return X(i0 + k); };
+            gthr[id] = run_seg(xat, items, s_thr[id], er, ea, &fast[id]);
+            if (gthr[id].cnt) {
+                Seg pre = tl;
+                double s = s_thr[id];
+                for (int k = 0; k < items; ++k) {
+                    const double sb = s;
+                    s = sb + X(i0 + k);
+                    const Seg el = elem_seg(X(i0 + k), sb, s, er, ea);
+                    if (el.cnt) {
+                        spec[t * tile + pre.cnt].run = pre.tr;
+                        spec[t * tile + pre.cnt].x = X(i0 + k);
+                    }
+                    pre = seg_op(pre, el);
+                }
+            }
+            tl = seg_op(tl, gthr[id]);
         }
+        agg[t] = tl;
+        if (!same(fold_random(gthr, t * nthr, (t + 1) * nthr, g), tl)) return 4;
     }
-    // walk: compose regular runs, add specials exactly; verify every value
-    int nspec = 0;
+    // pass 4: tile prefixes, specials in fold order, the chain
+    std::vector<Seg> pre(ntiles);
+    Seg acc = seg_identity();
+    for (size_t t = 0; t < ntiles; ++t) {
+        pre[t] = acc;
+        acc = seg_op(acc, agg[t]);
+    }
+    if (!same(fold_random(agg, 0, ntiles, g), acc)) return 4;
+    const uint64_t nspec = acc.cnt;
+    std::vector<double> gval(nspec + 1);
     double v = 0;
-    Tr run = tr_identity();
-    double run_start = 0;
-    for (size_t i = 0; i < n; ++i) {
-        double got;
-        if (special[i]) {
-            ++nspec;
+    uint64_t gi = 0;
+    for (size_t t = 0; t < ntiles; ++t)
+        for (uint64_t k = 0; k < agg[t].cnt; ++k, ++gi) {
+            SpecRec r = spec[t * tile + k];
+            if (k == 0) r.run = tr_compose(pre[t].tr, r.run);
             double vb;
-            if (!tr_apply(run, run_start, &vb)) return 1;
-            v = vb + x[i];
-            run = tr_identity();
-            run_start = v;
-            got = v;
-        } else {
-            run = tr_compose(run, tr_element(x[i], bin[i]));
-            if (!tr_apply(run, run_start, &got)) return 2;
+            if (!tr_apply(r.run, v, &vb)) return 1;
+            v = vb + r.x;
+            gval[gi] = v;
         }
-        if (got != ref[i]) {
-            std::fprintf(stderr, "mismatch at %zu: %.17g vs %.17g (special %d)\n", i, got, ref[i],
-                         special[i]);
-            return 3;
+    double tot;
+    if (!tr_apply(acc.tr, v, &tot)) return 6;
+    // pass 5: every value
+    for (size_t t = 0; t < ntiles; ++t) {
+        Seg tp = pre[t];
+        for (int j = 0; j < nthr; ++j) {
+            const size_t i0 = t * tile + (size_t)j * items;
+            const size_t id = t * nthr + j;
+            auto xat = [&](int k) { return X(i0 + k); };
+            std::vector<double> got(items);
+            const bool ok = run_values(
+                xat, items, s_thr[id], er, ea, fast[id], tp,
+                [&](uint64_t c) { return gval[c]; }, [&](int k, double a) { got[k] = a; });
+            if (!ok) return 2;
+            for (int k = 0; k < items; ++k) {
+                const size_t i = i0 + k;
+                if (i >= n) break;
+                if (got[k] != ref[i] && !(std::isnan(got[k]) && std::isnan(ref[i]))) {
+                    std::fprintf(stderr, "mismatch at %zu: %.17g vs %.17g\n", i, got[k], ref[i]);
+                    return 3;
+                }
+            }
+            tp = seg_op(tp, gthr[id]);
         }
     }
-    *nspec_out = nspec;
+    if (tot != ref[n - 1] && !(std::isnan(tot) && std::isnan(ref[n - 1]))) return 3;
+    *nspec_out = (long)nspec;
     return 0;
 }
 
@@ -87,33 +153,41 @@ int main()
 {
     std::mt19937_64 g(12345);
     int failures = 0, cases = 0;
-    long total_spec = 0;
-    for (int rep = 0; rep < 60; ++rep) {
+    long total_spec = 0, max_spec = 0;
+    std::uniform_real_distribution<double> U(0, 1);
+    std::normal_distribution<double> Z(0, 1);
+    for (int rep = 0; rep < 90; ++rep) {
         const size_t n = 1 + g() % 200000;
         std::vector<double> x(n);
-        const int kind = rep % 6;
-        std::uniform_real_distribution<double> U(0, 1);
-        std::normal_distribution<double> Z(0, 1);
+        const int kind = rep % 9;
         for (size_t i = 0; i < n; ++i) {
             switch (kind) {
-            case 0: x[i] = U(g); break;                             // flat
-            case 1: x[i] = std::exp(3 * Z(g)); break;               // heavy tail
-            case 2: x[i] = (g() % 3 == 0) ? 0.0 : U(g); break;      // zeros
+            case 0: x[i] = U(g); break;                              // flat
+            case 1: x[i] = std::exp(3 * Z(g)); break;                // heavy tail
+            case 2: x[i] = (g() % 3 == 0) ? 0.0 : U(g); break;       // zeros
             case 3: x[i] = std::ldexp(1.0, -(int)(g() % 60)); break; // powers of two: ties
-            case 4: x[i] = std::exp(-0.01 * (double)i); break;      // decaying
-            default: x[i] = std::exp(20 * Z(g)); break;             // huge dynamic range
+            case 4: x[i] = std::exp(-0.01 * (double)i); break;       // decaying
+            case 5: x[i] = std::exp(20 * Z(g)); break;               // huge dynamic range
+            case 6: x[i] = (i == n / 3) ? 1.0 : 0.0; break;          // one particle
+            case 7: x[i] = (i == n / 2) ? 1.0 : 1e-300 * U(g); break;  // one big, rest tiny
+            default:  // residual fractions scaled by 1/0: NaN (and an infinity)
+                x[i] = (i % 5 == 0) ? 0.0 : (i == n / 2 ? INFINITY : NAN);
+                break;
             }
         }
-        if (kind != 3) {
+        if (kind != 3 && kind != 8) {
             double s = 0;
             for (double v : x) s += v;
-            for (double &v : x) v /= s;  // normalised weights
+            if (s > 0)
+                for (double &v : x) v /= s;  // normalised weights
         }
-        for (int tile : {256, 4096}) {
-            int ns = 0;
-            int rc = check_case(x, tile, rep % 2 == 1, &ns);
+        for (int tile : {256, 2048}) {
+            long ns = 0;
+            const int rc = check_case(x, 8, tile, rep % 2 ? 1e-13 : 0.0, g, &ns);
             ++cases;
             total_spec += ns;
+            max_spec = std::max(max_spec, ns);
+            if (tile == 2048 && rep < 9) std::printf("kind %d n %zu specials %ld\n", kind, n, ns);
             if (rc) {
                 std::fprintf(stderr, "case %d kind %d n %zu tile %d failed rc=%d\n", rep, kind, n,
                              tile, rc);
@@ -121,15 +195,16 @@ int main()
             }
         }
     }
-    // exhaustive-ish tiny dyadic cases with many ties
+    // tiny dyadic cases with many ties and exact binade landings
     for (int rep = 0; rep < 20000; ++rep) {
         const size_t n = 1 + g() % 9;
         std::vector<double> x(n);
         for (size_t i = 0; i < n; ++i) x[i] = std::ldexp((double)(g() % 9), -(int)(g() % 56));
-        int ns = 0;
-        if (check_case(x, 4, false, &ns)) ++failures;
+        long ns = 0;
+        if (check_case(x, 2, 4, 0.0, g, &ns)) ++failures;
         ++cases;
     }
-    std::printf("cases %d failures %d specials %ld\n", cases, failures, total_spec);
+    std::printf("cases %d failures %d specials total %ld max %ld\n", cases, failures, total_spec,
+                max_spec);
     return failures ? 1 : 0;
 }

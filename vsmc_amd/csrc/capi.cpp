@@ -107,7 +107,7 @@ namespace {
 // Fold / resample scratch for up to M particles (and nuni uniforms).
 void alloc_resample(vsmcx_ctx *c, ResampleBuffers &rb, uint64_t M, uint64_t nuni, bool lists)
 {
-    const uint64_t tiles = (M + kFoldTile - 1) / kFoldTile;  // k_fold_inv tiles
+    const uint64_t tiles = (M + kFoldTile - 1) / kFoldTile;  // fold tiles
     const uint64_t utiles = (nuni + kTile - 1) / kTile + 1;   // sorted-uniform tiles
     if (lists) {
         rb.anc = c->alloc_n<uint32_t>(M);
@@ -121,30 +121,23 @@ void alloc_resample(vsmcx_ctx *c, ResampleBuffers &rb, uint64_t M, uint64_t nuni
     rb.uscan = c->alloc_n<double>(nuni ? nuni : 1);
     rb.nuni = nuni;
     FoldScratch &fs = rb.fs;
-    fs.flags_sum = c->alloc_zero<uint64_t>(tiles);
-    fs.agg_sum = c->alloc_n<double>(tiles);
-    fs.incl_sum = c->alloc_n<double>(tiles);
-    fs.flags_fold = c->alloc_zero<uint64_t>(tiles);
-    fs.val_fold = c->alloc_n<double>(tiles);
-    fs.agg_d0 = c->alloc_n<uint64_t>(tiles);
-    fs.agg_d1 = c->alloc_n<uint64_t>(tiles);
-    fs.agg_pp = c->alloc_n<uint32_t>(tiles);
-    fs.agg_b = c->alloc_n<int32_t>(tiles);
-    fs.flags_cnt = c->alloc_zero<uint64_t>(tiles);
-    fs.agg_cnt = c->alloc_n<uint64_t>(tiles);
-    fs.incl_cnt = c->alloc_n<uint64_t>(tiles);
-    fs.tile_counter = c->alloc_zero<unsigned long long>(1);
-    fs.epoch = c->alloc_zero<uint64_t>(1);
+    fs.tsum = c->alloc_n<double>(tiles);
+    fs.ts = c->alloc_n<double>(tiles);
+    fs.agg = c->alloc_n<Seg>(tiles);
+    fs.pre = c->alloc_n<Seg>(tiles);
+    fs.spec = c->alloc_n<SpecRec>(tiles * kFoldTile);
+    fs.gval = c->alloc_n<double>(M + 1);
+    fs.rep32 = c->alloc_n<uint32_t>(M);
+    fs.zs = c->alloc_n<uint64_t>(tiles);
+    fs.zsx = c->alloc_n<uint64_t>(tiles);
     fs.tiles = tiles;
 }
 
 void free_resample(vsmcx_ctx *c, ResampleBuffers &rb)
 {
     void *ps[] = {rb.anc, rb.zl, rb.sl, rb.rep_out, rb.a_out, rb.u_tsum, rb.u_texcl, rb.uscan,
-                  rb.u_tmax, rb.u_tmaxex, rb.fs.flags_sum, rb.fs.agg_sum, rb.fs.incl_sum,
-                  rb.fs.flags_fold, rb.fs.val_fold, rb.fs.agg_d0, rb.fs.agg_d1, rb.fs.agg_pp,
-                  rb.fs.agg_b, rb.fs.flags_cnt, rb.fs.agg_cnt, rb.fs.incl_cnt,
-                  rb.fs.tile_counter, rb.fs.epoch};
+                  rb.u_tmax, rb.u_tmaxex, rb.fs.tsum, rb.fs.ts, rb.fs.agg, rb.fs.pre, rb.fs.spec,
+                  rb.fs.gval, rb.fs.rep32, rb.fs.zs, rb.fs.zsx};
     for (void *p : ps) c->release(p);
     rb = ResampleBuffers();
 }

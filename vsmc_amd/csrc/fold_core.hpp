@@ -115,8 +115,9 @@ VSMCX_HD Tr tr_compose(const Tr &f, const Tr &g)
     return r;
 }
 
-// One fold step a' = fl(a + x) for a inside binade b, as a transducer.
-VSMCX_HD Tr tr_element(double x, int b)
+// x in grid units of binade b: integral part q (saturated at kSat) and how
+// the fraction rounds: 0 down, 1 up, 2 tie.
+VSMCX_HD uint64_t grid_units(double x, int b, int *kind)
 {
     const uint64_t bits = dbits(x);
     const int ex = (int)((bits >> 52) & 0x7ff);
@@ -124,20 +125,25 @@ VSMCX_HD Tr tr_element(double x, int b)
     const uint64_t mx = ex ? (frac | (1ull << 52)) : frac;
     const int exe = ex ? ex : 1;
     const int s = (b + 1023) - exe;  // bits of mx below the grid
-    uint64_t q;
-    int kind;  // 0 down, 1 up, 2 tie
     if (s <= 0) {
-        q = (-s >= 7) ? kSat : (mx << (-s));
-        kind = 0;
-    } else if (s >= 55) {
-        q = 0;
-        kind = 0;
-    } else {
-        q = mx >> s;
-        const uint64_t rem = mx & ((1ull << s) - 1);
-        const uint64_t half = 1ull << (s - 1);
-        kind = rem > half ? 1 : (rem == half ? 2 : 0);
+        *kind = 0;
+        return (-s >= 7) ? kSat : (mx << (-s));
     }
+    if (s >= 55) {
+        *kind = 0;
+        return 0;
+    }
+    const uint64_t rem = mx & ((1ull << s) - 1);
+    const uint64_t half = 1ull << (s - 1);
+    *kind = rem > half ? 1 : (rem == half ? 2 : 0);
+    return mx >> s;
+}
+
+// One fold step a' = fl(a + x) for a inside binade b, as a transducer.
+VSMCX_HD Tr tr_element(double x, int b)
+{
+    int kind;
+    const uint64_t q = grid_units(x, b, &kind);
     Tr t;
     t.b = b;
     if (kind == 0) {
@@ -154,6 +160,18 @@ VSMCX_HD Tr tr_element(double x, int b)
     if (t.d1 > kSat) t.d1 = kSat;
     const uint32_t p0 = (uint32_t)(t.d0 & 1), p1 = (uint32_t)((1 + t.d1) & 1);
     t.pp = p0 | (p1 << 1);
+    return t;
+}
+
+// A run of elements in binade b none of which is a tie adds the same number
+// of grid units d whatever the parity: the transducer (d, d, parity flip by d).
+VSMCX_HD Tr tr_units(uint64_t d, int b)
+{
+    Tr t;
+    t.d0 = d > kSat ? kSat : d;
+    t.d1 = t.d0;
+    t.pp = (uint32_t)(t.d0 & 1) | ((uint32_t)((1 + t.d0) & 1) << 1);
+    t.b = b;
     return t;
 }
 
@@ -196,26 +214,212 @@ VSMCX_HD bool tr_apply(const Tr &t, double v, double *out)
     return true;
 }
 
+// Lower end of the error interval around an approximate prefix S.
+VSMCX_HD double err_lo(double S, double eps_rel, double eps_abs)
+{
+    const double lo = S - (S * eps_rel + eps_abs);
+    return lo < 0 ? 0 : lo;
+}
+
 // Binade that every value within the error bound of S falls in, or kNoBinade.
 VSMCX_HD int certain_binade(double S, double eps_rel, double eps_abs)
 {
-    const double err = S * eps_rel + eps_abs;
-    double lo = S - err;
-    if (lo < 0) lo = 0;
-    const double hi = S + err;
-    const int bl = binade_of(lo), bh = binade_of(hi);
+    const double hi = S + (S * eps_rel + eps_abs);
+    const int bl = binade_of(err_lo(S, eps_rel, eps_abs)), bh = binade_of(hi);
     return bl == bh ? bl : kNoBinade;
 }
 
+// ---- segmented form of the whole fold -------------------------------------------
+// Every element is one of
+//   regular:     a_{i-1} and a_i lie for certain in one binade b -> its transducer;
+//   negligible:  x_i = 0, or x_i is below half the smallest grid a_{i-1}
+//                can have, so a_i = a_{i-1} exactly               -> identity;
+//   special:     anything else; evaluated later with one real FP64 add.
+// A Seg is the fold of a run of elements: the number of specials in it and the
+// transducer of the elements after its last special.  Regular elements between
+// two specials share one binade (a binade change forces a special), so the
+// composition never mixes binades when the classification bound holds.
+struct Seg {
+    Tr tr;
+    uint64_t cnt;
+};
+
+VSMCX_HD Seg seg_identity()
+{
+    Seg s;
+    s.tr = tr_identity();
+    s.cnt = 0;
+    return s;
+}
+
+// a then b
+VSMCX_HD Seg seg_op(const Seg &a, const Seg &b)
+{
+    Seg r;
+    const Tr c = tr_compose(a.tr, b.tr);
+    const uint64_t p = b.cnt != 0;
+    r.tr.d0 = pick(c.d0, b.tr.d0, p);
+    r.tr.d1 = pick(c.d1, b.tr.d1, p);
+    r.tr.pp = p ? b.tr.pp : c.pp;
+    r.tr.b = p ? b.tr.b : c.b;
+    r.cnt = a.cnt + b.cnt;
+    return r;
+}
+
+// Classify element x with approximate prefixes sb (before) and sa (after).
+// x = 0 never changes the running value (also past the end of the data, and
+// after a NaN or infinity); once the approximate prefix is not finite every
+// further non-zero element is special, which gives IEEE propagation.
+VSMCX_HD Seg elem_seg(double x, double sb, double sa, double eps_rel, double eps_abs)
+{
+    Seg s = seg_identity();
+    if (x == 0.0) return s;
+    if (!(binade_of(sa) < 1024)) {  // sa = sb + x is NaN or infinite
+        s.cnt = 1;
+        return s;
+    }
+    const int bp = certain_binade(sb, eps_rel, eps_abs);
+    const int bc = certain_binade(sa, eps_rel, eps_abs);
+    if (bp == bc && bp != kNoBinade) {
+        s.tr = tr_element(x, bc);
+    } else {
+        // a_{i-1} >= 2^blo, so its grid is at least 2^(blo-52); x < 2^(blo-53)
+        // (implied by binade_of(x) <= blo - 54) rounds away
+        const int blo = binade_of(err_lo(sb, eps_rel, eps_abs));
+        s.cnt = binade_of(x) <= blo - 54 ? 0 : 1;
+    }
+    return s;
+}
+
+// ---- one thread's run of consecutive elements -----------------------------------
+// x(k), k < n, are the elements; s is the approximate prefix before the first
+// (the per-element prefixes are s + x(0), then + x(1), ...).
+//
+// Fast path: when the error intervals around the first and the last prefix lie
+// in one binade b, every element is regular in b (the prefix and both ends of
+// its interval are monotone), and without ties the run adds the same number of
+// grid units whatever the parity.  fast_b receives b, or kNoBinade for the
+// element-by-element path.
+template <typename X>
+VSMCX_HD Seg run_seg(X x, int n, double s, double er, double ea, int *fast_b)
+{
+    double se = s;
+    for (int k = 0; k < n; ++k) se += x(k);
+    const int b0 = certain_binade(s, er, ea);
+    if (b0 < 1024 && b0 == certain_binade(se, er, ea)) {
+        uint64_t d = 0;
+        int tie = 0;
+        for (int k = 0; k < n; ++k) {
+            int kind;
+            d += grid_units(x(k), b0, &kind) + (kind == 1);
+            tie |= kind == 2;
+        }
+        if (!tie) {
+            *fast_b = b0;
+            Seg r;
+            r.tr = tr_units(d, b0);
+            r.cnt = 0;
+            return r;
+        }
+    }
+    *fast_b = kNoBinade;
+    Seg g = seg_identity();
+    for (int k = 0; k < n; ++k) {
+        const double xk = x(k);
+        const double sb = s;
+        s = sb + xk;
+        g = seg_op(g, elem_seg(xk, sb, s, er, ea));
+    }
+    return g;
+}
+
+// Exact fold values of the same run: pre is the summary of everything before
+// it, vb(c) the exact value right after special c (fold order).  out(k, a_k)
+// receives each value.  Returns false if a transducer does not apply or an
+// element taken as negligible would change the value (a broken bound).
+template <typename X, typename VB, typename OUT>
+VSMCX_HD bool run_values(X x, int n, double s, double er, double ea, int fast_b, Seg pre, VB vb,
+                         OUT out)
+{
+    bool ok = true;
+    uint64_t vcnt = pre.cnt;
+    double vbase = vcnt ? vb(vcnt - 1) : 0.0;
+    double a_prev;
+    if (!tr_apply(pre.tr, vbase, &a_prev)) {
+        ok = false;
+        a_prev = vbase;
+    }
+    if (fast_b != kNoBinade) {
+        if (binade_of(a_prev) != fast_b) ok = false;
+        uint64_t A = units_of(a_prev);
+        for (int k = 0; k < n; ++k) {
+            int kind;
+            A += grid_units(x(k), fast_b, &kind) + (kind == 1);
+            if (A >= kTwo53) ok = false;
+            out(k, value_of(fast_b, A & (kTwo53 - 1)));
+        }
+        return ok;
+    }
+    for (int k = 0; k < n; ++k) {
+        const double xk = x(k);
+        const double sb = s;
+        s = sb + xk;
+        const Seg el = elem_seg(xk, sb, s, er, ea);
+        pre = seg_op(pre, el);
+        if (pre.cnt != vcnt) {
+            vcnt = pre.cnt;
+            vbase = vb(vcnt - 1);
+        }
+        double a;
+        if (!tr_apply(pre.tr, vbase, &a)) {
+            ok = false;
+            a = vbase;
+        }
+        if (el.cnt == 0 && el.tr.b == kNoBinade && xk != 0.0 && a_prev + xk != a_prev) ok = false;
+        out(k, a);
+        a_prev = a;
+    }
+    return ok;
+}
+
+// One special element in fold order: the run transducer since the previous
+// special (or since the start, value 0) and x; the chain pass overwrites x
+// with the exact fold value right after the element.
+struct SpecRec {
+    Tr run;
+    double x;
+};
+
+#if defined(__HIPCC__)
+__device__ __forceinline__ Seg lane_up(const Seg &v, unsigned d)
+{
+    Seg r;
+    r.tr = lane_up(v.tr, d);
+    r.cnt = (uint64_t)__shfl_up((unsigned long long)v.cnt, d, 64);
+    return r;
+}
+__device__ __forceinline__ Seg lane_get(const Seg &v, int l)
+{
+    Seg r;
+    r.tr = lane_get(v.tr, l);
+    r.cnt = (uint64_t)__shfl((unsigned long long)v.cnt, l, 64);
+    return r;
+}
+#endif
+
 // Error bound of an approximate prefix sum of n non-negative terms against the
-// left fold over the same terms: both deviate from the exact real prefix by at
-// most ~n ulps plus a few ulps per term for a recomputed term; the factor 2^6
-// is headroom.  Too large a bound only costs more specials.
+// left fold over the same terms.  Any summation order of n non-negative terms
+// is within gamma_{n-1} * sum of the exact sum (Higham, Accuracy and Stability
+// of Numerical Algorithms, 2nd ed., eq. 4.4), so the two differ by at most
+// 2 * n * 2^-53 * sum; the bound below is twice that, and an absolute term
+// covers subnormal partial sums.  Both passes use the same x values.  Too
+// large a bound only costs more specials; too small breaks exactness, which
+// the device reports (tr_apply fails) instead of returning wrong values.
 VSMCX_HD void fold_eps(uint64_t n, double *eps_rel, double *eps_abs)
 {
-    const double nn = (double)n + 65536.0;
-    *eps_rel = nn * 0x1p-47;
-    *eps_abs = nn * 0x1p-1068;
+    const double nn = (double)n + 64.0;
+    *eps_rel = nn * 0x1p-51;
+    *eps_abs = nn * 0x1p-1070;
 }
 
 }  // namespace vsmcx

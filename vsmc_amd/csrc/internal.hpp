@@ -8,6 +8,7 @@
 #include <stdexcept>
 #include <string>
 
+#include "fold_core.hpp"
 #include "rng_core.hpp"
 
 namespace vsmcx {
@@ -15,8 +16,8 @@ namespace vsmcx {
 constexpr int kBlock = 256;                  // threads per workgroup
 constexpr int kItems = 16;                   // particles per thread
 constexpr int kTile = kBlock * kItems;       // particles per workgroup tile
-constexpr int kFoldItems = 8;                // elements per thread in k_fold_inv
-constexpr int kFoldTile = kBlock * kFoldItems;  // elements per k_fold_inv tile
+constexpr int kFoldItems = 8;                // elements per thread in the fold kernels
+constexpr int kFoldTile = kBlock * kFoldItems;  // elements per fold tile
 
 // Resampling schemes, same order as vsmc::ResampleScheme
 // (include/vsmc/resample/common.hpp:124-131).
@@ -63,6 +64,7 @@ struct FilterCtl {
     int prev_resampled;// the previous step resampled (move reads ancestors)
     int scheme;
     int err;
+    uint64_t specials; // special elements of the last fold
 };
 
 // One row of per-step history.
@@ -72,22 +74,17 @@ struct StepRow {
     uint64_t draws;
 };
 
-// Look-back bookkeeping for the exact-fold inversion kernel.
+// Scratch of the fold passes (kernels_fold.hip).
 struct FoldScratch {
-    uint64_t *flags_sum;   // approximate-sum look-back  [tiles]
-    double *agg_sum;       // tile sum (published with kStAgg)
-    double *incl_sum;      // inclusive prefix (published with kStIncl)
-    uint64_t *flags_fold;  // exact-fold look-back       [tiles]
-    double *val_fold;      // inclusive exact value      [tiles]
-    uint64_t *agg_d0;      // clean-tile transducer      [tiles]
-    uint64_t *agg_d1;
-    uint32_t *agg_pp;
-    int32_t *agg_b;
-    uint64_t *flags_cnt;   // zeros/surplus look-back    [tiles]
-    uint64_t *agg_cnt;     // tile counts
-    uint64_t *incl_cnt;    // inclusive counts
-    unsigned long long *tile_counter;  // device: tiles claimed in this launch
-    uint64_t *epoch;                   // device: look-back epoch of this launch
+    double *tsum;      // [tiles] tile sums of x
+    double *ts;        // [tiles] approximate exclusive prefix (fixed-order scan)
+    Seg *agg;          // [tiles] segment summary of each tile
+    Seg *pre;          // [tiles] exclusive segment summary
+    SpecRec *spec;     // [tiles * kFoldTile] special elements, tile-local order
+    double *gval;      // [M + 1] exact fold value after each special, fold order
+    uint32_t *rep32;   // [M] replication counts for the layout pass
+    uint64_t *zs;      // [tiles] zero slots << 32 | surplus copies
+    uint64_t *zsx;     // [tiles] exclusive prefix of zs
     size_t tiles;
 };
 

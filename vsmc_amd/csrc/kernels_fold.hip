@@ -1,16 +1,12 @@
 // kernels_fold.hip -- exact-fold resampling on the device.
 //
-// One kernel (k_fold_inv) per resample replaces vSMC's sequential
+// The passes below replace vSMC's sequential
 //   inversion  (include/vsmc/resample/common.hpp:57-84)  and
 //   cfrp_trans (include/vsmc/resample/common.hpp:87-118),
 // producing the replication counts, the keep-in-place ancestor layout and the
 // lists the zero-slot fill (k_zero_fill) consumes.  The running weight sum is
 // the exact left fold vSMC computes (see fold_core.hpp), so given the same
 // weights and uniforms the ancestors are the reference's, at any N.
-//
-// Work decomposition: one 256-thread workgroup per 4096-particle tile, tiles
-// claimed in dispatch order from an atomic counter, three decoupled
-// look-backs (approximate FP64 prefix, exact fold value, zero/surplus counts).
 #include "device_util.hpp"
 #include "fold_core.hpp"
 #include "internal.hpp"
@@ -117,6 +113,37 @@ struct InvStratified {
         const double b = u(n - 1);
         return a < b ? b : a;
     }
+    // v(w0 .. w0+len-1) into s_v (len <= kFoldTile); tmp holds len + 1 doubles.
+    // u_n are generated a Threefry block (four consecutive draws) per thread.
+    __device__ void fill_window(double *s_v, double *tmp, uint64_t w0, int len) const
+    {
+        const uint64_t nlo = w0 ? w0 - 1 : 0, nhi = w0 + (uint64_t)len;
+        const uint64_t p0 = nlo + eng.index, p1 = nhi + eng.index;  // stream positions
+        u64x4 c0;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) c0.v[j] = eng.ctr[j];
+        for (uint64_t bk = p0 / 4 + threadIdx.x; bk <= (p1 - 1) / 4; bk += blockDim.x) {
+            const u64x4 r = threefry4x64_20(ctr_add(c0, bk), eng.key);
+#pragma unroll
+            for (int w = 0; w < 4; ++w) {
+                const uint64_t p = 4 * bk + w;
+                if (p >= p0 && p < p1) {
+                    const uint64_t n = p - eng.index;
+                    tmp[n - nlo] = canonical64(r.v[w]) * delta + (double)n * delta;
+                }
+            }
+        }
+        __syncthreads();
+        for (int k = threadIdx.x; k < len; k += blockDim.x) {
+            const uint64_t n = w0 + k;
+            double a = tmp[n - nlo];
+            if (n > 0) {
+                const double b = tmp[n - 1 - nlo];
+                a = a < b ? b : a;
+            }
+            s_v[k] = a;
+        }
+    }
     __device__ uint64_t point_F(double c) const
     {
         if (nn == 0) return 0;
@@ -142,6 +169,10 @@ struct InvArray {
         const double a = uscan[n], b = tmax[n / kTile];
         return a < b ? b : a;
     }
+    __device__ void fill_window(double *s_v, double *, uint64_t w0, int len) const
+    {
+        for (int k = threadIdx.x; k < len; k += blockDim.x) s_v[k] = v(w0 + k);
+    }
     __device__ uint64_t point_F(double c) const
     {
         // binary search for the first n with v_n > c
@@ -158,82 +189,13 @@ struct InvArray {
 };
 
 // ------------------------------------------------------------------------
-// decoupled look-back
+// the fold passes
 // ------------------------------------------------------------------------
-enum : uint64_t { kStNone = 0, kStAgg = 1, kStIncl = 2, kStNC = 3 };
-
-__device__ __forceinline__ uint32_t lb_state(const uint64_t *flags, int64_t q, uint64_t epoch)
-{
-    const uint64_t f = atomic_load_rlx(flags + q);
-    return (f >> 2) == epoch ? (uint32_t)(f & 3) : (uint32_t)kStNone;
-}
-
-__device__ __forceinline__ void lb_publish(uint64_t *flags, uint64_t q, uint64_t epoch,
-                                           uint64_t st)
-{
-    atomic_store_rel(flags + q, (epoch << 2) | st);
-}
-
-// Exclusive prefix for tile t.  Called by all 64 lanes of one wave; every
-// lane returns the result.  Tiles before 0 count as an inclusive `start`.
-constexpr uint32_t kSpinLimit = 1u << 24;
-
-template <typename A, typename V, typename LoadAgg, typename LoadIncl, typename Compose,
-          typename Apply>
-__device__ V lookback(uint64_t t, const uint64_t *flags, uint64_t epoch, V start, A ident,
-                      LoadAgg load_agg, LoadIncl load_incl, Compose compose, Apply apply,
-                      int *timed_out)
-{
-    const int lane = threadIdx.x & 63;
-    A acc = ident;
-    int64_t hi = (int64_t)t - 1;
-    uint32_t spins = 0;
-    while (true) {
-        const int64_t q = hi - 63 + lane;
-        uint32_t st = q < 0 ? (uint32_t)kStIncl : lb_state(flags, q, epoch);
-        const uint64_t incl = __ballot(st == kStIncl);
-        const int l_incl = incl ? 63 - __clzll(incl) : -1;
-        const uint64_t notready = __ballot(lane > l_incl && st != kStAgg);
-        if (notready) {
-            // Wait on the nearest blocking predecessor alone (one broadcast load
-            // per poll instead of 64), backing off, then rescan the window.
-            // Bounded: a predecessor that never publishes is a bug; report it
-            // and leave instead of hanging the device.
-            const int wl = 63 - __clzll(notready);
-            const int64_t qw = hi - 63 + wl;
-            const uint32_t st_w = (uint32_t)__shfl((int)st, wl, 64);
-            uint32_t backoff = 1;
-            while (true) {
-                const uint32_t s2 = lb_state(flags, qw, epoch);
-                if (s2 != st_w) break;
-                if (++spins > kSpinLimit) {
-                    *timed_out = 1;
-                    return start;
-                }
-                for (uint32_t k = 0; k < backoff; ++k) __builtin_amdgcn_s_sleep(1);
-                backoff = backoff < 8 ? backoff * 2 : 8;
-            }
-            continue;
-        }
-        fence_acq_agent();  // payloads published before the flags we saw
-        A mine = (lane > l_incl) ? load_agg(q) : ident;
-        A comp = wave_incl_scan(mine, compose);
-        comp = lane_get(comp, 63);
-        acc = compose(comp, acc);
-        if (l_incl >= 0) {
-            V base = start;
-            const int64_t qi = hi - 63 + l_incl;
-            if (qi >= 0) base = load_incl(qi);
-            base = lane_get(base, l_incl);
-            return apply(acc, base);
-        }
-        hi -= 64;
-    }
-}
-
-// ------------------------------------------------------------------------
-// the kernel
-// ------------------------------------------------------------------------
+// Reduce-then-scan throughout: grid passes over 2048-element tiles and
+// single-workgroup scans over the per-tile summaries.  No workgroup waits for
+// another (a look-back chain across tiles would cross XCDs at every hop), and
+// every prefix is formed in one fixed order, so the passes that re-derive the
+// approximate prefix of an element get the same value bit for bit.
 struct FoldOut {
     uint32_t *anc;       // [M] ancestor index (zero slots filled by k_zero_fill)
     uint32_t *zl;        // [M] zero-slot list
@@ -250,6 +212,13 @@ constexpr int kFI = kFoldItems;
 constexpr int kFT = kFoldTile;
 constexpr int kFPad = kFT + kFT / kFI;
 __device__ __forceinline__ int fidx(int e) { return e + e / kFI; }
+
+constexpr int kScanBlock = 1024;  // single-workgroup tile scans
+constexpr int kScanItems = 8;     // tiles per thread per round
+
+struct SegOp {
+    __device__ __forceinline__ Seg operator()(const Seg &a, const Seg &b) const { return seg_op(a, b); }
+};
 
 // First n in [lo, hi) with v(n) > c (hi if none), v non-decreasing.  All 64
 // lanes of a wave probe 64 evenly spaced points per round (64-ary search).
@@ -277,320 +246,257 @@ __device__ uint64_t wave_first_greater(const Inv &inv, double c, uint64_t lo, ui
     return r < hi ? r : hi;
 }
 
-template <typename XSrc, typename Inv>
-__global__ __launch_bounds__(kBlock) void k_fold_inv(FilterCtl *ctl, XSrc xs, Inv inv, uint64_t M,
-                                                     FoldScratch fs, FoldOut out)
+// x of one tile into LDS (striped, coalesced); zero past the end, which
+// classifies as negligible.
+template <typename XSrc>
+__device__ __forceinline__ void load_tile(const XSrc &xs, uint64_t base, uint64_t M, double *s_x)
 {
-    if (!ctl->resample) return;
-    __shared__ double s_x[kFPad];  // x; later the uniform window / surplus prefix
-    __shared__ double s_a[kFPad];  // exact fold values
-    __shared__ double s_red[kBlock / kWave];
-    __shared__ Tr s_tr[kBlock / kWave];
-    __shared__ uint64_t s_u64[kBlock / kWave];
-    __shared__ int s_int[kBlock / kWave];
-    __shared__ unsigned long long s_tile;
-    __shared__ double s_bd[2];
-    __shared__ uint64_t s_bu[4];
-    __shared__ int s_timeout;
-
-    const int tid = threadIdx.x;
-    if (tid == 0) {
-        s_tile = atomicAdd(fs.tile_counter, 1ull);
-        s_timeout = 0;
-    }
-    const uint64_t epoch = *fs.epoch;
-    xs.bind(ctl);
-    inv.bind(ctl);
-    __syncthreads();
-    const uint64_t t = s_tile;
-    const uint64_t base = t * (uint64_t)kFT;
-    const int e0 = tid * kFI;  // this thread's elements: e0 .. e0 + kFI - 1
-
-    // ---- load x (striped, coalesced) into LDS --------------------------------
 #pragma unroll
     for (int k = 0; k < kFI; ++k) {
-        const int e = k * kBlock + tid;
+        const int e = k * kBlock + threadIdx.x;
         const uint64_t i = base + e;
         s_x[fidx(e)] = i < M ? xs.x(i) : 0.0;
     }
-    __syncthreads();
+}
 
-    // ---- approximate prefix: tile sum, look-back ------------------------------
-    double tsum = 0;
+// The segment pass and the apply pass derive every element's approximate
+// prefix from the tile's scanned start with the same code (thread_sum, the
+// block scan, run_seg), on the same data, so they classify every element
+// identically.
+__device__ __forceinline__ double thread_sum(const double *s_x, int e0)
+{
+    double s = 0;
 #pragma unroll 1
-    for (int k = 0; k < kFI; ++k) tsum += s_x[fidx(e0 + k)];
-    double tile_sum;
-    block_incl_scan<kBlock>(tsum, AddOp(), s_red, &tile_sum);
-    if (tid == 0) {
-        if (t == 0)
-            fs.incl_sum[t] = tile_sum;
-        else
-            fs.agg_sum[t] = tile_sum;
-        lb_publish(fs.flags_sum, t, epoch, t == 0 ? kStIncl : kStAgg);
-    }
-    if (tid < kWave) {
-        double S = 0.0;
-        if (t > 0) {
-            const double *agg = fs.agg_sum, *inc = fs.incl_sum;
-            S = lookback<double, double>(
-                t, fs.flags_sum, epoch, 0.0, 0.0, [=](int64_t q) { return agg[q]; },
-                [=](int64_t q) { return inc[q]; }, [](double a, double b) { return a + b; },
-                [](double a, double b) { return a + b; }, &s_timeout);
+    for (int k = 0; k < kFI; ++k) s += s_x[fidx(e0 + k)];
+    return s;
+}
+
+// Exclusive scan of n per-tile values by one workgroup of kScanBlock threads,
+// each taking kScanItems consecutive tiles per round; fixed order.  The loads
+// of a round are issued together (registers) so their latencies overlap.
+template <typename T, typename Op>
+__device__ T wg_scan(const T *in, T *out, uint64_t n, Op op, T ident, T *lds)
+{
+    T carry = ident;
+    for (uint64_t c0 = 0; c0 < n; c0 += (uint64_t)kScanBlock * kScanItems) {
+        const uint64_t i0 = c0 + (uint64_t)threadIdx.x * kScanItems;
+        T v[kScanItems];
+#pragma unroll
+        for (int k = 0; k < kScanItems; ++k) v[k] = i0 + k < n ? in[i0 + k] : ident;
+        T loc = ident;
+#pragma unroll
+        for (int k = 0; k < kScanItems; ++k) loc = op(loc, v[k]);
+        T tot;
+        T run = op(carry, block_excl_scan<kScanBlock>(loc, op, ident, lds, &tot));
+#pragma unroll
+        for (int k = 0; k < kScanItems; ++k) {
+            if (i0 + k < n) out[i0 + k] = run;
+            run = op(run, v[k]);
         }
-        if (tid == 0) {
-            s_bd[0] = S;
-            if (t > 0) {
-                fs.incl_sum[t] = S + tile_sum;
-                lb_publish(fs.flags_sum, t, epoch, kStIncl);
-            }
-        }
+        carry = op(carry, tot);
     }
+    return carry;
+}
+
+// ---- pass 1: tile sums of x ---------------------------------------------------
+template <typename XSrc>
+__global__ __launch_bounds__(kBlock) void k_fold_tsum(FilterCtl *ctl, XSrc xs, uint64_t M,
+                                                      FoldScratch fs)
+{
+    if (!ctl->resample) return;
+    __shared__ double s_red[kBlock / kWave];
+    xs.bind(ctl);
+    const uint64_t base = blockIdx.x * (uint64_t)kFT;
+    double s = 0;
+#pragma unroll
+    for (int k = 0; k < kFI; ++k) {
+        const uint64_t i = base + k * kBlock + threadIdx.x;
+        s += i < M ? xs.x(i) : 0.0;
+    }
+    s = block_reduce<kBlock>(s, AddOp(), s_red);
+    if (threadIdx.x == 0) fs.tsum[blockIdx.x] = s;
+}
+
+// ---- pass 2: approximate tile starts (one workgroup) ---------------------------
+__global__ __launch_bounds__(kScanBlock) void k_fold_scan_sum(FilterCtl *ctl, FoldScratch fs)
+{
+    if (!ctl->resample) return;
+    __shared__ double s_red[kScanBlock / kWave];
+    wg_scan<double>(fs.tsum, fs.ts, fs.tiles, AddOp(), 0.0, s_red);
+}
+
+// ---- pass 3: classification, tile segment summaries, special records -----------
+// A tile's specials are stored in tile order at spec[tile * kFT + k], each
+// with the run since the previous special of the tile (or since the tile start).
+template <typename XSrc>
+__global__ __launch_bounds__(kBlock) void k_fold_seg(FilterCtl *ctl, XSrc xs, uint64_t M,
+                                                     FoldScratch fs)
+{
+    if (!ctl->resample) return;
+    __shared__ double s_x[kFPad];
+    __shared__ double s_red[kBlock / kWave];
+    __shared__ Seg s_seg[kBlock / kWave];
+    const int tid = threadIdx.x;
+    const uint64_t t = blockIdx.x;
+    const uint64_t base = t * (uint64_t)kFT;
+    const int e0 = tid * kFI;
+    xs.bind(ctl);
+    load_tile(xs, base, M, s_x);
     __syncthreads();
-    if (tid == 0 && s_timeout) atomicOr(&ctl->err, (int)kErrTimeout);
-    const double S_t = s_bd[0];
-
-    // ---- classify the tile --------------------------------------------------
-    double eps_rel, eps_abs;
-    fold_eps(M, &eps_rel, &eps_abs);
-    const int b0 = certain_binade(S_t, eps_rel, eps_abs);
-    const int b1 = certain_binade(S_t + tile_sum, eps_rel, eps_abs);
-    const bool clean = t > 0 && b0 == b1 && b0 != kNoBinade;
-    const auto compose = [](const Tr &a, const Tr &c) { return tr_compose(a, c); };
-
-    // ---- per-thread transducers, computed before the exact start is known ------
-    // Clean tiles: binade bA = b0; publish the tile transducer.  Other tiles:
-    // bA = lowest binade the start can be in, bB = bA + 1; prefix scan in bA and
-    // reverse (suffix) scan in bB let the end value follow from the exact start
-    // with one walk and one real add when the tile holds a single crossing.
-    const int bA = clean ? b0 : binade_of(S_t - (S_t * eps_rel + eps_abs) > 0
-                                              ? S_t - (S_t * eps_rel + eps_abs) : 0.0);
-    const int bB = bA + 1;
-    Tr PA = tr_identity(), TA = tr_identity(), RB = tr_identity(), TB = tr_identity();
-    if (t > 0) {
-        Tr thr = tr_identity();
-#pragma unroll 1
-        for (int k = 0; k < kFI; ++k) thr = tr_compose(thr, tr_element(s_x[fidx(e0 + k)], bA));
-        PA = block_excl_scan<kBlock>(thr, compose, tr_identity(), s_tr, &TA);
-        if (clean) {
-            if (tid == 0) {
-                fs.agg_d0[t] = TA.d0;
-                fs.agg_d1[t] = TA.d1;
-                fs.agg_pp[t] = TA.pp;
-                fs.agg_b[t] = TA.b;
-                lb_publish(fs.flags_fold, t, epoch, kStAgg);
-            }
-        } else {
-            if (tid == 0) lb_publish(fs.flags_fold, t, epoch, kStNC);
-            Tr thb = tr_identity();
-#pragma unroll 1
-            for (int k = 0; k < kFI; ++k) thb = tr_compose(thb, tr_element(s_x[fidx(e0 + k)], bB));
-            // reverse inclusive scan: z_k = T_{255-k} o ... o T_255, via reversed order
-            Tr *s_rev = reinterpret_cast<Tr *>(s_a);  // kBlock entries, s_a not yet in use
-            s_rev[kBlock - 1 - tid] = thb;
-            __syncthreads();
-            const Tr y = s_rev[tid];
-            __syncthreads();
-            const Tr z = block_incl_scan<kBlock>(
-                y, [](const Tr &p, const Tr &q) { return tr_compose(q, p); }, s_tr, &TB);
-            s_rev[tid] = z;
-            __syncthreads();
-            RB = tid < kBlock - 1 ? s_rev[kBlock - 2 - tid] : tr_identity();  // threads > tid
-            __syncthreads();
-        }
-    }
-
-    // ---- exact start value of the tile -----------------------------------------
-    if (tid < kWave) {
-        double v0 = 0.0;
-        if (t > 0) {
-            const uint64_t *ad0 = fs.agg_d0, *ad1 = fs.agg_d1;
-            const uint32_t *app = fs.agg_pp;
-            const int32_t *ab = fs.agg_b;
-            const double *vf = fs.val_fold;
-            int *errp = &ctl->err;
-            v0 = lookback<Tr, double>(
-                t, fs.flags_fold, epoch, 0.0, tr_identity(),
-                [=](int64_t q) {
-                    Tr a;
-                    a.d0 = ad0[q];
-                    a.d1 = ad1[q];
-                    a.pp = app[q];
-                    a.b = ab[q];
-                    return a;
-                },
-                [=](int64_t q) { return vf[q]; }, compose,
-                [=](const Tr &a, double v) {
-                    double r;
-                    if (!tr_apply(a, v, &r)) {
-                        atomicOr(errp, (int)kErrFoldClass);
-                        r = v;
-                    }
-                    return r;
-                },
-                &s_timeout);
-        }
-        if (tid == 0) {
-            s_bd[1] = v0;
-            if (clean) {  // publish the inclusive value right away
-                double e;
-                if (!tr_apply(TA, v0, &e)) atomicOr(&ctl->err, (int)kErrFoldClass);
-                fs.val_fold[t] = e;
-                lb_publish(fs.flags_fold, t, epoch, kStIncl);
-            }
-        }
-    }
-    __syncthreads();
-    if (tid == 0 && s_timeout) atomicOr(&ctl->err, (int)kErrTimeout);
-    const double v0 = s_bd[1];
-    bool published = clean;
-
-    if (clean) {
-        // every element: the thread prefix applied to the start, then a walk
-        double v;
-        if (!tr_apply(PA, v0, &v)) {
-            atomicOr(&ctl->err, (int)kErrFoldClass);
-            v = v0;
-        }
+    const double s_thr =
+        fs.ts[t] + block_excl_scan<kBlock>(thread_sum(s_x, e0), AddOp(), 0.0, s_red, (double *)nullptr);
+    double er, ea;
+    fold_eps(M, &er, &ea);
+    const auto xat = [&](int k) { return s_x[fidx(e0 + k)]; };
+    int fast_b;
+    const Seg g = run_seg(xat, kFI, s_thr, er, ea, &fast_b);
+    Seg tile;
+    const Seg thr_pre = block_excl_scan<kBlock>(g, SegOp(), seg_identity(), s_seg, &tile);
+    if (tid == 0) fs.agg[t] = tile;
+    if (g.cnt) {
+        Seg pre = thr_pre;  // tile-local
+        double s = s_thr;
 #pragma unroll 1
         for (int k = 0; k < kFI; ++k) {
-            const int e = e0 + k;
-            double v2;
-            if (!tr_apply(tr_element(s_x[fidx(e)], bA), v, &v2)) {
-                atomicOr(&ctl->err, (int)kErrFoldClass);
-                v2 = v;
+            const double x = s_x[fidx(e0 + k)];
+            const double sb = s;
+            s = sb + x;
+            const Seg el = elem_seg(x, sb, s, er, ea);
+            if (el.cnt) {
+                SpecRec r;
+                r.run = pre.tr;
+                r.x = x;
+                fs.spec[base + pre.cnt] = r;
             }
-            s_a[fidx(e)] = v2;
-            v = v2;
+            pre = seg_op(pre, el);
         }
-        __syncthreads();
-    } else if (t > 0) {
-        // ---- end value first (the next tiles wait for it) ---------------------
-        const int b = binade_of(v0);
-        int ok = 0;
-        double endv = 0;
-        if (b == bB) {
-            ok = tr_apply(TB, v0, &endv);
-        } else if (b == bA) {
-            int first_fail = kFT;
-            double vprev = 0, vt;
-            if (!tr_apply(PA, v0, &vt)) {
-                first_fail = e0;
-            } else {
-#pragma unroll 1
-                for (int k = 0; k < kFI; ++k) {
-                    double v2;
-                    if (!tr_apply(tr_element(s_x[fidx(e0 + k)], bA), vt, &v2)) {
-                        first_fail = e0 + k;
-                        vprev = vt;
-                        break;
-                    }
-                    vt = v2;
-                }
-            }
-            const int j = block_reduce<kBlock>(first_fail, MinOp(), s_int);
-            if (j >= kFT) {
-                ok = tr_apply(TA, v0, &endv);
-            } else {
-                if (tid == j / kFI) {
-                    // owner: one real add at the crossing, then the rest in bB
-                    const double aj = vprev + s_x[fidx(j)];
-                    Tr part = tr_identity();
-#pragma unroll 1
-                    for (int e = j + 1; e < e0 + kFI; ++e)
-                        part = tr_compose(part, tr_element(s_x[fidx(e)], bB));
-                    double w;
-                    int okk = binade_of(aj) == bB && tr_apply(part, aj, &w);
-                    double en = 0;
-                    if (okk) okk = tr_apply(RB, w, &en);
-                    s_bd[0] = en;
-                    s_int[0] = okk;
-                }
-                __syncthreads();
-                ok = s_int[0];
-                endv = s_bd[0];
-                __syncthreads();
-            }
-        }
-        if (ok && tid == 0) {
-            fs.val_fold[t] = endv;
-            lb_publish(fs.flags_fold, t, epoch, kStIncl);
-        }
-        published = ok;
-        __syncthreads();
     }
+}
 
-    if (!clean) {
-        // ---- exact values of every element: carry the fold through the tile ----
-        // Each pass composes the elements from `pos` in the binade of the running
-        // value; the first element whose result would leave that binade gets one
-        // real FP64 add, and the next pass starts after it.  The first tile
-        // starts at 0 and crosses a binade every doubling, so its first
-        // elements are folded one by one.
-        double cur = v0;
-        int pos = 0;
-        if (t == 0) {
-            constexpr int kSerial = 256;
-            if (tid == 0) {
-                double a = 0;
-                for (int e = 0; e < kSerial; ++e) {
-                    a += s_x[fidx(e)];
-                    s_a[fidx(e)] = a;
-                }
+// ---- pass 4: tile segment prefixes and the exact values at the specials --------
+// One workgroup.  The exclusive segment summary of every tile, then the
+// specials in fold order, a chunk of kScanBlock at a time: each thread finds
+// its record (binary search over the tiles' special counts, kept in LDS for
+// up to kLdsTiles tiles), thread 0 runs the chain (one transducer application
+// and one real FP64 add per special), and the values go to gval in fold order.
+constexpr int kLdsTiles = 16384;
+
+__global__ __launch_bounds__(kScanBlock) void k_fold_scan_seg(FilterCtl *ctl, FoldScratch fs)
+{
+    if (!ctl->resample) return;
+    __shared__ Seg s_seg[kScanBlock / kWave];
+    __shared__ uint32_t s_cnt[kLdsTiles + 1];  // specials before each tile
+    __shared__ SpecRec s_r[kScanBlock];
+    __shared__ double s_v[kScanBlock];
+    const int tid = threadIdx.x;
+    const uint64_t T = fs.tiles;
+    const Seg tail = wg_scan<Seg>(fs.agg, fs.pre, T, SegOp(), seg_identity(), s_seg);
+    __syncthreads();  // pre[] written by other threads
+    const uint64_t S = tail.cnt;
+    const bool in_lds = T <= (uint64_t)kLdsTiles;
+    if (in_lds)
+        for (uint64_t t = tid; t < T; t += kScanBlock) s_cnt[t] = (uint32_t)fs.pre[t].cnt;
+    __syncthreads();
+    double a = 0.0;  // running exact value (thread 0)
+    int bad = 0;
+    for (uint64_t c0 = 0; c0 < S; c0 += kScanBlock) {
+        const uint64_t g = c0 + tid;
+        if (g < S) {
+            // last tile whose count of earlier specials is <= g
+            uint64_t lo = 0, hi = T - 1;
+            while (lo < hi) {
+                const uint64_t mid = (lo + hi + 1) / 2;
+                const uint64_t c = in_lds ? s_cnt[mid] : fs.pre[mid].cnt;
+                if (c <= g)
+                    lo = mid;
+                else
+                    hi = mid - 1;
             }
-            __syncthreads();
-            cur = s_a[fidx(kSerial - 1)];
-            pos = kSerial;
-        }
-        while (pos < kFT) {
-            const int b = binade_of(cur);
-            Tr thr = tr_identity();
-#pragma unroll 1
-            for (int k = 0; k < kFI; ++k) {
-                const int e = e0 + k;
-                if (e >= pos) thr = tr_compose(thr, tr_element(s_x[fidx(e)], b));
-            }
-            const Tr excl =
-                block_excl_scan<kBlock>(thr, compose, tr_identity(), s_tr, (Tr *)nullptr);
-            int first_fail = kFT;
-            double v;
-            if (!tr_apply(excl, cur, &v)) {
-                first_fail = e0;
-            } else {
-#pragma unroll 1
-                for (int k = 0; k < kFI; ++k) {
-                    const int e = e0 + k;
-                    if (e < pos) continue;
-                    double v2;
-                    if (!tr_apply(tr_element(s_x[fidx(e)], b), v, &v2)) {
-                        first_fail = e;
-                        break;
-                    }
-                    s_a[fidx(e)] = v2;
-                    v = v2;
-                }
-            }
-            const int j = block_reduce<kBlock>(first_fail, MinOp(), s_int);
-            if (j >= kFT) break;
-            if (tid == j / kFI) {
-                const double prev = (j == pos) ? cur : s_a[fidx(j - 1)];
-                s_a[fidx(j)] = prev + s_x[fidx(j)];
-            }
-            __syncthreads();
-            cur = s_a[fidx(j)];
-            pos = j + 1;
-            __syncthreads();
+            const Seg p = fs.pre[lo];
+            const uint64_t k = g - p.cnt;
+            SpecRec r = fs.spec[lo * (uint64_t)kFT + k];
+            if (k == 0) r.run = tr_compose(p.tr, r.run);
+            s_r[tid] = r;
         }
         __syncthreads();
         if (tid == 0) {
-            if (!published) {
-                fs.val_fold[t] = s_a[fidx(kFT - 1)];
-                lb_publish(fs.flags_fold, t, epoch, kStIncl);
-            } else if (fs.val_fold[t] != s_a[fidx(kFT - 1)]) {
-                atomicOr(&ctl->err, (int)kErrFoldClass);
+            const int n = (int)(S - c0 < (uint64_t)kScanBlock ? S - c0 : (uint64_t)kScanBlock);
+#pragma unroll 1
+            for (int k = 0; k < n; ++k) {
+                double v;
+                if (!tr_apply(s_r[k].run, a, &v)) {
+                    bad = 1;
+                    v = a;
+                }
+                a = v + s_r[k].x;
+                s_v[k] = a;
             }
         }
+        __syncthreads();
+        if (g < S) fs.gval[g] = s_v[tid];
+        __syncthreads();
     }
-    const double v_start = v0;
+    if (tid == 0) {
+        double tot;
+        if (!tr_apply(tail.tr, a, &tot)) {
+            bad = 1;
+            tot = a;
+        }
+        ctl->fold_total = tot;
+        ctl->specials = S;
+        if (bad) atomicOr(&ctl->err, (int)kErrFoldClass);
+    }
+}
+
+// ---- pass 5: exact values, inversion, replication --------------------------------
+// Exact a_i = (run since the last special) applied to the value after that
+// special.  Then c_i = F(a_i) (uniforms at or below a_i) and the replication
+// counts as differences; per tile the number of zero slots and of surplus
+// copies for the layout pass.
+template <typename XSrc, typename Inv>
+__global__ __launch_bounds__(kBlock) void k_fold_apply(FilterCtl *ctl, XSrc xs, Inv inv,
+                                                       uint64_t M, FoldScratch fs, FoldOut out)
+{
+    if (!ctl->resample) return;
+    __shared__ double s_x[kFPad];  // x; later the uniform window
+    __shared__ double s_a[kFPad];  // exact fold values
+    __shared__ double s_red[kBlock / kWave];
+    __shared__ Seg s_seg[kBlock / kWave];
+    __shared__ uint64_t s_u64[kBlock / kWave];
+    __shared__ uint64_t s_bu[2];
+
+    const int tid = threadIdx.x;
+    const uint64_t t = blockIdx.x;
+    const uint64_t base = t * (uint64_t)kFT;
+    const int e0 = tid * kFI;
+    xs.bind(ctl);
+    inv.bind(ctl);
+    load_tile(xs, base, M, s_x);
+    __syncthreads();
+
+    // ---- exact fold values ----------------------------------------------------
+    const double s_thr =
+        fs.ts[t] + block_excl_scan<kBlock>(thread_sum(s_x, e0), AddOp(), 0.0, s_red, (double *)nullptr);
+    const Seg tpre = fs.pre[t];
+    double er, ea;
+    fold_eps(M, &er, &ea);
+    const auto xat = [&](int k) { return s_x[fidx(e0 + k)]; };
+    int fast_b;
+    const Seg g = run_seg(xat, kFI, s_thr, er, ea, &fast_b);
+    const Seg thr_pre =
+        block_excl_scan<kBlock>(g, SegOp(), seg_identity(), s_seg, (Seg *)nullptr);
+    const double *gval = fs.gval;
+    int bad = !run_values(
+        xat, kFI, s_thr, er, ea, fast_b, seg_op(tpre, thr_pre),
+        [=](uint64_t c) { return gval[c]; }, [&](int k, double a) { s_a[fidx(e0 + k)] = a; });
+    double v_start = 0.0;  // exact a_{base-1}
+    if (t > 0) {
+        const double vb = tpre.cnt ? fs.gval[tpre.cnt - 1] : 0.0;
+        if (!tr_apply(tpre.tr, vb, &v_start)) bad = 1;
+    }
+    if (bad) atomicOr(&ctl->err, (int)kErrFoldClass);
+    __syncthreads();
+
     if (out.a_out) {
 #pragma unroll 1
         for (int k = 0; k < kFI; ++k) {
@@ -599,7 +505,6 @@ __global__ __launch_bounds__(kBlock) void k_fold_inv(FilterCtl *ctl, XSrc xs, In
             if (i < M) out.a_out[i] = s_a[fidx(e)];
         }
     }
-    if (t == fs.tiles - 1 && tid == 0) ctl->fold_total = s_a[fidx((int)((M - 1) - base))];
 
     if constexpr (!Inv::kInv) {
         return;
@@ -642,22 +547,31 @@ __global__ __launch_bounds__(kBlock) void k_fold_inv(FilterCtl *ctl, XSrc xs, In
             cstart = wlo;
 #pragma unroll
             for (int k = 0; k < kFI; ++k) c[k] = wlo;
-            for (uint64_t w0 = wlo; w0 < whi; w0 += kFPad) {
-                const int len = (int)((whi - w0) < (uint64_t)kFPad ? (whi - w0) : (uint64_t)kFPad);
+            for (uint64_t w0 = wlo; w0 < whi; w0 += kFT) {
+                const int len = (int)((whi - w0) < (uint64_t)kFT ? (whi - w0) : (uint64_t)kFT);
                 __syncthreads();
-#pragma unroll 1
-                for (int k = tid; k < len; k += kBlock) s_x[k] = inv.v(w0 + k);
+                inv.fill_window(s_x, s_a, w0, len);  // s_a is free once av[] is loaded
                 __syncthreads();
                 const double first = s_x[0], lastv = s_x[len - 1];
+                // the thread's values are sorted: gallop from the previous
+                // position, then bisect (first index with value > a)
+                int pos = 0;
 #pragma unroll
                 for (int k = 0; k < kFI; ++k) {
                     const double a = av[k];
                     if (a < first) continue;
                     if (a >= lastv) {
                         c[k] += len;
+                        pos = len;
                         continue;
                     }
-                    int lo = 0, hi = len;  // first index with value > a
+                    int lo = pos, step = 1, hi = pos;
+                    while (hi < len && s_x[hi] <= a) {
+                        lo = hi + 1;
+                        hi += step;
+                        step <<= 1;
+                    }
+                    if (hi > len) hi = len;
                     while (lo < hi) {
                         const int mid = (lo + hi) >> 1;
                         if (s_x[mid] <= a)
@@ -666,6 +580,7 @@ __global__ __launch_bounds__(kBlock) void k_fold_inv(FilterCtl *ctl, XSrc xs, In
                             hi = mid;
                     }
                     c[k] += lo;
+                    pos = lo;
                 }
             }
 #pragma unroll
@@ -685,9 +600,9 @@ __global__ __launch_bounds__(kBlock) void k_fold_inv(FilterCtl *ctl, XSrc xs, In
             cprev = tid == 0 ? cstart : up;
             __syncthreads();
         }
-        // replication counts
-        uint32_t rep[kFI];
-        uint64_t zs_thr = 0;  // zeros << 32 | surplus
+        // replication counts, staged in LDS for striped stores
+        uint32_t *s_rep = reinterpret_cast<uint32_t *>(s_x);
+        uint64_t zs_thr = 0;  // zeros << 32 | surplus copies
 #pragma unroll
         for (int k = 0; k < kFI; ++k) {
             const uint64_t i = base + e0 + k;
@@ -700,95 +615,113 @@ __global__ __launch_bounds__(kBlock) void k_fold_inv(FilterCtl *ctl, XSrc xs, In
                 else  // i == M-1
                     r = nn - cprev;
                 r += xs.ipart(i);
-            }
-            cprev = c[k];
-            rep[k] = (uint32_t)r;
-            if (i < M) {
                 if (r == 0)
                     zs_thr += 1ull << 32;
                 else if (r >= 2)
                     zs_thr += r - 1;
+                if (out.rep) out.rep[i] = r;
             }
-            if (out.rep && i < M) out.rep[i] = r;
+            cprev = c[k];
+            s_rep[e0 + k] = (uint32_t)r;
             if (M >= 2 && i == last_cmp) ctl->n_final = c[k];
         }
         if (M == 1 && t == 0 && tid == 0) ctl->n_final = 0;
         if (!out.anc) return;  // replication only (M != N): no cfrp layout
-
-        // ---- zero / surplus ranks: block scan + look-back --------------------
-        uint64_t zs_tile;
-        const uint64_t zs_excl =
-            block_excl_scan<kBlock>(zs_thr, AddOp(), (uint64_t)0, s_u64, &zs_tile);
-        if (tid == 0) {
-            if (t == 0)
-                fs.incl_cnt[t] = zs_tile;
-            else
-                fs.agg_cnt[t] = zs_tile;
-            lb_publish(fs.flags_cnt, t, epoch, t == 0 ? kStIncl : kStAgg);
-        }
-        if (tid < kWave) {
-            uint64_t pre = 0;
-            if (t > 0) {
-                const uint64_t *agg = fs.agg_cnt, *inc = fs.incl_cnt;
-                pre = lookback<uint64_t, uint64_t>(
-                    t, fs.flags_cnt, epoch, 0ull, 0ull, [=](int64_t q) { return agg[q]; },
-                    [=](int64_t q) { return inc[q]; },
-                    [](uint64_t a, uint64_t b) { return a + b; },
-                    [](uint64_t a, uint64_t b) { return a + b; }, &s_timeout);
-            }
-            if (tid == 0) {
-                s_bu[2] = pre;
-                if (t > 0) {
-                    fs.incl_cnt[t] = pre + zs_tile;
-                    lb_publish(fs.flags_cnt, t, epoch, kStIncl);
-                }
-                if (t == fs.tiles - 1) ctl->zeros = (pre + zs_tile) >> 32;
-            }
-        }
-        __syncthreads();
-        if (tid == 0 && s_timeout) atomicOr(&ctl->err, (int)kErrTimeout);
-        const uint64_t pre = s_bu[2];
-        uint64_t zrank = (pre >> 32) + (zs_excl >> 32);
-        uint64_t srank_local = zs_excl & 0xffffffffull;  // surplus before this thread (tile)
-        const uint64_t s_base = pre & 0xffffffffull;
-
-        // anc identity, zero list, per-item inclusive surplus prefix into LDS
-        uint32_t *s_sp = reinterpret_cast<uint32_t *>(s_x);  // kFT entries
-        __syncthreads();
+        const uint64_t zs_tile = block_reduce<kBlock>(zs_thr, AddOp(), s_u64);
+        if (tid == 0) fs.zs[t] = zs_tile;
 #pragma unroll
         for (int k = 0; k < kFI; ++k) {
-            const int e = e0 + k;
+            const int e = k * kBlock + tid;
             const uint64_t i = base + e;
-            if (i < M) {
-                out.anc[i] = (uint32_t)i;
-                if (rep[k] == 0) {
-                    if (zrank < M)
-                        out.zl[zrank] = (uint32_t)i;
-                    else
-                        atomicOr(&ctl->err, (int)kErrBounds);
-                    ++zrank;
-                }
-                if (rep[k] >= 2) srank_local += rep[k] - 1;
-            }
-            s_sp[e] = (uint32_t)srank_local;
+            if (i < M) fs.rep32[i] = s_rep[e];
         }
-        __syncthreads();
-        // expand surplus copies: positions [s_base, s_base + S_tile)
-        const uint32_t S_tile = (uint32_t)(zs_tile & 0xffffffffull);
-        for (uint32_t k = tid; k < S_tile; k += kBlock) {
-            int lo = 0, hi = kFT - 1;  // first e with s_sp[e] > k
-            while (lo < hi) {
-                const int mid = (lo + hi) >> 1;
-                if (s_sp[mid] > k)
-                    hi = mid;
-                else
-                    lo = mid + 1;
-            }
-            if (s_base + k < M)
-                out.sl[s_base + k] = (uint32_t)(base + lo);
+    }
+}
+
+// ---- pass 6: zero / surplus ranks of the tiles (one workgroup) ---------------------
+__global__ __launch_bounds__(kScanBlock) void k_fold_scan_cnt(FilterCtl *ctl, FoldScratch fs)
+{
+    if (!ctl->resample) return;
+    __shared__ uint64_t s_red[kScanBlock / kWave];
+    const uint64_t tot = wg_scan<uint64_t>(fs.zs, fs.zsx, fs.tiles, AddOp(), (uint64_t)0, s_red);
+    if (threadIdx.x == 0) ctl->zeros = tot >> 32;
+}
+
+// ---- pass 7: cfrp_trans layout (common.hpp:91-110) ------------------------------------
+// anc[i] = i where rep_i > 0; the k-th zero slot (in index order) goes to the
+// zero list at rank k, the k-th surplus copy (index order, rep_i - 1 copies
+// of i) to the surplus list at rank k; k_zero_fill pairs them.
+__global__ __launch_bounds__(kBlock) void k_fold_layout(FilterCtl *ctl, FoldScratch fs,
+                                                        FoldOut out, uint64_t M)
+{
+    if (!ctl->resample) return;
+    __shared__ uint32_t s_rep[kFT];
+    __shared__ uint32_t s_sp[kFT];  // inclusive surplus prefix per element
+    __shared__ uint64_t s_u64[kBlock / kWave];
+    const int tid = threadIdx.x;
+    const uint64_t t = blockIdx.x;
+    const uint64_t base = t * (uint64_t)kFT;
+    const int e0 = tid * kFI;
+#pragma unroll
+    for (int k = 0; k < kFI; ++k) {
+        const int e = k * kBlock + tid;
+        const uint64_t i = base + e;
+        s_rep[e] = i < M ? fs.rep32[i] : 1u;
+    }
+    __syncthreads();
+    uint64_t zs_thr = 0;
+#pragma unroll 1
+    for (int k = 0; k < kFI; ++k) {
+        const uint32_t r = s_rep[e0 + k];
+        if (r == 0)
+            zs_thr += 1ull << 32;
+        else if (r >= 2)
+            zs_thr += r - 1;
+    }
+    uint64_t zs_tile;
+    const uint64_t zs_excl =
+        block_excl_scan<kBlock>(zs_thr, AddOp(), (uint64_t)0, s_u64, &zs_tile);
+    const uint64_t pre = fs.zsx[t];
+    uint64_t zrank = (pre >> 32) + (zs_excl >> 32);
+    uint64_t srank_local = zs_excl & 0xffffffffull;  // surplus before this thread (tile)
+    const uint64_t s_base = pre & 0xffffffffull;
+#pragma unroll 1
+    for (int k = 0; k < kFI; ++k) {
+        const int e = e0 + k;
+        const uint64_t i = base + e;
+        const uint32_t r = s_rep[e];
+        if (i < M && r == 0) {
+            if (zrank < M)
+                out.zl[zrank] = (uint32_t)i;
             else
                 atomicOr(&ctl->err, (int)kErrBounds);
+            ++zrank;
         }
+        if (r >= 2) srank_local += r - 1;
+        s_sp[e] = (uint32_t)srank_local;
+    }
+#pragma unroll
+    for (int k = 0; k < kFI; ++k) {
+        const int e = k * kBlock + tid;
+        const uint64_t i = base + e;
+        if (i < M) out.anc[i] = (uint32_t)i;
+    }
+    __syncthreads();
+    // expand surplus copies: positions [s_base, s_base + S_tile)
+    const uint32_t S_tile = (uint32_t)(zs_tile & 0xffffffffull);
+    for (uint32_t k = tid; k < S_tile; k += kBlock) {
+        int lo = 0, hi = kFT - 1;  // first e with s_sp[e] > k
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (s_sp[mid] > k)
+                hi = mid;
+            else
+                lo = mid + 1;
+        }
+        if (s_base + k < M)
+            out.sl[s_base + k] = (uint32_t)(base + lo);
+        else
+            atomicOr(&ctl->err, (int)kErrBounds);
     }
 }
 
@@ -968,22 +901,25 @@ __global__ __launch_bounds__(kBlock) void k_sorted_values(FilterCtl *ctl, const 
 // ------------------------------------------------------------------------
 // host launchers
 // ------------------------------------------------------------------------
-// new look-back epoch and a zeroed tile counter for the next k_fold_inv
-__global__ void k_fold_begin(FilterCtl *ctl, uint64_t *epoch, unsigned long long *counter)
-{
-    if (!ctl->resample || threadIdx.x != 0) return;
-    *epoch += 1;
-    *counter = 0;
-}
-
+// Passes 1-4 give the fold total and the values at the specials; pass 5 runs
+// when anything per element is wanted (inversion or fold values); passes 6-7
+// when the ancestor layout is.
 template <typename XSrc, typename Inv>
 static void launch_fold(FilterCtl *ctl, XSrc xs, Inv inv, uint64_t M, FoldScratch &fs,
                         const FoldOut &out, hipStream_t st)
 {
     fs.tiles = (M + kFT - 1) / kFT;
-    hipLaunchKernelGGL(k_fold_begin, dim3(1), dim3(64), 0, st, ctl, fs.epoch, fs.tile_counter);
-    hipLaunchKernelGGL((k_fold_inv<XSrc, Inv>), dim3((unsigned)fs.tiles), dim3(kBlock), 0, st, ctl,
-                       xs, inv, M, fs, out);
+    const dim3 grid((unsigned)fs.tiles);
+    hipLaunchKernelGGL((k_fold_tsum<XSrc>), grid, dim3(kBlock), 0, st, ctl, xs, M, fs);
+    hipLaunchKernelGGL(k_fold_scan_sum, dim3(1), dim3(kScanBlock), 0, st, ctl, fs);
+    hipLaunchKernelGGL((k_fold_seg<XSrc>), grid, dim3(kBlock), 0, st, ctl, xs, M, fs);
+    hipLaunchKernelGGL(k_fold_scan_seg, dim3(1), dim3(kScanBlock), 0, st, ctl, fs);
+    if (!Inv::kInv && !out.a_out) return;
+    hipLaunchKernelGGL((k_fold_apply<XSrc, Inv>), grid, dim3(kBlock), 0, st, ctl, xs, inv, M, fs,
+                       out);
+    if (!Inv::kInv || !out.anc) return;
+    hipLaunchKernelGGL(k_fold_scan_cnt, dim3(1), dim3(kScanBlock), 0, st, ctl, fs);
+    hipLaunchKernelGGL(k_fold_layout, grid, dim3(kBlock), 0, st, ctl, fs, out, M);
 }
 
 template <typename W>
