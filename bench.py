@@ -98,8 +98,25 @@ def roofline(prec, res, N, W):
     return {"kernel": "k_cv_move<float>" if prec == V.MIXED else
             ("k_cv_move<double>" if prec == V.FP64_FUSED else "k_plain_move"),
             "bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None,
+            "frac": round(gbs / HBM_PEAK_GBS, 4),
+            "traffic": counter_traffic("k_cv_move<float>") if prec == V.MIXED else None,
             "bytes_per_launch": round(tot_b / nl), "avg_launch_ms": round(tot_s * 1e3 / nl, 5)}
+
+
+def counter_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed FETCH_SIZE / WRITE_SIZE
+    counter runs (profiles/hbm_traffic_counters.json, each counter scaled by a
+    1 GiB device copy measured the same way), or None."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic_counters.json")
+    try:
+        with open(path) as fh:
+            d = json.load(fh)
+    except (OSError, ValueError):
+        return None
+    for name, v in d.get("kernels", {}).items():
+        if name.endswith(kernel):
+            return v["traffic_bytes"]
+    return None
 
 
 def cpu_baseline(N, steps, threads):
@@ -130,7 +147,20 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=12)
     ap.add_argument("--cpu-n", type=int, default=1 << 24)
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only run a device-to-device copy of 1 GiB five times (calibrates the "
+                         "FETCH_SIZE / WRITE_SIZE counters against known bytes) and exit")
     a = ap.parse_args()
+    if a.copy_probe:
+        import torch
+        src = torch.empty(1 << 27, dtype=torch.float64, device="cuda:0").fill_(1.0)
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes_per_launch": 2 * src.numel() * 8,
+                          "read_bytes": src.numel() * 8, "write_bytes": src.numel() * 8}))
+        return
     if a.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
               "(it would shard particles with one all-reduce of three FP64 scalars per step); "

@@ -1,0 +1,246 @@
+// vsmcx.hpp -- C++ host layer over the C ABI in vsmcx.h, shaped like vSMC's
+// own host API so that vSMC user code can hand the hot path to the GPU.
+//
+//   vsmcx::Context                RAII over vsmcx_ctx (one HIP stream, HBM budget)
+//   vsmcx::ResampleScheme         vSMC's enum (include/vsmc/resample/common.hpp:124-141)
+//   vsmcx::HipResample            a functor with the signature of
+//                                 Particle<T>::resample_type
+//                                 (include/vsmc/core/particle.hpp:61-62); assign it
+//                                 with Sampler::resample_scheme(resample_type)
+//                                 (include/vsmc/core/sampler.hpp:235-264)
+//   vsmcx::Threefry4x64State      engine state transfer in the text layout of
+//                                 ThreefryEngine's stream operators
+//                                 (include/vsmc/rng/threefry.hpp:424-463)
+//   vsmcx::Filter                 a device-resident bootstrap filter of the pf
+//                                 model: a whole Sampler run (init, iterate,
+//                                 histories) in HBM
+//
+// Header only; link with libvsmcx.so.  Errors throw vsmcx::Error, which
+// carries the C ABI's status code and message (vSMC's runtime asserts throw
+// as well when VSMC_RUNTIME_ASSERT_AS_EXCEPTION is set,
+// include/vsmc/internal/assert.hpp:57-75).
+#ifndef VSMCX_HPP
+#define VSMCX_HPP
+
+#include <cstddef>
+#include <cstdint>
+#include <sstream>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "vsmcx.h"
+
+namespace vsmcx {
+
+class Error : public std::runtime_error {
+  public:
+    Error(int code, const std::string &msg) : std::runtime_error(msg), code_(code) {}
+    int code() const { return code_; }
+
+  private:
+    int code_;
+};
+
+inline void check(int rc)
+{
+    if (rc != VSMCX_OK) throw Error(rc, vsmcx_last_error());
+}
+
+class Context {
+  public:
+    explicit Context(int device = 0, std::uint64_t hbm_budget = 0)
+    {
+        check(vsmcx_ctx_create(device, hbm_budget, &h_));
+    }
+    ~Context()
+    {
+        if (h_) vsmcx_ctx_destroy(h_);
+    }
+    Context(const Context &) = delete;
+    Context &operator=(const Context &) = delete;
+    vsmcx_ctx *get() const { return h_; }
+    void sync() { check(vsmcx_ctx_sync(h_)); }
+    std::uint64_t allocated() const
+    {
+        std::uint64_t b = 0;
+        check(vsmcx_ctx_allocated(h_, &b));
+        return b;
+    }
+
+  private:
+    vsmcx_ctx *h_ = nullptr;
+};
+
+enum ResampleScheme {
+    Multinomial = VSMCX_MULTINOMIAL,
+    Residual = VSMCX_RESIDUAL,
+    Stratified = VSMCX_STRATIFIED,
+    Systematic = VSMCX_SYSTEMATIC,
+    ResidualStratified = VSMCX_RESIDUAL_STRATIFIED,
+    ResidualSystematic = VSMCX_RESIDUAL_SYSTEMATIC
+};
+
+// Threefry4x64 state: counter, key, the 4-word buffer and the index into it.
+// vSMC's engine prints "ctr[4] par[5] buffer[4] index" (par = key words and
+// their parity word), which is how the state crosses to and from a
+// vsmc::Threefry4x64 without touching its private members.
+struct Threefry4x64State : vsmcx_threefry4x64_state {
+    std::uint64_t parity = 0;  // par[4], carried through unchanged
+
+    template <typename Engine>
+    static Threefry4x64State from(const Engine &eng)
+    {
+        std::stringstream ss;
+        ss << eng;
+        Threefry4x64State s;
+        for (auto &w : s.ctr) ss >> w;
+        for (auto &w : s.key) ss >> w;
+        ss >> s.parity;
+        for (auto &w : s.buffer) ss >> w;
+        ss >> s.index;
+        if (!ss) throw Error(VSMCX_ERR_ARG, "cannot read the Threefry4x64 engine state");
+        return s;
+    }
+
+    template <typename Engine>
+    void to(Engine &eng) const
+    {
+        std::stringstream ss;
+        for (auto w : ctr) ss << w << ' ';
+        for (auto w : key) ss << w << ' ';
+        ss << parity << ' ';
+        for (auto w : buffer) ss << w << ' ';
+        // trailing blank: the engine's operator>> only commits the state when
+        // the stream is still good() after the last field
+        ss << index << ' ';
+        ss >> eng;
+        if (!ss) throw Error(VSMCX_ERR_ARG, "cannot write the Threefry4x64 engine state");
+    }
+};
+
+// Drop-in for Resample<Scheme>::operator()(M, N, rng, weight, replication):
+// the replication counts and the advanced engine are the ones vSMC's own
+// resampler produces for the same weights and engine state.
+class HipResample {
+  public:
+    HipResample(Context &ctx, ResampleScheme scheme) : ctx_(&ctx), scheme_(scheme) {}
+
+    template <typename Rng, typename IntType>
+    void operator()(std::size_t M, std::size_t N, Rng &rng, const double *weight,
+                    IntType *replication) const
+    {
+        Threefry4x64State s = Threefry4x64State::from(rng);
+        rep_.resize(M);
+        check(vsmcx_resample(ctx_->get(), (int)scheme_, M, N, &s, weight, rep_.data(), nullptr));
+        for (std::size_t i = 0; i < M; ++i) replication[i] = static_cast<IntType>(rep_[i]);
+        s.to(rng);
+    }
+
+    // Replication and, in the same call, the cfrp_trans layout
+    // (include/vsmc/resample/common.hpp:87-118) for M == N.
+    template <typename Rng, typename IntType>
+    void operator()(std::size_t N, Rng &rng, const double *weight, IntType *replication,
+                    IntType *copy_from) const
+    {
+        Threefry4x64State s = Threefry4x64State::from(rng);
+        rep_.resize(N);
+        cf_.resize(N);
+        check(vsmcx_resample(ctx_->get(), (int)scheme_, N, N, &s, weight, rep_.data(),
+                             cf_.data()));
+        for (std::size_t i = 0; i < N; ++i) {
+            replication[i] = static_cast<IntType>(rep_[i]);
+            copy_from[i] = static_cast<IntType>(cf_[i]);
+        }
+        s.to(rng);
+    }
+
+  private:
+    Context *ctx_;
+    ResampleScheme scheme_;
+    mutable std::vector<std::uint64_t> rep_, cf_;
+};
+
+// Model and run description of the device-resident pf filter; the defaults are
+// the pf example's (example/pf/include/pf_smp.hpp:96-112,150-238).
+struct FilterConfig {
+    std::uint64_t n = 10000;
+    std::uint64_t horizon = 100;
+    int precision = VSMCX_MIXED;
+    ResampleScheme scheme = Systematic;
+    double threshold = 0.5;  // Sampler::resample_threshold (core/sampler.hpp:270-271)
+    std::uint64_t seed = 42, resample_seed = 4242;
+    double dt = 0.1, sd_pos = 0.1, sd_vel = 0.5, prior_sd_pos = 2.0, prior_sd_vel = 1.0;
+    double obs_scale = 0.1, nu = 10.0;
+};
+
+class Filter {
+  public:
+    Filter(Context &ctx, const FilterConfig &c)
+    {
+        vsmcx_filter_desc d{};
+        d.n = c.n;
+        d.horizon = c.horizon;
+        d.precision = c.precision;
+        d.scheme = (int)c.scheme;
+        d.threshold = c.threshold;
+        d.seed = c.seed;
+        d.resample_seed = c.resample_seed;
+        d.dt = c.dt;
+        d.sd_pos = c.sd_pos;
+        d.sd_vel = c.sd_vel;
+        d.prior_sd_pos = c.prior_sd_pos;
+        d.prior_sd_vel = c.prior_sd_vel;
+        d.obs_scale = c.obs_scale;
+        d.nu = c.nu;
+        d.llh_const = llh_const(c.nu, c.obs_scale);
+        check(vsmcx_filter_create(ctx.get(), &d, &h_));
+    }
+    ~Filter()
+    {
+        if (h_) vsmcx_filter_destroy(h_);
+    }
+    Filter(const Filter &) = delete;
+    Filter &operator=(const Filter &) = delete;
+
+    // observations: T rows of (x, y)
+    void observations(const std::vector<double> &xy)
+    {
+        check(vsmcx_filter_set_obs(h_, xy.size() / 2, xy.data()));
+    }
+    // Sampler::initialize + iterate(n - 1) on the first call, iterate(n) after
+    void iterate(std::uint64_t n = 1) { check(vsmcx_filter_run(h_, n)); }
+    void finish() { check(vsmcx_filter_finish(h_)); }
+    std::uint64_t iter_size() const
+    {
+        std::uint64_t t = 0;
+        check(vsmcx_filter_steps(h_, &t));
+        return t;
+    }
+    std::vector<vsmcx_step_stats> history() const
+    {
+        std::vector<vsmcx_step_stats> h(iter_size());
+        check(vsmcx_filter_history(h_, h.size(), h.data()));
+        return h;
+    }
+    vsmcx_filter *get() const { return h_; }
+
+    // Student-t log normaliser per coordinate: lgamma((nu+1)/2) - lgamma(nu/2)
+    // - log(nu * pi) / 2 - log(scale)
+    static double llh_const(double nu, double scale);
+
+  private:
+    vsmcx_filter *h_ = nullptr;
+};
+
+}  // namespace vsmcx
+
+#include <cmath>
+
+inline double vsmcx::Filter::llh_const(double nu, double scale)
+{
+    return std::lgamma((nu + 1) / 2) - std::lgamma(nu / 2) -
+           0.5 * std::log(nu * 3.14159265358979323846) - std::log(scale);
+}
+
+#endif  // VSMCX_HPP

@@ -153,21 +153,33 @@ Tf64State engine_in(const vsmcx_threefry4x64_state &e)
     return s;
 }
 
-void engine_out(const Tf64State &s, vsmcx_threefry4x64_state &e)
+// Engine state back in vSMC's layout.  vSMC's operator() leaves the block of
+// the current counter in the buffer after every draw, also when the index has
+// reached 4 (include/vsmc/rng/threefry.hpp:355-364); with no draw at all the
+// buffer is whatever the caller's engine held (`prev`).
+void engine_out(const Tf64State &s, vsmcx_threefry4x64_state &e,
+                const vsmcx_threefry4x64_state *prev = nullptr)
 {
-    for (int j = 0; j < 4; ++j) {
-        e.ctr[j] = s.ctr[j];
-        e.key[j] = s.key[j];
+    bool drew = true;
+    if (prev) {
+        drew = s.index != (prev->index > 4 ? 4u : (uint32_t)prev->index);
+        for (int j = 0; j < 4; ++j) drew = drew || s.ctr[j] != prev->ctr[j];
     }
-    e.index = s.index;
-    if (s.index < 4) {
+    uint64_t buf[4] = {0, 0, 0, 0};
+    if (!drew) {
+        for (int j = 0; j < 4; ++j) buf[j] = prev->buffer[j];
+    } else if (prev || s.index < 4) {
         u64x4 c;
         for (int j = 0; j < 4; ++j) c.v[j] = s.ctr[j];
         const u64x4 b = threefry4x64_20(c, s.key);
-        for (int j = 0; j < 4; ++j) e.buffer[j] = b.v[j];
-    } else {
-        for (int j = 0; j < 4; ++j) e.buffer[j] = 0;
+        for (int j = 0; j < 4; ++j) buf[j] = b.v[j];
     }
+    for (int j = 0; j < 4; ++j) {
+        e.ctr[j] = s.ctr[j];
+        e.key[j] = s.key[j];
+        e.buffer[j] = buf[j];
+    }
+    e.index = s.index;
 }
 
 void check_dev_err(const FilterCtl &h)
@@ -397,7 +409,8 @@ int vsmcx_resample(vsmcx_ctx *ctx, int scheme, uint64_t M, uint64_t N,
                         copy_from[to++] = from;
             }
         }
-        engine_out(h.eng, *eng);
+        const vsmcx_threefry4x64_state in = *eng;
+        engine_out(h.eng, *eng, &in);
     } catch (...) {
         free_resample(ctx, rb);
         ctx->release(ctl);
@@ -469,7 +482,7 @@ int vsmcx_filter_create(vsmcx_ctx *ctx, const vsmcx_filter_desc *desc, vsmcx_fil
                     desc->prior_sd_vel, desc->obs_scale, desc->nu, desc->llh_const};
     f->N = desc->n;
     f->T = desc->horizon;
-    f->tiles = (f->N + kTile - 1) / kTile;
+    f->tiles = (f->N + kMoveTile - 1) / kMoveTile;
     try {
         f->st[0] = ctx->alloc(f->N * f->state_bytes());
         if (!f->plain) f->st[1] = ctx->alloc(f->N * f->state_bytes());
@@ -626,7 +639,10 @@ int vsmcx_filter_engine(vsmcx_filter *f, vsmcx_threefry4x64_state *eng)
     FilterCtl h;
     VSMCX_HIP(hipMemcpyAsync(&h, f->ctl, sizeof(h), hipMemcpyDeviceToHost, f->ctx->stream));
     f->ctx->sync();
-    engine_out(h.eng, *eng);
+    vsmcx_threefry4x64_state seeded{};  // the engine as the filter seeded it
+    seeded.key[0] = f->d.resample_seed;
+    seeded.index = 4;
+    engine_out(h.eng, *eng, &seeded);
     API_END
 }
 
@@ -759,7 +775,12 @@ int vsmcx_filter_debug_step(vsmcx_filter *f, double *weight, uint64_t *replicati
     if (weight) VSMCX_HIP(hipMemcpyAsync(weight, w, 8 * N, hipMemcpyDeviceToHost, s));
     c->sync();
     c->release(w);
-    if (eng_before) engine_out(h.eng, *eng_before);
+    if (eng_before) {
+        vsmcx_threefry4x64_state seeded{};
+        seeded.key[0] = f->d.resample_seed;
+        seeded.index = 4;
+        engine_out(h.eng, *eng_before, &seeded);
+    }
     if (resampled) *resampled = h.resample;
     f->rb.rep_out = c->alloc_n<uint64_t>(N);
     launch_resample_from_log_weights(f->ctl, f->lw, N, f->d.scheme, f->rb, f->hist, s);

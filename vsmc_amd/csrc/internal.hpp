@@ -16,6 +16,8 @@ namespace vsmcx {
 constexpr int kBlock = 256;                  // threads per workgroup
 constexpr int kItems = 16;                   // particles per thread
 constexpr int kTile = kBlock * kItems;       // particles per workgroup tile
+constexpr int kMoveItems = 4;                // particles per thread in the move kernels
+constexpr int kMoveTile = kBlock * kMoveItems;  // particles per move workgroup (one partial)
 constexpr int kFoldItems = 8;                // elements per thread in the fold kernels
 constexpr int kFoldTile = kBlock * kFoldItems;  // elements per fold tile
 

@@ -131,40 +131,57 @@ __global__ __launch_bounds__(kBlock) void k_cv_move(
     const double m_old = ctl->m, c_old = ctl->coeff;
     const double invN = 1.0 / (double)N;
     const Real ox = (Real)obs[2 * t], oy = (Real)obs[2 * t + 1];
-    const uint64_t base = blockIdx.x * (uint64_t)kTile;
+    const uint64_t base = blockIdx.x * (uint64_t)kMoveTile;
+
+    // Loads first, for all of the thread's particles (ancestor indices, then
+    // the gathered states and old log-weights), so their latencies overlap;
+    // then the arithmetic and the stores.
+    uint64_t src[kMoveItems];
+#pragma unroll
+    for (int k = 0; k < kMoveItems; ++k) {
+        const uint64_t i = base + (uint64_t)k * kBlock + threadIdx.x;
+        uint64_t j = (t > 0 && prev && i < N) ? (uint64_t)anc[i] : i;
+        src[k] = j < N ? j : (i < N ? i : 0);  // never read outside the state buffer
+    }
+    S4 s[kMoveItems];
+    double lwo[kMoveItems];
+#pragma unroll
+    for (int k = 0; k < kMoveItems; ++k) {
+        const uint64_t i = base + (uint64_t)k * kBlock + threadIdx.x;
+        lwo[k] = 0;
+        if (t > 0 && i < N) {
+            s[k] = st_in[src[k]];
+            if (!prev) lwo[k] = lw[i];
+        }
+    }
 
     double m = -INFINITY, se = 0, se2 = 0, mx = 0, my = 0;
-#pragma unroll 4
-    for (int k = 0; k < kItems; ++k) {
+#pragma unroll
+    for (int k = 0; k < kMoveItems; ++k) {
         const uint64_t i = base + (uint64_t)k * kBlock + threadIdx.x;
         if (i >= N) continue;
-        S4 s;
-        double lwo = 0;
+        S4 &sk = s[k];
         if (t > 0) {
-            uint64_t src = prev ? (uint64_t)anc[i] : i;
-            src = src < N ? src : i;  // never read outside the state buffer
-            s = st_in[src];
-            if (!prev) lwo = lw[i];
-            const double w = prev ? invN : exp(lwo - m_old) * c_old;
-            mx += w * (double)s.x;
-            my += w * (double)s.y;
+            const double w = prev ? invN : exp(lwo[k] - m_old) * c_old;
+            mx += w * (double)sk.x;
+            my += w * (double)sk.y;
         }
         Real z[4];
         cv_normals(z, k0, k1, i, t);
         if (t == 0) {
-            s.x = md.psd_pos * z[0];
-            s.y = md.psd_pos * z[1];
-            s.vx = md.psd_vel * z[2];
-            s.vy = md.psd_vel * z[3];
+            sk.x = md.psd_pos * z[0];
+            sk.y = md.psd_pos * z[1];
+            sk.vx = md.psd_vel * z[2];
+            sk.vy = md.psd_vel * z[3];
         } else {
-            s.x += md.sd_pos * z[0] + md.dt * s.vx;
-            s.y += md.sd_pos * z[1] + md.dt * s.vy;
-            s.vx += md.sd_vel * z[2];
-            s.vy += md.sd_vel * z[3];
+            sk.x += md.sd_pos * z[0] + md.dt * sk.vx;
+            sk.y += md.sd_pos * z[1] + md.dt * sk.vy;
+            sk.vx += md.sd_vel * z[2];
+            sk.vy += md.sd_vel * z[3];
         }
-        const Real inc = md.llh(s.x, s.y, ox, oy);
-        const double lwn = lwo + (double)inc;
-        st_out[i] = s;
+        const Real inc = md.llh(sk.x, sk.y, ox, oy);
+        const double lwn = lwo[k] + (double)inc;
+        st_out[i] = sk;
         lw[i] = lwn;
         if (lwn > m) {
             const double r = exp(m - lwn);
@@ -262,9 +279,9 @@ __global__ __launch_bounds__(kBlock) void k_cv_monitor(FilterCtl *ctl,
     __shared__ Part5 s_p5[kBlock / kWave];
     const int prev = ctl->prev_resampled;
     const double m = ctl->m, c = ctl->coeff, invN = 1.0 / (double)N;
-    const uint64_t base = blockIdx.x * (uint64_t)kTile;
+    const uint64_t base = blockIdx.x * (uint64_t)kMoveTile;
     Part5 v{0, 0, 0, 0};
-    for (int k = 0; k < kItems; ++k) {
+    for (int k = 0; k < kMoveItems; ++k) {
         const uint64_t i = base + (uint64_t)k * kBlock + threadIdx.x;
         if (i >= N) continue;
         const auto s = st[prev ? (uint64_t)anc[i] : i];
@@ -314,7 +331,7 @@ __global__ void k_cv_gather(const FilterCtl *ctl, const S4 *in, S4 *out, const u
         out[i] = in[prev ? (uint64_t)anc[i] : i];
 }
 
-static unsigned tiles_of(uint64_t N) { return (unsigned)((N + kTile - 1) / kTile); }
+static unsigned tiles_of(uint64_t N) { return (unsigned)((N + kMoveTile - 1) / kMoveTile); }
 
 void launch_cv_move(bool fp64, FilterCtl *ctl, const void *st_in, void *st_out,
                     const uint32_t *anc, double *lw, const double *obs, const CvParams &p,

@@ -274,8 +274,14 @@ __device__ __forceinline__ double thread_sum(const double *s_x, int e0)
 // Exclusive scan of n per-tile values by one workgroup of kScanBlock threads,
 // each taking kScanItems consecutive tiles per round; fixed order.  The loads
 // of a round are issued together (registers) so their latencies overlap.
-template <typename T, typename Op>
-__device__ T wg_scan(const T *in, T *out, uint64_t n, Op op, T ident, T *lds)
+struct NoVisit {
+    template <typename T>
+    __device__ void operator()(uint64_t, const T &) const {}
+};
+
+template <typename T, typename Op, typename Visit = NoVisit>
+__device__ T wg_scan(const T *in, T *out, uint64_t n, Op op, T ident, T *lds,
+                     Visit visit = Visit())
 {
     T carry = ident;
     for (uint64_t c0 = 0; c0 < n; c0 += (uint64_t)kScanBlock * kScanItems) {
@@ -290,7 +296,10 @@ __device__ T wg_scan(const T *in, T *out, uint64_t n, Op op, T ident, T *lds)
         T run = op(carry, block_excl_scan<kScanBlock>(loc, op, ident, lds, &tot));
 #pragma unroll
         for (int k = 0; k < kScanItems; ++k) {
-            if (i0 + k < n) out[i0 + k] = run;
+            if (i0 + k < n) {
+                out[i0 + k] = run;
+                visit(i0 + k, run);
+            }
             run = op(run, v[k]);
         }
         carry = op(carry, tot);
@@ -350,9 +359,11 @@ __global__ __launch_bounds__(kBlock) void k_fold_seg(FilterCtl *ctl, XSrc xs, ui
     const auto xat = [&](int k) { return s_x[fidx(e0 + k)]; };
     int fast_b;
     const Seg g = run_seg(xat, kFI, s_thr, er, ea, &fast_b);
-    Seg tile;
-    const Seg thr_pre = block_excl_scan<kBlock>(g, SegOp(), seg_identity(), s_seg, &tile);
-    if (tid == 0) fs.agg[t] = tile;
+    const Seg thr_pre =
+        block_excl_scan<kBlock>(g, SegOp(), seg_identity(), s_seg, (Seg *)nullptr);
+    // tile total from the last thread (a struct total through a pointer would
+    // keep the struct in scratch memory)
+    if (tid == kBlock - 1) fs.agg[t] = seg_op(thr_pre, g);
     if (g.cnt) {
         Seg pre = thr_pre;  // tile-local
         double s = s_thr;
@@ -363,10 +374,12 @@ __global__ __launch_bounds__(kBlock) void k_fold_seg(FilterCtl *ctl, XSrc xs, ui
             s = sb + x;
             const Seg el = elem_seg(x, sb, s, er, ea);
             if (el.cnt) {
-                SpecRec r;
-                r.run = pre.tr;
-                r.x = x;
-                fs.spec[base + pre.cnt] = r;
+                SpecRec *r = fs.spec + base + pre.cnt;
+                r->run.d0 = pre.tr.d0;
+                r->run.d1 = pre.tr.d1;
+                r->run.pp = pre.tr.pp;
+                r->run.b = pre.tr.b;
+                r->x = x;
             }
             pre = seg_op(pre, el);
         }
@@ -390,13 +403,13 @@ __global__ __launch_bounds__(kScanBlock) void k_fold_scan_seg(FilterCtl *ctl, Fo
     __shared__ double s_v[kScanBlock];
     const int tid = threadIdx.x;
     const uint64_t T = fs.tiles;
-    const Seg tail = wg_scan<Seg>(fs.agg, fs.pre, T, SegOp(), seg_identity(), s_seg);
-    __syncthreads();  // pre[] written by other threads
-    const uint64_t S = tail.cnt;
     const bool in_lds = T <= (uint64_t)kLdsTiles;
-    if (in_lds)
-        for (uint64_t t = tid; t < T; t += kScanBlock) s_cnt[t] = (uint32_t)fs.pre[t].cnt;
-    __syncthreads();
+    const Seg tail = wg_scan<Seg>(fs.agg, fs.pre, T, SegOp(), seg_identity(), s_seg,
+                                  [&](uint64_t t, const Seg &p) {
+                                      if (in_lds) s_cnt[t] = (uint32_t)p.cnt;
+                                  });
+    __syncthreads();  // pre[] and s_cnt written by other threads
+    const uint64_t S = tail.cnt;
     double a = 0.0;  // running exact value (thread 0)
     int bad = 0;
     for (uint64_t c0 = 0; c0 < S; c0 += kScanBlock) {
@@ -458,8 +471,11 @@ __global__ __launch_bounds__(kBlock) void k_fold_apply(FilterCtl *ctl, XSrc xs, 
                                                        uint64_t M, FoldScratch fs, FoldOut out)
 {
     if (!ctl->resample) return;
-    __shared__ double s_x[kFPad];  // x; later the uniform window
-    __shared__ double s_a[kFPad];  // exact fold values
+    // x, overwritten in place (each thread its own elements) by the exact
+    // values; later the uniform window and the staged replication counts
+    __shared__ double s_x[kFPad];
+    __shared__ double s_tmp[Inv::kWindow ? kFPad : 1];  // window generation
+    double *const s_a = s_x;
     __shared__ double s_red[kBlock / kWave];
     __shared__ Seg s_seg[kBlock / kWave];
     __shared__ uint64_t s_u64[kBlock / kWave];
@@ -550,7 +566,7 @@ __global__ __launch_bounds__(kBlock) void k_fold_apply(FilterCtl *ctl, XSrc xs, 
             for (uint64_t w0 = wlo; w0 < whi; w0 += kFT) {
                 const int len = (int)((whi - w0) < (uint64_t)kFT ? (whi - w0) : (uint64_t)kFT);
                 __syncthreads();
-                inv.fill_window(s_x, s_a, w0, len);  // s_a is free once av[] is loaded
+                inv.fill_window(s_x, s_tmp, w0, len);
                 __syncthreads();
                 const double first = s_x[0], lastv = s_x[len - 1];
                 // the thread's values are sorted: gallop from the previous
