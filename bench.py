@@ -119,6 +119,37 @@ def counter_traffic(kernel):
     return None
 
 
+def run_batched(nfilters, seed, tune=True):
+    """Config-3 workload: nfilters independent CV filters, particle counts
+    log-uniform in [2^8, 2^16], horizons uniform in [50, 500], each with its
+    own observations; all of them advanced by one persistent launch."""
+    rng = np.random.default_rng(seed)
+    ns = np.exp(rng.uniform(np.log(2 ** 8), np.log(2 ** 16), nfilters)).astype(np.uint64)
+    Ts = rng.integers(50, 501, nfilters).astype(np.uint64)
+    obs = V.cv_simulate_many(Ts, seed + 1)
+    ctx = V.Context(0)
+    b = V.Batch(ctx, ns, Ts, 100000 + np.arange(nfilters, dtype=np.uint64),
+                200000 + np.arange(nfilters, dtype=np.uint64), precision=V.MIXED,
+                scheme=V.SYSTEMATIC)
+    for f in range(nfilters):
+        b.set_obs(f, obs[f])
+    shape = b.tune(1) if tune else (0, 0)
+    done = np.array([b.steps(f) for f in range(nfilters)], np.uint64)
+    ms = b.run()
+    updates = float(np.sum(ns * (Ts - done)))
+    rfrac = float(np.mean([b.history(f)["resampled"].mean() for f in range(0, nfilters, 64)]))
+    b.close()
+    ctx.close()
+    return {"value": updates / (ms * 1e-3), "unit": "particle-updates/s", "ms": ms,
+            "filters": nfilters, "particles": int(ns.sum()), "particle_updates": updates,
+            "launch": {"workgroups": shape[0], "order": "most work first" if shape[1] == 0
+                       else "as given", "tuned": tune},
+            "resample_frac_sampled": rfrac,
+            "config": "config-3: CV filters, N log-uniform [2^8, 2^16], T uniform [50, 500], "
+                      "mixed precision, systematic, ESS<N/2, one persistent launch for all "
+                      "remaining steps (tuner steps excluded)"}
+
+
 def cpu_baseline(N, steps, threads):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     from oracle_ctypes import Oracle
@@ -147,6 +178,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=12)
     ap.add_argument("--cpu-n", type=int, default=1 << 24)
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--batch-filters", type=int, default=4096)
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a device-to-device copy of 1 GiB five times (calibrates the "
                          "FETCH_SIZE / WRITE_SIZE counters against known bytes) and exit")
@@ -218,6 +251,9 @@ def main():
         out["speedup_mixed_vs_fp64_plain"] = value / out["fp64_plain"]["value"]
     if V.FP64_FUSED in res:
         out["speedup_mixed_vs_fp64_fused"] = value / out["fp64_fused"]["value"]
+    if not a.no_batch:
+        print(f"[bench] batched: {a.batch_filters} filters", file=sys.stderr, flush=True)
+        out["batched"] = run_batched(a.batch_filters, 13)
     if not a.no_cpu:
         threads = min(16, os.cpu_count() or 1)
         print(f"[bench] cpu baseline: N={a.cpu_n}, {a.cpu_steps} steps, {threads} threads",

@@ -175,6 +175,49 @@ int vsmcx_filter_debug_step(vsmcx_filter *f, double *weight, uint64_t *replicati
                             uint64_t *copy_from, double *uniforms, uint64_t *nn,
                             vsmcx_threefry4x64_state *eng_before, int *resampled);
 
+/* ---- batched filters ----------------------------------------------------------
+ * Thousands of independent pf filters (each the vsmcx_filter model with its own
+ * particle count, horizon, observations and seeds) advanced together: one
+ * persistent kernel launch per vsmcx_batch_run, workgroups taking filters
+ * from a queue and running whole steps of one filter at a time.  A filter's
+ * results do not depend on the other filters in the batch or on the launch
+ * shape.  No single counterpart in vSMC: it replaces one Sampler per filter
+ * (include/vsmc/core/sampler.hpp:450-483, 773-834), each stepped in turn. */
+typedef struct vsmcx_batch vsmcx_batch;
+
+typedef struct {
+    uint64_t nfilters;
+    const uint64_t *n;             /* [nfilters] particles, 1 .. 2^18 */
+    const uint64_t *horizon;       /* [nfilters] steps */
+    const uint64_t *seed;          /* [nfilters] Philox4x32 keys of the move streams */
+    const uint64_t *resample_seed; /* [nfilters] Threefry4x64 seeds of the resample engines */
+    int precision;                 /* VSMCX_MIXED or VSMCX_FP64_FUSED */
+    int scheme;                    /* VSMCX_SYSTEMATIC or VSMCX_STRATIFIED */
+    double threshold;              /* resample when ESS < threshold * n */
+    /* model, shared by all filters (vsmcx_filter_desc's fields) */
+    double dt, sd_pos, sd_vel, prior_sd_pos, prior_sd_vel;
+    double obs_scale, nu, llh_const;
+} vsmcx_batch_desc;
+
+int vsmcx_batch_create(vsmcx_ctx *ctx, const vsmcx_batch_desc *desc, vsmcx_batch **out);
+int vsmcx_batch_destroy(vsmcx_batch *b);
+/* observation rows [first, first + count) of filter f, (x, y) pairs */
+int vsmcx_batch_set_obs(vsmcx_batch *b, uint64_t f, uint64_t first, uint64_t count,
+                        const double *obs);
+/* One launch: every filter advances by up to max_steps steps, as far as its
+ * observations allow.  elapsed_ms (may be NULL) receives the device time. */
+int vsmcx_batch_run(vsmcx_batch *b, uint64_t max_steps, double *elapsed_ms);
+/* Launch shape: workgroups of the persistent grid (0 = default) and the order
+ * filters are taken in (0 = most work first, 1 = as given). */
+int vsmcx_batch_set_launch(vsmcx_batch *b, uint32_t workgroups, int order);
+/* Autotuner: runs `steps` real steps under each candidate launch shape and
+ * keeps the fastest (the results do not depend on the shape). */
+int vsmcx_batch_tune(vsmcx_batch *b, uint64_t steps, uint32_t *workgroups, int *order);
+int vsmcx_batch_steps(vsmcx_batch *b, uint64_t f, uint64_t *t);
+int vsmcx_batch_history(vsmcx_batch *b, uint64_t f, uint64_t count, vsmcx_step_stats *out);
+int vsmcx_batch_state(vsmcx_batch *b, uint64_t f, double *state, double *log_weight);
+int vsmcx_batch_particles(vsmcx_batch *b, uint64_t *total);
+
 #ifdef __cplusplus
 }
 #endif

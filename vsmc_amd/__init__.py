@@ -53,6 +53,15 @@ class FilterDesc(C.Structure):
                 ("obs_scale", C.c_double), ("nu", C.c_double), ("llh_const", C.c_double)]
 
 
+class BatchDesc(C.Structure):
+    _fields_ = [("nfilters", C.c_uint64), ("n", C.c_void_p), ("horizon", C.c_void_p),
+                ("seed", C.c_void_p), ("resample_seed", C.c_void_p), ("precision", C.c_int),
+                ("scheme", C.c_int), ("threshold", C.c_double),
+                ("dt", C.c_double), ("sd_pos", C.c_double), ("sd_vel", C.c_double),
+                ("prior_sd_pos", C.c_double), ("prior_sd_vel", C.c_double),
+                ("obs_scale", C.c_double), ("nu", C.c_double), ("llh_const", C.c_double)]
+
+
 STEP_DTYPE = np.dtype([("ess", "f8"), ("log_z", "f8"), ("mean_x", "f8"), ("mean_y", "f8"),
                        ("resampled", "i4"), ("pad", "i4"), ("draws", "u8")])
 
@@ -99,6 +108,16 @@ def lib():
         "vsmcx_timer_destroy": ([vp], i32),
         "vsmcx_filter_profile": ([vp, i32], i32),
         "vsmcx_filter_move_times": ([vp, u64, P(u64), vp], i32),
+        "vsmcx_batch_create": ([vp, P(BatchDesc), P(vp)], i32),
+        "vsmcx_batch_destroy": ([vp], i32),
+        "vsmcx_batch_set_obs": ([vp, u64, u64, u64, vp], i32),
+        "vsmcx_batch_run": ([vp, u64, P(f64)], i32),
+        "vsmcx_batch_set_launch": ([vp, C.c_uint32, i32], i32),
+        "vsmcx_batch_tune": ([vp, u64, P(C.c_uint32), P(i32)], i32),
+        "vsmcx_batch_steps": ([vp, u64, P(u64)], i32),
+        "vsmcx_batch_history": ([vp, u64, u64, vp], i32),
+        "vsmcx_batch_state": ([vp, u64, vp, vp], i32),
+        "vsmcx_batch_particles": ([vp, P(u64)], i32),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)
@@ -335,3 +354,98 @@ def cv_simulate(T, seed, dt=0.1, sd_pos=0.1, sd_vel=0.5, prior_sd_pos=2.0, prior
             x[3] += sd_vel * z[3]
         obs[t] = x[:2] + obs_scale * g.standard_t(nu, 2)
     return obs
+
+
+def cv_simulate_many(horizons, seed, dt=0.1, sd_pos=0.1, sd_vel=0.5, prior_sd_pos=2.0,
+                     prior_sd_vel=1.0, obs_scale=0.1, nu=10.0):
+    """Observations of many independent CV trajectories (one per horizon),
+    simulated together from one seed; returns a list of (T_f, 2) arrays."""
+    Ts = np.asarray(horizons, np.int64)
+    k, tmax = Ts.size, int(Ts.max())
+    g = np.random.default_rng(seed)
+    x = np.empty((k, 4))
+    x[:, :2] = prior_sd_pos * g.standard_normal((k, 2))
+    x[:, 2:] = prior_sd_vel * g.standard_normal((k, 2))
+    obs = np.zeros((tmax, k, 2))
+    for t in range(tmax):
+        if t > 0:
+            z = g.standard_normal((k, 4))
+            x[:, 0] += sd_pos * z[:, 0] + dt * x[:, 2]
+            x[:, 1] += sd_pos * z[:, 1] + dt * x[:, 3]
+            x[:, 2] += sd_vel * z[:, 2]
+            x[:, 3] += sd_vel * z[:, 3]
+        obs[t] = x[:, :2] + obs_scale * g.standard_t(nu, (k, 2))
+    return [np.ascontiguousarray(obs[:Ts[f], f]) for f in range(k)]
+
+
+class Batch:
+    """Many independent CV filters advanced by one persistent launch (vsmcx_batch)."""
+
+    def __init__(self, ctx, n, horizon, seed, resample_seed, precision=MIXED, scheme=SYSTEMATIC,
+                 threshold=0.5, dt=0.1, sd_pos=0.1, sd_vel=0.5, prior_sd_pos=2.0,
+                 prior_sd_vel=1.0, obs_scale=0.1, nu=10.0):
+        self.ctx = ctx
+        self.n = np.ascontiguousarray(n, np.uint64)
+        self.horizon = np.ascontiguousarray(horizon, np.uint64)
+        self.seed = np.ascontiguousarray(seed, np.uint64)
+        self.resample_seed = np.ascontiguousarray(resample_seed, np.uint64)
+        d = BatchDesc(nfilters=self.n.size, n=self.n.ctypes.data,
+                      horizon=self.horizon.ctypes.data, seed=self.seed.ctypes.data,
+                      resample_seed=self.resample_seed.ctypes.data, precision=precision,
+                      scheme=scheme, threshold=threshold, dt=dt, sd_pos=sd_pos, sd_vel=sd_vel,
+                      prior_sd_pos=prior_sd_pos, prior_sd_vel=prior_sd_vel, obs_scale=obs_scale,
+                      nu=nu, llh_const=cv_llh_const(nu, obs_scale))
+        self.h = C.c_void_p()
+        _check(lib().vsmcx_batch_create(ctx.h, C.byref(d), C.byref(self.h)))
+
+    def close(self):
+        if self.h:
+            lib().vsmcx_batch_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_obs(self, f, obs, first=0):
+        o = np.ascontiguousarray(obs, np.float64).reshape(-1, 2)
+        _check(lib().vsmcx_batch_set_obs(self.h, f, first, o.shape[0], o.ctypes.data))
+
+    def run(self, max_steps=1 << 62):
+        ms = C.c_double()
+        _check(lib().vsmcx_batch_run(self.h, max_steps, C.byref(ms)))
+        return ms.value
+
+    def set_launch(self, workgroups=0, order=0):
+        _check(lib().vsmcx_batch_set_launch(self.h, workgroups, order))
+
+    def tune(self, steps=1):
+        wg, order = C.c_uint32(), C.c_int()
+        _check(lib().vsmcx_batch_tune(self.h, steps, C.byref(wg), C.byref(order)))
+        return wg.value, order.value
+
+    def steps(self, f):
+        v = C.c_uint64()
+        _check(lib().vsmcx_batch_steps(self.h, f, C.byref(v)))
+        return v.value
+
+    def history(self, f, count=None):
+        if count is None:
+            count = self.steps(f)
+        out = np.zeros(count, STEP_DTYPE)
+        _check(lib().vsmcx_batch_history(self.h, f, count, out.ctypes.data))
+        return out
+
+    def state(self, f):
+        n = int(self.n[f])
+        st = np.zeros((n, 4))
+        lw = np.zeros(n)
+        _check(lib().vsmcx_batch_state(self.h, f, st.ctypes.data, lw.ctypes.data))
+        return st, lw
+
+    def particles(self):
+        v = C.c_uint64()
+        _check(lib().vsmcx_batch_particles(self.h, C.byref(v)))
+        return v.value

@@ -1,0 +1,204 @@
+// cv_model.hpp -- the constant-velocity particle model of vSMC's pf example
+// (example/pf/include/pf_smp.hpp:96-112, 150-238): state layout, per-particle
+// normals and the Student-t log-likelihood, shared by the single-filter and
+// the batched kernels.
+#pragma once
+#include <math.h>
+
+#include "device_util.hpp"
+#include "internal.hpp"
+#include "rng_core.hpp"
+
+namespace vsmcx {
+
+struct alignas(16) S4f { float x, y, vx, vy; };
+struct alignas(32) S4d { double x, y, vx, vy; };
+
+template <typename Real> struct StateOf;
+template <> struct StateOf<float> { typedef S4f type; };
+template <> struct StateOf<double> { typedef S4d type; };
+
+// four N(0,1) from one Philox4x32-10 block, counter {t+1, id_lo, id_hi, 0}
+__device__ __forceinline__ void cv_normals(float z[4], uint32_t k0, uint32_t k1, uint64_t id,
+                                           uint64_t t)
+{
+    u32x4 c;
+    c.v[0] = (uint32_t)(t + 1);
+    c.v[1] = (uint32_t)id;
+    c.v[2] = (uint32_t)(id >> 32);
+    c.v[3] = 0;
+    const u32x4 w = philox4x32_10(c, k0, k1);
+    // hardware transcendentals: v_log_f32 (log2), v_sqrt_f32, and v_sin_f32 /
+    // v_cos_f32, which take the angle in revolutions (sin(2 pi u2) = sin_rev(u2))
+    const float ln2 = 0.69314718f;
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+        const float u1 = u01_oc_24(w.v[2 * j]);
+        const float u2 = u01_oc_24(w.v[2 * j + 1]);
+        const float r = __builtin_amdgcn_sqrtf(-2.0f * ln2 * __builtin_amdgcn_logf(u1));
+        z[2 * j] = r * __builtin_amdgcn_cosf(u2);
+        z[2 * j + 1] = r * __builtin_amdgcn_sinf(u2);
+    }
+}
+
+__device__ __forceinline__ void cv_normals(double z[4], uint32_t k0, uint32_t k1, uint64_t id,
+                                           uint64_t t)
+{
+    u32x4 c;
+    c.v[0] = (uint32_t)(t + 1);
+    c.v[1] = (uint32_t)id;
+    c.v[2] = (uint32_t)(id >> 32);
+    c.v[3] = 0;
+    const u32x4 w = philox4x32_10(c, k0, k1);
+    const double two_pi = 6.2831853071795865;
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+        const double u1 = u01_oc_53(w.v[2 * j]);
+        const double u2 = u01_oc_53(w.v[2 * j + 1]);
+        const double r = sqrt(-2 * log(u1));
+        z[2 * j] = r * cos(two_pi * u2);
+        z[2 * j + 1] = r * sin(two_pi * u2);
+    }
+}
+
+
+template <typename Real>
+struct CvModel {
+    Real dt, sd_pos, sd_vel, psd_pos, psd_vel, scale, nu, c2, half_nu1;
+    Real inv_scale, inv_nu;
+    __device__ void load(const CvParams &p)
+    {
+        dt = (Real)p.dt;
+        sd_pos = (Real)p.sd_pos;
+        sd_vel = (Real)p.sd_vel;
+        psd_pos = (Real)p.prior_sd_pos;
+        psd_vel = (Real)p.prior_sd_vel;
+        scale = (Real)p.obs_scale;
+        nu = (Real)p.nu;
+        c2 = (Real)(2 * p.llh_const);
+        half_nu1 = (Real)(0.5 * (p.nu + 1));
+        inv_scale = (Real)(1 / p.obs_scale);
+        inv_nu = (Real)(1 / p.nu);
+    }
+    // Student-t log-density of the observation, both coordinates
+    // (example/pf/include/pf_smp.hpp:96-112, log1p form of example/pf/pf_cl.cl)
+    __device__ Real llh(Real x, Real y, Real ox, Real oy) const;
+};
+
+// FP64: the reference's arithmetic (divisions, log1p)
+template <>
+__device__ __forceinline__ double CvModel<double>::llh(double x, double y, double ox,
+                                                       double oy) const
+{
+    const double dx = (x - ox) / scale;
+    const double dy = (y - oy) / scale;
+    const double lx = log1p(dx * dx / nu);
+    const double ly = log1p(dy * dy / nu);
+    return c2 - half_nu1 * (lx + ly);
+}
+
+// FP32: reciprocal multiplies and the hardware log (v_log_f32 is log2)
+template <>
+__device__ __forceinline__ float CvModel<float>::llh(float x, float y, float ox, float oy) const
+{
+    const float ln2 = 0.69314718f;
+    const float dx = (x - ox) * inv_scale;
+    const float dy = (y - oy) * inv_scale;
+    const float l2 = __builtin_amdgcn_logf((1.0f + dx * dx * inv_nu) * (1.0f + dy * dy * inv_nu));
+    return c2 - half_nu1 * ln2 * l2;
+}
+
+
+// x += sd_pos z0 + dt vx (old velocity), v += sd_vel z; the prior at t = 0
+// (example/pf/include/pf_smp.hpp:150-164, 207-238)
+template <typename Real, typename S4>
+__device__ __forceinline__ void cv_advance(const CvModel<Real> &md, S4 &s, const Real z[4],
+                                           uint64_t t)
+{
+    if (t == 0) {
+        s.x = md.psd_pos * z[0];
+        s.y = md.psd_pos * z[1];
+        s.vx = md.psd_vel * z[2];
+        s.vy = md.psd_vel * z[3];
+    } else {
+        s.x += md.sd_pos * z[0] + md.dt * s.vx;
+        s.y += md.sd_pos * z[1] + md.dt * s.vy;
+        s.vx += md.sd_vel * z[2];
+        s.vy += md.sd_vel * z[3];
+    }
+}
+
+// Per-workgroup sums of a step: e^(lw-M), e^(2(lw-M)), and the monitor sums.
+struct Part5 { double se, se2, mx, my; };
+__device__ __forceinline__ Part5 lane_xor(const Part5 &v, int m)
+{
+    return Part5{lane_xor(v.se, m), lane_xor(v.se2, m), lane_xor(v.mx, m), lane_xor(v.my, m)};
+}
+struct Part5Add {
+    __device__ Part5 operator()(const Part5 &a, const Part5 &b) const
+    {
+        return Part5{a.se + b.se, a.se2 + b.se2, a.mx + b.mx, a.my + b.my};
+    }
+};
+
+// Online log-sum-exp of FP64 log-weights: running max m, sum e^(lw-m) and
+// sum e^(2(lw-m)), one exp per value.
+__device__ __forceinline__ void lse_add(double &m, double &se, double &se2, double lw)
+{
+    if (lw > m) {
+        const double r = exp(m - lw);
+        se = se * r + 1;
+        se2 = se2 * r * r + 1;
+        m = lw;
+    } else {
+        const double e = exp(lw - m);
+        se += e;
+        se2 += e * e;
+    }
+}
+
+// After a step's reductions (one thread): log-sum-exp, ESS, log-evidence
+// increment LSE_new - LSE_old (LSE_old = log N after a resample or at the
+// start), the resampling decision (ess < threshold * N, core/particle.hpp:232)
+// and the history row; the previous step's MonitorMCMC mean goes to row t-1.
+__device__ __forceinline__ void cv_step_scalars(FilterCtl *ctl, double M, double se, double se2,
+                                                double mx, double my, StepRow *hist, uint64_t t)
+{
+    const double n = (double)ctl->n;
+    const double lse_old = (t == 0 || ctl->prev_resampled) ? log(n) : ctl->lse;
+    const double lse = M + log(se);
+    const double logz = (t == 0 ? 0.0 : ctl->logz) + (lse - lse_old);
+    const double ess = se * se / se2;
+    int res = ess < ctl->threshold * n;
+    if (!(se > 0) || !isfinite(lse) || !isfinite(ess)) {
+        atomicOr(&ctl->err, (int)kErrDegenerate);
+        res = 0;
+    }
+    ctl->m = M;
+    ctl->coeff = 1 / se;
+    ctl->lse = lse;
+    ctl->logz = logz;
+    ctl->ess = ess;
+    ctl->resample = res;
+    ctl->t = t;
+    ctl->prev_resampled = 0;
+    ctl->r_int = 0;
+    ctl->draws = 0;
+    if (hist) {
+        StepRow r;
+        r.ess = ess;
+        r.logz = logz;
+        r.mean_x = 0;
+        r.mean_y = 0;
+        r.resampled = res;
+        r.pad = 0;
+        r.draws = 0;
+        hist[t] = r;
+        if (t > 0) {
+            hist[t - 1].mean_x = mx;
+            hist[t - 1].mean_y = my;
+        }
+    }
+}
+
+}  // namespace vsmcx

@@ -1,0 +1,745 @@
+// fold_tiles.hpp -- the per-tile bodies of exact-fold resampling.
+//
+// vSMC's inversion (include/vsmc/resample/common.hpp:57-84) and cfrp_trans
+// (include/vsmc/resample/common.hpp:87-118) as work on 2048-element tiles:
+// tile sums, classification and segment summaries, the chain over special
+// elements, exact values with inversion and replication, and the keep-in-place
+// layout.  kernels_fold.hip runs them as grid passes over the tiles of one
+// large particle system; kernels_batch.hip runs them tile after tile inside
+// one workgroup per filter.  Both run the same code on the same data, so a
+// filter's ancestors do not depend on which path resamples it.
+#pragma once
+#include "device_util.hpp"
+#include "fold_core.hpp"
+#include "internal.hpp"
+
+namespace vsmcx {
+
+// ------------------------------------------------------------------------
+// x sources: the sequence being folded
+// ------------------------------------------------------------------------
+struct XArray {  // given weights (compat entry, plain FP64 path)
+    const double *w;
+    __device__ void bind(const FilterCtl *) {}
+    __device__ double x(uint64_t i) const { return w[i]; }
+    __device__ uint64_t ipart(uint64_t) const { return 0; }
+};
+
+struct XWeight {  // W_i = exp(lw_i - m) * coeff, computed from the log-weights
+    const double *lw;
+    double m, coeff;
+    __device__ void bind(const FilterCtl *c) { m = c->m; coeff = c->coeff; }
+    __device__ double x(uint64_t i) const { return exp(lw[i] - m) * coeff; }
+    __device__ uint64_t ipart(uint64_t) const { return 0; }
+};
+
+// residual.hpp:57-61: r_i = modf(N * W_i, &i_i)
+template <typename W>
+struct XResid {
+    W w;
+    double nd;
+    __device__ void bind(const FilterCtl *c) { w.bind(c); nd = (double)c->n; }
+    __device__ double x(uint64_t i) const
+    {
+        double ip;
+        return modf(nd * w.x(i), &ip);
+    }
+    __device__ uint64_t ipart(uint64_t i) const
+    {
+        double ip;
+        modf(nd * w.x(i), &ip);
+        return (uint64_t)ip;
+    }
+};
+
+// residual.hpp:62-63: r *= 1 / asum(r)
+template <typename W>
+struct XResidScaled {
+    XResid<W> r;
+    double rc;
+    __device__ void bind(const FilterCtl *c) { r.bind(c); rc = c->rcoeff; }
+    __device__ double x(uint64_t i) const { return r.x(i) * rc; }
+    __device__ uint64_t ipart(uint64_t i) const { return r.ipart(i); }
+};
+
+// ------------------------------------------------------------------------
+// uniform sources: F(c) = #{n < nn : v_n <= c} for the sorted sequence v
+// ------------------------------------------------------------------------
+struct InvNone {
+    static constexpr bool kInv = false;
+    static constexpr bool kWindow = false;
+    __device__ void bind(const FilterCtl *) {}
+};
+
+// U01SequenceSystematic (include/vsmc/rng/u01.hpp:253-287): u_n = u0 + n*delta
+struct InvSystematic {
+    static constexpr bool kInv = true;
+    static constexpr bool kWindow = false;
+    double u0, delta;
+    uint64_t nn;
+    __device__ void bind(const FilterCtl *c) { u0 = c->u0; delta = c->delta; nn = c->nn; }
+    __device__ double v(uint64_t n) const { return u0 + (double)n * delta; }
+    __device__ uint64_t F(double c) const
+    {
+        if (nn == 0) return 0;
+        const double est = (c - u0) * (double)nn;
+        uint64_t n = !(est > 0) ? 0 : (est >= (double)nn ? nn : (uint64_t)est);
+        while (n < nn && v(n) <= c) ++n;
+        while (n > 0 && v(n - 1) > c) --n;
+        return n;
+    }
+};
+
+// u_n of the stratified sequence, kept out of line (20 Threefry rounds)
+__device__ __noinline__ double strat_u(const Tf64State &eng, double delta, uint64_t n)
+{
+    return canonical64(tf64_stream_word(eng, n)) * delta + (double)n * delta;
+}
+
+// U01SequenceStratified (u01.hpp:212-243): u_n = U_n*delta + n*delta, with
+// U_n the n-th draw of the resample engine; v_n = max(u_n, u_{n-1}) gives
+// inversion's running-max semantics (u_{n-2} < u_n always).
+struct InvStratified {
+    static constexpr bool kInv = true;
+    static constexpr bool kWindow = true;
+    Tf64State eng;
+    double delta;
+    uint64_t nn;
+    __device__ void bind(const FilterCtl *c) { eng = c->eng; delta = c->delta; nn = c->nn; }
+    __device__ double u(uint64_t n) const { return strat_u(eng, delta, n); }
+    __device__ double v(uint64_t n) const
+    {
+        const double a = u(n);
+        if (n == 0) return a;
+        const double b = u(n - 1);
+        return a < b ? b : a;
+    }
+    // v(w0 .. w0+len-1) into s_v (len <= kFoldTile); tmp holds len + 1 doubles.
+    // u_n are generated a Threefry block (four consecutive draws) per thread.
+    __device__ void fill_window(double *s_v, double *tmp, uint64_t w0, int len) const
+    {
+        const uint64_t nlo = w0 ? w0 - 1 : 0, nhi = w0 + (uint64_t)len;
+        const uint64_t p0 = nlo + eng.index, p1 = nhi + eng.index;  // stream positions
+        u64x4 c0;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) c0.v[j] = eng.ctr[j];
+        for (uint64_t bk = p0 / 4 + threadIdx.x; bk <= (p1 - 1) / 4; bk += blockDim.x) {
+            const u64x4 r = threefry4x64_20(ctr_add(c0, bk), eng.key);
+#pragma unroll
+            for (int w = 0; w < 4; ++w) {
+                const uint64_t p = 4 * bk + w;
+                if (p >= p0 && p < p1) {
+                    const uint64_t n = p - eng.index;
+                    tmp[n - nlo] = canonical64(r.v[w]) * delta + (double)n * delta;
+                }
+            }
+        }
+        __syncthreads();
+        for (int k = threadIdx.x; k < len; k += blockDim.x) {
+            const uint64_t n = w0 + k;
+            double a = tmp[n - nlo];
+            if (n > 0) {
+                const double b = tmp[n - 1 - nlo];
+                a = a < b ? b : a;
+            }
+            s_v[k] = a;
+        }
+    }
+    __device__ uint64_t point_F(double c) const
+    {
+        if (nn == 0) return 0;
+        const double est = c * (double)nn;
+        uint64_t n = !(est > 0) ? 0 : (est >= (double)nn ? nn : (uint64_t)est);
+        while (n < nn && v(n) <= c) ++n;
+        while (n > 0 && v(n - 1) > c) --n;
+        return n;
+    }
+};
+
+// A sorted uniform array held as a tile-wise running max: v_n =
+// max(tile_prefix_max[n / kTile], uscan[n]).
+struct InvArray {
+    static constexpr bool kInv = true;
+    static constexpr bool kWindow = true;
+    const double *uscan;
+    const double *tmax;  // exclusive running max of earlier tiles
+    uint64_t nn;
+    __device__ void bind(const FilterCtl *c) { nn = c->nn; }
+    __device__ double v(uint64_t n) const
+    {
+        const double a = uscan[n], b = tmax[n / kTile];
+        return a < b ? b : a;
+    }
+    __device__ void fill_window(double *s_v, double *, uint64_t w0, int len) const
+    {
+        for (int k = threadIdx.x; k < len; k += blockDim.x) s_v[k] = v(w0 + k);
+    }
+    __device__ uint64_t point_F(double c) const
+    {
+        // binary search for the first n with v_n > c
+        uint64_t lo = 0, hi = nn;
+        while (lo < hi) {
+            const uint64_t mid = lo + (hi - lo) / 2;
+            if (v(mid) <= c)
+                lo = mid + 1;
+            else
+                hi = mid;
+        }
+        return lo;
+    }
+};
+
+// ------------------------------------------------------------------------
+// the fold passes
+// ------------------------------------------------------------------------
+// Reduce-then-scan throughout: grid passes over 2048-element tiles and
+// single-workgroup scans over the per-tile summaries.  No workgroup waits for
+// another (a look-back chain across tiles would cross XCDs at every hop), and
+// every prefix is formed in one fixed order, so the passes that re-derive the
+// approximate prefix of an element get the same value bit for bit.
+struct FoldOut {
+    uint32_t *anc;       // [M] ancestor index (zero slots filled by k_zero_fill)
+    uint32_t *zl;        // [M] zero-slot list
+    uint32_t *sl;        // [M] surplus-copy list
+    uint64_t *rep;       // [M] optional replication counts
+    double *a_out;       // [M] optional exact fold values
+};
+
+// Tiles of 2048 elements, 8 per thread.  Per-element data lives in LDS (one
+// padding double per 8 keeps the thread-consecutive reads conflict-free) and
+// the per-element loops are not unrolled, which keeps every instantiation
+// small enough for the instruction cache.
+constexpr int kFI = kFoldItems;
+constexpr int kFT = kFoldTile;
+constexpr int kFPad = kFT + kFT / kFI;
+__device__ __forceinline__ int fidx(int e) { return e + e / kFI; }
+
+constexpr int kScanBlock = 1024;  // single-workgroup tile scans
+constexpr int kScanItems = 8;     // tiles per thread per round
+
+struct SegOp {
+    __device__ __forceinline__ Seg operator()(const Seg &a, const Seg &b) const { return seg_op(a, b); }
+};
+
+// First n in [lo, hi) with v(n) > c (hi if none), v non-decreasing.  All 64
+// lanes of a wave probe 64 evenly spaced points per round (64-ary search).
+template <typename Inv>
+__device__ uint64_t wave_first_greater(const Inv &inv, double c, uint64_t lo, uint64_t hi)
+{
+    const int lane = threadIdx.x & 63;
+    while (hi - lo > 64) {
+        const uint64_t step = (hi - lo + 63) / 64;
+        const uint64_t n = lo + (uint64_t)lane * step;
+        const bool gt = n < hi ? inv.v(n) > c : true;
+        const uint64_t mask = __ballot(gt);
+        const int g = mask ? __ffsll((unsigned long long)mask) - 1 : 64;
+        if (g == 0) return lo;
+        const uint64_t nlo = lo + (uint64_t)(g - 1) * step + 1;
+        const uint64_t nhi = g < 64 ? lo + (uint64_t)g * step : hi;
+        lo = nlo;
+        hi = nhi < hi ? nhi : hi;
+    }
+    const uint64_t n = lo + (uint64_t)lane;
+    const bool gt = n < hi ? inv.v(n) > c : true;
+    const uint64_t mask = __ballot(gt);
+    if (mask == 0) return hi;  // all 64 remaining candidates are <= c
+    const uint64_t r = lo + (uint64_t)(__ffsll((unsigned long long)mask) - 1);
+    return r < hi ? r : hi;
+}
+
+// x of one tile into LDS (striped, coalesced); zero past the end, which
+// classifies as negligible.
+template <typename XSrc>
+__device__ __forceinline__ void load_tile(const XSrc &xs, uint64_t base, uint64_t M, double *s_x)
+{
+#pragma unroll
+    for (int k = 0; k < kFI; ++k) {
+        const int e = k * kBlock + threadIdx.x;
+        const uint64_t i = base + e;
+        s_x[fidx(e)] = i < M ? xs.x(i) : 0.0;
+    }
+}
+
+// The segment pass and the apply pass derive every element's approximate
+// prefix from the tile's scanned start with the same code (thread_sum, the
+// block scan, run_seg), on the same data, so they classify every element
+// identically.
+__device__ __forceinline__ double thread_sum(const double *s_x, int e0)
+{
+    double s = 0;
+#pragma unroll 1
+    for (int k = 0; k < kFI; ++k) s += s_x[fidx(e0 + k)];
+    return s;
+}
+
+// Exclusive scan of n per-tile values by one workgroup of kScanBlock threads,
+// each taking kScanItems consecutive tiles per round; fixed order.  The loads
+// of a round are issued together (registers) so their latencies overlap.
+struct NoVisit {
+    template <typename T>
+    __device__ void operator()(uint64_t, const T &) const {}
+};
+
+template <typename T, typename Op, typename Visit = NoVisit>
+__device__ T wg_scan(const T *in, T *out, uint64_t n, Op op, T ident, T *lds,
+                     Visit visit = Visit())
+{
+    T carry = ident;
+    for (uint64_t c0 = 0; c0 < n; c0 += (uint64_t)kScanBlock * kScanItems) {
+        const uint64_t i0 = c0 + (uint64_t)threadIdx.x * kScanItems;
+        T v[kScanItems];
+#pragma unroll
+        for (int k = 0; k < kScanItems; ++k) v[k] = i0 + k < n ? in[i0 + k] : ident;
+        T loc = ident;
+#pragma unroll
+        for (int k = 0; k < kScanItems; ++k) loc = op(loc, v[k]);
+        T tot;
+        T run = op(carry, block_excl_scan<kScanBlock>(loc, op, ident, lds, &tot));
+#pragma unroll
+        for (int k = 0; k < kScanItems; ++k) {
+            if (i0 + k < n) {
+                out[i0 + k] = run;
+                visit(i0 + k, run);
+            }
+            run = op(run, v[k]);
+        }
+        carry = op(carry, tot);
+    }
+    return carry;
+}
+
+// ---- tile sum of x (any fixed order) -----------------------------------------
+template <typename XSrc>
+__device__ __forceinline__ double fold_tile_sum(const XSrc &xs, uint64_t base, uint64_t M,
+                                                double *s_red)
+{
+    double s = 0;
+#pragma unroll
+    for (int k = 0; k < kFI; ++k) {
+        const uint64_t i = base + k * kBlock + threadIdx.x;
+        s += i < M ? xs.x(i) : 0.0;
+    }
+    return block_reduce<kBlock>(s, AddOp(), s_red);
+}
+
+// LDS of the per-tile bodies; the uniform-window buffer only where the
+// inversion needs one.
+template <bool kWindowBuf>
+struct FoldLdsT {
+    double x[kFPad];
+    double tmp[kWindowBuf ? kFPad : 1];
+    double red[kBlock / kWave];
+    Seg seg[kBlock / kWave];
+    uint64_t u64[kBlock / kWave];
+    uint64_t bu[2];
+};
+
+// ---- classification, tile segment summary, special records ----------------------
+// The tile's specials go to spec_tile[k] in tile order, each with the run since
+// the previous special of the tile (or since the tile start); the tile summary
+// to *agg_out (written by the last thread).
+template <typename XSrc, typename Lds>
+__device__ void fold_tile_seg(const XSrc &xs, uint64_t base, uint64_t M, double ts,
+                              SpecRec *spec_tile, Seg *agg_out, Lds &L)
+{
+    double *const s_x = L.x;
+    __syncthreads();  // the LDS of a previous tile is no longer read
+    const int tid = threadIdx.x;
+    const int e0 = tid * kFI;
+    load_tile(xs, base, M, s_x);
+    __syncthreads();
+    const double s_thr =
+        ts + block_excl_scan<kBlock>(thread_sum(s_x, e0), AddOp(), 0.0, L.red, (double *)nullptr);
+    double er, ea;
+    fold_eps(M, &er, &ea);
+    const auto xat = [&](int k) { return s_x[fidx(e0 + k)]; };
+    int fast_b;
+    const Seg g = run_seg(xat, kFI, s_thr, er, ea, &fast_b);
+    const Seg thr_pre =
+        block_excl_scan<kBlock>(g, SegOp(), seg_identity(), L.seg, (Seg *)nullptr);
+    // tile total from the last thread (a struct total through a pointer would
+    // keep the struct in scratch memory)
+    if (tid == kBlock - 1) *agg_out = seg_op(thr_pre, g);
+    if (g.cnt) {
+        Seg pre = thr_pre;  // tile-local
+        double s = s_thr;
+#pragma unroll 1
+        for (int k = 0; k < kFI; ++k) {
+            const double x = s_x[fidx(e0 + k)];
+            const double sb = s;
+            s = sb + x;
+            const Seg el = elem_seg(x, sb, s, er, ea);
+            if (el.cnt) {
+                SpecRec *r = spec_tile + pre.cnt;
+                r->run.d0 = pre.tr.d0;
+                r->run.d1 = pre.tr.d1;
+                r->run.pp = pre.tr.pp;
+                r->run.b = pre.tr.b;
+                r->x = x;
+            }
+            pre = seg_op(pre, el);
+        }
+    }
+}
+
+// ---- the chain over the specials in fold order -------------------------------------
+// T tiles with exclusive segment summaries pre[] (their special counts also
+// in cnt_lds when that is not null), records at spec[tile * kFT + k].  BLOCK
+// threads take a chunk of specials each round: each finds its record (binary
+// search over the counts), thread 0 runs the chain (one transducer application
+// and one real FP64 add per special), and the values go to gval in fold
+// order.  Returns the exact fold total in thread 0; *bad is set in thread 0
+// when a transducer does not apply.
+template <int BLOCK>
+__device__ double fold_chain(const Seg *pre, const uint32_t *cnt_lds, uint64_t T, const Seg &tail,
+                             const SpecRec *spec, double *gval, SpecRec *s_r, double *s_v,
+                             int *bad)
+{
+    const int tid = threadIdx.x;
+    const uint64_t S = tail.cnt;
+    double a = 0.0;  // running exact value (thread 0)
+    *bad = 0;
+    for (uint64_t c0 = 0; c0 < S; c0 += BLOCK) {
+        const uint64_t g = c0 + tid;
+        if (g < S) {
+            // last tile whose count of earlier specials is <= g
+            uint64_t lo = 0, hi = T - 1;
+            while (lo < hi) {
+                const uint64_t mid = (lo + hi + 1) / 2;
+                const uint64_t c = cnt_lds ? cnt_lds[mid] : pre[mid].cnt;
+                if (c <= g)
+                    lo = mid;
+                else
+                    hi = mid - 1;
+            }
+            const Seg p = pre[lo];
+            const uint64_t k = g - p.cnt;
+            SpecRec r = spec[lo * (uint64_t)kFT + k];
+            if (k == 0) r.run = tr_compose(p.tr, r.run);
+            s_r[tid] = r;
+        }
+        __syncthreads();
+        if (tid == 0) {
+            const int n = (int)(S - c0 < (uint64_t)BLOCK ? S - c0 : (uint64_t)BLOCK);
+#pragma unroll 1
+            for (int k = 0; k < n; ++k) {
+                double v;
+                if (!tr_apply(s_r[k].run, a, &v)) {
+                    *bad = 1;
+                    v = a;
+                }
+                a = v + s_r[k].x;
+                s_v[k] = a;
+            }
+        }
+        __syncthreads();
+        if (g < S) gval[g] = s_v[tid];
+        __syncthreads();
+    }
+    double tot = a;
+    if (tid == 0 && !tr_apply(tail.tr, a, &tot)) {
+        *bad = 1;
+        tot = a;
+    }
+    return tot;
+}
+
+// ---- pass 5: exact values, inversion, replication --------------------------------
+// Exact a_i = (run since the last special) applied to the value after that
+// special.  Then c_i = F(a_i) (uniforms at or below a_i) and the replication
+// counts as differences; per tile the number of zero slots and of surplus
+// copies for the layout pass.
+template <typename XSrc, typename Inv, typename Lds>
+__device__ void fold_tile_apply(FilterCtl *ctl, const XSrc &xs, const Inv &inv, uint64_t t,
+                                uint64_t M, double ts, const Seg tpre, const double *gval,
+                                const FoldOut &out, uint32_t *rep32, uint64_t *zs_out, Lds &L)
+{
+    // x, overwritten in place (each thread its own elements) by the exact
+    // values; later the uniform window and the staged replication counts
+    double *const s_x = L.x;
+    double *const s_tmp = L.tmp;
+    double *const s_a = s_x;
+    double *const s_red = L.red;
+    Seg *const s_seg = L.seg;
+    uint64_t *const s_u64 = L.u64;
+    uint64_t *const s_bu = L.bu;
+    __syncthreads();  // the LDS of a previous tile is no longer read
+
+    const int tid = threadIdx.x;
+    const uint64_t base = t * (uint64_t)kFT;
+    const int e0 = tid * kFI;
+    load_tile(xs, base, M, s_x);
+    __syncthreads();
+
+    // ---- exact fold values ----------------------------------------------------
+    const double s_thr =
+        ts + block_excl_scan<kBlock>(thread_sum(s_x, e0), AddOp(), 0.0, s_red, (double *)nullptr);
+    double er, ea;
+    fold_eps(M, &er, &ea);
+    const auto xat = [&](int k) { return s_x[fidx(e0 + k)]; };
+    int fast_b;
+    const Seg g = run_seg(xat, kFI, s_thr, er, ea, &fast_b);
+    const Seg thr_pre =
+        block_excl_scan<kBlock>(g, SegOp(), seg_identity(), s_seg, (Seg *)nullptr);
+    int bad = !run_values(
+        xat, kFI, s_thr, er, ea, fast_b, seg_op(tpre, thr_pre),
+        [=](uint64_t c) { return gval[c]; }, [&](int k, double a) { s_a[fidx(e0 + k)] = a; });
+    double v_start = 0.0;  // exact a_{base-1}
+    if (t > 0) {
+        const double vb = tpre.cnt ? gval[tpre.cnt - 1] : 0.0;
+        if (!tr_apply(tpre.tr, vb, &v_start)) bad = 1;
+    }
+    if (bad) atomicOr(&ctl->err, (int)kErrFoldClass);
+    __syncthreads();
+
+    if (out.a_out) {
+#pragma unroll 1
+        for (int k = 0; k < kFI; ++k) {
+            const int e = k * kBlock + tid;
+            const uint64_t i = base + e;
+            if (i < M) out.a_out[i] = s_a[fidx(e)];
+        }
+    }
+
+    if constexpr (!Inv::kInv) {
+        return;
+    } else {
+        // ---- inversion: c_i = F(a_i) for i <= M-2 -------------------------------
+        const uint64_t nn = inv.nn;
+        const uint64_t last_cmp = M >= 2 ? M - 2 : 0;  // highest index compared
+        uint64_t c[kFI];
+        double av[kFI];
+#pragma unroll
+        for (int k = 0; k < kFI; ++k) av[k] = s_a[fidx(e0 + k)];
+        uint64_t cstart = 0;  // F(a_{base-1}); 0 for the first tile
+        if constexpr (!Inv::kWindow) {
+#pragma unroll
+            for (int k = 0; k < kFI; ++k) {
+                const uint64_t i = base + e0 + k;
+                c[k] = (M >= 2 && i <= last_cmp) ? inv.F(av[k]) : 0;
+            }
+            if (t > 0) cstart = inv.F(v_start);
+        } else {
+            // the tile holds uniforms [F(a_{base-1}), F(a_{last compared}))
+            if (tid < kWave) {
+                uint64_t lo = 0, hi = 0;
+                if (M >= 2) {
+                    lo = t == 0 ? 0 : wave_first_greater(inv, v_start, 0, nn);
+                    hi = lo;
+                    if (base <= last_cmp) {
+                        const uint64_t il =
+                            (base + kFT - 1 < last_cmp) ? base + kFT - 1 : last_cmp;
+                        hi = wave_first_greater(inv, s_a[fidx((int)(il - base))], lo, nn);
+                    }
+                }
+                if (tid == 0) {
+                    s_bu[0] = lo;
+                    s_bu[1] = hi;
+                }
+            }
+            __syncthreads();
+            const uint64_t wlo = s_bu[0], whi = s_bu[1];
+            cstart = wlo;
+#pragma unroll
+            for (int k = 0; k < kFI; ++k) c[k] = wlo;
+            for (uint64_t w0 = wlo; w0 < whi; w0 += kFT) {
+                const int len = (int)((whi - w0) < (uint64_t)kFT ? (whi - w0) : (uint64_t)kFT);
+                __syncthreads();
+                inv.fill_window(s_x, s_tmp, w0, len);
+                __syncthreads();
+                const double first = s_x[0], lastv = s_x[len - 1];
+                // the thread's values are sorted: gallop from the previous
+                // position, then bisect (first index with value > a)
+                int pos = 0;
+#pragma unroll
+                for (int k = 0; k < kFI; ++k) {
+                    const double a = av[k];
+                    if (a < first) continue;
+                    if (a >= lastv) {
+                        c[k] += len;
+                        pos = len;
+                        continue;
+                    }
+                    int lo = pos, step = 1, hi = pos;
+                    while (hi < len && s_x[hi] <= a) {
+                        lo = hi + 1;
+                        hi += step;
+                        step <<= 1;
+                    }
+                    if (hi > len) hi = len;
+                    while (lo < hi) {
+                        const int mid = (lo + hi) >> 1;
+                        if (s_x[mid] <= a)
+                            lo = mid + 1;
+                        else
+                            hi = mid;
+                    }
+                    c[k] += lo;
+                    pos = lo;
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < kFI; ++k) {
+                const uint64_t i = base + e0 + k;
+                if (!(M >= 2 && i <= last_cmp)) c[k] = 0;
+            }
+        }
+        // c of the element before this thread's first item
+        uint64_t cprev;
+        {
+            const int lane = tid & 63, wid = tid / 64;
+            uint64_t up = lane_up(c[kFI - 1], 1);
+            if (lane == 63) s_u64[wid] = c[kFI - 1];
+            __syncthreads();
+            if (lane == 0) up = wid > 0 ? s_u64[wid - 1] : 0;
+            cprev = tid == 0 ? cstart : up;
+            __syncthreads();
+        }
+        // replication counts, staged in LDS for striped stores
+        uint32_t *s_rep = reinterpret_cast<uint32_t *>(s_x);
+        uint64_t zs_thr = 0;  // zeros << 32 | surplus copies
+#pragma unroll
+        for (int k = 0; k < kFI; ++k) {
+            const uint64_t i = base + e0 + k;
+            uint64_t r = 0;
+            if (i < M) {
+                if (M == 1)
+                    r = nn;
+                else if (i <= last_cmp)
+                    r = c[k] - cprev;
+                else  // i == M-1
+                    r = nn - cprev;
+                r += xs.ipart(i);
+                if (r == 0)
+                    zs_thr += 1ull << 32;
+                else if (r >= 2)
+                    zs_thr += r - 1;
+                if (out.rep) out.rep[i] = r;
+            }
+            cprev = c[k];
+            s_rep[e0 + k] = (uint32_t)r;
+            if (M >= 2 && i == last_cmp) ctl->n_final = c[k];
+        }
+        if (M == 1 && t == 0 && tid == 0) ctl->n_final = 0;
+        if (!out.anc) return;  // replication only (M != N): no cfrp layout
+        const uint64_t zs_tile = block_reduce<kBlock>(zs_thr, AddOp(), s_u64);
+        if (tid == 0) *zs_out = zs_tile;
+#pragma unroll
+        for (int k = 0; k < kFI; ++k) {
+            const int e = k * kBlock + tid;
+            const uint64_t i = base + e;
+            if (i < M) rep32[i] = s_rep[e];
+        }
+    }
+}
+
+
+// ---- pass 7: cfrp_trans layout (common.hpp:91-110) ------------------------------------
+// anc[i] = i where rep_i > 0; the k-th zero slot (in index order) goes to the
+// zero list at rank k, the k-th surplus copy (index order, rep_i - 1 copies
+// of i) to the surplus list at rank k; k_zero_fill pairs them.
+template <typename Lds>
+__device__ void fold_tile_layout(FilterCtl *ctl, uint64_t t, uint64_t M, const uint32_t *rep32,
+                                 uint64_t zsx, const FoldOut &out, Lds &L)
+{
+    uint32_t *const s_rep = reinterpret_cast<uint32_t *>(L.x);        // kFT entries
+    uint32_t *const s_sp = reinterpret_cast<uint32_t *>(L.x) + kFT;   // kFT entries
+    uint64_t *const s_u64 = L.u64;
+    __syncthreads();  // the LDS of a previous tile is no longer read
+    const int tid = threadIdx.x;
+    const uint64_t base = t * (uint64_t)kFT;
+    const int e0 = tid * kFI;
+#pragma unroll
+    for (int k = 0; k < kFI; ++k) {
+        const int e = k * kBlock + tid;
+        const uint64_t i = base + e;
+        s_rep[e] = i < M ? rep32[i] : 1u;
+    }
+    __syncthreads();
+    uint64_t zs_thr = 0;
+#pragma unroll 1
+    for (int k = 0; k < kFI; ++k) {
+        const uint32_t r = s_rep[e0 + k];
+        if (r == 0)
+            zs_thr += 1ull << 32;
+        else if (r >= 2)
+            zs_thr += r - 1;
+    }
+    uint64_t zs_tile;
+    const uint64_t zs_excl =
+        block_excl_scan<kBlock>(zs_thr, AddOp(), (uint64_t)0, s_u64, &zs_tile);
+    const uint64_t pre = zsx;
+    uint64_t zrank = (pre >> 32) + (zs_excl >> 32);
+    uint64_t srank_local = zs_excl & 0xffffffffull;  // surplus before this thread (tile)
+    const uint64_t s_base = pre & 0xffffffffull;
+#pragma unroll 1
+    for (int k = 0; k < kFI; ++k) {
+        const int e = e0 + k;
+        const uint64_t i = base + e;
+        const uint32_t r = s_rep[e];
+        if (i < M && r == 0) {
+            if (zrank < M)
+                out.zl[zrank] = (uint32_t)i;
+            else
+                atomicOr(&ctl->err, (int)kErrBounds);
+            ++zrank;
+        }
+        if (r >= 2) srank_local += r - 1;
+        s_sp[e] = (uint32_t)srank_local;
+    }
+#pragma unroll
+    for (int k = 0; k < kFI; ++k) {
+        const int e = k * kBlock + tid;
+        const uint64_t i = base + e;
+        if (i < M) out.anc[i] = (uint32_t)i;
+    }
+    __syncthreads();
+    // expand surplus copies: positions [s_base, s_base + S_tile)
+    const uint32_t S_tile = (uint32_t)(zs_tile & 0xffffffffull);
+    for (uint32_t k = tid; k < S_tile; k += kBlock) {
+        int lo = 0, hi = kFT - 1;  // first e with s_sp[e] > k
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (s_sp[mid] > k)
+                hi = mid;
+            else
+                lo = mid + 1;
+        }
+        if (s_base + k < M)
+            out.sl[s_base + k] = (uint32_t)(base + lo);
+        else
+            atomicOr(&ctl->err, (int)kErrBounds);
+    }
+}
+
+
+// ---- resampling bookkeeping (one thread) ---------------------------------------------
+// Non-residual schemes: nn = n uniforms, delta = 1/nn, and the systematic
+// sequence's one constructor draw u0 = U * delta (rng/u01.hpp:253-263).
+__device__ __forceinline__ void resample_prologue(FilterCtl *ctl, int scheme)
+{
+    ctl->nn = ctl->n;
+    ctl->r_int = 0;
+    ctl->delta = 1.0 / (double)ctl->nn;
+    if (scheme == kSystematic) ctl->u0 = canonical64(tf64_stream_word(ctl->eng, 0)) * ctl->delta;
+}
+
+// Engine bookkeeping: vSMC's lazy sequences draw min(nn, n_final + 1)
+// uniforms (rng/u01.hpp:38-40, resample/common.hpp:76-83); the systematic
+// sequence draws one, in its constructor.
+__device__ __forceinline__ void resample_epilogue(FilterCtl *ctl, StepRow *hist,
+                                                  int systematic_like, uint64_t M)
+{
+    uint64_t draws;
+    if (systematic_like)
+        draws = 1;
+    else
+        draws = ctl->nn == 0 || M <= 1 ? 0
+                : (ctl->n_final + 1 < ctl->nn ? ctl->n_final + 1 : ctl->nn);
+    tf64_advance(ctl->eng, draws);
+    ctl->draws = draws;
+    ctl->prev_resampled = 1;
+    if (hist) hist[ctl->t].draws = draws;
+}
+
+}  // namespace vsmcx

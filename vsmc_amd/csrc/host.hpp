@@ -1,0 +1,100 @@
+// host.hpp -- host-side plumbing shared by the C ABI translation units:
+// error translation at the boundary and the context (device, stream, HBM
+// budget, allocation bookkeeping).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <map>
+#include <stdexcept>
+#include <string>
+
+#include "../../include/vsmcx.h"
+#include "internal.hpp"
+
+using namespace vsmcx;
+
+namespace vsmcx_host {
+
+inline thread_local std::string g_err;
+
+struct BudgetError : std::runtime_error {
+    explicit BudgetError(const std::string &s) : std::runtime_error(s) {}
+};
+struct ArgError : std::runtime_error {
+    explicit ArgError(const std::string &s) : std::runtime_error(s) {}
+};
+struct DevError : std::runtime_error {
+    int code;
+    DevError(int c, const std::string &s) : std::runtime_error(s), code(c) {}
+};
+
+inline int fail(int code, const std::string &msg)
+{
+    g_err = msg;
+    return code;
+}
+
+}  // namespace vsmcx_host
+
+using namespace vsmcx_host;
+
+#define API_BEGIN try {
+#define API_END                                                   \
+    }                                                             \
+    catch (const ArgError &e) { return fail(VSMCX_ERR_ARG, e.what()); }          \
+    catch (const BudgetError &e) { return fail(VSMCX_ERR_BUDGET, e.what()); }    \
+    catch (const DevError &e) { return fail(e.code, e.what()); }                 \
+    catch (const HipError &e) { return fail(VSMCX_ERR_HIP, e.what()); }          \
+    catch (const std::exception &e) { return fail(VSMCX_ERR_INTERNAL, e.what()); } \
+    return VSMCX_OK;
+
+#define REQUIRE(cond, msg) \
+    do {                   \
+        if (!(cond)) throw ArgError(msg); \
+    } while (0)
+
+struct vsmcx_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    uint64_t budget = 0;
+    uint64_t allocated = 0;
+    std::map<void *, size_t> sizes;
+
+    void *alloc(size_t bytes)
+    {
+        if (bytes == 0) bytes = 1;
+        if (budget && allocated + bytes > budget)
+            throw BudgetError("device-memory budget exceeded: need " + std::to_string(bytes) +
+                              " B, " + std::to_string(budget - allocated) + " B left");
+        void *p = nullptr;
+        VSMCX_HIP(hipSetDevice(device));
+        VSMCX_HIP(hipMalloc(&p, bytes));
+        allocated += bytes;
+        sizes[p] = bytes;
+        return p;
+    }
+    template <typename T>
+    T *alloc_n(size_t n)
+    {
+        return static_cast<T *>(alloc(n * sizeof(T)));
+    }
+    template <typename T>
+    T *alloc_zero(size_t n)
+    {
+        T *p = alloc_n<T>(n);
+        VSMCX_HIP(hipMemsetAsync(p, 0, n * sizeof(T), stream));
+        return p;
+    }
+    void release(void *p)
+    {
+        if (!p) return;
+        auto it = sizes.find(p);
+        if (it != sizes.end()) {
+            allocated -= it->second;
+            sizes.erase(it);
+        }
+        (void)hipFree(p);
+    }
+    void sync() { VSMCX_HIP(hipStreamSynchronize(stream)); }
+};
+

@@ -16,7 +16,7 @@ namespace vsmcx {
 constexpr int kBlock = 256;                  // threads per workgroup
 constexpr int kItems = 16;                   // particles per thread
 constexpr int kTile = kBlock * kItems;       // particles per workgroup tile
-constexpr int kMoveItems = 4;                // particles per thread in the move kernels
+constexpr int kMoveItems = 8;                // particles per thread in the move kernels
 constexpr int kMoveTile = kBlock * kMoveItems;  // particles per move workgroup (one partial)
 constexpr int kFoldItems = 8;                // elements per thread in the fold kernels
 constexpr int kFoldTile = kBlock * kFoldItems;  // elements per fold tile
@@ -67,6 +67,12 @@ struct FilterCtl {
     int scheme;
     int err;
     uint64_t specials; // special elements of the last fold
+};
+
+// Parameters of the constant-velocity model (example/pf).
+struct CvParams {
+    double dt, sd_pos, sd_vel, prior_sd_pos, prior_sd_vel;
+    double obs_scale, nu, llh_const;
 };
 
 // One row of per-step history.

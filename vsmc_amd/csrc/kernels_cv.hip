@@ -22,98 +22,10 @@
 #include "device_util.hpp"
 #include "internal.hpp"
 #include "launch.hpp"
+#include "cv_model.hpp"
 #include "rng_core.hpp"
 
 namespace vsmcx {
-
-struct alignas(16) S4f { float x, y, vx, vy; };
-struct alignas(32) S4d { double x, y, vx, vy; };
-
-template <typename Real> struct StateOf;
-template <> struct StateOf<float> { typedef S4f type; };
-template <> struct StateOf<double> { typedef S4d type; };
-
-// four N(0,1) from one Philox4x32-10 block, counter {t+1, id_lo, id_hi, 0}
-__device__ __forceinline__ void cv_normals(float z[4], uint32_t k0, uint32_t k1, uint64_t id,
-                                           uint64_t t)
-{
-    u32x4 c;
-    c.v[0] = (uint32_t)(t + 1);
-    c.v[1] = (uint32_t)id;
-    c.v[2] = (uint32_t)(id >> 32);
-    c.v[3] = 0;
-    const u32x4 w = philox4x32_10(c, k0, k1);
-#pragma unroll
-    for (int j = 0; j < 2; ++j) {
-        const float u1 = u01_oc_24(w.v[2 * j]);
-        const float u2 = u01_oc_24(w.v[2 * j + 1]);
-        const float r = sqrtf(-2.0f * logf(u1));
-        float s, co;
-        sincospif(2.0f * u2, &s, &co);
-        z[2 * j] = r * co;
-        z[2 * j + 1] = r * s;
-    }
-}
-
-__device__ __forceinline__ void cv_normals(double z[4], uint32_t k0, uint32_t k1, uint64_t id,
-                                           uint64_t t)
-{
-    u32x4 c;
-    c.v[0] = (uint32_t)(t + 1);
-    c.v[1] = (uint32_t)id;
-    c.v[2] = (uint32_t)(id >> 32);
-    c.v[3] = 0;
-    const u32x4 w = philox4x32_10(c, k0, k1);
-    const double two_pi = 6.2831853071795865;
-#pragma unroll
-    for (int j = 0; j < 2; ++j) {
-        const double u1 = u01_oc_53(w.v[2 * j]);
-        const double u2 = u01_oc_53(w.v[2 * j + 1]);
-        const double r = sqrt(-2 * log(u1));
-        z[2 * j] = r * cos(two_pi * u2);
-        z[2 * j + 1] = r * sin(two_pi * u2);
-    }
-}
-
-__device__ __forceinline__ float cv_log1p(float x) { return log1pf(x); }
-__device__ __forceinline__ double cv_log1p(double x) { return log1p(x); }
-
-template <typename Real>
-struct CvModel {
-    Real dt, sd_pos, sd_vel, psd_pos, psd_vel, scale, nu, c2, half_nu1;
-    __device__ void load(const CvParams &p)
-    {
-        dt = (Real)p.dt;
-        sd_pos = (Real)p.sd_pos;
-        sd_vel = (Real)p.sd_vel;
-        psd_pos = (Real)p.prior_sd_pos;
-        psd_vel = (Real)p.prior_sd_vel;
-        scale = (Real)p.obs_scale;
-        nu = (Real)p.nu;
-        c2 = (Real)(2 * p.llh_const);
-        half_nu1 = (Real)(0.5 * (p.nu + 1));
-    }
-    __device__ Real llh(Real x, Real y, Real ox, Real oy) const
-    {
-        const Real dx = (x - ox) / scale;
-        const Real dy = (y - oy) / scale;
-        const Real lx = cv_log1p(dx * dx / nu);
-        const Real ly = cv_log1p(dy * dy / nu);
-        return c2 - half_nu1 * (lx + ly);
-    }
-};
-
-struct Part5 { double se, se2, mx, my; };
-__device__ __forceinline__ Part5 lane_xor(const Part5 &v, int m)
-{
-    return Part5{lane_xor(v.se, m), lane_xor(v.se2, m), lane_xor(v.mx, m), lane_xor(v.my, m)};
-}
-struct Part5Add {
-    __device__ Part5 operator()(const Part5 &a, const Part5 &b) const
-    {
-        return Part5{a.se + b.se, a.se2 + b.se2, a.mx + b.mx, a.my + b.my};
-    }
-};
 
 template <typename Real>
 __global__ __launch_bounds__(kBlock) void k_cv_move(
@@ -168,31 +80,12 @@ __global__ __launch_bounds__(kBlock) void k_cv_move(
         }
         Real z[4];
         cv_normals(z, k0, k1, i, t);
-        if (t == 0) {
-            sk.x = md.psd_pos * z[0];
-            sk.y = md.psd_pos * z[1];
-            sk.vx = md.psd_vel * z[2];
-            sk.vy = md.psd_vel * z[3];
-        } else {
-            sk.x += md.sd_pos * z[0] + md.dt * sk.vx;
-            sk.y += md.sd_pos * z[1] + md.dt * sk.vy;
-            sk.vx += md.sd_vel * z[2];
-            sk.vy += md.sd_vel * z[3];
-        }
+        cv_advance(md, sk, z, t);
         const Real inc = md.llh(sk.x, sk.y, ox, oy);
         const double lwn = lwo[k] + (double)inc;
         st_out[i] = sk;
         lw[i] = lwn;
-        if (lwn > m) {
-            const double r = exp(m - lwn);
-            se = se * r + 1;
-            se2 = se2 * r * r + 1;
-            m = lwn;
-        } else {
-            const double e = exp(lwn - m);
-            se += e;
-            se2 += e * e;
-        }
+        lse_add(m, se, se2, lwn);
     }
     const double M = block_reduce<kBlock>(m, MaxOp(), s_red);
     const double sc = (m == -INFINITY) ? 0.0 : exp(m - M);
@@ -230,43 +123,7 @@ __global__ __launch_bounds__(1024) void k_cv_reduce(FilterCtl *ctl, const double
         acc.my += q[4];
     }
     acc = block_reduce<1024>(acc, Part5Add(), s_p5);
-    if (threadIdx.x == 0) {
-        const double n = (double)ctl->n;
-        const double lse_old = (t == 0 || ctl->prev_resampled) ? log(n) : ctl->lse;
-        const double lse = M + log(acc.se);
-        const double logz = (t == 0 ? 0.0 : ctl->logz) + (lse - lse_old);
-        const double ess = acc.se * acc.se / acc.se2;
-        int res = ess < ctl->threshold * n;
-        if (!(acc.se > 0) || !isfinite(lse) || !isfinite(ess)) {
-            atomicOr(&ctl->err, (int)kErrDegenerate);
-            res = 0;
-        }
-        ctl->m = M;
-        ctl->coeff = 1 / acc.se;
-        ctl->lse = lse;
-        ctl->logz = logz;
-        ctl->ess = ess;
-        ctl->resample = res;
-        ctl->t = t;
-        ctl->prev_resampled = 0;
-        ctl->r_int = 0;
-        ctl->draws = 0;
-        if (hist) {
-            StepRow r;
-            r.ess = ess;
-            r.logz = logz;
-            r.mean_x = 0;
-            r.mean_y = 0;
-            r.resampled = res;
-            r.pad = 0;
-            r.draws = 0;
-            hist[t] = r;
-            if (t > 0) {
-                hist[t - 1].mean_x = acc.mx;
-                hist[t - 1].mean_y = acc.my;
-            }
-        }
-    }
+    if (threadIdx.x == 0) cv_step_scalars(ctl, M, acc.se, acc.se2, acc.mx, acc.my, hist, t);
 }
 
 // Monitor of the last step: sum_i W_i (x_i, y_i) over the post-resample state.

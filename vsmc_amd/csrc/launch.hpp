@@ -23,11 +23,6 @@ struct ResampleBuffers {
     FoldScratch fs{};
 };
 
-// Parameters of the constant-velocity model (example/pf).
-struct CvParams {
-    double dt, sd_pos, sd_vel, prior_sd_pos, prior_sd_vel;
-    double obs_scale, nu, llh_const;
-};
 
 // kernels_fold.hip
 void launch_resample_from_log_weights(FilterCtl *ctl, const double *lw, uint64_t M, int scheme,
@@ -67,5 +62,39 @@ void launch_plain_rest(FilterCtl *ctl, void *st, double *lw, double *w, double *
                        double *part, double *scal, uint64_t t, uint64_t N, int scheme,
                        ResampleBuffers &rb, StepRow *hist, hipStream_t s);
 uint64_t plain_partials(uint64_t N);
+
+
+// ---- batched filters (kernels_batch.hip) --------------------------------------------
+constexpr int kBatchMaxTiles = 128;  // filters of up to 128 * kFoldTile = 2^18 particles
+
+// One filter of a batch (host-built, uploaded once).
+struct BatchFilter {
+    uint64_t n;         // particles
+    uint64_t horizon;   // steps
+    uint64_t off;       // first particle in the pooled arrays (a multiple of kFoldTile)
+    uint64_t obs_off;   // first observation row
+    uint64_t hist_off;  // first history row
+    uint64_t seed;      // Philox4x32 key of the move stream
+};
+
+// Pooled device arrays of a batch; per-particle arrays are indexed off + i.
+struct BatchPool {
+    void *st0, *st1;       // states (S4f or S4d), ping-pong
+    double *lw;
+    uint32_t *anc, *zl, *sl, *rep32;
+    SpecRec *spec;
+    double *gval;
+    FilterCtl *ctl;        // [filters]
+    StepRow *hist;         // history rows
+    const double *obs;     // observation rows (x, y)
+    uint64_t *next_t;      // [filters] next step
+    const uint64_t *avail; // [filters] observations available
+    uint32_t *cur;         // [filters] state buffer holding the current state
+    int *err_any;          // OR of every filter's device error bits
+};
+
+void launch_batch_run(bool fp64, int scheme, const BatchFilter *filters, const uint32_t *order,
+                      uint32_t nf, const BatchPool &P, const CvParams &p, uint64_t max_steps,
+                      unsigned *queue, unsigned grid, hipStream_t st);
 
 }  // namespace vsmcx
