@@ -187,7 +187,7 @@ typedef struct vsmcx_batch vsmcx_batch;
 
 typedef struct {
     uint64_t nfilters;
-    const uint64_t *n;             /* [nfilters] particles, 1 .. 2^18 */
+    const uint64_t *n;             /* [nfilters] particles, 1 .. 2^20 */
     const uint64_t *horizon;       /* [nfilters] steps */
     const uint64_t *seed;          /* [nfilters] Philox4x32 keys of the move streams */
     const uint64_t *resample_seed; /* [nfilters] Threefry4x64 seeds of the resample engines */
@@ -217,6 +217,78 @@ int vsmcx_batch_steps(vsmcx_batch *b, uint64_t f, uint64_t *t);
 int vsmcx_batch_history(vsmcx_batch *b, uint64_t f, uint64_t count, vsmcx_step_stats *out);
 int vsmcx_batch_state(vsmcx_batch *b, uint64_t f, double *state, double *log_weight);
 int vsmcx_batch_particles(vsmcx_batch *b, uint64_t *total);
+
+/* ---- serving loop --------------------------------------------------------------
+ * Filtering jobs arrive over time (vsmcx_serve_submit), each with its own model,
+ * particle count, horizon and seeds; their observations arrive one step at a
+ * time (vsmcx_serve_observe).  Jobs are admitted in arrival order while a job
+ * slot and room in the HBM arena are free, keep their particle state on the
+ * device until their last step, and are advanced with continuous batching:
+ * each vsmcx_serve_poll finishes the tick in flight, admits jobs, and launches
+ * one persistent kernel over every running job with an observation waiting
+ * (one step each).  A job's results do not depend on the other jobs.  No
+ * counterpart in vSMC (one Sampler per job, stepped by its caller). */
+#define VSMCX_MODEL_CV 0  /* the pf example's constant-velocity model */
+#define VSMCX_MODEL_LG 1  /* linear-Gaussian: x_t = A x_{t-1} + q z, y_t = x_t + r v */
+
+typedef struct vsmcx_serve vsmcx_serve;
+
+typedef struct {
+    uint64_t hbm_budget;   /* bytes of device memory for the particle arena */
+    uint32_t max_jobs;     /* job slots on the device */
+    uint32_t pad;
+    uint64_t max_horizon;  /* longest job */
+    int precision;         /* VSMCX_MIXED or VSMCX_FP64_FUSED */
+    int scheme;            /* VSMCX_SYSTEMATIC or VSMCX_STRATIFIED */
+    double threshold;
+    /* constant-velocity model (vsmcx_filter_desc's fields) */
+    double dt, sd_pos, sd_vel, prior_sd_pos, prior_sd_vel, obs_scale, nu, llh_const;
+    /* linear-Gaussian model: dimension (<= 10), row-major A, noise sds, prior sd */
+    uint32_t lg_dim;
+    uint32_t pad2;
+    const double *lg_a;
+    double lg_q, lg_r, lg_s0;
+} vsmcx_serve_desc;
+
+typedef struct {
+    int model;             /* VSMCX_MODEL_CV or VSMCX_MODEL_LG */
+    int pad;
+    uint64_t n;            /* particles, 1 .. 2^20 */
+    uint64_t horizon;      /* steps (observations) */
+    uint64_t seed;         /* Philox4x32 key of the move stream */
+    uint64_t resample_seed;/* Threefry4x64 seed of the resample engine */
+} vsmcx_job_desc;
+
+typedef struct {
+    uint64_t steps;              /* job steps completed */
+    uint64_t particle_updates;   /* sum of n over completed steps */
+    uint64_t jobs_done, jobs_active, jobs_waiting;
+    double p50_us, p99_us;       /* step latency: observation arrival to step completion */
+    double busy_ms;              /* device time of the ticks */
+    double elapsed_ms;           /* first launch to last completion */
+    uint64_t peak_particles;     /* most particle slots in use at once */
+    uint64_t capacity_particles; /* particle slots the budget allows */
+} vsmcx_serve_summary;
+
+int vsmcx_serve_create(vsmcx_ctx *ctx, const vsmcx_serve_desc *desc, vsmcx_serve **out);
+int vsmcx_serve_destroy(vsmcx_serve *s);
+int vsmcx_serve_submit(vsmcx_serve *s, const vsmcx_job_desc *job, uint64_t *id);
+/* one step's observation: 2 values (CV) or lg_dim values (LG) */
+int vsmcx_serve_observe(vsmcx_serve *s, uint64_t id, const double *y);
+/* one observation each for `count` jobs (ys[k] holds job ids[k]'s values) */
+int vsmcx_serve_observe_many(vsmcx_serve *s, uint64_t count, const uint64_t *ids,
+                             const double *const *ys);
+/* jobs (waiting or running) that have used every observation they received
+ * and expect more: up to `max` ids, *count = how many there are */
+int vsmcx_serve_ready(vsmcx_serve *s, uint64_t max, uint64_t *ids, uint64_t *count);
+/* finish the tick in flight (waiting for it if `wait`, else only if done),
+ * admit jobs, launch the next tick; *launched = job steps in it */
+int vsmcx_serve_poll(vsmcx_serve *s, int wait, uint64_t *launched);
+/* state: 0 waiting, 1 running, 2 done */
+int vsmcx_serve_job(vsmcx_serve *s, uint64_t id, int *state, uint64_t *steps);
+int vsmcx_serve_history(vsmcx_serve *s, uint64_t id, uint64_t count, vsmcx_step_stats *out);
+int vsmcx_serve_forget(vsmcx_serve *s, uint64_t id);
+int vsmcx_serve_stats(vsmcx_serve *s, vsmcx_serve_summary *out);
 
 #ifdef __cplusplus
 }

@@ -174,4 +174,4 @@ def test_batch_rejects_unsupported(ctx):
     with pytest.raises(V.VsmcxError):
         V.Batch(ctx, [100], [5], [1], [2], scheme=V.MULTINOMIAL)
     with pytest.raises(V.VsmcxError):
-        V.Batch(ctx, [1 << 19], [5], [1], [2])
+        V.Batch(ctx, [(1 << 20) + 1], [5], [1], [2])

@@ -62,6 +62,33 @@ class BatchDesc(C.Structure):
                 ("obs_scale", C.c_double), ("nu", C.c_double), ("llh_const", C.c_double)]
 
 
+MODEL_CV, MODEL_LG = 0, 1
+
+
+class ServeDesc(C.Structure):
+    _fields_ = [("hbm_budget", C.c_uint64), ("max_jobs", C.c_uint32), ("pad", C.c_uint32),
+                ("max_horizon", C.c_uint64), ("precision", C.c_int), ("scheme", C.c_int),
+                ("threshold", C.c_double),
+                ("dt", C.c_double), ("sd_pos", C.c_double), ("sd_vel", C.c_double),
+                ("prior_sd_pos", C.c_double), ("prior_sd_vel", C.c_double),
+                ("obs_scale", C.c_double), ("nu", C.c_double), ("llh_const", C.c_double),
+                ("lg_dim", C.c_uint32), ("pad2", C.c_uint32), ("lg_a", C.c_void_p),
+                ("lg_q", C.c_double), ("lg_r", C.c_double), ("lg_s0", C.c_double)]
+
+
+class JobDesc(C.Structure):
+    _fields_ = [("model", C.c_int), ("pad", C.c_int), ("n", C.c_uint64), ("horizon", C.c_uint64),
+                ("seed", C.c_uint64), ("resample_seed", C.c_uint64)]
+
+
+class ServeSummary(C.Structure):
+    _fields_ = [("steps", C.c_uint64), ("particle_updates", C.c_uint64),
+                ("jobs_done", C.c_uint64), ("jobs_active", C.c_uint64),
+                ("jobs_waiting", C.c_uint64), ("p50_us", C.c_double), ("p99_us", C.c_double),
+                ("busy_ms", C.c_double), ("elapsed_ms", C.c_double),
+                ("peak_particles", C.c_uint64), ("capacity_particles", C.c_uint64)]
+
+
 STEP_DTYPE = np.dtype([("ess", "f8"), ("log_z", "f8"), ("mean_x", "f8"), ("mean_y", "f8"),
                        ("resampled", "i4"), ("pad", "i4"), ("draws", "u8")])
 
@@ -118,6 +145,15 @@ def lib():
         "vsmcx_batch_history": ([vp, u64, u64, vp], i32),
         "vsmcx_batch_state": ([vp, u64, vp, vp], i32),
         "vsmcx_batch_particles": ([vp, P(u64)], i32),
+        "vsmcx_serve_create": ([vp, P(ServeDesc), P(vp)], i32),
+        "vsmcx_serve_destroy": ([vp], i32),
+        "vsmcx_serve_submit": ([vp, P(JobDesc), P(u64)], i32),
+        "vsmcx_serve_observe": ([vp, u64, vp], i32),
+        "vsmcx_serve_poll": ([vp, i32, P(u64)], i32),
+        "vsmcx_serve_job": ([vp, u64, P(i32), P(u64)], i32),
+        "vsmcx_serve_history": ([vp, u64, u64, vp], i32),
+        "vsmcx_serve_forget": ([vp, u64], i32),
+        "vsmcx_serve_stats": ([vp, P(ServeSummary)], i32),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)
@@ -449,3 +485,89 @@ class Batch:
         v = C.c_uint64()
         _check(lib().vsmcx_batch_particles(self.h, C.byref(v)))
         return v.value
+
+
+def lg_matrix(dim=10, seed=0, radius=0.9):
+    """A random stable transition matrix: Gaussian entries scaled to spectral
+    radius `radius`."""
+    g = np.random.default_rng(seed)
+    m = g.standard_normal((dim, dim))
+    return radius * m / np.max(np.abs(np.linalg.eigvals(m)))
+
+
+def lg_simulate(A, T, seed, q=0.1, r=0.2, s0=0.25):
+    """Observations of x_t = A x_{t-1} + q z, y_t = x_t + r v, x_0 ~ N(0, s0^2 I)."""
+    g = np.random.default_rng(seed)
+    d = A.shape[0]
+    x = s0 * g.standard_normal(d)
+    y = np.zeros((T, d))
+    for t in range(T):
+        if t > 0:
+            x = A @ x + q * g.standard_normal(d)
+        y[t] = x + r * g.standard_normal(d)
+    return y
+
+
+class Serve:
+    """The serving loop (vsmcx_serve): jobs admitted under an HBM budget and
+    stepped with continuous batching as their observations arrive."""
+
+    def __init__(self, ctx, hbm_budget, max_jobs, max_horizon, lg_a, lg_q=0.1, lg_r=0.2,
+                 lg_s0=0.25, precision=MIXED, scheme=SYSTEMATIC, threshold=0.5, dt=0.1,
+                 sd_pos=0.1, sd_vel=0.5, prior_sd_pos=2.0, prior_sd_vel=1.0, obs_scale=0.1,
+                 nu=10.0):
+        self.ctx = ctx
+        self.lg_a = np.ascontiguousarray(lg_a, np.float64)
+        self.lg_dim = self.lg_a.shape[0]
+        d = ServeDesc(hbm_budget=int(hbm_budget), max_jobs=max_jobs, max_horizon=max_horizon,
+                      precision=precision, scheme=scheme, threshold=threshold, dt=dt,
+                      sd_pos=sd_pos, sd_vel=sd_vel, prior_sd_pos=prior_sd_pos,
+                      prior_sd_vel=prior_sd_vel, obs_scale=obs_scale, nu=nu,
+                      llh_const=cv_llh_const(nu, obs_scale), lg_dim=self.lg_dim,
+                      lg_a=self.lg_a.ctypes.data, lg_q=lg_q, lg_r=lg_r, lg_s0=lg_s0)
+        self.h = C.c_void_p()
+        _check(lib().vsmcx_serve_create(ctx.h, C.byref(d), C.byref(self.h)))
+
+    def close(self):
+        if self.h:
+            lib().vsmcx_serve_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def submit(self, model, n, horizon, seed, resample_seed):
+        j = JobDesc(model=model, n=n, horizon=horizon, seed=seed, resample_seed=resample_seed)
+        out = C.c_uint64()
+        _check(lib().vsmcx_serve_submit(self.h, C.byref(j), C.byref(out)))
+        return out.value
+
+    def observe(self, job, y):
+        y = np.ascontiguousarray(y, np.float64)
+        _check(lib().vsmcx_serve_observe(self.h, job, y.ctypes.data))
+
+    def poll(self, wait=False):
+        n = C.c_uint64()
+        _check(lib().vsmcx_serve_poll(self.h, 1 if wait else 0, C.byref(n)))
+        return n.value
+
+    def job(self, job):
+        st, steps = C.c_int(), C.c_uint64()
+        _check(lib().vsmcx_serve_job(self.h, job, C.byref(st), C.byref(steps)))
+        return st.value, steps.value
+
+    def history(self, job, count):
+        out = np.zeros(count, STEP_DTYPE)
+        _check(lib().vsmcx_serve_history(self.h, job, count, out.ctypes.data))
+        return out
+
+    def forget(self, job):
+        _check(lib().vsmcx_serve_forget(self.h, job))
+
+    def stats(self):
+        s = ServeSummary()
+        _check(lib().vsmcx_serve_stats(self.h, C.byref(s)))
+        return {k: getattr(s, k) for k, _ in ServeSummary._fields_}

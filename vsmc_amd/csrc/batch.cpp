@@ -20,6 +20,7 @@ struct vsmcx_batch {
     bool fp64 = false;
     int scheme = kSystematic;
     CvParams p{};
+    LgParams lg{};
     std::vector<BatchFilter> hf;
     std::vector<uint64_t> avail_h;
     uint64_t total = 0, total_obs = 0;
@@ -36,7 +37,8 @@ struct vsmcx_batch {
     void release_all()
     {
         void *ps[] = {df, order_work, order_given, P.st0, P.st1, P.lw, P.anc, P.zl, P.sl,
-                      P.rep32, P.spec, P.gval, P.ctl, P.hist, (void *)P.obs, P.next_t,
+                      P.rep32, P.spec, P.gval, P.ts, P.tagg, P.tpre, P.tzs, P.ctl, P.hist,
+                      (void *)P.obs, P.next_t,
                       (void *)P.avail, P.cur, P.err_any, queue};
         for (void *q : ps) ctx->release(q);
         if (ev0) (void)hipEventDestroy(ev0);
@@ -56,7 +58,7 @@ struct vsmcx_batch {
 
     void launch(uint64_t max_steps, unsigned wg, int ord)
     {
-        launch_batch_run(fp64, scheme, df, ord == 0 ? order_work : order_given, nf, P, p,
+        launch_batch_run(fp64, scheme, df, ord == 0 ? order_work : order_given, nf, P, p, lg,
                          max_steps, queue, wg, ctx->stream);
         VSMCX_HIP(hipGetLastError());
     }
@@ -93,15 +95,18 @@ int vsmcx_batch_create(vsmcx_ctx *ctx, const vsmcx_batch_desc *d, vsmcx_batch **
     for (uint32_t f = 0; f < b->nf; ++f) {
         const uint64_t n = d->n[f], h = d->horizon[f];
         REQUIRE(n >= 1 && n <= (uint64_t)kBatchMaxTiles * kFoldTile,
-                "batched filters hold 1 to 2^18 particles");
+                "batched filters hold 1 to 2^20 particles");
         REQUIRE(h >= 1, "horizon must be at least 1");
         BatchFilter &F = b->hf[f];
         F.n = n;
         F.horizon = h;
         F.off = off;
-        F.obs_off = obs;
+        F.soff = 4 * off;
+        F.obs_off = 2 * obs;
         F.hist_off = obs;
         F.seed = d->seed[f];
+        F.model = kModelCv;
+        F.pad = 0;
         off += (n + kFoldTile - 1) / kFoldTile * kFoldTile;
         obs += h;
     }
@@ -119,6 +124,10 @@ int vsmcx_batch_create(vsmcx_ctx *ctx, const vsmcx_batch_desc *d, vsmcx_batch **
         P.rep32 = ctx->alloc_n<uint32_t>(off);
         P.spec = ctx->alloc_n<SpecRec>(off);
         P.gval = ctx->alloc_n<double>(off);
+        P.ts = ctx->alloc_n<double>(off / kFoldTile);
+        P.tagg = ctx->alloc_n<Seg>(off / kFoldTile);
+        P.tpre = ctx->alloc_n<Seg>(off / kFoldTile);
+        P.tzs = ctx->alloc_n<uint64_t>(off / kFoldTile);
         P.ctl = ctx->alloc_n<FilterCtl>(b->nf);
         P.hist = ctx->alloc_zero<StepRow>(obs);
         P.obs = ctx->alloc_zero<double>(2 * obs);
@@ -182,7 +191,7 @@ int vsmcx_batch_set_obs(vsmcx_batch *b, uint64_t f, uint64_t first, uint64_t cou
     const BatchFilter &F = b->hf[f];
     REQUIRE(first + count <= F.horizon, "observations beyond the filter's horizon");
     if (count) {
-        VSMCX_HIP(hipMemcpyAsync((double *)b->P.obs + 2 * (F.obs_off + first), obs,
+        VSMCX_HIP(hipMemcpyAsync((double *)b->P.obs + F.obs_off + 2 * first, obs,
                                  16 * count, hipMemcpyHostToDevice, b->ctx->stream));
         if (first + count > b->avail_h[f]) {
             b->avail_h[f] = first + count;
@@ -297,7 +306,7 @@ int vsmcx_batch_state(vsmcx_batch *b, uint64_t f, double *state, double *log_wei
         const size_t sb = b->state_bytes();
         std::vector<unsigned char> raw(N * sb);
         std::vector<uint32_t> anc(N);
-        const void *src = (const unsigned char *)(cur ? b->P.st1 : b->P.st0) + F.off * sb;
+        const void *src = (const unsigned char *)(cur ? b->P.st1 : b->P.st0) + F.soff * (sb / 4);
         VSMCX_HIP(hipMemcpyAsync(raw.data(), src, N * sb, hipMemcpyDeviceToHost, s));
         VSMCX_HIP(hipMemcpyAsync(anc.data(), b->P.anc + F.off, 4 * N, hipMemcpyDeviceToHost, s));
         b->ctx->sync();

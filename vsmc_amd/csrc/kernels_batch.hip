@@ -15,19 +15,62 @@
 
 namespace vsmcx {
 
+// four N(0,1) from Philox block {t+1, id_lo, id_hi, j} (Box-Muller, as cv_normals)
+__device__ __forceinline__ void lg_normals(float z[4], uint32_t k0, uint32_t k1, uint64_t id,
+                                           uint64_t t, uint32_t j)
+{
+    u32x4 c;
+    c.v[0] = (uint32_t)(t + 1);
+    c.v[1] = (uint32_t)id;
+    c.v[2] = (uint32_t)(id >> 32);
+    c.v[3] = j;
+    const u32x4 w = philox4x32_10(c, k0, k1);
+    const float ln2 = 0.69314718f;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const float u1 = u01_oc_24(w.v[2 * h]);
+        const float u2 = u01_oc_24(w.v[2 * h + 1]);
+        const float r = __builtin_amdgcn_sqrtf(-2.0f * ln2 * __builtin_amdgcn_logf(u1));
+        z[2 * h] = r * __builtin_amdgcn_cosf(u2);
+        z[2 * h + 1] = r * __builtin_amdgcn_sinf(u2);
+    }
+}
+
+__device__ __forceinline__ void lg_normals(double z[4], uint32_t k0, uint32_t k1, uint64_t id,
+                                           uint64_t t, uint32_t j)
+{
+    u32x4 c;
+    c.v[0] = (uint32_t)(t + 1);
+    c.v[1] = (uint32_t)id;
+    c.v[2] = (uint32_t)(id >> 32);
+    c.v[3] = j;
+    const u32x4 w = philox4x32_10(c, k0, k1);
+    const double two_pi = 6.2831853071795865;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const double u1 = u01_oc_53(w.v[2 * h]);
+        const double u2 = u01_oc_53(w.v[2 * h + 1]);
+        const double r = sqrt(-2 * log(u1));
+        z[2 * h] = r * cos(two_pi * u2);
+        z[2 * h + 1] = r * sin(two_pi * u2);
+    }
+}
+
+template <typename Real>
+__device__ __forceinline__ Real lg_a(const LgParams &lp, int k);
+template <>
+__device__ __forceinline__ float lg_a<float>(const LgParams &lp, int k) { return lp.a[k]; }
+template <>
+__device__ __forceinline__ double lg_a<double>(const LgParams &lp, int k) { return lp.ad[k]; }
+
 template <typename Inv>
 struct BatchLds {
     FoldLdsT<Inv::kWindow> L;
-    double ts[kBatchMaxTiles];
-    Seg agg[kBatchMaxTiles];
-    Seg pre[kBatchMaxTiles];
-    uint64_t zs[kBatchMaxTiles];
-    uint64_t zsx[kBatchMaxTiles];
-    uint32_t cnt[kBatchMaxTiles];
     SpecRec r[kBlock];
     double v[kBlock];
     Part5 p5[kBlock / kWave];
     Seg tail;
+    Seg carry;
     uint64_t zeros;
     unsigned job;
     int flag;
@@ -45,22 +88,22 @@ struct SchemeOf<InvStratified> {
 };
 
 // Sum over the current state of W_i (x_i, y_i) -- MonitorMCMC's mean after the
-// last step (core/monitor.hpp:274-302).
+// last step (core/monitor.hpp:274-302); x, y are the first two coordinates.
 template <typename Real, typename Inv>
-__device__ void batch_final_monitor(const FilterCtl *ctl, const typename StateOf<Real>::type *st,
-                                    const uint32_t *anc, const double *lw, uint64_t N,
-                                    StepRow *row, BatchLds<Inv> &S)
+__device__ __noinline__ void batch_final_monitor(const FilterCtl *ctl, const Real *st, int dim,
+                                                 const uint32_t *anc, const double *lw,
+                                                 uint64_t N, StepRow *row, BatchLds<Inv> &S)
 {
     const int prev = ctl->prev_resampled;
     const double m = ctl->m, c = ctl->coeff, invN = 1.0 / (double)N;
     Part5 v{0, 0, 0, 0};
 #pragma unroll 1
     for (uint64_t i = threadIdx.x; i < N; i += kBlock) {
-        const uint64_t src = prev ? (uint64_t)anc[i] : i;
-        const auto s = st[src < N ? src : i];
+        uint64_t src = prev ? (uint64_t)anc[i] : i;
+        src = src < N ? src : i;
         const double w = prev ? invN : exp(lw[i] - m) * c;
-        v.mx += w * (double)s.x;
-        v.my += w * (double)s.y;
+        v.mx += w * (double)st[src * dim];
+        v.my += w * (double)st[src * dim + 1];
     }
     v = block_reduce<kBlock>(v, Part5Add(), S.p5);
     if (threadIdx.x == 0) {
@@ -69,21 +112,40 @@ __device__ void batch_final_monitor(const FilterCtl *ctl, const typename StateOf
     }
 }
 
+// The reductions that end every move (thread 0 writes the step's scalars).
+template <typename Inv>
+__device__ __forceinline__ void batch_move_end(FilterCtl *ctl, const BatchPool &P, uint32_t f,
+                                               uint32_t cur, StepRow *hist, uint64_t t, double m,
+                                               double se, double se2, double mx, double my,
+                                               BatchLds<Inv> &S)
+{
+    const double M = block_reduce<kBlock>(m, MaxOp(), S.L.red);
+    const double sc = (m == -INFINITY) ? 0.0 : exp(m - M);
+    Part5 v{se * sc, se2 * sc * sc, mx, my};
+    v = block_reduce<kBlock>(v, Part5Add(), S.p5);
+    if (threadIdx.x == 0) {
+        cv_step_scalars(ctl, M, v.se, v.se2, v.mx, v.my, hist, t);
+        P.cur[f] = 1 - cur;
+        S.flag = ctl->resample;
+        if (ctl->resample) resample_prologue(ctl, SchemeOf<Inv>::value);
+    }
+    __syncthreads();
+}
+
 // The phases of a step are separate (not inlined) functions: each gets its own
 // register allocation, and what crosses phases lives in LDS, so the kernel
 // keeps the occupancy of its largest phase instead of the sum of all.
 
-// move, likelihood, log-sum-exp, decision (thread 0 writes the scalars)
+// CV model: move, likelihood, log-sum-exp, decision
 template <typename Real, typename Inv>
-__device__ __noinline__ void batch_move(const BatchFilter &F, uint32_t f, const BatchPool &P,
-                                        const CvParams &p, uint64_t t, BatchLds<Inv> &S)
+__device__ __noinline__ void batch_move_cv(const BatchFilter &F, uint32_t f, const BatchPool &P,
+                                           const CvParams &p, uint64_t t, BatchLds<Inv> &S)
 {
     typedef typename StateOf<Real>::type S4;
     const int tid = threadIdx.x;
     FilterCtl *ctl = P.ctl + f;
-    StepRow *hist = P.hist + F.hist_off;
     const uint64_t N = F.n;
-    S4 *sa = (S4 *)P.st0 + F.off, *sb = (S4 *)P.st1 + F.off;
+    S4 *sa = (S4 *)((Real *)P.st0 + F.soff), *sb = (S4 *)((Real *)P.st1 + F.soff);
     const uint32_t cur = P.cur[f];
     const S4 *in = cur ? sb : sa;
     S4 *out = cur ? sa : sb;
@@ -94,7 +156,7 @@ __device__ __noinline__ void batch_move(const BatchFilter &F, uint32_t f, const 
     const int prev = t > 0 ? ctl->prev_resampled : 0;
     const double m_old = ctl->m, c_old = ctl->coeff;
     const double invN = 1.0 / (double)N;
-    const Real ox = (Real)P.obs[2 * (F.obs_off + t)], oy = (Real)P.obs[2 * (F.obs_off + t) + 1];
+    const Real ox = (Real)P.obs[F.obs_off + 2 * t], oy = (Real)P.obs[F.obs_off + 2 * t + 1];
     const uint32_t k0 = (uint32_t)F.seed, k1 = (uint32_t)(F.seed >> 32);
     double m = -INFINITY, se = 0, se2 = 0, mx = 0, my = 0;
 #pragma unroll 1
@@ -118,17 +180,75 @@ __device__ __noinline__ void batch_move(const BatchFilter &F, uint32_t f, const 
         lw[i] = lwn;
         lse_add(m, se, se2, lwn);
     }
-    const double M = block_reduce<kBlock>(m, MaxOp(), S.L.red);
-    const double sc = (m == -INFINITY) ? 0.0 : exp(m - M);
-    Part5 v{se * sc, se2 * sc * sc, mx, my};
-    v = block_reduce<kBlock>(v, Part5Add(), S.p5);
-    if (tid == 0) {
-        cv_step_scalars(ctl, M, v.se, v.se2, v.mx, v.my, hist, t);
-        P.cur[f] = 1 - cur;
-        S.flag = ctl->resample;
-        if (ctl->resample) resample_prologue(ctl, SchemeOf<Inv>::value);
+    batch_move_end(ctl, P, f, cur, P.hist + F.hist_off, t, m, se, se2, mx, my, S);
+}
+
+// Linear-Gaussian model: x_t = A x_{t-1} + q z (x_0 = s0 z), y_t = x_t + r v;
+// the normals of particle i at step t come from Philox blocks
+// {t+1, id_lo, id_hi, j}, j = 0, 1, 2 (four normals each).
+template <typename Real, typename Inv>
+__device__ __noinline__ void batch_move_lg(const BatchFilter &F, uint32_t f, const BatchPool &P,
+                                           const LgParams &lp, uint64_t t, BatchLds<Inv> &S)
+{
+    const int tid = threadIdx.x;
+    FilterCtl *ctl = P.ctl + f;
+    const uint64_t N = F.n;
+    const int D = lp.dim;
+    Real *sa = (Real *)P.st0 + F.soff, *sb = (Real *)P.st1 + F.soff;
+    const uint32_t cur = P.cur[f];
+    const Real *in = cur ? sb : sa;
+    Real *out = cur ? sa : sb;
+    double *lw = P.lw + F.off;
+    const uint32_t *anc = P.anc + F.off;
+    const int prev = t > 0 ? ctl->prev_resampled : 0;
+    const double m_old = ctl->m, c_old = ctl->coeff;
+    const double invN = 1.0 / (double)N;
+    const double *y = P.obs + F.obs_off + (uint64_t)D * t;
+    const Real q = (Real)lp.q, s0 = (Real)lp.s0, inv_r2 = (Real)(1 / (lp.r * lp.r));
+    const uint32_t k0 = (uint32_t)F.seed, k1 = (uint32_t)(F.seed >> 32);
+    double m = -INFINITY, se = 0, se2 = 0, mx = 0, my = 0;
+#pragma unroll 1
+    for (uint64_t i = tid; i < N; i += kBlock) {
+        Real x[kLgMaxDim], xn[kLgMaxDim];
+        double lwo = 0;
+        if (t > 0) {
+            uint64_t src = prev ? (uint64_t)anc[i] : i;
+            src = src < N ? src : i;
+#pragma unroll
+            for (int d = 0; d < kLgMaxDim; ++d) x[d] = d < D ? in[src * D + d] : (Real)0;
+            if (!prev) lwo = lw[i];
+            const double w = prev ? invN : exp(lwo - m_old) * c_old;
+            mx += w * (double)x[0];
+            my += w * (double)x[1];
+        }
+        Real z[12];
+#pragma unroll
+        for (int j = 0; j < 3; ++j) lg_normals(z + 4 * j, k0, k1, i, t, (uint32_t)j);
+        Real ss = 0;
+#pragma unroll
+        for (int d = 0; d < kLgMaxDim; ++d) {
+            if (d >= D) break;
+            Real v;
+            if (t == 0) {
+                v = s0 * z[d];
+            } else {
+                v = q * z[d];
+#pragma unroll
+                for (int e = 0; e < kLgMaxDim; ++e)
+                    if (e < D) v += lg_a<Real>(lp, d * D + e) * x[e];
+            }
+            xn[d] = v;
+            const Real r = (Real)y[d] - v;
+            ss += r * r;
+        }
+        const double lwn = lwo + (lp.llh_const - 0.5 * (double)(ss * inv_r2));
+#pragma unroll
+        for (int d = 0; d < kLgMaxDim; ++d)
+            if (d < D) out[i * D + d] = xn[d];
+        lw[i] = lwn;
+        lse_add(m, se, se2, lwn);
     }
-    __syncthreads();
+    batch_move_end(ctl, P, f, cur, P.hist + F.hist_off, t, m, se, se2, mx, my, S);
 }
 
 // tile sums and their exclusive scan (approximate tile starts)
@@ -140,15 +260,24 @@ __device__ __noinline__ void batch_fold_sums(const BatchFilter &F, FilterCtl *ct
     const uint64_t N = F.n, T = (N + kFT - 1) / kFT;
     XWeight xs{P.lw + F.off, 0.0, 0.0};
     xs.bind(ctl);
+    double *ts = P.ts + F.off / kFT;
 #pragma unroll 1
     for (uint64_t tt = 0; tt < T; ++tt) {
         const double s = fold_tile_sum(xs, tt * (uint64_t)kFT, N, S.L.red);
-        if (tid == 0) S.ts[tt] = s;
+        if (tid == 0) ts[tt] = s;
     }
     __syncthreads();
-    const double a = tid < (int)T ? S.ts[tid] : 0.0;
-    const double ex = block_excl_scan<kBlock>(a, AddOp(), 0.0, S.L.red, (double *)nullptr);
-    if (tid < (int)T) S.ts[tid] = ex;
+    // exclusive scan over the tiles, kBlock at a time, in a fixed order
+    double carry = 0.0;
+#pragma unroll 1
+    for (uint64_t c0 = 0; c0 < T; c0 += kBlock) {
+        const uint64_t i = c0 + tid;
+        const double a = i < T ? ts[i] : 0.0;
+        double tot;
+        const double ex = block_excl_scan<kBlock>(a, AddOp(), 0.0, S.L.red, &tot);
+        if (i < T) ts[i] = carry + ex;
+        carry += tot;
+    }
     __syncthreads();
 }
 
@@ -161,18 +290,27 @@ __device__ __noinline__ void batch_fold_segs(const BatchFilter &F, FilterCtl *ct
     const uint64_t N = F.n, T = (N + kFT - 1) / kFT;
     XWeight xs{P.lw + F.off, 0.0, 0.0};
     xs.bind(ctl);
+    const double *ts = P.ts + F.off / kFT;
+    Seg *agg = P.tagg + F.off / kFT, *pre = P.tpre + F.off / kFT;
 #pragma unroll 1
     for (uint64_t tt = 0; tt < T; ++tt)
-        fold_tile_seg(xs, tt * (uint64_t)kFT, N, S.ts[tt], P.spec + F.off + tt * (uint64_t)kFT,
-                      &S.agg[tt], S.L);
+        fold_tile_seg(xs, tt * (uint64_t)kFT, N, ts[tt], P.spec + F.off + tt * (uint64_t)kFT,
+                      agg + tt, S.L);
     __syncthreads();
-    const Seg a = tid < (int)T ? S.agg[tid] : seg_identity();
-    const Seg ex = block_excl_scan<kBlock>(a, SegOp(), seg_identity(), S.L.seg, (Seg *)nullptr);
-    if (tid < (int)T) {
-        S.pre[tid] = ex;
-        S.cnt[tid] = (uint32_t)ex.cnt;
+    if (tid == 0) S.carry = seg_identity();
+    __syncthreads();
+#pragma unroll 1
+    for (uint64_t c0 = 0; c0 < T; c0 += kBlock) {
+        const uint64_t i = c0 + tid;
+        const Seg a = i < T ? agg[i] : seg_identity();
+        const Seg ex = block_excl_scan<kBlock>(a, SegOp(), seg_identity(), S.L.seg, (Seg *)nullptr);
+        const Seg c = S.carry;
+        if (i < T) pre[i] = seg_op(c, ex);
+        __syncthreads();
+        if (tid == kBlock - 1) S.carry = seg_op(c, seg_op(ex, a));
+        __syncthreads();
     }
-    if (tid == kBlock - 1) S.tail = seg_op(ex, a);
+    if (tid == 0) S.tail = S.carry;
     __syncthreads();
 }
 
@@ -184,8 +322,8 @@ __device__ __noinline__ void batch_fold_chain(const BatchFilter &F, FilterCtl *c
     const uint64_t T = (F.n + kFT - 1) / kFT;
     int bad;
     const Seg tail = S.tail;
-    const double tot = fold_chain<kBlock>(S.pre, S.cnt, T, tail, P.spec + F.off, P.gval + F.off,
-                                          S.r, S.v, &bad);
+    const double tot = fold_chain<kBlock>(P.tpre + F.off / kFT, nullptr, T, tail, P.spec + F.off,
+                                          P.gval + F.off, S.r, S.v, &bad);
     if (threadIdx.x == 0) {
         ctl->fold_total = tot;
         ctl->specials = tail.cnt;
@@ -207,16 +345,23 @@ __device__ __noinline__ void batch_fold_apply(const BatchFilter &F, FilterCtl *c
     const FoldOut fo{P.anc + F.off, P.zl + F.off, P.sl + F.off, nullptr, nullptr};
 #pragma unroll 1
     for (uint64_t tt = 0; tt < T; ++tt)
-        fold_tile_apply(ctl, xs, inv, tt, N, S.ts[tt], S.pre[tt], P.gval + F.off, fo,
-                        P.rep32 + F.off, &S.zs[tt], S.L);
+        fold_tile_apply(ctl, xs, inv, tt, N, P.ts[F.off / kFT + tt], P.tpre[F.off / kFT + tt],
+                        P.gval + F.off, fo, P.rep32 + F.off, P.tzs + F.off / kFT + tt, S.L);
     __syncthreads();
-    const uint64_t a = tid < (int)T ? S.zs[tid] : 0;
-    uint64_t total;
-    const uint64_t ex = block_excl_scan<kBlock>(a, AddOp(), (uint64_t)0, S.L.u64, &total);
-    if (tid < (int)T) S.zsx[tid] = ex;
+    uint64_t *zs = P.tzs + F.off / kFT;
+    uint64_t carry = 0;
+#pragma unroll 1
+    for (uint64_t c0 = 0; c0 < T; c0 += kBlock) {
+        const uint64_t i = c0 + tid;
+        const uint64_t a = i < T ? zs[i] : 0;
+        uint64_t tot;
+        const uint64_t ex = block_excl_scan<kBlock>(a, AddOp(), (uint64_t)0, S.L.u64, &tot);
+        if (i < T) zs[i] = carry + ex;  // in place: exclusive zero/surplus ranks
+        carry += tot;
+    }
     if (tid == 0) {
-        S.zeros = total >> 32;
-        ctl->zeros = total >> 32;
+        S.zeros = carry >> 32;
+        ctl->zeros = carry >> 32;
     }
     __syncthreads();
 }
@@ -232,7 +377,7 @@ __device__ __noinline__ void batch_fold_layout(const BatchFilter &F, FilterCtl *
     const FoldOut fo{anc, P.zl + F.off, P.sl + F.off, nullptr, nullptr};
 #pragma unroll 1
     for (uint64_t tt = 0; tt < T; ++tt)
-        fold_tile_layout(ctl, tt, N, P.rep32 + F.off, S.zsx[tt], fo, S.L);
+        fold_tile_layout(ctl, tt, N, P.rep32 + F.off, P.tzs[F.off / kFT + tt], fo, S.L);
     __syncthreads();
     const uint64_t z = S.zeros < N ? S.zeros : N;
 #pragma unroll 1
@@ -251,10 +396,13 @@ __device__ __noinline__ void batch_fold_layout(const BatchFilter &F, FilterCtl *
 // One step t of filter f.
 template <typename Real, typename Inv>
 __device__ void batch_step(const BatchFilter &F, uint32_t f, const BatchPool &P,
-                           const CvParams &p, uint64_t t, BatchLds<Inv> &S)
+                           const CvParams &p, const LgParams &lp, uint64_t t, BatchLds<Inv> &S)
 {
     FilterCtl *ctl = P.ctl + f;
-    batch_move<Real, Inv>(F, f, P, p, t, S);
+    if (F.model == kModelLg)
+        batch_move_lg<Real, Inv>(F, f, P, lp, t, S);
+    else
+        batch_move_cv<Real, Inv>(F, f, P, p, t, S);
     if (S.flag) {
         batch_fold_sums(F, ctl, P, S);
         batch_fold_segs(F, ctl, P, S);
@@ -264,9 +412,9 @@ __device__ void batch_step(const BatchFilter &F, uint32_t f, const BatchPool &P,
     }
     if (t + 1 == F.horizon) {
         __syncthreads();
-        typedef typename StateOf<Real>::type S4;
-        const S4 *cur_state = (P.cur[f] ? (const S4 *)P.st1 : (const S4 *)P.st0) + F.off;
-        batch_final_monitor<Real, Inv>(ctl, cur_state, P.anc + F.off, P.lw + F.off, F.n,
+        const int dim = F.model == kModelLg ? lp.dim : 4;
+        const Real *cur_state = (P.cur[f] ? (const Real *)P.st1 : (const Real *)P.st0) + F.soff;
+        batch_final_monitor<Real, Inv>(ctl, cur_state, dim, P.anc + F.off, P.lw + F.off, F.n,
                                        P.hist + F.hist_off + t, S);
     }
     if (threadIdx.x == 0) {
@@ -281,8 +429,8 @@ __device__ void batch_step(const BatchFilter &F, uint32_t f, const BatchPool &P,
 template <typename Real, typename Inv>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_batch_run(const BatchFilter *filters,
                                                       const uint32_t *order, uint32_t nf,
-                                                      BatchPool P, CvParams p, uint64_t max_steps,
-                                                      unsigned *queue)
+                                                      BatchPool P, CvParams p, LgParams lp,
+                                                      uint64_t max_steps, unsigned *queue)
 {
     __shared__ BatchLds<Inv> S;
     for (;;) {
@@ -297,7 +445,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8))) 
         uint64_t t1 = P.avail[f] < F.horizon ? P.avail[f] : F.horizon;
         if (t1 > t0 + max_steps) t1 = t0 + max_steps;
 #pragma unroll 1
-        for (uint64_t t = t0; t < t1; ++t) batch_step<Real, Inv>(F, f, P, p, t, S);
+        for (uint64_t t = t0; t < t1; ++t) batch_step<Real, Inv>(F, f, P, p, lp, t, S);
     }
 }
 
@@ -307,25 +455,25 @@ __global__ void k_batch_queue_reset(unsigned *queue)
 }
 
 void launch_batch_run(bool fp64, int scheme, const BatchFilter *filters, const uint32_t *order,
-                      uint32_t nf, const BatchPool &P, const CvParams &p, uint64_t max_steps,
-                      unsigned *queue, unsigned grid, hipStream_t st)
+                      uint32_t nf, const BatchPool &P, const CvParams &p, const LgParams &lg,
+                      uint64_t max_steps, unsigned *queue, unsigned grid, hipStream_t st)
 {
     hipLaunchKernelGGL(k_batch_queue_reset, dim3(1), dim3(64), 0, st, queue);
     const dim3 g(grid ? grid : 1), b(kBlock);
     if (fp64) {
         if (scheme == kStratified)
             hipLaunchKernelGGL((k_batch_run<double, InvStratified>), g, b, 0, st, filters, order,
-                               nf, P, p, max_steps, queue);
+                               nf, P, p, lg, max_steps, queue);
         else
             hipLaunchKernelGGL((k_batch_run<double, InvSystematic>), g, b, 0, st, filters, order,
-                               nf, P, p, max_steps, queue);
+                               nf, P, p, lg, max_steps, queue);
     } else {
         if (scheme == kStratified)
             hipLaunchKernelGGL((k_batch_run<float, InvStratified>), g, b, 0, st, filters, order,
-                               nf, P, p, max_steps, queue);
+                               nf, P, p, lg, max_steps, queue);
         else
             hipLaunchKernelGGL((k_batch_run<float, InvSystematic>), g, b, 0, st, filters, order,
-                               nf, P, p, max_steps, queue);
+                               nf, P, p, lg, max_steps, queue);
     }
 }
 

@@ -65,28 +65,48 @@ uint64_t plain_partials(uint64_t N);
 
 
 // ---- batched filters (kernels_batch.hip) --------------------------------------------
-constexpr int kBatchMaxTiles = 128;  // filters of up to 128 * kFoldTile = 2^18 particles
+constexpr int kBatchMaxTiles = 512;  // filters of up to 512 * kFoldTile = 2^20 particles
 
 // One filter of a batch (host-built, uploaded once).
 struct BatchFilter {
     uint64_t n;         // particles
     uint64_t horizon;   // steps
-    uint64_t off;       // first particle in the pooled arrays (a multiple of kFoldTile)
-    uint64_t obs_off;   // first observation row
+    uint64_t off;       // first particle in the pooled per-particle arrays (a multiple of kFoldTile)
+    uint64_t soff;      // first state element (Real units) in the pooled state arrays
+    uint64_t obs_off;   // first observation element (doubles)
     uint64_t hist_off;  // first history row
     uint64_t seed;      // Philox4x32 key of the move stream
+    int32_t model;      // kModelCv or kModelLg
+    int32_t pad;
 };
 
-// Pooled device arrays of a batch; per-particle arrays are indexed off + i.
+enum : int32_t { kModelCv = 0, kModelLg = 1 };
+
+// Linear-Gaussian model: x_t = A x_{t-1} + q z, y_t = x_t + r v, x_0 ~ N(0, s0^2 I)
+constexpr int kLgMaxDim = 10;
+struct LgParams {
+    int32_t dim;
+    int32_t pad;
+    double q, r, s0;
+    double llh_const;   // -dim * log(r sqrt(2 pi))
+    float a[kLgMaxDim * kLgMaxDim];   // row-major A (FP32 copy)
+    double ad[kLgMaxDim * kLgMaxDim]; // row-major A (FP64)
+};
+
+// Pooled device arrays of a batch; per-particle arrays are indexed off + i,
+// states soff + i * dim.
 struct BatchPool {
-    void *st0, *st1;       // states (S4f or S4d), ping-pong
+    void *st0, *st1;       // states (float or double elements), ping-pong
     double *lw;
     uint32_t *anc, *zl, *sl, *rep32;
     SpecRec *spec;
     double *gval;
+    double *ts;            // per tile (index off / kFoldTile + tile): approximate start
+    Seg *tagg, *tpre;      // per tile: segment summary and exclusive prefix
+    uint64_t *tzs;         // per tile: zeros << 32 | surplus, then their exclusive prefix
     FilterCtl *ctl;        // [filters]
     StepRow *hist;         // history rows
-    const double *obs;     // observation rows (x, y)
+    const double *obs;     // observations (CV: 2 per step, LG: dim per step)
     uint64_t *next_t;      // [filters] next step
     const uint64_t *avail; // [filters] observations available
     uint32_t *cur;         // [filters] state buffer holding the current state
@@ -94,7 +114,7 @@ struct BatchPool {
 };
 
 void launch_batch_run(bool fp64, int scheme, const BatchFilter *filters, const uint32_t *order,
-                      uint32_t nf, const BatchPool &P, const CvParams &p, uint64_t max_steps,
-                      unsigned *queue, unsigned grid, hipStream_t st);
+                      uint32_t nf, const BatchPool &P, const CvParams &p, const LgParams &lg,
+                      uint64_t max_steps, unsigned *queue, unsigned grid, hipStream_t st);
 
 }  // namespace vsmcx

@@ -1,0 +1,524 @@
+// serve.cpp -- the serving loop (vsmcx_serve_*): filtering jobs that arrive
+// over time, admitted against an HBM budget, kept resident on the device
+// across steps, and advanced with continuous batching: every tick is one
+// persistent launch of the batched kernel over all jobs that have an
+// observation waiting; jobs admitted while a tick runs join the next one.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/vsmcx.h"
+#include "host.hpp"
+#include "internal.hpp"
+#include "launch.hpp"
+
+namespace {
+
+double now_us()
+{
+    using namespace std::chrono;
+    return (double)duration_cast<nanoseconds>(steady_clock::now().time_since_epoch()).count() * 1e-3;
+}
+
+// First-fit allocator over [0, cap) in whole units, coalescing on free.
+class RangeAlloc {
+  public:
+    explicit RangeAlloc(uint64_t cap = 0) { reset(cap); }
+    void reset(uint64_t cap)
+    {
+        free_.clear();
+        if (cap) free_[0] = cap;
+        cap_ = cap;
+        used_ = 0;
+    }
+    bool alloc(uint64_t n, uint64_t *off)
+    {
+        for (auto it = free_.begin(); it != free_.end(); ++it) {
+            if (it->second >= n) {
+                *off = it->first;
+                const uint64_t rest = it->second - n, start = it->first + n;
+                free_.erase(it);
+                if (rest) free_[start] = rest;
+                used_ += n;
+                return true;
+            }
+        }
+        return false;
+    }
+    void release(uint64_t off, uint64_t n)
+    {
+        used_ -= n;
+        auto it = free_.emplace(off, n).first;
+        auto next = std::next(it);
+        if (next != free_.end() && it->first + it->second == next->first) {
+            it->second += next->second;
+            free_.erase(next);
+        }
+        if (it != free_.begin()) {
+            auto prev = std::prev(it);
+            if (prev->first + prev->second == it->first) {
+                prev->second += it->second;
+                free_.erase(it);
+            }
+        }
+    }
+    uint64_t used() const { return used_; }
+
+  private:
+    std::map<uint64_t, uint64_t> free_;
+    uint64_t cap_ = 0, used_ = 0;
+};
+
+struct Job {
+    vsmcx_job_desc d{};
+    int state = 0;  // 0 queued, 1 running, 2 done
+    int slot = -1;
+    uint64_t done = 0;      // steps completed
+    uint64_t received = 0;  // observations received
+    uint64_t uploaded = 0;  // observations copied to the device
+    uint64_t p_off = 0, p_units = 0, s_off = 0, s_units = 0, o_off = 0, o_units = 0;
+    std::vector<double> obs;        // every observation received (host copy)
+    std::vector<double> arrival;    // arrival time of each observation (us)
+    std::vector<StepRow> history;   // filled when the job finishes
+};
+
+constexpr uint64_t kUnit = kFoldTile;  // particle-arena unit
+const uint64_t kZero64 = 0;
+const uint32_t kZero32 = 0;
+
+}  // namespace
+
+struct vsmcx_serve {
+    vsmcx_ctx *ctx = nullptr;
+    bool fp64 = false;
+    int scheme = kSystematic;
+    double threshold = 0.5;
+    CvParams cv{};
+    LgParams lg{};
+    uint32_t max_jobs = 0;
+    uint64_t max_horizon = 0;
+    RangeAlloc parts, states, obsr;
+    uint64_t cap_units = 0, cap_state = 0, cap_obs = 0;
+    BatchPool P{};
+    BatchFilter *df = nullptr;
+    uint32_t *order = nullptr;
+    unsigned *queue = nullptr;
+    unsigned grid_cap = 0;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::vector<int> free_slots;
+    std::vector<uint64_t> slot_job;
+    std::vector<BatchFilter> hF;  // per-slot host mirrors (copy sources)
+    std::vector<FilterCtl> hC;
+    std::unordered_map<uint64_t, Job> jobs;
+    std::deque<uint64_t> waiting;
+    uint64_t next_id = 1;
+    bool inflight = false;
+    std::vector<uint32_t> tick;
+    std::vector<double> lat_us;
+    uint64_t steps = 0, updates = 0, jobs_done = 0;
+    double busy_ms = 0, t_first = -1, t_last = 0;
+    uint64_t peak_units = 0;
+
+    int obs_dim(int model) const { return model == VSMCX_MODEL_LG ? lg.dim : 2; }
+    int state_dim(int model) const { return model == VSMCX_MODEL_LG ? lg.dim : 4; }
+    size_t real_bytes() const { return fp64 ? 8 : 4; }
+
+    void release_all()
+    {
+        void *ps[] = {df, order, queue, P.st0, P.st1, P.lw, P.anc, P.zl, P.sl, P.rep32,
+                      P.spec, P.gval, P.ts, P.tagg, P.tpre, P.tzs, P.ctl, P.hist, (void *)P.obs,
+                      P.next_t, (void *)P.avail,
+                      P.cur, P.err_any};
+        for (void *q : ps) ctx->release(q);
+        if (ev0) (void)hipEventDestroy(ev0);
+        if (ev1) (void)hipEventDestroy(ev1);
+    }
+
+    // admit waiting jobs in arrival order while slots and memory last
+    void admit()
+    {
+        hipStream_t s = ctx->stream;
+        while (!waiting.empty() && !free_slots.empty()) {
+            Job &J = jobs[waiting.front()];
+            const uint64_t units = (J.d.n + kUnit - 1) / kUnit;
+            const uint64_t sunits = units * kUnit * state_dim(J.d.model);
+            const uint64_t ounits = J.d.horizon * obs_dim(J.d.model);
+            uint64_t po, so, oo;
+            if (!parts.alloc(units, &po)) return;
+            if (!states.alloc(sunits, &so)) {
+                parts.release(po, units);
+                return;
+            }
+            if (!obsr.alloc(ounits, &oo)) {
+                parts.release(po, units);
+                states.release(so, sunits);
+                return;
+            }
+            const int slot = free_slots.back();
+            free_slots.pop_back();
+            J.slot = slot;
+            J.state = 1;
+            J.p_off = po;
+            J.p_units = units;
+            J.s_off = so;
+            J.s_units = sunits;
+            J.o_off = oo;
+            J.o_units = ounits;
+            slot_job[slot] = waiting.front();
+            waiting.pop_front();
+            peak_units = std::max(peak_units, parts.used());
+            BatchFilter &F = hF[slot];
+            F = BatchFilter{};
+            F.n = J.d.n;
+            F.horizon = J.d.horizon;
+            F.off = po * kUnit;
+            F.soff = so;
+            F.obs_off = oo;
+            F.hist_off = (uint64_t)slot * max_horizon;
+            F.seed = J.d.seed;
+            F.model = J.d.model == VSMCX_MODEL_LG ? kModelLg : kModelCv;
+            FilterCtl &h = hC[slot];
+            h = FilterCtl{};
+            h.n = J.d.n;
+            h.threshold = threshold;
+            h.scheme = scheme;
+            h.eng.key[0] = J.d.resample_seed;
+            h.eng.index = 4;
+            // sources are per-slot host mirrors (stable until the slot is reused,
+            // which is after this job's last tick has completed)
+            VSMCX_HIP(hipMemcpyAsync(df + slot, &F, sizeof(F), hipMemcpyHostToDevice, s));
+            VSMCX_HIP(hipMemcpyAsync(P.ctl + slot, &h, sizeof(h), hipMemcpyHostToDevice, s));
+            VSMCX_HIP(hipMemcpyAsync(P.next_t + slot, &kZero64, 8, hipMemcpyHostToDevice, s));
+            VSMCX_HIP(hipMemcpyAsync((uint64_t *)P.avail + slot, &kZero64, 8, hipMemcpyHostToDevice, s));
+            VSMCX_HIP(hipMemcpyAsync(P.cur + slot, &kZero32, 4, hipMemcpyHostToDevice, s));
+            upload(J);
+        }
+    }
+
+    // observations received but not yet on the device
+    void upload(Job &J)
+    {
+        if (J.state != 1 || J.uploaded == J.received) return;
+        const int D = obs_dim(J.d.model);
+        hipStream_t s = ctx->stream;
+        VSMCX_HIP(hipMemcpyAsync((double *)P.obs + J.o_off + (uint64_t)D * J.uploaded,
+                                 J.obs.data() + (size_t)D * J.uploaded,
+                                 8 * D * (J.received - J.uploaded), hipMemcpyHostToDevice, s));
+        J.uploaded = J.received;
+        // J lives in a node-based map and J.obs was reserved for the whole
+        // horizon, so both sources stay valid until the copies run
+        VSMCX_HIP(hipMemcpyAsync((uint64_t *)P.avail + J.slot, &J.uploaded, 8,
+                                 hipMemcpyHostToDevice, s));
+    }
+
+    // the tick in flight has finished: account for it, retire finished jobs
+    void complete()
+    {
+        const double t = now_us();
+        float ms = 0;
+        VSMCX_HIP(hipEventElapsedTime(&ms, ev0, ev1));
+        busy_ms += ms;
+        int e = 0;
+        VSMCX_HIP(hipMemcpy(&e, P.err_any, sizeof(int), hipMemcpyDeviceToHost));
+        if (e & kErrDegenerate) throw DevError(VSMCX_ERR_DEGENERATE, "a job's weights sum to zero or are not finite");
+        if (e) throw DevError(VSMCX_ERR_INTERNAL, "device consistency check failed (bits " + std::to_string(e) + ")");
+        for (uint32_t slot : tick) {
+            Job &J = jobs[slot_job[slot]];
+            lat_us.push_back(t - J.arrival[J.done]);
+            J.done += 1;
+            steps += 1;
+            updates += J.d.n;
+            if (J.done == J.d.horizon) {
+                J.history.resize(J.d.horizon);
+                VSMCX_HIP(hipMemcpy(J.history.data(), P.hist + (uint64_t)slot * max_horizon,
+                                    sizeof(StepRow) * J.d.horizon, hipMemcpyDeviceToHost));
+                parts.release(J.p_off, J.p_units);
+                states.release(J.s_off, J.s_units);
+                obsr.release(J.o_off, J.o_units);
+                free_slots.push_back(slot);
+                slot_job[slot] = 0;
+                J.state = 2;
+                J.slot = -1;
+                jobs_done += 1;
+            }
+        }
+        tick.clear();
+        inflight = false;
+        t_last = t;
+    }
+
+    // one tick: every running job with an observation on the device and not
+    // yet stepped
+    uint64_t launch()
+    {
+        tick.clear();
+        for (size_t slot = 0; slot < slot_job.size(); ++slot) {
+            if (!slot_job[slot]) continue;
+            const Job &J = jobs[slot_job[slot]];
+            if (J.state == 1 && J.uploaded > J.done) tick.push_back((uint32_t)slot);
+        }
+        if (tick.empty()) return 0;
+        hipStream_t s = ctx->stream;
+        VSMCX_HIP(hipMemcpyAsync(order, tick.data(), 4 * tick.size(), hipMemcpyHostToDevice, s));
+        VSMCX_HIP(hipEventRecord(ev0, s));
+        launch_batch_run(fp64, scheme, df, order, (uint32_t)tick.size(), P, cv, lg, 1, queue,
+                         std::min<unsigned>((unsigned)tick.size(), grid_cap), s);
+        VSMCX_HIP(hipGetLastError());
+        VSMCX_HIP(hipEventRecord(ev1, s));
+        inflight = true;
+        if (t_first < 0) t_first = now_us();
+        return tick.size();
+    }
+};
+
+int vsmcx_serve_create(vsmcx_ctx *ctx, const vsmcx_serve_desc *d, vsmcx_serve **out)
+{
+    API_BEGIN
+    REQUIRE(ctx && d && out, "NULL argument");
+    REQUIRE(d->precision == VSMCX_MIXED || d->precision == VSMCX_FP64_FUSED,
+            "the serving loop runs the mixed or the fused FP64 pipeline");
+    REQUIRE(d->scheme == VSMCX_SYSTEMATIC || d->scheme == VSMCX_STRATIFIED,
+            "the serving loop resamples with the systematic or the stratified scheme");
+    REQUIRE(d->max_jobs >= 1 && d->max_horizon >= 1, "max_jobs and max_horizon must be positive");
+    REQUIRE(d->lg_dim >= 1 && d->lg_dim <= (uint32_t)kLgMaxDim, "lg_dim must be 1 .. 10");
+    REQUIRE(d->lg_a, "NULL lg_a");
+    std::unique_ptr<vsmcx_serve> S(new vsmcx_serve);
+    S->ctx = ctx;
+    S->fp64 = d->precision == VSMCX_FP64_FUSED;
+    S->scheme = d->scheme;
+    S->threshold = d->threshold;
+    S->cv = CvParams{d->dt, d->sd_pos, d->sd_vel, d->prior_sd_pos, d->prior_sd_vel, d->obs_scale,
+                     d->nu, d->llh_const};
+    LgParams &lg = S->lg;
+    lg.dim = (int32_t)d->lg_dim;
+    lg.q = d->lg_q;
+    lg.r = d->lg_r;
+    lg.s0 = d->lg_s0;
+    lg.llh_const = -(double)d->lg_dim * std::log(d->lg_r * std::sqrt(2 * 3.14159265358979323846));
+    for (uint32_t i = 0; i < d->lg_dim * d->lg_dim; ++i) {
+        lg.ad[i] = d->lg_a[i];
+        lg.a[i] = (float)d->lg_a[i];
+    }
+    S->max_jobs = d->max_jobs;
+    S->max_horizon = d->max_horizon;
+    // The budget covers the particle arena: per particle 64 B of weights,
+    // ancestors, lists and fold scratch plus two states of the average model
+    // dimension ((4 + lg_dim) / 2, the CV/LG mix).
+    const double rb = S->real_bytes();
+    const double avg_dim = 0.5 * (4 + d->lg_dim);
+    const double per_particle = 64 + 2 * rb * avg_dim;
+    const uint64_t particles = (uint64_t)((double)d->hbm_budget / per_particle);
+    S->cap_units = particles / kUnit;
+    REQUIRE(S->cap_units >= 1, "HBM budget too small for one job");
+    const uint64_t P_ = S->cap_units * kUnit;
+    S->cap_state = (uint64_t)(P_ * avg_dim);
+    S->cap_obs = (uint64_t)d->max_jobs * d->max_horizon * std::max<uint32_t>(2, d->lg_dim);
+    S->parts.reset(S->cap_units);
+    S->states.reset(S->cap_state);
+    S->obsr.reset(S->cap_obs);
+    try {
+        BatchPool &P = S->P;
+        P.st0 = ctx->alloc(S->cap_state * S->real_bytes());
+        P.st1 = ctx->alloc(S->cap_state * S->real_bytes());
+        P.lw = ctx->alloc_n<double>(P_);
+        P.anc = ctx->alloc_n<uint32_t>(P_);
+        P.zl = ctx->alloc_n<uint32_t>(P_);
+        P.sl = ctx->alloc_n<uint32_t>(P_);
+        P.rep32 = ctx->alloc_n<uint32_t>(P_);
+        P.spec = ctx->alloc_n<SpecRec>(P_);
+        P.gval = ctx->alloc_n<double>(P_);
+        P.ts = ctx->alloc_n<double>(S->cap_units);
+        P.tagg = ctx->alloc_n<Seg>(S->cap_units);
+        P.tpre = ctx->alloc_n<Seg>(S->cap_units);
+        P.tzs = ctx->alloc_n<uint64_t>(S->cap_units);
+        P.ctl = ctx->alloc_n<FilterCtl>(d->max_jobs);
+        P.hist = ctx->alloc_n<StepRow>((uint64_t)d->max_jobs * d->max_horizon);
+        P.obs = ctx->alloc_n<double>(S->cap_obs);
+        P.next_t = ctx->alloc_zero<uint64_t>(d->max_jobs);
+        P.avail = ctx->alloc_zero<uint64_t>(d->max_jobs);
+        P.cur = ctx->alloc_zero<uint32_t>(d->max_jobs);
+        P.err_any = ctx->alloc_zero<int>(1);
+        S->df = ctx->alloc_n<BatchFilter>(d->max_jobs);
+        S->order = ctx->alloc_n<uint32_t>(d->max_jobs);
+        S->queue = ctx->alloc_zero<unsigned>(1);
+        VSMCX_HIP(hipEventCreate(&S->ev0));
+        VSMCX_HIP(hipEventCreate(&S->ev1));
+        ctx->sync();
+    } catch (...) {
+        S->release_all();
+        throw;
+    }
+    int cus = 256;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, ctx->device) == hipSuccess && prop.multiProcessorCount > 0)
+        cus = prop.multiProcessorCount;
+    S->grid_cap = (unsigned)cus * (S->scheme == kStratified ? 2 : 3);
+    S->slot_job.assign(d->max_jobs, 0);
+    S->hF.resize(d->max_jobs);
+    S->hC.resize(d->max_jobs);
+    for (int i = (int)d->max_jobs - 1; i >= 0; --i) S->free_slots.push_back(i);
+    *out = S.release();
+    API_END
+}
+
+int vsmcx_serve_destroy(vsmcx_serve *s)
+{
+    API_BEGIN
+    if (!s) return VSMCX_OK;
+    (void)hipStreamSynchronize(s->ctx->stream);
+    s->release_all();
+    delete s;
+    API_END
+}
+
+int vsmcx_serve_submit(vsmcx_serve *s, const vsmcx_job_desc *d, uint64_t *job)
+{
+    API_BEGIN
+    REQUIRE(s && d && job, "NULL argument");
+    REQUIRE(d->model == VSMCX_MODEL_CV || d->model == VSMCX_MODEL_LG, "unknown model");
+    REQUIRE(d->n >= 1 && d->n <= (uint64_t)kBatchMaxTiles * kFoldTile,
+            "jobs hold 1 to 2^20 particles");
+    REQUIRE(d->horizon >= 1 && d->horizon <= s->max_horizon, "horizon out of range");
+    const uint64_t id = s->next_id++;
+    Job &J = s->jobs[id];
+    J.d = *d;
+    J.obs.reserve((size_t)d->horizon * s->obs_dim(d->model));
+    s->waiting.push_back(id);
+    *job = id;
+    API_END
+}
+
+int vsmcx_serve_observe(vsmcx_serve *s, uint64_t job, const double *y)
+{
+    API_BEGIN
+    REQUIRE(s && y, "NULL argument");
+    auto it = s->jobs.find(job);
+    REQUIRE(it != s->jobs.end(), "unknown job");
+    Job &J = it->second;
+    REQUIRE(J.received < J.d.horizon, "more observations than the job's horizon");
+    const int D = s->obs_dim(J.d.model);
+    J.obs.insert(J.obs.end(), y, y + D);
+    J.arrival.push_back(now_us());
+    J.received += 1;
+    if (!s->inflight) s->upload(J);
+    API_END
+}
+
+int vsmcx_serve_observe_many(vsmcx_serve *s, uint64_t count, const uint64_t *ids,
+                             const double *const *ys)
+{
+    API_BEGIN
+    REQUIRE(s && (count == 0 || (ids && ys)), "NULL argument");
+    for (uint64_t k = 0; k < count; ++k) {
+        const int rc = vsmcx_serve_observe(s, ids[k], ys[k]);
+        if (rc != VSMCX_OK) return rc;
+    }
+    API_END
+}
+
+int vsmcx_serve_ready(vsmcx_serve *s, uint64_t max, uint64_t *ids, uint64_t *count)
+{
+    API_BEGIN
+    REQUIRE(s && count && (max == 0 || ids), "NULL argument");
+    uint64_t c = 0;
+    for (const auto &kv : s->jobs) {
+        const Job &J = kv.second;
+        if (J.state == 2 || J.received > J.done || J.received >= J.d.horizon) continue;
+        if (c < max) ids[c] = kv.first;
+        ++c;
+    }
+    *count = c;
+    API_END
+}
+
+int vsmcx_serve_poll(vsmcx_serve *s, int wait, uint64_t *launched)
+{
+    API_BEGIN
+    REQUIRE(s, "NULL serve");
+    if (launched) *launched = 0;
+    if (s->inflight) {
+        if (wait) {
+            VSMCX_HIP(hipEventSynchronize(s->ev1));
+        } else {
+            const hipError_t q = hipEventQuery(s->ev1);
+            if (q == hipErrorNotReady) return VSMCX_OK;
+            VSMCX_HIP(q);
+        }
+        s->complete();
+    }
+    s->admit();
+    for (auto &kv : s->jobs)
+        if (kv.second.state == 1) s->upload(kv.second);
+    const uint64_t n = s->launch();
+    if (launched) *launched = n;
+    API_END
+}
+
+int vsmcx_serve_job(vsmcx_serve *s, uint64_t job, int *state, uint64_t *steps)
+{
+    API_BEGIN
+    REQUIRE(s, "NULL serve");
+    auto it = s->jobs.find(job);
+    REQUIRE(it != s->jobs.end(), "unknown job");
+    if (state) *state = it->second.state;
+    if (steps) *steps = it->second.done;
+    API_END
+}
+
+int vsmcx_serve_history(vsmcx_serve *s, uint64_t job, uint64_t count, vsmcx_step_stats *out)
+{
+    API_BEGIN
+    REQUIRE(s && out, "NULL argument");
+    auto it = s->jobs.find(job);
+    REQUIRE(it != s->jobs.end(), "unknown job");
+    const Job &J = it->second;
+    REQUIRE(J.state == 2, "the job has not finished");
+    REQUIRE(count <= J.history.size(), "count beyond the job's horizon");
+    std::memcpy(out, J.history.data(), sizeof(StepRow) * count);
+    API_END
+}
+
+int vsmcx_serve_forget(vsmcx_serve *s, uint64_t job)
+{
+    API_BEGIN
+    REQUIRE(s, "NULL serve");
+    auto it = s->jobs.find(job);
+    REQUIRE(it != s->jobs.end(), "unknown job");
+    REQUIRE(it->second.state == 2, "only finished jobs can be forgotten");
+    s->jobs.erase(it);
+    API_END
+}
+
+int vsmcx_serve_stats(vsmcx_serve *s, vsmcx_serve_summary *out)
+{
+    API_BEGIN
+    REQUIRE(s && out, "NULL argument");
+    vsmcx_serve_summary r{};
+    r.steps = s->steps;
+    r.particle_updates = s->updates;
+    r.jobs_done = s->jobs_done;
+    uint64_t active = 0;
+    for (const auto &kv : s->jobs) active += kv.second.state == 1;
+    r.jobs_active = active;
+    r.jobs_waiting = s->waiting.size();
+    std::vector<double> l = s->lat_us;
+    if (!l.empty()) {
+        std::sort(l.begin(), l.end());
+        r.p50_us = l[(size_t)(0.50 * (l.size() - 1))];
+        r.p99_us = l[(size_t)(0.99 * (l.size() - 1))];
+    }
+    r.busy_ms = s->busy_ms;
+    r.elapsed_ms = s->t_first < 0 ? 0 : (s->t_last - s->t_first) * 1e-3;
+    r.peak_particles = s->peak_units * kUnit;
+    r.capacity_particles = s->cap_units * kUnit;
+    *out = r;
+    API_END
+}
