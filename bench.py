@@ -150,6 +150,67 @@ def run_batched(nfilters, seed, tune=True):
                       "remaining steps (tuner steps excluded)"}
 
 
+def run_serving(seconds, rate, seed, budget_gb=200.0, max_jobs=8192):
+    """Config-4 workload: Poisson arrivals (rate per second) for `seconds`, a
+    50/50 mix of the CV model and the 10-D linear-Gaussian model, N
+    log-uniform in [2^10, 2^20], T uniform in [20, 1000].  Pacing: closed
+    loop -- a job's next observation arrives as soon as its previous step has
+    completed (one step at a time).  Jobs are admitted under the HBM budget;
+    the run ends when every submitted job has finished."""
+    rng = np.random.default_rng(seed)
+    t_arr = np.cumsum(rng.exponential(1.0 / rate, int(rate * seconds * 1.5) + 16))
+    t_arr = t_arr[t_arr < seconds]
+    k = t_arr.size
+    models = rng.integers(0, 2, k)
+    ns = np.exp(rng.uniform(np.log(2 ** 10), np.log(2 ** 20), k)).astype(np.uint64)
+    Ts = rng.integers(20, 1001, k)
+    A = V.lg_matrix(10, seed=seed)
+    y_cv = V.cv_simulate_many(np.maximum(Ts, 1), seed + 1)
+    y_lg = V.lg_simulate_many(A, np.maximum(Ts, 1), seed + 2)
+    ctx = V.Context(0)
+    s = V.Serve(ctx, budget_gb * 1e9, max_jobs, 1000, A, precision=V.MIXED)
+    ids = {}
+    fed = {}
+    data = {}
+    t0 = time.perf_counter()
+    nxt = 0
+    while True:
+        now = time.perf_counter() - t0
+        while nxt < k and t_arr[nxt] <= now:
+            j = s.submit(int(models[nxt]), int(ns[nxt]), int(Ts[nxt]), 500000 + nxt, 600000 + nxt)
+            ids[j] = nxt
+            fed[j] = 0
+            data[j] = y_lg[nxt] if models[nxt] == V.MODEL_LG else y_cv[nxt]
+            nxt += 1
+        ready = s.ready()
+        if ready.size:
+            ys = [data[int(j)][fed[int(j)]] for j in ready]
+            s.observe_many(ready, ys)
+            for j in ready:
+                fed[int(j)] += 1
+        launched = s.poll(wait=True)
+        st = s.stats()
+        if nxt >= k and st["jobs_done"] == k:
+            break
+        if launched == 0 and nxt < k:
+            time.sleep(max(0.0, min(1e-3, t_arr[nxt] - (time.perf_counter() - t0))))
+    wall = time.perf_counter() - t0
+    st = s.stats()
+    s.close()
+    ctx.close()
+    return {"value": st["particle_updates"] / (st["elapsed_ms"] * 1e-3), "unit": "particle-updates/s",
+            "p50_step_ms": st["p50_us"] * 1e-3, "p99_step_ms": st["p99_us"] * 1e-3,
+            "jobs": k, "steps": st["steps"], "particle_updates": st["particle_updates"],
+            "device_busy_frac": st["busy_ms"] / st["elapsed_ms"] if st["elapsed_ms"] else None,
+            "peak_particles": st["peak_particles"], "capacity_particles": st["capacity_particles"],
+            "wall_s": wall,
+            "config": f"config-4: Poisson arrivals {rate}/s for {seconds}s, 50/50 CV and 10-D "
+                      f"linear-Gaussian (A spectral radius 0.9, q 0.1, r 0.2, s0 0.25), N "
+                      f"log-uniform [2^10, 2^20], T uniform [20, 1000], {budget_gb:g} GB budget, "
+                      f"mixed precision, systematic; pacing: closed loop (next observation when "
+                      f"the previous step completes)"}
+
+
 def cpu_baseline(N, steps, threads):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     from oracle_ctypes import Oracle
@@ -179,6 +240,9 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=12)
     ap.add_argument("--cpu-n", type=int, default=1 << 24)
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-seconds", type=float, default=0.5)
+    ap.add_argument("--serve-rate", type=float, default=2000.0)
     ap.add_argument("--batch-filters", type=int, default=4096)
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a device-to-device copy of 1 GiB five times (calibrates the "
@@ -254,6 +318,10 @@ def main():
     if not a.no_batch:
         print(f"[bench] batched: {a.batch_filters} filters", file=sys.stderr, flush=True)
         out["batched"] = run_batched(a.batch_filters, 13)
+    if not a.no_serve:
+        print(f"[bench] serving: {a.serve_rate}/s for {a.serve_seconds}s", file=sys.stderr,
+              flush=True)
+        out["serving"] = run_serving(a.serve_seconds, a.serve_rate, 17)
     if not a.no_cpu:
         threads = min(16, os.cpu_count() or 1)
         print(f"[bench] cpu baseline: N={a.cpu_n}, {a.cpu_steps} steps, {threads} threads",

@@ -7,8 +7,8 @@ batching).  Checked here:
   * each job's results are the same alone and in the mix (bit-identical);
   * admission respects the HBM budget: with room for two jobs at a time, six
     jobs run in waves, each still giving its alone result;
-  * a CV job served equals the same filter run by vsmcx_batch (and, in FP64,
-    the CPU oracle within 1e-9);
+  * a CV job served equals the same filter run by vsmcx_filter, bit for bit
+    (and, in FP64, the CPU oracle within 1e-9);
   * LG jobs: the log-evidence estimate against the exact Kalman value
     (oracle/kalman.py), over 8 seeds, within 4 standard errors plus 0.05.
 """
@@ -123,18 +123,26 @@ def test_serve_admission_under_budget(ctx, A):
         assert (alone[k] == mix[k]).all(), k
 
 
-def test_serve_cv_equals_batch_and_oracle(ctx, A, oracle):
-    model, n, T, seed, rseed, y = V.MODEL_CV, 5000, 20, 11, 12, V.cv_simulate(20, 3)
-    h = _run_alone(ctx, A, (model, n, T, seed, rseed, y), precision=V.FP64_FUSED)
-    b = V.Batch(ctx, [n], [T], [seed], [rseed], precision=V.FP64_FUSED)
-    b.set_obs(0, y)
-    b.run()
-    hb = b.history(0)
-    b.close()
-    assert (h == hb).all()
-    _, _, rec = oracle.cv_run(n, y, seed, rseed, V.SYSTEMATIC, 0.5)
-    assert (h["resampled"] == rec["resampled"]).all()
-    np.testing.assert_allclose(h["log_z"], rec["log_z"], rtol=1e-9, atol=1e-9)
+@pytest.mark.parametrize("precision", [V.MIXED, V.FP64_FUSED])
+def test_serve_cv_equals_single_filter(ctx, A, oracle, precision):
+    """A served CV job runs the single-filter pipeline's tile and job bodies on
+    the same tiles: its history equals vsmcx_filter's bit for bit (and, in
+    FP64, the CPU oracle within 1e-9)."""
+    n, T, seed, rseed = 5000, 20, 11, 12
+    y = V.cv_simulate(T, 3)
+    h = _run_alone(ctx, A, (V.MODEL_CV, n, T, seed, rseed, y), precision=precision)
+    f = V.Filter(ctx, V.cv_desc(n, T, precision=precision, scheme=V.SYSTEMATIC, seed=seed,
+                                resample_seed=rseed))
+    f.set_obs(y)
+    f.run(T)
+    f.finish()
+    hf = f.history()
+    f.close()
+    assert (h == hf).all()
+    if precision == V.FP64_FUSED:
+        _, _, rec = oracle.cv_run(n, y, seed, rseed, V.SYSTEMATIC, 0.5)
+        assert (h["resampled"] == rec["resampled"]).all()
+        np.testing.assert_allclose(h["log_z"], rec["log_z"], rtol=1e-9, atol=1e-9)
 
 
 @pytest.mark.parametrize("precision", [V.FP64_FUSED, V.MIXED])

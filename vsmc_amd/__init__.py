@@ -150,6 +150,8 @@ def lib():
         "vsmcx_serve_submit": ([vp, P(JobDesc), P(u64)], i32),
         "vsmcx_serve_observe": ([vp, u64, vp], i32),
         "vsmcx_serve_poll": ([vp, i32, P(u64)], i32),
+        "vsmcx_serve_observe_many": ([vp, u64, vp, vp], i32),
+        "vsmcx_serve_ready": ([vp, u64, vp, P(u64)], i32),
         "vsmcx_serve_job": ([vp, u64, P(i32), P(u64)], i32),
         "vsmcx_serve_history": ([vp, u64, u64, vp], i32),
         "vsmcx_serve_forget": ([vp, u64], i32),
@@ -508,6 +510,20 @@ def lg_simulate(A, T, seed, q=0.1, r=0.2, s0=0.25):
     return y
 
 
+def lg_simulate_many(A, horizons, seed, q=0.1, r=0.2, s0=0.25):
+    """lg_simulate for many independent trajectories at once (one per horizon)."""
+    Ts = np.asarray(horizons, np.int64)
+    k, tmax, d = Ts.size, int(Ts.max()), A.shape[0]
+    g = np.random.default_rng(seed)
+    x = s0 * g.standard_normal((k, d))
+    y = np.zeros((tmax, k, d))
+    for t in range(tmax):
+        if t > 0:
+            x = x @ A.T + q * g.standard_normal((k, d))
+        y[t] = x + r * g.standard_normal((k, d))
+    return [np.ascontiguousarray(y[:Ts[f], f]) for f in range(k)]
+
+
 class Serve:
     """The serving loop (vsmcx_serve): jobs admitted under an HBM budget and
     stepped with continuous batching as their observations arrive."""
@@ -548,6 +564,18 @@ class Serve:
     def observe(self, job, y):
         y = np.ascontiguousarray(y, np.float64)
         _check(lib().vsmcx_serve_observe(self.h, job, y.ctypes.data))
+
+    def observe_many(self, jobs, ys):
+        """One observation each: ys[k] (a float64 array) for jobs[k]."""
+        ids = np.ascontiguousarray(jobs, np.uint64)
+        ptrs = (C.c_void_p * len(ys))(*[y.ctypes.data for y in ys])
+        _check(lib().vsmcx_serve_observe_many(self.h, ids.size, ids.ctypes.data, ptrs))
+
+    def ready(self, max_ids=1 << 16):
+        ids = np.zeros(max_ids, np.uint64)
+        n = C.c_uint64()
+        _check(lib().vsmcx_serve_ready(self.h, max_ids, ids.ctypes.data, C.byref(n)))
+        return ids[:min(n.value, max_ids)]
 
     def poll(self, wait=False):
         n = C.c_uint64()

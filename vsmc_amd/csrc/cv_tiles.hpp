@@ -1,0 +1,164 @@
+// cv_tiles.hpp -- the per-tile and per-filter bodies of the CV step, shared by
+// the single-filter kernels (kernels_cv.hip) and the serving tick
+// (kernels_tick.hip), which run them on the same tiles in the same order.
+//
+// The move fuses, per particle, what vSMC spreads over MoveOMP's parallel
+// loop and WeightSet's sequential passes: the ancestor gather of the previous
+// resample (StateMatrix::copy, include/vsmc/core/state_matrix.hpp:285-300)
+// folded into the state read, one Philox4x32-10 block per particle per step,
+// the transition and Student-t log-likelihood of
+// example/pf/include/pf_smp.hpp:96-112,207-238, lw += inc in FP64
+// (core/weight_set.hpp:324-331), an online log-sum-exp (core/weight_set.hpp:
+// 108-125), and the previous step's MonitorMCMC mean (core/monitor.hpp:274-302).
+#pragma once
+#include "cv_model.hpp"
+#include "device_util.hpp"
+#include "internal.hpp"
+
+namespace vsmcx {
+
+// One move tile (kMoveTile particles): writes the tile's partials
+// (max, sum e, sum e^2, monitor x, monitor y) to part[0..4].
+template <typename Real>
+__device__ __forceinline__ void cv_move_tile(
+    const FilterCtl *ctl, const typename StateOf<Real>::type *__restrict__ st_in,
+    typename StateOf<Real>::type *__restrict__ st_out, const uint32_t *__restrict__ anc,
+    double *__restrict__ lw, const double *__restrict__ obs, const CvParams &p, uint32_t k0,
+    uint32_t k1, uint64_t t, uint64_t N, uint64_t tile, double *__restrict__ part)
+{
+    typedef typename StateOf<Real>::type S4;
+    __shared__ double s_red[kBlock / kWave];
+    __shared__ Part5 s_p5[kBlock / kWave];
+    CvModel<Real> md;
+    md.load(p);
+    const int prev = t > 0 ? ctl->prev_resampled : 0;
+    const double m_old = ctl->m, c_old = ctl->coeff;
+    const double invN = 1.0 / (double)N;
+    const Real ox = (Real)obs[2 * t], oy = (Real)obs[2 * t + 1];
+    const uint64_t base = tile * (uint64_t)kMoveTile;
+
+    // Loads first, for all of the thread's particles (ancestor indices, then
+    // the gathered states and old log-weights), so their latencies overlap;
+    // then the arithmetic and the stores.
+    uint64_t src[kMoveItems];
+#pragma unroll
+    for (int k = 0; k < kMoveItems; ++k) {
+        const uint64_t i = base + (uint64_t)k * kBlock + threadIdx.x;
+        uint64_t j = (t > 0 && prev && i < N) ? (uint64_t)anc[i] : i;
+        src[k] = j < N ? j : (i < N ? i : 0);  // never read outside the state buffer
+    }
+    S4 s[kMoveItems];
+    double lwo[kMoveItems];
+#pragma unroll
+    for (int k = 0; k < kMoveItems; ++k) {
+        const uint64_t i = base + (uint64_t)k * kBlock + threadIdx.x;
+        lwo[k] = 0;
+        if (t > 0 && i < N) {
+            s[k] = st_in[src[k]];
+            if (!prev) lwo[k] = lw[i];
+        }
+    }
+
+    double m = -INFINITY, se = 0, se2 = 0, mx = 0, my = 0;
+#pragma unroll
+    for (int k = 0; k < kMoveItems; ++k) {
+        const uint64_t i = base + (uint64_t)k * kBlock + threadIdx.x;
+        if (i >= N) continue;
+        S4 &sk = s[k];
+        if (t > 0) {
+            const double w = prev ? invN : exp(lwo[k] - m_old) * c_old;
+            mx += w * (double)sk.x;
+            my += w * (double)sk.y;
+        }
+        Real z[4];
+        cv_normals(z, k0, k1, i, t);
+        cv_advance(md, sk, z, t);
+        const Real inc = md.llh(sk.x, sk.y, ox, oy);
+        const double lwn = lwo[k] + (double)inc;
+        st_out[i] = sk;
+        lw[i] = lwn;
+        lse_add(m, se, se2, lwn);
+    }
+    const double M = block_reduce<kBlock>(m, MaxOp(), s_red);
+    const double sc = (m == -INFINITY) ? 0.0 : exp(m - M);
+    Part5 v{se * sc, se2 * sc * sc, mx, my};
+    v = block_reduce<kBlock>(v, Part5Add(), s_p5);
+    if (threadIdx.x == 0) {
+        part[0] = M;
+        part[1] = v.se;
+        part[2] = v.se2;
+        part[3] = v.mx;
+        part[4] = v.my;
+    }
+}
+
+// One workgroup of 1024 threads: combine the tile partials in a fixed order,
+// then ESS, log-evidence and the resample decision.
+__device__ __forceinline__ void cv_reduce_tiles(FilterCtl *ctl, const double *partials,
+                                                uint64_t ntiles, StepRow *hist, uint64_t t)
+{
+    __shared__ double s_red[1024 / kWave];
+    __shared__ Part5 s_p5[1024 / kWave];
+    double m = -INFINITY;
+    for (uint64_t j = threadIdx.x; j < ntiles; j += 1024) {
+        const double v = partials[5 * j];
+        m = m < v ? v : m;
+    }
+    const double M = block_reduce<1024>(m, MaxOp(), s_red);
+    Part5 acc{0, 0, 0, 0};
+    for (uint64_t j = threadIdx.x; j < ntiles; j += 1024) {
+        const double *q = partials + 5 * j;
+        const double sc = q[0] == -INFINITY ? 0.0 : exp(q[0] - M);
+        acc.se += q[1] * sc;
+        acc.se2 += q[2] * sc * sc;
+        acc.mx += q[3];
+        acc.my += q[4];
+    }
+    acc = block_reduce<1024>(acc, Part5Add(), s_p5);
+    if (threadIdx.x == 0) cv_step_scalars(ctl, M, acc.se, acc.se2, acc.mx, acc.my, hist, t);
+}
+
+// Monitor of the last step: sum_i W_i (x_i, y_i) over the post-resample state.
+template <typename Real>
+__device__ __forceinline__ void cv_monitor_tile(const FilterCtl *ctl,
+                                                const typename StateOf<Real>::type *st,
+                                                const uint32_t *anc, const double *lw,
+                                                uint64_t N, uint64_t tile, double *part)
+{
+    __shared__ Part5 s_p5[kBlock / kWave];
+    const int prev = ctl->prev_resampled;
+    const double m = ctl->m, c = ctl->coeff, invN = 1.0 / (double)N;
+    const uint64_t base = tile * (uint64_t)kMoveTile;
+    Part5 v{0, 0, 0, 0};
+    for (int k = 0; k < kMoveItems; ++k) {
+        const uint64_t i = base + (uint64_t)k * kBlock + threadIdx.x;
+        if (i >= N) continue;
+        const auto s = st[prev ? (uint64_t)anc[i] : i];
+        const double w = prev ? invN : exp(lw[i] - m) * c;
+        v.mx += w * (double)s.x;
+        v.my += w * (double)s.y;
+    }
+    v = block_reduce<kBlock>(v, Part5Add(), s_p5);
+    if (threadIdx.x == 0) {
+        part[3] = v.mx;
+        part[4] = v.my;
+    }
+}
+
+__device__ __forceinline__ void cv_monitor_reduce(const FilterCtl *ctl, const double *partials,
+                                                  uint64_t ntiles, StepRow *hist)
+{
+    __shared__ Part5 s_p5[1024 / kWave];
+    Part5 acc{0, 0, 0, 0};
+    for (uint64_t j = threadIdx.x; j < ntiles; j += 1024) {
+        acc.mx += partials[5 * j + 3];
+        acc.my += partials[5 * j + 4];
+    }
+    acc = block_reduce<1024>(acc, Part5Add(), s_p5);
+    if (threadIdx.x == 0 && hist) {
+        hist[ctl->t].mean_x = acc.mx;
+        hist[ctl->t].mean_y = acc.my;
+    }
+}
+
+}  // namespace vsmcx

@@ -11,57 +11,10 @@
 // runs it, in whatever company, under whatever launch shape.
 #include "cv_model.hpp"
 #include "fold_tiles.hpp"
+#include "lg_model.hpp"
 #include "launch.hpp"
 
 namespace vsmcx {
-
-// four N(0,1) from Philox block {t+1, id_lo, id_hi, j} (Box-Muller, as cv_normals)
-__device__ __forceinline__ void lg_normals(float z[4], uint32_t k0, uint32_t k1, uint64_t id,
-                                           uint64_t t, uint32_t j)
-{
-    u32x4 c;
-    c.v[0] = (uint32_t)(t + 1);
-    c.v[1] = (uint32_t)id;
-    c.v[2] = (uint32_t)(id >> 32);
-    c.v[3] = j;
-    const u32x4 w = philox4x32_10(c, k0, k1);
-    const float ln2 = 0.69314718f;
-#pragma unroll
-    for (int h = 0; h < 2; ++h) {
-        const float u1 = u01_oc_24(w.v[2 * h]);
-        const float u2 = u01_oc_24(w.v[2 * h + 1]);
-        const float r = __builtin_amdgcn_sqrtf(-2.0f * ln2 * __builtin_amdgcn_logf(u1));
-        z[2 * h] = r * __builtin_amdgcn_cosf(u2);
-        z[2 * h + 1] = r * __builtin_amdgcn_sinf(u2);
-    }
-}
-
-__device__ __forceinline__ void lg_normals(double z[4], uint32_t k0, uint32_t k1, uint64_t id,
-                                           uint64_t t, uint32_t j)
-{
-    u32x4 c;
-    c.v[0] = (uint32_t)(t + 1);
-    c.v[1] = (uint32_t)id;
-    c.v[2] = (uint32_t)(id >> 32);
-    c.v[3] = j;
-    const u32x4 w = philox4x32_10(c, k0, k1);
-    const double two_pi = 6.2831853071795865;
-#pragma unroll
-    for (int h = 0; h < 2; ++h) {
-        const double u1 = u01_oc_53(w.v[2 * h]);
-        const double u2 = u01_oc_53(w.v[2 * h + 1]);
-        const double r = sqrt(-2 * log(u1));
-        z[2 * h] = r * cos(two_pi * u2);
-        z[2 * h + 1] = r * sin(two_pi * u2);
-    }
-}
-
-template <typename Real>
-__device__ __forceinline__ Real lg_a(const LgParams &lp, int k);
-template <>
-__device__ __forceinline__ float lg_a<float>(const LgParams &lp, int k) { return lp.a[k]; }
-template <>
-__device__ __forceinline__ double lg_a<double>(const LgParams &lp, int k) { return lp.ad[k]; }
 
 template <typename Inv>
 struct BatchLds {

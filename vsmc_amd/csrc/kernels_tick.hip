@@ -1,0 +1,391 @@
+// kernels_tick.hip -- one serving tick: one step of every job in the tick,
+// each phase one grid launch over the tiles of all of them.
+//
+// A tick is the single-filter pipeline (kernels_cv.hip, kernels_fold.hip)
+// made segmented: the tile work of every job is one flat list of
+// (job, tile) items, and the per-job reductions and scans run one workgroup
+// per job.  The per-tile and per-job bodies are the single-filter ones
+// (cv_tiles.hpp, fold_tiles.hpp) on the same tiles in the same order, so a
+// served CV job computes exactly what vsmcx_filter computes, and no job's
+// arithmetic depends on the other jobs in the tick.  Jobs that do not
+// resample this step leave the fold launches at once.
+#include "cv_tiles.hpp"
+#include "fold_tiles.hpp"
+#include "launch.hpp"
+#include "lg_model.hpp"
+
+namespace vsmcx {
+
+// ---- the linear-Gaussian move on one tile (same tile shape as cv_move_tile) ----
+template <typename Real>
+__device__ __forceinline__ void lg_move_tile(const FilterCtl *ctl, const Real *__restrict__ in,
+                                             Real *__restrict__ out, const uint32_t *__restrict__ anc,
+                                             double *__restrict__ lw, const double *__restrict__ y,
+                                             const LgParams &lp, uint32_t k0, uint32_t k1,
+                                             uint64_t t, uint64_t N, uint64_t tile, double *part)
+{
+    __shared__ double s_red[kBlock / kWave];
+    __shared__ Part5 s_p5[kBlock / kWave];
+    const int D = lp.dim;
+    const int prev = t > 0 ? ctl->prev_resampled : 0;
+    const double m_old = ctl->m, c_old = ctl->coeff;
+    const double invN = 1.0 / (double)N;
+    const Real q = (Real)lp.q, s0 = (Real)lp.s0, inv_r2 = (Real)(1 / (lp.r * lp.r));
+    const uint64_t base = tile * (uint64_t)kMoveTile;
+    double m = -INFINITY, se = 0, se2 = 0, mx = 0, my = 0;
+#pragma unroll 1
+    for (int k = 0; k < kMoveItems; ++k) {
+        const uint64_t i = base + (uint64_t)k * kBlock + threadIdx.x;
+        if (i >= N) continue;
+        Real x[kLgMaxDim], xn[kLgMaxDim];
+        double lwo = 0;
+        if (t > 0) {
+            uint64_t src = prev ? (uint64_t)anc[i] : i;
+            src = src < N ? src : i;
+#pragma unroll
+            for (int d = 0; d < kLgMaxDim; ++d) x[d] = d < D ? in[src * D + d] : (Real)0;
+            if (!prev) lwo = lw[i];
+            const double w = prev ? invN : exp(lwo - m_old) * c_old;
+            mx += w * (double)x[0];
+            my += w * (double)x[1];
+        }
+        Real z[12];
+#pragma unroll
+        for (int j = 0; j < 3; ++j) lg_normals(z + 4 * j, k0, k1, i, t, (uint32_t)j);
+        Real ss = 0;
+#pragma unroll
+        for (int d = 0; d < kLgMaxDim; ++d) {
+            if (d >= D) break;
+            Real v;
+            if (t == 0) {
+                v = s0 * z[d];
+            } else {
+                v = q * z[d];
+#pragma unroll
+                for (int e = 0; e < kLgMaxDim; ++e)
+                    if (e < D) v += lg_a<Real>(lp, d * D + e) * x[e];
+            }
+            xn[d] = v;
+            const Real r = (Real)y[d] - v;
+            ss += r * r;
+        }
+        const double lwn = lwo + (lp.llh_const - 0.5 * (double)(ss * inv_r2));
+#pragma unroll
+        for (int d = 0; d < kLgMaxDim; ++d)
+            if (d < D) out[i * D + d] = xn[d];
+        lw[i] = lwn;
+        lse_add(m, se, se2, lwn);
+    }
+    const double M = block_reduce<kBlock>(m, MaxOp(), s_red);
+    const double sc = (m == -INFINITY) ? 0.0 : exp(m - M);
+    Part5 v{se * sc, se2 * sc * sc, mx, my};
+    v = block_reduce<kBlock>(v, Part5Add(), s_p5);
+    if (threadIdx.x == 0) {
+        part[0] = M;
+        part[1] = v.se;
+        part[2] = v.se2;
+        part[3] = v.mx;
+        part[4] = v.my;
+    }
+}
+
+// final monitor of a linear-Gaussian job: mean of the first two coordinates
+template <typename Real>
+__device__ __forceinline__ void lg_monitor_tile(const FilterCtl *ctl, const Real *st, int D,
+                                                const uint32_t *anc, const double *lw, uint64_t N,
+                                                uint64_t tile, double *part)
+{
+    __shared__ Part5 s_p5[kBlock / kWave];
+    const int prev = ctl->prev_resampled;
+    const double m = ctl->m, c = ctl->coeff, invN = 1.0 / (double)N;
+    const uint64_t base = tile * (uint64_t)kMoveTile;
+    Part5 v{0, 0, 0, 0};
+    for (int k = 0; k < kMoveItems; ++k) {
+        const uint64_t i = base + (uint64_t)k * kBlock + threadIdx.x;
+        if (i >= N) continue;
+        uint64_t src = prev ? (uint64_t)anc[i] : i;
+        src = src < N ? src : i;
+        const double w = prev ? invN : exp(lw[i] - m) * c;
+        v.mx += w * (double)st[src * D];
+        v.my += w * (double)st[src * D + 1];
+    }
+    v = block_reduce<kBlock>(v, Part5Add(), s_p5);
+    if (threadIdx.x == 0) {
+        part[3] = v.mx;
+        part[4] = v.my;
+    }
+}
+
+__device__ __forceinline__ uint64_t tiles_of_job(const BatchFilter &F)
+{
+    return (F.n + kFT - 1) / kFT;
+}
+
+// ---- move (all jobs' tiles) ---------------------------------------------------------
+template <typename Real>
+__global__ __launch_bounds__(kBlock) void k_tick_move(const TickItem *items,
+                                                      const BatchFilter *filters, BatchPool P,
+                                                      CvParams cv, LgParams lg)
+{
+    const TickItem it = items[blockIdx.x];
+    const BatchFilter F = filters[it.slot];
+    const FilterCtl *ctl = P.ctl + it.slot;
+    const uint64_t t = P.next_t[it.slot];
+    const uint32_t cur = P.cur[it.slot];
+    const uint32_t k0 = (uint32_t)F.seed, k1 = (uint32_t)(F.seed >> 32);
+    double *part = P.tpart + 5 * (F.off / kFT + it.tile);
+    if (F.model == kModelLg) {
+        Real *sa = (Real *)P.st0 + F.soff, *sb = (Real *)P.st1 + F.soff;
+        lg_move_tile<Real>(ctl, cur ? sb : sa, cur ? sa : sb, P.anc + F.off, P.lw + F.off,
+                           P.obs + F.obs_off + (uint64_t)lg.dim * t, lg, k0, k1, t, F.n, it.tile,
+                           part);
+    } else {
+        typedef typename StateOf<Real>::type S4;
+        S4 *sa = (S4 *)((Real *)P.st0 + F.soff), *sb = (S4 *)((Real *)P.st1 + F.soff);
+        cv_move_tile<Real>(ctl, cur ? sb : sa, cur ? sa : sb, P.anc + F.off, P.lw + F.off,
+                           P.obs + F.obs_off, cv, k0, k1, t, F.n, it.tile, part);
+    }
+}
+
+// ---- per job: reduction, decision, resample prologue -----------------------------
+__global__ __launch_bounds__(1024) void k_tick_reduce(const uint32_t *jobs,
+                                                      const BatchFilter *filters, BatchPool P,
+                                                      int scheme)
+{
+    const uint32_t slot = jobs[blockIdx.x];
+    const BatchFilter F = filters[slot];
+    FilterCtl *ctl = P.ctl + slot;
+    const uint64_t t = P.next_t[slot];
+    cv_reduce_tiles(ctl, P.tpart + 5 * (F.off / kFT), tiles_of_job(F), P.hist + F.hist_off, t);
+    if (threadIdx.x == 0) {
+        P.cur[slot] ^= 1u;
+        if (ctl->resample) resample_prologue(ctl, scheme);
+    }
+}
+
+// ---- fold passes -------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_tick_tsum(const TickItem *items,
+                                                      const BatchFilter *filters, BatchPool P)
+{
+    const TickItem it = items[blockIdx.x];
+    FilterCtl *ctl = P.ctl + it.slot;
+    if (!ctl->resample) return;
+    __shared__ double s_red[kBlock / kWave];
+    const BatchFilter F = filters[it.slot];
+    XWeight xs{P.lw + F.off, 0.0, 0.0};
+    xs.bind(ctl);
+    const double s = fold_tile_sum(xs, it.tile * (uint64_t)kFT, F.n, s_red);
+    if (threadIdx.x == 0) P.tsum[F.off / kFT + it.tile] = s;
+}
+
+__global__ __launch_bounds__(kScanBlock) void k_tick_scan_sum(const uint32_t *jobs,
+                                                              const BatchFilter *filters,
+                                                              BatchPool P)
+{
+    const uint32_t slot = jobs[blockIdx.x];
+    if (!P.ctl[slot].resample) return;
+    __shared__ double s_red[kScanBlock / kWave];
+    const BatchFilter F = filters[slot];
+    const uint64_t o = F.off / kFT;
+    wg_scan<double>(P.tsum + o, P.ts + o, tiles_of_job(F), AddOp(), 0.0, s_red);
+}
+
+__global__ __launch_bounds__(kBlock) void k_tick_seg(const TickItem *items,
+                                                     const BatchFilter *filters, BatchPool P)
+{
+    const TickItem it = items[blockIdx.x];
+    FilterCtl *ctl = P.ctl + it.slot;
+    if (!ctl->resample) return;
+    __shared__ FoldLdsT<false> L;
+    const BatchFilter F = filters[it.slot];
+    XWeight xs{P.lw + F.off, 0.0, 0.0};
+    xs.bind(ctl);
+    const uint64_t o = F.off / kFT + it.tile;
+    fold_tile_seg(xs, it.tile * (uint64_t)kFT, F.n, P.ts[o], P.spec + F.off + it.tile * (uint64_t)kFT,
+                  P.tagg + o, L);
+}
+
+__global__ __launch_bounds__(kScanBlock) void k_tick_scan_seg(const uint32_t *jobs,
+                                                              const BatchFilter *filters,
+                                                              BatchPool P)
+{
+    const uint32_t slot = jobs[blockIdx.x];
+    FilterCtl *ctl = P.ctl + slot;
+    if (!ctl->resample) return;
+    __shared__ Seg s_seg[kScanBlock / kWave];
+    __shared__ uint32_t s_cnt[kBatchMaxTiles + 1];
+    __shared__ SpecRec s_r[kScanBlock];
+    __shared__ double s_v[kScanBlock];
+    const BatchFilter F = filters[slot];
+    const uint64_t o = F.off / kFT, T = tiles_of_job(F);
+    const Seg tail = wg_scan<Seg>(P.tagg + o, P.tpre + o, T, SegOp(), seg_identity(), s_seg,
+                                  [&](uint64_t t, const Seg &p) { s_cnt[t] = (uint32_t)p.cnt; });
+    __syncthreads();
+    int bad;
+    const double tot = fold_chain<kScanBlock>(P.tpre + o, s_cnt, T, tail, P.spec + F.off,
+                                              P.gval + F.off, s_r, s_v, &bad);
+    if (threadIdx.x == 0) {
+        ctl->fold_total = tot;
+        ctl->specials = tail.cnt;
+        if (bad) atomicOr(&ctl->err, (int)kErrFoldClass);
+    }
+}
+
+template <typename Inv>
+__global__ __launch_bounds__(kBlock) void k_tick_apply(const TickItem *items,
+                                                       const BatchFilter *filters, BatchPool P)
+{
+    const TickItem it = items[blockIdx.x];
+    FilterCtl *ctl = P.ctl + it.slot;
+    if (!ctl->resample) return;
+    __shared__ FoldLdsT<Inv::kWindow> L;
+    const BatchFilter F = filters[it.slot];
+    XWeight xs{P.lw + F.off, 0.0, 0.0};
+    xs.bind(ctl);
+    Inv inv;
+    inv.bind(ctl);
+    const uint64_t o = F.off / kFT + it.tile;
+    const FoldOut fo{P.anc + F.off, P.zl + F.off, P.sl + F.off, nullptr, nullptr};
+    fold_tile_apply(ctl, xs, inv, it.tile, F.n, P.ts[o], P.tpre[o], P.gval + F.off, fo,
+                    P.rep32 + F.off, P.tzs + o, L);
+}
+
+__global__ __launch_bounds__(kScanBlock) void k_tick_scan_cnt(const uint32_t *jobs,
+                                                              const BatchFilter *filters,
+                                                              BatchPool P)
+{
+    const uint32_t slot = jobs[blockIdx.x];
+    FilterCtl *ctl = P.ctl + slot;
+    if (!ctl->resample) return;
+    __shared__ uint64_t s_red[kScanBlock / kWave];
+    const BatchFilter F = filters[slot];
+    const uint64_t o = F.off / kFT;
+    const uint64_t tot =
+        wg_scan<uint64_t>(P.tzs + o, P.tzsx + o, tiles_of_job(F), AddOp(), (uint64_t)0, s_red);
+    if (threadIdx.x == 0) ctl->zeros = tot >> 32;
+}
+
+__global__ __launch_bounds__(kBlock) void k_tick_layout(const TickItem *items,
+                                                        const BatchFilter *filters, BatchPool P)
+{
+    const TickItem it = items[blockIdx.x];
+    FilterCtl *ctl = P.ctl + it.slot;
+    if (!ctl->resample) return;
+    __shared__ FoldLdsT<false> L;
+    const BatchFilter F = filters[it.slot];
+    const FoldOut fo{P.anc + F.off, P.zl + F.off, P.sl + F.off, nullptr, nullptr};
+    fold_tile_layout(ctl, it.tile, F.n, P.rep32 + F.off, P.tzsx[F.off / kFT + it.tile], fo, L);
+}
+
+// zero slots: the tile's share [tile * kFT, (tile + 1) * kFT) of the pair lists
+__global__ __launch_bounds__(kBlock) void k_tick_zero_fill(const TickItem *items,
+                                                           const BatchFilter *filters,
+                                                           BatchPool P)
+{
+    const TickItem it = items[blockIdx.x];
+    FilterCtl *ctl = P.ctl + it.slot;
+    if (!ctl->resample) return;
+    const BatchFilter F = filters[it.slot];
+    const uint64_t N = F.n;
+    const uint64_t z = ctl->zeros < N ? ctl->zeros : N;
+    const uint64_t k0 = it.tile * (uint64_t)kFT;
+    const uint64_t k1 = k0 + kFT < z ? k0 + kFT : z;
+    const uint32_t *zl = P.zl + F.off, *sl = P.sl + F.off;
+    uint32_t *anc = P.anc + F.off;
+    for (uint64_t k = k0 + threadIdx.x; k < k1; k += kBlock) {
+        const uint32_t to = zl[k], from = sl[k];
+        if (to < N && from < N)
+            anc[to] = from;
+        else
+            atomicOr(&ctl->err, (int)kErrBounds);
+    }
+}
+
+__global__ void k_tick_epilogue(const uint32_t *jobs, uint32_t njobs, const BatchFilter *filters,
+                                BatchPool P, int scheme)
+{
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= njobs) return;
+    const uint32_t slot = jobs[j];
+    FilterCtl *ctl = P.ctl + slot;
+    const BatchFilter F = filters[slot];
+    if (ctl->resample) resample_epilogue(ctl, P.hist + F.hist_off, scheme == kSystematic, F.n);
+}
+
+// ---- the last step's monitor, for jobs that finish in this tick ----------------------
+template <typename Real>
+__global__ __launch_bounds__(kBlock) void k_tick_monitor(const TickItem *items,
+                                                         const BatchFilter *filters, BatchPool P,
+                                                         LgParams lg)
+{
+    const TickItem it = items[blockIdx.x];
+    const BatchFilter F = filters[it.slot];
+    const FilterCtl *ctl = P.ctl + it.slot;
+    if (P.next_t[it.slot] + 1 != F.horizon) return;
+    const uint32_t cur = P.cur[it.slot];  // flipped by the reduce: the new state
+    double *part = P.tpart + 5 * (F.off / kFT + it.tile);
+    if (F.model == kModelLg) {
+        const Real *st = (cur ? (const Real *)P.st1 : (const Real *)P.st0) + F.soff;
+        lg_monitor_tile<Real>(ctl, st, lg.dim, P.anc + F.off, P.lw + F.off, F.n, it.tile, part);
+    } else {
+        typedef typename StateOf<Real>::type S4;
+        const S4 *st = (const S4 *)((cur ? (const Real *)P.st1 : (const Real *)P.st0) + F.soff);
+        cv_monitor_tile<Real>(ctl, st, P.anc + F.off, P.lw + F.off, F.n, it.tile, part);
+    }
+}
+
+__global__ __launch_bounds__(1024) void k_tick_monitor_sum(const uint32_t *jobs,
+                                                           const BatchFilter *filters,
+                                                           BatchPool P)
+{
+    const uint32_t slot = jobs[blockIdx.x];
+    const BatchFilter F = filters[slot];
+    if (P.next_t[slot] + 1 != F.horizon) return;
+    cv_monitor_reduce(P.ctl + slot, P.tpart + 5 * (F.off / kFT), tiles_of_job(F),
+                      P.hist + F.hist_off);
+}
+
+// step counters and the error summary
+__global__ void k_tick_advance(const uint32_t *jobs, uint32_t njobs, BatchPool P)
+{
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= njobs) return;
+    const uint32_t slot = jobs[j];
+    P.next_t[slot] += 1;
+    const int e = P.ctl[slot].err;
+    if (e) atomicOr(P.err_any, e);
+}
+
+void launch_tick(bool fp64, int scheme, const TickItem *items, uint32_t nitems,
+                 const uint32_t *jobs, uint32_t njobs, const BatchFilter *filters,
+                 const BatchPool &P, const CvParams &cv, const LgParams &lg, hipStream_t st)
+{
+    if (!nitems || !njobs) return;
+    const dim3 gi(nitems), gj(njobs), b(kBlock), bs(kScanBlock);
+    const dim3 gsmall((njobs + 255) / 256), bsmall(256);
+    if (fp64)
+        hipLaunchKernelGGL(k_tick_move<double>, gi, b, 0, st, items, filters, P, cv, lg);
+    else
+        hipLaunchKernelGGL(k_tick_move<float>, gi, b, 0, st, items, filters, P, cv, lg);
+    hipLaunchKernelGGL(k_tick_reduce, gj, bs, 0, st, jobs, filters, P, scheme);
+    hipLaunchKernelGGL(k_tick_tsum, gi, b, 0, st, items, filters, P);
+    hipLaunchKernelGGL(k_tick_scan_sum, gj, bs, 0, st, jobs, filters, P);
+    hipLaunchKernelGGL(k_tick_seg, gi, b, 0, st, items, filters, P);
+    hipLaunchKernelGGL(k_tick_scan_seg, gj, bs, 0, st, jobs, filters, P);
+    if (scheme == kStratified)
+        hipLaunchKernelGGL(k_tick_apply<InvStratified>, gi, b, 0, st, items, filters, P);
+    else
+        hipLaunchKernelGGL(k_tick_apply<InvSystematic>, gi, b, 0, st, items, filters, P);
+    hipLaunchKernelGGL(k_tick_scan_cnt, gj, bs, 0, st, jobs, filters, P);
+    hipLaunchKernelGGL(k_tick_layout, gi, b, 0, st, items, filters, P);
+    hipLaunchKernelGGL(k_tick_zero_fill, gi, b, 0, st, items, filters, P);
+    hipLaunchKernelGGL(k_tick_epilogue, gsmall, bsmall, 0, st, jobs, njobs, filters, P, scheme);
+    if (fp64)
+        hipLaunchKernelGGL(k_tick_monitor<double>, gi, b, 0, st, items, filters, P, lg);
+    else
+        hipLaunchKernelGGL(k_tick_monitor<float>, gi, b, 0, st, items, filters, P, lg);
+    hipLaunchKernelGGL(k_tick_monitor_sum, gj, bs, 0, st, jobs, filters, P);
+    hipLaunchKernelGGL(k_tick_advance, gsmall, bsmall, 0, st, jobs, njobs, P);
+}
+
+}  // namespace vsmcx

@@ -103,7 +103,10 @@ struct BatchPool {
     double *gval;
     double *ts;            // per tile (index off / kFoldTile + tile): approximate start
     Seg *tagg, *tpre;      // per tile: segment summary and exclusive prefix
-    uint64_t *tzs;         // per tile: zeros << 32 | surplus, then their exclusive prefix
+    uint64_t *tzs;         // per tile: zeros << 32 | surplus (batched path: then their prefix)
+    double *tsum;          // per tile (serving tick): sum of the weights
+    uint64_t *tzsx;        // per tile (serving tick): exclusive prefix of tzs
+    double *tpart;         // per tile (serving tick): 5 move partials
     FilterCtl *ctl;        // [filters]
     StepRow *hist;         // history rows
     const double *obs;     // observations (CV: 2 per step, LG: dim per step)
@@ -112,6 +115,17 @@ struct BatchPool {
     uint32_t *cur;         // [filters] state buffer holding the current state
     int *err_any;          // OR of every filter's device error bits
 };
+
+// one (job slot, tile) work item of a serving tick
+struct TickItem {
+    uint32_t slot;
+    uint32_t tile;
+};
+
+// kernels_tick.hip: one step of every listed job, one grid launch per phase
+void launch_tick(bool fp64, int scheme, const TickItem *items, uint32_t nitems,
+                 const uint32_t *jobs, uint32_t njobs, const BatchFilter *filters,
+                 const BatchPool &P, const CvParams &cv, const LgParams &lg, hipStream_t st);
 
 void launch_batch_run(bool fp64, int scheme, const BatchFilter *filters, const uint32_t *order,
                       uint32_t nf, const BatchPool &P, const CvParams &p, const LgParams &lg,

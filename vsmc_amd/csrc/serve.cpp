@@ -1,8 +1,9 @@
 // serve.cpp -- the serving loop (vsmcx_serve_*): filtering jobs that arrive
 // over time, admitted against an HBM budget, kept resident on the device
-// across steps, and advanced with continuous batching: every tick is one
-// persistent launch of the batched kernel over all jobs that have an
-// observation waiting; jobs admitted while a tick runs join the next one.
+// across steps, and advanced with continuous batching: every tick steps all
+// jobs that have an observation waiting, each phase one grid launch over the
+// tiles of all of them (kernels_tick.hip); jobs admitted while a tick runs
+// join the next one.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -109,9 +110,9 @@ struct vsmcx_serve {
     uint64_t cap_units = 0, cap_state = 0, cap_obs = 0;
     BatchPool P{};
     BatchFilter *df = nullptr;
-    uint32_t *order = nullptr;
-    unsigned *queue = nullptr;
-    unsigned grid_cap = 0;
+    TickItem *items = nullptr;     // device: the tick's (job, tile) items
+    uint32_t *jobs_dev = nullptr;  // device: the tick's jobs
+    std::vector<TickItem> items_h;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     std::vector<int> free_slots;
     std::vector<uint64_t> slot_job;
@@ -133,8 +134,9 @@ struct vsmcx_serve {
 
     void release_all()
     {
-        void *ps[] = {df, order, queue, P.st0, P.st1, P.lw, P.anc, P.zl, P.sl, P.rep32,
-                      P.spec, P.gval, P.ts, P.tagg, P.tpre, P.tzs, P.ctl, P.hist, (void *)P.obs,
+        void *ps[] = {df, P.st0, P.st1, P.lw, P.anc, P.zl, P.sl, P.rep32,
+                      P.spec, P.gval, P.ts, P.tagg, P.tpre, P.tzs, P.tsum, P.tzsx, P.tpart, items,
+                      jobs_dev, P.ctl, P.hist, (void *)P.obs,
                       P.next_t, (void *)P.avail,
                       P.cur, P.err_any};
         for (void *q : ps) ctx->release(q);
@@ -266,11 +268,18 @@ struct vsmcx_serve {
             if (J.state == 1 && J.uploaded > J.done) tick.push_back((uint32_t)slot);
         }
         if (tick.empty()) return 0;
+        items_h.clear();
+        for (uint32_t slot : tick) {
+            const uint64_t T = (jobs[slot_job[slot]].d.n + kUnit - 1) / kUnit;
+            for (uint64_t tt = 0; tt < T; ++tt) items_h.push_back(TickItem{slot, (uint32_t)tt});
+        }
         hipStream_t s = ctx->stream;
-        VSMCX_HIP(hipMemcpyAsync(order, tick.data(), 4 * tick.size(), hipMemcpyHostToDevice, s));
+        VSMCX_HIP(hipMemcpyAsync(jobs_dev, tick.data(), 4 * tick.size(), hipMemcpyHostToDevice, s));
+        VSMCX_HIP(hipMemcpyAsync(items, items_h.data(), sizeof(TickItem) * items_h.size(),
+                                 hipMemcpyHostToDevice, s));
         VSMCX_HIP(hipEventRecord(ev0, s));
-        launch_batch_run(fp64, scheme, df, order, (uint32_t)tick.size(), P, cv, lg, 1, queue,
-                         std::min<unsigned>((unsigned)tick.size(), grid_cap), s);
+        launch_tick(fp64, scheme, items, (uint32_t)items_h.size(), jobs_dev,
+                    (uint32_t)tick.size(), df, P, cv, lg, s);
         VSMCX_HIP(hipGetLastError());
         VSMCX_HIP(hipEventRecord(ev1, s));
         inflight = true;
@@ -339,6 +348,11 @@ int vsmcx_serve_create(vsmcx_ctx *ctx, const vsmcx_serve_desc *d, vsmcx_serve **
         P.tagg = ctx->alloc_n<Seg>(S->cap_units);
         P.tpre = ctx->alloc_n<Seg>(S->cap_units);
         P.tzs = ctx->alloc_n<uint64_t>(S->cap_units);
+        P.tsum = ctx->alloc_n<double>(S->cap_units);
+        P.tzsx = ctx->alloc_n<uint64_t>(S->cap_units);
+        P.tpart = ctx->alloc_n<double>(5 * S->cap_units);
+        S->items = ctx->alloc_n<TickItem>(S->cap_units);
+        S->jobs_dev = ctx->alloc_n<uint32_t>(d->max_jobs);
         P.ctl = ctx->alloc_n<FilterCtl>(d->max_jobs);
         P.hist = ctx->alloc_n<StepRow>((uint64_t)d->max_jobs * d->max_horizon);
         P.obs = ctx->alloc_n<double>(S->cap_obs);
@@ -347,8 +361,6 @@ int vsmcx_serve_create(vsmcx_ctx *ctx, const vsmcx_serve_desc *d, vsmcx_serve **
         P.cur = ctx->alloc_zero<uint32_t>(d->max_jobs);
         P.err_any = ctx->alloc_zero<int>(1);
         S->df = ctx->alloc_n<BatchFilter>(d->max_jobs);
-        S->order = ctx->alloc_n<uint32_t>(d->max_jobs);
-        S->queue = ctx->alloc_zero<unsigned>(1);
         VSMCX_HIP(hipEventCreate(&S->ev0));
         VSMCX_HIP(hipEventCreate(&S->ev1));
         ctx->sync();
@@ -356,11 +368,6 @@ int vsmcx_serve_create(vsmcx_ctx *ctx, const vsmcx_serve_desc *d, vsmcx_serve **
         S->release_all();
         throw;
     }
-    int cus = 256;
-    hipDeviceProp_t prop;
-    if (hipGetDeviceProperties(&prop, ctx->device) == hipSuccess && prop.multiProcessorCount > 0)
-        cus = prop.multiProcessorCount;
-    S->grid_cap = (unsigned)cus * (S->scheme == kStratified ? 2 : 3);
     S->slot_job.assign(d->max_jobs, 0);
     S->hF.resize(d->max_jobs);
     S->hC.resize(d->max_jobs);
