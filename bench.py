@@ -172,6 +172,8 @@ def run_serving(seconds, rate, seed, budget_gb=200.0, max_jobs=8192):
     ids = {}
     fed = {}
     data = {}
+    fully_fed = set()
+    finished = 0
     t0 = time.perf_counter()
     nxt = 0
     while True:
@@ -187,10 +189,15 @@ def run_serving(seconds, rate, seed, budget_gb=200.0, max_jobs=8192):
             ys = [data[int(j)][fed[int(j)]] for j in ready]
             s.observe_many(ready, ys)
             for j in ready:
-                fed[int(j)] += 1
+                j = int(j)
+                fed[j] += 1
+                if fed[j] == len(data[j]):
+                    fully_fed.add(j)
         launched = s.poll(wait=True)
-        st = s.stats()
-        if nxt >= k and st["jobs_done"] == k:
+        for j in [j for j in fully_fed if s.job(j)[0] == 2]:
+            fully_fed.discard(j)
+            finished += 1
+        if nxt >= k and finished == k:
             break
         if launched == 0 and nxt < k:
             time.sleep(max(0.0, min(1e-3, t_arr[nxt] - (time.perf_counter() - t0))))

@@ -356,6 +356,44 @@ __global__ void k_tick_advance(const uint32_t *jobs, uint32_t njobs, BatchPool P
     if (e) atomicOr(P.err_any, e);
 }
 
+// One workgroup per slot initialisation, per observation run, and per 64
+// availability counts; every record is written by exactly one workgroup.
+__global__ void __launch_bounds__(64) k_tick_upload(TickUpload u, BatchFilter *filters, BatchPool P)
+{
+    const uint32_t b = blockIdx.x;
+    if (b < u.n_init) {
+        const SlotInit &r = u.init[b];
+        static_assert(sizeof(BatchFilter) % 4 == 0 && sizeof(FilterCtl) % 4 == 0, "word copies");
+        const uint32_t *fs = reinterpret_cast<const uint32_t *>(&r.f);
+        const uint32_t *cs = reinterpret_cast<const uint32_t *>(&r.c);
+        uint32_t *fd = reinterpret_cast<uint32_t *>(filters + r.slot);
+        uint32_t *cd = reinterpret_cast<uint32_t *>(P.ctl + r.slot);
+        for (uint32_t w = threadIdx.x; w < sizeof(BatchFilter) / 4; w += 64) fd[w] = fs[w];
+        for (uint32_t w = threadIdx.x; w < sizeof(FilterCtl) / 4; w += 64) cd[w] = cs[w];
+        if (threadIdx.x == 0) {
+            P.next_t[r.slot] = 0;
+            P.cur[r.slot] = 0;
+        }
+        return;
+    }
+    if (b < u.n_init + u.n_runs) {
+        const ObsRun &r = u.runs[b - u.n_init];
+        double *obs = const_cast<double *>(P.obs);  // read-only to the step kernels
+        for (uint32_t k = threadIdx.x; k < r.count; k += 64) obs[r.dst + k] = u.pool[r.src + k];
+        return;
+    }
+    const uint32_t k = (b - u.n_init - u.n_runs) * 64 + threadIdx.x;
+    if (k < u.n_avail) const_cast<uint64_t *>(P.avail)[u.avail[k].slot] = u.avail[k].avail;
+}
+
+void launch_tick_upload(const TickUpload &u, BatchFilter *filters, const BatchPool &P,
+                        hipStream_t st)
+{
+    const uint32_t g = u.n_init + u.n_runs + (u.n_avail + 63) / 64;
+    if (!g) return;
+    hipLaunchKernelGGL(k_tick_upload, dim3(g), dim3(64), 0, st, u, filters, P);
+}
+
 void launch_tick(bool fp64, int scheme, const TickItem *items, uint32_t nitems,
                  const uint32_t *jobs, uint32_t njobs, const BatchFilter *filters,
                  const BatchPool &P, const CvParams &cv, const LgParams &lg, hipStream_t st)

@@ -122,6 +122,37 @@ struct TickItem {
     uint32_t tile;
 };
 
+// One tick's host-to-device traffic, packed into one pinned buffer and sent
+// with one copy: slots (re)initialised for admitted jobs, runs of new
+// observations, and the observation count now on the device per slot.
+struct SlotInit {
+    BatchFilter f;
+    FilterCtl c;
+    uint32_t slot;
+    uint32_t pad[3];
+};
+struct ObsRun {
+    uint64_t dst;    // first double in BatchPool::obs
+    uint32_t src;    // first double in the staged pool
+    uint32_t count;  // doubles
+};
+struct AvailSet {
+    uint32_t slot;
+    uint32_t pad;
+    uint64_t avail;
+};
+struct TickUpload {
+    const SlotInit *init;
+    const ObsRun *runs;
+    const AvailSet *avail;
+    const double *pool;
+    uint32_t n_init, n_runs, n_avail, pad;
+};
+
+// kernels_tick.hip: apply a staged upload (one launch; nothing if it is empty)
+void launch_tick_upload(const TickUpload &u, BatchFilter *filters, const BatchPool &P,
+                        hipStream_t st);
+
 // kernels_tick.hip: one step of every listed job, one grid launch per phase
 void launch_tick(bool fp64, int scheme, const TickItem *items, uint32_t nitems,
                  const uint32_t *jobs, uint32_t njobs, const BatchFilter *filters,

@@ -92,8 +92,8 @@ struct Job {
 };
 
 constexpr uint64_t kUnit = kFoldTile;  // particle-arena unit
-const uint64_t kZero64 = 0;
-const uint32_t kZero32 = 0;
+
+uint64_t align16(uint64_t b) { return (b + 15) & ~(uint64_t)15; }
 
 }  // namespace
 
@@ -110,14 +110,20 @@ struct vsmcx_serve {
     uint64_t cap_units = 0, cap_state = 0, cap_obs = 0;
     BatchPool P{};
     BatchFilter *df = nullptr;
-    TickItem *items = nullptr;     // device: the tick's (job, tile) items
-    uint32_t *jobs_dev = nullptr;  // device: the tick's jobs
     std::vector<TickItem> items_h;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     std::vector<int> free_slots;
     std::vector<uint64_t> slot_job;
-    std::vector<BatchFilter> hF;  // per-slot host mirrors (copy sources)
-    std::vector<FilterCtl> hC;
+    // one tick's upload, staged on the host and sent as one copy
+    std::vector<SlotInit> st_init;
+    std::vector<ObsRun> st_runs;
+    std::vector<AvailSet> st_avail;
+    std::vector<double> st_pool;
+    std::vector<uint8_t> st_mark;  // per slot: already in st_avail
+    uint64_t pool_cap = 0, stage_cap = 0;
+    uint8_t *stage_h = nullptr;    // pinned
+    uint8_t *stage_d = nullptr;
+    int *err_h = nullptr;          // pinned: the device error bits after the last tick
     std::unordered_map<uint64_t, Job> jobs;
     std::deque<uint64_t> waiting;
     uint64_t next_id = 1;
@@ -135,13 +141,26 @@ struct vsmcx_serve {
     void release_all()
     {
         void *ps[] = {df, P.st0, P.st1, P.lw, P.anc, P.zl, P.sl, P.rep32,
-                      P.spec, P.gval, P.ts, P.tagg, P.tpre, P.tzs, P.tsum, P.tzsx, P.tpart, items,
-                      jobs_dev, P.ctl, P.hist, (void *)P.obs,
+                      P.spec, P.gval, P.ts, P.tagg, P.tpre, P.tzs, P.tsum, P.tzsx, P.tpart,
+                      stage_d, P.ctl, P.hist, (void *)P.obs,
                       P.next_t, (void *)P.avail,
                       P.cur, P.err_any};
         for (void *q : ps) ctx->release(q);
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
+        if (stage_h) (void)hipHostFree(stage_h);
+        if (err_h) (void)hipHostFree(err_h);
+    }
+
+    void mark_avail(const Job &J)
+    {
+        if (st_mark[J.slot]) {
+            for (AvailSet &a : st_avail)
+                if (a.slot == (uint32_t)J.slot) a.avail = J.uploaded;
+            return;
+        }
+        st_mark[J.slot] = 1;
+        st_avail.push_back(AvailSet{(uint32_t)J.slot, 0, J.uploaded});
     }
 
     // admit waiting jobs in arrival order while slots and memory last
@@ -177,8 +196,9 @@ struct vsmcx_serve {
             slot_job[slot] = waiting.front();
             waiting.pop_front();
             peak_units = std::max(peak_units, parts.used());
-            BatchFilter &F = hF[slot];
-            F = BatchFilter{};
+            SlotInit r{};
+            r.slot = (uint32_t)slot;
+            BatchFilter &F = r.f;
             F.n = J.d.n;
             F.horizon = J.d.horizon;
             F.off = po * kUnit;
@@ -187,38 +207,32 @@ struct vsmcx_serve {
             F.hist_off = (uint64_t)slot * max_horizon;
             F.seed = J.d.seed;
             F.model = J.d.model == VSMCX_MODEL_LG ? kModelLg : kModelCv;
-            FilterCtl &h = hC[slot];
-            h = FilterCtl{};
+            FilterCtl &h = r.c;
             h.n = J.d.n;
             h.threshold = threshold;
             h.scheme = scheme;
             h.eng.key[0] = J.d.resample_seed;
             h.eng.index = 4;
-            // sources are per-slot host mirrors (stable until the slot is reused,
-            // which is after this job's last tick has completed)
-            VSMCX_HIP(hipMemcpyAsync(df + slot, &F, sizeof(F), hipMemcpyHostToDevice, s));
-            VSMCX_HIP(hipMemcpyAsync(P.ctl + slot, &h, sizeof(h), hipMemcpyHostToDevice, s));
-            VSMCX_HIP(hipMemcpyAsync(P.next_t + slot, &kZero64, 8, hipMemcpyHostToDevice, s));
-            VSMCX_HIP(hipMemcpyAsync((uint64_t *)P.avail + slot, &kZero64, 8, hipMemcpyHostToDevice, s));
-            VSMCX_HIP(hipMemcpyAsync(P.cur + slot, &kZero32, 4, hipMemcpyHostToDevice, s));
+            st_init.push_back(r);
+            J.uploaded = 0;
+            mark_avail(J);
             upload(J);
         }
     }
 
-    // observations received but not yet on the device
+    // stage observations received but not yet on the device (whole
+    // observations, as many as the staged pool holds; the rest go next tick)
     void upload(Job &J)
     {
         if (J.state != 1 || J.uploaded == J.received) return;
-        const int D = obs_dim(J.d.model);
-        hipStream_t s = ctx->stream;
-        VSMCX_HIP(hipMemcpyAsync((double *)P.obs + J.o_off + (uint64_t)D * J.uploaded,
-                                 J.obs.data() + (size_t)D * J.uploaded,
-                                 8 * D * (J.received - J.uploaded), hipMemcpyHostToDevice, s));
-        J.uploaded = J.received;
-        // J lives in a node-based map and J.obs was reserved for the whole
-        // horizon, so both sources stay valid until the copies run
-        VSMCX_HIP(hipMemcpyAsync((uint64_t *)P.avail + J.slot, &J.uploaded, 8,
-                                 hipMemcpyHostToDevice, s));
+        const uint64_t D = obs_dim(J.d.model);
+        const uint64_t room = (pool_cap - st_pool.size()) / D;
+        const uint64_t k = std::min<uint64_t>(room, J.received - J.uploaded);
+        if (!k) return;
+        st_runs.push_back(ObsRun{J.o_off + D * J.uploaded, (uint32_t)st_pool.size(), (uint32_t)(D * k)});
+        st_pool.insert(st_pool.end(), J.obs.begin() + D * J.uploaded, J.obs.begin() + D * (J.uploaded + k));
+        J.uploaded += k;
+        mark_avail(J);
     }
 
     // the tick in flight has finished: account for it, retire finished jobs
@@ -228,8 +242,7 @@ struct vsmcx_serve {
         float ms = 0;
         VSMCX_HIP(hipEventElapsedTime(&ms, ev0, ev1));
         busy_ms += ms;
-        int e = 0;
-        VSMCX_HIP(hipMemcpy(&e, P.err_any, sizeof(int), hipMemcpyDeviceToHost));
+        const int e = *err_h;
         if (e & kErrDegenerate) throw DevError(VSMCX_ERR_DEGENERATE, "a job's weights sum to zero or are not finite");
         if (e) throw DevError(VSMCX_ERR_INTERNAL, "device consistency check failed (bits " + std::to_string(e) + ")");
         for (uint32_t slot : tick) {
@@ -257,8 +270,8 @@ struct vsmcx_serve {
         t_last = t;
     }
 
-    // one tick: every running job with an observation on the device and not
-    // yet stepped
+    // one tick: the staged upload, then one step of every running job with an
+    // observation on the device and not yet stepped.  Returns the jobs stepped.
     uint64_t launch()
     {
         tick.clear();
@@ -267,21 +280,51 @@ struct vsmcx_serve {
             const Job &J = jobs[slot_job[slot]];
             if (J.state == 1 && J.uploaded > J.done) tick.push_back((uint32_t)slot);
         }
-        if (tick.empty()) return 0;
         items_h.clear();
         for (uint32_t slot : tick) {
             const uint64_t T = (jobs[slot_job[slot]].d.n + kUnit - 1) / kUnit;
             for (uint64_t tt = 0; tt < T; ++tt) items_h.push_back(TickItem{slot, (uint32_t)tt});
         }
+        if (tick.empty() && st_init.empty() && st_runs.empty() && st_avail.empty()) return 0;
+        // pack: init | runs | avail | jobs | items | pool, each 16-byte aligned
+        uint64_t o = 0;
+        const uint64_t o_init = o; o = align16(o + sizeof(SlotInit) * st_init.size());
+        const uint64_t o_runs = o; o = align16(o + sizeof(ObsRun) * st_runs.size());
+        const uint64_t o_av = o;   o = align16(o + sizeof(AvailSet) * st_avail.size());
+        const uint64_t o_jobs = o; o = align16(o + 4 * tick.size());
+        const uint64_t o_items = o; o = align16(o + sizeof(TickItem) * items_h.size());
+        const uint64_t o_pool = o; o = align16(o + 8 * st_pool.size());
+        if (o > stage_cap) throw DevError(VSMCX_ERR_INTERNAL, "tick upload exceeds its staging buffer");
+        std::memcpy(stage_h + o_init, st_init.data(), sizeof(SlotInit) * st_init.size());
+        std::memcpy(stage_h + o_runs, st_runs.data(), sizeof(ObsRun) * st_runs.size());
+        std::memcpy(stage_h + o_av, st_avail.data(), sizeof(AvailSet) * st_avail.size());
+        std::memcpy(stage_h + o_jobs, tick.data(), 4 * tick.size());
+        std::memcpy(stage_h + o_items, items_h.data(), sizeof(TickItem) * items_h.size());
+        std::memcpy(stage_h + o_pool, st_pool.data(), 8 * st_pool.size());
         hipStream_t s = ctx->stream;
-        VSMCX_HIP(hipMemcpyAsync(jobs_dev, tick.data(), 4 * tick.size(), hipMemcpyHostToDevice, s));
-        VSMCX_HIP(hipMemcpyAsync(items, items_h.data(), sizeof(TickItem) * items_h.size(),
-                                 hipMemcpyHostToDevice, s));
         VSMCX_HIP(hipEventRecord(ev0, s));
-        launch_tick(fp64, scheme, items, (uint32_t)items_h.size(), jobs_dev,
-                    (uint32_t)tick.size(), df, P, cv, lg, s);
+        VSMCX_HIP(hipMemcpyAsync(stage_d, stage_h, o, hipMemcpyHostToDevice, s));
+        TickUpload u{};
+        u.init = (const SlotInit *)(stage_d + o_init);
+        u.runs = (const ObsRun *)(stage_d + o_runs);
+        u.avail = (const AvailSet *)(stage_d + o_av);
+        u.pool = (const double *)(stage_d + o_pool);
+        u.n_init = (uint32_t)st_init.size();
+        u.n_runs = (uint32_t)st_runs.size();
+        u.n_avail = (uint32_t)st_avail.size();
+        launch_tick_upload(u, df, P, s);
+        launch_tick(fp64, scheme, (const TickItem *)(stage_d + o_items), (uint32_t)items_h.size(),
+                    (const uint32_t *)(stage_d + o_jobs), (uint32_t)tick.size(), df, P, cv, lg, s);
         VSMCX_HIP(hipGetLastError());
+        VSMCX_HIP(hipMemcpyAsync(err_h, P.err_any, sizeof(int), hipMemcpyDeviceToHost, s));
         VSMCX_HIP(hipEventRecord(ev1, s));
+        for (const AvailSet &a : st_avail) st_mark[a.slot] = 0;
+        st_init.clear();
+        st_runs.clear();
+        st_avail.clear();
+        st_pool.clear();
+        // the staging buffer is in use until ev1: the next tick is staged only
+        // after complete() has waited for it
         inflight = true;
         if (t_first < 0) t_first = now_us();
         return tick.size();
@@ -351,8 +394,16 @@ int vsmcx_serve_create(vsmcx_ctx *ctx, const vsmcx_serve_desc *d, vsmcx_serve **
         P.tsum = ctx->alloc_n<double>(S->cap_units);
         P.tzsx = ctx->alloc_n<uint64_t>(S->cap_units);
         P.tpart = ctx->alloc_n<double>(5 * S->cap_units);
-        S->items = ctx->alloc_n<TickItem>(S->cap_units);
-        S->jobs_dev = ctx->alloc_n<uint32_t>(d->max_jobs);
+        // staging: every slot initialised, one run and one count per slot,
+        // the job and item lists of a tick, and the observation pool
+        S->pool_cap = std::max<uint64_t>(1 << 16, (uint64_t)d->max_jobs * std::max<uint32_t>(2, d->lg_dim) * 2);
+        S->stage_cap = align16(sizeof(SlotInit) * d->max_jobs) + align16(sizeof(ObsRun) * d->max_jobs) +
+                       align16(sizeof(AvailSet) * d->max_jobs) + align16(4 * (uint64_t)d->max_jobs) +
+                       align16(sizeof(TickItem) * S->cap_units) + align16(8 * S->pool_cap);
+        S->stage_d = (uint8_t *)ctx->alloc(S->stage_cap);
+        VSMCX_HIP(hipHostMalloc((void **)&S->stage_h, S->stage_cap, hipHostMallocDefault));
+        VSMCX_HIP(hipHostMalloc((void **)&S->err_h, sizeof(int), hipHostMallocDefault));
+        *S->err_h = 0;
         P.ctl = ctx->alloc_n<FilterCtl>(d->max_jobs);
         P.hist = ctx->alloc_n<StepRow>((uint64_t)d->max_jobs * d->max_horizon);
         P.obs = ctx->alloc_n<double>(S->cap_obs);
@@ -369,8 +420,7 @@ int vsmcx_serve_create(vsmcx_ctx *ctx, const vsmcx_serve_desc *d, vsmcx_serve **
         throw;
     }
     S->slot_job.assign(d->max_jobs, 0);
-    S->hF.resize(d->max_jobs);
-    S->hC.resize(d->max_jobs);
+    S->st_mark.assign(d->max_jobs, 0);
     for (int i = (int)d->max_jobs - 1; i >= 0; --i) S->free_slots.push_back(i);
     *out = S.release();
     API_END
@@ -415,7 +465,6 @@ int vsmcx_serve_observe(vsmcx_serve *s, uint64_t job, const double *y)
     J.obs.insert(J.obs.end(), y, y + D);
     J.arrival.push_back(now_us());
     J.received += 1;
-    if (!s->inflight) s->upload(J);
     API_END
 }
 
@@ -516,11 +565,13 @@ int vsmcx_serve_stats(vsmcx_serve *s, vsmcx_serve_summary *out)
     for (const auto &kv : s->jobs) active += kv.second.state == 1;
     r.jobs_active = active;
     r.jobs_waiting = s->waiting.size();
-    std::vector<double> l = s->lat_us;
-    if (!l.empty()) {
-        std::sort(l.begin(), l.end());
-        r.p50_us = l[(size_t)(0.50 * (l.size() - 1))];
-        r.p99_us = l[(size_t)(0.99 * (l.size() - 1))];
+    if (!s->lat_us.empty()) {
+        std::vector<double> l = s->lat_us;
+        const size_t i50 = (size_t)(0.50 * (l.size() - 1)), i99 = (size_t)(0.99 * (l.size() - 1));
+        std::nth_element(l.begin(), l.begin() + i99, l.end());
+        r.p99_us = l[i99];
+        std::nth_element(l.begin(), l.begin() + i50, l.begin() + i99);
+        r.p50_us = i50 == i99 ? r.p99_us : l[i50];
     }
     r.busy_ms = s->busy_ms;
     r.elapsed_ms = s->t_first < 0 ? 0 : (s->t_last - s->t_first) * 1e-3;
