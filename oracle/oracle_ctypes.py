@@ -51,6 +51,20 @@ STEP_DTYPE = np.dtype([("ess", "f8"), ("log_z", "f8"), ("mean_x", "f8"), ("mean_
                        ("resampled", "i4"), ("draws", "u8")], align=True)
 assert STEP_DTYPE.itemsize == C.sizeof(StepRecord)
 
+GMM_MAX_K = 6
+
+
+class GmmConfig(C.Structure):
+    _fields_ = [("K", C.c_int), ("sweeps", C.c_uint32), ("data", C.c_void_p),
+                ("ndata", C.c_size_t), ("threshold", C.c_double), ("cess_drop", C.c_double),
+                ("scheme", C.c_int), ("seed", C.c_uint64), ("resample_seed", C.c_uint64),
+                ("max_steps", C.c_size_t)]
+
+
+GMM_DTYPE = np.dtype([("alpha", "f8"), ("alpha_inc", "f8"), ("ess", "f8"), ("log_z", "f8"),
+                      ("path_integrand", "f8"), ("path_log_z", "f8"), ("resampled", "i4"),
+                      ("bisections", "u4"), ("accept", "u8", (3,))], align=True)
+
 
 def cv_param(**kw):
     """Config-0 CV model constants (dt 0.1, sd 0.1/0.5, Student-t nu=10 scale 0.1)."""
@@ -112,6 +126,18 @@ class Oracle:
         L.orc_cv_run_timed.argtypes = [C.POINTER(CvParam), C.c_size_t, C.c_size_t, f64p,
                                        C.c_uint64, C.c_uint64, C.c_int, C.c_double, C.c_void_p,
                                        C.c_int, f64p]
+        L.orc_gmm_prior.argtypes = [f64p, C.c_size_t, f64p]
+        L.orc_gmm_log_prior.argtypes = [C.c_int, f64p, f64p, f64p]
+        L.orc_gmm_log_prior.restype = C.c_double
+        L.orc_gmm_log_likelihood.argtypes = [C.c_int, f64p, f64p, f64p, f64p, C.c_size_t]
+        L.orc_gmm_log_likelihood.restype = C.c_double
+        L.orc_gmm_init_particle.argtypes = [C.c_int, C.c_uint64, C.c_uint64, f64p, f64p]
+        L.orc_gmm_mh_move.argtypes = [C.c_int, C.c_int, C.c_uint64, C.c_uint64, C.c_uint32,
+                                      C.c_uint32, C.c_double, f64p, f64p, f64p, C.c_size_t, f64p,
+                                      C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        L.orc_gmm_mh_move.restype = C.c_int
+        L.orc_gmm_run.argtypes = [C.POINTER(GmmConfig), C.c_size_t, C.c_void_p, C.c_void_p,
+                                  C.c_void_p, C.c_void_p, C.POINTER(C.c_size_t), C.c_int]
 
     # --- engines ---
     def tf64_engine(self, seed):
@@ -266,6 +292,58 @@ class Oracle:
         if rc != 0:
             raise MemoryError("orc_cv_run_timed failed")
         return rec, phase
+
+    # --- GMM tempered SMC ---
+    def gmm_prior(self, data):
+        data = np.ascontiguousarray(data, np.float64)
+        out = np.zeros(4)
+        self.lib.orc_gmm_prior(data, data.size, out)
+        return out
+
+    def gmm_log_prior(self, K, p, prior):
+        p = np.ascontiguousarray(p, np.float64)
+        return self.lib.orc_gmm_log_prior(K, p[:K].copy(), p[K:2 * K].copy(),
+                                          np.ascontiguousarray(prior, np.float64))
+
+    def gmm_log_likelihood(self, K, p, data):
+        p = np.ascontiguousarray(p, np.float64)
+        data = np.ascontiguousarray(data, np.float64)
+        return self.lib.orc_gmm_log_likelihood(K, p[:K].copy(), p[K:2 * K].copy(),
+                                               p[2 * K:3 * K].copy(), data, data.size)
+
+    def gmm_init_particle(self, K, seed, i, prior):
+        out = np.zeros(3 * K)
+        self.lib.orc_gmm_init_particle(K, seed, i, np.ascontiguousarray(prior, np.float64), out)
+        return out
+
+    def gmm_mh_move(self, K, move, seed, i, it, sweep, alpha, sd, prior, data, p, llh, lp):
+        p = np.array(p, np.float64)
+        a, b = C.c_double(llh), C.c_double(lp)
+        data = np.ascontiguousarray(data, np.float64)
+        acc = self.lib.orc_gmm_mh_move(K, move, seed, i, it, sweep, alpha,
+                                       np.ascontiguousarray(sd, np.float64),
+                                       np.ascontiguousarray(prior, np.float64), data, data.size,
+                                       p, C.byref(a), C.byref(b))
+        return acc, p, a.value, b.value
+
+    def gmm_run(self, N, data, K=4, sweeps=5, seed=1, resample_seed=2, scheme=STRATIFIED,
+                threshold=0.5, cess_drop=0.1, max_steps=1000, threads=0, want_state=True):
+        data = np.ascontiguousarray(data, np.float64)
+        cfg = GmmConfig(K, sweeps, data.ctypes.data, data.size, threshold, cess_drop, scheme,
+                        seed, resample_seed, max_steps)
+        rec = np.zeros(max_steps + 1, GMM_DTYPE)
+        params = np.zeros((N, 3 * K)) if want_state else None
+        llh = np.zeros(N) if want_state else None
+        lw = np.zeros(N) if want_state else None
+        rows = C.c_size_t()
+        rc = self.lib.orc_gmm_run(C.byref(cfg), N,
+                                  params.ctypes.data if want_state else None,
+                                  llh.ctypes.data if want_state else None,
+                                  lw.ctypes.data if want_state else None,
+                                  rec.ctypes.data, C.byref(rows), threads)
+        if rc != 0:
+            raise RuntimeError(f"orc_gmm_run failed ({rc})")
+        return params, llh, lw, rec[:rows.value]
 
 
 class RefHarness:

@@ -134,6 +134,43 @@ int orc_cv_run_timed(const orc_cv_param *p, size_t N, size_t T, const double *ob
                      uint64_t seed, uint64_t resample_seed, int scheme, double threshold,
                      orc_step_record *rec, int threads, double phase[4]);
 
+/* ---------------- GMM tempered SMC (gmm_oracle.c) ---------------- */
+#define ORC_GMM_MAX_K 6
+typedef struct {
+    int K;               /* mixture components, 1 .. ORC_GMM_MAX_K */
+    uint32_t sweeps;     /* MH sweeps per iteration (each: mu, lambda, weight) */
+    const double *data;
+    size_t ndata;
+    double threshold;    /* resample when ESS < threshold * N */
+    double cess_drop;    /* next alpha keeps CESS = 1 - cess_drop */
+    int scheme;          /* resampling scheme */
+    uint64_t seed, resample_seed;
+    size_t max_steps;
+} orc_gmm_config;
+
+typedef struct {
+    double alpha, alpha_inc, ess, log_z, path_integrand, path_log_z;
+    int32_t resampled;
+    uint32_t bisections;
+    uint64_t accept[3];
+} orc_gmm_record;
+
+/* prior = {mu0, sd0, shape0, scale0} from the data range */
+void orc_gmm_prior(const double *data, size_t n, double prior[4]);
+double orc_gmm_log_prior(int K, const double *mu, const double *lambda, const double prior[4]);
+double orc_gmm_log_likelihood(int K, const double *mu, const double *lambda, const double *weight,
+                              const double *data, size_t n);
+/* p = {mu[K], lambda[K], weight[K]} */
+void orc_gmm_init_particle(int K, uint64_t seed, uint64_t id, const double prior[4], double *p);
+int orc_gmm_mh_move(int K, int move, uint64_t seed, uint64_t id, uint32_t iter, uint32_t sweep,
+                    double alpha, const double sd[3], const double prior[4], const double *data,
+                    size_t n, double *p, double *llh, double *lp);
+/* Whole run (initialize, then iterate until alpha = 1 or max_steps).
+ * params_out: N x 3K, llh_out, lw_out: N (any may be NULL); rec: max_steps + 1
+ * rows (row 0 = initialize); *rows = rows written. */
+int orc_gmm_run(const orc_gmm_config *c, size_t N, double *params_out, double *llh_out,
+                double *lw_out, orc_gmm_record *rec, size_t *rows, int threads);
+
 #ifdef __cplusplus
 }
 #endif
