@@ -218,6 +218,64 @@ def run_serving(seconds, rate, seed, budget_gb=200.0, max_jobs=8192):
                       f"the previous step completes)"}
 
 
+GMM_FLOPS_PER_POINT = 6  # per component: x - mu, * A, FMA (2), exp2, accumulate
+
+
+def run_gmm(N, sweeps, seed, precision=V.MIXED, K=4, ndata=1000):
+    """Config-2 workload: K = 4 mixture, ndata points from means (-3, 0, 2, 5)
+    with component sds drawn in [0.5, 1.5], N particles x (4 mu, 4 lambda,
+    4 weights), adaptive tempering to CESS = 0.9, `sweeps` sweeps of the three
+    RW-MH moves per iteration, stratified resampling at ESS < N/2; run from
+    the prior to alpha = 1."""
+    y = V.gmm_simulate(ndata, seed)
+    ctx = V.Context(0)
+    g = V.Gmm(ctx, y, N, components=K, sweeps=sweeps, precision=precision, seed=seed + 1,
+              resample_seed=seed + 2)
+    t0 = time.perf_counter()
+    g.initialize()
+    iters, alpha = g.iterate()
+    wall = time.perf_counter() - t0
+    h = g.history()
+    t_temper, t_res, t_mh = g.times()
+    g.close()
+    ctx.close()
+    evals = float(N) * 3 * sweeps * iters
+    flops = evals * ndata * (GMM_FLOPS_PER_POINT * K + 2)
+    out = {"iterations": iters, "alpha": alpha, "wall_s": wall,
+           "ms_per_iteration": wall * 1e3 / max(iters, 1),
+           "particle_iterations_per_s": N * iters / wall,
+           "mh_evaluations_per_s": evals / wall,
+           "device_ms": {"temper_reweight": t_temper, "resample": t_res, "mh_sweeps": t_mh},
+           "log_z": float(h["log_z"][-1]), "path_log_z": float(h["path_log_z"][-1]),
+           "resample_frac": float(h["resampled"][1:].mean()) if iters else 0.0,
+           "accept_rate": (h["accept"][1:].sum(0) / (float(N) * sweeps * max(iters, 1))).tolist()}
+    if t_mh > 0:
+        achieved = flops / (t_mh * 1e-3) / 1e12
+        out["roofline"] = {"bound": "valu", "achieved": achieved, "peak": 157.3,
+                           "unit": "TFLOP/s", "frac": achieved / 157.3, "traffic": None,
+                           "kernel": "k_gmm_mh",
+                           "flops_per_point": GMM_FLOPS_PER_POINT * K + 2}
+    return out
+
+
+def cpu_gmm_baseline(N, sweeps, seed, threads, K=4, ndata=1000):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    from oracle_ctypes import Oracle
+    o = Oracle()
+    y = V.gmm_simulate(ndata, seed)
+    t0 = time.perf_counter()
+    _, _, _, rec = o.gmm_run(N, y, K=K, sweeps=sweeps, seed=seed + 1, resample_seed=seed + 2,
+                             threads=threads, want_state=False)
+    dt = time.perf_counter() - t0
+    iters = len(rec) - 1
+    return {"value": float(N) * 3 * sweeps * iters / dt, "unit": "mh-evaluations/s",
+            "cores": threads, "kind": "port",
+            "sample": f"GMM oracle (oracle/gmm_oracle.c, FP64, OpenMP over particles), N={N}, "
+                      f"{ndata} points, {sweeps} sweeps, whole run to alpha = 1 ({iters} "
+                      f"iterations)",
+            "seconds": round(dt, 3), "log_z": float(rec["log_z"][-1])}
+
+
 def cpu_baseline(N, steps, threads):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     from oracle_ctypes import Oracle
@@ -251,6 +309,12 @@ def main():
     ap.add_argument("--serve-seconds", type=float, default=0.5)
     ap.add_argument("--serve-rate", type=float, default=2000.0)
     ap.add_argument("--batch-filters", type=int, default=4096)
+    ap.add_argument("--no-gmm", action="store_true")
+    ap.add_argument("--gmm-n", type=int, default=1 << 20)
+    ap.add_argument("--gmm-sweeps", type=int, default=5)
+    ap.add_argument("--gmm-fp64", action="store_true",
+                    help="also run the GMM workload in FP64 at the same size")
+    ap.add_argument("--gmm-cpu-n", type=int, default=4096)
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a device-to-device copy of 1 GiB five times (calibrates the "
                          "FETCH_SIZE / WRITE_SIZE counters against known bytes) and exit")
@@ -329,6 +393,19 @@ def main():
         print(f"[bench] serving: {a.serve_rate}/s for {a.serve_seconds}s", file=sys.stderr,
               flush=True)
         out["serving"] = run_serving(a.serve_seconds, a.serve_rate, 17)
+    if not a.no_gmm:
+        print(f"[bench] gmm: N={a.gmm_n}, {a.gmm_sweeps} sweeps", file=sys.stderr, flush=True)
+        gm = run_gmm(a.gmm_n, a.gmm_sweeps, 23)
+        if a.gmm_fp64:
+            gm["fp64"] = run_gmm(a.gmm_n, a.gmm_sweeps, 23, precision=V.FP64_FUSED)
+        if not a.no_cpu:
+            gm["cpu_baseline"] = cpu_gmm_baseline(a.gmm_cpu_n, a.gmm_sweeps, 23,
+                                                  min(16, os.cpu_count() or 1))
+        gm["config"] = ("config-2: K=4 GMM, 1000 points from means (-3, 0, 2, 5), sds U[0.5, 1.5], "
+                        f"N={a.gmm_n} x 12 FP32 parameters, CESS 0.9 tempering, "
+                        f"{a.gmm_sweeps} RW-MH sweeps (mu, lambda, weight) per iteration, "
+                        "stratified resampling at ESS < N/2, mixed precision")
+        out["gmm"] = gm
     if not a.no_cpu:
         threads = min(16, os.cpu_count() or 1)
         print(f"[bench] cpu baseline: N={a.cpu_n}, {a.cpu_steps} steps, {threads} threads",

@@ -225,9 +225,10 @@ int vsmcx_batch_particles(vsmcx_batch *b, uint64_t *total);
  * slot and room in the HBM arena are free, keep their particle state on the
  * device until their last step, and are advanced with continuous batching:
  * each vsmcx_serve_poll finishes the tick in flight, admits jobs, and launches
- * one persistent kernel over every running job with an observation waiting
- * (one step each).  A job's results do not depend on the other jobs.  No
- * counterpart in vSMC (one Sampler per job, stepped by its caller). */
+ * one tick: one step of every running job with an observation waiting, each
+ * phase of the step one grid launch over the tiles of all those jobs.  A
+ * job's results do not depend on the other jobs.  No counterpart in vSMC
+ * (one Sampler per job, stepped by its caller). */
 #define VSMCX_MODEL_CV 0  /* the pf example's constant-velocity model */
 #define VSMCX_MODEL_LG 1  /* linear-Gaussian: x_t = A x_{t-1} + q z, y_t = x_t + r v */
 
@@ -289,6 +290,64 @@ int vsmcx_serve_job(vsmcx_serve *s, uint64_t id, int *state, uint64_t *steps);
 int vsmcx_serve_history(vsmcx_serve *s, uint64_t id, uint64_t count, vsmcx_step_stats *out);
 int vsmcx_serve_forget(vsmcx_serve *s, uint64_t id);
 int vsmcx_serve_stats(vsmcx_serve *s, vsmcx_serve_summary *out);
+
+/* ---- Gaussian-mixture SMC with adaptive tempering ------------------------------
+ * The reference's gmm_smc sampler (example/gmm/src/gmm_smc.cpp:43-52 driven by
+ * example/include/smc_do.hpp:57-66 under the "cess" schedule): particles carry
+ * K means, precisions and weights; each iteration picks the next temperature
+ * alpha by bisection so that the conditional ESS of the reweighting is
+ * 1 - cess_drop (example/include/move_alpha.hpp:242-279), reweights by
+ * alpha_inc * log-likelihood (example/include/move_smc.hpp:57-77), resamples
+ * when ESS < threshold * n, and runs `sweeps` sweeps of the random-walk MH
+ * moves on the means, the precisions and the weights
+ * (example/gmm/include/gmm_move.hpp:35-121).  Iterations continue until
+ * alpha = 1.  Records carry the SMC and the path-sampling log evidence
+ * (example/include/smc_do.hpp:75-79, include/vsmc/core/path.hpp:229-239). */
+typedef struct vsmcx_gmm vsmcx_gmm;
+
+typedef struct {
+    uint64_t n;            /* particles */
+    uint32_t components;   /* K, 1 .. 6 */
+    uint32_t sweeps;       /* MH sweeps per iteration, each: mu, lambda, weight moves */
+    int precision;         /* VSMCX_MIXED (FP32 parameters and likelihood terms,
+                              FP64 weights, search and evidence) or VSMCX_FP64_FUSED */
+    int scheme;            /* resampling scheme (the reference sampler: VSMCX_STRATIFIED) */
+    double threshold;      /* resample when ESS < threshold * n (0.5) */
+    double cess_drop;      /* next alpha keeps CESS = 1 - cess_drop (0.1) */
+    uint64_t seed;         /* Philox4x32 key of the initial draws and the MH moves */
+    uint64_t resample_seed;/* Threefry4x64 seed of the resample engine */
+    uint64_t max_steps;    /* iteration cap (history rows = max_steps + 1) */
+} vsmcx_gmm_desc;
+
+typedef struct {
+    double alpha, alpha_inc;    /* temperature after the iteration and its increment */
+    double ess;                 /* ESS after the reweighting (before any resample) */
+    double log_z;               /* SMC log evidence so far (with the likelihood constant) */
+    double path_integrand;      /* sum_i W_i log-likelihood_i after the MH sweeps */
+    double path_log_z;          /* path-sampling log evidence so far */
+    int32_t resampled;
+    uint32_t bisections;        /* steps of the tempering search */
+    uint64_t accept[3];         /* accepted mu, lambda, weight moves (all sweeps) */
+} vsmcx_gmm_step;
+
+/* data: ndata observations (the prior's location and scale come from their range) */
+int vsmcx_gmm_create(vsmcx_ctx *ctx, const vsmcx_gmm_desc *desc, const double *data,
+                     uint64_t ndata, vsmcx_gmm **out);
+int vsmcx_gmm_destroy(vsmcx_gmm *g);
+/* Sampler::initialize (include/vsmc/core/sampler.hpp:450-462): prior draws,
+ * alpha = 0, equal weights; writes history row 0 */
+int vsmcx_gmm_initialize(vsmcx_gmm *g);
+/* up to max_iter iterations, stopping once alpha = 1; *done = iterations run,
+ * *alpha = temperature after them (either may be NULL) */
+int vsmcx_gmm_iterate(vsmcx_gmm *g, uint64_t max_iter, uint64_t *done, double *alpha);
+int vsmcx_gmm_rows(vsmcx_gmm *g, uint64_t *rows);
+/* device time of the iterations so far: tempering search with the reweighting,
+ * resampling, and the MH sweeps (HIP events around each phase) */
+int vsmcx_gmm_times(vsmcx_gmm *g, double *temper_ms, double *resample_ms, double *mh_ms);
+int vsmcx_gmm_history(vsmcx_gmm *g, uint64_t count, vsmcx_gmm_step *out);
+/* params: n x 3K row-major (K means, K precisions, K weights); llh: n; lw: n
+ * unnormalised log weights (zero while the weights are equal); any may be NULL */
+int vsmcx_gmm_state(vsmcx_gmm *g, double *params, double *llh, double *lw);
 
 #ifdef __cplusplus
 }

@@ -92,6 +92,18 @@ class ServeSummary(C.Structure):
 STEP_DTYPE = np.dtype([("ess", "f8"), ("log_z", "f8"), ("mean_x", "f8"), ("mean_y", "f8"),
                        ("resampled", "i4"), ("pad", "i4"), ("draws", "u8")])
 
+
+class GmmDesc(C.Structure):
+    _fields_ = [("n", C.c_uint64), ("components", C.c_uint32), ("sweeps", C.c_uint32),
+                ("precision", C.c_int), ("scheme", C.c_int), ("threshold", C.c_double),
+                ("cess_drop", C.c_double), ("seed", C.c_uint64), ("resample_seed", C.c_uint64),
+                ("max_steps", C.c_uint64)]
+
+
+GMM_DTYPE = np.dtype([("alpha", "f8"), ("alpha_inc", "f8"), ("ess", "f8"), ("log_z", "f8"),
+                      ("path_integrand", "f8"), ("path_log_z", "f8"), ("resampled", "i4"),
+                      ("bisections", "u4"), ("accept", "u8", (3,))])
+
 _lib = None
 
 
@@ -156,6 +168,14 @@ def lib():
         "vsmcx_serve_history": ([vp, u64, u64, vp], i32),
         "vsmcx_serve_forget": ([vp, u64], i32),
         "vsmcx_serve_stats": ([vp, P(ServeSummary)], i32),
+        "vsmcx_gmm_create": ([vp, P(GmmDesc), vp, u64, P(vp)], i32),
+        "vsmcx_gmm_destroy": ([vp], i32),
+        "vsmcx_gmm_initialize": ([vp], i32),
+        "vsmcx_gmm_iterate": ([vp, u64, P(u64), P(f64)], i32),
+        "vsmcx_gmm_rows": ([vp, P(u64)], i32),
+        "vsmcx_gmm_times": ([vp, P(f64), P(f64), P(f64)], i32),
+        "vsmcx_gmm_history": ([vp, u64, vp], i32),
+        "vsmcx_gmm_state": ([vp, vp, vp, vp], i32),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)
@@ -599,3 +619,75 @@ class Serve:
         s = ServeSummary()
         _check(lib().vsmcx_serve_stats(self.h, C.byref(s)))
         return {k: getattr(s, k) for k, _ in ServeSummary._fields_}
+
+
+class Gmm:
+    """Gaussian-mixture SMC sampler with adaptive (CESS) tempering (vsmcx_gmm):
+    the reference's gmm_smc under the "cess" schedule.  initialize() draws from
+    the prior; iterate() runs iterations until alpha reaches 1."""
+
+    def __init__(self, ctx, data, n, components=4, sweeps=5, precision=MIXED, scheme=STRATIFIED,
+                 threshold=0.5, cess_drop=0.1, seed=1, resample_seed=2, max_steps=2000):
+        self.ctx = ctx
+        self.K = components
+        self.n = n
+        self.data = np.ascontiguousarray(data, np.float64)
+        d = GmmDesc(n=n, components=components, sweeps=sweeps, precision=precision,
+                    scheme=scheme, threshold=threshold, cess_drop=cess_drop, seed=seed,
+                    resample_seed=resample_seed, max_steps=max_steps)
+        self.h = C.c_void_p()
+        _check(lib().vsmcx_gmm_create(ctx.h, C.byref(d), self.data.ctypes.data, self.data.size,
+                                      C.byref(self.h)))
+
+    def close(self):
+        if self.h:
+            lib().vsmcx_gmm_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def initialize(self):
+        _check(lib().vsmcx_gmm_initialize(self.h))
+
+    def iterate(self, max_iter=1 << 62):
+        done, alpha = C.c_uint64(), C.c_double()
+        _check(lib().vsmcx_gmm_iterate(self.h, max_iter, C.byref(done), C.byref(alpha)))
+        return done.value, alpha.value
+
+    def rows(self):
+        r = C.c_uint64()
+        _check(lib().vsmcx_gmm_rows(self.h, C.byref(r)))
+        return r.value
+
+    def times(self):
+        """Device ms so far: (tempering + reweight, resampling, MH sweeps)."""
+        a, b, c = C.c_double(), C.c_double(), C.c_double()
+        _check(lib().vsmcx_gmm_times(self.h, C.byref(a), C.byref(b), C.byref(c)))
+        return a.value, b.value, c.value
+
+    def history(self):
+        r = self.rows()
+        out = np.zeros(r, GMM_DTYPE)
+        _check(lib().vsmcx_gmm_history(self.h, r, out.ctypes.data))
+        return out
+
+    def state(self):
+        p = np.zeros((self.n, 3 * self.K))
+        llh = np.zeros(self.n)
+        lw = np.zeros(self.n)
+        _check(lib().vsmcx_gmm_state(self.h, p.ctypes.data, llh.ctypes.data, lw.ctypes.data))
+        return p, llh, lw
+
+
+def gmm_simulate(n, seed, means=(-3.0, 0.0, 2.0, 5.0), sd_range=(0.5, 1.5)):
+    """Data of the GMM workload: n points from an equal-weight mixture with the
+    given means and component sds drawn uniformly in sd_range."""
+    rng = np.random.default_rng(seed)
+    means = np.asarray(means, np.float64)
+    sds = rng.uniform(sd_range[0], sd_range[1], means.size)
+    comp = rng.integers(0, means.size, n)
+    return means[comp] + sds[comp] * rng.standard_normal(n)

@@ -17,8 +17,8 @@ FLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17", "-ffp-contract=o
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
          "-Wno-unused-command-line-argument"]
 SOURCES = ["kernels_rng.hip", "kernels_fold.hip", "kernels_cv.hip", "kernels_plain.hip",
-           "kernels_batch.hip", "kernels_tick.hip", "capi.cpp", "batch.cpp",
-           "serve.cpp"]
+           "kernels_batch.hip", "kernels_tick.hip", "kernels_gmm.hip", "capi.cpp",
+           "batch.cpp", "serve.cpp", "gmm.cpp"]
 
 
 def _compile(src):

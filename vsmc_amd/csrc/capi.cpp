@@ -19,7 +19,7 @@
 #include "rng_core.hpp"
 
 
-namespace {
+namespace vsmcx {
 
 // Fold / resample scratch for up to M particles (and nuni uniforms).
 void alloc_resample(vsmcx_ctx *c, ResampleBuffers &rb, uint64_t M, uint64_t nuni, bool lists)
@@ -58,6 +58,10 @@ void free_resample(vsmcx_ctx *c, ResampleBuffers &rb)
     for (void *p : ps) c->release(p);
     rb = ResampleBuffers();
 }
+
+}  // namespace vsmcx
+
+namespace {
 
 Tf64State engine_in(const vsmcx_threefry4x64_state &e)
 {

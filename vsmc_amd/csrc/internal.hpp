@@ -39,6 +39,7 @@ enum DevErr : int {
     kErrWindow = 4,          // inversion window inconsistency
     kErrTimeout = 8,         // a look-back wait exceeded its spin limit
     kErrBounds = 16,         // a computed rank fell outside its buffer
+    kErrSearch = 32,         // the tempering search did not end within its passes
 };
 
 // Per-filter control block, resident in device memory.  Kernels of one step

@@ -162,4 +162,49 @@ void launch_batch_run(bool fp64, int scheme, const BatchFilter *filters, const u
                       uint32_t nf, const BatchPool &P, const CvParams &p, const LgParams &lg,
                       uint64_t max_steps, unsigned *queue, unsigned grid, hipStream_t st);
 
+// ---- Gaussian-mixture SMC with adaptive tempering (kernels_gmm.hip) --------------------
+// Sampler state besides the weight set (FilterCtl), resident on the device.
+struct GmmCtl {
+    double alpha, alpha_inc;   // temperature and its last increment
+    double l, u, a, a_old;     // tempering search (move_alpha.hpp:242-279)
+    double target, eps;        // CESS target 1 - drop, search tolerance
+    double zconst;             // sum of log-evidence increments
+    double path, integrand;    // path-sampling sum and last integrand
+    double llc;                // log-likelihood constant -n log(2 pi) / 2
+    double sd[3];              // proposal scales of the mu, lambda, weight moves
+    uint64_t t;                // iteration
+    uint32_t done, iters;      // search finished; bisection steps taken
+};
+
+// One history row; the layout of vsmcx_gmm_step.
+struct GmmRow {
+    double alpha, alpha_inc, ess, log_z, path_integrand, path_log_z;
+    int32_t resampled;
+    uint32_t bisections;
+    uint64_t accept[3];
+};
+
+constexpr int kGmmMaxK = 6;
+
+// Constants of one sampler (host-built, passed by value).
+struct GmmModel {
+    double mu0, sd0, shape0, scale0;  // prior from the data range (gmm_state.hpp:145-160)
+    const double *xd;                 // data, FP64
+    const float *xf;                  // data, FP32 (mixed path)
+    uint32_t n;                       // data points
+    uint32_t sweeps;                  // MH sweeps per iteration
+    uint64_t seed;                    // Philox key
+    uint64_t N;                       // particles
+};
+
+void launch_gmm_init(bool fp64, int K, const GmmModel &g, FilterCtl *ctl, GmmCtl *gc, void *par,
+                     double *llh, double *part, GmmRow *hist, hipStream_t st);
+void launch_gmm_temper(FilterCtl *ctl, GmmCtl *gc, const double *llh, double *lw, double *W,
+                       uint64_t N, uint32_t passes, double *part, GmmRow *hist, int scheme,
+                       hipStream_t st);
+void launch_gmm_mh(bool fp64, int K, const GmmModel &g, const FilterCtl *ctl, GmmCtl *gc,
+                   const void *pin, void *pout, const double *llh_in, double *llh_out,
+                   const uint32_t *anc, const double *lw, double *ppart, uint32_t *apart,
+                   GmmRow *hist, hipStream_t st);
+
 }  // namespace vsmcx
