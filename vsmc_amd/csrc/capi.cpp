@@ -47,6 +47,9 @@ void alloc_resample(vsmcx_ctx *c, ResampleBuffers &rb, uint64_t M, uint64_t nuni
     fs.rep32 = c->alloc_n<uint32_t>(M);
     fs.zs = c->alloc_n<uint64_t>(tiles);
     fs.zsx = c->alloc_n<uint64_t>(tiles);
+    fs.wx = c->alloc_n<double>(M);
+    fs.thr = c->alloc_n<Seg>(tiles * kBlock);
+    fs.thr_fb = c->alloc_n<int32_t>(tiles * kBlock);
     fs.tiles = tiles;
 }
 
@@ -54,7 +57,8 @@ void free_resample(vsmcx_ctx *c, ResampleBuffers &rb)
 {
     void *ps[] = {rb.anc, rb.zl, rb.sl, rb.rep_out, rb.a_out, rb.u_tsum, rb.u_texcl, rb.uscan,
                   rb.u_tmax, rb.u_tmaxex, rb.fs.tsum, rb.fs.ts, rb.fs.agg, rb.fs.pre, rb.fs.spec,
-                  rb.fs.gval, rb.fs.rep32, rb.fs.zs, rb.fs.zsx};
+                  rb.fs.gval, rb.fs.rep32, rb.fs.zs, rb.fs.zsx, rb.fs.wx, rb.fs.thr,
+                  rb.fs.thr_fb};
     for (void *p : ps) c->release(p);
     rb = ResampleBuffers();
 }

@@ -59,10 +59,12 @@ __device__ __forceinline__ void cv_move_tile(
         }
     }
 
-    double m = -INFINITY, se = 0, se2 = 0, mx = 0, my = 0;
+    double m = -INFINITY, mx = 0, my = 0;
+    double lwn[kMoveItems];
 #pragma unroll
     for (int k = 0; k < kMoveItems; ++k) {
         const uint64_t i = base + (uint64_t)k * kBlock + threadIdx.x;
+        lwn[k] = -INFINITY;
         if (i >= N) continue;
         S4 &sk = s[k];
         if (t > 0) {
@@ -74,14 +76,23 @@ __device__ __forceinline__ void cv_move_tile(
         cv_normals(z, k0, k1, i, t);
         cv_advance(md, sk, z, t);
         const Real inc = md.llh(sk.x, sk.y, ox, oy);
-        const double lwn = lwo[k] + (double)inc;
+        lwn[k] = lwo[k] + (double)inc;
         st_out[i] = sk;
-        lw[i] = lwn;
-        lse_add(m, se, se2, lwn);
+        lw[i] = lwn[k];
+        m = lwn[k] > m ? lwn[k] : m;
     }
+    // the tile's maximum first, then one exp per particle relative to it
     const double M = block_reduce<kBlock>(m, MaxOp(), s_red);
-    const double sc = (m == -INFINITY) ? 0.0 : exp(m - M);
-    Part5 v{se * sc, se2 * sc * sc, mx, my};
+    double se = 0, se2 = 0;
+    if (M != -INFINITY) {
+#pragma unroll
+        for (int k = 0; k < kMoveItems; ++k) {
+            const double e = exp(lwn[k] - M);
+            se += e;
+            se2 += e * e;
+        }
+    }
+    Part5 v{se, se2, mx, my};
     v = block_reduce<kBlock>(v, Part5Add(), s_p5);
     if (threadIdx.x == 0) {
         part[0] = M;

@@ -25,6 +25,18 @@ struct XArray {  // given weights (compat entry, plain FP64 path)
     __device__ uint64_t ipart(uint64_t) const { return 0; }
 };
 
+// The values of another source, read back from the array the tile-sum pass
+// wrote (same values, one FP64 exp per element saved per later pass); the
+// integer parts still come from the source.
+template <typename XSrc>
+struct XCached {
+    const double *wx;
+    XSrc src;
+    __device__ void bind(const FilterCtl *c) { src.bind(c); }
+    __device__ double x(uint64_t i) const { return wx[i]; }
+    __device__ uint64_t ipart(uint64_t i) const { return src.ipart(i); }
+};
+
 struct XWeight {  // W_i = exp(lw_i - m) * coeff, computed from the log-weights
     const double *lw;
     double m, coeff;
@@ -311,13 +323,15 @@ __device__ T wg_scan(const T *in, T *out, uint64_t n, Op op, T ident, T *lds,
 // ---- tile sum of x (any fixed order) -----------------------------------------
 template <typename XSrc>
 __device__ __forceinline__ double fold_tile_sum(const XSrc &xs, uint64_t base, uint64_t M,
-                                                double *s_red)
+                                                double *s_red, double *wx_out = nullptr)
 {
     double s = 0;
 #pragma unroll
     for (int k = 0; k < kFI; ++k) {
         const uint64_t i = base + k * kBlock + threadIdx.x;
-        s += i < M ? xs.x(i) : 0.0;
+        const double v = i < M ? xs.x(i) : 0.0;
+        if (wx_out && i < M) wx_out[i] = v;
+        s += v;
     }
     return block_reduce<kBlock>(s, AddOp(), s_red);
 }
@@ -340,7 +354,8 @@ struct FoldLdsT {
 // to *agg_out (written by the last thread).
 template <typename XSrc, typename Lds>
 __device__ void fold_tile_seg(const XSrc &xs, uint64_t base, uint64_t M, double ts,
-                              SpecRec *spec_tile, Seg *agg_out, Lds &L)
+                              SpecRec *spec_tile, Seg *agg_out, Lds &L, Seg *thr_out = nullptr,
+                              int32_t *fb_out = nullptr)
 {
     double *const s_x = L.x;
     __syncthreads();  // the LDS of a previous tile is no longer read
@@ -360,6 +375,11 @@ __device__ void fold_tile_seg(const XSrc &xs, uint64_t base, uint64_t M, double 
     // tile total from the last thread (a struct total through a pointer would
     // keep the struct in scratch memory)
     if (tid == kBlock - 1) *agg_out = seg_op(thr_pre, g);
+    // the apply pass can start from these instead of classifying again
+    if (thr_out) {
+        thr_out[tid] = thr_pre;
+        fb_out[tid] = fast_b;
+    }
     if (g.cnt) {
         Seg pre = thr_pre;  // tile-local
         double s = s_thr;
@@ -452,7 +472,8 @@ __device__ double fold_chain(const Seg *pre, const uint32_t *cnt_lds, uint64_t T
 template <typename XSrc, typename Inv, typename Lds>
 __device__ void fold_tile_apply(FilterCtl *ctl, const XSrc &xs, const Inv &inv, uint64_t t,
                                 uint64_t M, double ts, const Seg tpre, const double *gval,
-                                const FoldOut &out, uint32_t *rep32, uint64_t *zs_out, Lds &L)
+                                const FoldOut &out, uint32_t *rep32, uint64_t *zs_out, Lds &L,
+                                const Seg *thr_in = nullptr, const int32_t *fb_in = nullptr)
 {
     // x, overwritten in place (each thread its own elements) by the exact
     // values; later the uniform window and the staged replication counts
@@ -478,9 +499,14 @@ __device__ void fold_tile_apply(FilterCtl *ctl, const XSrc &xs, const Inv &inv, 
     fold_eps(M, &er, &ea);
     const auto xat = [&](int k) { return s_x[fidx(e0 + k)]; };
     int fast_b;
-    const Seg g = run_seg(xat, kFI, s_thr, er, ea, &fast_b);
-    const Seg thr_pre =
-        block_excl_scan<kBlock>(g, SegOp(), seg_identity(), s_seg, (Seg *)nullptr);
+    Seg thr_pre;
+    if (thr_in) {  // the segment pass's own classification of this thread's run
+        thr_pre = thr_in[tid];
+        fast_b = fb_in[tid];
+    } else {
+        const Seg g = run_seg(xat, kFI, s_thr, er, ea, &fast_b);
+        thr_pre = block_excl_scan<kBlock>(g, SegOp(), seg_identity(), s_seg, (Seg *)nullptr);
+    }
     int bad = !run_values(
         xat, kFI, s_thr, er, ea, fast_b, seg_op(tpre, thr_pre),
         [=](uint64_t c) { return gval[c]; }, [&](int k, double a) { s_a[fidx(e0 + k)] = a; });

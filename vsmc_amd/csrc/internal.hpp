@@ -94,6 +94,9 @@ struct FoldScratch {
     uint32_t *rep32;   // [M] replication counts for the layout pass
     uint64_t *zs;      // [tiles] zero slots << 32 | surplus copies
     uint64_t *zsx;     // [tiles] exclusive prefix of zs
+    double *wx;        // [M] the values x_i, written by the tile-sum pass (optional)
+    Seg *thr;          // [tiles * 256] each thread's exclusive segment prefix in its tile (optional)
+    int32_t *thr_fb;   // [tiles * 256] each thread's fast-path binade (with thr)
     size_t tiles;
 };
 

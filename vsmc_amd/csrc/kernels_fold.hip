@@ -23,7 +23,7 @@ __global__ __launch_bounds__(kBlock) void k_fold_tsum(FilterCtl *ctl, XSrc xs, u
     if (!ctl->resample) return;
     __shared__ double s_red[kBlock / kWave];
     xs.bind(ctl);
-    const double s = fold_tile_sum(xs, blockIdx.x * (uint64_t)kFT, M, s_red);
+    const double s = fold_tile_sum(xs, blockIdx.x * (uint64_t)kFT, M, s_red, fs.wx);
     if (threadIdx.x == 0) fs.tsum[blockIdx.x] = s;
 }
 
@@ -44,7 +44,8 @@ __global__ __launch_bounds__(kBlock) void k_fold_seg(FilterCtl *ctl, XSrc xs, ui
     __shared__ FoldLdsT<false> L;
     xs.bind(ctl);
     const uint64_t t = blockIdx.x;
-    fold_tile_seg(xs, t * (uint64_t)kFT, M, fs.ts[t], fs.spec + t * (uint64_t)kFT, fs.agg + t, L);
+    fold_tile_seg(xs, t * (uint64_t)kFT, M, fs.ts[t], fs.spec + t * (uint64_t)kFT, fs.agg + t, L,
+                  fs.thr ? fs.thr + t * kBlock : nullptr, fs.thr ? fs.thr_fb + t * kBlock : nullptr);
 }
 
 // ---- pass 4: tile segment prefixes and the exact values at the specials --------
@@ -86,7 +87,9 @@ __global__ __launch_bounds__(kBlock) void k_fold_apply(FilterCtl *ctl, XSrc xs, 
     xs.bind(ctl);
     inv.bind(ctl);
     const uint64_t t = blockIdx.x;
-    fold_tile_apply(ctl, xs, inv, t, M, fs.ts[t], fs.pre[t], fs.gval, out, fs.rep32, fs.zs + t, L);
+    fold_tile_apply(ctl, xs, inv, t, M, fs.ts[t], fs.pre[t], fs.gval, out, fs.rep32, fs.zs + t, L,
+                    fs.thr ? fs.thr + t * kBlock : nullptr,
+                    fs.thr ? fs.thr_fb + t * kBlock : nullptr);
 }
 
 // ---- pass 6: zero / surplus ranks of the tiles (one workgroup) ---------------------
@@ -282,11 +285,21 @@ static void launch_fold(FilterCtl *ctl, XSrc xs, Inv inv, uint64_t M, FoldScratc
     const dim3 grid((unsigned)fs.tiles);
     hipLaunchKernelGGL((k_fold_tsum<XSrc>), grid, dim3(kBlock), 0, st, ctl, xs, M, fs);
     hipLaunchKernelGGL(k_fold_scan_sum, dim3(1), dim3(kScanBlock), 0, st, ctl, fs);
-    hipLaunchKernelGGL((k_fold_seg<XSrc>), grid, dim3(kBlock), 0, st, ctl, xs, M, fs);
-    hipLaunchKernelGGL(k_fold_scan_seg, dim3(1), dim3(kScanBlock), 0, st, ctl, fs);
-    if (!Inv::kInv && !out.a_out) return;
-    hipLaunchKernelGGL((k_fold_apply<XSrc, Inv>), grid, dim3(kBlock), 0, st, ctl, xs, inv, M, fs,
-                       out);
+    if (fs.wx) {
+        // later passes read the values the tile-sum pass wrote
+        const XCached<XSrc> xc{fs.wx, xs};
+        hipLaunchKernelGGL((k_fold_seg<XCached<XSrc>>), grid, dim3(kBlock), 0, st, ctl, xc, M, fs);
+        hipLaunchKernelGGL(k_fold_scan_seg, dim3(1), dim3(kScanBlock), 0, st, ctl, fs);
+        if (!Inv::kInv && !out.a_out) return;
+        hipLaunchKernelGGL((k_fold_apply<XCached<XSrc>, Inv>), grid, dim3(kBlock), 0, st, ctl, xc,
+                           inv, M, fs, out);
+    } else {
+        hipLaunchKernelGGL((k_fold_seg<XSrc>), grid, dim3(kBlock), 0, st, ctl, xs, M, fs);
+        hipLaunchKernelGGL(k_fold_scan_seg, dim3(1), dim3(kScanBlock), 0, st, ctl, fs);
+        if (!Inv::kInv && !out.a_out) return;
+        hipLaunchKernelGGL((k_fold_apply<XSrc, Inv>), grid, dim3(kBlock), 0, st, ctl, xs, inv, M,
+                           fs, out);
+    }
     if (!Inv::kInv || !out.anc) return;
     hipLaunchKernelGGL(k_fold_scan_cnt, dim3(1), dim3(kScanBlock), 0, st, ctl, fs);
     hipLaunchKernelGGL(k_fold_layout, grid, dim3(kBlock), 0, st, ctl, fs, out, M);

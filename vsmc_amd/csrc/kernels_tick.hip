@@ -32,7 +32,10 @@ __device__ __forceinline__ void lg_move_tile(const FilterCtl *ctl, const Real *_
     const double invN = 1.0 / (double)N;
     const Real q = (Real)lp.q, s0 = (Real)lp.s0, inv_r2 = (Real)(1 / (lp.r * lp.r));
     const uint64_t base = tile * (uint64_t)kMoveTile;
-    double m = -INFINITY, se = 0, se2 = 0, mx = 0, my = 0;
+    double m = -INFINITY, mx = 0, my = 0;
+    double lwv[kMoveItems];
+#pragma unroll
+    for (int k = 0; k < kMoveItems; ++k) lwv[k] = -INFINITY;
 #pragma unroll 1
     for (int k = 0; k < kMoveItems; ++k) {
         const uint64_t i = base + (uint64_t)k * kBlock + threadIdx.x;
@@ -74,11 +77,21 @@ __device__ __forceinline__ void lg_move_tile(const FilterCtl *ctl, const Real *_
         for (int d = 0; d < kLgMaxDim; ++d)
             if (d < D) out[i * D + d] = xn[d];
         lw[i] = lwn;
-        lse_add(m, se, se2, lwn);
+        lwv[k] = lwn;
+        m = lwn > m ? lwn : m;
     }
+    // the tile's maximum first, then one exp per particle relative to it
     const double M = block_reduce<kBlock>(m, MaxOp(), s_red);
-    const double sc = (m == -INFINITY) ? 0.0 : exp(m - M);
-    Part5 v{se * sc, se2 * sc * sc, mx, my};
+    double se = 0, se2 = 0;
+    if (M != -INFINITY) {
+#pragma unroll
+        for (int k = 0; k < kMoveItems; ++k) {
+            const double e = exp(lwv[k] - M);
+            se += e;
+            se2 += e * e;
+        }
+    }
+    Part5 v{se, se2, mx, my};
     v = block_reduce<kBlock>(v, Part5Add(), s_p5);
     if (threadIdx.x == 0) {
         part[0] = M;

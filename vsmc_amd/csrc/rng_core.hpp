@@ -39,14 +39,15 @@ VSMCX_HD u32x4 philox4x32_10(u32x4 c, uint32_t k0, uint32_t k1)
             k0 += 0x9E3779B9u;
             k1 += 0xBB67AE85u;
         }
-        const uint32_t hi0 = mulhi32(0xCD9E8D57u, s2), lo1 = 0xCD9E8D57u * s2;
-        const uint32_t hi2 = mulhi32(0xD2511F53u, s0), lo3 = 0xD2511F53u * s0;
-        const uint32_t n0 = hi0 ^ k0 ^ s1;
-        const uint32_t n2 = hi2 ^ k1 ^ s3;
+        // one 32 x 32 -> 64 product gives both halves (v_mad_u64_u32 on gfx950)
+        const uint64_t p0 = (uint64_t)0xCD9E8D57u * s2;
+        const uint64_t p1 = (uint64_t)0xD2511F53u * s0;
+        const uint32_t n0 = (uint32_t)(p0 >> 32) ^ k0 ^ s1;
+        const uint32_t n2 = (uint32_t)(p1 >> 32) ^ k1 ^ s3;
         s0 = n0;
-        s1 = lo1;
+        s1 = (uint32_t)p0;
         s2 = n2;
-        s3 = lo3;
+        s3 = (uint32_t)p1;
     }
     u32x4 o;
     o.v[0] = s0; o.v[1] = s1; o.v[2] = s2; o.v[3] = s3;
