@@ -177,11 +177,11 @@ int vsmcx_filter_debug_step(vsmcx_filter *f, double *weight, uint64_t *replicati
 
 /* ---- batched filters ----------------------------------------------------------
  * Thousands of independent pf filters (each the vsmcx_filter model with its own
- * particle count, horizon, observations and seeds) advanced together: one
- * persistent kernel launch per vsmcx_batch_run, workgroups taking filters
- * from a queue and running whole steps of one filter at a time.  A filter's
- * results do not depend on the other filters in the batch or on the launch
- * shape.  No single counterpart in vSMC: it replaces one Sampler per filter
+ * particle count, horizon, observations and seeds) advanced together by
+ * vsmcx_batch_run, as one persistent launch or as one tick per step (see the
+ * strategies below).  A filter's results do not depend on the other filters
+ * in the batch, on the strategy or on the launch shape: they are the bits
+ * vsmcx_filter computes for the same filter.  No single counterpart in vSMC: it replaces one Sampler per filter
  * (include/vsmc/core/sampler.hpp:450-483, 773-834), each stepped in turn. */
 typedef struct vsmcx_batch vsmcx_batch;
 
@@ -210,8 +210,18 @@ int vsmcx_batch_run(vsmcx_batch *b, uint64_t max_steps, double *elapsed_ms);
 /* Launch shape: workgroups of the persistent grid (0 = default) and the order
  * filters are taken in (0 = most work first, 1 = as given). */
 int vsmcx_batch_set_launch(vsmcx_batch *b, uint32_t workgroups, int order);
-/* Autotuner: runs `steps` real steps under each candidate launch shape and
- * keeps the fastest (the results do not depend on the shape). */
+/* Execution strategy.  PERSISTENT: one launch, workgroups take filters from a
+ * queue and run whole steps of one filter each.  TICKS: one step of every
+ * filter with a step left per tick, each phase one grid launch over all their
+ * tiles (the serving loop's tick).  Both run the same tile bodies in the same
+ * orders, so the results are the same bits either way. */
+#define VSMCX_BATCH_PERSISTENT 0
+#define VSMCX_BATCH_TICKS 1
+int vsmcx_batch_set_strategy(vsmcx_batch *b, int strategy);
+int vsmcx_batch_get_strategy(vsmcx_batch *b, int *strategy);
+/* Autotuner: runs `steps` real steps under each candidate (ticks, and
+ * persistent grids of several sizes and both filter orders) and keeps the
+ * fastest (the results do not depend on the choice). */
 int vsmcx_batch_tune(vsmcx_batch *b, uint64_t steps, uint32_t *workgroups, int *order);
 int vsmcx_batch_steps(vsmcx_batch *b, uint64_t f, uint64_t *t);
 int vsmcx_batch_history(vsmcx_batch *b, uint64_t f, uint64_t count, vsmcx_step_stats *out);

@@ -81,10 +81,11 @@ def test_batch_independent_of_launch_and_company(ctx, precision):
     seeds = 3000 + np.arange(k, dtype=np.uint64)
     rseeds = 9000 + np.arange(k, dtype=np.uint64)
 
-    def run(idx, workgroups, order, chunks):
+    def run(idx, workgroups, order, chunks, strategy=V.BATCH_PERSISTENT):
         b = V.Batch(ctx, ns[idx], Ts[idx], seeds[idx], rseeds[idx], precision=precision)
         for j, f in enumerate(idx):
             b.set_obs(j, obs[f])
+        b.set_strategy(strategy)
         b.set_launch(workgroups, order)
         for c in chunks:
             b.run(c)
@@ -99,13 +100,45 @@ def test_batch_independent_of_launch_and_company(ctx, precision):
     bb = run(np.arange(k)[::-1].copy(), 7, 1, [3, 5])
     say("batch alone")
     alone = run(np.array([17]), 1, 0, [1])
+    say("batch crowd C (ticks, 4+2 steps first)")
+    cc = run(np.arange(k), 0, 0, [4, 2], strategy=V.BATCH_TICKS)
     for f in range(k):
         ha, (sa, la) = a[f]
-        hb, (sb, lb) = bb[f]
-        assert (ha == hb).all(), f
-        assert (sa == sb).all() and (la == lb).all(), f
+        for other in (bb, cc):
+            hb, (sb, lb) = other[f]
+            assert (ha == hb).all(), f
+            assert (sa == sb).all() and (la == lb).all(), f
     h1, (s1, l1) = alone[17]
     assert (h1 == a[17][0]).all() and (s1 == a[17][1][0]).all() and (l1 == a[17][1][1]).all()
+
+
+@pytest.mark.parametrize("precision", [V.MIXED, V.FP64_FUSED])
+@pytest.mark.parametrize("strategy", [V.BATCH_PERSISTENT, V.BATCH_TICKS])
+def test_batch_equals_single_filter(ctx, precision, strategy):
+    """Every engine runs the same tile bodies and the same fixed-order
+    reductions: a batched filter's history and final state are vsmcx_filter's
+    for the same filter, bit for bit, under either strategy."""
+    ns, Ts = [1500, 5000, 9000], [20, 15, 9]
+    seeds, rseeds = [31, 32, 33], [41, 42, 43]
+    obs = [_obs(T, 60 + i) for i, T in enumerate(Ts)]
+    b = V.Batch(ctx, ns, Ts, seeds, rseeds, precision=precision, scheme=V.STRATIFIED)
+    for f in range(3):
+        b.set_obs(f, obs[f])
+    b.set_strategy(strategy)
+    b.run()
+    for f in range(3):
+        fl = V.Filter(ctx, V.cv_desc(ns[f], Ts[f], precision=precision, scheme=V.STRATIFIED,
+                                     seed=seeds[f], resample_seed=rseeds[f]))
+        fl.set_obs(obs[f])
+        fl.run(Ts[f])
+        fl.finish()
+        hf = fl.history()
+        st, lw = fl.state()
+        fl.close()
+        assert (b.history(f) == hf).all(), f
+        bs, blw = b.state(f)
+        assert (bs == st).all() and (blw == lw).all(), f
+    b.close()
 
 
 def test_batch_mixed_first_step_coupled(ctx, oracle):
@@ -160,8 +193,8 @@ def test_batch_tuner_keeps_results(ctx):
         o = _obs(6, 70 + f)
         b.set_obs(f, o)
         ref.set_obs(f, o)
-    wg, order = b.tune(1)  # candidate shapes, one real step each
-    assert wg >= 1 and order in (0, 1)
+    wg, order = b.tune(1)  # candidate strategies and shapes, one real step each
+    assert wg >= 1 and order in (0, 1) and b.strategy() in (V.BATCH_PERSISTENT, V.BATCH_TICKS)
     b.run()
     ref.run()
     for f in range(16):

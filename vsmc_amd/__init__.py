@@ -16,6 +16,7 @@ MULTINOMIAL, RESIDUAL, STRATIFIED, SYSTEMATIC, RESIDUAL_STRATIFIED, RESIDUAL_SYS
 SCHEMES = {"multinomial": 0, "residual": 1, "stratified": 2, "systematic": 3,
            "residual_stratified": 4, "residual_systematic": 5}
 MIXED, FP64_FUSED, FP64_PLAIN = 0, 1, 2
+BATCH_PERSISTENT, BATCH_TICKS = 0, 1
 
 ERRORS = {-1: "VSMCX_ERR_ARG", -2: "VSMCX_ERR_HIP", -3: "VSMCX_ERR_DEGENERATE",
           -4: "VSMCX_ERR_BUDGET", -5: "VSMCX_ERR_INTERNAL", -6: "VSMCX_ERR_STATE"}
@@ -152,6 +153,8 @@ def lib():
         "vsmcx_batch_set_obs": ([vp, u64, u64, u64, vp], i32),
         "vsmcx_batch_run": ([vp, u64, P(f64)], i32),
         "vsmcx_batch_set_launch": ([vp, C.c_uint32, i32], i32),
+        "vsmcx_batch_set_strategy": ([vp, i32], i32),
+        "vsmcx_batch_get_strategy": ([vp, P(i32)], i32),
         "vsmcx_batch_tune": ([vp, u64, P(C.c_uint32), P(i32)], i32),
         "vsmcx_batch_steps": ([vp, u64, P(u64)], i32),
         "vsmcx_batch_history": ([vp, u64, u64, vp], i32),
@@ -483,6 +486,16 @@ class Batch:
         wg, order = C.c_uint32(), C.c_int()
         _check(lib().vsmcx_batch_tune(self.h, steps, C.byref(wg), C.byref(order)))
         return wg.value, order.value
+
+    def set_strategy(self, strategy):
+        """BATCH_PERSISTENT (one launch, workgroups take filters from a queue) or
+        BATCH_TICKS (one step of every active filter per tick, grid launches)."""
+        _check(lib().vsmcx_batch_set_strategy(self.h, strategy))
+
+    def strategy(self):
+        v = C.c_int()
+        _check(lib().vsmcx_batch_get_strategy(self.h, C.byref(v)))
+        return v.value
 
     def steps(self, f):
         v = C.c_uint64()

@@ -7,6 +7,7 @@
 #include <cstring>
 #include <memory>
 #include <numeric>
+#include <tuple>
 #include <vector>
 
 #include "../../include/vsmcx.h"
@@ -30,6 +31,12 @@ struct vsmcx_batch {
     unsigned *queue = nullptr;
     unsigned workgroups = 0;
     int order = 0;  // 0: most work first, 1: as given
+    int strategy = VSMCX_BATCH_TICKS;
+    std::vector<uint64_t> next_h;  // host mirror of each filter's next step
+    uint32_t *jobs_dev = nullptr;  // ticks: filters by steps remaining, descending
+    TickItem *items_dev = nullptr; // ticks: their tiles, in that order
+    std::vector<uint32_t> jobs_h;
+    std::vector<TickItem> items_h;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
 
     size_t state_bytes() const { return fp64 ? 32 : 16; }
@@ -37,9 +44,9 @@ struct vsmcx_batch {
     void release_all()
     {
         void *ps[] = {df, order_work, order_given, P.st0, P.st1, P.lw, P.anc, P.zl, P.sl,
-                      P.rep32, P.spec, P.gval, P.ts, P.tagg, P.tpre, P.tzs, P.ctl, P.hist,
-                      (void *)P.obs, P.next_t,
-                      (void *)P.avail, P.cur, P.err_any, queue};
+                      P.rep32, P.spec, P.gval, P.ts, P.tagg, P.tpre, P.tzs, P.tsum, P.tzsx,
+                      P.tpart, P.ctl, P.hist, (void *)P.obs, P.next_t,
+                      (void *)P.avail, P.cur, P.err_any, queue, jobs_dev, items_dev};
         for (void *q : ps) ctx->release(q);
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
@@ -56,10 +63,59 @@ struct vsmcx_batch {
         return std::min<unsigned>(nf, (unsigned)cus * per_cu);
     }
 
-    void launch(uint64_t max_steps, unsigned wg, int ord)
+    // steps filter f takes in a run of at most max_steps
+    uint64_t steps_of(uint32_t f, uint64_t max_steps) const
     {
-        launch_batch_run(fp64, scheme, df, ord == 0 ? order_work : order_given, nf, P, p, lg,
-                         max_steps, queue, wg, ctx->stream);
+        uint64_t t1 = std::min(avail_h[f], hf[f].horizon);
+        if (t1 <= next_h[f]) return 0;
+        return std::min(t1 - next_h[f], max_steps);
+    }
+
+    void launch(uint64_t max_steps, unsigned wg, int ord, int strat)
+    {
+        if (strat == VSMCX_BATCH_PERSISTENT) {
+            launch_batch_run(fp64, scheme, df, ord == 0 ? order_work : order_given, nf, P, p, lg,
+                             max_steps, queue, wg, ctx->stream);
+            VSMCX_HIP(hipGetLastError());
+        } else {
+            launch_ticks(max_steps);
+        }
+        for (uint32_t f = 0; f < nf; ++f) next_h[f] += steps_of(f, max_steps);
+    }
+
+    // One tick per step: every filter with a step left in this run, each phase
+    // one grid launch over all their tiles.  Sorted by steps left, the active
+    // filters of tick k are a prefix of one list, uploaded once.
+    void launch_ticks(uint64_t max_steps)
+    {
+        std::vector<std::pair<uint64_t, uint32_t>> act;
+        for (uint32_t f = 0; f < nf; ++f) {
+            const uint64_t sf = steps_of(f, max_steps);
+            if (sf) act.push_back({sf, f});
+        }
+        if (act.empty()) return;
+        std::stable_sort(act.begin(), act.end(),
+                         [](const std::pair<uint64_t, uint32_t> &a,
+                            const std::pair<uint64_t, uint32_t> &c) { return a.first > c.first; });
+        jobs_h.clear();
+        items_h.clear();
+        std::vector<uint64_t> item_end;  // items of the first j jobs
+        for (const auto &a : act) {
+            jobs_h.push_back(a.second);
+            const uint64_t T = (hf[a.second].n + kFoldTile - 1) / kFoldTile;
+            for (uint64_t tt = 0; tt < T; ++tt) items_h.push_back(TickItem{a.second, (uint32_t)tt});
+            item_end.push_back(items_h.size());
+        }
+        hipStream_t s = ctx->stream;
+        VSMCX_HIP(hipMemcpyAsync(jobs_dev, jobs_h.data(), 4 * jobs_h.size(), hipMemcpyHostToDevice, s));
+        VSMCX_HIP(hipMemcpyAsync(items_dev, items_h.data(), sizeof(TickItem) * items_h.size(),
+                                 hipMemcpyHostToDevice, s));
+        size_t nj = act.size();
+        for (uint64_t k = 0; k < act.front().first; ++k) {
+            while (nj > 0 && act[nj - 1].first <= k) --nj;  // filters done before tick k
+            launch_tick(fp64, scheme, items_dev, (uint32_t)item_end[nj - 1], jobs_dev, (uint32_t)nj,
+                        df, P, p, lg, s);
+        }
         VSMCX_HIP(hipGetLastError());
     }
 
@@ -113,6 +169,7 @@ int vsmcx_batch_create(vsmcx_ctx *ctx, const vsmcx_batch_desc *d, vsmcx_batch **
     b->total = off;
     b->total_obs = obs;
     b->avail_h.assign(b->nf, 0);
+    b->next_h.assign(b->nf, 0);
     try {
         BatchPool &P = b->P;
         P.st0 = ctx->alloc(off * b->state_bytes());
@@ -128,6 +185,11 @@ int vsmcx_batch_create(vsmcx_ctx *ctx, const vsmcx_batch_desc *d, vsmcx_batch **
         P.tagg = ctx->alloc_n<Seg>(off / kFoldTile);
         P.tpre = ctx->alloc_n<Seg>(off / kFoldTile);
         P.tzs = ctx->alloc_n<uint64_t>(off / kFoldTile);
+        P.tsum = ctx->alloc_n<double>(off / kFoldTile);
+        P.tzsx = ctx->alloc_n<uint64_t>(off / kFoldTile);
+        P.tpart = ctx->alloc_n<double>(5 * (off / kFoldTile));
+        b->jobs_dev = ctx->alloc_n<uint32_t>(b->nf);
+        b->items_dev = ctx->alloc_n<TickItem>(off / kFoldTile);
         P.ctl = ctx->alloc_n<FilterCtl>(b->nf);
         P.hist = ctx->alloc_zero<StepRow>(obs);
         P.obs = ctx->alloc_zero<double>(2 * obs);
@@ -219,7 +281,7 @@ int vsmcx_batch_run(vsmcx_batch *b, uint64_t max_steps, double *elapsed_ms)
     REQUIRE(b, "NULL batch");
     hipStream_t s = b->ctx->stream;
     VSMCX_HIP(hipEventRecord(b->ev0, s));
-    b->launch(max_steps, b->workgroups, b->order);
+    b->launch(max_steps, b->workgroups, b->order, b->strategy);
     VSMCX_HIP(hipEventRecord(b->ev1, s));
     b->check_errors();
     if (elapsed_ms) {
@@ -235,19 +297,22 @@ int vsmcx_batch_tune(vsmcx_batch *b, uint64_t steps, uint32_t *workgroups, int *
     API_BEGIN
     REQUIRE(b && steps >= 1, "NULL batch or zero steps");
     const unsigned full = b->default_workgroups();
-    std::vector<std::pair<unsigned, int>> cand;
+    // candidates: ticks, and persistent grids of 1/4 .. 4/4 of the resident
+    // workgroups in either filter order
+    std::vector<std::tuple<int, unsigned, int>> cand;
+    cand.push_back({VSMCX_BATCH_TICKS, b->workgroups, b->order});
     for (unsigned k = 1; k <= 4; ++k) {
         const unsigned wg = std::max(1u, std::min<unsigned>(b->nf, full * k / 4));
-        for (int ord = 0; ord < 2; ++ord) cand.push_back({wg, ord});
+        for (int ord = 0; ord < 2; ++ord) cand.push_back({VSMCX_BATCH_PERSISTENT, wg, ord});
     }
     std::sort(cand.begin(), cand.end());
     cand.erase(std::unique(cand.begin(), cand.end()), cand.end());
     hipStream_t s = b->ctx->stream;
     double best = 1e300;
-    std::pair<unsigned, int> pick{b->workgroups, b->order};
+    std::tuple<int, unsigned, int> pick{b->strategy, b->workgroups, b->order};
     for (const auto &c : cand) {
         VSMCX_HIP(hipEventRecord(b->ev0, s));
-        b->launch(steps, c.first, c.second);
+        b->launch(steps, std::get<1>(c), std::get<2>(c), std::get<0>(c));
         VSMCX_HIP(hipEventRecord(b->ev1, s));
         b->check_errors();
         float ms = 0;
@@ -257,10 +322,29 @@ int vsmcx_batch_tune(vsmcx_batch *b, uint64_t steps, uint32_t *workgroups, int *
             pick = c;
         }
     }
-    b->workgroups = pick.first;
-    b->order = pick.second;
-    if (workgroups) *workgroups = pick.first;
-    if (order) *order = pick.second;
+    b->strategy = std::get<0>(pick);
+    b->workgroups = std::get<1>(pick);
+    b->order = std::get<2>(pick);
+    if (workgroups) *workgroups = b->workgroups;
+    if (order) *order = b->order;
+    API_END
+}
+
+int vsmcx_batch_set_strategy(vsmcx_batch *b, int strategy)
+{
+    API_BEGIN
+    REQUIRE(b, "NULL batch");
+    REQUIRE(strategy == VSMCX_BATCH_PERSISTENT || strategy == VSMCX_BATCH_TICKS,
+            "strategy is VSMCX_BATCH_PERSISTENT or VSMCX_BATCH_TICKS");
+    b->strategy = strategy;
+    API_END
+}
+
+int vsmcx_batch_get_strategy(vsmcx_batch *b, int *strategy)
+{
+    API_BEGIN
+    REQUIRE(b && strategy, "NULL argument");
+    *strategy = b->strategy;
     API_END
 }
 

@@ -103,29 +103,58 @@ __device__ __forceinline__ void cv_move_tile(
     }
 }
 
-// One workgroup of 1024 threads: combine the tile partials in a fixed order,
-// then ESS, log-evidence and the resample decision.
-__device__ __forceinline__ void cv_reduce_tiles(FilterCtl *ctl, const double *partials,
-                                                uint64_t ntiles, StepRow *hist, uint64_t t)
+// Tile partials {max, sum e, sum e^2, monitor x, monitor y} of ntiles tiles,
+// combined in an order that the tile count alone fixes: lane l of the
+// workgroup's first wave takes tiles l, l + 64, l + 128, ... in turn, and the
+// 64 lane results meet in a fixed butterfly (every lane ends with the same
+// totals).  A 1024-thread reduce launch and a 256-thread batch workgroup
+// therefore compute the same bits.  Call from all 64 lanes of wave 0.
+__device__ __forceinline__ void tile_partials_lse(const double *partials, uint64_t ntiles,
+                                                  double &M, Part5 &acc)
 {
-    __shared__ double s_red[1024 / kWave];
-    __shared__ Part5 s_p5[1024 / kWave];
+    const int lane = threadIdx.x & (kWave - 1);
     double m = -INFINITY;
-    for (uint64_t j = threadIdx.x; j < ntiles; j += 1024) {
+    for (uint64_t j = lane; j < ntiles; j += kWave) {
         const double v = partials[5 * j];
         m = m < v ? v : m;
     }
-    const double M = block_reduce<1024>(m, MaxOp(), s_red);
-    Part5 acc{0, 0, 0, 0};
-    for (uint64_t j = threadIdx.x; j < ntiles; j += 1024) {
+    M = wave_reduce(m, MaxOp());
+    Part5 a{0, 0, 0, 0};
+    for (uint64_t j = lane; j < ntiles; j += kWave) {
         const double *q = partials + 5 * j;
         const double sc = q[0] == -INFINITY ? 0.0 : exp(q[0] - M);
-        acc.se += q[1] * sc;
-        acc.se2 += q[2] * sc * sc;
-        acc.mx += q[3];
-        acc.my += q[4];
+        a.se += q[1] * sc;
+        a.se2 += q[2] * sc * sc;
+        a.mx += q[3];
+        a.my += q[4];
     }
-    acc = block_reduce<1024>(acc, Part5Add(), s_p5);
+    acc = wave_reduce(a, Part5Add());
+}
+
+// The monitor fields alone, in the same order.
+__device__ __forceinline__ void tile_partials_monitor(const double *partials, uint64_t ntiles,
+                                                      double &mx, double &my)
+{
+    const int lane = threadIdx.x & (kWave - 1);
+    Part5 a{0, 0, 0, 0};
+    for (uint64_t j = lane; j < ntiles; j += kWave) {
+        a.mx += partials[5 * j + 3];
+        a.my += partials[5 * j + 4];
+    }
+    a = wave_reduce(a, Part5Add());
+    mx = a.mx;
+    my = a.my;
+}
+
+// After a step's move: ESS, log-evidence and the resample decision from the
+// tile partials (wave 0 of the calling workgroup; thread 0 writes).
+__device__ __forceinline__ void cv_reduce_tiles(FilterCtl *ctl, const double *partials,
+                                                uint64_t ntiles, StepRow *hist, uint64_t t)
+{
+    if (threadIdx.x >= kWave) return;
+    double M;
+    Part5 acc;
+    tile_partials_lse(partials, ntiles, M, acc);
     if (threadIdx.x == 0) cv_step_scalars(ctl, M, acc.se, acc.se2, acc.mx, acc.my, hist, t);
 }
 
@@ -159,16 +188,12 @@ __device__ __forceinline__ void cv_monitor_tile(const FilterCtl *ctl,
 __device__ __forceinline__ void cv_monitor_reduce(const FilterCtl *ctl, const double *partials,
                                                   uint64_t ntiles, StepRow *hist)
 {
-    __shared__ Part5 s_p5[1024 / kWave];
-    Part5 acc{0, 0, 0, 0};
-    for (uint64_t j = threadIdx.x; j < ntiles; j += 1024) {
-        acc.mx += partials[5 * j + 3];
-        acc.my += partials[5 * j + 4];
-    }
-    acc = block_reduce<1024>(acc, Part5Add(), s_p5);
+    if (threadIdx.x >= kWave) return;
+    double mx, my;
+    tile_partials_monitor(partials, ntiles, mx, my);
     if (threadIdx.x == 0 && hist) {
-        hist[ctl->t].mean_x = acc.mx;
-        hist[ctl->t].mean_y = acc.my;
+        hist[ctl->t].mean_x = mx;
+        hist[ctl->t].mean_y = my;
     }
 }
 

@@ -20,6 +20,8 @@ constexpr int kMoveItems = 8;                // particles per thread in the move
 constexpr int kMoveTile = kBlock * kMoveItems;  // particles per move workgroup (one partial)
 constexpr int kFoldItems = 8;                // elements per thread in the fold kernels
 constexpr int kFoldTile = kBlock * kFoldItems;  // elements per fold tile
+// the move's tile partials and the fold's per-tile arrays share one tiling
+static_assert(kMoveTile == kFoldTile, "move and fold tiles coincide");
 
 // Resampling schemes, same order as vsmc::ResampleScheme
 // (include/vsmc/resample/common.hpp:124-131).

@@ -9,10 +9,10 @@
 // tile-level scans in LDS.  Every reduction has a fixed order that depends on
 // the filter alone, so a filter's results are the same whichever workgroup
 // runs it, in whatever company, under whatever launch shape.
-#include "cv_model.hpp"
+#include "cv_tiles.hpp"
 #include "fold_tiles.hpp"
-#include "lg_model.hpp"
 #include "launch.hpp"
+#include "lg_tiles.hpp"
 
 namespace vsmcx {
 
@@ -42,45 +42,57 @@ struct SchemeOf<InvStratified> {
 
 // Sum over the current state of W_i (x_i, y_i) -- MonitorMCMC's mean after the
 // last step (core/monitor.hpp:274-302); x, y are the first two coordinates.
+// The single-filter and serving monitor tiles, tile after tile, then their
+// fixed-order reduction.
 template <typename Real, typename Inv>
-__device__ __noinline__ void batch_final_monitor(const FilterCtl *ctl, const Real *st, int dim,
-                                                 const uint32_t *anc, const double *lw,
-                                                 uint64_t N, StepRow *row, BatchLds<Inv> &S)
+__device__ __noinline__ void batch_final_monitor(const BatchFilter &F, uint32_t f,
+                                                 const FilterCtl *ctl, const BatchPool &P,
+                                                 const LgParams &lp, StepRow *row,
+                                                 BatchLds<Inv> &S)
 {
-    const int prev = ctl->prev_resampled;
-    const double m = ctl->m, c = ctl->coeff, invN = 1.0 / (double)N;
-    Part5 v{0, 0, 0, 0};
+    const uint64_t N = F.n, T = (N + kMoveTile - 1) / kMoveTile;
+    const Real *st = (P.cur[f] ? (const Real *)P.st1 : (const Real *)P.st0) + F.soff;
+    double *part = P.tpart + 5 * (F.off / kMoveTile);
 #pragma unroll 1
-    for (uint64_t i = threadIdx.x; i < N; i += kBlock) {
-        uint64_t src = prev ? (uint64_t)anc[i] : i;
-        src = src < N ? src : i;
-        const double w = prev ? invN : exp(lw[i] - m) * c;
-        v.mx += w * (double)st[src * dim];
-        v.my += w * (double)st[src * dim + 1];
+    for (uint64_t tt = 0; tt < T; ++tt) {
+        if (F.model == kModelLg)
+            lg_monitor_tile<Real>(ctl, st, lp.dim, P.anc + F.off, P.lw + F.off, N, tt,
+                                  part + 5 * tt);
+        else
+            cv_monitor_tile<Real>(ctl, (const typename StateOf<Real>::type *)st, P.anc + F.off,
+                                  P.lw + F.off, N, tt, part + 5 * tt);
     }
-    v = block_reduce<kBlock>(v, Part5Add(), S.p5);
-    if (threadIdx.x == 0) {
-        row->mean_x = v.mx;
-        row->mean_y = v.my;
+    __syncthreads();
+    if (threadIdx.x < kWave) {
+        double mx, my;
+        tile_partials_monitor(part, T, mx, my);
+        if (threadIdx.x == 0) {
+            row->mean_x = mx;
+            row->mean_y = my;
+        }
     }
+    __syncthreads();
 }
 
-// The reductions that end every move (thread 0 writes the step's scalars).
+// The reductions that end every move: the tile partials in their fixed order
+// (thread 0 writes the step's scalars).
 template <typename Inv>
 __device__ __forceinline__ void batch_move_end(FilterCtl *ctl, const BatchPool &P, uint32_t f,
-                                               uint32_t cur, StepRow *hist, uint64_t t, double m,
-                                               double se, double se2, double mx, double my,
+                                               uint32_t cur, StepRow *hist, uint64_t t,
+                                               const double *part, uint64_t ntiles,
                                                BatchLds<Inv> &S)
 {
-    const double M = block_reduce<kBlock>(m, MaxOp(), S.L.red);
-    const double sc = (m == -INFINITY) ? 0.0 : exp(m - M);
-    Part5 v{se * sc, se2 * sc * sc, mx, my};
-    v = block_reduce<kBlock>(v, Part5Add(), S.p5);
-    if (threadIdx.x == 0) {
-        cv_step_scalars(ctl, M, v.se, v.se2, v.mx, v.my, hist, t);
-        P.cur[f] = 1 - cur;
-        S.flag = ctl->resample;
-        if (ctl->resample) resample_prologue(ctl, SchemeOf<Inv>::value);
+    __syncthreads();  // every tile's partials written
+    if (threadIdx.x < kWave) {
+        double M;
+        Part5 v;
+        tile_partials_lse(part, ntiles, M, v);
+        if (threadIdx.x == 0) {
+            cv_step_scalars(ctl, M, v.se, v.se2, v.mx, v.my, hist, t);
+            P.cur[f] = 1 - cur;
+            S.flag = ctl->resample;
+            if (ctl->resample) resample_prologue(ctl, SchemeOf<Inv>::value);
+        }
     }
     __syncthreads();
 }
@@ -89,119 +101,43 @@ __device__ __forceinline__ void batch_move_end(FilterCtl *ctl, const BatchPool &
 // register allocation, and what crosses phases lives in LDS, so the kernel
 // keeps the occupancy of its largest phase instead of the sum of all.
 
-// CV model: move, likelihood, log-sum-exp, decision
+// CV model: the single filter's move tiles, tile after tile, then the
+// decision.
 template <typename Real, typename Inv>
 __device__ __noinline__ void batch_move_cv(const BatchFilter &F, uint32_t f, const BatchPool &P,
                                            const CvParams &p, uint64_t t, BatchLds<Inv> &S)
 {
     typedef typename StateOf<Real>::type S4;
-    const int tid = threadIdx.x;
     FilterCtl *ctl = P.ctl + f;
-    const uint64_t N = F.n;
+    const uint64_t N = F.n, T = (N + kMoveTile - 1) / kMoveTile;
     S4 *sa = (S4 *)((Real *)P.st0 + F.soff), *sb = (S4 *)((Real *)P.st1 + F.soff);
     const uint32_t cur = P.cur[f];
-    const S4 *in = cur ? sb : sa;
-    S4 *out = cur ? sa : sb;
-    double *lw = P.lw + F.off;
-    const uint32_t *anc = P.anc + F.off;
-    CvModel<Real> md;
-    md.load(p);
-    const int prev = t > 0 ? ctl->prev_resampled : 0;
-    const double m_old = ctl->m, c_old = ctl->coeff;
-    const double invN = 1.0 / (double)N;
-    const Real ox = (Real)P.obs[F.obs_off + 2 * t], oy = (Real)P.obs[F.obs_off + 2 * t + 1];
     const uint32_t k0 = (uint32_t)F.seed, k1 = (uint32_t)(F.seed >> 32);
-    double m = -INFINITY, se = 0, se2 = 0, mx = 0, my = 0;
+    double *part = P.tpart + 5 * (F.off / kMoveTile);
 #pragma unroll 1
-    for (uint64_t i = tid; i < N; i += kBlock) {
-        S4 s;
-        double lwo = 0;
-        if (t > 0) {
-            uint64_t src = prev ? (uint64_t)anc[i] : i;
-            src = src < N ? src : i;
-            s = in[src];
-            if (!prev) lwo = lw[i];
-            const double w = prev ? invN : exp(lwo - m_old) * c_old;
-            mx += w * (double)s.x;
-            my += w * (double)s.y;
-        }
-        Real z[4];
-        cv_normals(z, k0, k1, i, t);
-        cv_advance(md, s, z, t);
-        const double lwn = lwo + (double)md.llh(s.x, s.y, ox, oy);
-        out[i] = s;
-        lw[i] = lwn;
-        lse_add(m, se, se2, lwn);
-    }
-    batch_move_end(ctl, P, f, cur, P.hist + F.hist_off, t, m, se, se2, mx, my, S);
+    for (uint64_t tt = 0; tt < T; ++tt)
+        cv_move_tile<Real>(ctl, cur ? sb : sa, cur ? sa : sb, P.anc + F.off, P.lw + F.off,
+                           P.obs + F.obs_off, p, k0, k1, t, N, tt, part + 5 * tt);
+    batch_move_end(ctl, P, f, cur, P.hist + F.hist_off, t, part, T, S);
 }
 
-// Linear-Gaussian model: x_t = A x_{t-1} + q z (x_0 = s0 z), y_t = x_t + r v;
-// the normals of particle i at step t come from Philox blocks
-// {t+1, id_lo, id_hi, j}, j = 0, 1, 2 (four normals each).
+// Linear-Gaussian model: the serving tick's move tiles, tile after tile.
 template <typename Real, typename Inv>
 __device__ __noinline__ void batch_move_lg(const BatchFilter &F, uint32_t f, const BatchPool &P,
                                            const LgParams &lp, uint64_t t, BatchLds<Inv> &S)
 {
-    const int tid = threadIdx.x;
     FilterCtl *ctl = P.ctl + f;
-    const uint64_t N = F.n;
-    const int D = lp.dim;
+    const uint64_t N = F.n, T = (N + kMoveTile - 1) / kMoveTile;
     Real *sa = (Real *)P.st0 + F.soff, *sb = (Real *)P.st1 + F.soff;
     const uint32_t cur = P.cur[f];
-    const Real *in = cur ? sb : sa;
-    Real *out = cur ? sa : sb;
-    double *lw = P.lw + F.off;
-    const uint32_t *anc = P.anc + F.off;
-    const int prev = t > 0 ? ctl->prev_resampled : 0;
-    const double m_old = ctl->m, c_old = ctl->coeff;
-    const double invN = 1.0 / (double)N;
-    const double *y = P.obs + F.obs_off + (uint64_t)D * t;
-    const Real q = (Real)lp.q, s0 = (Real)lp.s0, inv_r2 = (Real)(1 / (lp.r * lp.r));
     const uint32_t k0 = (uint32_t)F.seed, k1 = (uint32_t)(F.seed >> 32);
-    double m = -INFINITY, se = 0, se2 = 0, mx = 0, my = 0;
+    double *part = P.tpart + 5 * (F.off / kMoveTile);
 #pragma unroll 1
-    for (uint64_t i = tid; i < N; i += kBlock) {
-        Real x[kLgMaxDim], xn[kLgMaxDim];
-        double lwo = 0;
-        if (t > 0) {
-            uint64_t src = prev ? (uint64_t)anc[i] : i;
-            src = src < N ? src : i;
-#pragma unroll
-            for (int d = 0; d < kLgMaxDim; ++d) x[d] = d < D ? in[src * D + d] : (Real)0;
-            if (!prev) lwo = lw[i];
-            const double w = prev ? invN : exp(lwo - m_old) * c_old;
-            mx += w * (double)x[0];
-            my += w * (double)x[1];
-        }
-        Real z[12];
-#pragma unroll
-        for (int j = 0; j < 3; ++j) lg_normals(z + 4 * j, k0, k1, i, t, (uint32_t)j);
-        Real ss = 0;
-#pragma unroll
-        for (int d = 0; d < kLgMaxDim; ++d) {
-            if (d >= D) break;
-            Real v;
-            if (t == 0) {
-                v = s0 * z[d];
-            } else {
-                v = q * z[d];
-#pragma unroll
-                for (int e = 0; e < kLgMaxDim; ++e)
-                    if (e < D) v += lg_a<Real>(lp, d * D + e) * x[e];
-            }
-            xn[d] = v;
-            const Real r = (Real)y[d] - v;
-            ss += r * r;
-        }
-        const double lwn = lwo + (lp.llh_const - 0.5 * (double)(ss * inv_r2));
-#pragma unroll
-        for (int d = 0; d < kLgMaxDim; ++d)
-            if (d < D) out[i * D + d] = xn[d];
-        lw[i] = lwn;
-        lse_add(m, se, se2, lwn);
-    }
-    batch_move_end(ctl, P, f, cur, P.hist + F.hist_off, t, m, se, se2, mx, my, S);
+    for (uint64_t tt = 0; tt < T; ++tt)
+        lg_move_tile<Real>(ctl, cur ? sb : sa, cur ? sa : sb, P.anc + F.off, P.lw + F.off,
+                           P.obs + F.obs_off + (uint64_t)lp.dim * t, lp, k0, k1, t, N, tt,
+                           part + 5 * tt);
+    batch_move_end(ctl, P, f, cur, P.hist + F.hist_off, t, part, T, S);
 }
 
 // tile sums and their exclusive scan (approximate tile starts)
@@ -365,10 +301,7 @@ __device__ void batch_step(const BatchFilter &F, uint32_t f, const BatchPool &P,
     }
     if (t + 1 == F.horizon) {
         __syncthreads();
-        const int dim = F.model == kModelLg ? lp.dim : 4;
-        const Real *cur_state = (P.cur[f] ? (const Real *)P.st1 : (const Real *)P.st0) + F.soff;
-        batch_final_monitor<Real, Inv>(ctl, cur_state, dim, P.anc + F.off, P.lw + F.off, F.n,
-                                       P.hist + F.hist_off + t, S);
+        batch_final_monitor<Real, Inv>(F, f, ctl, P, lp, P.hist + F.hist_off + t, S);
     }
     if (threadIdx.x == 0) {
         P.next_t[f] = t + 1;

@@ -1,0 +1,127 @@
+// lg_tiles.hpp -- the per-tile bodies of the linear-Gaussian model's step
+// (x_t = A x_{t-1} + q z, y_t = x_t + r v), shared by the serving tick
+// (kernels_tick.hip) and the persistent batch kernel (kernels_batch.hip),
+// which run them on the same tiles in the same order; the tile partials
+// have the layout of the CV tiles (cv_tiles.hpp).
+#pragma once
+#include "cv_model.hpp"
+#include "device_util.hpp"
+#include "launch.hpp"
+#include "lg_model.hpp"
+
+namespace vsmcx {
+
+// ---- the linear-Gaussian move on one tile (same tile shape as cv_move_tile) ----
+template <typename Real>
+__device__ __forceinline__ void lg_move_tile(const FilterCtl *ctl, const Real *__restrict__ in,
+                                             Real *__restrict__ out, const uint32_t *__restrict__ anc,
+                                             double *__restrict__ lw, const double *__restrict__ y,
+                                             const LgParams &lp, uint32_t k0, uint32_t k1,
+                                             uint64_t t, uint64_t N, uint64_t tile, double *part)
+{
+    __shared__ double s_red[kBlock / kWave];
+    __shared__ Part5 s_p5[kBlock / kWave];
+    const int D = lp.dim;
+    const int prev = t > 0 ? ctl->prev_resampled : 0;
+    const double m_old = ctl->m, c_old = ctl->coeff;
+    const double invN = 1.0 / (double)N;
+    const Real q = (Real)lp.q, s0 = (Real)lp.s0, inv_r2 = (Real)(1 / (lp.r * lp.r));
+    const uint64_t base = tile * (uint64_t)kMoveTile;
+    double m = -INFINITY, mx = 0, my = 0;
+    double lwv[kMoveItems];
+#pragma unroll
+    for (int k = 0; k < kMoveItems; ++k) lwv[k] = -INFINITY;
+#pragma unroll 1
+    for (int k = 0; k < kMoveItems; ++k) {
+        const uint64_t i = base + (uint64_t)k * kBlock + threadIdx.x;
+        if (i >= N) continue;
+        Real x[kLgMaxDim], xn[kLgMaxDim];
+        double lwo = 0;
+        if (t > 0) {
+            uint64_t src = prev ? (uint64_t)anc[i] : i;
+            src = src < N ? src : i;
+#pragma unroll
+            for (int d = 0; d < kLgMaxDim; ++d) x[d] = d < D ? in[src * D + d] : (Real)0;
+            if (!prev) lwo = lw[i];
+            const double w = prev ? invN : exp(lwo - m_old) * c_old;
+            mx += w * (double)x[0];
+            my += w * (double)x[1];
+        }
+        Real z[12];
+#pragma unroll
+        for (int j = 0; j < 3; ++j) lg_normals(z + 4 * j, k0, k1, i, t, (uint32_t)j);
+        Real ss = 0;
+#pragma unroll
+        for (int d = 0; d < kLgMaxDim; ++d) {
+            if (d >= D) break;
+            Real v;
+            if (t == 0) {
+                v = s0 * z[d];
+            } else {
+                v = q * z[d];
+#pragma unroll
+                for (int e = 0; e < kLgMaxDim; ++e)
+                    if (e < D) v += lg_a<Real>(lp, d * D + e) * x[e];
+            }
+            xn[d] = v;
+            const Real r = (Real)y[d] - v;
+            ss += r * r;
+        }
+        const double lwn = lwo + (lp.llh_const - 0.5 * (double)(ss * inv_r2));
+#pragma unroll
+        for (int d = 0; d < kLgMaxDim; ++d)
+            if (d < D) out[i * D + d] = xn[d];
+        lw[i] = lwn;
+        lwv[k] = lwn;
+        m = lwn > m ? lwn : m;
+    }
+    // the tile's maximum first, then one exp per particle relative to it
+    const double M = block_reduce<kBlock>(m, MaxOp(), s_red);
+    double se = 0, se2 = 0;
+    if (M != -INFINITY) {
+#pragma unroll
+        for (int k = 0; k < kMoveItems; ++k) {
+            const double e = exp(lwv[k] - M);
+            se += e;
+            se2 += e * e;
+        }
+    }
+    Part5 v{se, se2, mx, my};
+    v = block_reduce<kBlock>(v, Part5Add(), s_p5);
+    if (threadIdx.x == 0) {
+        part[0] = M;
+        part[1] = v.se;
+        part[2] = v.se2;
+        part[3] = v.mx;
+        part[4] = v.my;
+    }
+}
+
+// final monitor of a linear-Gaussian job: mean of the first two coordinates
+template <typename Real>
+__device__ __forceinline__ void lg_monitor_tile(const FilterCtl *ctl, const Real *st, int D,
+                                                const uint32_t *anc, const double *lw, uint64_t N,
+                                                uint64_t tile, double *part)
+{
+    __shared__ Part5 s_p5[kBlock / kWave];
+    const int prev = ctl->prev_resampled;
+    const double m = ctl->m, c = ctl->coeff, invN = 1.0 / (double)N;
+    const uint64_t base = tile * (uint64_t)kMoveTile;
+    Part5 v{0, 0, 0, 0};
+    for (int k = 0; k < kMoveItems; ++k) {
+        const uint64_t i = base + (uint64_t)k * kBlock + threadIdx.x;
+        if (i >= N) continue;
+        uint64_t src = prev ? (uint64_t)anc[i] : i;
+        src = src < N ? src : i;
+        const double w = prev ? invN : exp(lw[i] - m) * c;
+        v.mx += w * (double)st[src * D];
+        v.my += w * (double)st[src * D + 1];
+    }
+    v = block_reduce<kBlock>(v, Part5Add(), s_p5);
+    if (threadIdx.x == 0) {
+        part[3] = v.mx;
+        part[4] = v.my;
+    }
+}
+
+}  // namespace vsmcx
