@@ -111,7 +111,7 @@ def test_serve_admission_under_budget(ctx, A):
     """Room for about two of the jobs at a time: they run in waves, results unchanged."""
     jobs = [(V.MODEL_CV, 60000, 10, 77 + k, 88 + k, V.cv_simulate(10, 80 + k)) for k in range(6)]
     alone = [_run_alone(ctx, A, j) for j in jobs]
-    per_particle = 64 + 2 * 4 * 7  # arena bytes per particle (mixed, CV/LG average state)
+    per_particle = 80 + 2 * 4 * 7  # arena bytes per particle (mixed, CV/LG average state)
     budget = int(2.2 * 61440 * per_particle)
     s = _serve(ctx, A, budget=budget)
     mix = _run_mix(s, jobs, stagger=0)

@@ -45,7 +45,7 @@ struct vsmcx_batch {
     {
         void *ps[] = {df, order_work, order_given, P.st0, P.st1, P.lw, P.anc, P.zl, P.sl,
                       P.rep32, P.spec, P.gval, P.ts, P.tagg, P.tpre, P.tzs, P.tsum, P.tzsx,
-                      P.tpart, P.ctl, P.hist, (void *)P.obs, P.next_t,
+                      P.tpart, P.wx, P.thr, P.thr_fb, P.ctl, P.hist, (void *)P.obs, P.next_t,
                       (void *)P.avail, P.cur, P.err_any, queue, jobs_dev, items_dev};
         for (void *q : ps) ctx->release(q);
         if (ev0) (void)hipEventDestroy(ev0);
@@ -188,6 +188,9 @@ int vsmcx_batch_create(vsmcx_ctx *ctx, const vsmcx_batch_desc *d, vsmcx_batch **
         P.tsum = ctx->alloc_n<double>(off / kFoldTile);
         P.tzsx = ctx->alloc_n<uint64_t>(off / kFoldTile);
         P.tpart = ctx->alloc_n<double>(5 * (off / kFoldTile));
+        P.wx = ctx->alloc_n<double>(off);
+        P.thr = ctx->alloc_n<Seg>(off / kFoldTile * kBlock);
+        P.thr_fb = ctx->alloc_n<int32_t>(off / kFoldTile * kBlock);
         b->jobs_dev = ctx->alloc_n<uint32_t>(b->nf);
         b->items_dev = ctx->alloc_n<TickItem>(off / kFoldTile);
         P.ctl = ctx->alloc_n<FilterCtl>(b->nf);

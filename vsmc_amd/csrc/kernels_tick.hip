@@ -74,7 +74,7 @@ __global__ __launch_bounds__(kBlock) void k_tick_tsum(const TickItem *items,
     const BatchFilter F = filters[it.slot];
     XWeight xs{P.lw + F.off, 0.0, 0.0};
     xs.bind(ctl);
-    const double s = fold_tile_sum(xs, it.tile * (uint64_t)kFT, F.n, s_red);
+    const double s = fold_tile_sum(xs, it.tile * (uint64_t)kFT, F.n, s_red, P.wx + F.off);
     if (threadIdx.x == 0) P.tsum[F.off / kFT + it.tile] = s;
 }
 
@@ -98,11 +98,11 @@ __global__ __launch_bounds__(kBlock) void k_tick_seg(const TickItem *items,
     if (!ctl->resample) return;
     __shared__ FoldLdsT<false> L;
     const BatchFilter F = filters[it.slot];
-    XWeight xs{P.lw + F.off, 0.0, 0.0};
-    xs.bind(ctl);
+    // the weights the tile-sum pass wrote
+    const XCached<XWeight> xs{P.wx + F.off, XWeight{P.lw + F.off, 0.0, 0.0}};
     const uint64_t o = F.off / kFT + it.tile;
     fold_tile_seg(xs, it.tile * (uint64_t)kFT, F.n, P.ts[o], P.spec + F.off + it.tile * (uint64_t)kFT,
-                  P.tagg + o, L);
+                  P.tagg + o, L, P.thr + o * kBlock, P.thr_fb + o * kBlock);
 }
 
 __global__ __launch_bounds__(kScanBlock) void k_tick_scan_seg(const uint32_t *jobs,
@@ -140,14 +140,14 @@ __global__ __launch_bounds__(kBlock) void k_tick_apply(const TickItem *items,
     if (!ctl->resample) return;
     __shared__ FoldLdsT<Inv::kWindow> L;
     const BatchFilter F = filters[it.slot];
-    XWeight xs{P.lw + F.off, 0.0, 0.0};
-    xs.bind(ctl);
+    // the cached weights and the segment pass's per-thread prefixes
+    const XCached<XWeight> xs{P.wx + F.off, XWeight{P.lw + F.off, 0.0, 0.0}};
     Inv inv;
     inv.bind(ctl);
     const uint64_t o = F.off / kFT + it.tile;
     const FoldOut fo{P.anc + F.off, P.zl + F.off, P.sl + F.off, nullptr, nullptr};
     fold_tile_apply(ctl, xs, inv, it.tile, F.n, P.ts[o], P.tpre[o], P.gval + F.off, fo,
-                    P.rep32 + F.off, P.tzs + o, L);
+                    P.rep32 + F.off, P.tzs + o, L, P.thr + o * kBlock, P.thr_fb + o * kBlock);
 }
 
 __global__ __launch_bounds__(kScanBlock) void k_tick_scan_cnt(const uint32_t *jobs,

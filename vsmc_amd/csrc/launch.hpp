@@ -107,6 +107,9 @@ struct BatchPool {
     double *tsum;          // per tile (serving tick): sum of the weights
     uint64_t *tzsx;        // per tile (serving tick): exclusive prefix of tzs
     double *tpart;         // per tile (serving tick): 5 move partials
+    double *wx;            // per particle (ticks): the weights the tile-sum pass wrote
+    Seg *thr;              // per tile x 256 (ticks): each thread's segment prefix in its tile
+    int32_t *thr_fb;       // per tile x 256 (ticks): each thread's fast-path binade
     FilterCtl *ctl;        // [filters]
     StepRow *hist;         // history rows
     const double *obs;     // observations (CV: 2 per step, LG: dim per step)

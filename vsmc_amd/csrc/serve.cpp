@@ -142,6 +142,7 @@ struct vsmcx_serve {
     {
         void *ps[] = {df, P.st0, P.st1, P.lw, P.anc, P.zl, P.sl, P.rep32,
                       P.spec, P.gval, P.ts, P.tagg, P.tpre, P.tzs, P.tsum, P.tzsx, P.tpart,
+                      P.wx, P.thr, P.thr_fb,
                       stage_d, P.ctl, P.hist, (void *)P.obs,
                       P.next_t, (void *)P.avail,
                       P.cur, P.err_any};
@@ -361,12 +362,12 @@ int vsmcx_serve_create(vsmcx_ctx *ctx, const vsmcx_serve_desc *d, vsmcx_serve **
     }
     S->max_jobs = d->max_jobs;
     S->max_horizon = d->max_horizon;
-    // The budget covers the particle arena: per particle 64 B of weights,
+    // The budget covers the particle arena: per particle 80 B of weights,
     // ancestors, lists and fold scratch plus two states of the average model
     // dimension ((4 + lg_dim) / 2, the CV/LG mix).
     const double rb = S->real_bytes();
     const double avg_dim = 0.5 * (4 + d->lg_dim);
-    const double per_particle = 64 + 2 * rb * avg_dim;
+    const double per_particle = 80 + 2 * rb * avg_dim;
     const uint64_t particles = (uint64_t)((double)d->hbm_budget / per_particle);
     S->cap_units = particles / kUnit;
     REQUIRE(S->cap_units >= 1, "HBM budget too small for one job");
@@ -394,6 +395,9 @@ int vsmcx_serve_create(vsmcx_ctx *ctx, const vsmcx_serve_desc *d, vsmcx_serve **
         P.tsum = ctx->alloc_n<double>(S->cap_units);
         P.tzsx = ctx->alloc_n<uint64_t>(S->cap_units);
         P.tpart = ctx->alloc_n<double>(5 * S->cap_units);
+        P.wx = ctx->alloc_n<double>(P_);
+        P.thr = ctx->alloc_n<Seg>(S->cap_units * kBlock);
+        P.thr_fb = ctx->alloc_n<int32_t>(S->cap_units * kBlock);
         // staging: every slot initialised, one run and one count per slot,
         // the job and item lists of a tick, and the observation pool
         S->pool_cap = std::max<uint64_t>(1 << 16, (uint64_t)d->max_jobs * std::max<uint32_t>(2, d->lg_dim) * 2);
