@@ -301,6 +301,35 @@ int vsmcx_serve_history(vsmcx_serve *s, uint64_t id, uint64_t count, vsmcx_step_
 int vsmcx_serve_forget(vsmcx_serve *s, uint64_t id);
 int vsmcx_serve_stats(vsmcx_serve *s, vsmcx_serve_summary *out);
 
+/* ---- device-resident particle values ----------------------------------------------
+ * Storage for vSMC's value type T of Particle<T> (include/vsmc/core/particle.hpp:
+ * 64-70, 230-250) kept in HBM: n rows of dim values (FP32 for VSMCX_MIXED, FP64
+ * for VSMCX_FP64_FUSED).  vsmcx_state_copy is StateMatrix::copy
+ * (include/vsmc/core/state_matrix.hpp:285-300), x[to] = x[copy_from[to]], done
+ * as a device gather; the C++ adaptor vsmcx::StateHIP (include/vsmcx.hpp)
+ * wraps it as a vSMC value type. */
+typedef struct vsmcx_state vsmcx_state;
+
+int vsmcx_state_create(vsmcx_ctx *ctx, uint64_t n, uint32_t dim, int precision,
+                       vsmcx_state **out);
+int vsmcx_state_destroy(vsmcx_state *s);
+/* a second state with the same rows (Particle<T> copies its value) */
+int vsmcx_state_clone(vsmcx_state *s, vsmcx_state **out);
+int vsmcx_state_size(vsmcx_state *s, uint64_t *n, uint32_t *dim);
+/* rows: n x dim, row-major, FP64 on the host */
+int vsmcx_state_write(vsmcx_state *s, const double *rows);
+int vsmcx_state_read(vsmcx_state *s, double *rows);
+/* the current rows on the device (changes after vsmcx_state_copy), for user kernels */
+int vsmcx_state_data(vsmcx_state *s, void **device_rows);
+/* StateMatrix::copy: n must equal the state's size; copy_from is a host array */
+int vsmcx_state_copy(vsmcx_state *s, uint64_t n, const uint64_t *copy_from);
+/* The pf example's move_state (example/pf/include/pf_smp.hpp:150-238) for every
+ * particle of a 4-column state: the prior draw at iter 0, the transition after,
+ * with the model constants and Philox key of `model`; inc (host, n values)
+ * receives each particle's log-likelihood increment of observation obs_xy. */
+int vsmcx_state_cv_move(vsmcx_state *s, const vsmcx_filter_desc *model, uint64_t iter,
+                        const double *obs_xy, double *inc);
+
 /* ---- Gaussian-mixture SMC with adaptive tempering ------------------------------
  * The reference's gmm_smc sampler (example/gmm/src/gmm_smc.cpp:43-52 driven by
  * example/include/smc_do.hpp:57-66 under the "cess" schedule): particles carry

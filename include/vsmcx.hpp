@@ -14,6 +14,17 @@
 //   vsmcx::Filter                 a device-resident bootstrap filter of the pf
 //                                 model: a whole Sampler run (init, iterate,
 //                                 histories) in HBM
+//   vsmcx::StateHIP<Dim, FP>      a value type T for vsmc::Particle<T> whose
+//                                 particles live in HBM; its copy(N, copy_from)
+//                                 is StateMatrix::copy
+//                                 (include/vsmc/core/state_matrix.hpp:285-300)
+//                                 as a device gather
+//   vsmcx::CvMoveHIP              the pf example's init and move
+//                                 (example/pf/include/pf_smp.hpp:150-238) over a
+//                                 StateHIP<4, FP>, assignable to Sampler's
+//                                 init_type and move_type
+//                                 (include/vsmc/core/sampler.hpp:65-69); it
+//                                 updates vSMC's own WeightSet on the host
 //
 // Header only; link with libvsmcx.so.  Errors throw vsmcx::Error, which
 // carries the C ABI's status code and message (vSMC's runtime asserts throw
@@ -27,6 +38,7 @@
 #include <sstream>
 #include <stdexcept>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "vsmcx.h"
@@ -161,6 +173,95 @@ class HipResample {
     mutable std::vector<std::uint64_t> rep_, cf_;
 };
 
+// The context StateHIP values use when vsmc::Particle<T> constructs them with
+// T(N) alone: device 0, created on first use, kept for the life of the process.
+inline Context &default_context()
+{
+    static Context *c = new Context(0);
+    return *c;
+}
+
+// A vSMC value type with its particles in HBM: N rows of Dim values, FP32 or
+// FP64 (FP).  vsmc::Particle<StateHIP<Dim, FP>> calls copy(N, copy_from) after
+// every resample (include/vsmc/core/particle.hpp:230-250).
+template <std::size_t Dim, typename FP = double>
+class StateHIP {
+  public:
+    typedef std::size_t size_type;
+    typedef FP state_type;
+
+    explicit StateHIP(size_type N) : StateHIP(N, default_context()) {}
+    StateHIP(size_type N, Context &ctx)
+    {
+        check(vsmcx_state_create(ctx.get(), N, (std::uint32_t)Dim,
+                                 sizeof(FP) == 8 ? VSMCX_FP64_FUSED : VSMCX_MIXED, &h_));
+    }
+    StateHIP(const StateHIP &o) { check(vsmcx_state_clone(o.h_, &h_)); }
+    StateHIP &operator=(const StateHIP &o)
+    {
+        if (this != &o) {
+            vsmcx_state *c = nullptr;
+            check(vsmcx_state_clone(o.h_, &c));
+            if (h_) vsmcx_state_destroy(h_);
+            h_ = c;
+        }
+        return *this;
+    }
+    StateHIP(StateHIP &&o) noexcept : h_(o.h_) { o.h_ = nullptr; }
+    StateHIP &operator=(StateHIP &&o) noexcept
+    {
+        std::swap(h_, o.h_);
+        return *this;
+    }
+    ~StateHIP()
+    {
+        if (h_) vsmcx_state_destroy(h_);
+    }
+
+    size_type size() const
+    {
+        std::uint64_t n = 0;
+        check(vsmcx_state_size(h_, &n, nullptr));
+        return (size_type)n;
+    }
+    static constexpr std::size_t dim() { return Dim; }
+
+    // x[to] = x[copy_from[to]] for every particle
+    template <typename IntType>
+    void copy(size_type N, const IntType *copy_from)
+    {
+        std::vector<std::uint64_t> cf(copy_from, copy_from + N);
+        check(vsmcx_state_copy(h_, N, cf.data()));
+    }
+
+    // host access, synchronous: all rows (N x Dim, row-major)
+    std::vector<double> rows() const
+    {
+        std::vector<double> r(size() * Dim);
+        check(vsmcx_state_read(h_, r.data()));
+        return r;
+    }
+    void rows(const std::vector<double> &r) { check(vsmcx_state_write(h_, r.data())); }
+    // column pos of every particle, as StateMatrix::read_state
+    template <typename OutputIter>
+    void read_state(std::size_t pos, OutputIter first) const
+    {
+        const std::vector<double> r = rows();
+        for (size_type i = 0; i != r.size() / Dim; ++i, ++first) *first = r[i * Dim + pos];
+    }
+    // the current rows on the device, for user kernels
+    void *data() const
+    {
+        void *p = nullptr;
+        check(vsmcx_state_data(h_, &p));
+        return p;
+    }
+    vsmcx_state *get() const { return h_; }
+
+  private:
+    vsmcx_state *h_ = nullptr;
+};
+
 // Model and run description of the device-resident pf filter; the defaults are
 // the pf example's (example/pf/include/pf_smp.hpp:96-112,150-238).
 struct FilterConfig {
@@ -232,6 +333,66 @@ class Filter {
   private:
     vsmcx_filter *h_ = nullptr;
 };
+
+// The pf example's init and move over a StateHIP<4, FP>: the prior draw (as
+// init_type, size_t(Particle<T> &, void *)) or the transition at iteration
+// iter (as move_type, size_t(size_t, Particle<T> &)) of every particle on the
+// device, then the log-likelihood increments into the Particle's own
+// WeightSet: set_log_weight at init, add_log_weight after
+// (include/vsmc/core/weight_set.hpp:297-331).  Observation iter is row iter of
+// the (x, y) rows given.
+class CvMoveHIP {
+  public:
+    CvMoveHIP(const FilterConfig &c, const std::vector<double> &obs_xy) : obs_(obs_xy)
+    {
+        d_.n = c.n;
+        d_.horizon = c.horizon;
+        d_.precision = c.precision;
+        d_.scheme = (int)c.scheme;
+        d_.threshold = c.threshold;
+        d_.seed = c.seed;
+        d_.resample_seed = c.resample_seed;
+        d_.dt = c.dt;
+        d_.sd_pos = c.sd_pos;
+        d_.sd_vel = c.sd_vel;
+        d_.prior_sd_pos = c.prior_sd_pos;
+        d_.prior_sd_vel = c.prior_sd_vel;
+        d_.obs_scale = c.obs_scale;
+        d_.nu = c.nu;
+        d_.llh_const = Filter::llh_const(c.nu, c.obs_scale);
+    }
+
+    template <typename P>
+    std::size_t operator()(P &particle, void *)
+    {
+        step(particle, 0);
+        particle.weight_set().set_log_weight(inc_.data());
+        return 0;
+    }
+    template <typename P>
+    std::size_t operator()(std::size_t iter, P &particle)
+    {
+        step(particle, iter);
+        particle.weight_set().add_log_weight(inc_.data());
+        return 0;
+    }
+    // the increments of the last call
+    const std::vector<double> &increments() const { return inc_; }
+
+  private:
+    template <typename P>
+    void step(P &particle, std::size_t iter)
+    {
+        if (2 * iter + 1 >= obs_.size()) throw Error(VSMCX_ERR_ARG, "no observation for this iteration");
+        inc_.resize(particle.size());
+        check(vsmcx_state_cv_move(particle.value().get(), &d_, iter, &obs_[2 * iter], inc_.data()));
+    }
+
+    vsmcx_filter_desc d_{};
+    std::vector<double> obs_;
+    std::vector<double> inc_;
+};
+typedef CvMoveHIP CvInitHIP;
 
 }  // namespace vsmcx
 

@@ -4,6 +4,7 @@
 // helpers of the debug and state entry points.  The bodies are in
 // cv_tiles.hpp.  State is AoS: 4 x FP32 (16 B) in the mixed pipeline, 4 x
 // FP64 (32 B) in the FP64 pipelines.
+#include <algorithm>
 #include <math.h>
 
 #include "device_util.hpp"
@@ -118,6 +119,72 @@ void launch_cv_gather(bool fp64, FilterCtl *ctl, const void *st_in, void *st_out
     else
         hipLaunchKernelGGL(k_cv_gather<S4f>, dim3(tiles_of(N) * 16), dim3(256), 0, st, ctl,
                            (const S4f *)st_in, (S4f *)st_out, anc, N);
+}
+
+// ---- device-resident particle values (the vSMC value-type adaptor) ------------------
+// One pf-model step over a bare state (no weight set on the device): the
+// transition (or the prior draw at t == 0) and the log-likelihood increment of
+// every particle, with the move kernel's normals and arithmetic.
+template <typename Real>
+__global__ __launch_bounds__(kBlock) void k_state_cv_move(typename StateOf<Real>::type *st,
+                                                          double *inc, CvParams p, uint32_t k0,
+                                                          uint32_t k1, uint64_t t, Real ox,
+                                                          Real oy, uint64_t N)
+{
+    CvModel<Real> md;
+    md.load(p);
+    for (uint64_t i = blockIdx.x * (uint64_t)kBlock + threadIdx.x; i < N;
+         i += (uint64_t)gridDim.x * kBlock) {
+        typename StateOf<Real>::type s = st[i];
+        Real z[4];
+        cv_normals(z, k0, k1, i, t);
+        cv_advance(md, s, z, t);
+        st[i] = s;
+        inc[i] = (double)md.llh(s.x, s.y, ox, oy);
+    }
+}
+
+// StateMatrix::copy (core/state_matrix.hpp:285-300) as a gather into the
+// other buffer: out[to] = in[copy_from[to]], rows of dim values.
+template <typename Real>
+__global__ __launch_bounds__(kBlock) void k_state_gather(const Real *in, Real *out,
+                                                         const uint64_t *copy_from, uint32_t dim,
+                                                         uint64_t N, int *err)
+{
+    for (uint64_t e = blockIdx.x * (uint64_t)kBlock + threadIdx.x; e < N * dim;
+         e += (uint64_t)gridDim.x * kBlock) {
+        const uint64_t to = e / dim, d = e - to * dim;
+        const uint64_t from = copy_from[to];
+        if (from < N)
+            out[e] = in[from * dim + d];
+        else
+            atomicOr(err, (int)kErrBounds);
+    }
+}
+
+static unsigned grid_of(uint64_t n) { return (unsigned)std::min<uint64_t>((n + kBlock - 1) / kBlock, 1u << 16); }
+
+void launch_state_cv_move(bool fp64, void *st, double *inc, const CvParams &p, uint64_t seed,
+                          uint64_t t, double ox, double oy, uint64_t N, hipStream_t s)
+{
+    const uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32);
+    if (fp64)
+        hipLaunchKernelGGL(k_state_cv_move<double>, dim3(grid_of(N)), dim3(kBlock), 0, s,
+                           (S4d *)st, inc, p, k0, k1, t, ox, oy, N);
+    else
+        hipLaunchKernelGGL(k_state_cv_move<float>, dim3(grid_of(N)), dim3(kBlock), 0, s,
+                           (S4f *)st, inc, p, k0, k1, t, (float)ox, (float)oy, N);
+}
+
+void launch_state_gather(bool fp64, const void *in, void *out, const uint64_t *copy_from,
+                         uint32_t dim, uint64_t N, int *err, hipStream_t s)
+{
+    if (fp64)
+        hipLaunchKernelGGL(k_state_gather<double>, dim3(grid_of(N * dim)), dim3(kBlock), 0, s,
+                           (const double *)in, (double *)out, copy_from, dim, N, err);
+    else
+        hipLaunchKernelGGL(k_state_gather<float>, dim3(grid_of(N * dim)), dim3(kBlock), 0, s,
+                           (const float *)in, (float *)out, copy_from, dim, N, err);
 }
 
 }  // namespace vsmcx

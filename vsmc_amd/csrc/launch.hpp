@@ -54,6 +54,12 @@ void launch_cv_weights(FilterCtl *ctl, const double *lw, uint64_t N, double *w, 
 void launch_cv_gather(bool fp64, FilterCtl *ctl, const void *st_in, void *st_out,
                       const uint32_t *anc, uint64_t N, hipStream_t st);
 
+// kernels_cv.hip: device-resident particle values (the vSMC value-type adaptor)
+void launch_state_cv_move(bool fp64, void *st, double *inc, const CvParams &p, uint64_t seed,
+                          uint64_t t, double ox, double oy, uint64_t N, hipStream_t s);
+void launch_state_gather(bool fp64, const void *in, void *out, const uint64_t *copy_from,
+                         uint32_t dim, uint64_t N, int *err, hipStream_t s);
+
 // kernels_plain.hip: the un-fused FP64 baseline
 void launch_plain_init_weights(double *w, uint64_t N, hipStream_t s);
 void launch_plain_move(void *st, double *inc, const double *obs, const CvParams &p, uint64_t seed,
