@@ -324,6 +324,11 @@ VSMCX_HD Seg run_seg(X x, int n, double s, double er, double ea, int *fast_b)
     }
     *fast_b = kNoBinade;
     Seg g = seg_identity();
+    // the element-by-element path is rare: kept rolled, so the common path
+    // stays small in the instruction cache
+#if defined(__HIP_DEVICE_COMPILE__)
+#pragma unroll 1
+#endif
     for (int k = 0; k < n; ++k) {
         const double xk = x(k);
         const double sb = s;
@@ -360,6 +365,9 @@ VSMCX_HD bool run_values(X x, int n, double s, double er, double ea, int fast_b,
         }
         return ok;
     }
+#if defined(__HIP_DEVICE_COMPILE__)
+#pragma unroll 1
+#endif
     for (int k = 0; k < n; ++k) {
         const double xk = x(k);
         const double sb = s;

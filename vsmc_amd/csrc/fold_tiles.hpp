@@ -157,6 +157,16 @@ struct InvStratified {
             s_v[k] = a;
         }
     }
+    // u_n = (n + U_n) delta with U_n in [0, 1), so the first n with v_n > c
+    // lies within a few places of floor(c nn): a bracket [b0, b0 + 6) the
+    // caller verifies at both ends before trusting it.
+    __device__ bool bracket(double c, uint64_t *b0) const
+    {
+        const double est = c * (double)nn;
+        const uint64_t e = !(est > 0) ? 0 : (est >= (double)nn ? nn : (uint64_t)est);
+        *b0 = e >= 2 ? e - 2 : 0;
+        return true;
+    }
     __device__ uint64_t point_F(double c) const
     {
         if (nn == 0) return 0;
@@ -186,6 +196,7 @@ struct InvArray {
     {
         for (int k = threadIdx.x; k < len; k += blockDim.x) s_v[k] = v(w0 + k);
     }
+    __device__ bool bracket(double, uint64_t *) const { return false; }  // no closed form
     __device__ uint64_t point_F(double c) const
     {
         // binary search for the first n with v_n > c
@@ -239,6 +250,24 @@ template <typename Inv>
 __device__ uint64_t wave_first_greater(const Inv &inv, double c, uint64_t lo, uint64_t hi)
 {
     const int lane = threadIdx.x & 63;
+    uint64_t b0;
+    if (inv.bracket(c, &b0)) {
+        // one pass over b0-1 .. b0+62: v is non-decreasing, so if v(b0-1) <= c
+        // (or b0-1 lies below the range) the first lane above c is the answer
+        const uint64_t n = b0 + (uint64_t)lane;  // candidate + 1
+        bool gt;
+        if (n < lo + 1)
+            gt = false;
+        else if (n - 1 >= hi)
+            gt = true;
+        else
+            gt = inv.v(n - 1) > c;
+        const uint64_t mask = __ballot(gt);
+        if (mask && !(mask & 1)) {
+            const uint64_t r = b0 - 1 + (uint64_t)(__ffsll((unsigned long long)mask) - 1);
+            return r < lo ? lo : (r < hi ? r : hi);
+        }
+    }
     while (hi - lo > 64) {
         const uint64_t step = (hi - lo + 63) / 64;
         const uint64_t n = lo + (uint64_t)lane * step;
@@ -585,6 +614,20 @@ __device__ void fold_tile_apply(FilterCtl *ctl, const XSrc &xs, const Inv &inv, 
                         c[k] += len;
                         pos = len;
                         continue;
+                    }
+                    uint64_t g0;
+                    if (inv.bracket(a, &g0)) {
+                        // answer in [b, b + 6) iff v[b-1] <= a and v[b+6] > a
+                        const int64_t b = (int64_t)g0 - (int64_t)w0;
+                        const int bl = b < 0 ? 0 : (b > len ? len : (int)b);
+                        const int bh = b + 6 > len ? len : (b + 6 < 0 ? 0 : (int)(b + 6));
+                        if ((bl == 0 || s_x[bl - 1] <= a) && (bh == len || s_x[bh] > a)) {
+                            int q = bl;
+                            while (q < bh && s_x[q] <= a) ++q;
+                            c[k] += q;
+                            pos = q;
+                            continue;
+                        }
                     }
                     int lo = pos, step = 1, hi = pos;
                     while (hi < len && s_x[hi] <= a) {
