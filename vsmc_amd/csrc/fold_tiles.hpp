@@ -321,12 +321,12 @@ struct NoVisit {
     __device__ void operator()(uint64_t, const T &) const {}
 };
 
-template <typename T, typename Op, typename Visit = NoVisit>
-__device__ T wg_scan(const T *in, T *out, uint64_t n, Op op, T ident, T *lds,
-                     Visit visit = Visit())
+template <int BLOCK, typename T, typename Op, typename Visit = NoVisit>
+__device__ T wg_scan_b(const T *in, T *out, uint64_t n, Op op, T ident, T *lds,
+                       Visit visit = Visit())
 {
     T carry = ident;
-    for (uint64_t c0 = 0; c0 < n; c0 += (uint64_t)kScanBlock * kScanItems) {
+    for (uint64_t c0 = 0; c0 < n; c0 += (uint64_t)BLOCK * kScanItems) {
         const uint64_t i0 = c0 + (uint64_t)threadIdx.x * kScanItems;
         T v[kScanItems];
 #pragma unroll
@@ -335,7 +335,7 @@ __device__ T wg_scan(const T *in, T *out, uint64_t n, Op op, T ident, T *lds,
 #pragma unroll
         for (int k = 0; k < kScanItems; ++k) loc = op(loc, v[k]);
         T tot;
-        T run = op(carry, block_excl_scan<kScanBlock>(loc, op, ident, lds, &tot));
+        T run = op(carry, block_excl_scan<BLOCK>(loc, op, ident, lds, &tot));
 #pragma unroll
         for (int k = 0; k < kScanItems; ++k) {
             if (i0 + k < n) {
@@ -347,6 +347,13 @@ __device__ T wg_scan(const T *in, T *out, uint64_t n, Op op, T ident, T *lds,
         carry = op(carry, tot);
     }
     return carry;
+}
+
+template <typename T, typename Op, typename Visit = NoVisit>
+__device__ T wg_scan(const T *in, T *out, uint64_t n, Op op, T ident, T *lds,
+                     Visit visit = Visit())
+{
+    return wg_scan_b<kScanBlock>(in, out, n, op, ident, lds, visit);
 }
 
 // ---- tile sum of x (any fixed order) -----------------------------------------

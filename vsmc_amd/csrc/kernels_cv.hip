@@ -27,7 +27,7 @@ __global__ __launch_bounds__(kBlock) void k_cv_move(
                        partials + 5 * (uint64_t)blockIdx.x);
 }
 
-__global__ __launch_bounds__(1024) void k_cv_reduce(FilterCtl *ctl, const double *partials,
+__global__ __launch_bounds__(kWave) void k_cv_reduce(FilterCtl *ctl, const double *partials,
                                                     uint64_t ntiles, StepRow *hist, uint64_t t)
 {
     cv_reduce_tiles(ctl, partials, ntiles, hist, t);
@@ -43,7 +43,7 @@ __global__ __launch_bounds__(kBlock) void k_cv_monitor(FilterCtl *ctl,
     cv_monitor_tile<Real>(ctl, st, anc, lw, N, blockIdx.x, partials + 5 * (uint64_t)blockIdx.x);
 }
 
-__global__ __launch_bounds__(1024) void k_cv_monitor_sum(FilterCtl *ctl, const double *partials,
+__global__ __launch_bounds__(kWave) void k_cv_monitor_sum(FilterCtl *ctl, const double *partials,
                                                          uint64_t ntiles, StepRow *hist)
 {
     cv_monitor_reduce(ctl, partials, ntiles, hist);
@@ -88,7 +88,7 @@ void launch_cv_move(bool fp64, FilterCtl *ctl, const void *st_in, void *st_out,
 void launch_cv_reduce(FilterCtl *ctl, const double *partials, uint64_t ntiles, StepRow *hist,
                       uint64_t t, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_cv_reduce, dim3(1), dim3(1024), 0, st, ctl, partials, ntiles, hist, t);
+    hipLaunchKernelGGL(k_cv_reduce, dim3(1), dim3(kWave), 0, st, ctl, partials, ntiles, hist, t);
 }
 
 void launch_cv_monitor(bool fp64, FilterCtl *ctl, const void *stp, const uint32_t *anc,
@@ -101,7 +101,7 @@ void launch_cv_monitor(bool fp64, FilterCtl *ctl, const void *stp, const uint32_
     else
         hipLaunchKernelGGL(k_cv_monitor<float>, dim3(tiles_of(N)), dim3(kBlock), 0, st, ctl,
                            (const S4f *)stp, anc, lw, N, partials);
-    hipLaunchKernelGGL(k_cv_monitor_sum, dim3(1), dim3(1024), 0, st, ctl, partials,
+    hipLaunchKernelGGL(k_cv_monitor_sum, dim3(1), dim3(kWave), 0, st, ctl, partials,
                        (uint64_t)tiles_of(N), hist);
 }
 

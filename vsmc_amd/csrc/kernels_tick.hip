@@ -16,6 +16,11 @@
 
 namespace vsmcx {
 
+// Per-job kernels: the scans of a job's tile summaries (a job holds at most
+// kBatchMaxTiles = 512 tiles) run in workgroups of kJobBlock threads; the
+// reductions of tile partials use one wave.
+constexpr int kJobBlock = 256;
+
 __device__ __forceinline__ uint64_t tiles_of_job(const BatchFilter &F)
 {
     return (F.n + kFT - 1) / kFT;
@@ -48,7 +53,7 @@ __global__ __launch_bounds__(kBlock) void k_tick_move(const TickItem *items,
 }
 
 // ---- per job: reduction, decision, resample prologue -----------------------------
-__global__ __launch_bounds__(1024) void k_tick_reduce(const uint32_t *jobs,
+__global__ __launch_bounds__(kWave) void k_tick_reduce(const uint32_t *jobs,
                                                       const BatchFilter *filters, BatchPool P,
                                                       int scheme)
 {
@@ -78,16 +83,16 @@ __global__ __launch_bounds__(kBlock) void k_tick_tsum(const TickItem *items,
     if (threadIdx.x == 0) P.tsum[F.off / kFT + it.tile] = s;
 }
 
-__global__ __launch_bounds__(kScanBlock) void k_tick_scan_sum(const uint32_t *jobs,
-                                                              const BatchFilter *filters,
-                                                              BatchPool P)
+__global__ __launch_bounds__(kJobBlock) void k_tick_scan_sum(const uint32_t *jobs,
+                                                             const BatchFilter *filters,
+                                                             BatchPool P)
 {
     const uint32_t slot = jobs[blockIdx.x];
     if (!P.ctl[slot].resample) return;
-    __shared__ double s_red[kScanBlock / kWave];
+    __shared__ double s_red[kJobBlock / kWave];
     const BatchFilter F = filters[slot];
     const uint64_t o = F.off / kFT;
-    wg_scan<double>(P.tsum + o, P.ts + o, tiles_of_job(F), AddOp(), 0.0, s_red);
+    wg_scan_b<kJobBlock>(P.tsum + o, P.ts + o, tiles_of_job(F), AddOp(), 0.0, s_red);
 }
 
 __global__ __launch_bounds__(kBlock) void k_tick_seg(const TickItem *items,
@@ -105,25 +110,25 @@ __global__ __launch_bounds__(kBlock) void k_tick_seg(const TickItem *items,
                   P.tagg + o, L, P.thr + o * kBlock, P.thr_fb + o * kBlock);
 }
 
-__global__ __launch_bounds__(kScanBlock) void k_tick_scan_seg(const uint32_t *jobs,
-                                                              const BatchFilter *filters,
-                                                              BatchPool P)
+__global__ __launch_bounds__(kJobBlock) void k_tick_scan_seg(const uint32_t *jobs,
+                                                             const BatchFilter *filters,
+                                                             BatchPool P)
 {
     const uint32_t slot = jobs[blockIdx.x];
     FilterCtl *ctl = P.ctl + slot;
     if (!ctl->resample) return;
-    __shared__ Seg s_seg[kScanBlock / kWave];
+    __shared__ Seg s_seg[kJobBlock / kWave];
     __shared__ uint32_t s_cnt[kBatchMaxTiles + 1];
-    __shared__ SpecRec s_r[kScanBlock];
-    __shared__ double s_v[kScanBlock];
+    __shared__ SpecRec s_r[kJobBlock];
+    __shared__ double s_v[kJobBlock];
     const BatchFilter F = filters[slot];
     const uint64_t o = F.off / kFT, T = tiles_of_job(F);
-    const Seg tail = wg_scan<Seg>(P.tagg + o, P.tpre + o, T, SegOp(), seg_identity(), s_seg,
-                                  [&](uint64_t t, const Seg &p) { s_cnt[t] = (uint32_t)p.cnt; });
+    const Seg tail = wg_scan_b<kJobBlock>(P.tagg + o, P.tpre + o, T, SegOp(), seg_identity(), s_seg,
+                                          [&](uint64_t t, const Seg &p) { s_cnt[t] = (uint32_t)p.cnt; });
     __syncthreads();
     int bad;
-    const double tot = fold_chain<kScanBlock>(P.tpre + o, s_cnt, T, tail, P.spec + F.off,
-                                              P.gval + F.off, s_r, s_v, &bad);
+    const double tot = fold_chain<kJobBlock>(P.tpre + o, s_cnt, T, tail, P.spec + F.off,
+                                             P.gval + F.off, s_r, s_v, &bad);
     if (threadIdx.x == 0) {
         ctl->fold_total = tot;
         ctl->specials = tail.cnt;
@@ -150,18 +155,18 @@ __global__ __launch_bounds__(kBlock) void k_tick_apply(const TickItem *items,
                     P.rep32 + F.off, P.tzs + o, L, P.thr + o * kBlock, P.thr_fb + o * kBlock);
 }
 
-__global__ __launch_bounds__(kScanBlock) void k_tick_scan_cnt(const uint32_t *jobs,
-                                                              const BatchFilter *filters,
-                                                              BatchPool P)
+__global__ __launch_bounds__(kJobBlock) void k_tick_scan_cnt(const uint32_t *jobs,
+                                                             const BatchFilter *filters,
+                                                             BatchPool P)
 {
     const uint32_t slot = jobs[blockIdx.x];
     FilterCtl *ctl = P.ctl + slot;
     if (!ctl->resample) return;
-    __shared__ uint64_t s_red[kScanBlock / kWave];
+    __shared__ uint64_t s_red[kJobBlock / kWave];
     const BatchFilter F = filters[slot];
     const uint64_t o = F.off / kFT;
-    const uint64_t tot =
-        wg_scan<uint64_t>(P.tzs + o, P.tzsx + o, tiles_of_job(F), AddOp(), (uint64_t)0, s_red);
+    const uint64_t tot = wg_scan_b<kJobBlock>(P.tzs + o, P.tzsx + o, tiles_of_job(F), AddOp(),
+                                              (uint64_t)0, s_red);
     if (threadIdx.x == 0) ctl->zeros = tot >> 32;
 }
 
@@ -234,26 +239,22 @@ __global__ __launch_bounds__(kBlock) void k_tick_monitor(const TickItem *items,
     }
 }
 
-__global__ __launch_bounds__(1024) void k_tick_monitor_sum(const uint32_t *jobs,
-                                                           const BatchFilter *filters,
-                                                           BatchPool P)
+// One wave per job: the final monitor's reduction for jobs at their last
+// step, then the step counter and the error summary.
+__global__ __launch_bounds__(kWave) void k_tick_finish(const uint32_t *jobs,
+                                                       const BatchFilter *filters, BatchPool P)
 {
     const uint32_t slot = jobs[blockIdx.x];
     const BatchFilter F = filters[slot];
-    if (P.next_t[slot] + 1 != F.horizon) return;
-    cv_monitor_reduce(P.ctl + slot, P.tpart + 5 * (F.off / kFT), tiles_of_job(F),
-                      P.hist + F.hist_off);
-}
-
-// step counters and the error summary
-__global__ void k_tick_advance(const uint32_t *jobs, uint32_t njobs, BatchPool P)
-{
-    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= njobs) return;
-    const uint32_t slot = jobs[j];
-    P.next_t[slot] += 1;
-    const int e = P.ctl[slot].err;
-    if (e) atomicOr(P.err_any, e);
+    const uint64_t t = P.next_t[slot];
+    if (t + 1 == F.horizon)
+        cv_monitor_reduce(P.ctl + slot, P.tpart + 5 * (F.off / kFT), tiles_of_job(F),
+                          P.hist + F.hist_off);
+    if (threadIdx.x == 0) {
+        P.next_t[slot] = t + 1;
+        const int e = P.ctl[slot].err;
+        if (e) atomicOr(P.err_any, e);
+    }
 }
 
 // One workgroup per slot initialisation, per observation run, and per 64
@@ -299,13 +300,13 @@ void launch_tick(bool fp64, int scheme, const TickItem *items, uint32_t nitems,
                  const BatchPool &P, const CvParams &cv, const LgParams &lg, hipStream_t st)
 {
     if (!nitems || !njobs) return;
-    const dim3 gi(nitems), gj(njobs), b(kBlock), bs(kScanBlock);
+    const dim3 gi(nitems), gj(njobs), b(kBlock), bs(kJobBlock), bw(kWave);
     const dim3 gsmall((njobs + 255) / 256), bsmall(256);
     if (fp64)
         hipLaunchKernelGGL(k_tick_move<double>, gi, b, 0, st, items, filters, P, cv, lg);
     else
         hipLaunchKernelGGL(k_tick_move<float>, gi, b, 0, st, items, filters, P, cv, lg);
-    hipLaunchKernelGGL(k_tick_reduce, gj, bs, 0, st, jobs, filters, P, scheme);
+    hipLaunchKernelGGL(k_tick_reduce, gj, bw, 0, st, jobs, filters, P, scheme);
     hipLaunchKernelGGL(k_tick_tsum, gi, b, 0, st, items, filters, P);
     hipLaunchKernelGGL(k_tick_scan_sum, gj, bs, 0, st, jobs, filters, P);
     hipLaunchKernelGGL(k_tick_seg, gi, b, 0, st, items, filters, P);
@@ -322,8 +323,7 @@ void launch_tick(bool fp64, int scheme, const TickItem *items, uint32_t nitems,
         hipLaunchKernelGGL(k_tick_monitor<double>, gi, b, 0, st, items, filters, P, lg);
     else
         hipLaunchKernelGGL(k_tick_monitor<float>, gi, b, 0, st, items, filters, P, lg);
-    hipLaunchKernelGGL(k_tick_monitor_sum, gj, bs, 0, st, jobs, filters, P);
-    hipLaunchKernelGGL(k_tick_advance, gsmall, bsmall, 0, st, jobs, njobs, P);
+    hipLaunchKernelGGL(k_tick_finish, gj, bw, 0, st, jobs, filters, P);
 }
 
 }  // namespace vsmcx
