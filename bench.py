@@ -39,34 +39,63 @@ def move_bytes(prec, prev_resampled):
     return 2 * sb + 8 + (4 if prev_resampled else 8)
 
 
-def run_precision(ctx, prec, N, W, K, obs, schemes, profile=True):
+def run_precision(ctx, prec, N, W, K, obs, schemes, profile=True, concurrent=True):
+    """The six filters of config 1, each advanced one step per bench step.
+    concurrent: every filter has its own context (its own HIP stream, default
+    priority), so one filter's single-workgroup scans overlap another's
+    full-grid passes; the total is host wall-clock between full
+    synchronisations.  Otherwise the filters share ctx's stream in turn."""
     T = W + K
-    filters = [V.Filter(ctx, V.cv_desc(N, T, precision=prec, scheme=s, seed=2024 + s,
-                                       resample_seed=4242 + s)) for s in schemes]
+    ctxs = [V.Context(0) for _ in schemes] if concurrent else [ctx] * len(schemes)
+    filters = [V.Filter(c, V.cv_desc(N, T, precision=prec, scheme=s, seed=2024 + s,
+                                     resample_seed=4242 + s)) for c, s in zip(ctxs, schemes)]
     for f in filters:
         f.set_obs(obs)
     for _ in range(W):
         for f in filters:
             f.run(1)
-    V.lib().vsmcx_ctx_sync(ctx.h)
-    nmark = K * len(filters) + 1
-    tm = V.Timer(ctx, nmark)
+    uniq = list({id(c): c for c in ctxs}.values())
+
+    def sync_all():
+        for c in uniq:
+            V.lib().vsmcx_ctx_sync(c.h)
+
+    sync_all()
     for f in filters:
         f.profile(profile)
+    if concurrent:
+        tms = [V.Timer(c, K + 1) for c in ctxs]
+    else:
+        tm = V.Timer(ctx, K * len(filters) + 1)
     t0 = time.perf_counter()
     m = 0
     for k in range(K):
-        for f in filters:
-            tm.record(m)
-            m += 1
+        for j, f in enumerate(filters):
+            if concurrent:
+                tms[j].record(k)
+            else:
+                tm.record(m)
+                m += 1
             f.run(1)
-    tm.record(m)
-    V.lib().vsmcx_ctx_sync(ctx.h)
+    if concurrent:
+        for t in tms:
+            t.record(K)
+    else:
+        tm.record(m)
+    sync_all()
     wall = time.perf_counter() - t0
-    per = np.array([tm.elapsed(i, i + 1) for i in range(nmark - 1)]).reshape(K, len(filters))
-    total_ms = tm.elapsed(0, nmark - 1)
-    tm.close()
-    out = {"total_ms": total_ms, "wall_s": wall, "per_filter_step_ms": per}
+    if concurrent:
+        per = np.array([[tms[j].elapsed(k, k + 1) for j in range(len(filters))] for k in range(K)])
+        total_ms = wall * 1e3
+        for t in tms:
+            t.close()
+    else:
+        per = np.array([tm.elapsed(i, i + 1) for i in range(K * len(filters))]).reshape(
+            K, len(filters))
+        total_ms = tm.elapsed(0, K * len(filters))
+        tm.close()
+    out = {"total_ms": total_ms, "wall_s": wall, "per_filter_step_ms": per,
+           "concurrent": concurrent}
     moves = []
     hist = []
     for f in filters:
@@ -76,6 +105,9 @@ def run_precision(ctx, prec, N, W, K, obs, schemes, profile=True):
         moves.append(mt)
         hist.append(h)
         f.close()
+    if concurrent:
+        for c in uniq:
+            c.close()
     out["move_ms"] = moves
     out["hist"] = hist
     return out
@@ -314,6 +346,8 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=2000.0)
     ap.add_argument("--batch-filters", type=int, default=4096)
     ap.add_argument("--no-gmm", action="store_true")
+    ap.add_argument("--serial-filters", action="store_true",
+                    help="config 1: advance the six filters in turn on one stream")
     ap.add_argument("--gmm-n", type=int, default=1 << 20)
     ap.add_argument("--gmm-sweeps", type=int, default=5)
     ap.add_argument("--gmm-fp64", action="store_true",
@@ -345,37 +379,49 @@ def main():
     ctx = V.Context(0)
     res = {}
     precs = [V.MIXED] + ([] if a.no_fp64 else [V.FP64_FUSED, V.FP64_PLAIN])
+    serial = {}
     for prec in precs:
         print(f"[bench] precision {prec}: {len(schemes)} filters x N={N}, W={W} K={K}",
               file=sys.stderr, flush=True)
-        res[prec] = run_precision(ctx, prec, N, W, K, obs, schemes)
+        res[prec] = run_precision(ctx, prec, N, W, K, obs, schemes, profile=False,
+                                  concurrent=not a.serial_filters)
+        # the move kernel's roofline from a short run with the filters in turn
+        # (no other filter's kernels beside it)
+        ks = min(K, 8)
+        serial[prec] = run_precision(ctx, prec, N, 2, ks, obs[:2 + ks], schemes, profile=True,
+                                     concurrent=False)
     ctx.close()
 
     nf = len(schemes)
     mixed = res[V.MIXED]
     value = nf * N * K / (mixed["total_ms"] * 1e-3)
-    steps_ms = mixed["per_filter_step_ms"].sum(axis=1)
+    # a bench step: every filter one step; concurrently, the slowest filter's
+    # step bounds it, in turn the sum
+    steps_ms = (mixed["per_filter_step_ms"].max(axis=1) if mixed["concurrent"]
+                else mixed["per_filter_step_ms"].sum(axis=1))
     out = {
         "metric": METRIC, "value": value, "unit": "particle-updates/s", "n_gpus": 1,
         "steps": K, "warmup": W, "ms_per_step": mixed["total_ms"] / K,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp32+fp64",
         "data": "synthetic (observations simulated from the CV model, seed 11)",
         "config": {"workload": "config-1: CV bootstrap filter, N=2^24 per filter, six filters "
-                               "(one per resampling scheme) advanced in turn each step, "
-                               "ESS<N/2 resampling, T=W+K",
+                               "(one per resampling scheme) advanced one step each per bench "
+                               "step, " + ("each on its own HIP stream" if mixed["concurrent"]
+                                           else "in turn on one stream") +
+                               ", ESS<N/2 resampling, T=W+K",
                    "n": N, "filters_per_step": nf,
                    "schemes": [SCHEME_NAMES[s] for s in schemes], "horizon": W + K},
         "p50_step_ms": float(np.percentile(steps_ms, 50)),
         "p99_step_ms": float(np.percentile(steps_ms, 99)),
-        "roofline": roofline(V.MIXED, mixed, N, W),
+        "roofline": roofline(V.MIXED, serial[V.MIXED], N, 2),
     }
-    per_scheme = {}
+    per_scheme = {}  # each scheme's step on its own (the short run with the filters in turn)
     for j, s in enumerate(schemes):
-        ms = mixed["per_filter_step_ms"][:, j]
+        ms = serial[V.MIXED]["per_filter_step_ms"][:, j]
         h = mixed["hist"][j]
         per_scheme[SCHEME_NAMES[s]] = {
-            "updates_per_s": N * K / (ms.sum() * 1e-3), "p50_ms": float(np.percentile(ms, 50)),
-            "p99_ms": float(np.percentile(ms, 99)),
+            "updates_per_s": N * len(ms) / (ms.sum() * 1e-3),
+            "p50_ms": float(np.percentile(ms, 50)), "p99_ms": float(np.percentile(ms, 99)),
             "resample_frac": float(h["resampled"][W:].mean()), "log_z": float(h["log_z"][-1])}
     out["per_scheme"] = per_scheme
     for prec, name in ((V.FP64_FUSED, "fp64_fused"), (V.FP64_PLAIN, "fp64_plain")):
@@ -383,7 +429,7 @@ def main():
             r = res[prec]
             v = nf * N * K / (r["total_ms"] * 1e-3)
             out[name] = {"value": v, "ms_per_step": r["total_ms"] / K,
-                         "roofline": roofline(prec, r, N, W),
+                         "roofline": roofline(prec, serial[prec], N, 2),
                          "log_z": {SCHEME_NAMES[s]: float(r["hist"][j]["log_z"][-1])
                                    for j, s in enumerate(schemes)}}
     if V.FP64_PLAIN in res:
