@@ -120,6 +120,10 @@ class Oracle:
         L.orc_cv_normals.argtypes = [C.c_uint64, C.c_uint64, C.c_uint64, f64p]
         L.orc_cv_llh.argtypes = [C.POINTER(CvParam)] + [C.c_double] * 4
         L.orc_cv_llh.restype = C.c_double
+        L.orc_cv_move_many.argtypes = [C.POINTER(CvParam), C.c_uint64, C.c_size_t, C.c_size_t,
+                                       f64p, C.c_double, C.c_double, f64p, f64p]
+        L.orc_cv_llh_many.argtypes = [C.POINTER(CvParam), C.c_size_t, f64p, C.c_double,
+                                      C.c_double, f64p]
         L.orc_cv_run.argtypes = [C.POINTER(CvParam), C.c_size_t, C.c_size_t, f64p, C.c_uint64,
                                  C.c_uint64, C.c_int, C.c_double, C.c_void_p, C.c_void_p,
                                  C.c_void_p, C.c_int]
@@ -263,6 +267,25 @@ class Oracle:
         z = np.zeros(4)
         self.lib.orc_cv_normals(seed, i, t, z)
         return z
+
+    def cv_move_many(self, seed, t, states, ox, oy, param=None):
+        """One CV step (prior draw at t == 0) of every row of states (n x 4):
+        the new states and the log-likelihood increments, in FP64."""
+        st = np.ascontiguousarray(states, np.float64).reshape(-1, 4)
+        out = np.zeros_like(st)
+        inc = np.zeros(st.shape[0])
+        p = param if param is not None else cv_param()
+        self.lib.orc_cv_move_many(C.byref(p), seed, t, st.shape[0], st, ox, oy, out, inc)
+        return out, inc
+
+    def cv_llh_many(self, states, ox, oy, param=None):
+        """FP64 log-likelihood increment of observation (ox, oy) at every row
+        (x, y, vx, vy) of states."""
+        st = np.ascontiguousarray(states, np.float64).reshape(-1, 4)
+        inc = np.zeros(st.shape[0])
+        p = param if param is not None else cv_param()
+        self.lib.orc_cv_llh_many(C.byref(p), st.shape[0], st, ox, oy, inc)
+        return inc
 
     def cv_run(self, N, obs, seed, resample_seed, scheme=SYSTEMATIC, threshold=0.5,
                param=None, threads=0, want_state=True):

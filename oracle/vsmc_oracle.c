@@ -516,6 +516,43 @@ void orc_cv_normals(uint64_t seed, uint64_t id, uint64_t t, double z[4])
     }
 }
 
+/* One step of the CV model for n particles from the given states (the
+ * transition of orc_cv_run; the prior draw at t == 0): st_out and the
+ * log-likelihood increments inc. */
+void orc_cv_move_many(const orc_cv_param *p, uint64_t seed, size_t t, size_t n,
+                      const double *st_in, double ox, double oy, double *st_out, double *inc)
+{
+#pragma omp parallel for schedule(static)
+    for (long long ii = 0; ii < (long long)n; ++ii) {
+        const size_t i = (size_t)ii;
+        double z[4];
+        orc_cv_normals(seed, i, t, z);
+        double *s = st_out + 4 * i;
+        if (t == 0) {
+            s[0] = p->prior_sd_pos * z[0];
+            s[1] = p->prior_sd_pos * z[1];
+            s[2] = p->prior_sd_vel * z[2];
+            s[3] = p->prior_sd_vel * z[3];
+        } else {
+            const double *q = st_in + 4 * i;
+            s[0] = q[0] + (p->sd_pos * z[0] + p->dt * q[2]);
+            s[1] = q[1] + (p->sd_pos * z[1] + p->dt * q[3]);
+            s[2] = q[2] + p->sd_vel * z[2];
+            s[3] = q[3] + p->sd_vel * z[3];
+        }
+        inc[i] = orc_cv_llh(p, s[0], s[1], ox, oy);
+    }
+}
+
+/* orc_cv_llh of every row (x, y, vx, vy) of st. */
+void orc_cv_llh_many(const orc_cv_param *p, size_t n, const double *st, double ox, double oy,
+                     double *inc)
+{
+#pragma omp parallel for schedule(static)
+    for (long long i = 0; i < (long long)n; ++i)
+        inc[i] = orc_cv_llh(p, st[4 * i], st[4 * i + 1], ox, oy);
+}
+
 double orc_cv_llh(const orc_cv_param *p, double x, double y, double ox, double oy)
 {
     double dx = (x - ox) / p->obs_scale;

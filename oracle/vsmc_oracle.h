@@ -120,6 +120,10 @@ typedef struct {
 /* one FP64 Box-Muller block: 4 normals from philox(ctr={t+1,id_lo,id_hi,0}, key=seed) */
 void orc_cv_normals(uint64_t seed, uint64_t id, uint64_t t, double z[4]);
 double orc_cv_llh(const orc_cv_param *p, double x, double y, double ox, double oy);
+void orc_cv_llh_many(const orc_cv_param *p, size_t n, const double *st, double ox, double oy,
+                     double *inc);
+void orc_cv_move_many(const orc_cv_param *p, uint64_t seed, size_t t, size_t n,
+                      const double *st_in, double ox, double oy, double *st_out, double *inc);
 
 /* Full filter run, FP64, vSMC step order (core/sampler.hpp:773-834).
  * state: N x 4 row-major (x, y, vx, vy) out; lw: N out; rec: T records.

@@ -246,6 +246,50 @@ def test_filter_mixed_first_step_coupled(ctx, oracle):
     f.close()
 
 
+@pytest.mark.parametrize("scheme", [V.SYSTEMATIC, V.STRATIFIED])
+def test_filter_mixed_log_evidence_coupled(ctx, oracle, scheme):
+    """The mixed pipeline's log-evidence against FP64 arithmetic on the mixed
+    pipeline's own particles.  Step 0: the FP64 likelihood at the mixed prior
+    draws (the same draws through vsmcx_state_cv_move, whose increments give
+    the filter's first log-evidence exactly).  Step t > 0: the FP64 oracle
+    re-runs the step from the mixed filter's population and weights (FP64
+    normals, transition and likelihood).  Each step adds
+    log sum_i W_i exp(inc_i).  Along the trajectory, resamples included, the
+    mixed log-evidence stays within 1e-6 relative of that FP64 value, and
+    within 1e-7 per step absolute on average: the error the FP32 state and
+    likelihood add."""
+    N, T = 1 << 16, 30
+    obs = _obs(T, 6)
+    desc = V.cv_desc(N, T, precision=V.MIXED, scheme=scheme, seed=42, resample_seed=4242)
+    s = V.State(ctx, N, 4, V.MIXED)
+    inc32 = s.cv_move(desc, 0, obs[0])
+    x0 = s.read()
+    s.close()
+    f = V.Filter(ctx, desc)
+    f.set_obs(obs)
+    f.run(1)
+
+    def lme(a):
+        m = a.max()
+        return m + np.log(np.exp(a - m).sum())
+
+    z0 = f.history(1)["log_z"][0]
+    assert abs(z0 - (lme(inc32) - np.log(N))) <= 1e-12 * abs(z0)
+    lz = lme(oracle.cv_llh_many(x0, obs[0, 0], obs[0, 1])) - np.log(N)
+    worst = abs(z0 - lz) / abs(lz)
+    for t in range(1, T):
+        st_prev, lw = f.state()
+        f.run(1)
+        _, inc = oracle.cv_move_many(42, t, st_prev, obs[t, 0], obs[t, 1])
+        lz += lme(lw + inc) - lme(lw)
+        got = f.history(t + 1)["log_z"][t]
+        worst = max(worst, abs(got - lz) / abs(lz))
+    f.close()
+    say("mixed coupled log-evidence: final", got, "fp64", lz, "worst relative error", worst)
+    assert worst <= 1e-6
+    assert abs(got - lz) <= 1e-7 * T
+
+
 def test_filter_mixed_vs_fp64_statistical(ctx, oracle):
     """After the first ancestor flip the mixed and FP64 runs are different Monte
     Carlo realisations (vSMC gives each slot its own stream, and one flip shifts

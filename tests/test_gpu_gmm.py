@@ -75,11 +75,12 @@ def test_gmm_mixed_initial_likelihoods(ctx, oracle):
     p32, l32, _ = g32.state()
     g64.close()
     g32.close()
-    # The same Philox words; the FP32 path converts them with 24-bit uniforms,
-    # so an exponential -log(u) with u within 2^-24 of 1 differs: a handful of
-    # parameters.  Everywhere else the draws agree to FP32 rounding.
+    # The same Philox words; the FP32 path converts them with 23-bit midpoint
+    # uniforms (cells of 2^-23), so an exponential -log(u) moves by more than
+    # 2e-5 relative wherever 1 - u < ~3e-3: about 0.15% of the particles.
+    # Everywhere else the draws agree to FP32 rounding.
     same = np.isclose(p32, p64, rtol=2e-5, atol=2e-5).all(1)
-    assert same.mean() > 0.999
+    assert same.mean() > 0.995
     # The FP32 likelihood of the FP32 parameters against the FP64 likelihood
     # of the same parameters (the likelihood moves by ~1e4 per unit lambda
     # for prior draws, so the rounding of the parameters themselves is not

@@ -253,3 +253,21 @@ def test_oracle_vs_live_reference(oracle):
             onxt = np.zeros(4, np.uint64)
             oracle.lib.orc_tf64_stream(C.byref(e), 4, onxt)
             assert (onxt == nxt).all()
+
+
+def test_cv_move_many_is_cv_run_step(oracle):
+    """The vectorised step used by the coupled mixed-precision test is the
+    oracle filter's own step: the same states and increments after step 0, and
+    after a second step with no resample in between."""
+    import vsmc_amd as V
+    N = 300
+    obs = V.cv_simulate(2, 4)
+    st1, lw1, rec = oracle.cv_run(N, obs[:1], 42, 4242, 3, 0.0)
+    s0, inc0 = oracle.cv_move_many(42, 0, np.zeros((N, 4)), obs[0, 0], obs[0, 1])
+    assert (s0 == st1).all()
+    assert (oracle.cv_llh_many(s0, obs[0, 0], obs[0, 1]) == inc0).all()
+    st2, lw2, _ = oracle.cv_run(N, obs, 42, 4242, 3, 0.0)  # threshold 0: never resamples
+    s1, inc1 = oracle.cv_move_many(42, 1, s0, obs[1, 0], obs[1, 1])
+    assert (s1 == st2).all()
+    lw = inc0 + inc1
+    np.testing.assert_allclose(lw - lw.max(), lw2, rtol=0, atol=1e-12)

@@ -117,7 +117,7 @@ def lib():
         raise RuntimeError(f"{LIB_PATH} is missing: run `python vsmc_amd/build.py` "
                            "(there is no CPU fallback)")
     L = C.CDLL(LIB_PATH)
-    vp, u64, i32, f64 = C.c_void_p, C.c_uint64, C.c_int, C.c_double
+    vp, u64, u32, i32, f64 = C.c_void_p, C.c_uint64, C.c_uint32, C.c_int, C.c_double
     P = C.POINTER
     sig = {
         "vsmcx_last_error": ([], C.c_char_p),
@@ -171,6 +171,14 @@ def lib():
         "vsmcx_serve_history": ([vp, u64, u64, vp], i32),
         "vsmcx_serve_forget": ([vp, u64], i32),
         "vsmcx_serve_stats": ([vp, P(ServeSummary)], i32),
+        "vsmcx_state_create": ([vp, u64, u32, i32, P(vp)], i32),
+        "vsmcx_state_destroy": ([vp], i32),
+        "vsmcx_state_clone": ([vp, P(vp)], i32),
+        "vsmcx_state_size": ([vp, P(u64), P(u32)], i32),
+        "vsmcx_state_write": ([vp, vp], i32),
+        "vsmcx_state_read": ([vp, vp], i32),
+        "vsmcx_state_copy": ([vp, u64, vp], i32),
+        "vsmcx_state_cv_move": ([vp, P(FilterDesc), u64, vp, vp], i32),
         "vsmcx_gmm_create": ([vp, P(GmmDesc), vp, u64, P(vp)], i32),
         "vsmcx_gmm_destroy": ([vp], i32),
         "vsmcx_gmm_initialize": ([vp], i32),
@@ -632,6 +640,48 @@ class Serve:
         s = ServeSummary()
         _check(lib().vsmcx_serve_stats(self.h, C.byref(s)))
         return {k: getattr(s, k) for k, _ in ServeSummary._fields_}
+
+
+class State:
+    """Device-resident particle values (vsmcx_state): n rows of dim values in
+    HBM, FP32 for MIXED and FP64 otherwise; the storage of vSMC's value type
+    with StateMatrix::copy as a device gather."""
+
+    def __init__(self, ctx, n, dim, precision=MIXED):
+        self.ctx = ctx
+        self.n, self.dim = n, dim
+        self.h = C.c_void_p()
+        _check(lib().vsmcx_state_create(ctx.h, n, dim, precision, C.byref(self.h)))
+
+    def close(self):
+        if self.h:
+            lib().vsmcx_state_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        self.close()
+
+    def write(self, rows):
+        r = np.ascontiguousarray(rows, np.float64).reshape(self.n, self.dim)
+        _check(lib().vsmcx_state_write(self.h, r.ctypes.data))
+
+    def read(self):
+        out = np.zeros((self.n, self.dim))
+        _check(lib().vsmcx_state_read(self.h, out.ctypes.data))
+        return out
+
+    def copy(self, copy_from):
+        cf = np.ascontiguousarray(copy_from, np.uint64)
+        _check(lib().vsmcx_state_copy(self.h, cf.size, cf.ctypes.data))
+
+    def cv_move(self, desc, iteration, obs_xy):
+        """The pf model's move of every row (the prior draw at iteration 0)
+        with the constants and Philox key of desc; returns the increments."""
+        o = np.ascontiguousarray(obs_xy, np.float64).reshape(2)
+        inc = np.zeros(self.n)
+        _check(lib().vsmcx_state_cv_move(self.h, C.byref(desc), iteration, o.ctypes.data,
+                                         inc.ctypes.data))
+        return inc
 
 
 class Gmm:

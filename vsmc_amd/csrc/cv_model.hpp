@@ -28,17 +28,8 @@ __device__ __forceinline__ void cv_normals(float z[4], uint32_t k0, uint32_t k1,
     c.v[2] = (uint32_t)(id >> 32);
     c.v[3] = 0;
     const u32x4 w = philox4x32_10(c, k0, k1);
-    // hardware transcendentals: v_log_f32 (log2), v_sqrt_f32, and v_sin_f32 /
-    // v_cos_f32, which take the angle in revolutions (sin(2 pi u2) = sin_rev(u2))
-    const float ln2 = 0.69314718f;
-#pragma unroll
-    for (int j = 0; j < 2; ++j) {
-        const float u1 = u01_oc_24(w.v[2 * j]);
-        const float u2 = u01_oc_24(w.v[2 * j + 1]);
-        const float r = __builtin_amdgcn_sqrtf(-2.0f * ln2 * __builtin_amdgcn_logf(u1));
-        z[2 * j] = r * __builtin_amdgcn_cosf(u2);
-        z[2 * j + 1] = r * __builtin_amdgcn_sinf(u2);
-    }
+    box_muller_f32(w.v[0], w.v[1], z[0], z[1]);
+    box_muller_f32(w.v[2], w.v[3], z[2], z[3]);
 }
 
 __device__ __forceinline__ void cv_normals(double z[4], uint32_t k0, uint32_t k1, uint64_t id,
@@ -62,10 +53,26 @@ __device__ __forceinline__ void cv_normals(double z[4], uint32_t k0, uint32_t k1
 }
 
 
+// One observation (ox, oy) as unevaluated sums hi + lo of two Reals.  In FP32
+// a particle's residual x - ox is then (x - hi) - lo, where x - hi is exact for
+// x within a factor 2 of hi (Sterbenz): the residual of a particle near the
+// observation carries its own rounding only, not the observation's FP32
+// rounding, which all particles would share.  In FP64, lo = 0.
+template <typename Real>
+struct CvObs {
+    Real hx, lx, hy, ly;
+    __device__ __forceinline__ CvObs(double ox, double oy)
+        : hx((Real)ox), lx((Real)(ox - (double)(Real)ox)), hy((Real)oy),
+          ly((Real)(oy - (double)(Real)oy))
+    {
+    }
+};
+
 template <typename Real>
 struct CvModel {
-    Real dt, sd_pos, sd_vel, psd_pos, psd_vel, scale, nu, c2, half_nu1;
+    Real dt, sd_pos, sd_vel, psd_pos, psd_vel, scale, nu;
     Real inv_scale, inv_nu;
+    double c2, half_nu1, half_nu1_ln2;
     __device__ void load(const CvParams &p)
     {
         dt = (Real)p.dt;
@@ -75,37 +82,51 @@ struct CvModel {
         psd_vel = (Real)p.prior_sd_vel;
         scale = (Real)p.obs_scale;
         nu = (Real)p.nu;
-        c2 = (Real)(2 * p.llh_const);
-        half_nu1 = (Real)(0.5 * (p.nu + 1));
         inv_scale = (Real)(1 / p.obs_scale);
         inv_nu = (Real)(1 / p.nu);
+        c2 = 2 * p.llh_const;
+        half_nu1 = 0.5 * (p.nu + 1);
+        half_nu1_ln2 = half_nu1 * 0.69314718055994531;
     }
     // Student-t log-density of the observation, both coordinates
-    // (example/pf/include/pf_smp.hpp:96-112, log1p form of example/pf/pf_cl.cl)
-    __device__ Real llh(Real x, Real y, Real ox, Real oy) const;
+    // (example/pf/include/pf_smp.hpp:96-112, log1p form of example/pf/pf_cl.cl),
+    // returned in FP64
+    __device__ double llh(Real x, Real y, const CvObs<Real> &o) const;
 };
 
 // FP64: the reference's arithmetic (divisions, log1p)
 template <>
-__device__ __forceinline__ double CvModel<double>::llh(double x, double y, double ox,
-                                                       double oy) const
+__device__ __forceinline__ double CvModel<double>::llh(double x, double y,
+                                                       const CvObs<double> &o) const
 {
-    const double dx = (x - ox) / scale;
-    const double dy = (y - oy) / scale;
+    const double dx = (x - o.hx) / scale;
+    const double dy = (y - o.hy) / scale;
     const double lx = log1p(dx * dx / nu);
     const double ly = log1p(dy * dy / nu);
     return c2 - half_nu1 * (lx + ly);
 }
 
-// FP32: reciprocal multiplies and the hardware log (v_log_f32 is log2)
+// FP32: the residuals, squares and log2(1 + s) with s = (1 + a)(1 + b) - 1 in
+// FP32 on the hardware log (v_log_f32 is log2); the constant and the factor
+// (nu + 1)/2 ln 2 are applied in FP64, so their FP32 roundings, shared by all
+// particles, do not enter.  log2(1 + s) = log2(u) s / (u - 1) with u = 1 + s
+// rounded (Goldberg, "What every computer scientist should know about
+// floating-point arithmetic", 1991, Theorem 4): the correction cancels the
+// rounding of u, which for particles near the observation (s ~ 1e-7) is as
+// large as s itself; u == 1 leaves log2(1 + s) = s / ln 2.
 template <>
-__device__ __forceinline__ float CvModel<float>::llh(float x, float y, float ox, float oy) const
+__device__ __forceinline__ double CvModel<float>::llh(float x, float y,
+                                                      const CvObs<float> &o) const
 {
-    const float ln2 = 0.69314718f;
-    const float dx = (x - ox) * inv_scale;
-    const float dy = (y - oy) * inv_scale;
-    const float l2 = __builtin_amdgcn_logf((1.0f + dx * dx * inv_nu) * (1.0f + dy * dy * inv_nu));
-    return c2 - half_nu1 * ln2 * l2;
+    const float dx = ((x - o.hx) - o.lx) * inv_scale;
+    const float dy = ((y - o.hy) - o.ly) * inv_scale;
+    const float a = dx * dx * inv_nu, b = dy * dy * inv_nu;
+    const float s = __builtin_fmaf(a, b, a + b);
+    const float u = 1.0f + s;
+    const float d = u - 1.0f;  // exact
+    const float l2 = d == 0.0f ? s * 1.44269504f
+                               : __builtin_amdgcn_logf(u) * (s * __builtin_amdgcn_rcpf(d));
+    return c2 - half_nu1_ln2 * (double)l2;
 }
 
 

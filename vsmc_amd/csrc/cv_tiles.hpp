@@ -34,7 +34,7 @@ __device__ __forceinline__ void cv_move_tile(
     const int prev = t > 0 ? ctl->prev_resampled : 0;
     const double m_old = ctl->m, c_old = ctl->coeff;
     const double invN = 1.0 / (double)N;
-    const Real ox = (Real)obs[2 * t], oy = (Real)obs[2 * t + 1];
+    const CvObs<Real> o(obs[2 * t], obs[2 * t + 1]);
     const uint64_t base = tile * (uint64_t)kMoveTile;
 
     // Loads first, for all of the thread's particles (ancestor indices, then
@@ -75,8 +75,7 @@ __device__ __forceinline__ void cv_move_tile(
         Real z[4];
         cv_normals(z, k0, k1, i, t);
         cv_advance(md, sk, z, t);
-        const Real inc = md.llh(sk.x, sk.y, ox, oy);
-        lwn[k] = lwo[k] + (double)inc;
+        lwn[k] = lwo[k] + md.llh(sk.x, sk.y, o);
         st_out[i] = sk;
         lw[i] = lwn[k];
         m = lwn[k] > m ? lwn[k] : m;

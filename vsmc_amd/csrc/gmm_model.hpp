@@ -50,16 +50,12 @@ __device__ __forceinline__ void gmm_bm(uint32_t a, uint32_t b, double &z0, doubl
 }
 __device__ __forceinline__ void gmm_bm(uint32_t a, uint32_t b, float &z0, float &z1)
 {
-    // v_log_f32 is log2; v_sin/cos_f32 take revolutions
-    const float u1 = u01_oc_24(a), u2 = u01_oc_24(b);
-    const float r = __builtin_amdgcn_sqrtf(-2.0f * 0.69314718f * __builtin_amdgcn_logf(u1));
-    z0 = r * __builtin_amdgcn_cosf(u2);
-    z1 = r * __builtin_amdgcn_sinf(u2);
+    box_muller_f32(a, b, z0, z1);
 }
 __device__ __forceinline__ double gmm_log_u01(uint32_t a, double) { return log(u01_oc_53(a)); }
 __device__ __forceinline__ double gmm_log_u01(uint32_t a, float)
 {
-    return (double)(0.69314718f * __builtin_amdgcn_logf(u01_oc_24(a)));
+    return (double)(0.69314718f * __builtin_amdgcn_logf(u01_oo_24(a)));
 }
 __device__ __forceinline__ double gmm_log(double x) { return log(x); }
 __device__ __forceinline__ float gmm_log(float x) { return 0.69314718f * __builtin_amdgcn_logf(x); }

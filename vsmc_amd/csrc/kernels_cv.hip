@@ -128,11 +128,12 @@ void launch_cv_gather(bool fp64, FilterCtl *ctl, const void *st_in, void *st_out
 template <typename Real>
 __global__ __launch_bounds__(kBlock) void k_state_cv_move(typename StateOf<Real>::type *st,
                                                           double *inc, CvParams p, uint32_t k0,
-                                                          uint32_t k1, uint64_t t, Real ox,
-                                                          Real oy, uint64_t N)
+                                                          uint32_t k1, uint64_t t, double ox,
+                                                          double oy, uint64_t N)
 {
     CvModel<Real> md;
     md.load(p);
+    const CvObs<Real> o(ox, oy);
     for (uint64_t i = blockIdx.x * (uint64_t)kBlock + threadIdx.x; i < N;
          i += (uint64_t)gridDim.x * kBlock) {
         typename StateOf<Real>::type s = st[i];
@@ -140,7 +141,7 @@ __global__ __launch_bounds__(kBlock) void k_state_cv_move(typename StateOf<Real>
         cv_normals(z, k0, k1, i, t);
         cv_advance(md, s, z, t);
         st[i] = s;
-        inc[i] = (double)md.llh(s.x, s.y, ox, oy);
+        inc[i] = md.llh(s.x, s.y, o);
     }
 }
 
@@ -173,7 +174,7 @@ void launch_state_cv_move(bool fp64, void *st, double *inc, const CvParams &p, u
                            (S4d *)st, inc, p, k0, k1, t, ox, oy, N);
     else
         hipLaunchKernelGGL(k_state_cv_move<float>, dim3(grid_of(N)), dim3(kBlock), 0, s,
-                           (S4f *)st, inc, p, k0, k1, t, (float)ox, (float)oy, N);
+                           (S4f *)st, inc, p, k0, k1, t, ox, oy, N);
 }
 
 void launch_state_gather(bool fp64, const void *in, void *out, const uint64_t *copy_from,

@@ -21,15 +21,8 @@ __device__ __forceinline__ void lg_normals(float z[4], uint32_t k0, uint32_t k1,
     c.v[2] = (uint32_t)(id >> 32);
     c.v[3] = j;
     const u32x4 w = philox4x32_10(c, k0, k1);
-    const float ln2 = 0.69314718f;
-#pragma unroll
-    for (int h = 0; h < 2; ++h) {
-        const float u1 = u01_oc_24(w.v[2 * h]);
-        const float u2 = u01_oc_24(w.v[2 * h + 1]);
-        const float r = __builtin_amdgcn_sqrtf(-2.0f * ln2 * __builtin_amdgcn_logf(u1));
-        z[2 * h] = r * __builtin_amdgcn_cosf(u2);
-        z[2 * h + 1] = r * __builtin_amdgcn_sinf(u2);
-    }
+    box_muller_f32(w.v[0], w.v[1], z[0], z[1]);
+    box_muller_f32(w.v[2], w.v[3], z[2], z[3]);
 }
 
 __device__ __forceinline__ void lg_normals(double z[4], uint32_t k0, uint32_t k1, uint64_t id,

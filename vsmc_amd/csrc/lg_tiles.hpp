@@ -25,7 +25,8 @@ __device__ __forceinline__ void lg_move_tile(const FilterCtl *ctl, const Real *_
     const int prev = t > 0 ? ctl->prev_resampled : 0;
     const double m_old = ctl->m, c_old = ctl->coeff;
     const double invN = 1.0 / (double)N;
-    const Real q = (Real)lp.q, s0 = (Real)lp.s0, inv_r2 = (Real)(1 / (lp.r * lp.r));
+    const Real q = (Real)lp.q, s0 = (Real)lp.s0;
+    const double inv_r2 = 1 / (lp.r * lp.r);
     const uint64_t base = tile * (uint64_t)kMoveTile;
     double m = -INFINITY, mx = 0, my = 0;
     double lwv[kMoveItems];
@@ -64,10 +65,15 @@ __device__ __forceinline__ void lg_move_tile(const FilterCtl *ctl, const Real *_
                     if (e < D) v += lg_a<Real>(lp, d * D + e) * x[e];
             }
             xn[d] = v;
-            const Real r = (Real)y[d] - v;
+            // y_d as hi + lo: y_hi - v is exact near the observation, so the
+            // observation's own FP32 rounding, shared by all particles, does
+            // not enter the residual (lo = 0 in FP64)
+            const double yd = y[d];
+            const Real yh = (Real)yd, yl = (Real)(yd - (double)yh);
+            const Real r = (yh - v) + yl;
             ss += r * r;
         }
-        const double lwn = lwo + (lp.llh_const - 0.5 * (double)(ss * inv_r2));
+        const double lwn = lwo + (lp.llh_const - 0.5 * inv_r2 * (double)ss);
 #pragma unroll
         for (int d = 0; d < kLgMaxDim; ++d)
             if (d < D) out[i * D + d] = xn[d];

@@ -214,8 +214,26 @@ VSMCX_HD double canonical64(uint64_t x)
     return r;
 }
 
-// u01_open_closed_32_24 / _32_53 (include/vsmc/rng/u01.h:114-116, 139-141)
-VSMCX_HD float u01_oc_24(uint32_t u) { return (1.0f + (float)(u >> 8)) * (1.0f / 16777216.0f); }
+// u01_open_closed_32_53 (include/vsmc/rng/u01.h:139-141)
 VSMCX_HD double u01_oc_53(uint32_t u) { return (1.0 + (double)u) * (1.0 / 4294967296.0); }
+
+// u01_open_open_32_24 (include/vsmc/rng/u01.h:116-118): (0.5 + (u >> 9)) 2^-23,
+// the midpoint of one of 2^23 equal cells, so its mean is that of the 32-bit
+// uniform.  The open-closed 24-bit form (u01.h:112-113) takes the top of each
+// 2^-24 cell instead, which lowers E[-2 log u] by about 1e-6: every FP32
+// normal from it would be slightly short, a bias shared by all particles.
+VSMCX_HD float u01_oo_24(uint32_t u) { return (0.5f + (float)(u >> 9)) * (1.0f / 8388608.0f); }
+
+#if defined(__HIPCC__)
+// Box-Muller pair in FP32 on the hardware transcendentals: v_log_f32 is log2,
+// v_sin_f32 / v_cos_f32 take the angle in revolutions (sin(2 pi u) = sin_rev(u)).
+__device__ __forceinline__ void box_muller_f32(uint32_t a, uint32_t b, float &z0, float &z1)
+{
+    const float u1 = u01_oo_24(a), u2 = u01_oo_24(b);
+    const float r = __builtin_amdgcn_sqrtf(-2.0f * 0.69314718f * __builtin_amdgcn_logf(u1));
+    z0 = r * __builtin_amdgcn_cosf(u2);
+    z1 = r * __builtin_amdgcn_sinf(u2);
+}
+#endif
 
 }  // namespace vsmcx
