@@ -246,8 +246,9 @@ def test_filter_mixed_first_step_coupled(ctx, oracle):
     f.close()
 
 
-@pytest.mark.parametrize("scheme", [V.SYSTEMATIC, V.STRATIFIED])
-def test_filter_mixed_log_evidence_coupled(ctx, oracle, scheme):
+@pytest.mark.parametrize("N,T,scheme", [(1 << 16, 30, V.SYSTEMATIC), (1 << 16, 30, V.STRATIFIED),
+                                         (1 << 24, 10, V.SYSTEMATIC)])
+def test_filter_mixed_log_evidence_coupled(ctx, oracle, N, T, scheme):
     """The mixed pipeline's log-evidence against FP64 arithmetic on the mixed
     pipeline's own particles.  Step 0: the FP64 likelihood at the mixed prior
     draws (the same draws through vsmcx_state_cv_move, whose increments give
@@ -258,7 +259,6 @@ def test_filter_mixed_log_evidence_coupled(ctx, oracle, scheme):
     mixed log-evidence stays within 1e-6 relative of that FP64 value, and
     within 1e-7 per step absolute on average: the error the FP32 state and
     likelihood add."""
-    N, T = 1 << 16, 30
     obs = _obs(T, 6)
     desc = V.cv_desc(N, T, precision=V.MIXED, scheme=scheme, seed=42, resample_seed=4242)
     s = V.State(ctx, N, 4, V.MIXED)
@@ -285,7 +285,8 @@ def test_filter_mixed_log_evidence_coupled(ctx, oracle, scheme):
         got = f.history(t + 1)["log_z"][t]
         worst = max(worst, abs(got - lz) / abs(lz))
     f.close()
-    say("mixed coupled log-evidence: final", got, "fp64", lz, "worst relative error", worst)
+    say("mixed coupled log-evidence N", N, "T", T, "final", got, "fp64", lz,
+        "worst relative error", worst)
     assert worst <= 1e-6
     assert abs(got - lz) <= 1e-7 * T
 

@@ -108,18 +108,51 @@ __device__ __forceinline__ void cv_move_tile(
 // 64 lane results meet in a fixed butterfly (every lane ends with the same
 // totals).  A 1024-thread reduce launch and a 256-thread batch workgroup
 // therefore compute the same bits.  Call from all 64 lanes of wave 0.
+//
+// Each lane loads kPartAhead of its tiles at once and evaluates their exps
+// together before adding them in tile order: the order of the additions is
+// that of the plain loop, but one wave no longer waits a memory latency per
+// tile (8192 tiles at N = 2^24 are 128 per lane).
+constexpr int kPartAhead = 8;
+
 __device__ __forceinline__ void tile_partials_lse(const double *partials, uint64_t ntiles,
                                                   double &M, Part5 &acc)
 {
     const int lane = threadIdx.x & (kWave - 1);
+    constexpr uint64_t kStep = (uint64_t)kPartAhead * kWave;
     double m = -INFINITY;
-    for (uint64_t j = lane; j < ntiles; j += kWave) {
+    uint64_t j = lane;
+    for (; j + (kPartAhead - 1) * kWave < ntiles; j += kStep) {
+        double v[kPartAhead];
+#pragma unroll
+        for (int u = 0; u < kPartAhead; ++u) v[u] = partials[5 * (j + (uint64_t)u * kWave)];
+#pragma unroll
+        for (int u = 0; u < kPartAhead; ++u) m = m < v[u] ? v[u] : m;
+    }
+    for (; j < ntiles; j += kWave) {
         const double v = partials[5 * j];
         m = m < v ? v : m;
     }
     M = wave_reduce(m, MaxOp());
     Part5 a{0, 0, 0, 0};
-    for (uint64_t j = lane; j < ntiles; j += kWave) {
+    j = lane;
+    for (; j + (kPartAhead - 1) * kWave < ntiles; j += kStep) {
+        double q[kPartAhead][5], sc[kPartAhead];
+#pragma unroll
+        for (int u = 0; u < kPartAhead; ++u)
+#pragma unroll
+            for (int c = 0; c < 5; ++c) q[u][c] = partials[5 * (j + (uint64_t)u * kWave) + c];
+#pragma unroll
+        for (int u = 0; u < kPartAhead; ++u) sc[u] = q[u][0] == -INFINITY ? 0.0 : exp(q[u][0] - M);
+#pragma unroll
+        for (int u = 0; u < kPartAhead; ++u) {
+            a.se += q[u][1] * sc[u];
+            a.se2 += q[u][2] * sc[u] * sc[u];
+            a.mx += q[u][3];
+            a.my += q[u][4];
+        }
+    }
+    for (; j < ntiles; j += kWave) {
         const double *q = partials + 5 * j;
         const double sc = q[0] == -INFINITY ? 0.0 : exp(q[0] - M);
         a.se += q[1] * sc;
@@ -135,8 +168,23 @@ __device__ __forceinline__ void tile_partials_monitor(const double *partials, ui
                                                       double &mx, double &my)
 {
     const int lane = threadIdx.x & (kWave - 1);
+    constexpr uint64_t kStep = (uint64_t)kPartAhead * kWave;
     Part5 a{0, 0, 0, 0};
-    for (uint64_t j = lane; j < ntiles; j += kWave) {
+    uint64_t j = lane;
+    for (; j + (kPartAhead - 1) * kWave < ntiles; j += kStep) {
+        double x[kPartAhead], y[kPartAhead];
+#pragma unroll
+        for (int u = 0; u < kPartAhead; ++u) {
+            x[u] = partials[5 * (j + (uint64_t)u * kWave) + 3];
+            y[u] = partials[5 * (j + (uint64_t)u * kWave) + 4];
+        }
+#pragma unroll
+        for (int u = 0; u < kPartAhead; ++u) {
+            a.mx += x[u];
+            a.my += y[u];
+        }
+    }
+    for (; j < ntiles; j += kWave) {
         a.mx += partials[5 * j + 3];
         a.my += partials[5 * j + 4];
     }

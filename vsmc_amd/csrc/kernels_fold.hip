@@ -178,9 +178,28 @@ __global__ void k_plain_prologue(FilterCtl *ctl, int scheme)
 // computed with a fixed reduce-then-scan order (deterministic), then held
 // as a tile-wise running max so the sequence is sorted by construction.
 // ------------------------------------------------------------------------
-__device__ __forceinline__ double sorted_term(const Tf64State &e, uint64_t n, uint64_t nn)
+// The terms of n = n0 .. n0 + K - 1 (0 past nn).  Word n of the engine's
+// stream is word (n + index) mod 4 of the block at counter ctr + (n + index)/4
+// (the mapping of tf64_stream_word), so a run of consecutive n needs one
+// Threefry4x64-20 block per four words.  n0 is a multiple of 4 in every
+// caller, so the block changes at the same k in every lane.
+template <int K>
+__device__ __forceinline__ void sorted_terms(const Tf64State &e, uint64_t n0, uint64_t nn,
+                                             double term[K])
 {
-    return log(1 - canonical64(tf64_stream_word(e, n))) / (double)(nn - n);
+    u64x4 c0;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) c0.v[j] = e.ctr[j];
+    const uint64_t g0 = n0 + e.index;
+    u64x4 b = {};
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const uint64_t g = g0 + k, n = n0 + k;
+        const unsigned w = (unsigned)(g & 3);
+        if (k == 0 || w == 0) b = threefry4x64_20(ctr_add(c0, g >> 2), e.key);
+        const uint64_t word = w == 0 ? b.v[0] : w == 1 ? b.v[1] : w == 2 ? b.v[2] : b.v[3];
+        term[k] = n < nn ? log(1 - canonical64(word)) / (double)(nn - n) : 0.0;
+    }
 }
 
 __global__ __launch_bounds__(kBlock) void k_sorted_tilesum(FilterCtl *ctl, double *tsum)
@@ -191,11 +210,11 @@ __global__ __launch_bounds__(kBlock) void k_sorted_tilesum(FilterCtl *ctl, doubl
     const uint64_t base = blockIdx.x * (uint64_t)kTile;
     if (base >= nn) return;
     const Tf64State e = ctl->eng;
+    double term[kItems];
+    sorted_terms<kItems>(e, base + threadIdx.x * (uint64_t)kItems, nn, term);
     double s = 0;
-    for (int k = 0; k < kItems; ++k) {
-        const uint64_t n = base + threadIdx.x * kItems + k;
-        if (n < nn) s += sorted_term(e, n, nn);
-    }
+#pragma unroll
+    for (int k = 0; k < kItems; ++k) s += term[k];
     s = block_reduce<kBlock>(s, AddOp(), s_red);
     if (threadIdx.x == 0) tsum[blockIdx.x] = s;
 }
@@ -241,12 +260,10 @@ __global__ __launch_bounds__(kBlock) void k_sorted_values(FilterCtl *ctl, const 
     if (base >= nn) return;
     const Tf64State e = ctl->eng;
     double term[kItems];
+    sorted_terms<kItems>(e, base + threadIdx.x * (uint64_t)kItems, nn, term);
     double s = 0;
-    for (int k = 0; k < kItems; ++k) {
-        const uint64_t n = base + threadIdx.x * kItems + k;
-        term[k] = n < nn ? sorted_term(e, n, nn) : 0.0;
-        s += term[k];
-    }
+#pragma unroll
+    for (int k = 0; k < kItems; ++k) s += term[k];
     const double incl = block_incl_scan<kBlock>(s, AddOp(), s_red);
     double lm = texcl[blockIdx.x] + (incl - s);
     double mx = -1.0;
