@@ -160,7 +160,7 @@ struct vsmcx_filter {
             launch_plain_rest(ctl, st[0], lw, w, inc, ppart, scal, t, N, d.scheme, rb, hist, s);
         } else {
             launch_cv_reduce(ctl, partials, tiles, hist, t, s);
-            launch_resample_from_log_weights(ctl, lw, N, d.scheme, rb, hist, s);
+            launch_resample_from_log_weights(ctl, lw, N, d.scheme, rb, hist, s, partials);
             cur ^= 1;
         }
         ++t;
@@ -708,7 +708,7 @@ int vsmcx_filter_debug_step(vsmcx_filter *f, double *weight, uint64_t *replicati
     }
     if (resampled) *resampled = h.resample;
     f->rb.rep_out = c->alloc_n<uint64_t>(N);
-    launch_resample_from_log_weights(f->ctl, f->lw, N, f->d.scheme, f->rb, f->hist, s);
+    launch_resample_from_log_weights(f->ctl, f->lw, N, f->d.scheme, f->rb, f->hist, s, f->partials);
     VSMCX_HIP(hipGetLastError());
     if (h.resample) {
         if (replication)

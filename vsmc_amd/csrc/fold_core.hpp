@@ -420,12 +420,15 @@ __device__ __forceinline__ Seg lane_get(const Seg &v, int l)
 // is within gamma_{n-1} * sum of the exact sum (Higham, Accuracy and Stability
 // of Numerical Algorithms, 2nd ed., eq. 4.4), so the two differ by at most
 // 2 * n * 2^-53 * sum; the bound below is twice that, and an absolute term
-// covers subnormal partial sums.  Both passes use the same x values.  Too
-// large a bound only costs more specials; too small breaks exactness, which
-// the device reports (tr_apply fails) instead of returning wrong values.
+// covers subnormal partial sums.  The tile starts may also come from terms
+// that differ from the fold's own x_i by at most 1500 u relative each (the
+// filter's tile sums from the move's log-sum-exp partials); the 512 extra
+// units below (2048 u) cover that on their own.  Too large a bound only costs
+// more specials; too small breaks exactness, which the device reports
+// (tr_apply fails) instead of returning wrong values.
 VSMCX_HD void fold_eps(uint64_t n, double *eps_rel, double *eps_abs)
 {
-    const double nn = (double)n + 64.0;
+    const double nn = (double)n + 576.0;
     *eps_rel = nn * 0x1p-51;
     *eps_abs = nn * 0x1p-1070;
 }

@@ -291,13 +291,16 @@ __device__ uint64_t wave_first_greater(const Inv &inv, double c, uint64_t lo, ui
 // x of one tile into LDS (striped, coalesced); zero past the end, which
 // classifies as negligible.
 template <typename XSrc>
-__device__ __forceinline__ void load_tile(const XSrc &xs, uint64_t base, uint64_t M, double *s_x)
+__device__ __forceinline__ void load_tile(const XSrc &xs, uint64_t base, uint64_t M, double *s_x,
+                                          double *wx_out = nullptr)
 {
 #pragma unroll
     for (int k = 0; k < kFI; ++k) {
         const int e = k * kBlock + threadIdx.x;
         const uint64_t i = base + e;
-        s_x[fidx(e)] = i < M ? xs.x(i) : 0.0;
+        const double v = i < M ? xs.x(i) : 0.0;
+        s_x[fidx(e)] = v;
+        if (wx_out && i < M) wx_out[i] = v;
     }
 }
 
@@ -321,9 +324,8 @@ struct NoVisit {
     __device__ void operator()(uint64_t, const T &) const {}
 };
 
-template <int BLOCK, typename T, typename Op, typename Visit = NoVisit>
-__device__ T wg_scan_b(const T *in, T *out, uint64_t n, Op op, T ident, T *lds,
-                       Visit visit = Visit())
+template <int BLOCK, typename T, typename Op, typename Visit = NoVisit, typename In = const T *>
+__device__ T wg_scan_b(In in, T *out, uint64_t n, Op op, T ident, T *lds, Visit visit = Visit())
 {
     T carry = ident;
     for (uint64_t c0 = 0; c0 < n; c0 += (uint64_t)BLOCK * kScanItems) {
@@ -349,12 +351,27 @@ __device__ T wg_scan_b(const T *in, T *out, uint64_t n, Op op, T ident, T *lds,
     return carry;
 }
 
-template <typename T, typename Op, typename Visit = NoVisit>
-__device__ T wg_scan(const T *in, T *out, uint64_t n, Op op, T ident, T *lds,
-                     Visit visit = Visit())
+template <typename T, typename Op, typename Visit = NoVisit, typename In = const T *>
+__device__ T wg_scan(In in, T *out, uint64_t n, Op op, T ident, T *lds, Visit visit = Visit())
 {
-    return wg_scan_b<kScanBlock>(in, out, n, op, ident, lds, visit);
+    return wg_scan_b<kScanBlock, T, Op, Visit, In>(in, out, n, op, ident, lds, visit);
 }
+
+// Tile sums of the normalised weights x_i = exp(lw_i - m) coeff taken from the
+// move's tile partials {max_t, sum_t exp(lw_i - max_t), ...} (5 doubles per
+// tile, the move and fold tiles coincide): sum_t exp(max_t - m) coeff.  Each
+// term differs from the x_i the fold adds by the roundings of the exp
+// arguments, at most 2 u |lw_i - m| <= 1490 u relative for a term that does
+// not underflow, plus a few u for the exps and products; fold_eps covers it.
+struct TileSumsFromLse {
+    const double *part;
+    double m, coeff;
+    __device__ double operator[](uint64_t t) const
+    {
+        const double mt = part[5 * t];
+        return mt == -INFINITY ? 0.0 : part[5 * t + 1] * exp(mt - m) * coeff;
+    }
+};
 
 // ---- tile sum of x (any fixed order) -----------------------------------------
 template <typename XSrc>
@@ -391,13 +408,13 @@ struct FoldLdsT {
 template <typename XSrc, typename Lds>
 __device__ void fold_tile_seg(const XSrc &xs, uint64_t base, uint64_t M, double ts,
                               SpecRec *spec_tile, Seg *agg_out, Lds &L, Seg *thr_out = nullptr,
-                              int32_t *fb_out = nullptr)
+                              int32_t *fb_out = nullptr, double *wx_out = nullptr)
 {
     double *const s_x = L.x;
     __syncthreads();  // the LDS of a previous tile is no longer read
     const int tid = threadIdx.x;
     const int e0 = tid * kFI;
-    load_tile(xs, base, M, s_x);
+    load_tile(xs, base, M, s_x, wx_out);
     __syncthreads();
     const double s_thr =
         ts + block_excl_scan<kBlock>(thread_sum(s_x, e0), AddOp(), 0.0, L.red, (double *)nullptr);

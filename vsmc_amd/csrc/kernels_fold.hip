@@ -35,17 +35,34 @@ __global__ __launch_bounds__(kScanBlock) void k_fold_scan_sum(FilterCtl *ctl, Fo
     wg_scan<double>(fs.tsum, fs.ts, fs.tiles, AddOp(), 0.0, s_red);
 }
 
+// ---- pass 2 from the move's partials: tile starts of the filter's weights -------
+// The filter's move leaves each tile's {max, sum exp(lw - max)}; the tile sums
+// of x_i = exp(lw_i - m) coeff follow from them (TileSumsFromLse), so the
+// tile-sum pass is not needed.
+__global__ __launch_bounds__(kScanBlock) void k_fold_scan_sum_lse(FilterCtl *ctl,
+                                                                  const double *lse_part,
+                                                                  FoldScratch fs)
+{
+    if (!ctl->resample) return;
+    __shared__ double s_red[kScanBlock / kWave];
+    const TileSumsFromLse in{lse_part, ctl->m, ctl->coeff};
+    wg_scan<double>(in, fs.ts, fs.tiles, AddOp(), 0.0, s_red);
+}
+
 // ---- pass 3: classification, tile segment summaries, special records -----------
+// wx_out: where the pass leaves the x values for the apply pass (when no
+// tile-sum pass wrote them).
 template <typename XSrc>
 __global__ __launch_bounds__(kBlock) void k_fold_seg(FilterCtl *ctl, XSrc xs, uint64_t M,
-                                                     FoldScratch fs)
+                                                     FoldScratch fs, double *wx_out)
 {
     if (!ctl->resample) return;
     __shared__ FoldLdsT<false> L;
     xs.bind(ctl);
     const uint64_t t = blockIdx.x;
     fold_tile_seg(xs, t * (uint64_t)kFT, M, fs.ts[t], fs.spec + t * (uint64_t)kFT, fs.agg + t, L,
-                  fs.thr ? fs.thr + t * kBlock : nullptr, fs.thr ? fs.thr_fb + t * kBlock : nullptr);
+                  fs.thr ? fs.thr + t * kBlock : nullptr, fs.thr ? fs.thr_fb + t * kBlock : nullptr,
+                  wx_out);
 }
 
 // ---- pass 4: tile segment prefixes and the exact values at the specials --------
@@ -294,24 +311,44 @@ __global__ __launch_bounds__(kBlock) void k_sorted_values(FilterCtl *ctl, const 
 // Passes 1-4 give the fold total and the values at the specials; pass 5 runs
 // when anything per element is wanted (inversion or fold values); passes 6-7
 // when the ancestor layout is.
+// lse_part (XWeight sources only): the move's tile partials of these
+// log-weights, from which the tile starts follow without a tile-sum pass.
 template <typename XSrc, typename Inv>
 static void launch_fold(FilterCtl *ctl, XSrc xs, Inv inv, uint64_t M, FoldScratch &fs,
-                        const FoldOut &out, hipStream_t st)
+                        const FoldOut &out, hipStream_t st, const double *lse_part = nullptr)
 {
     fs.tiles = (M + kFT - 1) / kFT;
     const dim3 grid((unsigned)fs.tiles);
+    if (lse_part && fs.wx) {
+        hipLaunchKernelGGL(k_fold_scan_sum_lse, dim3(1), dim3(kScanBlock), 0, st, ctl, lse_part,
+                           fs);
+        // the segment pass computes x and leaves it for the apply pass, which
+        // reads x more than once per element
+        hipLaunchKernelGGL((k_fold_seg<XSrc>), grid, dim3(kBlock), 0, st, ctl, xs, M, fs, fs.wx);
+        hipLaunchKernelGGL(k_fold_scan_seg, dim3(1), dim3(kScanBlock), 0, st, ctl, fs);
+        if (!Inv::kInv && !out.a_out) return;
+        const XCached<XSrc> xc{fs.wx, xs};
+        hipLaunchKernelGGL((k_fold_apply<XCached<XSrc>, Inv>), grid, dim3(kBlock), 0, st, ctl, xc,
+                           inv, M, fs, out);
+        if (!Inv::kInv || !out.anc) return;
+        hipLaunchKernelGGL(k_fold_scan_cnt, dim3(1), dim3(kScanBlock), 0, st, ctl, fs);
+        hipLaunchKernelGGL(k_fold_layout, grid, dim3(kBlock), 0, st, ctl, fs, out, M);
+        return;
+    }
     hipLaunchKernelGGL((k_fold_tsum<XSrc>), grid, dim3(kBlock), 0, st, ctl, xs, M, fs);
     hipLaunchKernelGGL(k_fold_scan_sum, dim3(1), dim3(kScanBlock), 0, st, ctl, fs);
     if (fs.wx) {
         // later passes read the values the tile-sum pass wrote
         const XCached<XSrc> xc{fs.wx, xs};
-        hipLaunchKernelGGL((k_fold_seg<XCached<XSrc>>), grid, dim3(kBlock), 0, st, ctl, xc, M, fs);
+        hipLaunchKernelGGL((k_fold_seg<XCached<XSrc>>), grid, dim3(kBlock), 0, st, ctl, xc, M, fs,
+                           (double *)nullptr);
         hipLaunchKernelGGL(k_fold_scan_seg, dim3(1), dim3(kScanBlock), 0, st, ctl, fs);
         if (!Inv::kInv && !out.a_out) return;
         hipLaunchKernelGGL((k_fold_apply<XCached<XSrc>, Inv>), grid, dim3(kBlock), 0, st, ctl, xc,
                            inv, M, fs, out);
     } else {
-        hipLaunchKernelGGL((k_fold_seg<XSrc>), grid, dim3(kBlock), 0, st, ctl, xs, M, fs);
+        hipLaunchKernelGGL((k_fold_seg<XSrc>), grid, dim3(kBlock), 0, st, ctl, xs, M, fs,
+                           (double *)nullptr);
         hipLaunchKernelGGL(k_fold_scan_seg, dim3(1), dim3(kScanBlock), 0, st, ctl, fs);
         if (!Inv::kInv && !out.a_out) return;
         hipLaunchKernelGGL((k_fold_apply<XSrc, Inv>), grid, dim3(kBlock), 0, st, ctl, xs, inv, M,
@@ -324,7 +361,7 @@ static void launch_fold(FilterCtl *ctl, XSrc xs, Inv inv, uint64_t M, FoldScratc
 
 template <typename W>
 static void resample_pipeline(FilterCtl *ctl, W w, uint64_t M, int scheme, ResampleBuffers &rb,
-                              StepRow *hist, hipStream_t st)
+                              StepRow *hist, hipStream_t st, const double *lse_part = nullptr)
 {
     FoldOut out{rb.anc, rb.zl, rb.sl, rb.rep_out, rb.a_out};
     const bool residual = scheme == kResidual || scheme == kResidualStratified ||
@@ -362,11 +399,11 @@ static void resample_pipeline(FilterCtl *ctl, W w, uint64_t M, int scheme, Resam
             launch_fold(ctl, xs, InvSystematic(), M, rb.fs, out, st);
     } else {
         if (scheme == kMultinomial)
-            launch_fold(ctl, w, ia, M, rb.fs, out, st);
+            launch_fold(ctl, w, ia, M, rb.fs, out, st, lse_part);
         else if (scheme == kStratified)
-            launch_fold(ctl, w, InvStratified(), M, rb.fs, out, st);
+            launch_fold(ctl, w, InvStratified(), M, rb.fs, out, st, lse_part);
         else
-            launch_fold(ctl, w, InvSystematic(), M, rb.fs, out, st);
+            launch_fold(ctl, w, InvSystematic(), M, rb.fs, out, st, lse_part);
     }
     if (rb.anc) {
         const unsigned zg = (unsigned)((M + kTile - 1) / kTile) * 4;
@@ -378,9 +415,10 @@ static void resample_pipeline(FilterCtl *ctl, W w, uint64_t M, int scheme, Resam
 }
 
 void launch_resample_from_log_weights(FilterCtl *ctl, const double *lw, uint64_t M, int scheme,
-                                      ResampleBuffers &rb, StepRow *hist, hipStream_t st)
+                                      ResampleBuffers &rb, StepRow *hist, hipStream_t st,
+                                      const double *lse_part)
 {
-    resample_pipeline(ctl, XWeight{lw, 0.0, 0.0}, M, scheme, rb, hist, st);
+    resample_pipeline(ctl, XWeight{lw, 0.0, 0.0}, M, scheme, rb, hist, st, lse_part);
 }
 
 void launch_resample_from_weights(FilterCtl *ctl, const double *w, uint64_t M, int scheme,

@@ -25,8 +25,11 @@ struct ResampleBuffers {
 
 
 // kernels_fold.hip
+// lse_part: the move's tile partials of lw ({max, sum exp(lw - max), ...},
+// 5 doubles per 2048-particle tile), or nullptr when lw has none
 void launch_resample_from_log_weights(FilterCtl *ctl, const double *lw, uint64_t M, int scheme,
-                                      ResampleBuffers &rb, StepRow *hist, hipStream_t st);
+                                      ResampleBuffers &rb, StepRow *hist, hipStream_t st,
+                                      const double *lse_part = nullptr);
 void launch_resample_from_weights(FilterCtl *ctl, const double *w, uint64_t M, int scheme,
                                   ResampleBuffers &rb, StepRow *hist, hipStream_t st);
 void launch_fold_values(FilterCtl *ctl, const double *x, uint64_t M, ResampleBuffers &rb,
