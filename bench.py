@@ -39,6 +39,29 @@ def move_bytes(prec, prev_resampled):
     return 2 * sb + 8 + (4 if prev_resampled else 8)
 
 
+def step_bytes(dim, hist, first=0):
+    """Minimal algorithmic HBM bytes per particle of the steps first.. of one
+    filter's history (FP32 state of dim values): the state write (4 dim) and,
+    after step 0, its read (4 dim); the log-weight write (8); after step 0 the
+    ancestor index (4) when the previous step resampled, else the old
+    log-weight (8); on a resampling step one weight read (8) and one ancestor
+    write (4).  The extra passes of the exact fold are not counted."""
+    res = hist["resampled"].astype(bool)
+    t = np.arange(len(res))
+    prev = np.concatenate([[False], res[:-1]])
+    b = 4 * dim + 8 + np.where(t > 0, 4 * dim + np.where(prev, 4, 8), 0) + np.where(res, 12, 0)
+    return float(b[first:].sum())
+
+
+def whole_step_roofline(total_bytes, seconds, scope):
+    """HBM roofline of a whole step loop: minimal algorithmic bytes of every
+    timed particle-update over the timed interval."""
+    a = total_bytes / seconds / 1e9
+    return {"bound": "hbm", "achieved": round(a, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(a / HBM_PEAK_GBS, 4), "traffic": None, "algorithmic_bytes": total_bytes,
+            "scope": scope}
+
+
 def run_precision(ctx, prec, N, W, K, obs, schemes, profile=True, concurrent=True):
     """The six filters of config 1, each advanced one step per bench step.
     concurrent: every filter has its own context (its own HIP stream, default
@@ -171,7 +194,9 @@ def run_batched(nfilters, seed, tune=True):
     done = np.array([b.steps(f) for f in range(nfilters)], np.uint64)
     ms = b.run()
     updates = float(np.sum(ns * (Ts - done)))
-    rfrac = float(np.mean([b.history(f)["resampled"].mean() for f in range(0, nfilters, 64)]))
+    hists = [b.history(f) for f in range(nfilters)]
+    rfrac = float(np.mean([h["resampled"].mean() for h in hists]))
+    nbytes = sum(float(ns[f]) * step_bytes(4, hists[f], int(done[f])) for f in range(nfilters))
     b.close()
     ctx.close()
     return {"value": updates / (ms * 1e-3), "unit": "particle-updates/s", "ms": ms,
@@ -179,7 +204,11 @@ def run_batched(nfilters, seed, tune=True):
             "launch": {"strategy": "ticks" if strategy == V.BATCH_TICKS else "persistent",
                        "workgroups": shape[0], "order": "most work first" if shape[1] == 0
                        else "as given", "tuned": tune},
-            "resample_frac_sampled": rfrac,
+            "resample_frac": rfrac,
+            "roofline": whole_step_roofline(
+                nbytes, ms * 1e-3, "all kernels of every timed step: minimal algorithmic bytes "
+                "(state read/write, log-weight write, ancestor or log-weight read; on "
+                "resampling steps one weight read and one ancestor write) over the timed run"),
             "config": "config-3: CV filters, N log-uniform [2^8, 2^16], T uniform [50, 500], "
                       "mixed precision, systematic, ESS<N/2; the tuner picks one persistent "
                       "launch or one tick per step; the timed run covers every remaining step "
@@ -239,6 +268,10 @@ def run_serving(seconds, rate, seed, budget_gb=200.0, max_jobs=8192):
             time.sleep(max(0.0, min(1e-3, t_arr[nxt] - (time.perf_counter() - t0))))
     wall = time.perf_counter() - t0
     st = s.stats()
+    nbytes = 0.0
+    for j, q in ids.items():
+        dim = 10 if models[q] == V.MODEL_LG else 4
+        nbytes += float(ns[q]) * step_bytes(dim, s.history(j, int(Ts[q])))
     s.close()
     ctx.close()
     return {"value": st["particle_updates"] / (st["elapsed_ms"] * 1e-3), "unit": "particle-updates/s",
@@ -247,6 +280,11 @@ def run_serving(seconds, rate, seed, budget_gb=200.0, max_jobs=8192):
             "device_busy_frac": st["busy_ms"] / st["elapsed_ms"] if st["elapsed_ms"] else None,
             "peak_particles": st["peak_particles"], "capacity_particles": st["capacity_particles"],
             "wall_s": wall,
+            "roofline": whole_step_roofline(
+                nbytes, st["elapsed_ms"] * 1e-3, "all kernels of every tick: minimal algorithmic "
+                "bytes of every job step (FP32 state of 4 or 10 values read/written, log-weight "
+                "write, ancestor or log-weight read; on resampling steps one weight read and one "
+                "ancestor write) over the serving interval"),
             "config": f"config-4: Poisson arrivals {rate}/s for {seconds}s, 50/50 CV and 10-D "
                       f"linear-Gaussian (A spectral radius 0.9, q 0.1, r 0.2, s0 0.25), N "
                       f"log-uniform [2^10, 2^20], T uniform [20, 1000], {budget_gb:g} GB budget, "
