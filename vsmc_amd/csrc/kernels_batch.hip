@@ -140,28 +140,22 @@ __device__ __noinline__ void batch_move_lg(const BatchFilter &F, uint32_t f, con
     batch_move_end(ctl, P, f, cur, P.hist + F.hist_off, t, part, T, S);
 }
 
-// tile sums and their exclusive scan (approximate tile starts)
+// approximate tile starts: the exclusive scan of the tile sums, which follow
+// from the move's tile partials (TileSumsFromLse)
 template <typename Inv>
 __device__ __noinline__ void batch_fold_sums(const BatchFilter &F, FilterCtl *ctl,
                                              const BatchPool &P, BatchLds<Inv> &S)
 {
     const int tid = threadIdx.x;
     const uint64_t N = F.n, T = (N + kFT - 1) / kFT;
-    XWeight xs{P.lw + F.off, 0.0, 0.0};
-    xs.bind(ctl);
+    const TileSumsFromLse in{P.tpart + 5 * (F.off / kFT), ctl->m, ctl->coeff};
     double *ts = P.ts + F.off / kFT;
-#pragma unroll 1
-    for (uint64_t tt = 0; tt < T; ++tt) {
-        const double s = fold_tile_sum(xs, tt * (uint64_t)kFT, N, S.L.red);
-        if (tid == 0) ts[tt] = s;
-    }
-    __syncthreads();
     // exclusive scan over the tiles, kBlock at a time, in a fixed order
     double carry = 0.0;
 #pragma unroll 1
     for (uint64_t c0 = 0; c0 < T; c0 += kBlock) {
         const uint64_t i = c0 + tid;
-        const double a = i < T ? ts[i] : 0.0;
+        const double a = i < T ? in[i] : 0.0;
         double tot;
         const double ex = block_excl_scan<kBlock>(a, AddOp(), 0.0, S.L.red, &tot);
         if (i < T) ts[i] = carry + ex;

@@ -69,30 +69,20 @@ __global__ __launch_bounds__(kWave) void k_tick_reduce(const uint32_t *jobs,
 }
 
 // ---- fold passes -------------------------------------------------------------------
-__global__ __launch_bounds__(kBlock) void k_tick_tsum(const TickItem *items,
-                                                      const BatchFilter *filters, BatchPool P)
-{
-    const TickItem it = items[blockIdx.x];
-    FilterCtl *ctl = P.ctl + it.slot;
-    if (!ctl->resample) return;
-    __shared__ double s_red[kBlock / kWave];
-    const BatchFilter F = filters[it.slot];
-    XWeight xs{P.lw + F.off, 0.0, 0.0};
-    xs.bind(ctl);
-    const double s = fold_tile_sum(xs, it.tile * (uint64_t)kFT, F.n, s_red, P.wx + F.off);
-    if (threadIdx.x == 0) P.tsum[F.off / kFT + it.tile] = s;
-}
-
+// The tile starts come from the move's tile partials (TileSumsFromLse), as in
+// the single filter: no tile-sum pass.
 __global__ __launch_bounds__(kJobBlock) void k_tick_scan_sum(const uint32_t *jobs,
                                                              const BatchFilter *filters,
                                                              BatchPool P)
 {
     const uint32_t slot = jobs[blockIdx.x];
-    if (!P.ctl[slot].resample) return;
+    const FilterCtl *ctl = P.ctl + slot;
+    if (!ctl->resample) return;
     __shared__ double s_red[kJobBlock / kWave];
     const BatchFilter F = filters[slot];
     const uint64_t o = F.off / kFT;
-    wg_scan_b<kJobBlock>(P.tsum + o, P.ts + o, tiles_of_job(F), AddOp(), 0.0, s_red);
+    const TileSumsFromLse in{P.tpart + 5 * o, ctl->m, ctl->coeff};
+    wg_scan_b<kJobBlock>(in, P.ts + o, tiles_of_job(F), AddOp(), 0.0, s_red);
 }
 
 __global__ __launch_bounds__(kBlock) void k_tick_seg(const TickItem *items,
@@ -103,11 +93,12 @@ __global__ __launch_bounds__(kBlock) void k_tick_seg(const TickItem *items,
     if (!ctl->resample) return;
     __shared__ FoldLdsT<false> L;
     const BatchFilter F = filters[it.slot];
-    // the weights the tile-sum pass wrote
-    const XCached<XWeight> xs{P.wx + F.off, XWeight{P.lw + F.off, 0.0, 0.0}};
+    // x from the log-weights, left in P.wx for the apply pass
+    XWeight xs{P.lw + F.off, 0.0, 0.0};
+    xs.bind(ctl);
     const uint64_t o = F.off / kFT + it.tile;
     fold_tile_seg(xs, it.tile * (uint64_t)kFT, F.n, P.ts[o], P.spec + F.off + it.tile * (uint64_t)kFT,
-                  P.tagg + o, L, P.thr + o * kBlock, P.thr_fb + o * kBlock);
+                  P.tagg + o, L, P.thr + o * kBlock, P.thr_fb + o * kBlock, P.wx + F.off);
 }
 
 __global__ __launch_bounds__(kJobBlock) void k_tick_scan_seg(const uint32_t *jobs,
@@ -307,7 +298,6 @@ void launch_tick(bool fp64, int scheme, const TickItem *items, uint32_t nitems,
     else
         hipLaunchKernelGGL(k_tick_move<float>, gi, b, 0, st, items, filters, P, cv, lg);
     hipLaunchKernelGGL(k_tick_reduce, gj, bw, 0, st, jobs, filters, P, scheme);
-    hipLaunchKernelGGL(k_tick_tsum, gi, b, 0, st, items, filters, P);
     hipLaunchKernelGGL(k_tick_scan_sum, gj, bs, 0, st, jobs, filters, P);
     hipLaunchKernelGGL(k_tick_seg, gi, b, 0, st, items, filters, P);
     hipLaunchKernelGGL(k_tick_scan_seg, gj, bs, 0, st, jobs, filters, P);
