@@ -413,7 +413,8 @@ def main():
     W, K, N = max(1, a.warmup), a.steps, a.n
     schemes = list(range(6)) if a.schemes == "all" else [SCHEME_NAMES.index(s) for s in
                                                          a.schemes.split(",")]
-    obs = V.cv_simulate(W + K, 11)
+    ks = min(K, 8)  # steps of the short serial run that gives the move kernel's roofline
+    obs = V.cv_simulate(max(W + K, 2 + ks), 11)
     ctx = V.Context(0)
     res = {}
     precs = [V.MIXED] + ([] if a.no_fp64 else [V.FP64_FUSED, V.FP64_PLAIN])
@@ -421,11 +422,10 @@ def main():
     for prec in precs:
         print(f"[bench] precision {prec}: {len(schemes)} filters x N={N}, W={W} K={K}",
               file=sys.stderr, flush=True)
-        res[prec] = run_precision(ctx, prec, N, W, K, obs, schemes, profile=False,
+        res[prec] = run_precision(ctx, prec, N, W, K, obs[:W + K], schemes, profile=False,
                                   concurrent=not a.serial_filters)
         # the move kernel's roofline from a short run with the filters in turn
         # (no other filter's kernels beside it)
-        ks = min(K, 8)
         serial[prec] = run_precision(ctx, prec, N, 2, ks, obs[:2 + ks], schemes, profile=True,
                                      concurrent=False)
     ctx.close()
