@@ -19,6 +19,12 @@
 //                                 is StateMatrix::copy
 //                                 (include/vsmc/core/state_matrix.hpp:285-300)
 //                                 as a device gather
+//   vsmcx::PfSampler              a Sampler<T>-shaped driver of the device
+//                                 filter: initialize(data file), iterate(n),
+//                                 Sampler's histories and its print table
+//                                 (include/vsmc/core/sampler.hpp:593-721), as
+//                                 the pf example saves it and example/pf/pf.R
+//                                 reads it
 //   vsmcx::CvMoveHIP              the pf example's init and move
 //                                 (example/pf/include/pf_smp.hpp:150-238) over a
 //                                 StateHIP<4, FP>, assignable to Sampler's
@@ -35,6 +41,9 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <fstream>
+#include <memory>
+#include <ostream>
 #include <sstream>
 #include <stdexcept>
 #include <string>
@@ -333,6 +342,138 @@ class Filter {
   private:
     vsmcx_filter *h_ = nullptr;
 };
+
+// A Sampler<T>-shaped driver of the device-resident pf filter
+// (include/vsmc/core/sampler.hpp).  initialize() is Sampler::initialize(param)
+// with the pf example's param, the observation file of (x, y) pairs
+// (example/pf/include/pf_smp.hpp:113-125, 175-176): the init step, its
+// resample and the monitor (do_init, core/sampler.hpp:773-781).  iterate(n)
+// is n further steps (do_iter, 783-792).  The histories have Sampler's
+// meanings (do_resample, 816-822): size and ESS before the resample, whether
+// it resampled, the accept counts of the init and of the move (the pf init and
+// move return 1 per particle, so N), and the "pos" monitor (pos.x, pos.y)
+// after the resample (MonitorMCMC, the pf example's cv_est).  print() and
+// operator<< write Sampler::print's table (core/sampler.hpp:679-721):
+//   Size Resampled Accept.0 ESS pos.x pos.y
+// every field followed by the separator, each row ended by '\n', numbers in
+// the stream's own format -- the file the pf example writes
+// (example/pf/include/pf_mpi_do.hpp:58-64) and example/pf/pf.R:53-111 reads.
+class PfSampler {
+  public:
+    // Sampler<T>(N, scheme, threshold) (core/sampler.hpp:101-104); the model
+    // constants, seeds and precision come from base (its n, scheme and
+    // threshold are replaced)
+    PfSampler(Context &ctx, std::size_t N, ResampleScheme scheme, double threshold = 0.5,
+              const FilterConfig &base = FilterConfig())
+        : ctx_(&ctx), cfg_(base)
+    {
+        cfg_.n = N;
+        cfg_.scheme = scheme;
+        cfg_.threshold = threshold;
+    }
+
+    // Sampler::initialize with the pf data file: one (x, y) pair per line,
+    // read to the end of the file; the filter's horizon is the number of pairs
+    PfSampler &initialize(const char *file)
+    {
+        std::ifstream in(file);
+        if (!in) throw Error(VSMCX_ERR_ARG, std::string("cannot open ") + file);
+        std::vector<double> xy;
+        double x, y;
+        while (in >> x >> y) {
+            xy.push_back(x);
+            xy.push_back(y);
+        }
+        return initialize(xy);
+    }
+    // the same from T rows of (x, y)
+    PfSampler &initialize(const std::vector<double> &xy)
+    {
+        if (xy.size() < 2) throw Error(VSMCX_ERR_ARG, "no observations");
+        cfg_.horizon = xy.size() / 2;
+        filter_.reset();  // Sampler::do_reset: a new run
+        filter_.reset(new Filter(*ctx_, cfg_));
+        filter_->observations(xy);
+        filter_->iterate(1);
+        refresh();
+        return *this;
+    }
+    PfSampler &iterate(std::size_t num = 1)
+    {
+        if (!filter_) throw Error(VSMCX_ERR_STATE, "iterate before initialize");
+        if (num) filter_->iterate(num);
+        refresh();
+        return *this;
+    }
+
+    std::size_t size() const { return (std::size_t)cfg_.n; }
+    std::size_t iter_size() const { return hist_.size(); }
+    std::size_t size_history(std::size_t iter) const
+    {
+        (void)at(iter);
+        return size();
+    }
+    double ess_history(std::size_t iter) const { return at(iter).ess; }
+    bool resampled_history(std::size_t iter) const { return at(iter).resampled != 0; }
+    std::size_t accept_history(std::size_t id, std::size_t iter) const
+    {
+        if (id != 0) throw Error(VSMCX_ERR_ARG, "the pf sampler has one move");
+        (void)at(iter);
+        return size();
+    }
+    // the "pos" monitor, i = 0 (pos.x) or 1 (pos.y)
+    double monitor_record(std::size_t i, std::size_t iter) const
+    {
+        return i == 0 ? at(iter).mean_x : at(iter).mean_y;
+    }
+    // running log-evidence after iter (not part of Sampler's table)
+    double log_z_history(std::size_t iter) const { return at(iter).log_z; }
+    Filter &filter() { return *filter_; }
+
+    template <typename CharT, typename Traits>
+    std::basic_ostream<CharT, Traits> &print(std::basic_ostream<CharT, Traits> &os,
+                                             char sepchar = '\t') const
+    {
+        if (iter_size() == 0 || !os.good()) return os;
+        os << "Size" << sepchar << "Resampled" << sepchar << "Accept.0" << sepchar;
+        os << "ESS" << sepchar << "pos.x" << sepchar << "pos.y" << sepchar << '\n';
+        for (std::size_t iter = 0; iter != iter_size(); ++iter) {
+            os << size_history(iter) << sepchar << (std::size_t)resampled_history(iter) << sepchar
+               << accept_history(0, iter) << sepchar;
+            os << ess_history(iter) << sepchar << monitor_record(0, iter) << sepchar
+               << monitor_record(1, iter) << sepchar;
+            os << '\n';
+        }
+        return os;
+    }
+
+  private:
+    Context *ctx_;
+    FilterConfig cfg_;
+    std::unique_ptr<Filter> filter_;
+    std::vector<vsmcx_step_stats> hist_;
+
+    // the last step's monitor is evaluated by the next step's move; finish()
+    // evaluates it now (the next move recomputes it)
+    void refresh()
+    {
+        filter_->finish();
+        hist_ = filter_->history();
+    }
+    const vsmcx_step_stats &at(std::size_t iter) const
+    {
+        if (iter >= hist_.size()) throw Error(VSMCX_ERR_ARG, "iteration out of range");
+        return hist_[iter];
+    }
+};
+
+// Sampler's operator<< (core/sampler.hpp:950-953)
+template <typename CharT, typename Traits>
+inline std::basic_ostream<CharT, Traits> &operator<<(std::basic_ostream<CharT, Traits> &os,
+                                                      const PfSampler &sampler)
+{
+    return sampler.print(os);
+}
 
 // The pf example's init and move over a StateHIP<4, FP>: the prior draw (as
 // init_type, size_t(Particle<T> &, void *)) or the transition at iteration
