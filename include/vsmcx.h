@@ -330,6 +330,43 @@ int vsmcx_state_copy(vsmcx_state *s, uint64_t n, const uint64_t *copy_from);
 int vsmcx_state_cv_move(vsmcx_state *s, const vsmcx_filter_desc *model, uint64_t iter,
                         const double *obs_xy, double *inc);
 
+/* ---- device weight set: moves written as user kernels ------------------------------
+ * vSMC's WeightSet (include/vsmc/core/weight_set.hpp:61-470) with its log-weights in
+ * HBM, updated from device arrays -- the increments a user's own HIP kernel writes, as
+ * MoveCL kernels do (include/vsmc/opencl/backend_cl.hpp:689-708; such kernels draw
+ * randoms with include/vsmcx_device.hpp, the analogue of rng/urng.h and
+ * rng/normal01.h) -- and Particle::resample (include/vsmc/core/particle.hpp:230-250)
+ * on the device: the ESS test at `threshold`, the scheme's resampling by the exact fold
+ * (vSMC's inversion and cfrp_trans), and the gather of a vsmcx_state in place.  The
+ * Threefry4x64 resample engine is seeded with resample_seed.  set / add / resample
+ * enqueue on the context's stream (vsmcx_ctx_stream) and return without waiting; a
+ * user kernel launched on that stream is ordered with them.  Every set or add is one
+ * history row (at most max_steps): ESS before resampling, log-evidence (log sum_i W_i
+ * exp(inc_i) accumulated), resampled, engine draws. */
+typedef struct vsmcx_weights vsmcx_weights;
+
+/* the context's HIP stream (a hipStream_t) */
+int vsmcx_ctx_stream(vsmcx_ctx *ctx, void **hip_stream);
+int vsmcx_weights_create(vsmcx_ctx *ctx, uint64_t n, int scheme, double threshold,
+                         uint64_t resample_seed, uint64_t max_steps, vsmcx_weights **out);
+int vsmcx_weights_destroy(vsmcx_weights *w);
+/* a device buffer of n doubles owned by w, for a user kernel's increments */
+int vsmcx_weights_buffer(vsmcx_weights *w, double **inc_dev);
+/* set_log_weight (weight_set.hpp:296-307): lw = inc; a new run, history row 0 */
+int vsmcx_weights_set_log(vsmcx_weights *w, const double *inc_dev);
+/* add_log_weight (weight_set.hpp:324-331): lw += inc (lw = inc after a resample) */
+int vsmcx_weights_add_log(vsmcx_weights *w, const double *inc_dev);
+/* Particle::resample(op, threshold): if the last set / add left ESS < threshold * n,
+ * resample and gather state's rows in place (state may be NULL); else nothing */
+int vsmcx_weights_resample(vsmcx_weights *w, vsmcx_state *state);
+/* the ancestors of the last resample on the device: n uint32 (cfrp_trans copy_from) */
+int vsmcx_weights_ancestors(vsmcx_weights *w, const uint32_t **anc_dev);
+int vsmcx_weights_steps(vsmcx_weights *w, uint64_t *steps);
+/* rows 0 .. count-1; waits for the device and checks its error bits */
+int vsmcx_weights_history(vsmcx_weights *w, uint64_t count, vsmcx_step_stats *out);
+/* read_weight (weight_set.hpp:197-201): the n normalised weights */
+int vsmcx_weights_read(vsmcx_weights *w, double *weight);
+
 /* ---- Gaussian-mixture SMC with adaptive tempering ------------------------------
  * The reference's gmm_smc sampler (example/gmm/src/gmm_smc.cpp:43-52 driven by
  * example/include/smc_do.hpp:57-66 under the "cess" schedule): particles carry

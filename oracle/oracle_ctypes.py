@@ -124,6 +124,10 @@ class Oracle:
                                        f64p, C.c_double, C.c_double, f64p, f64p]
         L.orc_cv_llh_many.argtypes = [C.POINTER(CvParam), C.c_size_t, f64p, C.c_double,
                                       C.c_double, f64p]
+        L.orc_cburng4x32_stream.argtypes = [C.c_uint32, C.c_size_t, C.c_void_p]
+        L.orc_normal01_4x32_53_stream.argtypes = [C.c_uint32, C.c_size_t, f64p]
+        L.orc_clpf_run.argtypes = [C.c_size_t, C.c_size_t, f64p, C.c_uint32, C.c_uint64, C.c_int,
+                                   C.c_double, f64p, C.c_void_p]
         L.orc_cv_run.argtypes = [C.POINTER(CvParam), C.c_size_t, C.c_size_t, f64p, C.c_uint64,
                                  C.c_uint64, C.c_int, C.c_double, C.c_void_p, C.c_void_p,
                                  C.c_void_p, C.c_int]
@@ -287,6 +291,30 @@ class Oracle:
         self.lib.orc_cv_llh_many(C.byref(p), st.shape[0], st, ox, oy, inc)
         return inc
 
+    def cburng4x32_stream(self, seed, n):
+        """n draws of urng.h's cburng4x32 (Threefry4x32-20) seeded with seed."""
+        out = np.zeros(n, np.uint32)
+        self.lib.orc_cburng4x32_stream(seed, n, out.ctypes.data)
+        return out
+
+    def normal01_4x32_53_stream(self, seed, n):
+        """n draws of normal01.h's normal01_4x32_53 over cburng4x32(seed)."""
+        out = np.zeros(n)
+        self.lib.orc_normal01_4x32_53_stream(seed, n, out)
+        return out
+
+    def clpf_run(self, N, obs, seed, resample_seed, scheme=SYSTEMATIC, threshold=0.5):
+        """The OpenCL pf example's kernels (pf_cl.cl) on vSMC's weight set and
+        resampler: final states (N x 4) and per-step records."""
+        obs = np.ascontiguousarray(obs, np.float64).reshape(-1, 2)
+        T = obs.shape[0]
+        st = np.zeros((N, 4))
+        rec = np.zeros(T, STEP_DTYPE)
+        if self.lib.orc_clpf_run(N, T, obs, seed, resample_seed, scheme, threshold, st,
+                                 rec.ctypes.data) != 0:
+            raise MemoryError("orc_clpf_run failed")
+        return st, rec
+
     def cv_run(self, N, obs, seed, resample_seed, scheme=SYSTEMATIC, threshold=0.5,
                param=None, threads=0, want_state=True):
         obs = np.ascontiguousarray(obs, np.float64).reshape(-1, 2)
@@ -391,6 +419,23 @@ class RefHarness:
         L.ref_cv_run.argtypes = [C.POINTER(CvParam), C.c_size_t, C.c_size_t, f64p, C.c_uint64,
                                  C.c_uint64, C.c_int, C.c_double, C.c_void_p, C.c_void_p,
                                  C.c_void_p]
+        L.ref_u01_32.argtypes = [u32p, C.c_size_t, f64p]
+        L.ref_u01_64.argtypes = [u64p, C.c_size_t, f64p]
+
+    def u01_32(self, u):
+        """The eight 32-bit u01 conversions (closed_closed, closed_open,
+        open_closed, open_open; _24 then _53) of every u: (n, 8)."""
+        u = np.ascontiguousarray(u, np.uint32)
+        out = np.zeros((u.size, 8))
+        self.lib.ref_u01_32(u, u.size, out)
+        return out
+
+    def u01_64(self, u):
+        """The four 64-bit u01 conversions of every u: (n, 4)."""
+        u = np.ascontiguousarray(u, np.uint64)
+        out = np.zeros((u.size, 4))
+        self.lib.ref_u01_64(u, u.size, out)
+        return out
 
     def philox4x32_block(self, ctr, key):
         out = np.zeros(4, np.uint32)

@@ -16,6 +16,7 @@
 #include <vsmc/resample/residual_systematic.hpp>
 #include <vsmc/resample/stratified.hpp>
 #include <vsmc/resample/systematic.hpp>
+#include <vsmc/rng/u01.h>
 #include <vsmc/rng/philox.hpp>
 #include <vsmc/rng/threefry.hpp>
 #include <vsmc/rng/u01.hpp>
@@ -328,6 +329,35 @@ int ref_cv_run(const ref_cv_param *p, size_t N, size_t T, const double *obs, uin
             for (int j = 0; j < 4; ++j) state_out[4 * i + j] = particle.value().state(i, j);
     if (lw_out) particle.weight_set().read_log_weight(lw_out);
     return 0;
+}
+
+// The twelve integer-to-uniform conversions of include/vsmc/rng/u01.h:
+// out holds 8 values per 32-bit input (closed_closed, closed_open,
+// open_closed, open_open; the _24 four as float then the _53 four) and 4 per
+// 64-bit input (the _64_53 four in the same order).
+void ref_u01_32(const uint32_t *u, size_t n, double *out)
+{
+    for (size_t i = 0; i < n; ++i) {
+        double *o = out + 8 * i;
+        o[0] = u01_closed_closed_32_24(u[i]);
+        o[1] = u01_closed_open_32_24(u[i]);
+        o[2] = u01_open_closed_32_24(u[i]);
+        o[3] = u01_open_open_32_24(u[i]);
+        o[4] = u01_closed_closed_32_53(u[i]);
+        o[5] = u01_closed_open_32_53(u[i]);
+        o[6] = u01_open_closed_32_53(u[i]);
+        o[7] = u01_open_open_32_53(u[i]);
+    }
+}
+void ref_u01_64(const uint64_t *u, size_t n, double *out)
+{
+    for (size_t i = 0; i < n; ++i) {
+        double *o = out + 4 * i;
+        o[0] = u01_closed_closed_64_53(u[i]);
+        o[1] = u01_closed_open_64_53(u[i]);
+        o[2] = u01_open_closed_64_53(u[i]);
+        o[3] = u01_open_open_64_53(u[i]);
+    }
 }
 
 }  // extern "C"

@@ -701,3 +701,163 @@ int orc_cv_run_timed(const orc_cv_param *p, size_t N, size_t T, const double *ob
     return orc_cv_run_impl(p, N, T, obs, seed, resample_seed, scheme, threshold, NULL, NULL,
                            rec, threads, phase);
 }
+
+/* ======================================================================
+ * The OpenCL-C RNG layer (include/vsmc/rng/urng.h:179-226 with the default
+ * cburng4x32 = Threefry4x32-20, urng.h:307-332; include/vsmc/rng/normal01.h:
+ * 118-139) and the OpenCL pf example's kernels (example/pf/pf_cl.cl:46-122),
+ * the convention user moves follow.  The block function is pinned by the
+ * Random123 known-answer vectors; the urng conventions (counter word 0
+ * bumped without carry, block words returned last to first) and normal01's
+ * draw order are restated from the source: Random123 is absent here, so
+ * urng.h and normal01.h cannot be compiled -- parity unpinned for them.
+ * ====================================================================== */
+void orc_cburng4x32_init(orc_cburng4x32 *r, uint32_t seed)
+{
+    for (int j = 0; j < 4; ++j) r->key[j] = r->ctr[j] = r->rnd[j] = 0;
+    r->key[0] = seed;
+    r->remain = 0;
+}
+
+uint32_t orc_cburng4x32_rand(orc_cburng4x32 *r)
+{
+    if (r->remain > 0) return r->rnd[--r->remain];
+    ++r->ctr[0];
+    orc_threefry4x32_block(r->ctr, r->key, r->rnd);
+    r->remain = 3;
+    return r->rnd[3];
+}
+
+void orc_normal01_4x32_53_init(orc_normal01_53 *n, orc_cburng4x32 *r)
+{
+    n->u1 = orc_u01_open_closed_32_53(orc_cburng4x32_rand(r));
+    n->u2 = orc_u01_open_closed_32_53(orc_cburng4x32_rand(r));
+    n->saved = 1;
+}
+
+double orc_normal01_4x32_53_rand(orc_normal01_53 *n, orc_cburng4x32 *r)
+{
+    const double c_2pi = 6.2831853071795865;
+    if (n->saved) {
+        n->saved = 0;
+        return sqrt(-2 * log(n->u1)) * cos(c_2pi * n->u2);
+    }
+    orc_normal01_4x32_53_init(n, r);
+    return sqrt(-2 * log(n->u1)) * sin(c_2pi * n->u2);
+}
+
+void orc_cburng4x32_stream(uint32_t seed, size_t n, uint32_t *out)
+{
+    orc_cburng4x32 r;
+    orc_cburng4x32_init(&r, seed);
+    for (size_t i = 0; i < n; ++i) out[i] = orc_cburng4x32_rand(&r);
+}
+
+void orc_normal01_4x32_53_stream(uint32_t seed, size_t n, double *out)
+{
+    orc_cburng4x32 r;
+    orc_cburng4x32_init(&r, seed);
+    orc_normal01_53 z;
+    orc_normal01_4x32_53_init(&z, &r);
+    for (size_t i = 0; i < n; ++i) out[i] = orc_normal01_4x32_53_rand(&z, &r);
+}
+
+/* log_likelihood of pf_cl.cl:46-58 (unnormalised, scale 10, nu 10) */
+static double orc_clpf_llh(const double *s, double ox, double oy)
+{
+    const double scale = 10, nu = 10;
+    double lx = scale * (s[0] - ox);
+    double ly = scale * (s[1] - oy);
+    lx = log1p(lx * lx / nu);
+    ly = log1p(ly * ly / nu);
+    return -0.5 * (nu + 1) * (lx + ly);
+}
+
+/* The OpenCL pf example with fp_type double and NORMAL01_4x32 =
+ * normal01_4x32_53: cv_init (pf_cl.cl:60-88) and cv_move (90-122), engine
+ * key SEED + i, the counter of slot i kept across iterations (it stays with
+ * the slot through a resample), set_log_weight at iteration 0 and
+ * add_log_weight after (the MoveCL driver), Particle::resample at threshold
+ * (core/particle.hpp:230-250) with the Threefry4x64 resample engine seeded
+ * with resample_seed, StateMatrix::copy in index order.  rec: ESS before the
+ * resample, log-evidence, resampled, draws (means 0). */
+int orc_clpf_run(size_t N, size_t T, const double *obs, uint32_t seed, uint64_t resample_seed,
+                 int scheme, double threshold, double *state_out, orc_step_record *rec)
+{
+    double *st = (double *)malloc(sizeof(double) * 4 * N);
+    double *lw = (double *)malloc(sizeof(double) * N);
+    double *w = (double *)malloc(sizeof(double) * N);
+    double *inc = (double *)malloc(sizeof(double) * N);
+    uint32_t *ctr = (uint32_t *)malloc(sizeof(uint32_t) * 4 * N);
+    uint64_t *rep = (uint64_t *)malloc(sizeof(uint64_t) * N);
+    uint64_t *cf = (uint64_t *)malloc(sizeof(uint64_t) * N);
+    if (!st || !lw || !w || !inc || !ctr || !rep || !cf) {
+        free(st); free(lw); free(w); free(inc); free(ctr); free(rep); free(cf);
+        return -1;
+    }
+    orc_tf64_engine reng;
+    orc_tf64_seed(&reng, resample_seed);
+    const double sd_pos = sqrt(0.02), sd_vel = sqrt(0.001), delta = 0.1;
+    double log_z = 0;
+    for (size_t i = 0; i < N; ++i) w[i] = 1.0 / (double)N;
+    for (size_t t = 0; t < T; ++t) {
+        const double ox = obs[2 * t], oy = obs[2 * t + 1];
+        for (size_t i = 0; i < N; ++i) {
+            orc_cburng4x32 r;
+            orc_cburng4x32_init(&r, (uint32_t)(seed + i));
+            if (t > 0)
+                for (int j = 0; j < 4; ++j) r.ctr[j] = ctr[4 * i + j];
+            orc_normal01_53 z;
+            orc_normal01_4x32_53_init(&z, &r);
+            double *s = st + 4 * i;
+            if (t == 0) {
+                s[0] = orc_normal01_4x32_53_rand(&z, &r) * 2;
+                s[1] = orc_normal01_4x32_53_rand(&z, &r) * 2;
+                s[2] = orc_normal01_4x32_53_rand(&z, &r) * 1;
+                s[3] = orc_normal01_4x32_53_rand(&z, &r) * 1;
+            } else {
+                s[0] += orc_normal01_4x32_53_rand(&z, &r) * sd_pos + delta * s[2];
+                s[1] += orc_normal01_4x32_53_rand(&z, &r) * sd_pos + delta * s[3];
+                s[2] += orc_normal01_4x32_53_rand(&z, &r) * sd_vel;
+                s[3] += orc_normal01_4x32_53_rand(&z, &r) * sd_vel;
+            }
+            inc[i] = orc_clpf_llh(s, ox, oy);
+            for (int j = 0; j < 4; ++j) ctr[4 * i + j] = r.ctr[j];
+        }
+        double imax = inc[0];
+        for (size_t i = 0; i < N; ++i)
+            if (imax < inc[i]) imax = inc[i];
+        double sum = 0;
+        for (size_t i = 0; i < N; ++i) sum += w[i] * exp(inc[i] - imax);
+        log_z += log(sum) + imax;
+        double ess;
+        if (t == 0) {
+            for (size_t i = 0; i < N; ++i) lw[i] = inc[i];
+            ess = orc_weightset_set_log_weight(N, lw, w);
+        } else {
+            ess = orc_weightset_add_log_weight(N, lw, inc, w);
+        }
+        const int resampled = ess < threshold * (double)N;
+        size_t draws = 0;
+        if (resampled) {
+            draws = orc_resample(scheme, N, N, &reng, w, rep);
+            orc_cfrp_trans(N, N, rep, cf);
+            for (size_t to = 0; to < N; ++to) {
+                const size_t from = cf[to];
+                if (from != to) memcpy(st + 4 * to, st + 4 * from, sizeof(double) * 4);
+            }
+            for (size_t i = 0; i < N; ++i) { w[i] = 1.0 / (double)N; lw[i] = 0; }
+        }
+        if (rec) {
+            rec[t].ess = ess;
+            rec[t].log_z = log_z;
+            rec[t].mean_x = 0;
+            rec[t].mean_y = 0;
+            rec[t].resampled = resampled;
+            rec[t].draws = draws;
+        }
+    }
+    if (state_out) memcpy(state_out, st, sizeof(double) * 4 * N);
+    free(st); free(lw); free(w); free(inc); free(ctr); free(rep); free(cf);
+    return 0;
+}

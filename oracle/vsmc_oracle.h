@@ -175,6 +175,18 @@ int orc_gmm_mh_move(int K, int move, uint64_t seed, uint64_t id, uint32_t iter, 
 int orc_gmm_run(const orc_gmm_config *c, size_t N, double *params_out, double *llh_out,
                 double *lw_out, orc_gmm_record *rec, size_t *rows, int threads);
 
+/* ---- OpenCL-C RNG layer and the OpenCL pf example (user-kernel convention) ---- */
+typedef struct { uint32_t key[4], ctr[4], rnd[4]; unsigned remain; } orc_cburng4x32;
+typedef struct { double u1, u2; int saved; } orc_normal01_53;
+void orc_cburng4x32_init(orc_cburng4x32 *r, uint32_t seed);
+uint32_t orc_cburng4x32_rand(orc_cburng4x32 *r);
+void orc_normal01_4x32_53_init(orc_normal01_53 *n, orc_cburng4x32 *r);
+double orc_normal01_4x32_53_rand(orc_normal01_53 *n, orc_cburng4x32 *r);
+void orc_cburng4x32_stream(uint32_t seed, size_t n, uint32_t *out);
+void orc_normal01_4x32_53_stream(uint32_t seed, size_t n, double *out);
+int orc_clpf_run(size_t N, size_t T, const double *obs, uint32_t seed, uint64_t resample_seed,
+                 int scheme, double threshold, double *state_out, orc_step_record *rec);
+
 #ifdef __cplusplus
 }
 #endif

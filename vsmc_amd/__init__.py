@@ -171,6 +171,17 @@ def lib():
         "vsmcx_serve_history": ([vp, u64, u64, vp], i32),
         "vsmcx_serve_forget": ([vp, u64], i32),
         "vsmcx_serve_stats": ([vp, P(ServeSummary)], i32),
+        "vsmcx_ctx_stream": ([vp, P(vp)], i32),
+        "vsmcx_weights_create": ([vp, u64, i32, f64, u64, u64, P(vp)], i32),
+        "vsmcx_weights_destroy": ([vp], i32),
+        "vsmcx_weights_buffer": ([vp, P(vp)], i32),
+        "vsmcx_weights_set_log": ([vp, vp], i32),
+        "vsmcx_weights_add_log": ([vp, vp], i32),
+        "vsmcx_weights_resample": ([vp, vp], i32),
+        "vsmcx_weights_ancestors": ([vp, P(vp)], i32),
+        "vsmcx_weights_steps": ([vp, P(u64)], i32),
+        "vsmcx_weights_history": ([vp, u64, vp], i32),
+        "vsmcx_weights_read": ([vp, vp], i32),
         "vsmcx_state_create": ([vp, u64, u32, i32, P(vp)], i32),
         "vsmcx_state_destroy": ([vp], i32),
         "vsmcx_state_clone": ([vp, P(vp)], i32),
@@ -682,6 +693,60 @@ class State:
         _check(lib().vsmcx_state_cv_move(self.h, C.byref(desc), iteration, o.ctypes.data,
                                          inc.ctypes.data))
         return inc
+
+
+class Weights:
+    """Device weight set (vsmcx_weights): vSMC's WeightSet in HBM, updated from
+    device arrays of increments (a user kernel's output: pass its device
+    address), with Particle::resample on the device.  Increments live in
+    device memory, so the set / add calls take device addresses (buffer()
+    gives one owned by the set)."""
+
+    def __init__(self, ctx, n, scheme=SYSTEMATIC, threshold=0.5, resample_seed=4242,
+                 max_steps=1000):
+        self.ctx = ctx
+        self.n = n
+        self.h = C.c_void_p()
+        _check(lib().vsmcx_weights_create(ctx.h, n, scheme, threshold, resample_seed, max_steps,
+                                          C.byref(self.h)))
+
+    def close(self):
+        if self.h:
+            lib().vsmcx_weights_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        self.close()
+
+    def buffer(self):
+        p = C.c_void_p()
+        _check(lib().vsmcx_weights_buffer(self.h, C.byref(p)))
+        return p.value
+
+    def set_log(self, inc_dev):
+        _check(lib().vsmcx_weights_set_log(self.h, inc_dev))
+
+    def add_log(self, inc_dev):
+        _check(lib().vsmcx_weights_add_log(self.h, inc_dev))
+
+    def resample(self, state=None):
+        _check(lib().vsmcx_weights_resample(self.h, state.h if state is not None else None))
+
+    def steps(self):
+        t = C.c_uint64()
+        _check(lib().vsmcx_weights_steps(self.h, C.byref(t)))
+        return t.value
+
+    def history(self, count=None):
+        count = self.steps() if count is None else count
+        out = np.zeros(count, STEP_DTYPE)
+        _check(lib().vsmcx_weights_history(self.h, count, out.ctypes.data))
+        return out
+
+    def read(self):
+        w = np.zeros(self.n)
+        _check(lib().vsmcx_weights_read(self.h, w.ctypes.data))
+        return w
 
 
 class Gmm:

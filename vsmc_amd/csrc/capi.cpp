@@ -107,13 +107,6 @@ void engine_out(const Tf64State &s, vsmcx_threefry4x64_state &e,
     e.index = s.index;
 }
 
-void check_dev_err(const FilterCtl &h)
-{
-    if (h.err & kErrDegenerate) throw DevError(VSMCX_ERR_DEGENERATE, "weights sum to zero or are not finite");
-    if (h.err) throw DevError(VSMCX_ERR_INTERNAL, "device consistency check failed (bits " + std::to_string(h.err) + ")");
-}
-
-bool scheme_ok(int s) { return s >= 0 && s <= 5; }
 
 }  // namespace
 

@@ -21,11 +21,6 @@
 using namespace vsmcx;
 using namespace vsmcx_host;
 
-namespace vsmcx {
-void alloc_resample(vsmcx_ctx *c, ResampleBuffers &rb, uint64_t M, uint64_t nuni, bool lists);
-void free_resample(vsmcx_ctx *c, ResampleBuffers &rb);
-}  // namespace vsmcx
-
 static_assert(sizeof(GmmRow) == sizeof(vsmcx_gmm_step), "history row layout");
 
 struct vsmcx_gmm {

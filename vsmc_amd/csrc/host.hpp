@@ -34,6 +34,15 @@ inline int fail(int code, const std::string &msg)
     return code;
 }
 
+// the device's error bits of a control block, as the C ABI's status
+inline void check_dev_err(const FilterCtl &h)
+{
+    if (h.err & kErrDegenerate) throw DevError(VSMCX_ERR_DEGENERATE, "weights sum to zero or are not finite");
+    if (h.err) throw DevError(VSMCX_ERR_INTERNAL, "device consistency check failed (bits " + std::to_string(h.err) + ")");
+}
+
+inline bool scheme_ok(int s) { return s >= 0 && s <= 5; }
+
 }  // namespace vsmcx_host
 
 using namespace vsmcx_host;
@@ -97,4 +106,12 @@ struct vsmcx_ctx {
     }
     void sync() { VSMCX_HIP(hipStreamSynchronize(stream)); }
 };
+
+namespace vsmcx {
+struct ResampleBuffers;
+// Fold / resample scratch for up to M particles and nuni uniforms, with the
+// ancestor and cfrp lists when lists is set (capi.cpp)
+void alloc_resample(vsmcx_ctx *c, ResampleBuffers &rb, uint64_t M, uint64_t nuni, bool lists);
+void free_resample(vsmcx_ctx *c, ResampleBuffers &rb);
+}  // namespace vsmcx
 

@@ -63,6 +63,13 @@ void launch_state_cv_move(bool fp64, void *st, double *inc, const CvParams &p, u
 void launch_state_gather(bool fp64, const void *in, void *out, const uint64_t *copy_from,
                          uint32_t dim, uint64_t N, int *err, hipStream_t s);
 
+// kernels_weights.hip (the device weight set)
+void launch_ws_add(const FilterCtl *ctl, double *lw, const double *inc, uint64_t N, int set,
+                   double *part, hipStream_t s);
+void launch_ws_gather(bool fp64, const FilterCtl *ctl, void *x, const uint32_t *anc,
+                      uint32_t dim, uint64_t N, int *err, hipStream_t s);
+void launch_ws_done(FilterCtl *ctl, hipStream_t s);
+
 // kernels_plain.hip: the un-fused FP64 baseline
 void launch_plain_init_weights(double *w, uint64_t N, hipStream_t s);
 void launch_plain_move(void *st, double *inc, const double *obs, const CvParams &p, uint64_t seed,
