@@ -137,6 +137,16 @@ int vsmcx_filter_set_obs(vsmcx_filter *f, uint64_t T, const double *obs);
 int vsmcx_filter_run(vsmcx_filter *f, uint64_t nsteps);
 /* finish: evaluate the monitor of the last step, synchronise, check errors */
 int vsmcx_filter_finish(vsmcx_filter *f);
+/* Checkpoint / resume (on-disk state; the reference's hdf5store, utility/hdf5io.hpp:
+ * 610-746, writes a Sampler's state and summary to HDF5, which this image lacks): a
+ * fused filter's particles, log-weights, ancestors, weight scalars, log-evidence,
+ * resample-engine state, observations and history in one file of this build's own
+ * versioned layout.  vsmcx_filter_load creates a filter on ctx that resumes exactly
+ * where the save left it: later steps equal an uninterrupted run's bit for bit. */
+int vsmcx_filter_save(vsmcx_filter *f, const char *path);
+/* the descriptor the filter was created with (or loaded from) */
+int vsmcx_filter_describe(vsmcx_filter *f, vsmcx_filter_desc *out);
+int vsmcx_filter_load(vsmcx_ctx *ctx, const char *path, vsmcx_filter **out);
 /* steps completed so far */
 int vsmcx_filter_steps(vsmcx_filter *f, uint64_t *t);
 /* history rows [0, count) (after finish) */

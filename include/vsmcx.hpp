@@ -11,6 +11,10 @@
 //   vsmcx::Threefry4x64State      engine state transfer in the text layout of
 //                                 ThreefryEngine's stream operators
 //                                 (include/vsmc/rng/threefry.hpp:424-463)
+//   vsmcx::Philox4x32State        the same for PhiloxEngine, whose operator>>
+//                                 reads its fields in another order than its
+//                                 operator<< writes them
+//                                 (include/vsmc/rng/philox.hpp:455-484)
 //   vsmcx::Filter                 a device-resident bootstrap filter of the pf
 //                                 model: a whole Sampler run (init, iterate,
 //                                 histories) in HBM
@@ -137,6 +141,65 @@ struct Threefry4x64State : vsmcx_threefry4x64_state {
         ss << index << ' ';
         ss >> eng;
         if (!ss) throw Error(VSMCX_ERR_ARG, "cannot write the Threefry4x64 engine state");
+    }
+};
+
+// Philox4x32 engine state as text.  vSMC's PhiloxEngine writes
+// "ctr key buffer index" (include/vsmc/rng/philox.hpp:455-467) but its
+// operator>> reads "buffer ctr key index" (469-484), so a state written by
+// one does not come back through the other.  This type reads the written
+// layout (from), writes it back in the same layout (write / read round trip),
+// and restores an engine through its own operator>> in the order that
+// operator reads (to), so the engine continues its stream exactly.
+struct Philox4x32State {
+    std::uint32_t ctr[4] = {0, 0, 0, 0};
+    std::uint32_t key[2] = {0, 0};
+    std::uint32_t buffer[4] = {0, 0, 0, 0};
+    std::uint32_t index = 4;
+
+    template <typename Engine>
+    static Philox4x32State from(const Engine &eng)
+    {
+        std::stringstream ss;
+        ss << eng;
+        Philox4x32State s;
+        s.read(ss);
+        if (!ss) throw Error(VSMCX_ERR_ARG, "cannot read the Philox4x32 engine state");
+        return s;
+    }
+
+    template <typename Engine>
+    void to(Engine &eng) const
+    {
+        std::stringstream ss;
+        for (auto w : buffer) ss << w << ' ';
+        for (auto w : ctr) ss << w << ' ';
+        for (auto w : key) ss << w << ' ';
+        // trailing blank: operator>> commits the state only when the stream
+        // is still good() after the last field
+        ss << index << ' ';
+        ss >> eng;
+        if (!ss) throw Error(VSMCX_ERR_ARG, "cannot write the Philox4x32 engine state");
+    }
+
+    // the engine's written layout: ctr key buffer index
+    template <typename CharT, typename Traits>
+    void write(std::basic_ostream<CharT, Traits> &os) const
+    {
+        for (auto w : ctr) os << w << ' ';
+        for (auto w : key) os << w << ' ';
+        for (auto w : buffer) os << w << ' ';
+        os << index;
+    }
+    template <typename CharT, typename Traits>
+    void read(std::basic_istream<CharT, Traits> &is)
+    {
+        Philox4x32State s;
+        for (auto &w : s.ctr) is >> w;
+        for (auto &w : s.key) is >> w;
+        for (auto &w : s.buffer) is >> w;
+        is >> s.index;
+        if (is) *this = s;
     }
 };
 
@@ -306,10 +369,16 @@ class Filter {
         d.llh_const = llh_const(c.nu, c.obs_scale);
         check(vsmcx_filter_create(ctx.get(), &d, &h_));
     }
+    // a filter resumed from a checkpoint that save() wrote (vsmcx_filter_load)
+    Filter(Context &ctx, const char *checkpoint)
+    {
+        check(vsmcx_filter_load(ctx.get(), checkpoint, &h_));
+    }
     ~Filter()
     {
         if (h_) vsmcx_filter_destroy(h_);
     }
+    void save(const char *path) const { check(vsmcx_filter_save(h_, path)); }
     Filter(const Filter &) = delete;
     Filter &operator=(const Filter &) = delete;
 

@@ -11,10 +11,13 @@ include), compiled for the host as tests/cpp/_bin/libvsmcx_device_host.so:
     against the oracle's restatement; beyond the block function that is a
     reading of the source -- Random123 is absent, so urng.h and normal01.h
     cannot be compiled here (parity unpinned for their conventions).  The
-    first four draws are checked to be one block's words last to first.
+    first four draws are checked to be one block's words last to first;
+  * vsmcx::Philox4x32State (include/vsmcx.hpp) against vSMC's own Philox4x32
+    (oracle/_ref/philox_state, built against the reference headers).
 """
 import ctypes as C
 import os
+import subprocess
 
 import numpy as np
 import pytest
@@ -154,3 +157,20 @@ def test_header_normal01_stream(dh, oracle):
         z = np.zeros(20000)
         dh.dh_normal01_stream(kind, 11, z.size, z)
         assert abs(z.mean()) < 0.05 and abs(z.std() - 1) < 0.03
+
+
+PHILOX_STATE = os.path.join(ROOT, "oracle", "_ref", "philox_state")
+
+
+@pytest.mark.skipif(not os.path.exists(PHILOX_STATE),
+                    reason="oracle/_ref/philox_state not built (needs the vSMC tree at build time)")
+def test_philox_state_text_round_trip():
+    """vSMC's PhiloxEngine writes ctr key buffer index but reads buffer ctr key
+    index (philox.hpp:455-484): its own pair does not round-trip a mid-block
+    state.  vsmcx::Philox4x32State does, and the engine it restores continues
+    the original's stream."""
+    r = subprocess.run([PHILOX_STATE], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stderr
+    ref_pair, state_pair, same_stream = (int(v) for v in r.stdout.split())
+    assert ref_pair == 0
+    assert state_pair == 1 and same_stream == 1

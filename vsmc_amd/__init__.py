@@ -182,6 +182,9 @@ def lib():
         "vsmcx_weights_steps": ([vp, P(u64)], i32),
         "vsmcx_weights_history": ([vp, u64, vp], i32),
         "vsmcx_weights_read": ([vp, vp], i32),
+        "vsmcx_filter_save": ([vp, C.c_char_p], i32),
+        "vsmcx_filter_load": ([vp, C.c_char_p, P(vp)], i32),
+        "vsmcx_filter_describe": ([vp, P(FilterDesc)], i32),
         "vsmcx_state_create": ([vp, u64, u32, i32, P(vp)], i32),
         "vsmcx_state_destroy": ([vp], i32),
         "vsmcx_state_clone": ([vp, P(vp)], i32),
@@ -360,6 +363,21 @@ class Filter:
 
     def run(self, nsteps):
         _check(lib().vsmcx_filter_run(self.h, nsteps))
+
+    def save(self, path):
+        """Checkpoint the whole device state to path (fused pipelines)."""
+        _check(lib().vsmcx_filter_save(self.h, os.fsencode(path)))
+
+    @classmethod
+    def load(cls, ctx, path):
+        """A filter resumed from a checkpoint, on ctx."""
+        f = cls.__new__(cls)
+        f.ctx = ctx
+        f.h = C.c_void_p()
+        _check(lib().vsmcx_filter_load(ctx.h, os.fsencode(path), C.byref(f.h)))
+        f.desc = FilterDesc()
+        _check(lib().vsmcx_filter_describe(f.h, C.byref(f.desc)))
+        return f
 
     def finish(self):
         _check(lib().vsmcx_filter_finish(self.h))

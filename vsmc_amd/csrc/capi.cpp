@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -510,6 +511,156 @@ int vsmcx_filter_history(vsmcx_filter *f, uint64_t count, vsmcx_step_stats *out)
     VSMCX_HIP(hipMemcpyAsync(out, f->hist, sizeof(StepRow) * count, hipMemcpyDeviceToHost,
                              f->ctx->stream));
     f->ctx->sync();
+    API_END
+}
+
+// ---- checkpoint / resume --------------------------------------------------------
+// A fused filter's whole device state in one file of this build's own layout:
+//   "VSMCXCK1", u32 version, u32 sizeof(FilterCtl), u32 sizeof(StepRow),
+//   u32 state bytes per particle, vsmcx_filter_desc, u64 t, u64 nobs,
+//   FilterCtl (weights' scalars, log-evidence, the resample engine, flags),
+//   state rows (N), log-weights (N f64), ancestors (N u32),
+//   observations (nobs x 2 f64), history (horizon rows).
+// Loading creates the filter and resumes it where the save left it: the
+// steps after the load compute exactly what an uninterrupted run computes.
+extern "C++" {
+namespace {
+const char kCkMagic[8] = {'V', 'S', 'M', 'C', 'X', 'C', 'K', '1'};
+const uint32_t kCkVersion = 1;
+
+struct CkHeader {
+    char magic[8];
+    uint32_t version, ctl_bytes, row_bytes, state_bytes;
+    vsmcx_filter_desc desc;
+    uint64_t t, nobs;
+};
+
+struct File {
+    std::FILE *fp;
+    File(const char *path, const char *mode) : fp(std::fopen(path, mode))
+    {
+        if (!fp) throw ArgError(std::string("cannot open ") + path);
+    }
+    ~File()
+    {
+        if (fp) std::fclose(fp);
+    }
+    void put(const void *p, size_t n)
+    {
+        if (n && std::fwrite(p, 1, n, fp) != n) throw DevError(VSMCX_ERR_INTERNAL, "checkpoint write failed");
+    }
+    void get(void *p, size_t n)
+    {
+        if (n && std::fread(p, 1, n, fp) != n) throw ArgError("checkpoint file is truncated");
+    }
+    void close()
+    {
+        const int rc = std::fclose(fp);
+        fp = nullptr;
+        if (rc != 0) throw DevError(VSMCX_ERR_INTERNAL, "checkpoint write failed");
+    }
+};
+
+template <typename T>
+std::vector<T> to_host(vsmcx_ctx *c, const void *d, size_t n)
+{
+    std::vector<T> h(n);
+    if (n) VSMCX_HIP(hipMemcpyAsync(h.data(), d, n * sizeof(T), hipMemcpyDeviceToHost, c->stream));
+    return h;
+}
+}  // namespace
+}  // extern "C++"
+
+int vsmcx_filter_describe(vsmcx_filter *f, vsmcx_filter_desc *out)
+{
+    API_BEGIN
+    REQUIRE(f && out, "NULL argument");
+    *out = f->d;
+    API_END
+}
+
+int vsmcx_filter_save(vsmcx_filter *f, const char *path)
+{
+    API_BEGIN
+    REQUIRE(f && path, "NULL argument");
+    REQUIRE(!f->plain, "checkpoints need a fused pipeline (VSMCX_MIXED or VSMCX_FP64_FUSED)");
+    const uint64_t N = f->N;
+    CkHeader h{};
+    std::memcpy(h.magic, kCkMagic, 8);
+    h.version = kCkVersion;
+    h.ctl_bytes = sizeof(FilterCtl);
+    h.row_bytes = sizeof(StepRow);
+    h.state_bytes = (uint32_t)f->state_bytes();
+    h.desc = f->d;
+    h.t = f->t;
+    h.nobs = f->nobs;
+    FilterCtl ctl;
+    VSMCX_HIP(hipMemcpyAsync(&ctl, f->ctl, sizeof(ctl), hipMemcpyDeviceToHost, f->ctx->stream));
+    auto st = to_host<unsigned char>(f->ctx, f->st[f->cur], N * f->state_bytes());
+    auto lw = to_host<double>(f->ctx, f->lw, N);
+    auto anc = to_host<uint32_t>(f->ctx, f->rb.anc, N);
+    auto obs = to_host<double>(f->ctx, f->obs, 2 * f->nobs);
+    auto hist = to_host<StepRow>(f->ctx, f->hist, f->T);
+    f->ctx->sync();
+    check_dev_err(ctl);
+    File out(path, "wb");
+    out.put(&h, sizeof(h));
+    out.put(&ctl, sizeof(ctl));
+    out.put(st.data(), st.size());
+    out.put(lw.data(), 8 * N);
+    out.put(anc.data(), 4 * N);
+    out.put(obs.data(), 8 * obs.size());
+    out.put(hist.data(), sizeof(StepRow) * hist.size());
+    out.close();
+    API_END
+}
+
+int vsmcx_filter_load(vsmcx_ctx *ctx, const char *path, vsmcx_filter **out)
+{
+    API_BEGIN
+    REQUIRE(ctx && path && out, "NULL argument");
+    File in(path, "rb");
+    CkHeader h;
+    in.get(&h, sizeof(h));
+    REQUIRE(std::memcmp(h.magic, kCkMagic, 8) == 0, "not a vsmcx filter checkpoint");
+    REQUIRE(h.version == kCkVersion && h.ctl_bytes == sizeof(FilterCtl) &&
+                h.row_bytes == sizeof(StepRow),
+            "checkpoint written by an incompatible build");
+    REQUIRE(h.desc.precision == VSMCX_MIXED || h.desc.precision == VSMCX_FP64_FUSED,
+            "checkpoint of an unsupported pipeline");
+    REQUIRE(h.state_bytes == (h.desc.precision == VSMCX_MIXED ? 16u : 32u),
+            "checkpoint state size does not match its precision");
+    REQUIRE(h.nobs <= h.desc.horizon && h.t <= h.nobs, "checkpoint counters out of range");
+    FilterCtl ctl;
+    in.get(&ctl, sizeof(ctl));
+    const uint64_t N = h.desc.n;
+    std::vector<unsigned char> st(N * h.state_bytes);
+    std::vector<double> lw(N), obs(2 * h.nobs);
+    std::vector<uint32_t> anc(N);
+    std::vector<StepRow> hist(h.desc.horizon);
+    in.get(st.data(), st.size());
+    in.get(lw.data(), 8 * N);
+    in.get(anc.data(), 4 * N);
+    in.get(obs.data(), 8 * obs.size());
+    in.get(hist.data(), sizeof(StepRow) * hist.size());
+    vsmcx_filter *f = nullptr;
+    const int rc = vsmcx_filter_create(ctx, &h.desc, &f);
+    if (rc != VSMCX_OK) return rc;
+    std::unique_ptr<vsmcx_filter, int (*)(vsmcx_filter *)> guard(f, vsmcx_filter_destroy);
+    hipStream_t s = ctx->stream;
+    VSMCX_HIP(hipMemcpyAsync(f->st[0], st.data(), st.size(), hipMemcpyHostToDevice, s));
+    VSMCX_HIP(hipMemcpyAsync(f->lw, lw.data(), 8 * N, hipMemcpyHostToDevice, s));
+    VSMCX_HIP(hipMemcpyAsync(f->rb.anc, anc.data(), 4 * N, hipMemcpyHostToDevice, s));
+    if (h.nobs)
+        VSMCX_HIP(hipMemcpyAsync(f->obs, obs.data(), 8 * obs.size(), hipMemcpyHostToDevice, s));
+    VSMCX_HIP(hipMemcpyAsync(f->hist, hist.data(), sizeof(StepRow) * hist.size(),
+                             hipMemcpyHostToDevice, s));
+    VSMCX_HIP(hipMemcpyAsync(f->ctl, &ctl, sizeof(ctl), hipMemcpyHostToDevice, s));
+    ctx->sync();
+    f->cur = 0;
+    f->t = h.t;
+    f->nobs = h.nobs;
+    *out = guard.release();
     API_END
 }
 
