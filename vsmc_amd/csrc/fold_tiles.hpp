@@ -80,6 +80,7 @@ struct XResidScaled {
 struct InvNone {
     static constexpr bool kInv = false;
     static constexpr bool kWindow = false;
+    static constexpr bool kWindowTmp = false;
     __device__ void bind(const FilterCtl *) {}
 };
 
@@ -87,6 +88,7 @@ struct InvNone {
 struct InvSystematic {
     static constexpr bool kInv = true;
     static constexpr bool kWindow = false;
+    static constexpr bool kWindowTmp = false;
     double u0, delta;
     uint64_t nn;
     __device__ void bind(const FilterCtl *c) { u0 = c->u0; delta = c->delta; nn = c->nn; }
@@ -102,10 +104,12 @@ struct InvSystematic {
     }
 };
 
-// u_n of the stratified sequence, kept out of line (20 Threefry rounds)
-__device__ __noinline__ double strat_u(const Tf64State &eng, double delta, uint64_t n)
+// u_n of the stratified sequence, kept out of line (20 Threefry rounds).  The
+// engine state is read where it lives, in the control block: a reference to
+// a local copy would put that copy in scratch memory for the call.
+__device__ __noinline__ double strat_u(const Tf64State *eng, double delta, uint64_t n)
 {
-    return canonical64(tf64_stream_word(eng, n)) * delta + (double)n * delta;
+    return canonical64(tf64_stream_word(*eng, n)) * delta + (double)n * delta;
 }
 
 // U01SequenceStratified (u01.hpp:212-243): u_n = U_n*delta + n*delta, with
@@ -114,10 +118,11 @@ __device__ __noinline__ double strat_u(const Tf64State &eng, double delta, uint6
 struct InvStratified {
     static constexpr bool kInv = true;
     static constexpr bool kWindow = true;
-    Tf64State eng;
+    static constexpr bool kWindowTmp = true;  // fill_window stages raw uniforms
+    const Tf64State *eng;  // the control block's resample engine (global memory)
     double delta;
     uint64_t nn;
-    __device__ void bind(const FilterCtl *c) { eng = c->eng; delta = c->delta; nn = c->nn; }
+    __device__ void bind(const FilterCtl *c) { eng = &c->eng; delta = c->delta; nn = c->nn; }
     __device__ double u(uint64_t n) const { return strat_u(eng, delta, n); }
     __device__ double v(uint64_t n) const
     {
@@ -131,17 +136,22 @@ struct InvStratified {
     __device__ void fill_window(double *s_v, double *tmp, uint64_t w0, int len) const
     {
         const uint64_t nlo = w0 ? w0 - 1 : 0, nhi = w0 + (uint64_t)len;
-        const uint64_t p0 = nlo + eng.index, p1 = nhi + eng.index;  // stream positions
+        const uint64_t index = eng->index;
+        const uint64_t p0 = nlo + index, p1 = nhi + index;  // stream positions
         u64x4 c0;
+        uint64_t key[4];
 #pragma unroll
-        for (int j = 0; j < 4; ++j) c0.v[j] = eng.ctr[j];
+        for (int j = 0; j < 4; ++j) {
+            c0.v[j] = eng->ctr[j];
+            key[j] = eng->key[j];
+        }
         for (uint64_t bk = p0 / 4 + threadIdx.x; bk <= (p1 - 1) / 4; bk += blockDim.x) {
-            const u64x4 r = threefry4x64_20(ctr_add(c0, bk), eng.key);
+            const u64x4 r = threefry4x64_20(ctr_add(c0, bk), key);
 #pragma unroll
             for (int w = 0; w < 4; ++w) {
                 const uint64_t p = 4 * bk + w;
                 if (p >= p0 && p < p1) {
-                    const uint64_t n = p - eng.index;
+                    const uint64_t n = p - index;
                     tmp[n - nlo] = canonical64(r.v[w]) * delta + (double)n * delta;
                 }
             }
@@ -183,6 +193,7 @@ struct InvStratified {
 struct InvArray {
     static constexpr bool kInv = true;
     static constexpr bool kWindow = true;
+    static constexpr bool kWindowTmp = false;  // fill_window reads the array
     const double *uscan;
     const double *tmax;  // exclusive running max of earlier tiles
     uint64_t nn;
@@ -389,8 +400,9 @@ __device__ __forceinline__ double fold_tile_sum(const XSrc &xs, uint64_t base, u
     return block_reduce<kBlock>(s, AddOp(), s_red);
 }
 
-// LDS of the per-tile bodies; the uniform-window buffer only where the
-// inversion needs one.
+// LDS of the per-tile bodies; the second window buffer only where the
+// inversion stages raw uniforms in it (stratified), which halves the LDS of
+// the other windowed inversion and doubles its workgroups per CU.
 template <bool kWindowBuf>
 struct FoldLdsT {
     double x[kFPad];

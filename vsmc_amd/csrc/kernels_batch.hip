@@ -18,7 +18,7 @@ namespace vsmcx {
 
 template <typename Inv>
 struct BatchLds {
-    FoldLdsT<Inv::kWindow> L;
+    FoldLdsT<Inv::kWindowTmp> L;
     SpecRec r[kBlock];
     double v[kBlock];
     Part5 p5[kBlock / kWave];

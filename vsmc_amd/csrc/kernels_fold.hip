@@ -100,7 +100,7 @@ __global__ __launch_bounds__(kBlock) void k_fold_apply(FilterCtl *ctl, XSrc xs, 
                                                        uint64_t M, FoldScratch fs, FoldOut out)
 {
     if (!ctl->resample) return;
-    __shared__ FoldLdsT<Inv::kWindow> L;
+    __shared__ FoldLdsT<Inv::kWindowTmp> L;
     xs.bind(ctl);
     inv.bind(ctl);
     const uint64_t t = blockIdx.x;

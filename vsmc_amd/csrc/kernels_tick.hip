@@ -134,7 +134,7 @@ __global__ __launch_bounds__(kBlock) void k_tick_apply(const TickItem *items,
     const TickItem it = items[blockIdx.x];
     FilterCtl *ctl = P.ctl + it.slot;
     if (!ctl->resample) return;
-    __shared__ FoldLdsT<Inv::kWindow> L;
+    __shared__ FoldLdsT<Inv::kWindowTmp> L;
     const BatchFilter F = filters[it.slot];
     // the cached weights and the segment pass's per-thread prefixes
     const XCached<XWeight> xs{P.wx + F.off, XWeight{P.lw + F.off, 0.0, 0.0}};

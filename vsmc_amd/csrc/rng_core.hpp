@@ -41,7 +41,9 @@ VSMCX_HD uint64_t tf64_stream_word(const Tf64State &e, uint64_t n)
         word = m % 4;
     }
     const u64x4 b = threefry4x64_20(c, e.key);
-    return b.v[word];
+    // a select, not b.v[word]: a runtime index would send the block through
+    // scratch memory
+    return word == 0 ? b.v[0] : word == 1 ? b.v[1] : word == 2 ? b.v[2] : b.v[3];
 }
 
 // Advance the engine by d draws (as d successive operator() calls would).
