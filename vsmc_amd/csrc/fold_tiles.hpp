@@ -64,14 +64,16 @@ struct XResid {
     }
 };
 
-// residual.hpp:62-63: r *= 1 / asum(r)
-template <typename W>
-struct XResidScaled {
-    XResid<W> r;
+// residual.hpp:62-63: r *= 1 / asum(r), from the fractions r_i and integer
+// parts that the residual tile-sum pass stored (the values XResid computes),
+// so the scaled fold takes no exp per element.
+struct XResidStored {
+    const double *r;
+    const uint32_t *ip;
     double rc;
-    __device__ void bind(const FilterCtl *c) { r.bind(c); rc = c->rcoeff; }
-    __device__ double x(uint64_t i) const { return r.x(i) * rc; }
-    __device__ uint64_t ipart(uint64_t i) const { return r.ipart(i); }
+    __device__ void bind(const FilterCtl *c) { rc = c->rcoeff; }
+    __device__ double x(uint64_t i) const { return r[i] * rc; }
+    __device__ uint64_t ipart(uint64_t i) const { return ip[i]; }
 };
 
 // ------------------------------------------------------------------------
@@ -84,6 +86,16 @@ struct InvNone {
     __device__ void bind(const FilterCtl *) {}
 };
 
+// Counts and indices of the uniform sequences stay below 2^31 (every entry
+// point bounds N so), so their conversions go through 32 bits: exact, and one
+// instruction each instead of a 64-bit sequence.
+__device__ __forceinline__ double cnt_d(uint64_t n) { return (double)(uint32_t)n; }
+// floor(est) clamped to [0, nn]
+__device__ __forceinline__ uint64_t cnt_floor(double est, uint64_t nn)
+{
+    return !(est > 0) ? 0 : (est >= cnt_d(nn) ? nn : (uint64_t)(uint32_t)est);
+}
+
 // U01SequenceSystematic (include/vsmc/rng/u01.hpp:253-287): u_n = u0 + n*delta
 struct InvSystematic {
     static constexpr bool kInv = true;
@@ -92,12 +104,11 @@ struct InvSystematic {
     double u0, delta;
     uint64_t nn;
     __device__ void bind(const FilterCtl *c) { u0 = c->u0; delta = c->delta; nn = c->nn; }
-    __device__ double v(uint64_t n) const { return u0 + (double)n * delta; }
+    __device__ double v(uint64_t n) const { return u0 + cnt_d(n) * delta; }
     __device__ uint64_t F(double c) const
     {
         if (nn == 0) return 0;
-        const double est = (c - u0) * (double)nn;
-        uint64_t n = !(est > 0) ? 0 : (est >= (double)nn ? nn : (uint64_t)est);
+        uint64_t n = cnt_floor((c - u0) * cnt_d(nn), nn);
         while (n < nn && v(n) <= c) ++n;
         while (n > 0 && v(n - 1) > c) --n;
         return n;
@@ -109,7 +120,7 @@ struct InvSystematic {
 // a local copy would put that copy in scratch memory for the call.
 __device__ __noinline__ double strat_u(const Tf64State *eng, double delta, uint64_t n)
 {
-    return canonical64(tf64_stream_word(*eng, n)) * delta + (double)n * delta;
+    return canonical64(tf64_stream_word(*eng, n)) * delta + cnt_d(n) * delta;
 }
 
 // U01SequenceStratified (u01.hpp:212-243): u_n = U_n*delta + n*delta, with
@@ -152,7 +163,7 @@ struct InvStratified {
                 const uint64_t p = 4 * bk + w;
                 if (p >= p0 && p < p1) {
                     const uint64_t n = p - index;
-                    tmp[n - nlo] = canonical64(r.v[w]) * delta + (double)n * delta;
+                    tmp[n - nlo] = canonical64(r.v[w]) * delta + cnt_d(n) * delta;
                 }
             }
         }
@@ -172,16 +183,14 @@ struct InvStratified {
     // caller verifies at both ends before trusting it.
     __device__ bool bracket(double c, uint64_t *b0) const
     {
-        const double est = c * (double)nn;
-        const uint64_t e = !(est > 0) ? 0 : (est >= (double)nn ? nn : (uint64_t)est);
+        const uint64_t e = cnt_floor(c * cnt_d(nn), nn);
         *b0 = e >= 2 ? e - 2 : 0;
         return true;
     }
     __device__ uint64_t point_F(double c) const
     {
         if (nn == 0) return 0;
-        const double est = c * (double)nn;
-        uint64_t n = !(est > 0) ? 0 : (est >= (double)nn ? nn : (uint64_t)est);
+        uint64_t n = cnt_floor(c * cnt_d(nn), nn);
         while (n < nn && v(n) <= c) ++n;
         while (n > 0 && v(n - 1) > c) --n;
         return n;
@@ -382,6 +391,15 @@ struct TileSumsFromLse {
         const double mt = part[5 * t];
         return mt == -INFINITY ? 0.0 : part[5 * t + 1] * exp(mt - m) * coeff;
     }
+};
+
+// Tile sums of the scaled residual fractions r_i * rc from the tile sums of
+// r_i: each differs from a sum of the fold's own products by the rounding of
+// the products and of one more multiply, a few u relative, inside fold_eps.
+struct TileSumsScaled {
+    const double *tsum;
+    double rc;
+    __device__ double operator[](uint64_t t) const { return tsum[t] * rc; }
 };
 
 // ---- tile sum of x (any fixed order) -----------------------------------------
@@ -747,8 +765,10 @@ template <typename Lds>
 __device__ void fold_tile_layout(FilterCtl *ctl, uint64_t t, uint64_t M, const uint32_t *rep32,
                                  uint64_t zsx, const FoldOut &out, Lds &L)
 {
-    uint32_t *const s_rep = reinterpret_cast<uint32_t *>(L.x);        // kFT entries
-    uint32_t *const s_sp = reinterpret_cast<uint32_t *>(L.x) + kFT;   // kFT entries
+    // kFPad entries each, element e at fidx(e): the per-thread loops read
+    // consecutive entries, which the padding spreads over the LDS banks
+    uint32_t *const s_rep = reinterpret_cast<uint32_t *>(L.x);
+    uint32_t *const s_sp = reinterpret_cast<uint32_t *>(L.x) + kFPad;
     uint64_t *const s_u64 = L.u64;
     __syncthreads();  // the LDS of a previous tile is no longer read
     const int tid = threadIdx.x;
@@ -758,13 +778,13 @@ __device__ void fold_tile_layout(FilterCtl *ctl, uint64_t t, uint64_t M, const u
     for (int k = 0; k < kFI; ++k) {
         const int e = k * kBlock + tid;
         const uint64_t i = base + e;
-        s_rep[e] = i < M ? rep32[i] : 1u;
+        s_rep[fidx(e)] = i < M ? rep32[i] : 1u;
     }
     __syncthreads();
     uint64_t zs_thr = 0;
 #pragma unroll 1
     for (int k = 0; k < kFI; ++k) {
-        const uint32_t r = s_rep[e0 + k];
+        const uint32_t r = s_rep[fidx(e0 + k)];
         if (r == 0)
             zs_thr += 1ull << 32;
         else if (r >= 2)
@@ -781,7 +801,7 @@ __device__ void fold_tile_layout(FilterCtl *ctl, uint64_t t, uint64_t M, const u
     for (int k = 0; k < kFI; ++k) {
         const int e = e0 + k;
         const uint64_t i = base + e;
-        const uint32_t r = s_rep[e];
+        const uint32_t r = s_rep[fidx(e)];
         if (i < M && r == 0) {
             if (zrank < M)
                 out.zl[zrank] = (uint32_t)i;
@@ -790,7 +810,7 @@ __device__ void fold_tile_layout(FilterCtl *ctl, uint64_t t, uint64_t M, const u
             ++zrank;
         }
         if (r >= 2) srank_local += r - 1;
-        s_sp[e] = (uint32_t)srank_local;
+        s_sp[fidx(e)] = (uint32_t)srank_local;
     }
 #pragma unroll
     for (int k = 0; k < kFI; ++k) {
@@ -805,7 +825,7 @@ __device__ void fold_tile_layout(FilterCtl *ctl, uint64_t t, uint64_t M, const u
         int lo = 0, hi = kFT - 1;  // first e with s_sp[e] > k
         while (lo < hi) {
             const int mid = (lo + hi) >> 1;
-            if (s_sp[mid] > k)
+            if (s_sp[fidx(mid)] > k)
                 hi = mid;
             else
                 lo = mid + 1;

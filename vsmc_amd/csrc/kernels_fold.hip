@@ -157,20 +157,68 @@ __global__ void k_resample_epilogue(FilterCtl *ctl, StepRow *hist, int systemati
     resample_epilogue(ctl, hist, systematic_like, M);
 }
 
-// residual: integral-part total R (exact integer sum)
+// Residual, the asum fold's tile-sum pass: r_i = modf(N W_i, &i_i)
+// (residual.hpp:57-61) once per element.  The fractions r_i go to wx and the
+// integer parts to rep32, where the scaled fold reads them back (its
+// inversion pass reads every i_i of a tile before it writes the tile's
+// replication counts there); the tile sums of r_i go to tsum and those of i_i
+// to zs, which the start scan consumes before the inversion pass reuses it.
 template <typename W>
-__global__ __launch_bounds__(kBlock) void k_residual_int(FilterCtl *ctl, W w, uint64_t M)
+__global__ __launch_bounds__(kBlock) void k_resid_tsum(FilterCtl *ctl, W w, uint64_t M,
+                                                       FoldScratch fs)
 {
     if (!ctl->resample) return;
+    __shared__ double s_red[kBlock / kWave];
     __shared__ uint64_t s_u64[kBlock / kWave];
-    XResid<W> r{w, 0.0};
-    r.bind(ctl);
-    uint64_t s = 0;
-    for (uint64_t i = blockIdx.x * (uint64_t)kBlock + threadIdx.x; i < M;
-         i += (uint64_t)gridDim.x * kBlock)
-        s += r.ipart(i);
-    s = block_reduce<kBlock>(s, AddOp(), s_u64);
-    if (threadIdx.x == 0 && s) atomicAdd((unsigned long long *)&ctl->r_int, (unsigned long long)s);
+    XResid<W> xr{w, 0.0};
+    xr.bind(ctl);
+    const uint64_t base = blockIdx.x * (uint64_t)kFT;
+    double s = 0;
+    uint64_t is = 0;
+#pragma unroll
+    for (int k = 0; k < kFI; ++k) {
+        const uint64_t i = base + k * kBlock + threadIdx.x;
+        double r = 0.0;
+        if (i < M) {
+            double ip;
+            r = modf(xr.nd * xr.w.x(i), &ip);
+            fs.wx[i] = r;
+            fs.rep32[i] = (uint32_t)ip;
+            is += (uint64_t)ip;
+        }
+        s += r;
+    }
+    s = block_reduce<kBlock>(s, AddOp(), s_red);
+    is = block_reduce<kBlock>(is, AddOp(), s_u64);
+    if (threadIdx.x == 0) {
+        fs.tsum[blockIdx.x] = s;
+        fs.zs[blockIdx.x] = is;
+    }
+}
+
+// Residual: tile starts of the fractions (one workgroup), and the total R of
+// the integer parts (exact integer sum).
+__global__ __launch_bounds__(kScanBlock) void k_fold_scan_sum_resid(FilterCtl *ctl, FoldScratch fs)
+{
+    if (!ctl->resample) return;
+    __shared__ double s_red[kScanBlock / kWave];
+    __shared__ uint64_t s_u64[kScanBlock / kWave];
+    wg_scan<double>(fs.tsum, fs.ts, fs.tiles, AddOp(), 0.0, s_red);
+    uint64_t r = 0;
+    for (uint64_t t = threadIdx.x; t < fs.tiles; t += kScanBlock) r += fs.zs[t];
+    r = block_reduce<kScanBlock>(r, AddOp(), s_u64);
+    if (threadIdx.x == 0) ctl->r_int = r;
+}
+
+// Residual: tile starts of the scaled fractions r_i * rc from the tile sums of
+// r_i the asum fold left in tsum (TileSumsScaled).
+__global__ __launch_bounds__(kScanBlock) void k_fold_scan_sum_scaled(FilterCtl *ctl,
+                                                                     FoldScratch fs)
+{
+    if (!ctl->resample) return;
+    __shared__ double s_red[kScanBlock / kWave];
+    const TileSumsScaled in{fs.tsum, ctl->rcoeff};
+    wg_scan<double>(in, fs.ts, fs.tiles, AddOp(), 0.0, s_red);
 }
 
 // after the residual fold total: rcoeff, nn, delta and (systematic) u0
@@ -359,6 +407,23 @@ static void launch_fold(FilterCtl *ctl, XSrc xs, Inv inv, uint64_t M, FoldScratc
     hipLaunchKernelGGL(k_fold_layout, grid, dim3(kBlock), 0, st, ctl, fs, out, M);
 }
 
+// Passes 3-7 once the tile starts are in fs.ts, x read from the source as is.
+template <typename XSrc, typename Inv>
+static void launch_fold_from_starts(FilterCtl *ctl, XSrc xs, Inv inv, uint64_t M,
+                                    FoldScratch &fs, const FoldOut &out, hipStream_t st)
+{
+    const dim3 grid((unsigned)fs.tiles);
+    hipLaunchKernelGGL((k_fold_seg<XSrc>), grid, dim3(kBlock), 0, st, ctl, xs, M, fs,
+                       (double *)nullptr);
+    hipLaunchKernelGGL(k_fold_scan_seg, dim3(1), dim3(kScanBlock), 0, st, ctl, fs);
+    if (!Inv::kInv && !out.a_out) return;
+    hipLaunchKernelGGL((k_fold_apply<XSrc, Inv>), grid, dim3(kBlock), 0, st, ctl, xs, inv, M, fs,
+                       out);
+    if (!Inv::kInv || !out.anc) return;
+    hipLaunchKernelGGL(k_fold_scan_cnt, dim3(1), dim3(kScanBlock), 0, st, ctl, fs);
+    hipLaunchKernelGGL(k_fold_layout, grid, dim3(kBlock), 0, st, ctl, fs, out, M);
+}
+
 template <typename W>
 static void resample_pipeline(FilterCtl *ctl, W w, uint64_t M, int scheme, ResampleBuffers &rb,
                               StepRow *hist, hipStream_t st, const double *lse_part = nullptr)
@@ -366,12 +431,16 @@ static void resample_pipeline(FilterCtl *ctl, W w, uint64_t M, int scheme, Resam
     FoldOut out{rb.anc, rb.zl, rb.sl, rb.rep_out, rb.a_out};
     const bool residual = scheme == kResidual || scheme == kResidualStratified ||
                           scheme == kResidualSystematic;
-    const unsigned grid = (unsigned)((M + kTile - 1) / kTile);
+    FoldScratch &fs = rb.fs;
+    fs.tiles = (M + kFT - 1) / kFT;
     if (residual) {
-        hipLaunchKernelGGL((k_residual_int<W>), dim3(grid), dim3(kBlock), 0, st, ctl, w, M);
-        XResid<W> xr{w, 0.0};
-        launch_fold(ctl, xr, InvNone(), M, rb.fs, FoldOut{nullptr, nullptr, nullptr, nullptr, nullptr},
-                    st);
+        // asum of the fractions: one exp per element, in the tile-sum pass
+        hipLaunchKernelGGL((k_resid_tsum<W>), dim3((unsigned)fs.tiles), dim3(kBlock), 0, st, ctl,
+                           w, M, fs);
+        hipLaunchKernelGGL(k_fold_scan_sum_resid, dim3(1), dim3(kScanBlock), 0, st, ctl, fs);
+        const XCached<XResid<W>> xr{fs.wx, XResid<W>{w, 0.0}};
+        launch_fold_from_starts(ctl, xr, InvNone(), M, fs,
+                                FoldOut{nullptr, nullptr, nullptr, nullptr, nullptr}, st);
         hipLaunchKernelGGL(k_residual_prologue, dim3(1), dim3(64), 0, st, ctl, scheme);
     } else {
         hipLaunchKernelGGL(k_plain_prologue, dim3(1), dim3(64), 0, st, ctl, scheme);
@@ -390,13 +459,16 @@ static void resample_pipeline(FilterCtl *ctl, W w, uint64_t M, int scheme, Resam
     }
     InvArray ia{rb.uscan, rb.u_tmaxex, 0};
     if (residual) {
-        XResidScaled<W> xs{XResid<W>{w, 0.0}, 0.0};
+        // the scaled fractions: tile starts from the asum fold's tile sums, x
+        // and integer parts from what its tile-sum pass stored
+        hipLaunchKernelGGL(k_fold_scan_sum_scaled, dim3(1), dim3(kScanBlock), 0, st, ctl, fs);
+        const XResidStored xs{fs.wx, fs.rep32, 0.0};
         if (scheme == kResidual)
-            launch_fold(ctl, xs, ia, M, rb.fs, out, st);
+            launch_fold_from_starts(ctl, xs, ia, M, fs, out, st);
         else if (scheme == kResidualStratified)
-            launch_fold(ctl, xs, InvStratified(), M, rb.fs, out, st);
+            launch_fold_from_starts(ctl, xs, InvStratified(), M, fs, out, st);
         else
-            launch_fold(ctl, xs, InvSystematic(), M, rb.fs, out, st);
+            launch_fold_from_starts(ctl, xs, InvSystematic(), M, fs, out, st);
     } else {
         if (scheme == kMultinomial)
             launch_fold(ctl, w, ia, M, rb.fs, out, st, lse_part);
