@@ -263,6 +263,32 @@ constexpr int kScanItems = 8;     // tiles per thread per round
 struct SegOp {
     __device__ __forceinline__ Seg operator()(const Seg &a, const Seg &b) const { return seg_op(a, b); }
 };
+struct SatAddOp {
+    __device__ __forceinline__ uint64_t operator()(uint64_t a, uint64_t b) const { return sat_add(a, b); }
+};
+
+// Exclusive segment prefix of each thread's run in its tile (identity for
+// thread 0).  When every thread took the fast path in one binade b, the runs
+// are tr_units(d_t, b), and tr_units(d, b) then tr_units(e, b) is
+// tr_units(d + e, b) (saturating like tr_compose), so one integer scan of d
+// gives the same prefixes as the transducer scan at a fraction of its cost.
+// That is the case in every tile away from the few binade crossings of the
+// running sum.
+template <typename Lds>
+__device__ __forceinline__ Seg thread_seg_prefix(const Seg &g, int fast_b, Lds &L)
+{
+    if (threadIdx.x == 0) L.bu[0] = (uint64_t)(uint32_t)fast_b;
+    __syncthreads();
+    const int fb0 = (int)(uint32_t)L.bu[0];
+    if (__syncthreads_and(fast_b == fb0 && fb0 != kNoBinade)) {
+        const uint64_t ex =
+            block_excl_scan<kBlock>(g.tr.d0, SatAddOp(), (uint64_t)0, L.u64, (uint64_t *)nullptr);
+        Seg r = seg_identity();
+        if (threadIdx.x != 0) r.tr = tr_units(ex, fb0);
+        return r;
+    }
+    return block_excl_scan<kBlock>(g, SegOp(), seg_identity(), L.seg, (Seg *)nullptr);
+}
 
 // First n in [lo, hi) with v(n) > c (hi if none), v non-decreasing.  All 64
 // lanes of a wave probe 64 evenly spaced points per round (64-ary search).
@@ -453,8 +479,7 @@ __device__ void fold_tile_seg(const XSrc &xs, uint64_t base, uint64_t M, double 
     const auto xat = [&](int k) { return s_x[fidx(e0 + k)]; };
     int fast_b;
     const Seg g = run_seg(xat, kFI, s_thr, er, ea, &fast_b);
-    const Seg thr_pre =
-        block_excl_scan<kBlock>(g, SegOp(), seg_identity(), L.seg, (Seg *)nullptr);
+    const Seg thr_pre = thread_seg_prefix(g, fast_b, L);
     // tile total from the last thread (a struct total through a pointer would
     // keep the struct in scratch memory)
     if (tid == kBlock - 1) *agg_out = seg_op(thr_pre, g);
@@ -564,7 +589,6 @@ __device__ void fold_tile_apply(FilterCtl *ctl, const XSrc &xs, const Inv &inv, 
     double *const s_tmp = L.tmp;
     double *const s_a = s_x;
     double *const s_red = L.red;
-    Seg *const s_seg = L.seg;
     uint64_t *const s_u64 = L.u64;
     uint64_t *const s_bu = L.bu;
     __syncthreads();  // the LDS of a previous tile is no longer read
@@ -588,7 +612,7 @@ __device__ void fold_tile_apply(FilterCtl *ctl, const XSrc &xs, const Inv &inv, 
         fast_b = fb_in[tid];
     } else {
         const Seg g = run_seg(xat, kFI, s_thr, er, ea, &fast_b);
-        thr_pre = block_excl_scan<kBlock>(g, SegOp(), seg_identity(), s_seg, (Seg *)nullptr);
+        thr_pre = thread_seg_prefix(g, fast_b, L);
     }
     int bad = !run_values(
         xat, kFI, s_thr, er, ea, fast_b, seg_op(tpre, thr_pre),
