@@ -83,6 +83,7 @@ struct InvNone {
     static constexpr bool kInv = false;
     static constexpr bool kWindow = false;
     static constexpr bool kWindowTmp = false;
+    static constexpr bool kPairF = false;
     __device__ void bind(const FilterCtl *) {}
 };
 
@@ -101,6 +102,7 @@ struct InvSystematic {
     static constexpr bool kInv = true;
     static constexpr bool kWindow = false;
     static constexpr bool kWindowTmp = false;
+    static constexpr bool kPairF = false;
     double u0, delta;
     uint64_t nn;
     __device__ void bind(const FilterCtl *c) { u0 = c->u0; delta = c->delta; nn = c->nn; }
@@ -130,6 +132,7 @@ struct InvStratified {
     static constexpr bool kInv = true;
     static constexpr bool kWindow = true;
     static constexpr bool kWindowTmp = true;  // fill_window stages raw uniforms
+    static constexpr bool kPairF = true;
     const Tf64State *eng;  // the control block's resample engine (global memory)
     double delta;
     uint64_t nn;
@@ -142,7 +145,7 @@ struct InvStratified {
         const double b = u(n - 1);
         return a < b ? b : a;
     }
-    // v(w0 .. w0+len-1) into s_v (len <= kFoldTile); tmp holds len + 1 doubles.
+    // v(w0 .. w0+len-1) into s_v (len < kFPad); tmp holds len + 1 doubles.
     // u_n are generated a Threefry block (four consecutive draws) per thread.
     __device__ void fill_window(double *s_v, double *tmp, uint64_t w0, int len) const
     {
@@ -195,6 +198,40 @@ struct InvStratified {
         while (n > 0 && v(n - 1) > c) --n;
         return n;
     }
+    // F(c_lo) and F(c_hi) (first n with v_n > c, nn if none) in one pass of
+    // the whole wave: lanes 0-31 take c_lo, lanes 32-63 c_hi.  Lane j of a
+    // half draws u_m for m = b0 - 2 + j around the bracket of its value and
+    // takes u_{m-1} from the lane below, so each v_m costs one Threefry block.
+    // v is non-decreasing, so when v_{b0-1} <= c the first m with v_m > c
+    // (m >= nn counts as above every value) is F(c).  Returns false when a
+    // half cannot conclude that way; the caller then searches.
+    __device__ bool wave_F2(double c_lo, double c_hi, uint64_t *f_lo, uint64_t *f_hi) const
+    {
+        const int lane = threadIdx.x & 63, j = lane & 31;
+        const double c = lane < 32 ? c_lo : c_hi;
+        uint64_t b0;
+        bracket(c, &b0);
+        const int64_t m = (int64_t)b0 - 2 + j;
+        const bool in = m >= 0 && (uint64_t)m < nn;
+        const double um = in ? u((uint64_t)m) : 0.0;
+        const double up = __shfl_up(um, 1, 32);  // u_{m-1}, from lane j - 1
+        bool gt = false;
+        if (j >= 1 && m >= 0) {
+            if ((uint64_t)m >= nn) {
+                gt = true;
+            } else {
+                const double vm = m == 0 ? um : (um < up ? up : um);
+                gt = vm > c;
+            }
+        }
+        const uint64_t mask = __ballot(gt);
+        const uint32_t hl = (uint32_t)mask, hh = (uint32_t)(mask >> 32);
+        const uint64_t bl = __shfl(b0, 0), bh = __shfl(b0, 32);
+        if (!hl || (hl & 2u) || !hh || (hh & 2u)) return false;
+        *f_lo = bl - 2 + (uint64_t)(__ffs(hl) - 1);
+        *f_hi = bh - 2 + (uint64_t)(__ffs(hh) - 1);
+        return true;
+    }
 };
 
 // A sorted uniform array held as a tile-wise running max: v_n =
@@ -203,6 +240,7 @@ struct InvArray {
     static constexpr bool kInv = true;
     static constexpr bool kWindow = true;
     static constexpr bool kWindowTmp = false;  // fill_window reads the array
+    static constexpr bool kPairF = false;
     const double *uscan;
     const double *tmax;  // exclusive running max of earlier tiles
     uint64_t nn;
@@ -255,6 +293,10 @@ struct FoldOut {
 constexpr int kFI = kFoldItems;
 constexpr int kFT = kFoldTile;
 constexpr int kFPad = kFT + kFT / kFI;
+// Uniforms per window of the windowed inversions: the whole LDS array but one
+// entry (stratified staging holds len + 1 draws), so a tile whose uniforms
+// exceed its 2048 elements by up to 12% still takes one window.
+constexpr int kWin = kFPad - 1;
 __device__ __forceinline__ int fidx(int e) { return e + e / kFI; }
 
 constexpr int kScanBlock = 1024;  // single-workgroup tile scans
@@ -657,12 +699,23 @@ __device__ void fold_tile_apply(FilterCtl *ctl, const XSrc &xs, const Inv &inv, 
             if (tid < kWave) {
                 uint64_t lo = 0, hi = 0;
                 if (M >= 2) {
-                    lo = t == 0 ? 0 : wave_first_greater(inv, v_start, 0, nn);
-                    hi = lo;
-                    if (base <= last_cmp) {
-                        const uint64_t il =
-                            (base + kFT - 1 < last_cmp) ? base + kFT - 1 : last_cmp;
-                        hi = wave_first_greater(inv, s_a[fidx((int)(il - base))], lo, nn);
+                    const bool need_hi = base <= last_cmp;
+                    const uint64_t il = (base + kFT - 1 < last_cmp) ? base + kFT - 1 : last_cmp;
+                    const double c_hi = need_hi ? s_a[fidx((int)(il - base))] : v_start;
+                    bool done = false;
+                    if constexpr (Inv::kPairF) {
+                        // F(c_hi) >= F(v_start) = lo for sorted values; the max
+                        // keeps the search's lower bound for any others
+                        uint64_t flo, fhi;
+                        if (inv.wave_F2(v_start, c_hi, &flo, &fhi)) {
+                            lo = t == 0 ? 0 : flo;
+                            hi = need_hi ? (fhi < lo ? lo : fhi) : lo;
+                            done = true;
+                        }
+                    }
+                    if (!done) {
+                        lo = t == 0 ? 0 : wave_first_greater(inv, v_start, 0, nn);
+                        hi = need_hi ? wave_first_greater(inv, c_hi, lo, nn) : lo;
                     }
                 }
                 if (tid == 0) {
@@ -675,8 +728,8 @@ __device__ void fold_tile_apply(FilterCtl *ctl, const XSrc &xs, const Inv &inv, 
             cstart = wlo;
 #pragma unroll
             for (int k = 0; k < kFI; ++k) c[k] = wlo;
-            for (uint64_t w0 = wlo; w0 < whi; w0 += kFT) {
-                const int len = (int)((whi - w0) < (uint64_t)kFT ? (whi - w0) : (uint64_t)kFT);
+            for (uint64_t w0 = wlo; w0 < whi; w0 += kWin) {
+                const int len = (int)((whi - w0) < (uint64_t)kWin ? (whi - w0) : (uint64_t)kWin);
                 __syncthreads();
                 inv.fill_window(s_x, s_tmp, w0, len);
                 __syncthreads();
