@@ -190,14 +190,6 @@ struct InvStratified {
         *b0 = e >= 2 ? e - 2 : 0;
         return true;
     }
-    __device__ uint64_t point_F(double c) const
-    {
-        if (nn == 0) return 0;
-        uint64_t n = cnt_floor(c * cnt_d(nn), nn);
-        while (n < nn && v(n) <= c) ++n;
-        while (n > 0 && v(n - 1) > c) --n;
-        return n;
-    }
     // F(c_lo) and F(c_hi) (first n with v_n > c, nn if none) in one pass of
     // the whole wave: lanes 0-31 take c_lo, lanes 32-63 c_hi.  Lane j of a
     // half draws u_m for m = b0 - 2 + j around the bracket of its value and
@@ -255,19 +247,6 @@ struct InvArray {
         for (int k = threadIdx.x; k < len; k += blockDim.x) s_v[k] = v(w0 + k);
     }
     __device__ bool bracket(double, uint64_t *) const { return false; }  // no closed form
-    __device__ uint64_t point_F(double c) const
-    {
-        // binary search for the first n with v_n > c
-        uint64_t lo = 0, hi = nn;
-        while (lo < hi) {
-            const uint64_t mid = lo + (hi - lo) / 2;
-            if (v(mid) <= c)
-                lo = mid + 1;
-            else
-                hi = mid;
-        }
-        return lo;
-    }
 };
 
 // ------------------------------------------------------------------------
@@ -332,48 +311,48 @@ __device__ __forceinline__ Seg thread_seg_prefix(const Seg &g, int fast_b, Lds &
     return block_excl_scan<kBlock>(g, SegOp(), seg_identity(), L.seg, (Seg *)nullptr);
 }
 
-// First n in [lo, hi) with v(n) > c (hi if none), v non-decreasing.  All 64
-// lanes of a wave probe 64 evenly spaced points per round (64-ary search).
+// F(c_lo) and F(c_hi) (first n in [0, nn) with v(n) > c, nn if none) by one
+// wave: lanes 0-31 run a 32-ary search for c_lo while lanes 32-63 run one for
+// c_hi, so the two searches share their rounds of dependent loads.
 template <typename Inv>
-__device__ uint64_t wave_first_greater(const Inv &inv, double c, uint64_t lo, uint64_t hi)
+__device__ void wave_first_greater2(const Inv &inv, double c_lo, double c_hi, uint64_t nn,
+                                    uint64_t *f_lo, uint64_t *f_hi)
 {
-    const int lane = threadIdx.x & 63;
-    uint64_t b0;
-    if (inv.bracket(c, &b0)) {
-        // one pass over b0-1 .. b0+62: v is non-decreasing, so if v(b0-1) <= c
-        // (or b0-1 lies below the range) the first lane above c is the answer
-        const uint64_t n = b0 + (uint64_t)lane;  // candidate + 1
-        bool gt;
-        if (n < lo + 1)
-            gt = false;
-        else if (n - 1 >= hi)
-            gt = true;
-        else
-            gt = inv.v(n - 1) > c;
-        const uint64_t mask = __ballot(gt);
-        if (mask && !(mask & 1)) {
-            const uint64_t r = b0 - 1 + (uint64_t)(__ffsll((unsigned long long)mask) - 1);
-            return r < lo ? lo : (r < hi ? r : hi);
+    const int lane = threadIdx.x & 63, j = lane & 31, sh = lane & 32;
+    const double c = sh ? c_hi : c_lo;
+    uint64_t lo = 0, hi = nn;  // the answer lies in [lo, hi]; the same in every lane of a half
+    bool done = false;
+    for (;;) {
+        const bool act = !done && hi - lo > 32;
+        if (!__ballot(act)) break;
+        const uint64_t step = act ? (hi - lo + 31) / 32 : 0;
+        const uint64_t n = lo + (uint64_t)j * step;
+        const bool gt = act && (n < hi ? inv.v(n) > c : true);
+        const uint32_t hm = (uint32_t)(__ballot(gt) >> sh);
+        if (act) {
+            const int g = hm ? __ffs(hm) - 1 : 32;
+            if (g == 0) {
+                hi = lo;
+                done = true;
+            } else {
+                const uint64_t nlo = lo + (uint64_t)(g - 1) * step + 1;
+                const uint64_t nhi = g < 32 ? lo + (uint64_t)g * step : hi;
+                lo = nlo;
+                hi = nhi < hi ? nhi : hi;
+            }
         }
     }
-    while (hi - lo > 64) {
-        const uint64_t step = (hi - lo + 63) / 64;
-        const uint64_t n = lo + (uint64_t)lane * step;
-        const bool gt = n < hi ? inv.v(n) > c : true;
-        const uint64_t mask = __ballot(gt);
-        const int g = mask ? __ffsll((unsigned long long)mask) - 1 : 64;
-        if (g == 0) return lo;
-        const uint64_t nlo = lo + (uint64_t)(g - 1) * step + 1;
-        const uint64_t nhi = g < 64 ? lo + (uint64_t)g * step : hi;
-        lo = nlo;
-        hi = nhi < hi ? nhi : hi;
+    // at most 32 candidates left in a half that is not done
+    const uint64_t n = lo + (uint64_t)j;
+    const bool gt = done || (n < hi ? inv.v(n) > c : true);
+    const uint32_t hm = (uint32_t)(__ballot(gt) >> sh);
+    uint64_t r = lo;
+    if (!done) {
+        r = hm ? lo + (uint64_t)(__ffs(hm) - 1) : hi;
+        r = r < hi ? r : hi;
     }
-    const uint64_t n = lo + (uint64_t)lane;
-    const bool gt = n < hi ? inv.v(n) > c : true;
-    const uint64_t mask = __ballot(gt);
-    if (mask == 0) return hi;  // all 64 remaining candidates are <= c
-    const uint64_t r = lo + (uint64_t)(__ffsll((unsigned long long)mask) - 1);
-    return r < hi ? r : hi;
+    *f_lo = __shfl(r, 0);
+    *f_hi = __shfl(r, 32);
 }
 
 // x of one tile into LDS (striped, coalesced); zero past the end, which
@@ -703,9 +682,9 @@ __device__ void fold_tile_apply(FilterCtl *ctl, const XSrc &xs, const Inv &inv, 
                     const uint64_t il = (base + kFT - 1 < last_cmp) ? base + kFT - 1 : last_cmp;
                     const double c_hi = need_hi ? s_a[fidx((int)(il - base))] : v_start;
                     bool done = false;
+                    // F(c_hi) >= F(v_start) = lo for sorted values; the max
+                    // keeps the lower bound lo for any others
                     if constexpr (Inv::kPairF) {
-                        // F(c_hi) >= F(v_start) = lo for sorted values; the max
-                        // keeps the search's lower bound for any others
                         uint64_t flo, fhi;
                         if (inv.wave_F2(v_start, c_hi, &flo, &fhi)) {
                             lo = t == 0 ? 0 : flo;
@@ -714,8 +693,10 @@ __device__ void fold_tile_apply(FilterCtl *ctl, const XSrc &xs, const Inv &inv, 
                         }
                     }
                     if (!done) {
-                        lo = t == 0 ? 0 : wave_first_greater(inv, v_start, 0, nn);
-                        hi = need_hi ? wave_first_greater(inv, c_hi, lo, nn) : lo;
+                        uint64_t flo, fhi;
+                        wave_first_greater2(inv, v_start, c_hi, nn, &flo, &fhi);
+                        lo = t == 0 ? 0 : flo;
+                        hi = need_hi ? (fhi < lo ? lo : fhi) : lo;
                     }
                 }
                 if (tid == 0) {
