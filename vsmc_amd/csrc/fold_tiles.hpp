@@ -823,9 +823,12 @@ template <typename Lds>
 __device__ void fold_tile_layout(FilterCtl *ctl, uint64_t t, uint64_t M, const uint32_t *rep32,
                                  uint64_t zsx, const FoldOut &out, Lds &L)
 {
-    // kFPad entries each, element e at fidx(e): the per-thread loops read
-    // consecutive entries, which the padding spreads over the LDS banks
+    // kFPad entries each, element e at fidx(e): the per-thread loop reads
+    // consecutive entries, which the padding spreads over the LDS banks.  Once
+    // every thread holds its counts in registers, s_rep takes the tile's zero
+    // slots in rank order instead (s_z), for striped stores of the zero list.
     uint32_t *const s_rep = reinterpret_cast<uint32_t *>(L.x);
+    uint32_t *const s_z = s_rep;
     uint32_t *const s_sp = reinterpret_cast<uint32_t *>(L.x) + kFPad;
     uint64_t *const s_u64 = L.u64;
     __syncthreads();  // the LDS of a previous tile is no longer read
@@ -839,34 +842,31 @@ __device__ void fold_tile_layout(FilterCtl *ctl, uint64_t t, uint64_t M, const u
         s_rep[fidx(e)] = i < M ? rep32[i] : 1u;
     }
     __syncthreads();
+    uint32_t rv[kFI];
     uint64_t zs_thr = 0;
-#pragma unroll 1
+#pragma unroll
     for (int k = 0; k < kFI; ++k) {
         const uint32_t r = s_rep[fidx(e0 + k)];
+        rv[k] = r;
         if (r == 0)
             zs_thr += 1ull << 32;
         else if (r >= 2)
             zs_thr += r - 1;
     }
     uint64_t zs_tile;
+    // (its barriers also end every read of s_rep before s_z is written)
     const uint64_t zs_excl =
         block_excl_scan<kBlock>(zs_thr, AddOp(), (uint64_t)0, s_u64, &zs_tile);
     const uint64_t pre = zsx;
-    uint64_t zrank = (pre >> 32) + (zs_excl >> 32);
+    uint32_t zl_local = (uint32_t)(zs_excl >> 32);   // zero slots before this thread (tile)
     uint64_t srank_local = zs_excl & 0xffffffffull;  // surplus before this thread (tile)
     const uint64_t s_base = pre & 0xffffffffull;
-#pragma unroll 1
+#pragma unroll
     for (int k = 0; k < kFI; ++k) {
         const int e = e0 + k;
         const uint64_t i = base + e;
-        const uint32_t r = s_rep[fidx(e)];
-        if (i < M && r == 0) {
-            if (zrank < M)
-                out.zl[zrank] = (uint32_t)i;
-            else
-                atomicOr(&ctl->err, (int)kErrBounds);
-            ++zrank;
-        }
+        const uint32_t r = rv[k];
+        if (i < M && r == 0) s_z[zl_local++] = (uint32_t)e;
         if (r >= 2) srank_local += r - 1;
         s_sp[fidx(e)] = (uint32_t)srank_local;
     }
@@ -877,6 +877,15 @@ __device__ void fold_tile_layout(FilterCtl *ctl, uint64_t t, uint64_t M, const u
         if (i < M) out.anc[i] = (uint32_t)i;
     }
     __syncthreads();
+    // the zero list: ranks [z_base, z_base + Z_tile), striped
+    const uint64_t z_base = pre >> 32;
+    const uint32_t Z_tile = (uint32_t)(zs_tile >> 32);
+    for (uint32_t k = tid; k < Z_tile; k += kBlock) {
+        if (z_base + k < M)
+            out.zl[z_base + k] = (uint32_t)(base + s_z[k]);
+        else
+            atomicOr(&ctl->err, (int)kErrBounds);
+    }
     // expand surplus copies: positions [s_base, s_base + S_tile)
     const uint32_t S_tile = (uint32_t)(zs_tile & 0xffffffffull);
     for (uint32_t k = tid; k < S_tile; k += kBlock) {
