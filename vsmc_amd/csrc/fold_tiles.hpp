@@ -107,10 +107,27 @@ struct InvSystematic {
     uint64_t nn;
     __device__ void bind(const FilterCtl *c) { u0 = c->u0; delta = c->delta; nn = c->nn; }
     __device__ double v(uint64_t n) const { return u0 + cnt_d(n) * delta; }
+    // [v(n) <= c] over n in [0, nn), 1 below the range and 0 above it: non-
+    // increasing in n, and F(c) is its first zero.
+    __device__ int le(int64_t n, double c) const
+    {
+        if (n < 0) return 1;
+        if (n >= (int64_t)nn) return 0;
+        return v((uint64_t)n) <= c;
+    }
+    // F(c) is floor((c - u0) nn) + 1 up to the roundings of that estimate
+    // and of v (a few units of 2^-53 on values <= 1, with N < 2^31).  So F
+    // is counted without branches over the three places from the estimate
+    // - 1 once the two places outside them confirm the bracket; anything
+    // else takes the search.
     __device__ uint64_t F(double c) const
     {
         if (nn == 0) return 0;
-        uint64_t n = cnt_floor((c - u0) * cnt_d(nn), nn);
+        const uint64_t n0 = cnt_floor((c - u0) * cnt_d(nn), nn);
+        const int64_t a = (int64_t)n0;
+        if (le(a - 2, c) && !le(a + 2, c))
+            return (uint64_t)(a - 1 + le(a - 1, c) + le(a, c) + le(a + 1, c));
+        uint64_t n = n0;
         while (n < nn && v(n) <= c) ++n;
         while (n > 0 && v(n - 1) > c) --n;
         return n;
