@@ -196,7 +196,17 @@ def test_filter_debug_step_ancestors(ctx, oracle, scheme):
     """Given the weights the device resampled from (and, for the sorted-uniform
     schemes, the uniforms it inverted), vSMC's sequential inversion and
     cfrp_trans give exactly the device's replication and ancestors."""
-    N, T = 20000, 6
+    _debug_step_ancestors(ctx, oracle, scheme, 20000, 6)
+
+
+@pytest.mark.parametrize("scheme", range(6))
+def test_filter_debug_step_ancestors_full_size(ctx, oracle, scheme):
+    """The same at the bench's N = 2^24 (8192 fold tiles), two resampling
+    steps: the second gathers through the first one's ancestors."""
+    _debug_step_ancestors(ctx, oracle, scheme, 1 << 24, 2)
+
+
+def _debug_step_ancestors(ctx, oracle, scheme, N, T):
     obs = _obs(T, 11)
     f = V.Filter(ctx, V.cv_desc(N, T, precision=V.MIXED, scheme=scheme, threshold=1.01))
     f.set_obs(obs)
@@ -206,7 +216,6 @@ def test_filter_debug_step_ancestors(ctx, oracle, scheme):
         assert d["resampled"]
         w = d["weight"]
         e0 = d["engine"]
-        oe = V.ThreefryState  # noqa: F841
         from oracle_ctypes import TF64Engine
         oeng = TF64Engine()
         for j in range(4):
