@@ -256,7 +256,8 @@ def test_filter_mixed_first_step_coupled(ctx, oracle):
 
 
 @pytest.mark.parametrize("N,T,scheme", [(1 << 16, 30, V.SYSTEMATIC), (1 << 16, 30, V.STRATIFIED),
-                                         (1 << 24, 10, V.SYSTEMATIC)])
+                                         (1 << 24, 10, V.SYSTEMATIC), (1 << 24, 10, V.STRATIFIED),
+                                         (1 << 24, 10, V.RESIDUAL), (1 << 24, 10, V.MULTINOMIAL)])
 def test_filter_mixed_log_evidence_coupled(ctx, oracle, N, T, scheme):
     """The mixed pipeline's log-evidence against FP64 arithmetic on the mixed
     pipeline's own particles.  Step 0: the FP64 likelihood at the mixed prior
