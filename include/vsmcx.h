@@ -91,6 +91,19 @@ int vsmcx_resample(vsmcx_ctx *ctx, int scheme, uint64_t M, uint64_t N,
                    vsmcx_threefry4x64_state *eng, const double *weight,
                    uint64_t *replication, uint64_t *copy_from);
 
+/* vsmcx_resample_noalloc: no counterpart in vSMC.  It is vsmcx_resample (same
+ * arguments, same results bit for bit) for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): it calls no hipMalloc / hipFree.  Every device buffer it
+ * uses is carved from one fixed array per GPU (about 26 MB, held from library
+ * load until the process ends), so M and N must be at most VSMCX_NOALLOC_MAX
+ * (VSMCX_ERR_ARG beyond it).  Calls on one GPU share that array: calls that
+ * overlap on one GPU (two contexts or two host threads) race. */
+#define VSMCX_NOALLOC_MAX (1ull << 18)
+int vsmcx_resample_noalloc(vsmcx_ctx *ctx, int scheme, uint64_t M, uint64_t N,
+                           vsmcx_threefry4x64_state *eng, const double *weight,
+                           uint64_t *replication, uint64_t *copy_from);
+
 /* Exact left fold out[i] = fl(out[i-1] + x[i]) of non-negative x, computed on
  * the device (the running sum inside include/vsmc/resample/common.hpp:76-77
  * and math::asum, include/vsmc/math/cblas.hpp:53-63).  Host pointers. */

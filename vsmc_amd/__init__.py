@@ -130,6 +130,7 @@ def lib():
         "vsmcx_threefry4x64_fill": ([vp, vp, vp, u64, vp], i32),
         "vsmcx_threefry4x32_fill": ([vp, vp, vp, u64, vp], i32),
         "vsmcx_resample": ([vp, i32, u64, u64, P(ThreefryState), vp, vp, vp], i32),
+        "vsmcx_resample_noalloc": ([vp, i32, u64, u64, P(ThreefryState), vp, vp, vp], i32),
         "vsmcx_fold_prefix": ([vp, u64, vp, vp], i32),
         "vsmcx_filter_create": ([vp, P(FilterDesc), P(vp)], i32),
         "vsmcx_filter_destroy": ([vp], i32),
@@ -274,15 +275,18 @@ class Context:
         return out
 
     # ---- resampling ----
-    def resample(self, scheme, weight, engine, N=None, copy_from=True):
-        """Resample<Scheme>(M, N, rng, w, rep) (+ cfrp_trans); engine advanced in place."""
+    def resample(self, scheme, weight, engine, N=None, copy_from=True, noalloc=False):
+        """Resample<Scheme>(M, N, rng, w, rep) (+ cfrp_trans); engine advanced in place.
+        noalloc: through vsmcx_resample_noalloc (M, N <= NOALLOC_MAX; no device
+        allocation in the call)."""
         w = np.ascontiguousarray(weight, np.float64)
         M = w.size
         N = M if N is None else N
         rep = np.zeros(M, np.uint64)
         cf = np.zeros(N, np.uint64) if copy_from else None
-        _check(lib().vsmcx_resample(self.h, scheme, M, N, C.byref(engine), w.ctypes.data,
-                                    rep.ctypes.data, _ptr(cf)))
+        fn = lib().vsmcx_resample_noalloc if noalloc else lib().vsmcx_resample
+        _check(fn(self.h, scheme, M, N, C.byref(engine), w.ctypes.data, rep.ctypes.data,
+                  _ptr(cf)))
         return rep, cf
 
     def fold_prefix(self, x):

@@ -17,41 +17,24 @@
 #include "host.hpp"
 #include "internal.hpp"
 #include "launch.hpp"
+#include "resample_layout.hpp"
 #include "rng_core.hpp"
 
 
 namespace vsmcx {
 
+// the context's budgeted allocator in the shape layout_resample takes
+struct CtxAlloc {
+    vsmcx_ctx *c;
+    template <class T>
+    T *n(uint64_t count) { return c->alloc_n<T>(count); }
+};
+
 // Fold / resample scratch for up to M particles (and nuni uniforms).
 void alloc_resample(vsmcx_ctx *c, ResampleBuffers &rb, uint64_t M, uint64_t nuni, bool lists)
 {
-    const uint64_t tiles = (M + kFoldTile - 1) / kFoldTile;  // fold tiles
-    const uint64_t utiles = (nuni + kTile - 1) / kTile + 1;   // sorted-uniform tiles
-    if (lists) {
-        rb.anc = c->alloc_n<uint32_t>(M);
-        rb.zl = c->alloc_n<uint32_t>(M);
-        rb.sl = c->alloc_n<uint32_t>(M);
-    }
-    rb.u_tsum = c->alloc_n<double>(utiles);
-    rb.u_texcl = c->alloc_n<double>(utiles);
-    rb.u_tmax = c->alloc_n<double>(utiles);
-    rb.u_tmaxex = c->alloc_n<double>(utiles);
-    rb.uscan = c->alloc_n<double>(nuni ? nuni : 1);
-    rb.nuni = nuni;
-    FoldScratch &fs = rb.fs;
-    fs.tsum = c->alloc_n<double>(tiles);
-    fs.ts = c->alloc_n<double>(tiles);
-    fs.agg = c->alloc_n<Seg>(tiles);
-    fs.pre = c->alloc_n<Seg>(tiles);
-    fs.spec = c->alloc_n<SpecRec>(tiles * kFoldTile);
-    fs.gval = c->alloc_n<double>(M + 1);
-    fs.rep32 = c->alloc_n<uint32_t>(M);
-    fs.zs = c->alloc_n<uint64_t>(tiles);
-    fs.zsx = c->alloc_n<uint64_t>(tiles);
-    fs.wx = c->alloc_n<double>(M);
-    fs.thr = c->alloc_n<Seg>(tiles * kBlock);
-    fs.thr_fb = c->alloc_n<int32_t>(tiles * kBlock);
-    fs.tiles = tiles;
+    CtxAlloc a{c};
+    layout_resample(a, rb, M, nuni, lists);
 }
 
 void free_resample(vsmcx_ctx *c, ResampleBuffers &rb)
@@ -110,6 +93,69 @@ void engine_out(const Tf64State &s, vsmcx_threefry4x64_state &e,
 
 
 }  // namespace
+
+namespace vsmcx_host {
+
+bool resample_args(vsmcx_ctx *ctx, int scheme, uint64_t M, uint64_t N,
+                   const vsmcx_threefry4x64_state *eng, const double *weight,
+                   const uint64_t *replication)
+{
+    REQUIRE(ctx && eng, "NULL argument");
+    REQUIRE(scheme_ok(scheme), "unknown resampling scheme");
+    if (M == 0) return false;
+    REQUIRE(weight && replication, "NULL weight/replication");
+    REQUIRE(M < (1ull << 31) && N < (1ull << 31), "sizes must be below 2^31");
+    for (uint64_t i = 0; i < M; ++i)
+        REQUIRE(weight[i] >= 0 && std::isfinite(weight[i]), "weights must be finite and >= 0");
+    return true;
+}
+
+void resample_on(vsmcx_ctx *ctx, int scheme, uint64_t M, uint64_t N,
+                 vsmcx_threefry4x64_state *eng, const double *weight, uint64_t *replication,
+                 uint64_t *copy_from, ResampleBuffers &rb, FilterCtl *ctl, double *w,
+                 uint64_t *cf_dev)
+{
+    const bool lists = copy_from && M == N;
+    FilterCtl h{};
+    h.n = N;
+    h.resample = 1;
+    h.scheme = scheme;
+    h.eng = engine_in(*eng);
+    VSMCX_HIP(hipMemcpyAsync(ctl, &h, sizeof(h), hipMemcpyHostToDevice, ctx->stream));
+    VSMCX_HIP(hipMemcpyAsync(w, weight, 8 * M, hipMemcpyHostToDevice, ctx->stream));
+    launch_resample_from_weights(ctl, w, M, scheme, rb, nullptr, ctx->stream);
+    VSMCX_HIP(hipGetLastError());
+    VSMCX_HIP(hipMemcpyAsync(&h, ctl, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
+    VSMCX_HIP(hipMemcpyAsync(replication, rb.rep_out, 8 * M, hipMemcpyDeviceToHost,
+                             ctx->stream));
+    std::vector<uint32_t> anc;
+    if (lists && cf_dev) {
+        // widened on the device, copied straight into the caller's array
+        launch_widen_u32(rb.anc, cf_dev, M, ctx->stream);
+        VSMCX_HIP(hipGetLastError());
+        VSMCX_HIP(hipMemcpyAsync(copy_from, cf_dev, 8 * M, hipMemcpyDeviceToHost, ctx->stream));
+    } else if (lists) {
+        anc.resize(M);
+        VSMCX_HIP(hipMemcpyAsync(anc.data(), rb.anc, 4 * M, hipMemcpyDeviceToHost, ctx->stream));
+    }
+    ctx->sync();
+    check_dev_err(h);
+    if (copy_from) {
+        if (lists) {
+            if (!cf_dev)
+                for (uint64_t i = 0; i < M; ++i) copy_from[i] = anc[i];
+        } else {  // cfrp_trans M != N branch: expansion (common.hpp:110-117)
+            uint64_t to = 0;
+            for (uint64_t from = 0; from < M; ++from)
+                for (uint64_t r = 0; r < replication[from] && to < N; ++r)
+                    copy_from[to++] = from;
+        }
+    }
+    const vsmcx_threefry4x64_state in = *eng;
+    engine_out(h.eng, *eng, &in);
+}
+
+}  // namespace vsmcx_host
 
 struct vsmcx_filter {
     vsmcx_ctx *ctx = nullptr;
@@ -201,6 +247,7 @@ int vsmcx_ctx_create(int device, uint64_t hbm_budget, vsmcx_ctx **out)
     c->device = device;
     c->budget = hbm_budget;
     VSMCX_HIP(hipSetDevice(device));
+    c->scratch = noalloc_scratch(device);
     VSMCX_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     *out = c.release();
     API_END
@@ -282,13 +329,7 @@ int vsmcx_resample(vsmcx_ctx *ctx, int scheme, uint64_t M, uint64_t N,
                    uint64_t *copy_from)
 {
     API_BEGIN
-    REQUIRE(ctx && eng, "NULL argument");
-    REQUIRE(scheme_ok(scheme), "unknown resampling scheme");
-    if (M == 0) return VSMCX_OK;
-    REQUIRE(weight && replication, "NULL weight/replication");
-    REQUIRE(M < (1ull << 31) && N < (1ull << 31), "sizes must be below 2^31");
-    for (uint64_t i = 0; i < M; ++i)
-        REQUIRE(weight[i] >= 0 && std::isfinite(weight[i]), "weights must be finite and >= 0");
+    if (!resample_args(ctx, scheme, M, N, eng, weight, replication)) return VSMCX_OK;
     const bool lists = copy_from && M == N;
     ResampleBuffers rb;
     FilterCtl *ctl = nullptr;
@@ -298,38 +339,7 @@ int vsmcx_resample(vsmcx_ctx *ctx, int scheme, uint64_t M, uint64_t N,
         rb.rep_out = ctx->alloc_n<uint64_t>(M);
         ctl = ctx->alloc_n<FilterCtl>(1);
         w = ctx->alloc_n<double>(M);
-        FilterCtl h{};
-        h.n = N;
-        h.resample = 1;
-        h.scheme = scheme;
-        h.eng = engine_in(*eng);
-        VSMCX_HIP(hipMemcpyAsync(ctl, &h, sizeof(h), hipMemcpyHostToDevice, ctx->stream));
-        VSMCX_HIP(hipMemcpyAsync(w, weight, 8 * M, hipMemcpyHostToDevice, ctx->stream));
-        launch_resample_from_weights(ctl, w, M, scheme, rb, nullptr, ctx->stream);
-        VSMCX_HIP(hipGetLastError());
-        VSMCX_HIP(hipMemcpyAsync(&h, ctl, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
-        VSMCX_HIP(hipMemcpyAsync(replication, rb.rep_out, 8 * M, hipMemcpyDeviceToHost,
-                                 ctx->stream));
-        std::vector<uint32_t> anc;
-        if (lists) {
-            anc.resize(M);
-            VSMCX_HIP(hipMemcpyAsync(anc.data(), rb.anc, 4 * M, hipMemcpyDeviceToHost,
-                                     ctx->stream));
-        }
-        ctx->sync();
-        check_dev_err(h);
-        if (copy_from) {
-            if (lists) {
-                for (uint64_t i = 0; i < M; ++i) copy_from[i] = anc[i];
-            } else {  // cfrp_trans M != N branch: expansion (common.hpp:110-117)
-                uint64_t to = 0;
-                for (uint64_t from = 0; from < M; ++from)
-                    for (uint64_t r = 0; r < replication[from] && to < N; ++r)
-                        copy_from[to++] = from;
-            }
-        }
-        const vsmcx_threefry4x64_state in = *eng;
-        engine_out(h.eng, *eng, &in);
+        resample_on(ctx, scheme, M, N, eng, weight, replication, copy_from, rb, ctl, w, nullptr);
     } catch (...) {
         free_resample(ctx, rb);
         ctx->release(ctl);

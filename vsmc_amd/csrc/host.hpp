@@ -65,6 +65,7 @@ using namespace vsmcx_host;
 struct vsmcx_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
+    void *scratch = nullptr;  // this device's fixed array of vsmcx_resample_noalloc
     uint64_t budget = 0;
     uint64_t allocated = 0;
     std::map<void *, size_t> sizes;
@@ -114,4 +115,23 @@ struct ResampleBuffers;
 void alloc_resample(vsmcx_ctx *c, ResampleBuffers &rb, uint64_t M, uint64_t nuni, bool lists);
 void free_resample(vsmcx_ctx *c, ResampleBuffers &rb);
 }  // namespace vsmcx
+
+namespace vsmcx_host {
+// The compat resample (vsmcx_resample's semantics) shared by its entry points
+// (capi.cpp).  resample_args checks the arguments (ArgError) and returns false
+// when there is nothing to do (M == 0).  resample_on runs it on buffers the
+// caller provides: rb laid out for (M, N, copy_from && M == N), rep_out set,
+// ctl one control block, w M doubles, and cf_dev either M uint64 (copy_from is
+// then widened on the device and copied straight out) or NULL.
+bool resample_args(vsmcx_ctx *ctx, int scheme, uint64_t M, uint64_t N,
+                   const vsmcx_threefry4x64_state *eng, const double *weight,
+                   const uint64_t *replication);
+void resample_on(vsmcx_ctx *ctx, int scheme, uint64_t M, uint64_t N,
+                 vsmcx_threefry4x64_state *eng, const double *weight, uint64_t *replication,
+                 uint64_t *copy_from, vsmcx::ResampleBuffers &rb, FilterCtl *ctl, double *w,
+                 uint64_t *cf_dev);
+// noalloc.cpp: the address of this device's fixed scratch array for
+// vsmcx_resample_noalloc, looked up once per device (the device must be current)
+void *noalloc_scratch(int device);
+}  // namespace vsmcx_host
 

@@ -506,4 +506,20 @@ void launch_fold_values(FilterCtl *ctl, const double *x, uint64_t M, ResampleBuf
                 FoldOut{nullptr, nullptr, nullptr, nullptr, rb.a_out}, st);
 }
 
+__global__ __launch_bounds__(kBlock) void k_widen_u32(const uint32_t *in, uint64_t *out,
+                                                      uint64_t n)
+{
+    for (uint64_t i = blockIdx.x * (uint64_t)kBlock + threadIdx.x; i < n;
+         i += (uint64_t)gridDim.x * kBlock)
+        out[i] = in[i];
+}
+
+void launch_widen_u32(const uint32_t *in, uint64_t *out, uint64_t n, hipStream_t st)
+{
+    if (!n) return;
+    const uint64_t g = (n + kBlock - 1) / kBlock;
+    hipLaunchKernelGGL(k_widen_u32, dim3((unsigned)(g < 2048 ? g : 2048)), dim3(kBlock), 0, st,
+                       in, out, n);
+}
+
 }  // namespace vsmcx

@@ -34,6 +34,8 @@ void launch_resample_from_weights(FilterCtl *ctl, const double *w, uint64_t M, i
                                   ResampleBuffers &rb, StepRow *hist, hipStream_t st);
 void launch_fold_values(FilterCtl *ctl, const double *x, uint64_t M, ResampleBuffers &rb,
                         hipStream_t st);
+// out[i] = in[i] for i < n (uint32 -> uint64)
+void launch_widen_u32(const uint32_t *in, uint64_t *out, uint64_t n, hipStream_t st);
 
 // kernels_rng.hip
 void launch_philox4x32_fill(uint32_t k0, uint32_t k1, const uint32_t ctr0[4], uint64_t nblocks,
