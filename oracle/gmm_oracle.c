@@ -205,17 +205,74 @@ static void alpha2sd(double alpha, double sd[3])
 }
 
 /* cess01 (move_alpha.hpp:203-211) -> WeightSet::compute_cess with use_log
- * false: the multipliers are exp(a_diff * llh) unshifted. */
-static double cess_at(size_t N, const double *W, const double *llh, double a_diff)
+ * false (include/vsmc/core/weight_set.hpp:411-431): the multipliers are
+ * b_i = exp(a_diff * llh_i) unshifted (move_alpha.hpp:256-262), and
+ * CESS = (sum_i W_i b_i)^2 / sum_i W_i b_i^2 accumulated in index order.
+ *
+ * ORC_PLANT_CESS (never set by oracle/Makefile) plants a plausible mistake,
+ * for the tests that show the golden vectors catch it:
+ *   1  the ESS of the reweighted weights instead of the CESS (ess01), with the
+ *      target (1 - drop) * ESS of the current weights (move_alpha.hpp:213-222)
+ *   2  use_log semantics: the multipliers exponentiated a second time
+ *   3  the search started at alpha + 0.1 instead of alpha + 0.05 */
+double orc_gmm_cess(size_t N, const double *W, const double *llh, double a_diff)
 {
     double above = 0, below = 0;
     for (size_t i = 0; i < N; ++i) {
-        const double b = exp(a_diff * llh[i]);
+        double b = exp(a_diff * llh[i]);
+#if defined(ORC_PLANT_CESS) && ORC_PLANT_CESS == 2
+        b = exp(b);
+#endif
         const double wb = W[i] * b;
         above += wb;
+#if defined(ORC_PLANT_CESS) && ORC_PLANT_CESS == 1
+        below += wb * wb;
+#else
         below += wb * b;
+#endif
     }
     return above * above / below;
+}
+
+/* alpha_ess<T, cess01<T>>::alpha_iter (move_alpha.hpp:242-279): bisection on
+ * [alpha, 1.05] from alpha + 0.05 until the interval is within
+ * e = max(1e-10, 1e-3 drop) or the lower end passes 1; then the clamp to
+ * [0, 1] and alpha_inc of gmm_state::alpha (gmm_state.hpp:163-172).  W are
+ * the current normalised weights.  Returns the number of CESS evaluations. */
+uint32_t orc_gmm_alpha_search(size_t N, const double *W, const double *llh, double alpha,
+                              double drop, double *alpha_out, double *alpha_inc_out)
+{
+    const double e = 1e-10 > 1e-3 * drop ? 1e-10 : 1e-3 * drop;
+    double target = 1 - drop;
+    double l = alpha, u = 1.05, a = l + 0.05;
+#if defined(ORC_PLANT_CESS) && ORC_PLANT_CESS == 3
+    a = l + 0.1;
+#endif
+#if defined(ORC_PLANT_CESS) && ORC_PLANT_CESS == 1
+    {
+        double ss = 0;
+        for (size_t i = 0; i < N; ++i) ss += W[i] * W[i];
+        target = (1 - drop) / ss;
+    }
+#endif
+    const double a_old = l;
+    uint32_t iters = 0;
+    while (fabs(u - l) > e && l <= 1) {
+        const double cess = orc_gmm_cess(N, W, llh, a - a_old);
+        ++iters;
+        if (cess < target) {
+            u = a;
+            a = 0.5 * (a + l);
+        } else {
+            l = a;
+            a = 0.5 * (a + u);
+        }
+    }
+    a = a < 1 ? a : 1;
+    a = a > 0 ? a : 0;
+    *alpha_inc_out = a > 0 ? a - alpha : 0;
+    *alpha_out = a;
+    return iters;
 }
 
 int orc_gmm_run(const orc_gmm_config *c, size_t N, double *params_out, double *llh_out,
@@ -275,29 +332,12 @@ int orc_gmm_run(const orc_gmm_config *c, size_t N, double *params_out, double *l
         rec[0] = r;
     }
     size_t t = 0;
-    const double e = 1e-10 > 1e-3 * c->cess_drop ? 1e-10 : 1e-3 * c->cess_drop;
-    const double ess_new = 1 - c->cess_drop;
     while (alpha < 1 && t < c->max_steps) {
         ++t;
         /* move_smc: alpha_iter (cess bisection) */
         if (alpha <= 0) zconst = 0;
-        double l = alpha, u = 1.05, a = l + 0.05;
-        const double a_old = l;
-        uint32_t iters = 0;
-        while (fabs(u - l) > e && l <= 1) {
-            const double cess = cess_at(N, W, llh, a - a_old);
-            ++iters;
-            if (cess < ess_new) {
-                u = a;
-                a = 0.5 * (a + l);
-            } else {
-                l = a;
-                a = 0.5 * (a + u);
-            }
-        }
-        a = a < 1 ? a : 1;
-        a = a > 0 ? a : 0;
-        const double a_inc = a > 0 ? a - alpha : 0;
+        double a, a_inc;
+        const uint32_t iters = orc_gmm_alpha_search(N, W, llh, alpha, c->cess_drop, &a, &a_inc);
         alpha = a;
         double sd[3];
         alpha2sd(alpha, sd);

@@ -144,6 +144,11 @@ class Oracle:
                                       C.c_uint32, C.c_double, f64p, f64p, f64p, C.c_size_t, f64p,
                                       C.POINTER(C.c_double), C.POINTER(C.c_double)]
         L.orc_gmm_mh_move.restype = C.c_int
+        L.orc_gmm_cess.argtypes = [C.c_size_t, f64p, f64p, C.c_double]
+        L.orc_gmm_cess.restype = C.c_double
+        L.orc_gmm_alpha_search.argtypes = [C.c_size_t, f64p, f64p, C.c_double, C.c_double,
+                                           C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        L.orc_gmm_alpha_search.restype = C.c_uint32
         L.orc_gmm_run.argtypes = [C.POINTER(GmmConfig), C.c_size_t, C.c_void_p, C.c_void_p,
                                   C.c_void_p, C.c_void_p, C.POINTER(C.c_size_t), C.c_int]
 
@@ -377,6 +382,20 @@ class Oracle:
                                        p, C.byref(a), C.byref(b))
         return acc, p, a.value, b.value
 
+    def gmm_cess(self, W, llh, a_diff):
+        """CESS of the reweighting by exp(a_diff * llh) of normalised weights W."""
+        W = np.ascontiguousarray(W, np.float64)
+        llh = np.ascontiguousarray(llh, np.float64)
+        return self.lib.orc_gmm_cess(W.size, W, llh, a_diff)
+
+    def gmm_alpha_search(self, W, llh, alpha, drop):
+        """Next temperature: (alpha, alpha_inc, CESS evaluations)."""
+        W = np.ascontiguousarray(W, np.float64)
+        llh = np.ascontiguousarray(llh, np.float64)
+        a, ai = C.c_double(), C.c_double()
+        it = self.lib.orc_gmm_alpha_search(W.size, W, llh, alpha, drop, C.byref(a), C.byref(ai))
+        return a.value, ai.value, int(it)
+
     def gmm_run(self, N, data, K=4, sweeps=5, seed=1, resample_seed=2, scheme=STRATIFIED,
                 threshold=0.5, cess_drop=0.1, max_steps=1000, threads=0, want_state=True):
         data = np.ascontiguousarray(data, np.float64)
@@ -419,6 +438,11 @@ class RefHarness:
         L.ref_cv_run.argtypes = [C.POINTER(CvParam), C.c_size_t, C.c_size_t, f64p, C.c_uint64,
                                  C.c_uint64, C.c_int, C.c_double, C.c_void_p, C.c_void_p,
                                  C.c_void_p]
+        L.ref_cess.argtypes = [C.c_size_t, f64p, f64p, C.c_int, f64p]
+        L.ref_cess.restype = C.c_double
+        L.ref_alpha_ess.argtypes = [C.c_size_t, f64p, f64p, C.c_double, C.c_double,
+                                    C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                    C.POINTER(C.c_uint64), f64p]
         L.ref_u01_32.argtypes = [u32p, C.c_size_t, f64p]
         L.ref_u01_64.argtypes = [u64p, C.c_size_t, f64p]
 
@@ -512,6 +536,25 @@ class RefHarness:
             incp = inc.ctypes.data
         ess = self.lib.ref_weightset(lw0.size, lw0, incp, lw, w)
         return lw, w, ess
+
+    def cess(self, lw, mul, use_log=False):
+        """WeightSet(N).set_log_weight(lw); cess(mul, use_log): (CESS, weights)."""
+        lw = np.ascontiguousarray(lw, np.float64)
+        mul = np.ascontiguousarray(mul, np.float64)
+        w = np.zeros_like(lw)
+        c = self.lib.ref_cess(lw.size, lw, mul, 1 if use_log else 0, w)
+        return c, w
+
+    def alpha_ess(self, lw, llh, alpha, drop):
+        """alpha_ess<T, cess01<T>>::alpha_iter from alpha: (alpha, alpha_inc, CESS
+        evaluations, weights)."""
+        lw = np.ascontiguousarray(lw, np.float64)
+        llh = np.ascontiguousarray(llh, np.float64)
+        w = np.zeros_like(lw)
+        a, ai, n = C.c_double(), C.c_double(), C.c_uint64()
+        self.lib.ref_alpha_ess(lw.size, lw, llh, alpha, drop, C.byref(a), C.byref(ai),
+                               C.byref(n), w)
+        return a.value, ai.value, int(n.value), w
 
     def cv_run(self, N, obs, seed, resample_seed, scheme=SYSTEMATIC, threshold=0.5, param=None):
         obs = np.ascontiguousarray(obs, np.float64).reshape(-1, 2)

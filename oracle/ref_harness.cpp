@@ -20,6 +20,7 @@
 #include <vsmc/rng/philox.hpp>
 #include <vsmc/rng/threefry.hpp>
 #include <vsmc/rng/u01.hpp>
+#include <vsmc/smp/backend_seq.hpp>
 
 #include <cmath>
 #include <cstdint>
@@ -27,6 +28,56 @@
 #include <functional>
 #include <random>
 #include <vector>
+
+// --- the gmm example's tempering search, driven on a harness value type ------
+// move_alpha.hpp (example/include) needs only vsmc::Particle<T> and the value
+// type's state(i, 0).log_likelihood(), state(0, 0).alpha() and alpha(a); the
+// gmm example's own gmm_state cannot be included here (its header pulls in
+// example/include/common.hpp, which includes core/sampler.hpp, which does not
+// compile with this toolchain).  The harness type below exposes exactly those
+// members; alpha(a) is the clamp and alpha_inc of gmm_state::alpha
+// (example/gmm/include/gmm_state.hpp:163-172).  Its weight set counts the
+// compute_cess calls (a virtual of vsmc::WeightSet), i.e. the bisection steps.
+#include <move_alpha.hpp>  // example/include, on the include path (oracle/Makefile)
+
+struct HarnessParam {
+    double ll = 0, a = 0, a_inc = 0;
+    double &log_likelihood() { return ll; }
+    double log_likelihood() const { return ll; }
+    double &alpha() { return a; }
+    double alpha() const { return a; }
+    double &alpha_inc() { return a_inc; }
+};
+
+class CountingWeightSet : public vsmc::WeightSet
+{
+    public :
+    explicit CountingWeightSet(size_type N) : vsmc::WeightSet(N), calls(0) {}
+    mutable std::size_t calls;
+    double compute_cess(const double *first, bool use_log) const
+    {
+        ++calls;
+        return vsmc::WeightSet::compute_cess(first, use_log);
+    }
+};
+
+class HarnessGmmState : public vsmc::StateSEQ<vsmc::StateMatrix<vsmc::RowMajor, 1, HarnessParam> >
+{
+    public :
+    typedef CountingWeightSet weight_set_type;
+    explicit HarnessGmmState(size_type N)
+        : vsmc::StateSEQ<vsmc::StateMatrix<vsmc::RowMajor, 1, HarnessParam> >(N) {}
+    void alpha(double a)
+    {
+        a = a < 1 ? a : 1;
+        a = a > 0 ? a : 0;
+        double a_inc = a > 0 ? a - this->state(0, 0).alpha() : 0;
+        for (size_type i = 0; i != this->size(); ++i) {
+            this->state(i, 0).alpha() = a;
+            this->state(i, 0).alpha_inc() = a_inc;
+        }
+    }
+};
 
 typedef vsmc::Philox4x32 Ph32;
 typedef vsmc::Threefry4x64 Tf64;
@@ -189,6 +240,39 @@ double ref_weightset(size_t N, const double *lw0, const double *inc, double *lw_
     ws.read_log_weight(lw_out);
     ws.read_weight(w_out);
     return ws.ess();
+}
+
+// --- WeightSet::cess (core/weight_set.hpp:161-186, 411-431) -----------------
+// weights from set_log_weight(lw) (read back into w_out); mul are the
+// multipliers (use_log 0) or their logarithms (use_log 1)
+double ref_cess(size_t N, const double *lw, const double *mul, int use_log, double *w_out)
+{
+    vsmc::WeightSet ws(N);
+    ws.set_log_weight(lw);
+    if (w_out) ws.read_weight(w_out);
+    return ws.cess(mul, use_log != 0);
+}
+
+// --- alpha_ess<T, cess01<T>>::alpha_iter (example/include/move_alpha.hpp:
+// 234-279) on a Particle<HarnessGmmState> with log-weights lw, per-particle
+// log-likelihoods llh and current temperature alpha; returns the new alpha,
+// alpha_inc, and the number of CESS evaluations; w_out: the weights it used
+void ref_alpha_ess(size_t N, const double *lw, const double *llh, double alpha, double drop,
+                   double *alpha_out, double *alpha_inc_out, uint64_t *cess_calls, double *w_out)
+{
+    vsmc::Particle<HarnessGmmState> particle(N);
+    particle.weight_set().set_log_weight(lw);
+    if (w_out) particle.weight_set().read_weight(w_out);
+    for (size_t i = 0; i < N; ++i) {
+        particle.value().state(i, 0).log_likelihood() = llh[i];
+        particle.value().state(i, 0).alpha() = alpha;
+    }
+    particle.weight_set().calls = 0;
+    alpha_ess<HarnessGmmState, cess01<HarnessGmmState> > search(drop);
+    search.alpha_iter(1, particle);
+    *alpha_out = particle.value().state(0, 0).alpha();
+    *alpha_inc_out = particle.value().state(0, 0).alpha_inc();
+    *cess_calls = particle.weight_set().calls;
 }
 
 // --- CV particle filter driven through vsmc::Particle<T> -------------------

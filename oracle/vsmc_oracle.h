@@ -172,6 +172,9 @@ int orc_gmm_mh_move(int K, int move, uint64_t seed, uint64_t id, uint32_t iter, 
 /* Whole run (initialize, then iterate until alpha = 1 or max_steps).
  * params_out: N x 3K, llh_out, lw_out: N (any may be NULL); rec: max_steps + 1
  * rows (row 0 = initialize); *rows = rows written. */
+double orc_gmm_cess(size_t N, const double *W, const double *llh, double a_diff);
+uint32_t orc_gmm_alpha_search(size_t N, const double *W, const double *llh, double alpha,
+                              double drop, double *alpha_out, double *alpha_inc_out);
 int orc_gmm_run(const orc_gmm_config *c, size_t N, double *params_out, double *llh_out,
                 double *lw_out, orc_gmm_record *rec, size_t *rows, int threads);
 

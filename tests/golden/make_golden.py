@@ -5,7 +5,8 @@ oracle/_ref/libvsmcref.so from it):  python tests/golden/make_golden.py
 
 Every output is data produced by vSMC's own templates (engines,
 uniform_real_distribution, U01Sequence*, Resample<Scheme>, cfrp_trans,
-WeightSet, and a Particle<T>-driven CV filter).  Inputs are seeded here so the
+WeightSet, a Particle<T>-driven CV filter, WeightSet::cess and the gmm
+example's alpha_ess<T, cess01<T>> tempering search).  Inputs are seeded here so the
 fixtures are reproducible.
 """
 import json
@@ -93,6 +94,33 @@ def main():
             out[f"cv_{scheme}_{f}"] = rec[f]
         out[f"cv_{scheme}_state"] = st
         out[f"cv_{scheme}_lw"] = lwf
+
+    # --- GMM tempering: WeightSet::cess and alpha_ess<T, cess01<T>>::alpha_iter ---
+    # (a generator of its own, so the vectors above do not move)
+    trng = np.random.default_rng(20261204)
+    tcases = []
+    for N in (1, 7, 1000, 20000):
+        for lw_kind in ("equal", "spread", "heavy"):
+            for llh_sd in (0.5, 30.0, 3000.0):
+                tcases.append((N, lw_kind, llh_sd))
+    tcases += [(1000, "spread", "underflow"), (20000, "heavy", "underflow")]
+    out["temper_ncases"] = np.array(len(tcases))
+    for ci, (N, lw_kind, llh_sd) in enumerate(tcases):
+        lw = (np.zeros(N) if lw_kind == "equal" else
+              trng.normal(size=N) * (2.0 if lw_kind == "spread" else 40.0))
+        if llh_sd == "underflow":  # every multiplier underflows once a_diff > 0.0075
+            llh = -1e5 - trng.random(N)
+        else:
+            llh = -500.0 + trng.normal(size=N) * llh_sd
+        alpha = float(trng.choice([0.0, 0.3, 0.95, 1 - 1e-7]))
+        drop = float(trng.choice([0.1, 0.5, 1e-9]))
+        a, ai, calls, w = r.alpha_ess(lw, llh, alpha, drop)
+        a_diff = float(trng.choice([1e-4, 0.01, 0.5]))
+        cess, _ = r.cess(lw, a_diff * llh, use_log=True)
+        out[f"temper_lw_{ci}"], out[f"temper_llh_{ci}"] = lw, llh
+        out[f"temper_w_{ci}"] = w
+        out[f"temper_in_{ci}"] = np.array([alpha, drop, a_diff])
+        out[f"temper_out_{ci}"] = np.array([a, ai, float(calls), cess])
 
     np.savez_compressed(os.path.join(HERE, "ref_vectors.npz"), **out)
     meta = {

@@ -271,3 +271,29 @@ def test_cv_move_many_is_cv_run_step(oracle):
     assert (s1 == st2).all()
     lw = inc0 + inc1
     np.testing.assert_allclose(lw - lw.max(), lw2, rtol=0, atol=1e-12)
+
+
+def test_cv_llh_is_the_student_t_density(oracle):
+    """The CV model's log-likelihood increment is the log density of two
+    independent Student-t(nu = 10, scale 0.1) coordinates, as scipy evaluates
+    it.  The reference's pf likelihood (example/pf/include/pf_smp.hpp:96-111)
+    is the same up to the per-coordinate constant lgamma((nu+1)/2) -
+    lgamma(nu/2) - log(nu pi)/2 - log(scale) (llh_const), which shifts log Z by
+    2 llh_const per step and no weight; this pins that constant and the
+    oracle's whole expression independently of both."""
+    from scipy import stats
+
+    import vsmc_amd as V
+    rng = np.random.default_rng(31)
+    st = rng.normal(size=(2000, 4)) * np.array([0.3, 0.3, 1.0, 1.0])
+    st[:10, :2] = 0.25  # at the observation: the density's peak
+    for ox, oy in ((0.25, 0.25), (-1.3, 2.0)):
+        got = oracle.cv_llh_many(st, ox, oy)
+        want = (stats.t.logpdf(st[:, 0] - ox, df=10, scale=0.1) +
+                stats.t.logpdf(st[:, 1] - oy, df=10, scale=0.1))
+        assert np.allclose(got, want, rtol=1e-13, atol=1e-12)
+    for nu, s in ((10.0, 0.1), (3.0, 2.5)):
+        want = float(stats.t.logpdf(0.0, df=nu, scale=s))
+        assert abs(V.cv_llh_const(nu, s) - want) < 1e-13
+        from oracle_ctypes import cv_param
+        assert abs(cv_param(nu=nu, obs_scale=s).llh_const - want) < 1e-13
