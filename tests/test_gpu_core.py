@@ -238,6 +238,55 @@ def _debug_step_ancestors(ctx, oracle, scheme, N, T):
     f.close()
 
 
+@pytest.mark.parametrize("scheme", [V.MULTINOMIAL, V.RESIDUAL])
+def test_filter_ancestors_from_seed_full_size(ctx, oracle, scheme):
+    """The sorted-uniform schemes at N = 2^24 from the seed: from the weights
+    the device resampled and the engine state before the resample, the oracle
+    runs vSMC's whole resampler (its own uniforms: the sequential FP64 fold
+    lmax += log(1 - U) / (nn - n) of rng/u01.hpp:185-187 with glibc log and
+    exp; then inversion and cfrp_trans).  The device evaluates the same fold
+    exactly, so the uniforms can differ only where its log or exp rounds
+    otherwise than glibc's: by at most one unit in the last place (about 4%
+    of them).  The replication counts and ancestors are equal over three
+    resampling steps: a one-ulp shift moves an ancestor only when a
+    cumulative weight falls between the two neighbouring doubles."""
+    N, T = 1 << 24, 3
+    obs = _obs(T, 11)
+    f = V.Filter(ctx, V.cv_desc(N, T, precision=V.MIXED, scheme=scheme, threshold=1.01))
+    f.set_obs(obs)
+    from oracle_ctypes import TF64Engine
+    for t in range(T):
+        say("from seed", scheme, t)
+        d = f.debug_step()
+        assert d["resampled"]
+        e0 = d["engine"]
+
+        def engine():
+            oe = TF64Engine()
+            for j in range(4):
+                oe.ctr[j] = e0.ctr[j]
+                oe.key[j] = e0.key[j]
+                oe.buf[j] = e0.buffer[j]
+            oe.index = e0.index
+            return oe
+
+        w = d["weight"]
+        nn = int(d["nn"])
+        u_dev = d["uniforms"][:nn]
+        u_ref, _ = oracle.u01seq(0, nn, engine=engine())
+        du = np.flatnonzero(u_dev != u_ref)
+        rep, _, _ = oracle.resample(scheme, w, engine=engine())
+        bad = np.flatnonzero(d["rep"] != rep)
+        say("from seed", scheme, t, "nn", nn, "uniforms differing", du.size,
+            "max |du|", float(np.abs(u_dev - u_ref).max()), "replication differing", bad.size)
+        assert bad.size == 0, (t, bad[:5])
+        assert (d["copy_from"] == oracle.cfrp_trans(rep)).all(), t
+        # one ulp of a value in [0.5, 1) at most
+        assert float(np.abs(u_dev - u_ref).max()) <= 2.0 ** -53, t
+        assert du.size <= nn // 10, du.size
+    f.close()
+
+
 def test_filter_mixed_first_step_coupled(ctx, oracle):
     """Before any resample has moved particles between slots the mixed pipeline
     sees the same particles as FP64: ESS within 1e-5 relative and log-evidence
@@ -257,7 +306,9 @@ def test_filter_mixed_first_step_coupled(ctx, oracle):
 
 @pytest.mark.parametrize("N,T,scheme", [(1 << 16, 30, V.SYSTEMATIC), (1 << 16, 30, V.STRATIFIED),
                                          (1 << 24, 10, V.SYSTEMATIC), (1 << 24, 10, V.STRATIFIED),
-                                         (1 << 24, 10, V.RESIDUAL), (1 << 24, 10, V.MULTINOMIAL)])
+                                         (1 << 24, 10, V.RESIDUAL), (1 << 24, 10, V.MULTINOMIAL),
+                                         (1 << 24, 10, V.RESIDUAL_STRATIFIED),
+                                         (1 << 24, 10, V.RESIDUAL_SYSTEMATIC)])
 def test_filter_mixed_log_evidence_coupled(ctx, oracle, N, T, scheme):
     """The mixed pipeline's log-evidence against FP64 arithmetic on the mixed
     pipeline's own particles.  Step 0: the FP64 likelihood at the mixed prior

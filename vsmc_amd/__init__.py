@@ -6,6 +6,7 @@ no CPU fallback: if the library is missing every entry point raises.
 import ctypes as C
 import math
 import os
+import weakref
 
 import numpy as np
 
@@ -221,14 +222,22 @@ def _ptr(a):
 
 
 class Context:
-    """One HIP stream on one device (vsmcx_ctx)."""
+    """One HIP stream on one device (vsmcx_ctx).  Objects created on it
+    (filters, batches, timers, ...) are closed before it, so none outlives the
+    context's device memory."""
 
     def __init__(self, device=0, budget=0):
         self.h = C.c_void_p()
+        self._children = weakref.WeakSet()
         _check(lib().vsmcx_ctx_create(device, budget, C.byref(self.h)))
+
+    def _adopt(self, obj):
+        self._children.add(obj)
 
     def close(self):
         if self.h:
+            for c in list(self._children):
+                c.close()
             lib().vsmcx_ctx_destroy(self.h)
             self.h = C.c_void_p()
 
@@ -301,6 +310,7 @@ class Timer:
 
     def __init__(self, ctx, n):
         self.h = C.c_void_p()
+        ctx._adopt(self)
         _check(lib().vsmcx_timer_create(ctx.h, n, C.byref(self.h)))
 
     def record(self, i):
@@ -348,6 +358,7 @@ class Filter:
         self.ctx = ctx
         self.desc = desc
         self.h = C.c_void_p()
+        ctx._adopt(self)
         _check(lib().vsmcx_filter_create(ctx.h, C.byref(desc), C.byref(self.h)))
 
     def close(self):
@@ -378,6 +389,7 @@ class Filter:
         f = cls.__new__(cls)
         f.ctx = ctx
         f.h = C.c_void_p()
+        ctx._adopt(f)
         _check(lib().vsmcx_filter_load(ctx.h, os.fsencode(path), C.byref(f.h)))
         f.desc = FilterDesc()
         _check(lib().vsmcx_filter_describe(f.h, C.byref(f.desc)))
@@ -498,6 +510,7 @@ class Batch:
                       prior_sd_pos=prior_sd_pos, prior_sd_vel=prior_sd_vel, obs_scale=obs_scale,
                       nu=nu, llh_const=cv_llh_const(nu, obs_scale))
         self.h = C.c_void_p()
+        ctx._adopt(self)
         _check(lib().vsmcx_batch_create(ctx.h, C.byref(d), C.byref(self.h)))
 
     def close(self):
@@ -616,6 +629,7 @@ class Serve:
                       llh_const=cv_llh_const(nu, obs_scale), lg_dim=self.lg_dim,
                       lg_a=self.lg_a.ctypes.data, lg_q=lg_q, lg_r=lg_r, lg_s0=lg_s0)
         self.h = C.c_void_p()
+        ctx._adopt(self)
         _check(lib().vsmcx_serve_create(ctx.h, C.byref(d), C.byref(self.h)))
 
     def close(self):
@@ -684,6 +698,7 @@ class State:
         self.ctx = ctx
         self.n, self.dim = n, dim
         self.h = C.c_void_p()
+        ctx._adopt(self)
         _check(lib().vsmcx_state_create(ctx.h, n, dim, precision, C.byref(self.h)))
 
     def close(self):
@@ -729,6 +744,7 @@ class Weights:
         self.ctx = ctx
         self.n = n
         self.h = C.c_void_p()
+        ctx._adopt(self)
         _check(lib().vsmcx_weights_create(ctx.h, n, scheme, threshold, resample_seed, max_steps,
                                           C.byref(self.h)))
 
@@ -786,6 +802,7 @@ class Gmm:
                     scheme=scheme, threshold=threshold, cess_drop=cess_drop, seed=seed,
                     resample_seed=resample_seed, max_steps=max_steps)
         self.h = C.c_void_p()
+        ctx._adopt(self)
         _check(lib().vsmcx_gmm_create(ctx.h, C.byref(d), self.data.ctypes.data, self.data.size,
                                       C.byref(self.h)))
 

@@ -39,7 +39,7 @@ void alloc_resample(vsmcx_ctx *c, ResampleBuffers &rb, uint64_t M, uint64_t nuni
 
 void free_resample(vsmcx_ctx *c, ResampleBuffers &rb)
 {
-    void *ps[] = {rb.anc, rb.zl, rb.sl, rb.rep_out, rb.a_out, rb.u_tsum, rb.u_texcl, rb.uscan,
+    void *ps[] = {rb.anc, rb.zl, rb.sl, rb.rep_out, rb.a_out, rb.ufs.tsum, rb.ufs.ts, rb.uscan,
                   rb.u_tmax, rb.u_tmaxex, rb.fs.tsum, rb.fs.ts, rb.fs.agg, rb.fs.pre, rb.fs.spec,
                   rb.fs.gval, rb.fs.rep32, rb.fs.zs, rb.fs.zsx, rb.fs.wx, rb.fs.thr,
                   rb.fs.thr_fb};

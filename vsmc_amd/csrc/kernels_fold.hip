@@ -239,11 +239,16 @@ __global__ void k_plain_prologue(FilterCtl *ctl, int scheme)
 }
 // ------------------------------------------------------------------------
 // sorted uniforms (U01SequenceSorted, u01.hpp:165-202):
-//   lmax_n = sum_{k<=n} log(1 - U_k) / (nn - k);  u_n = 1 - exp(lmax_n)
-// computed with a fixed reduce-then-scan order (deterministic), then held
-// as a tile-wise running max so the sequence is sorted by construction.
+//   lmax_n = fl(lmax_{n-1} + t_n),  t_n = log(1 - U_n) / (nn - n);  u_n = 1 - exp(lmax_n)
+// vSMC accumulates lmax as a sequential FP64 left fold.  The terms are <= 0
+// and round-to-nearest is symmetric, so -lmax_n is exactly the left fold of
+// the non-negative x_n = -t_n: the passes below evaluate that fold with the
+// exact-fold machinery (the same passes as the weight fold), so lmax is
+// vSMC's bit for bit whenever the terms are.  The uniforms are then held as a
+// tile-wise running max (the identity when exp is monotone), so the array is
+// sorted by construction.
 // ------------------------------------------------------------------------
-// The terms of n = n0 .. n0 + K - 1 (0 past nn).  Word n of the engine's
+// The terms x_n of n = n0 .. n0 + K - 1 (0 past nn).  Word n of the engine's
 // stream is word (n + index) mod 4 of the block at counter ctr + (n + index)/4
 // (the mapping of tf64_stream_word), so a run of consecutive n needs one
 // Threefry4x64-20 block per four words.  n0 is a multiple of 4 in every
@@ -263,79 +268,77 @@ __device__ __forceinline__ void sorted_terms(const Tf64State &e, uint64_t n0, ui
         const unsigned w = (unsigned)(g & 3);
         if (k == 0 || w == 0) b = threefry4x64_20(ctr_add(c0, g >> 2), e.key);
         const uint64_t word = w == 0 ? b.v[0] : w == 1 ? b.v[1] : w == 2 ? b.v[2] : b.v[3];
-        term[k] = n < nn ? log(1 - canonical64(word)) / (double)(nn - n) : 0.0;
+        // x_n = -t_n, the negation of vSMC's rounded quotient
+        term[k] = n < nn ? -(log(1 - canonical64(word)) / (double)(nn - n)) : 0.0;
     }
 }
 
-__global__ __launch_bounds__(kBlock) void k_sorted_tilesum(FilterCtl *ctl, double *tsum)
+// Pass 1 of the uniform fold, over fold tiles of the nuni-element capacity:
+// every x_n into ux (n < nuni; 0 from nn on), tile sums into tsum.  Each
+// thread takes kFI consecutive n (whole Threefry blocks), so the sum of a tile
+// is taken in another order than the weight fold's tile-sum pass; any fixed
+// order is within the classification's error bound.
+__global__ __launch_bounds__(kBlock) void k_sorted_terms(FilterCtl *ctl, double *ux, double *tsum,
+                                                         uint64_t nuni)
 {
     if (!ctl->resample) return;
     __shared__ double s_red[kBlock / kWave];
     const uint64_t nn = ctl->nn;
-    const uint64_t base = blockIdx.x * (uint64_t)kTile;
-    if (base >= nn) return;
-    const Tf64State e = ctl->eng;
-    double term[kItems];
-    sorted_terms<kItems>(e, base + threadIdx.x * (uint64_t)kItems, nn, term);
+    const uint64_t n0 = blockIdx.x * (uint64_t)kFT + threadIdx.x * (uint64_t)kFI;
+    double term[kFI];
+    sorted_terms<kFI>(ctl->eng, n0, nn, term);
     double s = 0;
 #pragma unroll
-    for (int k = 0; k < kItems; ++k) s += term[k];
+    for (int k = 0; k < kFI; ++k) {
+        if (n0 + k < nuni) ux[n0 + k] = term[k];
+        s += term[k];
+    }
     s = block_reduce<kBlock>(s, AddOp(), s_red);
     if (threadIdx.x == 0) tsum[blockIdx.x] = s;
 }
 
-// single workgroup: exclusive scan of tile sums (fixed order)
-__global__ __launch_bounds__(1024) void k_scan_tiles(FilterCtl *ctl, const double *tsum,
-                                                     double *texcl, uint64_t ntiles, int is_max)
+// single workgroup: exclusive running max of the tile maxima (fixed order)
+__global__ __launch_bounds__(1024) void k_scan_tiles_max(FilterCtl *ctl, const double *tmax,
+                                                         double *texcl, uint64_t ntiles)
 {
     if (!ctl->resample) return;
     __shared__ double s_red[1024 / kWave];
-    double carry = is_max ? -1.0 : 0.0;
+    double carry = -1.0;
     for (uint64_t b = 0; b < ntiles; b += 1024) {
         const uint64_t i = b + threadIdx.x;
-        double v = i < ntiles ? tsum[i] : (is_max ? -1.0 : 0.0);
+        const double v = i < ntiles ? tmax[i] : -1.0;
         double tot;
-        double incl = is_max ? block_incl_scan<1024>(v, MaxOp(), s_red, &tot)
-                             : block_incl_scan<1024>(v, AddOp(), s_red, &tot);
-        double ex = is_max ? (threadIdx.x == 0 ? -1.0 : 0.0) : incl - v;
-        if (is_max) {
-            double up = lane_up(incl, 1);
-            const int lane = threadIdx.x & 63, wid = threadIdx.x / 64;
-            // s_red holds per-wave inclusive maxima after the scan
-            ex = lane == 0 ? (wid > 0 ? s_red[wid - 1] : -1.0) : up;
-            ex = ex < carry ? carry : ex;
-        } else {
-            ex = carry + ex;
-        }
+        const double incl = block_incl_scan<1024>(v, MaxOp(), s_red, &tot);
+        // s_red holds per-wave inclusive maxima after the scan
+        double ex = lane_up(incl, 1);
+        const int lane = threadIdx.x & 63, wid = threadIdx.x / 64;
+        if (lane == 0) ex = wid > 0 ? s_red[wid - 1] : -1.0;
+        ex = ex < carry ? carry : ex;
         if (i < ntiles) texcl[i] = ex;
-        carry = is_max ? (carry < tot ? tot : carry) : carry + tot;
+        carry = carry < tot ? tot : carry;
         __syncthreads();
     }
 }
 
-// per tile: lmax = prefix + local inclusive prefix; u = 1 - exp(lmax);
-// local inclusive running max of u -> uscan; tile max -> tmaxv
-__global__ __launch_bounds__(kBlock) void k_sorted_values(FilterCtl *ctl, const double *texcl,
-                                                          double *uscan, double *tmaxv)
+// Per kTile tile of the exact fold values a_n = -lmax_n (in uscan): u_n =
+// 1 - exp(-a_n), the tile-local inclusive running max back into uscan (in
+// place: each thread reads its own elements before any write), tile max into
+// tmaxv.
+__global__ __launch_bounds__(kBlock) void k_sorted_values(FilterCtl *ctl, double *uscan,
+                                                          double *tmaxv)
 {
     if (!ctl->resample) return;
     __shared__ double s_red[kBlock / kWave];
     const uint64_t nn = ctl->nn;
     const uint64_t base = blockIdx.x * (uint64_t)kTile;
     if (base >= nn) return;
-    const Tf64State e = ctl->eng;
-    double term[kItems];
-    sorted_terms<kItems>(e, base + threadIdx.x * (uint64_t)kItems, nn, term);
-    double s = 0;
-#pragma unroll
-    for (int k = 0; k < kItems; ++k) s += term[k];
-    const double incl = block_incl_scan<kBlock>(s, AddOp(), s_red);
-    double lm = texcl[blockIdx.x] + (incl - s);
+    const uint64_t n0 = base + threadIdx.x * (uint64_t)kItems;
     double mx = -1.0;
     double uu[kItems];
+#pragma unroll
     for (int k = 0; k < kItems; ++k) {
-        lm += term[k];
-        const double u = 1 - exp(lm);
+        const uint64_t n = n0 + k;
+        const double u = n < nn ? 1 - exp(-uscan[n]) : -1.0;
         mx = mx < u ? u : mx;
         uu[k] = mx;
     }
@@ -346,8 +349,9 @@ __global__ __launch_bounds__(kBlock) void k_sorted_values(FilterCtl *ctl, const 
         const int lane = threadIdx.x & 63, wid = threadIdx.x / 64;
         if (lane == 0) mex = wid > 0 ? s_red[wid - 1] : -1.0;
     }
+#pragma unroll
     for (int k = 0; k < kItems; ++k) {
-        const uint64_t n = base + threadIdx.x * kItems + k;
+        const uint64_t n = n0 + k;
         if (n < nn) uscan[n] = uu[k] < mex ? mex : uu[k];
     }
     if (threadIdx.x == kBlock - 1) tmaxv[blockIdx.x] = minc;
@@ -447,15 +451,22 @@ static void resample_pipeline(FilterCtl *ctl, W w, uint64_t M, int scheme, Resam
     }
     const bool sorted = scheme == kMultinomial || scheme == kResidual;
     if (sorted) {
-        // uniforms: nn <= rb.nuni of them
-        const unsigned ut = (unsigned)((rb.nuni + kTile - 1) / kTile);
-        hipLaunchKernelGGL(k_sorted_tilesum, dim3(ut), dim3(kBlock), 0, st, ctl, rb.u_tsum);
-        hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(1024), 0, st, ctl, rb.u_tsum, rb.u_texcl,
-                           (uint64_t)ut, 0);
-        hipLaunchKernelGGL(k_sorted_values, dim3(ut), dim3(kBlock), 0, st, ctl, rb.u_texcl,
-                           rb.uscan, rb.u_tmax);
-        hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(1024), 0, st, ctl, rb.u_tmax, rb.u_tmaxex,
-                           (uint64_t)ut, 1);
+        // uniforms: nn <= rb.nuni of them.  The exact fold of the x_n over the
+        // capacity (zeros from nn on leave the fold unchanged), in place in
+        // uscan, with its own tile sums and starts (the residual asum fold's
+        // tile sums in fs.tsum are still to be read by the scaled fold)
+        FoldScratch &ufs = rb.ufs;
+        const uint64_t nu = rb.nuni;
+        const dim3 fgrid((unsigned)ufs.tiles);
+        hipLaunchKernelGGL(k_sorted_terms, fgrid, dim3(kBlock), 0, st, ctl, rb.uscan, ufs.tsum, nu);
+        hipLaunchKernelGGL(k_fold_scan_sum, dim3(1), dim3(kScanBlock), 0, st, ctl, ufs);
+        launch_fold_from_starts(ctl, XArray{rb.uscan}, InvNone(), nu, ufs,
+                                FoldOut{nullptr, nullptr, nullptr, nullptr, rb.uscan}, st);
+        const unsigned ut = (unsigned)((nu + kTile - 1) / kTile);
+        hipLaunchKernelGGL(k_sorted_values, dim3(ut), dim3(kBlock), 0, st, ctl, rb.uscan,
+                           rb.u_tmax);
+        hipLaunchKernelGGL(k_scan_tiles_max, dim3(1), dim3(1024), 0, st, ctl, rb.u_tmax,
+                           rb.u_tmaxex, (uint64_t)ut);
     }
     InvArray ia{rb.uscan, rb.u_tmaxex, 0};
     if (residual) {

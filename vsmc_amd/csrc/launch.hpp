@@ -14,13 +14,14 @@ struct ResampleBuffers {
     uint32_t *sl = nullptr;      // [M] surplus-copy list
     uint64_t *rep_out = nullptr; // [M] optional replication output
     double *a_out = nullptr;     // [M] optional exact fold output
-    double *u_tsum = nullptr;    // [tiles] sorted uniforms: tile sums
-    double *u_texcl = nullptr;   // [tiles] exclusive prefix
-    double *uscan = nullptr;     // [M] running-max uniforms within tile
-    double *u_tmax = nullptr;    // [tiles] tile maxima
-    double *u_tmaxex = nullptr;  // [tiles] exclusive running max of tiles
+    double *uscan = nullptr;     // [nuni] sorted uniforms: running max within kTile tiles
+    double *u_tmax = nullptr;    // [nuni / kTile] tile maxima
+    double *u_tmaxex = nullptr;  // [nuni / kTile] exclusive running max of tiles
     uint64_t nuni = 0;           // capacity of the uniform arrays
     FoldScratch fs{};
+    // the sorted uniforms' own fold: tile sums and starts of its own, every
+    // other array shared with fs (the two folds run one after the other)
+    FoldScratch ufs{};
 };
 
 

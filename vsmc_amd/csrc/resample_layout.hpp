@@ -22,17 +22,19 @@ constexpr uint64_t layout_round(uint64_t b)
 template <class A>
 constexpr void layout_resample(A &a, ResampleBuffers &rb, uint64_t M, uint64_t nuni, bool lists)
 {
-    const uint64_t tiles = (M + kFoldTile - 1) / kFoldTile;  // fold tiles
-    const uint64_t utiles = (nuni + kTile - 1) / kTile + 1;   // sorted-uniform tiles
+    // the fold scratch serves the weight fold (M elements) and the sorted
+    // uniforms' fold (nuni elements)
+    const uint64_t F = M > nuni ? M : nuni;
+    const uint64_t tiles = (F + kFoldTile - 1) / kFoldTile;    // fold tiles
+    const uint64_t utiles = (nuni + kFoldTile - 1) / kFoldTile; // fold tiles of the uniforms
+    const uint64_t mtiles = (nuni + kTile - 1) / kTile;         // running-max tiles
     if (lists) {
         rb.anc = a.template n<uint32_t>(M);
         rb.zl = a.template n<uint32_t>(M);
         rb.sl = a.template n<uint32_t>(M);
     }
-    rb.u_tsum = a.template n<double>(utiles);
-    rb.u_texcl = a.template n<double>(utiles);
-    rb.u_tmax = a.template n<double>(utiles);
-    rb.u_tmaxex = a.template n<double>(utiles);
+    rb.u_tmax = a.template n<double>(mtiles + 1);
+    rb.u_tmaxex = a.template n<double>(mtiles + 1);
     rb.uscan = a.template n<double>(nuni ? nuni : 1);
     rb.nuni = nuni;
     FoldScratch &fs = rb.fs;
@@ -41,14 +43,20 @@ constexpr void layout_resample(A &a, ResampleBuffers &rb, uint64_t M, uint64_t n
     fs.agg = a.template n<Seg>(tiles);
     fs.pre = a.template n<Seg>(tiles);
     fs.spec = a.template n<SpecRec>(tiles * kFoldTile);
-    fs.gval = a.template n<double>(M + 1);
+    fs.gval = a.template n<double>(F + 1);
     fs.rep32 = a.template n<uint32_t>(M);
     fs.zs = a.template n<uint64_t>(tiles);
     fs.zsx = a.template n<uint64_t>(tiles);
     fs.wx = a.template n<double>(M);
     fs.thr = a.template n<Seg>(tiles * kBlock);
     fs.thr_fb = a.template n<int32_t>(tiles * kBlock);
-    fs.tiles = tiles;
+    fs.tiles = (M + kFoldTile - 1) / kFoldTile;
+    FoldScratch &u = rb.ufs;
+    u = fs;
+    u.tsum = a.template n<double>(utiles + 1);
+    u.ts = a.template n<double>(utiles + 1);
+    u.wx = nullptr;
+    u.tiles = utiles;
 }
 
 // Counts the bytes a layout takes (every buffer rounded up to kLayoutAlign).
