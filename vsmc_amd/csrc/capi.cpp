@@ -613,15 +613,27 @@ int vsmcx_filter_save(vsmcx_filter *f, const char *path)
     auto hist = to_host<StepRow>(f->ctx, f->hist, f->T);
     f->ctx->sync();
     check_dev_err(ctl);
-    File out(path, "wb");
-    out.put(&h, sizeof(h));
-    out.put(&ctl, sizeof(ctl));
-    out.put(st.data(), st.size());
-    out.put(lw.data(), 8 * N);
-    out.put(anc.data(), 4 * N);
-    out.put(obs.data(), 8 * obs.size());
-    out.put(hist.data(), sizeof(StepRow) * hist.size());
-    out.close();
+    // written beside the target and renamed over it, so a failed write
+    // leaves an earlier checkpoint at `path` intact
+    const std::string tmp = std::string(path) + ".tmp";
+    try {
+        File out(tmp.c_str(), "wb");
+        out.put(&h, sizeof(h));
+        out.put(&ctl, sizeof(ctl));
+        out.put(st.data(), st.size());
+        out.put(lw.data(), 8 * N);
+        out.put(anc.data(), 4 * N);
+        out.put(obs.data(), 8 * obs.size());
+        out.put(hist.data(), sizeof(StepRow) * hist.size());
+        out.close();
+    } catch (...) {
+        std::remove(tmp.c_str());
+        throw;
+    }
+    if (std::rename(tmp.c_str(), path) != 0) {
+        std::remove(tmp.c_str());
+        throw DevError(VSMCX_ERR_INTERNAL, std::string("cannot rename the checkpoint to ") + path);
+    }
     API_END
 }
 
@@ -641,9 +653,21 @@ int vsmcx_filter_load(vsmcx_ctx *ctx, const char *path, vsmcx_filter **out)
     REQUIRE(h.state_bytes == (h.desc.precision == VSMCX_MIXED ? 16u : 32u),
             "checkpoint state size does not match its precision");
     REQUIRE(h.nobs <= h.desc.horizon && h.t <= h.nobs, "checkpoint counters out of range");
+    // sizes checked against the file before anything is allocated from them
+    REQUIRE(h.desc.n >= 1 && h.desc.n < (1ull << 31), "checkpoint particle count out of range");
+    REQUIRE(h.desc.horizon >= 1 && h.desc.horizon < (1ull << 32),
+            "checkpoint horizon out of range");
+    REQUIRE(scheme_ok(h.desc.scheme), "checkpoint of an unknown resampling scheme");
+    const uint64_t N = h.desc.n;
+    const uint64_t payload = sizeof(CkHeader) + sizeof(FilterCtl) + N * h.state_bytes + 12 * N +
+                             16 * h.nobs + sizeof(StepRow) * h.desc.horizon;
+    REQUIRE(std::fseek(in.fp, 0, SEEK_END) == 0, "cannot read the checkpoint");
+    const long fsize = std::ftell(in.fp);
+    REQUIRE(fsize >= 0 && (uint64_t)fsize == payload, "checkpoint file size does not match its header");
+    REQUIRE(std::fseek(in.fp, (long)sizeof(CkHeader), SEEK_SET) == 0, "cannot read the checkpoint");
     FilterCtl ctl;
     in.get(&ctl, sizeof(ctl));
-    const uint64_t N = h.desc.n;
+    REQUIRE(ctl.n == N, "checkpoint control block does not match its header");
     std::vector<unsigned char> st(N * h.state_bytes);
     std::vector<double> lw(N), obs(2 * h.nobs);
     std::vector<uint32_t> anc(N);
