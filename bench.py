@@ -3,13 +3,15 @@
 Workload (one bench "step"): one filter time step of six constant-velocity
 bootstrap particle filters, one per vSMC resampling scheme (multinomial,
 residual, stratified, systematic, residual-stratified, residual-systematic),
-each with N = 2^24 particles, FP32 state and likelihood, FP64 log-sum-exp,
-exact FP64 weight fold, resampling when ESS < N/2.  W warmup steps (the
-first is the init step) then K timed steps; W + K = 200 = T by default.
+run in turn on one HIP stream, each with N = 2^24 particles, FP32 state and
+likelihood, FP64 log-sum-exp, exact FP64 weight fold, resampling when
+ESS < N/2.  W warmup steps (the first is the init step) then K timed steps;
+W + K = 200 = T by default.
 
 value = 6 * N * K / (device time of the K steps), inputs resident in HBM.
 The same workload is also timed for the fused FP64 pipeline and for the
-un-fused FP64 pipeline (one kernel per vSMC operation).
+un-fused FP64 pipeline (one kernel per vSMC operation), and, as an extra,
+with the six filters on six concurrent streams.
 """
 import argparse
 import json
@@ -51,6 +53,15 @@ def step_bytes(dim, hist, first=0):
     prev = np.concatenate([[False], res[:-1]])
     b = 4 * dim + 8 + np.where(t > 0, 4 * dim + np.where(prev, 4, 8), 0) + np.where(res, 12, 0)
     return float(b[first:].sum())
+
+
+def c1_step_bytes(prec, hist, first):
+    """Algorithmic bytes per particle-update of the fused / plain step over the
+    steps first.. of a filter's history: 48 + 24 r (mixed), 80 + 24 r (fused
+    FP64), 168 + 120 r (plain FP64), r = 1 on a resampling step."""
+    base, per_res = {V.MIXED: (48, 24), V.FP64_FUSED: (80, 24), V.FP64_PLAIN: (168, 120)}[prec]
+    res = hist["resampled"][first:].astype(np.float64)
+    return float(np.sum(base + per_res * res))
 
 
 def whole_step_roofline(total_bytes, seconds, scope):
@@ -155,6 +166,9 @@ def roofline(prec, res, N, W):
             "bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(gbs / HBM_PEAK_GBS, 4),
             "traffic": counter_traffic("k_cv_move<float>") if prec == V.MIXED else None,
+            "traffic_source": ("FETCH_SIZE + WRITE_SIZE per launch from a separate rocprofv3 --pmc "
+                               "run of this workload, scaled by a 1 GiB copy "
+                               "(profiles/hbm_traffic_counters.json)") if prec == V.MIXED else None,
             "bytes_per_launch": round(tot_b / nl), "avg_launch_ms": round(tot_s * 1e3 / nl, 5)}
 
 
@@ -332,39 +346,68 @@ def run_gmm(N, sweeps, seed, precision=V.MIXED, K=4, ndata=1000):
     return out
 
 
-def cpu_gmm_baseline(N, sweeps, seed, threads, K=4, ndata=1000):
+def host_cpu():
+    """(threads available to this process -- what nproc reports --, CPU model)."""
+    try:
+        threads = len(os.sched_getaffinity(0))
+    except AttributeError:
+        threads = os.cpu_count() or 1
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return threads, model
+
+
+def cpu_gmm_baseline(N, sweeps, seed, threads, iters, K=4, ndata=1000):
+    """The GMM oracle's MH-evaluation rate over the first `iters` iterations
+    at N particles (the rate does not depend on the iteration count)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     from oracle_ctypes import Oracle
     o = Oracle()
     y = V.gmm_simulate(ndata, seed)
     t0 = time.perf_counter()
     _, _, _, rec = o.gmm_run(N, y, K=K, sweeps=sweeps, seed=seed + 1, resample_seed=seed + 2,
-                             threads=threads, want_state=False)
+                             threads=threads, want_state=False, max_steps=iters)
     dt = time.perf_counter() - t0
-    iters = len(rec) - 1
-    return {"value": float(N) * 3 * sweeps * iters / dt, "unit": "mh-evaluations/s",
-            "cores": threads, "kind": "port",
+    done = len(rec) - 1
+    return {"value": float(N) * 3 * sweeps * done / dt, "unit": "mh-evaluations/s",
+            "cores": threads, "kind": "port", "cpu": host_cpu()[1],
             "sample": f"GMM oracle (oracle/gmm_oracle.c, FP64, OpenMP over particles), N={N}, "
-                      f"{ndata} points, {sweeps} sweeps, whole run to alpha = 1 ({iters} "
-                      f"iterations)",
-            "seconds": round(dt, 3), "log_z": float(rec["log_z"][-1])}
+                      f"{ndata} points, {sweeps} sweeps, the first {done} iterations from the "
+                      f"prior (initialisation included)",
+            "seconds": round(dt, 3)}
 
 
 def cpu_baseline(N, steps, threads):
+    """vSMC's CPU step on the host cores, once on each of its SMP backends:
+    the move (and the gather) over all threads, normalisation and resampling
+    sequential as in vSMC.  value: the faster backend."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     from oracle_ctypes import Oracle
     o = Oracle()
     obs = V.cv_simulate(steps, 7)
-    t0 = time.perf_counter()
-    rec, phase = o.cv_run_timed(N, obs, 42, 4242, V.SYSTEMATIC, 0.5, threads=threads)
-    dt = time.perf_counter() - t0
-    return {"value": N * steps / dt, "unit": "particle-updates/s", "cores": threads,
-            "kind": "port",
-            "sample": f"vSMC-semantics CPU path (oracle/vsmc_oracle.c: OpenMP move, sequential "
-                      f"normalise/inversion/cfrp_trans/copy), N={N}, {steps} steps, systematic",
-            "seconds": round(dt, 3),
-            "phase_s": {"move": round(phase[0], 3), "normalise": round(phase[1], 3),
-                        "resample": round(phase[2], 3), "copy": round(phase[3], 3)}}
+    out = {}
+    for backend in ("omp", "threads"):
+        t0 = time.perf_counter()
+        rec, phase = o.cv_run_timed(N, obs, 42, 4242, V.SYSTEMATIC, 0.5, threads=threads,
+                                    backend=backend)
+        dt = time.perf_counter() - t0
+        out[backend] = {"value": N * steps / dt, "seconds": round(dt, 3),
+                        "phase_s": {"move": round(phase[0], 3), "normalise": round(phase[1], 3),
+                                    "resample": round(phase[2], 3), "copy": round(phase[3], 3)}}
+    best = max(out, key=lambda b: out[b]["value"])
+    return {"value": out[best]["value"], "unit": "particle-updates/s", "cores": threads,
+            "kind": "port", "cpu": host_cpu()[1], "backend": best,
+            "sample": f"vSMC-semantics CPU path (oracle/vsmc_oracle.c: move and gather over "
+                      f"{threads} threads, sequential normalise/inversion/cfrp_trans), N={N}, "
+                      f"{steps} steps, systematic, on vSMC's OpenMP and std::thread backends",
+            "backends": out}
 
 
 def main():
@@ -385,12 +428,14 @@ def main():
     ap.add_argument("--batch-filters", type=int, default=4096)
     ap.add_argument("--no-gmm", action="store_true")
     ap.add_argument("--serial-filters", action="store_true",
-                    help="config 1: advance the six filters in turn on one stream")
+                    help="config 1: only the six filters in turn on one stream (the headline "
+                         "workload; skips the extra run on six concurrent streams)")
     ap.add_argument("--gmm-n", type=int, default=1 << 20)
     ap.add_argument("--gmm-sweeps", type=int, default=5)
     ap.add_argument("--gmm-fp64", action="store_true",
                     help="also run the GMM workload in FP64 at the same size")
-    ap.add_argument("--gmm-cpu-n", type=int, default=4096)
+    ap.add_argument("--gmm-cpu-n", type=int, default=1 << 16)
+    ap.add_argument("--gmm-cpu-iters", type=int, default=3)
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a device-to-device copy of 1 GiB five times (calibrates the "
                          "FETCH_SIZE / WRITE_SIZE counters against known bytes) and exit")
@@ -422,21 +467,35 @@ def main():
     for prec in precs:
         print(f"[bench] precision {prec}: {len(schemes)} filters x N={N}, W={W} K={K}",
               file=sys.stderr, flush=True)
+        # the six filters in turn on one stream: C1's workload
         res[prec] = run_precision(ctx, prec, N, W, K, obs[:W + K], schemes, profile=False,
-                                  concurrent=not a.serial_filters)
-        # the move kernel's roofline from a short run with the filters in turn
-        # (no other filter's kernels beside it)
+                                  concurrent=False)
+        # the move kernel's roofline from a short run with every move launch
+        # bracketed by HIP events
         serial[prec] = run_precision(ctx, prec, N, 2, ks, obs[:2 + ks], schemes, profile=True,
                                      concurrent=False)
+    conc = None
+    if not a.serial_filters:
+        print(f"[bench] concurrent streams: {len(schemes)} filters x N={N}", file=sys.stderr,
+              flush=True)
+        conc = run_precision(ctx, V.MIXED, N, W, K, obs[:W + K], schemes, profile=False,
+                             concurrent=True)
     ctx.close()
 
     nf = len(schemes)
     mixed = res[V.MIXED]
     value = nf * N * K / (mixed["total_ms"] * 1e-3)
-    # a bench step: every filter one step; concurrently, the slowest filter's
-    # step bounds it, in turn the sum
-    steps_ms = (mixed["per_filter_step_ms"].max(axis=1) if mixed["concurrent"]
-                else mixed["per_filter_step_ms"].sum(axis=1))
+    # a bench step: every filter one step, in turn
+    steps_ms = mixed["per_filter_step_ms"].sum(axis=1)
+
+    def step_roofline(prec, r):
+        b = sum(N * c1_step_bytes(prec, h, W) for h in r["hist"])
+        return whole_step_roofline(
+            b, r["total_ms"] * 1e-3, "every kernel of the K timed steps of the six filters: "
+            "algorithmic bytes per particle-update " +
+            {V.MIXED: "48 + 24 r", V.FP64_FUSED: "80 + 24 r",
+             V.FP64_PLAIN: "168 + 120 r"}[prec] + " (r = 1 on a resampling step)")
+
     out = {
         "metric": METRIC, "value": value, "unit": "particle-updates/s", "n_gpus": 1,
         "steps": K, "warmup": W, "ms_per_step": mixed["total_ms"] / K,
@@ -444,30 +503,38 @@ def main():
         "data": "synthetic (observations simulated from the CV model, seed 11)",
         "config": {"workload": "config-1: CV bootstrap filter, N=2^24 per filter, six filters "
                                "(one per resampling scheme) advanced one step each per bench "
-                               "step, " + ("each on its own HIP stream" if mixed["concurrent"]
-                                           else "in turn on one stream") +
-                               ", ESS<N/2 resampling, T=W+K",
+                               "step, in turn on one HIP stream, ESS<N/2 resampling, T=W+K",
                    "n": N, "filters_per_step": nf,
                    "schemes": [SCHEME_NAMES[s] for s in schemes], "horizon": W + K},
         "p50_step_ms": float(np.percentile(steps_ms, 50)),
         "p99_step_ms": float(np.percentile(steps_ms, 99)),
         "roofline": roofline(V.MIXED, serial[V.MIXED], N, 2),
+        "whole_step_roofline": step_roofline(V.MIXED, mixed),
     }
-    per_scheme = {}  # each scheme's step on its own (the short run with the filters in turn)
+    per_scheme = {}  # each scheme's step within the in-turn run
     for j, s in enumerate(schemes):
-        ms = serial[V.MIXED]["per_filter_step_ms"][:, j]
+        ms = mixed["per_filter_step_ms"][:, j]
         h = mixed["hist"][j]
         per_scheme[SCHEME_NAMES[s]] = {
             "updates_per_s": N * len(ms) / (ms.sum() * 1e-3),
             "p50_ms": float(np.percentile(ms, 50)), "p99_ms": float(np.percentile(ms, 99)),
             "resample_frac": float(h["resampled"][W:].mean()), "log_z": float(h["log_z"][-1])}
     out["per_scheme"] = per_scheme
+    if conc is not None:
+        cms = conc["per_filter_step_ms"].max(axis=1)
+        out["concurrent_streams"] = {
+            "value": nf * N * K / (conc["total_ms"] * 1e-3), "ms_per_step": conc["total_ms"] / K,
+            "p50_step_ms": float(np.percentile(cms, 50)),
+            "p99_step_ms": float(np.percentile(cms, 99)),
+            "note": "the same six filters, each on its own HIP stream (host wall-clock between "
+                    "full synchronisations); not C1's in-turn workload"}
     for prec, name in ((V.FP64_FUSED, "fp64_fused"), (V.FP64_PLAIN, "fp64_plain")):
         if prec in res:
             r = res[prec]
             v = nf * N * K / (r["total_ms"] * 1e-3)
             out[name] = {"value": v, "ms_per_step": r["total_ms"] / K,
                          "roofline": roofline(prec, serial[prec], N, 2),
+                         "whole_step_roofline": step_roofline(prec, r),
                          "log_z": {SCHEME_NAMES[s]: float(r["hist"][j]["log_z"][-1])
                                    for j, s in enumerate(schemes)}}
     if V.FP64_PLAIN in res:
@@ -487,17 +554,17 @@ def main():
         if a.gmm_fp64:
             gm["fp64"] = run_gmm(a.gmm_n, a.gmm_sweeps, 23, precision=V.FP64_FUSED)
         if not a.no_cpu:
-            gm["cpu_baseline"] = cpu_gmm_baseline(a.gmm_cpu_n, a.gmm_sweeps, 23,
-                                                  min(16, os.cpu_count() or 1))
+            gm["cpu_baseline"] = cpu_gmm_baseline(a.gmm_cpu_n, a.gmm_sweeps, 23, host_cpu()[0],
+                                                  a.gmm_cpu_iters)
         gm["config"] = ("config-2: K=4 GMM, 1000 points from means (-3, 0, 2, 5), sds U[0.5, 1.5], "
                         f"N={a.gmm_n} x 12 FP32 parameters, CESS 0.9 tempering, "
                         f"{a.gmm_sweeps} RW-MH sweeps (mu, lambda, weight) per iteration, "
                         "stratified resampling at ESS < N/2, mixed precision")
         out["gmm"] = gm
     if not a.no_cpu:
-        threads = min(16, os.cpu_count() or 1)
-        print(f"[bench] cpu baseline: N={a.cpu_n}, {a.cpu_steps} steps, {threads} threads",
-              file=sys.stderr, flush=True)
+        threads, model = host_cpu()
+        print(f"[bench] cpu baseline: N={a.cpu_n}, {a.cpu_steps} steps, {threads} threads "
+              f"({model})", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(a.cpu_n, a.cpu_steps, threads)
     print(json.dumps(out))
 

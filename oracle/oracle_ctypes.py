@@ -133,7 +133,7 @@ class Oracle:
                                  C.c_void_p, C.c_int]
         L.orc_cv_run_timed.argtypes = [C.POINTER(CvParam), C.c_size_t, C.c_size_t, f64p,
                                        C.c_uint64, C.c_uint64, C.c_int, C.c_double, C.c_void_p,
-                                       C.c_int, f64p]
+                                       C.c_int, C.c_int, f64p]
         L.orc_gmm_prior.argtypes = [f64p, C.c_size_t, f64p]
         L.orc_gmm_log_prior.argtypes = [C.c_int, f64p, f64p, f64p]
         L.orc_gmm_log_prior.restype = C.c_double
@@ -337,14 +337,17 @@ class Oracle:
         return st, lw, rec
 
     def cv_run_timed(self, N, obs, seed, resample_seed, scheme=SYSTEMATIC, threshold=0.5,
-                     param=None, threads=0):
+                     param=None, threads=0, backend="omp"):
+        """The CPU path timed per phase (move, normalise, resample, copy) on
+        vSMC's OpenMP backend ("omp") or std::thread backend ("threads")."""
         obs = np.ascontiguousarray(obs, np.float64).reshape(-1, 2)
         T = obs.shape[0]
         p = param if param is not None else cv_param()
         rec = np.zeros(T, STEP_DTYPE)
         phase = np.zeros(4)
         rc = self.lib.orc_cv_run_timed(C.byref(p), N, T, obs, seed, resample_seed, scheme,
-                                       threshold, rec.ctypes.data, threads, phase)
+                                       threshold, rec.ctypes.data, threads,
+                                       1 if backend == "threads" else 0, phase)
         if rc != 0:
             raise MemoryError("orc_cv_run_timed failed")
         return rec, phase

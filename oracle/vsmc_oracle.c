@@ -11,6 +11,8 @@
  */
 #include "vsmc_oracle.h"
 
+#include <pthread.h>
+
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
@@ -571,25 +573,119 @@ static double orc_now(void)
 #endif
 }
 
+/* The two vSMC SMP backends the CPU path can run the move and the copy on.
+ * OpenMP: MoveOMP / StateOMP::copy, one `parallel for` over the particles
+ * (include/vsmc/smp/backend_omp.hpp:58-66, 104-117).  std::thread: MoveSTD /
+ * StateSTD::copy (include/vsmc/smp/backend_std.hpp:54-62, 98-109) through
+ * vsmc::parallel_for / parallel_reduce, which split [0, N) with
+ * ThreadNum::partition (include/vsmc/thread/thread_num.hpp:64-89: blocks of
+ * ceil(N / threads), or 1 when N < threads) and start one thread per block,
+ * joined before the call returns (include/vsmc/thread/parallel_for.hpp:52-63);
+ * here POSIX threads do the same. */
+typedef struct {
+    const orc_cv_param *p;
+    double *st, *inc;
+    const uint64_t *cf;
+    uint64_t seed;
+    size_t t;
+    double ox, oy;
+    int what; /* 0 move, 1 copy */
+} orc_cv_work;
+
+static void orc_cv_move_range(const orc_cv_work *w, size_t b, size_t e)
+{
+    const orc_cv_param *p = w->p;
+    for (size_t i = b; i < e; ++i) {
+        double z[4];
+        orc_cv_normals(w->seed, i, w->t, z);
+        double *s = w->st + 4 * i;
+        if (w->t == 0) {
+            s[0] = p->prior_sd_pos * z[0];
+            s[1] = p->prior_sd_pos * z[1];
+            s[2] = p->prior_sd_vel * z[2];
+            s[3] = p->prior_sd_vel * z[3];
+        } else {
+            s[0] += p->sd_pos * z[0] + p->dt * s[2];
+            s[1] += p->sd_pos * z[1] + p->dt * s[3];
+            s[2] += p->sd_vel * z[2];
+            s[3] += p->sd_vel * z[3];
+        }
+        w->inc[i] = orc_cv_llh(p, s[0], s[1], w->ox, w->oy);
+    }
+}
+
+/* StateMatrix<RowMajor>::copy (core/state_matrix.hpp:285-300): in place is
+ * safe in parallel, since cfrp_trans never reads a slot it overwrites */
+static void orc_cv_copy_range(const orc_cv_work *w, size_t b, size_t e)
+{
+    for (size_t to = b; to < e; ++to) {
+        const size_t from = (size_t)w->cf[to];
+        if (from != to) memcpy(w->st + 4 * to, w->st + 4 * from, sizeof(double) * 4);
+    }
+}
+
+typedef struct {
+    const orc_cv_work *w;
+    size_t b, e;
+} orc_cv_block;
+
+static void *orc_cv_block_run(void *arg)
+{
+    const orc_cv_block *k = (const orc_cv_block *)arg;
+    if (k->w->what == 0)
+        orc_cv_move_range(k->w, k->b, k->e);
+    else
+        orc_cv_copy_range(k->w, k->b, k->e);
+    return NULL;
+}
+
+static void orc_cv_parallel(const orc_cv_work *w, size_t N, int threads, int backend)
+{
+    if (backend == ORC_BACKEND_THREADS && threads > 1 && N > 1) {
+        const size_t tn = (size_t)threads;
+        const size_t bs = N < tn ? 1 : (N % tn ? N / tn + 1 : N / tn);
+        pthread_t tid[ORC_MAX_THREADS];
+        orc_cv_block blk[ORC_MAX_THREADS];
+        size_t nb = 0;
+        for (size_t b = 0; b < N && nb < ORC_MAX_THREADS; b += bs, ++nb) {
+            blk[nb].w = w;
+            blk[nb].b = b;
+            blk[nb].e = b + bs < N ? b + bs : N;
+            if (pthread_create(&tid[nb], NULL, orc_cv_block_run, &blk[nb]) != 0) {
+                orc_cv_block_run(&blk[nb]);
+                tid[nb] = 0;
+            }
+        }
+        for (size_t k = 0; k < nb; ++k)
+            if (tid[k]) pthread_join(tid[k], NULL);
+        return;
+    }
+#pragma omp parallel for schedule(static)
+    for (long long ii = 0; ii < (long long)N; ++ii) {
+        if (w->what == 0)
+            orc_cv_move_range(w, (size_t)ii, (size_t)ii + 1);
+        else
+            orc_cv_copy_range(w, (size_t)ii, (size_t)ii + 1);
+    }
+}
+
 static int orc_cv_run_impl(const orc_cv_param *p, size_t N, size_t T, const double *obs,
                            uint64_t seed, uint64_t resample_seed, int scheme, double threshold,
                            double *state_out, double *lw_out, orc_step_record *rec, int threads,
-                           double phase[4])
+                           int backend, double phase[4])
 {
 #ifdef _OPENMP
     if (threads > 0) omp_set_num_threads(threads);
-#else
-    (void)threads;
 #endif
+    if (threads > ORC_MAX_THREADS) threads = ORC_MAX_THREADS;
     double *st = (double *)malloc(sizeof(double) * 4 * N);
-    double *st2 = (double *)malloc(sizeof(double) * 4 * N);
     double *lw = (double *)malloc(sizeof(double) * N);
     double *w = (double *)malloc(sizeof(double) * N);
     double *inc = (double *)malloc(sizeof(double) * N);
     uint64_t *rep = (uint64_t *)malloc(sizeof(uint64_t) * N);
     uint64_t *cf = (uint64_t *)malloc(sizeof(uint64_t) * N);
-    if (!st || !st2 || !lw || !w || !inc || !rep || !cf) {
-        free(st); free(st2); free(lw); free(w); free(inc); free(rep); free(cf);
+    if (!st || !lw || !w || !inc || !rep || !cf) {
+        free(st); free(lw); free(w); free(inc); free(rep); free(cf);
         return -1;
     }
     orc_tf64_engine reng;
@@ -603,26 +699,9 @@ static int orc_cv_run_impl(const orc_cv_param *p, size_t N, size_t T, const doub
     for (size_t t = 0; t < T; ++t) {
         double t0 = orc_now();
         const double ox = obs[2 * t], oy = obs[2 * t + 1];
-        /* init (t == 0) or move (t >= 1): MoveOMP parallel loop */
-#pragma omp parallel for schedule(static)
-        for (long long ii = 0; ii < (long long)N; ++ii) {
-            size_t i = (size_t)ii;
-            double z[4];
-            orc_cv_normals(seed, i, t, z);
-            double *s = st + 4 * i;
-            if (t == 0) {
-                s[0] = p->prior_sd_pos * z[0];
-                s[1] = p->prior_sd_pos * z[1];
-                s[2] = p->prior_sd_vel * z[2];
-                s[3] = p->prior_sd_vel * z[3];
-            } else {
-                s[0] += p->sd_pos * z[0] + p->dt * s[2];
-                s[1] += p->sd_pos * z[1] + p->dt * s[3];
-                s[2] += p->sd_vel * z[2];
-                s[3] += p->sd_vel * z[3];
-            }
-            inc[i] = orc_cv_llh(p, s[0], s[1], ox, oy);
-        }
+        /* init (t == 0) or move (t >= 1): MoveOMP / MoveSTD over the particles */
+        orc_cv_work work = {p, st, inc, cf, seed, t, ox, oy, 0};
+        orc_cv_parallel(&work, N, threads, backend);
         double t1 = orc_now();
         /* log-evidence increment: log sum_i W_i exp(inc_i)
          * (example/include/move_smc.hpp:64-71), with a max shift */
@@ -649,11 +728,9 @@ static int orc_cv_run_impl(const orc_cv_param *p, size_t N, size_t T, const doub
             draws = orc_resample(scheme, N, N, &reng, w, rep);
             orc_cfrp_trans(N, N, rep, cf);
             t3 = orc_now();
-            /* StateMatrix<RowMajor>::copy (core/state_matrix.hpp:285-300) */
-            for (size_t to = 0; to < N; ++to) {
-                size_t from = cf[to];
-                if (from != to) memcpy(st + 4 * to, st + 4 * from, sizeof(double) * 4);
-            }
+            /* StateOMP::copy / StateSTD::copy: a parallel loop over the slots */
+            orc_cv_work cw = {p, st, inc, cf, seed, t, ox, oy, 1};
+            orc_cv_parallel(&cw, N, threads, backend);
             /* set_equal_weight (core/weight_set.hpp:96-106) */
             for (size_t i = 0; i < N; ++i) { w[i] = 1.0 / (double)N; lw[i] = 0; }
         }
@@ -682,7 +759,7 @@ static int orc_cv_run_impl(const orc_cv_param *p, size_t N, size_t T, const doub
     }
     if (state_out) memcpy(state_out, st, sizeof(double) * 4 * N);
     if (lw_out) memcpy(lw_out, lw, sizeof(double) * N);
-    free(st); free(st2); free(lw); free(w); free(inc); free(rep); free(cf);
+    free(st); free(lw); free(w); free(inc); free(rep); free(cf);
     return 0;
 }
 
@@ -691,15 +768,15 @@ int orc_cv_run(const orc_cv_param *p, size_t N, size_t T, const double *obs,
                double *state, double *lw, orc_step_record *rec, int threads)
 {
     return orc_cv_run_impl(p, N, T, obs, seed, resample_seed, scheme, threshold, state, lw,
-                           rec, threads, NULL);
+                           rec, threads, ORC_BACKEND_OMP, NULL);
 }
 
 int orc_cv_run_timed(const orc_cv_param *p, size_t N, size_t T, const double *obs,
                      uint64_t seed, uint64_t resample_seed, int scheme, double threshold,
-                     orc_step_record *rec, int threads, double phase[4])
+                     orc_step_record *rec, int threads, int backend, double phase[4])
 {
     return orc_cv_run_impl(p, N, T, obs, seed, resample_seed, scheme, threshold, NULL, NULL,
-                           rec, threads, phase);
+                           rec, threads, backend, phase);
 }
 
 /* ======================================================================

@@ -134,9 +134,13 @@ int orc_cv_run(const orc_cv_param *p, size_t N, size_t T, const double *obs,
 
 /* timing-oriented variant of orc_cv_run: same arithmetic, returns per-phase
  * seconds in phase[4] = {move, normalise, resample, copy} */
+/* backends of the timed CPU path: vSMC's OpenMP and std::thread SMP backends */
+#define ORC_BACKEND_OMP 0
+#define ORC_BACKEND_THREADS 1
+#define ORC_MAX_THREADS 1024
 int orc_cv_run_timed(const orc_cv_param *p, size_t N, size_t T, const double *obs,
                      uint64_t seed, uint64_t resample_seed, int scheme, double threshold,
-                     orc_step_record *rec, int threads, double phase[4]);
+                     orc_step_record *rec, int threads, int backend, double phase[4]);
 
 /* ---------------- GMM tempered SMC (gmm_oracle.c) ---------------- */
 #define ORC_GMM_MAX_K 6
