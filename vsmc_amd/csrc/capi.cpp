@@ -199,8 +199,17 @@ struct vsmcx_filter {
         if (plain) {
             launch_plain_rest(ctl, st[0], lw, w, inc, ppart, scal, t, N, d.scheme, rb, hist, s);
         } else {
-            launch_cv_reduce(ctl, partials, tiles, hist, t, s);
-            launch_resample_from_log_weights(ctl, lw, N, d.scheme, rb, hist, s, partials);
+            // non-residual schemes: the reduce also runs the resample prologue and
+            // the fold's tile starts
+            const bool fuse = d.scheme == kMultinomial || d.scheme == kStratified ||
+                              d.scheme == kSystematic;
+            if (fuse) {
+                rb.fs.tiles = tiles;
+                launch_cv_reduce(ctl, partials, tiles, hist, t, s, d.scheme, &rb.fs);
+            } else {
+                launch_cv_reduce(ctl, partials, tiles, hist, t, s);
+            }
+            launch_resample_from_log_weights(ctl, lw, N, d.scheme, rb, hist, s, partials, fuse);
             cur ^= 1;
         }
         ++t;
@@ -899,7 +908,7 @@ int vsmcx_filter_debug_step(vsmcx_filter *f, double *weight, uint64_t *replicati
         const bool sorted = f->d.scheme == kMultinomial || f->d.scheme == kResidual;
         if (uniforms && sorted) {
             us.resize(N);
-            tm.resize((N + kTile - 1) / kTile + 1);
+            tm.resize((N + kFoldTile - 1) / kFoldTile + 1);
             VSMCX_HIP(hipMemcpyAsync(us.data(), f->rb.uscan, 8 * N, hipMemcpyDeviceToHost, s));
             VSMCX_HIP(hipMemcpyAsync(tm.data(), f->rb.u_tmaxex, 8 * tm.size(),
                                      hipMemcpyDeviceToHost, s));
@@ -911,7 +920,7 @@ int vsmcx_filter_debug_step(vsmcx_filter *f, double *weight, uint64_t *replicati
         if (nn) *nn = h2.nn;
         if (uniforms && sorted)
             for (uint64_t k = 0; k < h2.nn; ++k)
-                uniforms[k] = us[k] < tm[k / kTile] ? tm[k / kTile] : us[k];
+                uniforms[k] = us[k] < tm[k / kFoldTile] ? tm[k / kFoldTile] : us[k];
     } else {
         c->sync();
     }

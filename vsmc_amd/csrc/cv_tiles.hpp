@@ -103,56 +103,78 @@ __device__ __forceinline__ void cv_move_tile(
 }
 
 // Tile partials {max, sum e, sum e^2, monitor x, monitor y} of ntiles tiles,
-// combined in an order that the tile count alone fixes: lane l of the
-// workgroup's first wave takes tiles l, l + 64, l + 128, ... in turn, and the
-// 64 lane results meet in a fixed butterfly (every lane ends with the same
-// totals).  A 1024-thread reduce launch and a 256-thread batch workgroup
-// therefore compute the same bits.  Call from all 64 lanes of wave 0.
+// combined in an order that the tile count alone fixes.  The tiles are dealt to
+// nv = ceil(ntiles / 512) "virtual waves" (at most 16): lane l of virtual wave
+// w takes tiles w*64 + l, then + 64*nv, ... in turn; the 64 lane results of a
+// virtual wave meet in a fixed butterfly; the virtual waves' results are
+// added in order of w.  Up to 512 tiles (a filter of up to 2^20 particles)
+// that is a single wave: a 64-thread tick reduce, a 256-thread batch
+// workgroup and the 1024-thread reduce of a single filter compute the same
+// bits.  Larger filters spread their tiles over up to 16 waves of the
+// single-filter reduce.
 //
 // Each lane loads kPartAhead of its tiles at once and evaluates their exps
 // together before adding them in tile order: the order of the additions is
-// that of the plain loop, but one wave no longer waits a memory latency per
-// tile (8192 tiles at N = 2^24 are 128 per lane).
+// that of the plain loop, but a wave does not wait a memory latency per tile.
 constexpr int kPartAhead = 8;
+constexpr int kPartMaxWaves = 16;
 
-__device__ __forceinline__ void tile_partials_lse(const double *partials, uint64_t ntiles,
-                                                  double &M, Part5 &acc)
+__device__ __forceinline__ int part_vwaves(uint64_t ntiles)
+{
+    const uint64_t w = (ntiles + (uint64_t)kPartAhead * kWave - 1) / ((uint64_t)kPartAhead * kWave);
+    return w < 1 ? 1 : (w > (uint64_t)kPartMaxWaves ? kPartMaxWaves : (int)w);
+}
+
+// The maximum over the tiles of virtual wave w of nv (every lane gets it).
+// A: tiles loaded at once per lane; the additions below are in tile order
+// whatever A is, so A only trades registers for memory latency.
+template <int A = kPartAhead>
+__device__ __forceinline__ double vwave_max(const double *partials, uint64_t ntiles, int w, int nv)
 {
     const int lane = threadIdx.x & (kWave - 1);
-    constexpr uint64_t kStep = (uint64_t)kPartAhead * kWave;
+    const uint64_t S = (uint64_t)kWave * nv, kStep = (uint64_t)A * S;
     double m = -INFINITY;
-    uint64_t j = lane;
-    for (; j + (kPartAhead - 1) * kWave < ntiles; j += kStep) {
-        double v[kPartAhead];
+    uint64_t j = (uint64_t)w * kWave + lane;
+    for (; j + (A - 1) * S < ntiles; j += kStep) {
+        double v[A];
 #pragma unroll
-        for (int u = 0; u < kPartAhead; ++u) v[u] = partials[5 * (j + (uint64_t)u * kWave)];
+        for (int u = 0; u < A; ++u) v[u] = partials[5 * (j + (uint64_t)u * S)];
 #pragma unroll
-        for (int u = 0; u < kPartAhead; ++u) m = m < v[u] ? v[u] : m;
+        for (int u = 0; u < A; ++u) m = m < v[u] ? v[u] : m;
     }
-    for (; j < ntiles; j += kWave) {
+    for (; j < ntiles; j += S) {
         const double v = partials[5 * j];
         m = m < v ? v : m;
     }
-    M = wave_reduce(m, MaxOp());
+    return wave_reduce(m, MaxOp());
+}
+
+// The sums over the tiles of virtual wave w of nv, scaled to the maximum M.
+template <int A = kPartAhead>
+__device__ __forceinline__ Part5 vwave_sums(const double *partials, uint64_t ntiles, int w, int nv,
+                                            double M)
+{
+    const int lane = threadIdx.x & (kWave - 1);
+    const uint64_t S = (uint64_t)kWave * nv, kStep = (uint64_t)A * S;
     Part5 a{0, 0, 0, 0};
-    j = lane;
-    for (; j + (kPartAhead - 1) * kWave < ntiles; j += kStep) {
-        double q[kPartAhead][5], sc[kPartAhead];
+    uint64_t j = (uint64_t)w * kWave + lane;
+    for (; j + (A - 1) * S < ntiles; j += kStep) {
+        double q[A][5], sc[A];
 #pragma unroll
-        for (int u = 0; u < kPartAhead; ++u)
+        for (int u = 0; u < A; ++u)
 #pragma unroll
-            for (int c = 0; c < 5; ++c) q[u][c] = partials[5 * (j + (uint64_t)u * kWave) + c];
+            for (int c = 0; c < 5; ++c) q[u][c] = partials[5 * (j + (uint64_t)u * S) + c];
 #pragma unroll
-        for (int u = 0; u < kPartAhead; ++u) sc[u] = q[u][0] == -INFINITY ? 0.0 : exp(q[u][0] - M);
+        for (int u = 0; u < A; ++u) sc[u] = q[u][0] == -INFINITY ? 0.0 : exp(q[u][0] - M);
 #pragma unroll
-        for (int u = 0; u < kPartAhead; ++u) {
+        for (int u = 0; u < A; ++u) {
             a.se += q[u][1] * sc[u];
             a.se2 += q[u][2] * sc[u] * sc[u];
             a.mx += q[u][3];
             a.my += q[u][4];
         }
     }
-    for (; j < ntiles; j += kWave) {
+    for (; j < ntiles; j += S) {
         const double *q = partials + 5 * j;
         const double sc = q[0] == -INFINITY ? 0.0 : exp(q[0] - M);
         a.se += q[1] * sc;
@@ -160,41 +182,106 @@ __device__ __forceinline__ void tile_partials_lse(const double *partials, uint64
         a.mx += q[3];
         a.my += q[4];
     }
-    acc = wave_reduce(a, Part5Add());
+    return wave_reduce(a, Part5Add());
 }
 
-// The monitor fields alone, in the same order.
-__device__ __forceinline__ void tile_partials_monitor(const double *partials, uint64_t ntiles,
-                                                      double &mx, double &my)
+// The monitor fields of virtual wave w of nv.
+template <int A = kPartAhead>
+__device__ __forceinline__ Part5 vwave_monitor(const double *partials, uint64_t ntiles, int w, int nv)
 {
     const int lane = threadIdx.x & (kWave - 1);
-    constexpr uint64_t kStep = (uint64_t)kPartAhead * kWave;
+    const uint64_t S = (uint64_t)kWave * nv, kStep = (uint64_t)A * S;
     Part5 a{0, 0, 0, 0};
-    uint64_t j = lane;
-    for (; j + (kPartAhead - 1) * kWave < ntiles; j += kStep) {
-        double x[kPartAhead], y[kPartAhead];
+    uint64_t j = (uint64_t)w * kWave + lane;
+    for (; j + (A - 1) * S < ntiles; j += kStep) {
+        double x[A], y[A];
 #pragma unroll
-        for (int u = 0; u < kPartAhead; ++u) {
-            x[u] = partials[5 * (j + (uint64_t)u * kWave) + 3];
-            y[u] = partials[5 * (j + (uint64_t)u * kWave) + 4];
+        for (int u = 0; u < A; ++u) {
+            x[u] = partials[5 * (j + (uint64_t)u * S) + 3];
+            y[u] = partials[5 * (j + (uint64_t)u * S) + 4];
         }
 #pragma unroll
-        for (int u = 0; u < kPartAhead; ++u) {
+        for (int u = 0; u < A; ++u) {
             a.mx += x[u];
             a.my += y[u];
         }
     }
-    for (; j < ntiles; j += kWave) {
+    for (; j < ntiles; j += S) {
         a.mx += partials[5 * j + 3];
         a.my += partials[5 * j + 4];
     }
-    a = wave_reduce(a, Part5Add());
+    return wave_reduce(a, Part5Add());
+}
+
+// One wave (all 64 lanes), ntiles <= 512: the single virtual wave.
+__device__ __forceinline__ void tile_partials_lse(const double *partials, uint64_t ntiles,
+                                                  double &M, Part5 &acc)
+{
+    M = vwave_max(partials, ntiles, 0, 1);
+    acc = vwave_sums(partials, ntiles, 0, 1, M);
+}
+
+__device__ __forceinline__ void tile_partials_monitor(const double *partials, uint64_t ntiles,
+                                                      double &mx, double &my)
+{
+    const Part5 a = vwave_monitor(partials, ntiles, 0, 1);
     mx = a.mx;
     my = a.my;
 }
 
+// LDS of the workgroup-wide forms
+struct PartLds {
+    double m[kPartMaxWaves];
+    Part5 p[kPartMaxWaves];
+};
+
+// Every thread of a workgroup of any multiple of 64 threads; every thread
+// gets the results.
+__device__ __forceinline__ void tile_partials_lse_block(const double *partials, uint64_t ntiles,
+                                                        double &M, Part5 &acc, PartLds &L)
+{
+    const int nv = part_vwaves(ntiles);
+    const int pw = threadIdx.x / kWave, npw = blockDim.x / kWave;
+    const int lane = threadIdx.x & (kWave - 1);
+    for (int w = pw; w < nv; w += npw) {
+        const double m = vwave_max<4>(partials, ntiles, w, nv);
+        if (lane == 0) L.m[w] = m;
+    }
+    __syncthreads();
+    M = L.m[0];
+    for (int w = 1; w < nv; ++w) M = M < L.m[w] ? L.m[w] : M;
+    for (int w = pw; w < nv; w += npw) {
+        const Part5 a = vwave_sums<4>(partials, ntiles, w, nv, M);
+        if (lane == 0) L.p[w] = a;
+    }
+    __syncthreads();
+    acc = L.p[0];
+    for (int w = 1; w < nv; ++w) acc = Part5Add()(acc, L.p[w]);
+    __syncthreads();  // L may be reused
+}
+
+__device__ __forceinline__ void tile_partials_monitor_block(const double *partials,
+                                                            uint64_t ntiles, double &mx,
+                                                            double &my, PartLds &L)
+{
+    const int nv = part_vwaves(ntiles);
+    const int pw = threadIdx.x / kWave, npw = blockDim.x / kWave;
+    const int lane = threadIdx.x & (kWave - 1);
+    for (int w = pw; w < nv; w += npw) {
+        const Part5 a = vwave_monitor<4>(partials, ntiles, w, nv);
+        if (lane == 0) L.p[w] = a;
+    }
+    __syncthreads();
+    Part5 acc = L.p[0];
+    for (int w = 1; w < nv; ++w) acc = Part5Add()(acc, L.p[w]);
+    mx = acc.mx;
+    my = acc.my;
+    __syncthreads();
+}
+
 // After a step's move: ESS, log-evidence and the resample decision from the
-// tile partials (wave 0 of the calling workgroup; thread 0 writes).
+// tile partials (wave 0 of the calling workgroup; thread 0 writes); up to
+// 512 tiles.
 __device__ __forceinline__ void cv_reduce_tiles(FilterCtl *ctl, const double *partials,
                                                 uint64_t ntiles, StepRow *hist, uint64_t t)
 {

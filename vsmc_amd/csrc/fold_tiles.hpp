@@ -243,8 +243,8 @@ struct InvStratified {
     }
 };
 
-// A sorted uniform array held as a tile-wise running max: v_n =
-// max(tile_prefix_max[n / kTile], uscan[n]).
+// A sorted uniform array held as a tile-wise running max over fold tiles: v_n =
+// max(tile_prefix_max[n / kFoldTile], uscan[n]).
 struct InvArray {
     static constexpr bool kInv = true;
     static constexpr bool kWindow = true;
@@ -256,7 +256,7 @@ struct InvArray {
     __device__ void bind(const FilterCtl *c) { nn = c->nn; }
     __device__ double v(uint64_t n) const
     {
-        const double a = uscan[n], b = tmax[n / kTile];
+        const double a = uscan[n], b = tmax[n / kFoldTile];
         return a < b ? b : a;
     }
     __device__ void fill_window(double *s_v, double *, uint64_t w0, int len) const
@@ -280,6 +280,10 @@ struct FoldOut {
     uint32_t *sl;        // [M] surplus-copy list
     uint64_t *rep;       // [M] optional replication counts
     double *a_out;       // [M] optional exact fold values
+    // The sorted-uniform fold (a_i = -lmax_i): with u_tmax set, a_out receives
+    // u_i = 1 - exp(-a_i) held as a running max within the tile instead, and
+    // u_tmax[tile] the tile's maximum.
+    double *u_tmax = nullptr;
 };
 
 // Tiles of 2048 elements, 8 per thread.  Per-element data lives in LDS (one
@@ -664,6 +668,26 @@ __device__ void fold_tile_apply(FilterCtl *ctl, const XSrc &xs, const Inv &inv, 
     __syncthreads();
 
     if (out.a_out) {
+        if (out.u_tmax) {
+            // U01SequenceSorted's u = 1 - exp(lmax), lmax = -a (u01.hpp:185-187),
+            // as inversion reads it: the running max (bins only advance)
+            double mx = -1.0;
+#pragma unroll 1
+            for (int k = 0; k < kFI; ++k) {
+                const double u = 1 - exp(-s_a[fidx(e0 + k)]);
+                mx = mx < u ? u : mx;
+                s_a[fidx(e0 + k)] = mx;
+            }
+            double tmax;
+            const double mex = block_excl_scan<kBlock>(mx, MaxOp(), -1.0, s_red, &tmax);
+#pragma unroll 1
+            for (int k = 0; k < kFI; ++k) {
+                const double u = s_a[fidx(e0 + k)];
+                s_a[fidx(e0 + k)] = u < mex ? mex : u;
+            }
+            if (tid == 0) out.u_tmax[t] = tmax;
+            __syncthreads();
+        }
 #pragma unroll 1
         for (int k = 0; k < kFI; ++k) {
             const int e = k * kBlock + tid;

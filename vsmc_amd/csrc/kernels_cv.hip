@@ -12,6 +12,7 @@
 #include "launch.hpp"
 #include "cv_model.hpp"
 #include "cv_tiles.hpp"
+#include "fold_tiles.hpp"
 #include "rng_core.hpp"
 
 namespace vsmcx {
@@ -27,10 +28,36 @@ __global__ __launch_bounds__(kBlock) void k_cv_move(
                        partials + 5 * (uint64_t)blockIdx.x);
 }
 
-__global__ __launch_bounds__(kWave) void k_cv_reduce(FilterCtl *ctl, const double *partials,
-                                                    uint64_t ntiles, StepRow *hist, uint64_t t)
+// After the move: ESS, log-evidence and the resample decision from the tile
+// partials; with prologue_scheme >= 0 and a resample, also the scheme's
+// uniform-grid prologue and the fold's approximate tile starts of the
+// filter's weights (TileSumsFromLse) into fs.ts, which the fold would
+// otherwise compute in two more launches.
+__global__ __launch_bounds__(kScanBlock) void k_cv_reduce(FilterCtl *ctl, const double *partials,
+                                                         uint64_t ntiles, StepRow *hist,
+                                                         uint64_t t, int prologue_scheme,
+                                                         FoldScratch fs)
 {
-    cv_reduce_tiles(ctl, partials, ntiles, hist, t);
+    __shared__ PartLds L;
+    __shared__ double s_red[kScanBlock / kWave];
+    __shared__ double s_mc[2];
+    __shared__ int s_res;
+    double M;
+    Part5 acc;
+    tile_partials_lse_block(partials, ntiles, M, acc, L);
+    if (threadIdx.x == 0) {
+        cv_step_scalars(ctl, M, acc.se, acc.se2, acc.mx, acc.my, hist, t);
+        const int res = ctl->resample;
+        if (res && prologue_scheme >= 0) resample_prologue(ctl, prologue_scheme);
+        s_mc[0] = ctl->m;
+        s_mc[1] = ctl->coeff;
+        s_res = res;
+    }
+    if (prologue_scheme < 0 || !fs.ts) return;
+    __syncthreads();
+    if (!s_res) return;
+    const TileSumsFromLse in{partials, s_mc[0], s_mc[1]};
+    wg_scan<double>(in, fs.ts, fs.tiles, AddOp(), 0.0, s_red);
 }
 
 // Monitor of the last step: sum_i W_i (x_i, y_i) over the post-resample state.
@@ -43,10 +70,17 @@ __global__ __launch_bounds__(kBlock) void k_cv_monitor(FilterCtl *ctl,
     cv_monitor_tile<Real>(ctl, st, anc, lw, N, blockIdx.x, partials + 5 * (uint64_t)blockIdx.x);
 }
 
-__global__ __launch_bounds__(kWave) void k_cv_monitor_sum(FilterCtl *ctl, const double *partials,
-                                                         uint64_t ntiles, StepRow *hist)
+__global__ __launch_bounds__(kScanBlock) void k_cv_monitor_sum(FilterCtl *ctl,
+                                                              const double *partials,
+                                                              uint64_t ntiles, StepRow *hist)
 {
-    cv_monitor_reduce(ctl, partials, ntiles, hist);
+    __shared__ PartLds L;
+    double mx, my;
+    tile_partials_monitor_block(partials, ntiles, mx, my, L);
+    if (threadIdx.x == 0 && hist) {
+        hist[ctl->t].mean_x = mx;
+        hist[ctl->t].mean_y = my;
+    }
 }
 
 // W_i exactly as the resampler computes it (XWeight in kernels_fold.hip)
@@ -86,9 +120,12 @@ void launch_cv_move(bool fp64, FilterCtl *ctl, const void *st_in, void *st_out,
 }
 
 void launch_cv_reduce(FilterCtl *ctl, const double *partials, uint64_t ntiles, StepRow *hist,
-                      uint64_t t, hipStream_t st)
+                      uint64_t t, hipStream_t st, int prologue_scheme, const FoldScratch *starts)
 {
-    hipLaunchKernelGGL(k_cv_reduce, dim3(1), dim3(kWave), 0, st, ctl, partials, ntiles, hist, t);
+    FoldScratch fs{};
+    if (starts) fs = *starts;
+    hipLaunchKernelGGL(k_cv_reduce, dim3(1), dim3(kScanBlock), 0, st, ctl, partials, ntiles, hist,
+                       t, starts ? prologue_scheme : -1, fs);
 }
 
 void launch_cv_monitor(bool fp64, FilterCtl *ctl, const void *stp, const uint32_t *anc,
@@ -101,7 +138,7 @@ void launch_cv_monitor(bool fp64, FilterCtl *ctl, const void *stp, const uint32_
     else
         hipLaunchKernelGGL(k_cv_monitor<float>, dim3(tiles_of(N)), dim3(kBlock), 0, st, ctl,
                            (const S4f *)stp, anc, lw, N, partials);
-    hipLaunchKernelGGL(k_cv_monitor_sum, dim3(1), dim3(kWave), 0, st, ctl, partials,
+    hipLaunchKernelGGL(k_cv_monitor_sum, dim3(1), dim3(kScanBlock), 0, st, ctl, partials,
                        (uint64_t)tiles_of(N), hist);
 }
 

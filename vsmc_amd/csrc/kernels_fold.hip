@@ -110,12 +110,25 @@ __global__ __launch_bounds__(kBlock) void k_fold_apply(FilterCtl *ctl, XSrc xs, 
 }
 
 // ---- pass 6: zero / surplus ranks of the tiles (one workgroup) ---------------------
-__global__ __launch_bounds__(kScanBlock) void k_fold_scan_cnt(FilterCtl *ctl, FoldScratch fs)
+// The resample's engine bookkeeping (resample_epilogue) rides along when
+// epi.on: n_final is known once the apply pass is done, and nothing after
+// this pass reads the engine.
+struct Epi {
+    StepRow *hist;
+    int on;
+    int systematic_like;
+};
+
+__global__ __launch_bounds__(kScanBlock) void k_fold_scan_cnt(FilterCtl *ctl, FoldScratch fs,
+                                                             Epi epi, uint64_t M)
 {
     if (!ctl->resample) return;
     __shared__ uint64_t s_red[kScanBlock / kWave];
     const uint64_t tot = wg_scan<uint64_t>(fs.zs, fs.zsx, fs.tiles, AddOp(), (uint64_t)0, s_red);
-    if (threadIdx.x == 0) ctl->zeros = tot >> 32;
+    if (threadIdx.x == 0) {
+        ctl->zeros = tot >> 32;
+        if (epi.on) resample_epilogue(ctl, epi.hist, epi.systematic_like, M);
+    }
 }
 
 // ---- pass 7: cfrp_trans layout ----------------------------------------------------
@@ -244,9 +257,9 @@ __global__ void k_plain_prologue(FilterCtl *ctl, int scheme)
 // and round-to-nearest is symmetric, so -lmax_n is exactly the left fold of
 // the non-negative x_n = -t_n: the passes below evaluate that fold with the
 // exact-fold machinery (the same passes as the weight fold), so lmax is
-// vSMC's bit for bit whenever the terms are.  The uniforms are then held as a
-// tile-wise running max (the identity when exp is monotone), so the array is
-// sorted by construction.
+// vSMC's bit for bit whenever the terms are.  The fold's apply pass turns the
+// values into the uniforms, held as a running max per fold tile (the identity
+// when exp is monotone), so the array is sorted by construction.
 // ------------------------------------------------------------------------
 // The terms x_n of n = n0 .. n0 + K - 1 (0 past nn).  Word n of the engine's
 // stream is word (n + index) mod 4 of the block at counter ctr + (n + index)/4
@@ -257,6 +270,11 @@ template <int K>
 __device__ __forceinline__ void sorted_terms(const Tf64State &e, uint64_t n0, uint64_t nn,
                                              double term[K])
 {
+    if (n0 >= nn) {  // past the sequence: no draws
+#pragma unroll
+        for (int k = 0; k < K; ++k) term[k] = 0.0;
+        return;
+    }
     u64x4 c0;
 #pragma unroll
     for (int j = 0; j < 4; ++j) c0.v[j] = e.ctr[j];
@@ -283,15 +301,24 @@ __global__ __launch_bounds__(kBlock) void k_sorted_terms(FilterCtl *ctl, double 
 {
     if (!ctl->resample) return;
     __shared__ double s_red[kBlock / kWave];
+    __shared__ double s_x[kFPad];
     const uint64_t nn = ctl->nn;
-    const uint64_t n0 = blockIdx.x * (uint64_t)kFT + threadIdx.x * (uint64_t)kFI;
+    const uint64_t base = blockIdx.x * (uint64_t)kFT;
+    const int e0 = threadIdx.x * kFI;
     double term[kFI];
-    sorted_terms<kFI>(ctl->eng, n0, nn, term);
+    sorted_terms<kFI>(ctl->eng, base + e0, nn, term);
     double s = 0;
 #pragma unroll
     for (int k = 0; k < kFI; ++k) {
-        if (n0 + k < nuni) ux[n0 + k] = term[k];
+        s_x[fidx(e0 + k)] = term[k];
         s += term[k];
+    }
+    __syncthreads();
+    // stored striped: one coalesced store instruction per element of each thread
+#pragma unroll
+    for (int k = 0; k < kFI; ++k) {
+        const int e = k * kBlock + threadIdx.x;
+        if (base + e < nuni) ux[base + e] = s_x[fidx(e)];
     }
     s = block_reduce<kBlock>(s, AddOp(), s_red);
     if (threadIdx.x == 0) tsum[blockIdx.x] = s;
@@ -320,43 +347,6 @@ __global__ __launch_bounds__(1024) void k_scan_tiles_max(FilterCtl *ctl, const d
     }
 }
 
-// Per kTile tile of the exact fold values a_n = -lmax_n (in uscan): u_n =
-// 1 - exp(-a_n), the tile-local inclusive running max back into uscan (in
-// place: each thread reads its own elements before any write), tile max into
-// tmaxv.
-__global__ __launch_bounds__(kBlock) void k_sorted_values(FilterCtl *ctl, double *uscan,
-                                                          double *tmaxv)
-{
-    if (!ctl->resample) return;
-    __shared__ double s_red[kBlock / kWave];
-    const uint64_t nn = ctl->nn;
-    const uint64_t base = blockIdx.x * (uint64_t)kTile;
-    if (base >= nn) return;
-    const uint64_t n0 = base + threadIdx.x * (uint64_t)kItems;
-    double mx = -1.0;
-    double uu[kItems];
-#pragma unroll
-    for (int k = 0; k < kItems; ++k) {
-        const uint64_t n = n0 + k;
-        const double u = n < nn ? 1 - exp(-uscan[n]) : -1.0;
-        mx = mx < u ? u : mx;
-        uu[k] = mx;
-    }
-    // thread-exclusive running max across the tile
-    const double minc = block_incl_scan<kBlock>(mx, MaxOp(), s_red);
-    double mex = lane_up(minc, 1);
-    {
-        const int lane = threadIdx.x & 63, wid = threadIdx.x / 64;
-        if (lane == 0) mex = wid > 0 ? s_red[wid - 1] : -1.0;
-    }
-#pragma unroll
-    for (int k = 0; k < kItems; ++k) {
-        const uint64_t n = n0 + k;
-        if (n < nn) uscan[n] = uu[k] < mex ? mex : uu[k];
-    }
-    if (threadIdx.x == kBlock - 1) tmaxv[blockIdx.x] = minc;
-}
-
 // ------------------------------------------------------------------------
 // host launchers
 // ------------------------------------------------------------------------
@@ -367,13 +357,15 @@ __global__ __launch_bounds__(kBlock) void k_sorted_values(FilterCtl *ctl, double
 // log-weights, from which the tile starts follow without a tile-sum pass.
 template <typename XSrc, typename Inv>
 static void launch_fold(FilterCtl *ctl, XSrc xs, Inv inv, uint64_t M, FoldScratch &fs,
-                        const FoldOut &out, hipStream_t st, const double *lse_part = nullptr)
+                        const FoldOut &out, hipStream_t st, const double *lse_part = nullptr,
+                        bool pre_started = false, Epi epi = Epi{nullptr, 0, 0})
 {
     fs.tiles = (M + kFT - 1) / kFT;
     const dim3 grid((unsigned)fs.tiles);
     if (lse_part && fs.wx) {
-        hipLaunchKernelGGL(k_fold_scan_sum_lse, dim3(1), dim3(kScanBlock), 0, st, ctl, lse_part,
-                           fs);
+        if (!pre_started)  // else the reduce already wrote the starts
+            hipLaunchKernelGGL(k_fold_scan_sum_lse, dim3(1), dim3(kScanBlock), 0, st, ctl,
+                               lse_part, fs);
         // the segment pass computes x and leaves it for the apply pass, which
         // reads x more than once per element
         hipLaunchKernelGGL((k_fold_seg<XSrc>), grid, dim3(kBlock), 0, st, ctl, xs, M, fs, fs.wx);
@@ -383,7 +375,7 @@ static void launch_fold(FilterCtl *ctl, XSrc xs, Inv inv, uint64_t M, FoldScratc
         hipLaunchKernelGGL((k_fold_apply<XCached<XSrc>, Inv>), grid, dim3(kBlock), 0, st, ctl, xc,
                            inv, M, fs, out);
         if (!Inv::kInv || !out.anc) return;
-        hipLaunchKernelGGL(k_fold_scan_cnt, dim3(1), dim3(kScanBlock), 0, st, ctl, fs);
+        hipLaunchKernelGGL(k_fold_scan_cnt, dim3(1), dim3(kScanBlock), 0, st, ctl, fs, epi, M);
         hipLaunchKernelGGL(k_fold_layout, grid, dim3(kBlock), 0, st, ctl, fs, out, M);
         return;
     }
@@ -407,14 +399,15 @@ static void launch_fold(FilterCtl *ctl, XSrc xs, Inv inv, uint64_t M, FoldScratc
                            fs, out);
     }
     if (!Inv::kInv || !out.anc) return;
-    hipLaunchKernelGGL(k_fold_scan_cnt, dim3(1), dim3(kScanBlock), 0, st, ctl, fs);
+    hipLaunchKernelGGL(k_fold_scan_cnt, dim3(1), dim3(kScanBlock), 0, st, ctl, fs, epi, M);
     hipLaunchKernelGGL(k_fold_layout, grid, dim3(kBlock), 0, st, ctl, fs, out, M);
 }
 
 // Passes 3-7 once the tile starts are in fs.ts, x read from the source as is.
 template <typename XSrc, typename Inv>
 static void launch_fold_from_starts(FilterCtl *ctl, XSrc xs, Inv inv, uint64_t M,
-                                    FoldScratch &fs, const FoldOut &out, hipStream_t st)
+                                    FoldScratch &fs, const FoldOut &out, hipStream_t st,
+                                    Epi epi = Epi{nullptr, 0, 0})
 {
     const dim3 grid((unsigned)fs.tiles);
     hipLaunchKernelGGL((k_fold_seg<XSrc>), grid, dim3(kBlock), 0, st, ctl, xs, M, fs,
@@ -424,13 +417,14 @@ static void launch_fold_from_starts(FilterCtl *ctl, XSrc xs, Inv inv, uint64_t M
     hipLaunchKernelGGL((k_fold_apply<XSrc, Inv>), grid, dim3(kBlock), 0, st, ctl, xs, inv, M, fs,
                        out);
     if (!Inv::kInv || !out.anc) return;
-    hipLaunchKernelGGL(k_fold_scan_cnt, dim3(1), dim3(kScanBlock), 0, st, ctl, fs);
+    hipLaunchKernelGGL(k_fold_scan_cnt, dim3(1), dim3(kScanBlock), 0, st, ctl, fs, epi, M);
     hipLaunchKernelGGL(k_fold_layout, grid, dim3(kBlock), 0, st, ctl, fs, out, M);
 }
 
 template <typename W>
 static void resample_pipeline(FilterCtl *ctl, W w, uint64_t M, int scheme, ResampleBuffers &rb,
-                              StepRow *hist, hipStream_t st, const double *lse_part = nullptr)
+                              StepRow *hist, hipStream_t st, const double *lse_part = nullptr,
+                              bool pre_started = false)
 {
     FoldOut out{rb.anc, rb.zl, rb.sl, rb.rep_out, rb.a_out};
     const bool residual = scheme == kResidual || scheme == kResidualStratified ||
@@ -446,9 +440,12 @@ static void resample_pipeline(FilterCtl *ctl, W w, uint64_t M, int scheme, Resam
         launch_fold_from_starts(ctl, xr, InvNone(), M, fs,
                                 FoldOut{nullptr, nullptr, nullptr, nullptr, nullptr}, st);
         hipLaunchKernelGGL(k_residual_prologue, dim3(1), dim3(64), 0, st, ctl, scheme);
-    } else {
+    } else if (!pre_started) {
         hipLaunchKernelGGL(k_plain_prologue, dim3(1), dim3(64), 0, st, ctl, scheme);
     }
+    // with the ancestor lists the count scan carries the engine bookkeeping
+    const int sys_like = (scheme == kSystematic || scheme == kResidualSystematic) ? 1 : 0;
+    const Epi epi{hist, rb.anc ? 1 : 0, sys_like};
     const bool sorted = scheme == kMultinomial || scheme == kResidual;
     if (sorted) {
         // uniforms: nn <= rb.nuni of them.  The exact fold of the x_n over the
@@ -460,13 +457,12 @@ static void resample_pipeline(FilterCtl *ctl, W w, uint64_t M, int scheme, Resam
         const dim3 fgrid((unsigned)ufs.tiles);
         hipLaunchKernelGGL(k_sorted_terms, fgrid, dim3(kBlock), 0, st, ctl, rb.uscan, ufs.tsum, nu);
         hipLaunchKernelGGL(k_fold_scan_sum, dim3(1), dim3(kScanBlock), 0, st, ctl, ufs);
+        // the apply pass leaves u = 1 - exp(-a) as a running max per fold tile
         launch_fold_from_starts(ctl, XArray{rb.uscan}, InvNone(), nu, ufs,
-                                FoldOut{nullptr, nullptr, nullptr, nullptr, rb.uscan}, st);
-        const unsigned ut = (unsigned)((nu + kTile - 1) / kTile);
-        hipLaunchKernelGGL(k_sorted_values, dim3(ut), dim3(kBlock), 0, st, ctl, rb.uscan,
-                           rb.u_tmax);
+                                FoldOut{nullptr, nullptr, nullptr, nullptr, rb.uscan, rb.u_tmax},
+                                st);
         hipLaunchKernelGGL(k_scan_tiles_max, dim3(1), dim3(1024), 0, st, ctl, rb.u_tmax,
-                           rb.u_tmaxex, (uint64_t)ut);
+                           rb.u_tmaxex, (uint64_t)ufs.tiles);
     }
     InvArray ia{rb.uscan, rb.u_tmaxex, 0};
     if (residual) {
@@ -475,33 +471,34 @@ static void resample_pipeline(FilterCtl *ctl, W w, uint64_t M, int scheme, Resam
         hipLaunchKernelGGL(k_fold_scan_sum_scaled, dim3(1), dim3(kScanBlock), 0, st, ctl, fs);
         const XResidStored xs{fs.wx, fs.rep32, 0.0};
         if (scheme == kResidual)
-            launch_fold_from_starts(ctl, xs, ia, M, fs, out, st);
+            launch_fold_from_starts(ctl, xs, ia, M, fs, out, st, epi);
         else if (scheme == kResidualStratified)
-            launch_fold_from_starts(ctl, xs, InvStratified(), M, fs, out, st);
+            launch_fold_from_starts(ctl, xs, InvStratified(), M, fs, out, st, epi);
         else
-            launch_fold_from_starts(ctl, xs, InvSystematic(), M, fs, out, st);
+            launch_fold_from_starts(ctl, xs, InvSystematic(), M, fs, out, st, epi);
     } else {
         if (scheme == kMultinomial)
-            launch_fold(ctl, w, ia, M, rb.fs, out, st, lse_part);
+            launch_fold(ctl, w, ia, M, rb.fs, out, st, lse_part, pre_started, epi);
         else if (scheme == kStratified)
-            launch_fold(ctl, w, InvStratified(), M, rb.fs, out, st, lse_part);
+            launch_fold(ctl, w, InvStratified(), M, rb.fs, out, st, lse_part, pre_started, epi);
         else
-            launch_fold(ctl, w, InvSystematic(), M, rb.fs, out, st, lse_part);
+            launch_fold(ctl, w, InvSystematic(), M, rb.fs, out, st, lse_part, pre_started, epi);
     }
     if (rb.anc) {
         const unsigned zg = (unsigned)((M + kTile - 1) / kTile) * 4;
         hipLaunchKernelGGL(k_zero_fill, dim3(zg ? zg : 1), dim3(kBlock), 0, st, ctl, rb.zl, rb.sl,
                            rb.anc, M);
     }
-    hipLaunchKernelGGL(k_resample_epilogue, dim3(1), dim3(64), 0, st, ctl, hist,
-                       (scheme == kSystematic || scheme == kResidualSystematic) ? 1 : 0, M);
+    if (!epi.on)
+        hipLaunchKernelGGL(k_resample_epilogue, dim3(1), dim3(64), 0, st, ctl, hist, sys_like, M);
 }
 
 void launch_resample_from_log_weights(FilterCtl *ctl, const double *lw, uint64_t M, int scheme,
                                       ResampleBuffers &rb, StepRow *hist, hipStream_t st,
-                                      const double *lse_part)
+                                      const double *lse_part, bool pre_started)
 {
-    resample_pipeline(ctl, XWeight{lw, 0.0, 0.0}, M, scheme, rb, hist, st, lse_part);
+    resample_pipeline(ctl, XWeight{lw, 0.0, 0.0}, M, scheme, rb, hist, st, lse_part,
+                      pre_started && lse_part);
 }
 
 void launch_resample_from_weights(FilterCtl *ctl, const double *w, uint64_t M, int scheme,

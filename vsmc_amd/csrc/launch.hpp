@@ -14,9 +14,9 @@ struct ResampleBuffers {
     uint32_t *sl = nullptr;      // [M] surplus-copy list
     uint64_t *rep_out = nullptr; // [M] optional replication output
     double *a_out = nullptr;     // [M] optional exact fold output
-    double *uscan = nullptr;     // [nuni] sorted uniforms: running max within kTile tiles
-    double *u_tmax = nullptr;    // [nuni / kTile] tile maxima
-    double *u_tmaxex = nullptr;  // [nuni / kTile] exclusive running max of tiles
+    double *uscan = nullptr;     // [nuni] sorted uniforms: running max within fold tiles
+    double *u_tmax = nullptr;    // [nuni / kFoldTile] tile maxima
+    double *u_tmaxex = nullptr;  // [nuni / kFoldTile] exclusive running max of tiles
     uint64_t nuni = 0;           // capacity of the uniform arrays
     FoldScratch fs{};
     // the sorted uniforms' own fold: tile sums and starts of its own, every
@@ -30,7 +30,7 @@ struct ResampleBuffers {
 // 5 doubles per 2048-particle tile), or nullptr when lw has none
 void launch_resample_from_log_weights(FilterCtl *ctl, const double *lw, uint64_t M, int scheme,
                                       ResampleBuffers &rb, StepRow *hist, hipStream_t st,
-                                      const double *lse_part = nullptr);
+                                      const double *lse_part = nullptr, bool pre_started = false);
 void launch_resample_from_weights(FilterCtl *ctl, const double *w, uint64_t M, int scheme,
                                   ResampleBuffers &rb, StepRow *hist, hipStream_t st);
 void launch_fold_values(FilterCtl *ctl, const double *x, uint64_t M, ResampleBuffers &rb,
@@ -51,8 +51,13 @@ void launch_threefry4x32_fill(const uint32_t key[4], const uint32_t ctr0[4], uin
 void launch_cv_move(bool fp64, FilterCtl *ctl, const void *st_in, void *st_out,
                     const uint32_t *anc, double *lw, const double *obs, const CvParams &p,
                     uint64_t seed, uint64_t t, uint64_t N, double *partials, hipStream_t st);
+// starts (optional, with prologue_scheme a non-residual scheme): on a resample
+// the reduce also runs the scheme's prologue and writes the fold's tile starts
+// of the filter's weights into starts->ts; the resample then runs with
+// pre_started set
 void launch_cv_reduce(FilterCtl *ctl, const double *partials, uint64_t ntiles, StepRow *hist,
-                      uint64_t t, hipStream_t st);
+                      uint64_t t, hipStream_t st, int prologue_scheme = -1,
+                      const FoldScratch *starts = nullptr);
 void launch_cv_monitor(bool fp64, FilterCtl *ctl, const void *state, const uint32_t *anc,
                        const double *lw, uint64_t N, double *partials, StepRow *hist,
                        hipStream_t st);

@@ -27,14 +27,13 @@ constexpr void layout_resample(A &a, ResampleBuffers &rb, uint64_t M, uint64_t n
     const uint64_t F = M > nuni ? M : nuni;
     const uint64_t tiles = (F + kFoldTile - 1) / kFoldTile;    // fold tiles
     const uint64_t utiles = (nuni + kFoldTile - 1) / kFoldTile; // fold tiles of the uniforms
-    const uint64_t mtiles = (nuni + kTile - 1) / kTile;         // running-max tiles
     if (lists) {
         rb.anc = a.template n<uint32_t>(M);
         rb.zl = a.template n<uint32_t>(M);
         rb.sl = a.template n<uint32_t>(M);
     }
-    rb.u_tmax = a.template n<double>(mtiles + 1);
-    rb.u_tmaxex = a.template n<double>(mtiles + 1);
+    rb.u_tmax = a.template n<double>(utiles + 1);
+    rb.u_tmaxex = a.template n<double>(utiles + 1);
     rb.uscan = a.template n<double>(nuni ? nuni : 1);
     rb.nuni = nuni;
     FoldScratch &fs = rb.fs;
