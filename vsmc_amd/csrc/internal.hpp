@@ -20,6 +20,7 @@ constexpr int kMoveItems = 8;                // particles per thread in the move
 constexpr int kMoveTile = kBlock * kMoveItems;  // particles per move workgroup (one partial)
 constexpr int kFoldItems = 8;                // elements per thread in the fold kernels
 constexpr int kFoldTile = kBlock * kFoldItems;  // elements per fold tile
+constexpr int kFoldChunk = 64;                  // tiles per chunk of the segment scan
 // the move's tile partials and the fold's per-tile arrays share one tiling
 static_assert(kMoveTile == kFoldTile, "move and fold tiles coincide");
 
@@ -90,7 +91,11 @@ struct FoldScratch {
     double *tsum;      // [tiles] tile sums of x
     double *ts;        // [tiles] approximate exclusive prefix (fixed-order scan)
     Seg *agg;          // [tiles] segment summary of each tile
-    Seg *pre;          // [tiles] exclusive segment summary
+    Seg *pre;          // [tiles] exclusive segment summary within the tile's chunk of
+                       // kFoldChunk tiles (compose with cpre of the chunk)
+    Seg *cagg;         // [chunks] segment summary of each chunk
+    Seg *cpre;         // [chunks] exclusive segment summary of the chunks
+    uint32_t *tcnt;    // [tiles] pre[t].cnt: specials before tile t within its chunk
     SpecRec *spec;     // [tiles * kFoldTile] special elements, tile-local order
     double *gval;      // [M + 1] exact fold value after each special, fold order
     uint32_t *rep32;   // [M] replication counts for the layout pass

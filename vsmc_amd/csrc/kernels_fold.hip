@@ -65,33 +65,134 @@ __global__ __launch_bounds__(kBlock) void k_fold_seg(FilterCtl *ctl, XSrc xs, ui
                   wx_out);
 }
 
-// ---- pass 4: tile segment prefixes and the exact values at the specials --------
-// One workgroup: the exclusive segment summary of every tile (the special
-// counts also kept in LDS for up to kLdsTiles tiles), then the chain.
-constexpr int kLdsTiles = 16384;
+// ---- pass 4a: segment prefixes within chunks (one wave per chunk) ----------------
+// Each chunk of kFoldChunk consecutive tiles: every tile's exclusive segment
+// summary within the chunk (pre, and its special count in tcnt), and the
+// chunk's summary (cagg).  Spread over many CUs, the loads overlap; a single
+// workgroup over every tile summary was bound by the cache-line misses one CU
+// keeps in flight.
+__global__ __launch_bounds__(kWave) void k_fold_scan_chunks(FilterCtl *ctl, FoldScratch fs)
+{
+    if (!ctl->resample) return;
+    const uint64_t T = fs.tiles;
+    const uint64_t c = blockIdx.x;
+    const uint64_t t = c * kFoldChunk + threadIdx.x;
+    const Seg v = t < T ? fs.agg[t] : seg_identity();
+    const Seg incl = wave_incl_scan(v, SegOp());
+    Seg ex = lane_up(incl, 1);
+    if (threadIdx.x == 0) ex = seg_identity();
+    if (t < T) {
+        fs.pre[t] = ex;
+        fs.tcnt[t] = (uint32_t)ex.cnt;
+    }
+    if (threadIdx.x == kWave - 1) fs.cagg[c] = incl;
+}
+
+// ---- pass 4b: chunk prefixes and the exact values at the specials -----------------
+// One workgroup: the exclusive summary of every chunk (the special counts also
+// kept in LDS for up to kLdsChunks chunks), then the chain.
+constexpr int kLdsChunks = 16384;
+static_assert(kFoldChunk == kWave, "one wave scans one chunk");
+
+// The chain over the specials in fold order, with the tile prefixes in two
+// levels (chunk prefix cpre, then pre within the chunk): as fold_chain, the
+// record of special g is found from the counts (a binary search over the
+// chunks, then the chunk's kFoldChunk tile counts, loaded at once).
+template <int BLOCK>
+__device__ double fold_chain_chunked(const FoldScratch &fs, const uint32_t *ccnt_lds,
+                                     uint64_t C, uint64_t T, const Seg &tail, SpecRec *s_r,
+                                     double *s_v, int *bad)
+{
+    const int tid = threadIdx.x;
+    const uint64_t S = tail.cnt;
+    double a = 0.0;  // running exact value (thread 0)
+    *bad = 0;
+    for (uint64_t c0 = 0; c0 < S; c0 += BLOCK) {
+        const uint64_t g = c0 + tid;
+        if (g < S) {
+            // last chunk whose count of earlier specials is <= g
+            uint64_t lo = 0, hi = C - 1;
+            while (lo < hi) {
+                const uint64_t mid = (lo + hi + 1) / 2;
+                const uint64_t cc = ccnt_lds ? ccnt_lds[mid] : fs.cpre[mid].cnt;
+                if (cc <= g)
+                    lo = mid;
+                else
+                    hi = mid - 1;
+            }
+            const Seg cp = fs.cpre[lo];
+            const uint64_t r = g - cp.cnt;  // rank within the chunk
+            const uint64_t t0 = lo * kFoldChunk;
+            const int len = (int)(T - t0 < (uint64_t)kFoldChunk ? T - t0 : (uint64_t)kFoldChunk);
+            // last tile of the chunk with count <= r (counts are non-decreasing, the first 0)
+            int j = 0;
+#pragma unroll 8
+            for (int q = 1; q < len; ++q) j += fs.tcnt[t0 + q] <= r ? 1 : 0;
+            const uint64_t t = t0 + j;
+            const Seg tp = fs.pre[t];
+            const Seg p = seg_op(cp, tp);
+            const uint64_t k = r - tp.cnt;
+            SpecRec rec = fs.spec[t * (uint64_t)kFT + k];
+            if (k == 0) rec.run = tr_compose(p.tr, rec.run);
+            s_r[tid] = rec;
+        }
+        __syncthreads();
+        if (tid == 0) {
+            const int n = (int)(S - c0 < (uint64_t)BLOCK ? S - c0 : (uint64_t)BLOCK);
+#pragma unroll 1
+            for (int k = 0; k < n; ++k) {
+                double v;
+                if (!tr_apply(s_r[k].run, a, &v)) {
+                    *bad = 1;
+                    v = a;
+                }
+                a = v + s_r[k].x;
+                s_v[k] = a;
+            }
+        }
+        __syncthreads();
+        if (g < S) fs.gval[g] = s_v[tid];
+        __syncthreads();
+    }
+    double tot = a;
+    if (tid == 0 && !tr_apply(tail.tr, a, &tot)) {
+        *bad = 1;
+        tot = a;
+    }
+    return tot;
+}
 
 __global__ __launch_bounds__(kScanBlock) void k_fold_scan_seg(FilterCtl *ctl, FoldScratch fs)
 {
     if (!ctl->resample) return;
     __shared__ Seg s_seg[kScanBlock / kWave];
-    __shared__ uint32_t s_cnt[kLdsTiles + 1];  // specials before each tile
+    __shared__ uint32_t s_cnt[kLdsChunks + 1];  // specials before each chunk
     __shared__ SpecRec s_r[kScanBlock];
     __shared__ double s_v[kScanBlock];
     const uint64_t T = fs.tiles;
-    const bool in_lds = T <= (uint64_t)kLdsTiles;
-    const Seg tail = wg_scan<Seg>(fs.agg, fs.pre, T, SegOp(), seg_identity(), s_seg,
-                                  [&](uint64_t t, const Seg &p) {
-                                      if (in_lds) s_cnt[t] = (uint32_t)p.cnt;
+    const uint64_t C = (T + kFoldChunk - 1) / kFoldChunk;
+    const bool in_lds = C <= (uint64_t)kLdsChunks;
+    const Seg tail = wg_scan<Seg>(fs.cagg, fs.cpre, C, SegOp(), seg_identity(), s_seg,
+                                  [&](uint64_t c, const Seg &p) {
+                                      if (in_lds) s_cnt[c] = (uint32_t)p.cnt;
                                   });
-    __syncthreads();  // pre[] and s_cnt written by other threads
+    __syncthreads();  // cpre[] and s_cnt written by other threads
     int bad;
-    const double tot = fold_chain<kScanBlock>(fs.pre, in_lds ? s_cnt : nullptr, T, tail, fs.spec,
-                                              fs.gval, s_r, s_v, &bad);
+    const double tot =
+        fold_chain_chunked<kScanBlock>(fs, in_lds ? s_cnt : nullptr, C, T, tail, s_r, s_v, &bad);
     if (threadIdx.x == 0) {
         ctl->fold_total = tot;
         ctl->specials = tail.cnt;
         if (bad) atomicOr(&ctl->err, (int)kErrFoldClass);
     }
+}
+
+// both passes of the segment scan
+static void launch_scan_seg(FilterCtl *ctl, FoldScratch &fs, hipStream_t st)
+{
+    const unsigned C = (unsigned)((fs.tiles + kFoldChunk - 1) / kFoldChunk);
+    hipLaunchKernelGGL(k_fold_scan_chunks, dim3(C ? C : 1), dim3(kWave), 0, st, ctl, fs);
+    hipLaunchKernelGGL(k_fold_scan_seg, dim3(1), dim3(kScanBlock), 0, st, ctl, fs);
 }
 
 // ---- pass 5: exact values, inversion, replication --------------------------------
@@ -104,7 +205,8 @@ __global__ __launch_bounds__(kBlock) void k_fold_apply(FilterCtl *ctl, XSrc xs, 
     xs.bind(ctl);
     inv.bind(ctl);
     const uint64_t t = blockIdx.x;
-    fold_tile_apply(ctl, xs, inv, t, M, fs.ts[t], fs.pre[t], fs.gval, out, fs.rep32, fs.zs + t, L,
+    const Seg tpre = seg_op(fs.cpre[t / kFoldChunk], fs.pre[t]);
+    fold_tile_apply(ctl, xs, inv, t, M, fs.ts[t], tpre, fs.gval, out, fs.rep32, fs.zs + t, L,
                     fs.thr ? fs.thr + t * kBlock : nullptr,
                     fs.thr ? fs.thr_fb + t * kBlock : nullptr);
 }
@@ -369,7 +471,7 @@ static void launch_fold(FilterCtl *ctl, XSrc xs, Inv inv, uint64_t M, FoldScratc
         // the segment pass computes x and leaves it for the apply pass, which
         // reads x more than once per element
         hipLaunchKernelGGL((k_fold_seg<XSrc>), grid, dim3(kBlock), 0, st, ctl, xs, M, fs, fs.wx);
-        hipLaunchKernelGGL(k_fold_scan_seg, dim3(1), dim3(kScanBlock), 0, st, ctl, fs);
+        launch_scan_seg(ctl, fs, st);
         if (!Inv::kInv && !out.a_out) return;
         const XCached<XSrc> xc{fs.wx, xs};
         hipLaunchKernelGGL((k_fold_apply<XCached<XSrc>, Inv>), grid, dim3(kBlock), 0, st, ctl, xc,
@@ -386,14 +488,14 @@ static void launch_fold(FilterCtl *ctl, XSrc xs, Inv inv, uint64_t M, FoldScratc
         const XCached<XSrc> xc{fs.wx, xs};
         hipLaunchKernelGGL((k_fold_seg<XCached<XSrc>>), grid, dim3(kBlock), 0, st, ctl, xc, M, fs,
                            (double *)nullptr);
-        hipLaunchKernelGGL(k_fold_scan_seg, dim3(1), dim3(kScanBlock), 0, st, ctl, fs);
+        launch_scan_seg(ctl, fs, st);
         if (!Inv::kInv && !out.a_out) return;
         hipLaunchKernelGGL((k_fold_apply<XCached<XSrc>, Inv>), grid, dim3(kBlock), 0, st, ctl, xc,
                            inv, M, fs, out);
     } else {
         hipLaunchKernelGGL((k_fold_seg<XSrc>), grid, dim3(kBlock), 0, st, ctl, xs, M, fs,
                            (double *)nullptr);
-        hipLaunchKernelGGL(k_fold_scan_seg, dim3(1), dim3(kScanBlock), 0, st, ctl, fs);
+        launch_scan_seg(ctl, fs, st);
         if (!Inv::kInv && !out.a_out) return;
         hipLaunchKernelGGL((k_fold_apply<XSrc, Inv>), grid, dim3(kBlock), 0, st, ctl, xs, inv, M,
                            fs, out);
@@ -412,7 +514,7 @@ static void launch_fold_from_starts(FilterCtl *ctl, XSrc xs, Inv inv, uint64_t M
     const dim3 grid((unsigned)fs.tiles);
     hipLaunchKernelGGL((k_fold_seg<XSrc>), grid, dim3(kBlock), 0, st, ctl, xs, M, fs,
                        (double *)nullptr);
-    hipLaunchKernelGGL(k_fold_scan_seg, dim3(1), dim3(kScanBlock), 0, st, ctl, fs);
+    launch_scan_seg(ctl, fs, st);
     if (!Inv::kInv && !out.a_out) return;
     hipLaunchKernelGGL((k_fold_apply<XSrc, Inv>), grid, dim3(kBlock), 0, st, ctl, xs, inv, M, fs,
                        out);

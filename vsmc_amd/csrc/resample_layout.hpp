@@ -41,6 +41,10 @@ constexpr void layout_resample(A &a, ResampleBuffers &rb, uint64_t M, uint64_t n
     fs.ts = a.template n<double>(tiles);
     fs.agg = a.template n<Seg>(tiles);
     fs.pre = a.template n<Seg>(tiles);
+    const uint64_t chunks = (tiles + kFoldChunk - 1) / kFoldChunk;
+    fs.cagg = a.template n<Seg>(chunks);
+    fs.cpre = a.template n<Seg>(chunks);
+    fs.tcnt = a.template n<uint32_t>(tiles);
     fs.spec = a.template n<SpecRec>(tiles * kFoldTile);
     fs.gval = a.template n<double>(F + 1);
     fs.rep32 = a.template n<uint32_t>(M);
