@@ -204,6 +204,7 @@ def run_batched(nfilters, seed, tune=True):
     for f in range(nfilters):
         b.set_obs(f, obs[f])
     shape = b.tune(1) if tune else (0, 0)
+    report = b.tune_report() if tune else []
     strategy = b.strategy()
     done = np.array([b.steps(f) for f in range(nfilters)], np.uint64)
     ms = b.run()
@@ -217,25 +218,32 @@ def run_batched(nfilters, seed, tune=True):
             "filters": nfilters, "particles": int(ns.sum()), "particle_updates": updates,
             "launch": {"strategy": "ticks" if strategy == V.BATCH_TICKS else "persistent",
                        "workgroups": shape[0], "order": "most work first" if shape[1] == 0
-                       else "as given", "tuned": tune},
+                       else "as given", "tuned": tune,
+                       "candidates_ms": [{"strategy": "ticks" if c[0] == V.BATCH_TICKS
+                                          else "persistent", "workgroups": c[1],
+                                          "order": c[2], "ms": round(c[3], 3)}
+                                         for c in report]},
             "resample_frac": rfrac,
             "roofline": whole_step_roofline(
                 nbytes, ms * 1e-3, "all kernels of every timed step: minimal algorithmic bytes "
                 "(state read/write, log-weight write, ancestor or log-weight read; on "
                 "resampling steps one weight read and one ancestor write) over the timed run"),
             "config": "config-3: CV filters, N log-uniform [2^8, 2^16], T uniform [50, 500], "
-                      "mixed precision, systematic, ESS<N/2; the tuner picks one persistent "
-                      "launch or one tick per step; the timed run covers every remaining step "
-                      "(tuner steps excluded)"}
+                      "mixed precision, systematic, ESS<N/2; the tuner times one persistent "
+                      "launch (several grids and orders) and one tick per step on the same first "
+                      "step, restoring the state after each; the timed run covers every step"}
 
 
-def run_serving(seconds, rate, seed, budget_gb=200.0, max_jobs=8192):
+def run_serving(seconds, rate, seed, budget_gb=200.0, max_jobs=8192, pacing="closed",
+                obs_interval_ms=8.0):
     """Config-4 workload: Poisson arrivals (rate per second) for `seconds`, a
     50/50 mix of the CV model and the 10-D linear-Gaussian model, N
-    log-uniform in [2^10, 2^20], T uniform in [20, 1000].  Pacing: closed
-    loop -- a job's next observation arrives as soon as its previous step has
-    completed (one step at a time).  Jobs are admitted under the HBM budget;
-    the run ends when every submitted job has finished."""
+    log-uniform in [2^10, 2^20], T uniform in [20, 1000].  Pacing "closed": a
+    job's next observation arrives as soon as its previous step has completed
+    (one step at a time).  Pacing "open": a job's k-th observation arrives at
+    its submission time + k * obs_interval_ms whatever the device does, so the
+    latency includes queueing.  Jobs are admitted under the HBM budget; the run
+    ends when every submitted job has finished."""
     rng = np.random.default_rng(seed)
     t_arr = np.cumsum(rng.exponential(1.0 / rate, int(rate * seconds * 1.5) + 16))
     t_arr = t_arr[t_arr < seconds]
@@ -251,8 +259,10 @@ def run_serving(seconds, rate, seed, budget_gb=200.0, max_jobs=8192):
     ids = {}
     fed = {}
     data = {}
+    sub_t = {}
     fully_fed = set()
     finished = 0
+    dt_obs = obs_interval_ms * 1e-3
     t0 = time.perf_counter()
     nxt = 0
     while True:
@@ -261,20 +271,37 @@ def run_serving(seconds, rate, seed, budget_gb=200.0, max_jobs=8192):
             j = s.submit(int(models[nxt]), int(ns[nxt]), int(Ts[nxt]), 500000 + nxt, 600000 + nxt)
             ids[j] = nxt
             fed[j] = 0
+            sub_t[j] = t_arr[nxt]
             data[j] = y_lg[nxt] if models[nxt] == V.MODEL_LG else y_cv[nxt]
             nxt += 1
-        ready = s.ready()
-        if ready.size:
-            ys = [data[int(j)][fed[int(j)]] for j in ready]
-            s.observe_many(ready, ys)
-            for j in ready:
-                j = int(j)
-                fed[j] += 1
+        if pacing == "closed":
+            ready = s.ready()
+            if ready.size:
+                ys = [data[int(j)][fed[int(j)]] for j in ready]
+                s.observe_many(ready, ys)
+                for j in ready:
+                    j = int(j)
+                    fed[j] += 1
+                    if fed[j] == len(data[j]):
+                        fully_fed.add(j)
+        else:
+            due, ys = [], []
+            for j in list(fed):
+                if j in fully_fed:
+                    continue
+                upto = min(len(data[j]), int((now - sub_t[j]) / dt_obs) + 1)
+                while fed[j] < upto:
+                    due.append(j)
+                    ys.append(data[j][fed[j]])
+                    fed[j] += 1
                 if fed[j] == len(data[j]):
                     fully_fed.add(j)
+            if due:
+                s.observe_many(np.array(due, np.uint64), ys)
         launched = s.poll(wait=True)
         for j in [j for j in fully_fed if s.job(j)[0] == 2]:
             fully_fed.discard(j)
+            del fed[j]
             finished += 1
         if nxt >= k and finished == k:
             break
@@ -288,12 +315,15 @@ def run_serving(seconds, rate, seed, budget_gb=200.0, max_jobs=8192):
         nbytes += float(ns[q]) * step_bytes(dim, s.history(j, int(Ts[q])))
     s.close()
     ctx.close()
+    pace = ("closed loop (next observation when the previous step completes)"
+            if pacing == "closed" else
+            f"open loop (a job's observations every {obs_interval_ms:g} ms from its submission)")
     return {"value": st["particle_updates"] / (st["elapsed_ms"] * 1e-3), "unit": "particle-updates/s",
             "p50_step_ms": st["p50_us"] * 1e-3, "p99_step_ms": st["p99_us"] * 1e-3,
             "jobs": k, "steps": st["steps"], "particle_updates": st["particle_updates"],
             "device_busy_frac": st["busy_ms"] / st["elapsed_ms"] if st["elapsed_ms"] else None,
             "peak_particles": st["peak_particles"], "capacity_particles": st["capacity_particles"],
-            "wall_s": wall,
+            "wall_s": wall, "pacing": pacing,
             "roofline": whole_step_roofline(
                 nbytes, st["elapsed_ms"] * 1e-3, "all kernels of every tick: minimal algorithmic "
                 "bytes of every job step (FP32 state of 4 or 10 values read/written, log-weight "
@@ -302,8 +332,7 @@ def run_serving(seconds, rate, seed, budget_gb=200.0, max_jobs=8192):
             "config": f"config-4: Poisson arrivals {rate}/s for {seconds}s, 50/50 CV and 10-D "
                       f"linear-Gaussian (A spectral radius 0.9, q 0.1, r 0.2, s0 0.25), N "
                       f"log-uniform [2^10, 2^20], T uniform [20, 1000], {budget_gb:g} GB budget, "
-                      f"mixed precision, systematic; pacing: closed loop (next observation when "
-                      f"the previous step completes)"}
+                      f"mixed precision, systematic; pacing: {pace}"}
 
 
 GMM_FLOPS_PER_POINT = 6  # per component: x - mu, * A, FMA (2), exp2, accumulate
@@ -425,6 +454,9 @@ def main():
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-seconds", type=float, default=0.5)
     ap.add_argument("--serve-rate", type=float, default=2000.0)
+    ap.add_argument("--serve-open-ms", type=float, default=8.0,
+                    help="also serve with open-loop pacing, one observation per job every this "
+                         "many ms (0: skip)")
     ap.add_argument("--batch-filters", type=int, default=4096)
     ap.add_argument("--no-gmm", action="store_true")
     ap.add_argument("--serial-filters", action="store_true",
@@ -548,6 +580,12 @@ def main():
         print(f"[bench] serving: {a.serve_rate}/s for {a.serve_seconds}s", file=sys.stderr,
               flush=True)
         out["serving"] = run_serving(a.serve_seconds, a.serve_rate, 17)
+        if a.serve_open_ms > 0:
+            print(f"[bench] serving open loop: observations every {a.serve_open_ms} ms",
+                  file=sys.stderr, flush=True)
+            out["serving"]["open_loop"] = run_serving(a.serve_seconds, a.serve_rate, 17,
+                                                      pacing="open",
+                                                      obs_interval_ms=a.serve_open_ms)
     if not a.no_gmm:
         print(f"[bench] gmm: N={a.gmm_n}, {a.gmm_sweeps} sweeps", file=sys.stderr, flush=True)
         gm = run_gmm(a.gmm_n, a.gmm_sweeps, 23)

@@ -242,10 +242,16 @@ int vsmcx_batch_set_launch(vsmcx_batch *b, uint32_t workgroups, int order);
 #define VSMCX_BATCH_TICKS 1
 int vsmcx_batch_set_strategy(vsmcx_batch *b, int strategy);
 int vsmcx_batch_get_strategy(vsmcx_batch *b, int *strategy);
-/* Autotuner: runs `steps` real steps under each candidate (ticks, and
- * persistent grids of several sizes and both filter orders) and keeps the
- * fastest (the results do not depend on the choice). */
+/* Autotuner, after the OpenCL manager's local-size sweep (profile_kernel,
+ * include/vsmc/opencl/cl_manager.hpp:385-441): every candidate (ticks, and
+ * persistent grids of several sizes and both filter orders) runs the same
+ * `steps` steps from the same state, which is restored after each, so tuning
+ * advances no filter; the fastest is kept (the results do not depend on the
+ * choice).  vsmcx_batch_tune_report gives the last tuning's candidates (up to
+ * max) and their times in ms; *count receives how many there were. */
 int vsmcx_batch_tune(vsmcx_batch *b, uint64_t steps, uint32_t *workgroups, int *order);
+int vsmcx_batch_tune_report(vsmcx_batch *b, uint64_t max, uint64_t *count, int *strategy,
+                            uint32_t *workgroups, int *order, double *ms);
 int vsmcx_batch_steps(vsmcx_batch *b, uint64_t f, uint64_t *t);
 int vsmcx_batch_history(vsmcx_batch *b, uint64_t f, uint64_t count, vsmcx_step_stats *out);
 int vsmcx_batch_state(vsmcx_batch *b, uint64_t f, double *state, double *log_weight);
@@ -257,11 +263,13 @@ int vsmcx_batch_particles(vsmcx_batch *b, uint64_t *total);
  * time (vsmcx_serve_observe).  Jobs are admitted in arrival order while a job
  * slot and room in the HBM arena are free, keep their particle state on the
  * device until their last step, and are advanced with continuous batching:
- * each vsmcx_serve_poll finishes the tick in flight, admits jobs, and launches
- * one tick: one step of every running job with an observation waiting, each
- * phase of the step one grid launch over the tiles of all those jobs.  A
- * job's results do not depend on the other jobs.  No counterpart in vSMC
- * (one Sampler per job, stepped by its caller). */
+ * each vsmcx_serve_poll completes the ticks the device has finished, admits
+ * jobs, and launches one tick: one step of every running job with an
+ * observation waiting, each phase of the step one grid launch over the tiles
+ * of all those jobs.  Up to two ticks are in flight, so the host stages the
+ * next tick while the device runs one; a job's steps stay in order.  A job's
+ * results do not depend on the other jobs.  No counterpart in vSMC (one
+ * Sampler per job, stepped by its caller). */
 #define VSMCX_MODEL_CV 0  /* the pf example's constant-velocity model */
 #define VSMCX_MODEL_LG 1  /* linear-Gaussian: x_t = A x_{t-1} + q z, y_t = x_t + r v */
 
@@ -315,8 +323,10 @@ int vsmcx_serve_observe_many(vsmcx_serve *s, uint64_t count, const uint64_t *ids
 /* jobs (waiting or running) that have used every observation they received
  * and expect more: up to `max` ids, *count = how many there are */
 int vsmcx_serve_ready(vsmcx_serve *s, uint64_t max, uint64_t *ids, uint64_t *count);
-/* finish the tick in flight (waiting for it if `wait`, else only if done),
- * admit jobs, launch the next tick; *launched = job steps in it */
+/* complete the ticks the device has finished; with two in flight, wait for
+ * the oldest (`wait`) or return at once; admit jobs, launch the next tick
+ * (*launched = job steps in it).  With `wait`, when nothing could be
+ * launched, wait for the oldest tick in flight and try once more. */
 int vsmcx_serve_poll(vsmcx_serve *s, int wait, uint64_t *launched);
 /* state: 0 waiting, 1 running, 2 done */
 int vsmcx_serve_job(vsmcx_serve *s, uint64_t id, int *state, uint64_t *steps);

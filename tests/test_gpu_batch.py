@@ -193,8 +193,15 @@ def test_batch_tuner_keeps_results(ctx):
         o = _obs(6, 70 + f)
         b.set_obs(f, o)
         ref.set_obs(f, o)
-    wg, order = b.tune(1)  # candidate strategies and shapes, one real step each
+    wg, order = b.tune(2)  # candidate strategies and shapes, each on the same two steps
     assert wg >= 1 and order in (0, 1) and b.strategy() in (V.BATCH_PERSISTENT, V.BATCH_TICKS)
+    # tuning restores the state after every candidate: no filter advanced
+    assert all(b.steps(f) == 0 for f in range(16))
+    rep = b.tune_report()
+    assert len(rep) >= 2 and all(ms > 0 for *_, ms in rep)
+    assert {r[0] for r in rep} == {V.BATCH_PERSISTENT, V.BATCH_TICKS}
+    best = min(rep, key=lambda r: r[3])
+    assert b.strategy() == best[0] and (best[0] == V.BATCH_TICKS or wg == best[1])
     b.run()
     ref.run()
     for f in range(16):

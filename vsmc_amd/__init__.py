@@ -158,6 +158,7 @@ def lib():
         "vsmcx_batch_set_strategy": ([vp, i32], i32),
         "vsmcx_batch_get_strategy": ([vp, P(i32)], i32),
         "vsmcx_batch_tune": ([vp, u64, P(C.c_uint32), P(i32)], i32),
+        "vsmcx_batch_tune_report": ([vp, u64, P(u64), vp, vp, vp, vp], i32),
         "vsmcx_batch_steps": ([vp, u64, P(u64)], i32),
         "vsmcx_batch_history": ([vp, u64, u64, vp], i32),
         "vsmcx_batch_state": ([vp, u64, vp, vp], i32),
@@ -537,9 +538,24 @@ class Batch:
         _check(lib().vsmcx_batch_set_launch(self.h, workgroups, order))
 
     def tune(self, steps=1):
+        """Time every launch candidate on the same `steps` steps (state restored
+        after each: nothing advances) and keep the fastest."""
         wg, order = C.c_uint32(), C.c_int()
         _check(lib().vsmcx_batch_tune(self.h, steps, C.byref(wg), C.byref(order)))
         return wg.value, order.value
+
+    def tune_report(self):
+        """[(strategy, workgroups, order, ms)] of the last tuning."""
+        n = C.c_uint64()
+        _check(lib().vsmcx_batch_tune_report(self.h, 0, C.byref(n), None, None, None, None))
+        k = n.value
+        st = np.zeros(k, np.int32)
+        wg = np.zeros(k, np.uint32)
+        od = np.zeros(k, np.int32)
+        ms = np.zeros(k)
+        _check(lib().vsmcx_batch_tune_report(self.h, k, C.byref(n), st.ctypes.data,
+                                             wg.ctypes.data, od.ctypes.data, ms.ctypes.data))
+        return [(int(st[i]), int(wg[i]), int(od[i]), float(ms[i])) for i in range(k)]
 
     def set_strategy(self, strategy):
         """BATCH_PERSISTENT (one launch, workgroups take filters from a queue) or

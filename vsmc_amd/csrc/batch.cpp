@@ -38,8 +38,43 @@ struct vsmcx_batch {
     std::vector<uint32_t> jobs_h;
     std::vector<TickItem> items_h;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // the tuner's candidates and their times (ms per tuning run)
+    struct Cand {
+        int strategy;
+        unsigned wg;
+        int order;
+        double ms;
+    };
+    std::vector<Cand> tuned;
 
     size_t state_bytes() const { return fp64 ? 32 : 16; }
+
+    // Everything a step reads and then overwrites, so that every tuning
+    // candidate starts from the same state: both state buffers, log-weights,
+    // ancestors, control blocks, step counters, buffer selectors, error bits
+    // and the history rows.  The rest of the pool is scratch a step writes
+    // before it reads.
+    std::vector<std::pair<void *, size_t>> mutable_arrays() const
+    {
+        return {{P.st0, total * state_bytes()},
+                {P.st1, total * state_bytes()},
+                {P.lw, 8 * total},
+                {P.anc, 4 * total},
+                {P.ctl, sizeof(FilterCtl) * nf},
+                {P.next_t, 8 * (size_t)nf},
+                {P.cur, 4 * (size_t)nf},
+                {P.err_any, sizeof(int)},
+                {P.hist, sizeof(StepRow) * total_obs}};
+    }
+    void copy_arrays(const std::vector<void *> &saved, bool restore)
+    {
+        const auto arrs = mutable_arrays();
+        for (size_t k = 0; k < arrs.size(); ++k) {
+            void *live = arrs[k].first;
+            VSMCX_HIP(hipMemcpyAsync(restore ? live : saved[k], restore ? saved[k] : live,
+                                     arrs[k].second, hipMemcpyDeviceToDevice, ctx->stream));
+        }
+    }
 
     void release_all()
     {
@@ -311,25 +346,59 @@ int vsmcx_batch_tune(vsmcx_batch *b, uint64_t steps, uint32_t *workgroups, int *
     std::sort(cand.begin(), cand.end());
     cand.erase(std::unique(cand.begin(), cand.end()), cand.end());
     hipStream_t s = b->ctx->stream;
-    double best = 1e300;
-    std::tuple<int, unsigned, int> pick{b->strategy, b->workgroups, b->order};
-    for (const auto &c : cand) {
-        VSMCX_HIP(hipEventRecord(b->ev0, s));
-        b->launch(steps, std::get<1>(c), std::get<2>(c), std::get<0>(c));
-        VSMCX_HIP(hipEventRecord(b->ev1, s));
-        b->check_errors();
-        float ms = 0;
-        VSMCX_HIP(hipEventElapsedTime(&ms, b->ev0, b->ev1));
-        if (ms < best) {
-            best = ms;
-            pick = c;
+    // every candidate runs the same `steps` steps from the same state, which is
+    // restored afterwards: tuning advances no filter
+    const auto arrs = b->mutable_arrays();
+    std::vector<void *> saved;
+    const std::vector<uint64_t> next_saved = b->next_h;
+    try {
+        for (const auto &a : arrs) saved.push_back(b->ctx->alloc(a.second));
+        b->copy_arrays(saved, false);
+        double best = 1e300;
+        std::tuple<int, unsigned, int> pick{b->strategy, b->workgroups, b->order};
+        b->tuned.clear();
+        for (const auto &c : cand) {
+            VSMCX_HIP(hipEventRecord(b->ev0, s));
+            b->launch(steps, std::get<1>(c), std::get<2>(c), std::get<0>(c));
+            VSMCX_HIP(hipEventRecord(b->ev1, s));
+            b->check_errors();
+            float ms = 0;
+            VSMCX_HIP(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+            b->tuned.push_back({std::get<0>(c), std::get<1>(c), std::get<2>(c), (double)ms});
+            if (ms < best) {
+                best = ms;
+                pick = c;
+            }
+            b->copy_arrays(saved, true);
+            b->next_h = next_saved;
         }
+        b->ctx->sync();
+        b->strategy = std::get<0>(pick);
+        b->workgroups = std::get<1>(pick);
+        b->order = std::get<2>(pick);
+    } catch (...) {
+        for (void *q : saved) b->ctx->release(q);
+        throw;
     }
-    b->strategy = std::get<0>(pick);
-    b->workgroups = std::get<1>(pick);
-    b->order = std::get<2>(pick);
+    for (void *q : saved) b->ctx->release(q);
     if (workgroups) *workgroups = b->workgroups;
     if (order) *order = b->order;
+    API_END
+}
+
+int vsmcx_batch_tune_report(vsmcx_batch *b, uint64_t max, uint64_t *count, int *strategy,
+                            uint32_t *workgroups, int *order, double *ms)
+{
+    API_BEGIN
+    REQUIRE(b && count, "NULL argument");
+    *count = b->tuned.size();
+    for (uint64_t k = 0; k < std::min<uint64_t>(max, b->tuned.size()); ++k) {
+        const auto &c = b->tuned[k];
+        if (strategy) strategy[k] = c.strategy;
+        if (workgroups) workgroups[k] = c.wg;
+        if (order) order[k] = c.order;
+        if (ms) ms[k] = c.ms;
+    }
     API_END
 }
 

@@ -83,6 +83,7 @@ struct Job {
     int state = 0;  // 0 queued, 1 running, 2 done
     int slot = -1;
     uint64_t done = 0;      // steps completed
+    uint64_t issued = 0;    // steps launched (done plus the ticks in flight)
     uint64_t received = 0;  // observations received
     uint64_t uploaded = 0;  // observations copied to the device
     uint64_t p_off = 0, p_units = 0, s_off = 0, s_units = 0, o_off = 0, o_units = 0;
@@ -94,6 +95,20 @@ struct Job {
 constexpr uint64_t kUnit = kFoldTile;  // particle-arena unit
 
 uint64_t align16(uint64_t b) { return (b + 15) & ~(uint64_t)15; }
+
+// Up to two ticks are in flight on the stream: while the device runs one, the
+// host stages and launches the next.  Each has its own staging buffers.
+constexpr int kTickDepth = 2;
+
+// One launched tick: its jobs, its staging buffer, and the finished jobs
+// whose history rows it copies back (offset into that buffer's history area,
+// or kNoRows when that area was full and the rows are read at completion).
+struct Tick {
+    std::vector<uint32_t> slots;
+    int buf = 0;
+    std::vector<std::pair<uint32_t, uint64_t>> finished;  // (slot, first row)
+};
+constexpr uint64_t kNoRows = ~0ull;
 
 }  // namespace
 
@@ -111,7 +126,6 @@ struct vsmcx_serve {
     BatchPool P{};
     BatchFilter *df = nullptr;
     std::vector<TickItem> items_h;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
     std::vector<int> free_slots;
     std::vector<uint64_t> slot_job;
     // one tick's upload, staged on the host and sent as one copy
@@ -121,13 +135,18 @@ struct vsmcx_serve {
     std::vector<double> st_pool;
     std::vector<uint8_t> st_mark;  // per slot: already in st_avail
     uint64_t pool_cap = 0, stage_cap = 0;
-    uint8_t *stage_h = nullptr;    // pinned
-    uint8_t *stage_d = nullptr;
-    int *err_h = nullptr;          // pinned: the device error bits after the last tick
+    // per staging buffer (one per tick in flight)
+    uint8_t *stage_h[kTickDepth] = {};   // pinned
+    uint8_t *stage_d[kTickDepth] = {};
+    int *err_h[kTickDepth] = {};         // pinned: the device error bits after that tick
+    StepRow *hist_h[kTickDepth] = {};    // pinned: history rows of jobs finishing in it
+    uint64_t hist_cap = 0;               // rows per buffer
+    hipEvent_t tev0[kTickDepth] = {}, tev1[kTickDepth] = {};
+    std::deque<Tick> flight;             // launched, not yet completed (oldest first)
+    int next_buf = 0;
     std::unordered_map<uint64_t, Job> jobs;
     std::deque<uint64_t> waiting;
     uint64_t next_id = 1;
-    bool inflight = false;
     std::vector<uint32_t> tick;
     std::vector<double> lat_us;
     uint64_t steps = 0, updates = 0, jobs_done = 0;
@@ -143,14 +162,17 @@ struct vsmcx_serve {
         void *ps[] = {df, P.st0, P.st1, P.lw, P.anc, P.zl, P.sl, P.rep32,
                       P.spec, P.gval, P.ts, P.tagg, P.tpre, P.tzs, P.tsum, P.tzsx, P.tpart,
                       P.wx, P.thr, P.thr_fb,
-                      stage_d, P.ctl, P.hist, (void *)P.obs,
+                      stage_d[0], stage_d[1], P.ctl, P.hist, (void *)P.obs,
                       P.next_t, (void *)P.avail,
                       P.cur, P.err_any};
         for (void *q : ps) ctx->release(q);
-        if (ev0) (void)hipEventDestroy(ev0);
-        if (ev1) (void)hipEventDestroy(ev1);
-        if (stage_h) (void)hipHostFree(stage_h);
-        if (err_h) (void)hipHostFree(err_h);
+        for (int b = 0; b < kTickDepth; ++b) {
+            if (tev0[b]) (void)hipEventDestroy(tev0[b]);
+            if (tev1[b]) (void)hipEventDestroy(tev1[b]);
+            if (stage_h[b]) (void)hipHostFree(stage_h[b]);
+            if (err_h[b]) (void)hipHostFree(err_h[b]);
+            if (hist_h[b]) (void)hipHostFree(hist_h[b]);
+        }
     }
 
     void mark_avail(const Job &J)
@@ -236,50 +258,79 @@ struct vsmcx_serve {
         mark_avail(J);
     }
 
-    // the tick in flight has finished: account for it, retire finished jobs
+    // the oldest tick in flight has finished: account for it, retire the jobs
+    // it finished (their history rows are on the host already)
     void complete()
     {
+        const Tick k = flight.front();
+        flight.pop_front();
         const double t = now_us();
         float ms = 0;
-        VSMCX_HIP(hipEventElapsedTime(&ms, ev0, ev1));
+        VSMCX_HIP(hipEventElapsedTime(&ms, tev0[k.buf], tev1[k.buf]));
         busy_ms += ms;
-        const int e = *err_h;
+        const int e = *err_h[k.buf];
         if (e & kErrDegenerate) throw DevError(VSMCX_ERR_DEGENERATE, "a job's weights sum to zero or are not finite");
         if (e) throw DevError(VSMCX_ERR_INTERNAL, "device consistency check failed (bits " + std::to_string(e) + ")");
-        for (uint32_t slot : tick) {
+        for (uint32_t slot : k.slots) {
             Job &J = jobs[slot_job[slot]];
             lat_us.push_back(t - J.arrival[J.done]);
             J.done += 1;
             steps += 1;
             updates += J.d.n;
-            if (J.done == J.d.horizon) {
-                J.history.resize(J.d.horizon);
-                VSMCX_HIP(hipMemcpy(J.history.data(), P.hist + (uint64_t)slot * max_horizon,
-                                    sizeof(StepRow) * J.d.horizon, hipMemcpyDeviceToHost));
-                parts.release(J.p_off, J.p_units);
-                states.release(J.s_off, J.s_units);
-                obsr.release(J.o_off, J.o_units);
-                free_slots.push_back(slot);
-                slot_job[slot] = 0;
-                J.state = 2;
-                J.slot = -1;
-                jobs_done += 1;
-            }
         }
-        tick.clear();
-        inflight = false;
+        for (const auto &fr : k.finished) {
+            const uint32_t slot = fr.first;
+            Job &J = jobs[slot_job[slot]];
+            J.history.resize(J.d.horizon);
+            if (fr.second != kNoRows) {
+                std::memcpy(J.history.data(), hist_h[k.buf] + fr.second,
+                            sizeof(StepRow) * J.d.horizon);
+            } else {  // the tick's history area was full: read the rows now
+                VSMCX_HIP(hipMemcpyAsync(J.history.data(), P.hist + (uint64_t)slot * max_horizon,
+                                         sizeof(StepRow) * J.d.horizon, hipMemcpyDeviceToHost,
+                                         ctx->stream));
+                ctx->sync();
+            }
+            parts.release(J.p_off, J.p_units);
+            states.release(J.s_off, J.s_units);
+            obsr.release(J.o_off, J.o_units);
+            free_slots.push_back(slot);
+            slot_job[slot] = 0;
+            J.state = 2;
+            J.slot = -1;
+            jobs_done += 1;
+        }
         t_last = t;
     }
 
+    // complete every tick in flight that the device has finished; with
+    // `wait`, at least the oldest one
+    void retire(bool wait)
+    {
+        while (!flight.empty()) {
+            const int b = flight.front().buf;
+            if (wait) {
+                VSMCX_HIP(hipEventSynchronize(tev1[b]));
+                wait = false;
+            } else {
+                const hipError_t q = hipEventQuery(tev1[b]);
+                if (q == hipErrorNotReady) return;
+                VSMCX_HIP(q);
+            }
+            complete();
+        }
+    }
+
     // one tick: the staged upload, then one step of every running job with an
-    // observation on the device and not yet stepped.  Returns the jobs stepped.
+    // observation on the device not yet stepped (nor in a tick in flight).
+    // Returns the jobs stepped.  Needs a free staging buffer.
     uint64_t launch()
     {
         tick.clear();
         for (size_t slot = 0; slot < slot_job.size(); ++slot) {
             if (!slot_job[slot]) continue;
             const Job &J = jobs[slot_job[slot]];
-            if (J.state == 1 && J.uploaded > J.done) tick.push_back((uint32_t)slot);
+            if (J.state == 1 && J.uploaded > J.issued) tick.push_back((uint32_t)slot);
         }
         items_h.clear();
         for (uint32_t slot : tick) {
@@ -287,6 +338,8 @@ struct vsmcx_serve {
             for (uint64_t tt = 0; tt < T; ++tt) items_h.push_back(TickItem{slot, (uint32_t)tt});
         }
         if (tick.empty() && st_init.empty() && st_runs.empty() && st_avail.empty()) return 0;
+        const int b = next_buf;
+        uint8_t *sh = stage_h[b], *sd = stage_d[b];
         // pack: init | runs | avail | jobs | items | pool, each 16-byte aligned
         uint64_t o = 0;
         const uint64_t o_init = o; o = align16(o + sizeof(SlotInit) * st_init.size());
@@ -296,37 +349,54 @@ struct vsmcx_serve {
         const uint64_t o_items = o; o = align16(o + sizeof(TickItem) * items_h.size());
         const uint64_t o_pool = o; o = align16(o + 8 * st_pool.size());
         if (o > stage_cap) throw DevError(VSMCX_ERR_INTERNAL, "tick upload exceeds its staging buffer");
-        std::memcpy(stage_h + o_init, st_init.data(), sizeof(SlotInit) * st_init.size());
-        std::memcpy(stage_h + o_runs, st_runs.data(), sizeof(ObsRun) * st_runs.size());
-        std::memcpy(stage_h + o_av, st_avail.data(), sizeof(AvailSet) * st_avail.size());
-        std::memcpy(stage_h + o_jobs, tick.data(), 4 * tick.size());
-        std::memcpy(stage_h + o_items, items_h.data(), sizeof(TickItem) * items_h.size());
-        std::memcpy(stage_h + o_pool, st_pool.data(), 8 * st_pool.size());
+        std::memcpy(sh + o_init, st_init.data(), sizeof(SlotInit) * st_init.size());
+        std::memcpy(sh + o_runs, st_runs.data(), sizeof(ObsRun) * st_runs.size());
+        std::memcpy(sh + o_av, st_avail.data(), sizeof(AvailSet) * st_avail.size());
+        std::memcpy(sh + o_jobs, tick.data(), 4 * tick.size());
+        std::memcpy(sh + o_items, items_h.data(), sizeof(TickItem) * items_h.size());
+        std::memcpy(sh + o_pool, st_pool.data(), 8 * st_pool.size());
         hipStream_t s = ctx->stream;
-        VSMCX_HIP(hipEventRecord(ev0, s));
-        VSMCX_HIP(hipMemcpyAsync(stage_d, stage_h, o, hipMemcpyHostToDevice, s));
+        VSMCX_HIP(hipEventRecord(tev0[b], s));
+        VSMCX_HIP(hipMemcpyAsync(sd, sh, o, hipMemcpyHostToDevice, s));
         TickUpload u{};
-        u.init = (const SlotInit *)(stage_d + o_init);
-        u.runs = (const ObsRun *)(stage_d + o_runs);
-        u.avail = (const AvailSet *)(stage_d + o_av);
-        u.pool = (const double *)(stage_d + o_pool);
+        u.init = (const SlotInit *)(sd + o_init);
+        u.runs = (const ObsRun *)(sd + o_runs);
+        u.avail = (const AvailSet *)(sd + o_av);
+        u.pool = (const double *)(sd + o_pool);
         u.n_init = (uint32_t)st_init.size();
         u.n_runs = (uint32_t)st_runs.size();
         u.n_avail = (uint32_t)st_avail.size();
         launch_tick_upload(u, df, P, s);
-        launch_tick(fp64, scheme, (const TickItem *)(stage_d + o_items), (uint32_t)items_h.size(),
-                    (const uint32_t *)(stage_d + o_jobs), (uint32_t)tick.size(), df, P, cv, lg, s);
+        launch_tick(fp64, scheme, (const TickItem *)(sd + o_items), (uint32_t)items_h.size(),
+                    (const uint32_t *)(sd + o_jobs), (uint32_t)tick.size(), df, P, cv, lg, s);
         VSMCX_HIP(hipGetLastError());
-        VSMCX_HIP(hipMemcpyAsync(err_h, P.err_any, sizeof(int), hipMemcpyDeviceToHost, s));
-        VSMCX_HIP(hipEventRecord(ev1, s));
+        // the history rows of the jobs this tick finishes, copied back behind it
+        Tick k;
+        k.buf = b;
+        uint64_t rows = 0;
+        for (uint32_t slot : tick) {
+            Job &J = jobs[slot_job[slot]];
+            J.issued += 1;
+            if (J.issued < J.d.horizon) continue;
+            uint64_t at = kNoRows;
+            if (rows + J.d.horizon <= hist_cap) {
+                at = rows;
+                VSMCX_HIP(hipMemcpyAsync(hist_h[b] + at, P.hist + (uint64_t)slot * max_horizon,
+                                         sizeof(StepRow) * J.d.horizon, hipMemcpyDeviceToHost, s));
+                rows += J.d.horizon;
+            }
+            k.finished.push_back({slot, at});
+        }
+        VSMCX_HIP(hipMemcpyAsync(err_h[b], P.err_any, sizeof(int), hipMemcpyDeviceToHost, s));
+        VSMCX_HIP(hipEventRecord(tev1[b], s));
         for (const AvailSet &a : st_avail) st_mark[a.slot] = 0;
         st_init.clear();
         st_runs.clear();
         st_avail.clear();
         st_pool.clear();
-        // the staging buffer is in use until ev1: the next tick is staged only
-        // after complete() has waited for it
-        inflight = true;
+        k.slots = tick;
+        flight.push_back(std::move(k));
+        next_buf = (b + 1) % kTickDepth;
         if (t_first < 0) t_first = now_us();
         return tick.size();
     }
@@ -404,10 +474,18 @@ int vsmcx_serve_create(vsmcx_ctx *ctx, const vsmcx_serve_desc *d, vsmcx_serve **
         S->stage_cap = align16(sizeof(SlotInit) * d->max_jobs) + align16(sizeof(ObsRun) * d->max_jobs) +
                        align16(sizeof(AvailSet) * d->max_jobs) + align16(4 * (uint64_t)d->max_jobs) +
                        align16(sizeof(TickItem) * S->cap_units) + align16(8 * S->pool_cap);
-        S->stage_d = (uint8_t *)ctx->alloc(S->stage_cap);
-        VSMCX_HIP(hipHostMalloc((void **)&S->stage_h, S->stage_cap, hipHostMallocDefault));
-        VSMCX_HIP(hipHostMalloc((void **)&S->err_h, sizeof(int), hipHostMallocDefault));
-        *S->err_h = 0;
+        // history rows copied back per tick: 64 finishing jobs at the longest horizon
+        S->hist_cap = 64 * d->max_horizon;
+        for (int b = 0; b < kTickDepth; ++b) {
+            S->stage_d[b] = (uint8_t *)ctx->alloc(S->stage_cap);
+            VSMCX_HIP(hipHostMalloc((void **)&S->stage_h[b], S->stage_cap, hipHostMallocDefault));
+            VSMCX_HIP(hipHostMalloc((void **)&S->err_h[b], sizeof(int), hipHostMallocDefault));
+            *S->err_h[b] = 0;
+            VSMCX_HIP(hipHostMalloc((void **)&S->hist_h[b], sizeof(StepRow) * S->hist_cap,
+                                    hipHostMallocDefault));
+            VSMCX_HIP(hipEventCreate(&S->tev0[b]));
+            VSMCX_HIP(hipEventCreate(&S->tev1[b]));
+        }
         P.ctl = ctx->alloc_n<FilterCtl>(d->max_jobs);
         P.hist = ctx->alloc_n<StepRow>((uint64_t)d->max_jobs * d->max_horizon);
         P.obs = ctx->alloc_n<double>(S->cap_obs);
@@ -416,8 +494,6 @@ int vsmcx_serve_create(vsmcx_ctx *ctx, const vsmcx_serve_desc *d, vsmcx_serve **
         P.cur = ctx->alloc_zero<uint32_t>(d->max_jobs);
         P.err_any = ctx->alloc_zero<int>(1);
         S->df = ctx->alloc_n<BatchFilter>(d->max_jobs);
-        VSMCX_HIP(hipEventCreate(&S->ev0));
-        VSMCX_HIP(hipEventCreate(&S->ev1));
         ctx->sync();
     } catch (...) {
         S->release_all();
@@ -504,20 +580,26 @@ int vsmcx_serve_poll(vsmcx_serve *s, int wait, uint64_t *launched)
     API_BEGIN
     REQUIRE(s, "NULL serve");
     if (launched) *launched = 0;
-    if (s->inflight) {
-        if (wait) {
-            VSMCX_HIP(hipEventSynchronize(s->ev1));
-        } else {
-            const hipError_t q = hipEventQuery(s->ev1);
-            if (q == hipErrorNotReady) return VSMCX_OK;
-            VSMCX_HIP(q);
-        }
-        s->complete();
+    // ticks the device has finished; with both staging buffers in use, wait
+    // for the oldest (or return at once without wait)
+    s->retire(false);
+    if (s->flight.size() >= (size_t)kTickDepth) {
+        if (!wait) return VSMCX_OK;
+        s->retire(true);
     }
     s->admit();
     for (auto &kv : s->jobs)
         if (kv.second.state == 1) s->upload(kv.second);
-    const uint64_t n = s->launch();
+    uint64_t n = s->launch();
+    if (n == 0 && wait && !s->flight.empty()) {
+        // nothing new to run: wait for the oldest tick, then try again (its
+        // completion may free a slot or make a job ready)
+        s->retire(true);
+        s->admit();
+        for (auto &kv : s->jobs)
+            if (kv.second.state == 1) s->upload(kv.second);
+        n = s->launch();
+    }
     if (launched) *launched = n;
     API_END
 }
