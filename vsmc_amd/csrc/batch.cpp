@@ -461,13 +461,20 @@ int vsmcx_batch_state(vsmcx_batch *b, uint64_t f, double *state, double *log_wei
     if (state) {
         const size_t sb = b->state_bytes();
         std::vector<unsigned char> raw(N * sb);
-        std::vector<uint32_t> anc(N);
+        std::vector<uint32_t> anc(N), sl(N);
         const void *src = (const unsigned char *)(cur ? b->P.st1 : b->P.st0) + F.soff * (sb / 4);
         VSMCX_HIP(hipMemcpyAsync(raw.data(), src, N * sb, hipMemcpyDeviceToHost, s));
         VSMCX_HIP(hipMemcpyAsync(anc.data(), b->P.anc + F.off, 4 * N, hipMemcpyDeviceToHost, s));
+        VSMCX_HIP(hipMemcpyAsync(sl.data(), b->P.sl + F.off, 4 * N, hipMemcpyDeviceToHost, s));
         b->ctx->sync();
         for (uint64_t i = 0; i < N; ++i) {
-            const uint64_t j = h.prev_resampled && anc[i] < N ? anc[i] : i;
+            uint64_t j = i;
+            if (h.prev_resampled) {
+                const uint32_t a = anc[i];
+                const uint64_t q = (a & kAncZero) ? (uint64_t)(a & ~kAncZero) : N;
+                j = (a & kAncZero) ? (q < N ? sl[q] : i) : a;
+                j = j < N ? j : i;
+            }
             for (int c = 0; c < 4; ++c) {
                 if (b->fp64) {
                     double v;

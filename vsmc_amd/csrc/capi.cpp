@@ -189,7 +189,7 @@ struct vsmcx_filter {
         if (plain)
             launch_plain_move(st[0], inc, obs, p, d.seed, t, N, s);
         else
-            launch_cv_move(fp64, ctl, st[cur], st[1 - cur], rb.anc, lw, obs, p, d.seed, t, N,
+            launch_cv_move(fp64, ctl, st[cur], st[1 - cur], rb.anc, rb.sl, lw, obs, p, d.seed, t, N,
                            partials, s);
     }
     // everything after the move: weights, decision, resampling, monitor
@@ -209,7 +209,9 @@ struct vsmcx_filter {
             } else {
                 launch_cv_reduce(ctl, partials, tiles, hist, t, s);
             }
-            launch_resample_from_log_weights(ctl, lw, N, d.scheme, rb, hist, s, partials, fuse);
+            // encoded ancestors: the next move resolves zero slots through rb.sl
+            launch_resample_from_log_weights(ctl, lw, N, d.scheme, rb, hist, s, partials, fuse,
+                                             true);
             cur ^= 1;
         }
         ++t;
@@ -503,7 +505,8 @@ int vsmcx_filter_finish(vsmcx_filter *f)
     API_BEGIN
     REQUIRE(f, "NULL filter");
     if (f->t > 0 && !f->plain)
-        launch_cv_monitor(f->fp64, f->ctl, f->st[f->cur], f->rb.anc, f->lw, f->N, f->partials,
+        launch_cv_monitor(f->fp64, f->ctl, f->st[f->cur], f->rb.anc, f->rb.sl, f->lw, f->N,
+                          f->partials,
                           f->hist, f->ctx->stream);
     VSMCX_HIP(hipGetLastError());
     FilterCtl h;
@@ -618,10 +621,14 @@ int vsmcx_filter_save(vsmcx_filter *f, const char *path)
     auto st = to_host<unsigned char>(f->ctx, f->st[f->cur], N * f->state_bytes());
     auto lw = to_host<double>(f->ctx, f->lw, N);
     auto anc = to_host<uint32_t>(f->ctx, f->rb.anc, N);
+    auto sl = to_host<uint32_t>(f->ctx, f->rb.sl, N);
     auto obs = to_host<double>(f->ctx, f->obs, 2 * f->nobs);
     auto hist = to_host<StepRow>(f->ctx, f->hist, f->T);
     f->ctx->sync();
     check_dev_err(ctl);
+    // the file holds decoded ancestors (every slot's source)
+    if (ctl.prev_resampled)
+        for (uint64_t i = 0; i < N; ++i) anc[i] = anc_source(anc.data(), sl.data(), i);
     // written beside the target and renamed over it, so a failed write
     // leaves an earlier checkpoint at `path` intact
     const std::string tmp = std::string(path) + ".tmp";
@@ -717,7 +724,8 @@ int vsmcx_filter_state(vsmcx_filter *f, double *state, double *log_weight)
         f->ctx->sync();
     } else if (state) {
         void *tmp = f->ctx->alloc(N * f->state_bytes());
-        launch_cv_gather(f->fp64, f->ctl, f->st[f->cur], tmp, f->rb.anc, N, f->ctx->stream);
+        launch_cv_gather(f->fp64, f->ctl, f->st[f->cur], tmp, f->rb.anc, f->rb.sl, N,
+                         f->ctx->stream);
         if (f->fp64) {
             VSMCX_HIP(hipMemcpyAsync(state, tmp, 32 * N, hipMemcpyDeviceToHost, f->ctx->stream));
             f->ctx->sync();
@@ -877,7 +885,8 @@ int vsmcx_filter_debug_step(vsmcx_filter *f, double *weight, uint64_t *replicati
     vsmcx_ctx *c = f->ctx;
     hipStream_t s = c->stream;
     const uint64_t N = f->N;
-    launch_cv_move(f->fp64, f->ctl, f->st[f->cur], f->st[1 - f->cur], f->rb.anc, f->lw, f->obs,
+    launch_cv_move(f->fp64, f->ctl, f->st[f->cur], f->st[1 - f->cur], f->rb.anc, f->rb.sl, f->lw,
+                   f->obs,
                    f->p, f->d.seed, f->t, N, f->partials, s);
     launch_cv_reduce(f->ctl, f->partials, f->tiles, f->hist, f->t, s);
     FilterCtl h;

@@ -23,8 +23,9 @@ template <typename Real>
 __device__ __forceinline__ void cv_move_tile(
     const FilterCtl *ctl, const typename StateOf<Real>::type *__restrict__ st_in,
     typename StateOf<Real>::type *__restrict__ st_out, const uint32_t *__restrict__ anc,
-    double *__restrict__ lw, const double *__restrict__ obs, const CvParams &p, uint32_t k0,
-    uint32_t k1, uint64_t t, uint64_t N, uint64_t tile, double *__restrict__ part)
+    const uint32_t *__restrict__ sl, double *__restrict__ lw, const double *__restrict__ obs,
+    const CvParams &p, uint32_t k0, uint32_t k1, uint64_t t, uint64_t N, uint64_t tile,
+    double *__restrict__ part)
 {
     typedef typename StateOf<Real>::type S4;
     __shared__ double s_red[kBlock / kWave];
@@ -44,7 +45,7 @@ __device__ __forceinline__ void cv_move_tile(
 #pragma unroll
     for (int k = 0; k < kMoveItems; ++k) {
         const uint64_t i = base + (uint64_t)k * kBlock + threadIdx.x;
-        uint64_t j = (t > 0 && prev && i < N) ? (uint64_t)anc[i] : i;
+        uint64_t j = (t > 0 && prev && i < N) ? (uint64_t)anc_source(anc, sl, i) : i;
         src[k] = j < N ? j : (i < N ? i : 0);  // never read outside the state buffer
     }
     S4 s[kMoveItems];
@@ -296,8 +297,9 @@ __device__ __forceinline__ void cv_reduce_tiles(FilterCtl *ctl, const double *pa
 template <typename Real>
 __device__ __forceinline__ void cv_monitor_tile(const FilterCtl *ctl,
                                                 const typename StateOf<Real>::type *st,
-                                                const uint32_t *anc, const double *lw,
-                                                uint64_t N, uint64_t tile, double *part)
+                                                const uint32_t *anc, const uint32_t *sl,
+                                                const double *lw, uint64_t N, uint64_t tile,
+                                                double *part)
 {
     __shared__ Part5 s_p5[kBlock / kWave];
     const int prev = ctl->prev_resampled;
@@ -307,7 +309,7 @@ __device__ __forceinline__ void cv_monitor_tile(const FilterCtl *ctl,
     for (int k = 0; k < kMoveItems; ++k) {
         const uint64_t i = base + (uint64_t)k * kBlock + threadIdx.x;
         if (i >= N) continue;
-        const auto s = st[prev ? (uint64_t)anc[i] : i];
+        const auto s = st[prev ? (uint64_t)anc_source(anc, sl, i) : i];
         const double w = prev ? invN : exp(lw[i] - m) * c;
         v.mx += w * (double)s.x;
         v.my += w * (double)s.y;

@@ -284,7 +284,12 @@ struct FoldOut {
     // u_i = 1 - exp(-a_i) held as a running max within the tile instead, and
     // u_tmax[tile] the tile's maximum.
     double *u_tmax = nullptr;
+    // encoded ancestors: a zero slot's anc holds kAncZero | its zero rank, and
+    // readers take the source from the surplus list (anc_source); no zero list
+    // is written and no zero-fill pass runs
+    bool encode = false;
 };
+
 
 // Tiles of 2048 elements, 8 per thread.  Per-element data lives in LDS (one
 // padding double per 8 keeps the thread-consecutive reads conflict-free) and
@@ -911,21 +916,49 @@ __device__ void fold_tile_layout(FilterCtl *ctl, uint64_t t, uint64_t M, const u
         if (r >= 2) srank_local += r - 1;
         s_sp[fidx(e)] = (uint32_t)srank_local;
     }
-#pragma unroll
-    for (int k = 0; k < kFI; ++k) {
-        const int e = k * kBlock + tid;
-        const uint64_t i = base + e;
-        if (i < M) out.anc[i] = (uint32_t)i;
-    }
-    __syncthreads();
-    // the zero list: ranks [z_base, z_base + Z_tile), striped
     const uint64_t z_base = pre >> 32;
-    const uint32_t Z_tile = (uint32_t)(zs_tile >> 32);
-    for (uint32_t k = tid; k < Z_tile; k += kBlock) {
-        if (z_base + k < M)
-            out.zl[z_base + k] = (uint32_t)(base + s_z[k]);
-        else
-            atomicOr(&ctl->err, (int)kErrBounds);
+    if (out.encode) {
+        // every slot's ancestor word, staged for striped stores: i, or the
+        // zero marker with the slot's global zero rank (s_z is not used)
+        __syncthreads();
+        uint32_t zr = (uint32_t)(zs_excl >> 32);
+#pragma unroll
+        for (int k = 0; k < kFI; ++k) {
+            const int e = e0 + k;
+            const uint64_t i = base + e;
+            uint32_t a = (uint32_t)i;
+            if (rv[k] == 0) {
+                const uint64_t rank = z_base + zr++;
+                if (rank < M)
+                    a = kAncZero | (uint32_t)rank;
+                else  // never read through the surplus list out of bounds
+                    atomicOr(&ctl->err, (int)kErrBounds);
+            }
+            s_z[fidx(e)] = a;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < kFI; ++k) {
+            const int e = k * kBlock + tid;
+            const uint64_t i = base + e;
+            if (i < M) out.anc[i] = s_z[fidx(e)];
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < kFI; ++k) {
+            const int e = k * kBlock + tid;
+            const uint64_t i = base + e;
+            if (i < M) out.anc[i] = (uint32_t)i;
+        }
+        __syncthreads();
+        // the zero list: ranks [z_base, z_base + Z_tile), striped
+        const uint32_t Z_tile = (uint32_t)(zs_tile >> 32);
+        for (uint32_t k = tid; k < Z_tile; k += kBlock) {
+            if (z_base + k < M)
+                out.zl[z_base + k] = (uint32_t)(base + s_z[k]);
+            else
+                atomicOr(&ctl->err, (int)kErrBounds);
+        }
     }
     // expand surplus copies: positions [s_base, s_base + S_tile)
     const uint32_t S_tile = (uint32_t)(zs_tile & 0xffffffffull);

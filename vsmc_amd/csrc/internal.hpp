@@ -24,6 +24,21 @@ constexpr int kFoldChunk = 64;                  // tiles per chunk of the segmen
 // the move's tile partials and the fold's per-tile arrays share one tiling
 static_assert(kMoveTile == kFoldTile, "move and fold tiles coincide");
 
+// Zero-slot marker of encoded ancestors: a slot that the resample left empty
+// holds kAncZero | its zero rank, and its source is the surplus copy of that
+// rank (cfrp_trans's pairing, include/vsmc/resample/common.hpp:91-110).  N <
+// 2^31, so ranks fit below the marker.
+constexpr uint32_t kAncZero = 0x80000000u;
+
+// The source slot of slot i after a resample: anc[i], or, for an encoded zero
+// slot, the surplus copy of its rank in sl.  Plain (decoded) ancestors pass
+// through unchanged.
+__host__ __device__ inline uint32_t anc_source(const uint32_t *anc, const uint32_t *sl, uint64_t i)
+{
+    const uint32_t a = anc[i];
+    return (a & kAncZero) ? sl[a & ~kAncZero] : a;
+}
+
 // Resampling schemes, same order as vsmc::ResampleScheme
 // (include/vsmc/resample/common.hpp:124-131).
 enum Scheme : int {

@@ -56,11 +56,11 @@ __device__ __noinline__ void batch_final_monitor(const BatchFilter &F, uint32_t 
 #pragma unroll 1
     for (uint64_t tt = 0; tt < T; ++tt) {
         if (F.model == kModelLg)
-            lg_monitor_tile<Real>(ctl, st, lp.dim, P.anc + F.off, P.lw + F.off, N, tt,
+            lg_monitor_tile<Real>(ctl, st, lp.dim, P.anc + F.off, P.sl + F.off, P.lw + F.off, N, tt,
                                   part + 5 * tt);
         else
             cv_monitor_tile<Real>(ctl, (const typename StateOf<Real>::type *)st, P.anc + F.off,
-                                  P.lw + F.off, N, tt, part + 5 * tt);
+                                  P.sl + F.off, P.lw + F.off, N, tt, part + 5 * tt);
     }
     __syncthreads();
     if (threadIdx.x < kWave) {
@@ -116,7 +116,7 @@ __device__ __noinline__ void batch_move_cv(const BatchFilter &F, uint32_t f, con
     double *part = P.tpart + 5 * (F.off / kMoveTile);
 #pragma unroll 1
     for (uint64_t tt = 0; tt < T; ++tt)
-        cv_move_tile<Real>(ctl, cur ? sb : sa, cur ? sa : sb, P.anc + F.off, P.lw + F.off,
+        cv_move_tile<Real>(ctl, cur ? sb : sa, cur ? sa : sb, P.anc + F.off, P.sl + F.off, P.lw + F.off,
                            P.obs + F.obs_off, p, k0, k1, t, N, tt, part + 5 * tt);
     batch_move_end(ctl, P, f, cur, P.hist + F.hist_off, t, part, T, S);
 }
@@ -134,7 +134,7 @@ __device__ __noinline__ void batch_move_lg(const BatchFilter &F, uint32_t f, con
     double *part = P.tpart + 5 * (F.off / kMoveTile);
 #pragma unroll 1
     for (uint64_t tt = 0; tt < T; ++tt)
-        lg_move_tile<Real>(ctl, cur ? sb : sa, cur ? sa : sb, P.anc + F.off, P.lw + F.off,
+        lg_move_tile<Real>(ctl, cur ? sb : sa, cur ? sa : sb, P.anc + F.off, P.sl + F.off, P.lw + F.off,
                            P.obs + F.obs_off + (uint64_t)lp.dim * t, lp, k0, k1, t, N, tt,
                            part + 5 * tt);
     batch_move_end(ctl, P, f, cur, P.hist + F.hist_off, t, part, T, S);
@@ -257,20 +257,12 @@ __device__ __noinline__ void batch_fold_layout(const BatchFilter &F, FilterCtl *
     const int tid = threadIdx.x;
     const uint64_t N = F.n, T = (N + kFT - 1) / kFT;
     uint32_t *anc = P.anc + F.off;
-    const FoldOut fo{anc, P.zl + F.off, P.sl + F.off, nullptr, nullptr};
+    // encoded ancestors (zero slots resolved through the surplus list by the
+    // next move), as in the tick: no zero-fill loop
+    const FoldOut fo{anc, P.zl + F.off, P.sl + F.off, nullptr, nullptr, nullptr, true};
 #pragma unroll 1
     for (uint64_t tt = 0; tt < T; ++tt)
         fold_tile_layout(ctl, tt, N, P.rep32 + F.off, P.tzs[F.off / kFT + tt], fo, S.L);
-    __syncthreads();
-    const uint64_t z = S.zeros < N ? S.zeros : N;
-#pragma unroll 1
-    for (uint64_t k = tid; k < z; k += kBlock) {
-        const uint32_t to = fo.zl[k], from = fo.sl[k];
-        if (to < N && from < N)
-            anc[to] = from;
-        else
-            atomicOr(&ctl->err, (int)kErrBounds);
-    }
     __syncthreads();
     if (tid == 0)
         resample_epilogue(ctl, P.hist + F.hist_off, SchemeOf<Inv>::value == kSystematic, N);

@@ -21,10 +21,10 @@ template <typename Real>
 __global__ __launch_bounds__(kBlock) void k_cv_move(
     FilterCtl *ctl, const typename StateOf<Real>::type *__restrict__ st_in,
     typename StateOf<Real>::type *__restrict__ st_out, const uint32_t *__restrict__ anc,
-    double *__restrict__ lw, const double *__restrict__ obs, CvParams p, uint32_t k0,
-    uint32_t k1, uint64_t t, uint64_t N, double *__restrict__ partials)
+    const uint32_t *__restrict__ sl, double *__restrict__ lw, const double *__restrict__ obs,
+    CvParams p, uint32_t k0, uint32_t k1, uint64_t t, uint64_t N, double *__restrict__ partials)
 {
-    cv_move_tile<Real>(ctl, st_in, st_out, anc, lw, obs, p, k0, k1, t, N, blockIdx.x,
+    cv_move_tile<Real>(ctl, st_in, st_out, anc, sl, lw, obs, p, k0, k1, t, N, blockIdx.x,
                        partials + 5 * (uint64_t)blockIdx.x);
 }
 
@@ -64,10 +64,12 @@ __global__ __launch_bounds__(kScanBlock) void k_cv_reduce(FilterCtl *ctl, const 
 template <typename Real>
 __global__ __launch_bounds__(kBlock) void k_cv_monitor(FilterCtl *ctl,
                                                        const typename StateOf<Real>::type *st,
-                                                       const uint32_t *anc, const double *lw,
-                                                       uint64_t N, double *partials)
+                                                       const uint32_t *anc, const uint32_t *sl,
+                                                       const double *lw, uint64_t N,
+                                                       double *partials)
 {
-    cv_monitor_tile<Real>(ctl, st, anc, lw, N, blockIdx.x, partials + 5 * (uint64_t)blockIdx.x);
+    cv_monitor_tile<Real>(ctl, st, anc, sl, lw, N, blockIdx.x,
+                          partials + 5 * (uint64_t)blockIdx.x);
 }
 
 __global__ __launch_bounds__(kScanBlock) void k_cv_monitor_sum(FilterCtl *ctl,
@@ -94,28 +96,29 @@ __global__ void k_cv_weights(const FilterCtl *ctl, const double *lw, uint64_t N,
 
 template <typename S4>
 __global__ void k_cv_gather(const FilterCtl *ctl, const S4 *in, S4 *out, const uint32_t *anc,
-                            uint64_t N)
+                            const uint32_t *sl, uint64_t N)
 {
     const int prev = ctl->prev_resampled;
     for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < N;
          i += (uint64_t)gridDim.x * blockDim.x)
-        out[i] = in[prev ? (uint64_t)anc[i] : i];
+        out[i] = in[prev ? (uint64_t)anc_source(anc, sl, i) : i];
 }
 
 static unsigned tiles_of(uint64_t N) { return (unsigned)((N + kMoveTile - 1) / kMoveTile); }
 
 void launch_cv_move(bool fp64, FilterCtl *ctl, const void *st_in, void *st_out,
-                    const uint32_t *anc, double *lw, const double *obs, const CvParams &p,
-                    uint64_t seed, uint64_t t, uint64_t N, double *partials, hipStream_t st)
+                    const uint32_t *anc, const uint32_t *sl, double *lw, const double *obs,
+                    const CvParams &p, uint64_t seed, uint64_t t, uint64_t N, double *partials,
+                    hipStream_t st)
 {
     const uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32);
     if (fp64)
         hipLaunchKernelGGL(k_cv_move<double>, dim3(tiles_of(N)), dim3(kBlock), 0, st, ctl,
-                           (const S4d *)st_in, (S4d *)st_out, anc, lw, obs, p, k0, k1, t, N,
+                           (const S4d *)st_in, (S4d *)st_out, anc, sl, lw, obs, p, k0, k1, t, N,
                            partials);
     else
         hipLaunchKernelGGL(k_cv_move<float>, dim3(tiles_of(N)), dim3(kBlock), 0, st, ctl,
-                           (const S4f *)st_in, (S4f *)st_out, anc, lw, obs, p, k0, k1, t, N,
+                           (const S4f *)st_in, (S4f *)st_out, anc, sl, lw, obs, p, k0, k1, t, N,
                            partials);
 }
 
@@ -129,15 +132,15 @@ void launch_cv_reduce(FilterCtl *ctl, const double *partials, uint64_t ntiles, S
 }
 
 void launch_cv_monitor(bool fp64, FilterCtl *ctl, const void *stp, const uint32_t *anc,
-                       const double *lw, uint64_t N, double *partials, StepRow *hist,
-                       hipStream_t st)
+                       const uint32_t *sl, const double *lw, uint64_t N, double *partials,
+                       StepRow *hist, hipStream_t st)
 {
     if (fp64)
         hipLaunchKernelGGL(k_cv_monitor<double>, dim3(tiles_of(N)), dim3(kBlock), 0, st, ctl,
-                           (const S4d *)stp, anc, lw, N, partials);
+                           (const S4d *)stp, anc, sl, lw, N, partials);
     else
         hipLaunchKernelGGL(k_cv_monitor<float>, dim3(tiles_of(N)), dim3(kBlock), 0, st, ctl,
-                           (const S4f *)stp, anc, lw, N, partials);
+                           (const S4f *)stp, anc, sl, lw, N, partials);
     hipLaunchKernelGGL(k_cv_monitor_sum, dim3(1), dim3(kScanBlock), 0, st, ctl, partials,
                        (uint64_t)tiles_of(N), hist);
 }
@@ -148,14 +151,14 @@ void launch_cv_weights(FilterCtl *ctl, const double *lw, uint64_t N, double *w, 
 }
 
 void launch_cv_gather(bool fp64, FilterCtl *ctl, const void *st_in, void *st_out,
-                      const uint32_t *anc, uint64_t N, hipStream_t st)
+                      const uint32_t *anc, const uint32_t *sl, uint64_t N, hipStream_t st)
 {
     if (fp64)
         hipLaunchKernelGGL(k_cv_gather<S4d>, dim3(tiles_of(N) * 16), dim3(256), 0, st, ctl,
-                           (const S4d *)st_in, (S4d *)st_out, anc, N);
+                           (const S4d *)st_in, (S4d *)st_out, anc, sl, N);
     else
         hipLaunchKernelGGL(k_cv_gather<S4f>, dim3(tiles_of(N) * 16), dim3(256), 0, st, ctl,
-                           (const S4f *)st_in, (S4f *)st_out, anc, N);
+                           (const S4f *)st_in, (S4f *)st_out, anc, sl, N);
 }
 
 // ---- device-resident particle values (the vSMC value-type adaptor) ------------------

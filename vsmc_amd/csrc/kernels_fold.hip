@@ -526,9 +526,10 @@ static void launch_fold_from_starts(FilterCtl *ctl, XSrc xs, Inv inv, uint64_t M
 template <typename W>
 static void resample_pipeline(FilterCtl *ctl, W w, uint64_t M, int scheme, ResampleBuffers &rb,
                               StepRow *hist, hipStream_t st, const double *lse_part = nullptr,
-                              bool pre_started = false)
+                              bool pre_started = false, bool encode = false)
 {
     FoldOut out{rb.anc, rb.zl, rb.sl, rb.rep_out, rb.a_out};
+    out.encode = encode && rb.anc;
     const bool residual = scheme == kResidual || scheme == kResidualStratified ||
                           scheme == kResidualSystematic;
     FoldScratch &fs = rb.fs;
@@ -586,7 +587,7 @@ static void resample_pipeline(FilterCtl *ctl, W w, uint64_t M, int scheme, Resam
         else
             launch_fold(ctl, w, InvSystematic(), M, rb.fs, out, st, lse_part, pre_started, epi);
     }
-    if (rb.anc) {
+    if (rb.anc && !out.encode) {
         const unsigned zg = (unsigned)((M + kTile - 1) / kTile) * 4;
         hipLaunchKernelGGL(k_zero_fill, dim3(zg ? zg : 1), dim3(kBlock), 0, st, ctl, rb.zl, rb.sl,
                            rb.anc, M);
@@ -597,10 +598,10 @@ static void resample_pipeline(FilterCtl *ctl, W w, uint64_t M, int scheme, Resam
 
 void launch_resample_from_log_weights(FilterCtl *ctl, const double *lw, uint64_t M, int scheme,
                                       ResampleBuffers &rb, StepRow *hist, hipStream_t st,
-                                      const double *lse_part, bool pre_started)
+                                      const double *lse_part, bool pre_started, bool encode)
 {
     resample_pipeline(ctl, XWeight{lw, 0.0, 0.0}, M, scheme, rb, hist, st, lse_part,
-                      pre_started && lse_part);
+                      pre_started && lse_part, encode);
 }
 
 void launch_resample_from_weights(FilterCtl *ctl, const double *w, uint64_t M, int scheme,

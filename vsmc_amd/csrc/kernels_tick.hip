@@ -41,13 +41,13 @@ __global__ __launch_bounds__(kBlock) void k_tick_move(const TickItem *items,
     double *part = P.tpart + 5 * (F.off / kFT + it.tile);
     if (F.model == kModelLg) {
         Real *sa = (Real *)P.st0 + F.soff, *sb = (Real *)P.st1 + F.soff;
-        lg_move_tile<Real>(ctl, cur ? sb : sa, cur ? sa : sb, P.anc + F.off, P.lw + F.off,
+        lg_move_tile<Real>(ctl, cur ? sb : sa, cur ? sa : sb, P.anc + F.off, P.sl + F.off, P.lw + F.off,
                            P.obs + F.obs_off + (uint64_t)lg.dim * t, lg, k0, k1, t, F.n, it.tile,
                            part);
     } else {
         typedef typename StateOf<Real>::type S4;
         S4 *sa = (S4 *)((Real *)P.st0 + F.soff), *sb = (S4 *)((Real *)P.st1 + F.soff);
-        cv_move_tile<Real>(ctl, cur ? sb : sa, cur ? sa : sb, P.anc + F.off, P.lw + F.off,
+        cv_move_tile<Real>(ctl, cur ? sb : sa, cur ? sa : sb, P.anc + F.off, P.sl + F.off, P.lw + F.off,
                            P.obs + F.obs_off, cv, k0, k1, t, F.n, it.tile, part);
     }
 }
@@ -169,32 +169,10 @@ __global__ __launch_bounds__(kBlock) void k_tick_layout(const TickItem *items,
     if (!ctl->resample) return;
     __shared__ FoldLdsT<false> L;
     const BatchFilter F = filters[it.slot];
-    const FoldOut fo{P.anc + F.off, P.zl + F.off, P.sl + F.off, nullptr, nullptr};
+    // encoded ancestors: the move of the next step reads zero slots' sources
+    // from the surplus list, so there is no zero-fill pass
+    const FoldOut fo{P.anc + F.off, P.zl + F.off, P.sl + F.off, nullptr, nullptr, nullptr, true};
     fold_tile_layout(ctl, it.tile, F.n, P.rep32 + F.off, P.tzsx[F.off / kFT + it.tile], fo, L);
-}
-
-// zero slots: the tile's share [tile * kFT, (tile + 1) * kFT) of the pair lists
-__global__ __launch_bounds__(kBlock) void k_tick_zero_fill(const TickItem *items,
-                                                           const BatchFilter *filters,
-                                                           BatchPool P)
-{
-    const TickItem it = items[blockIdx.x];
-    FilterCtl *ctl = P.ctl + it.slot;
-    if (!ctl->resample) return;
-    const BatchFilter F = filters[it.slot];
-    const uint64_t N = F.n;
-    const uint64_t z = ctl->zeros < N ? ctl->zeros : N;
-    const uint64_t k0 = it.tile * (uint64_t)kFT;
-    const uint64_t k1 = k0 + kFT < z ? k0 + kFT : z;
-    const uint32_t *zl = P.zl + F.off, *sl = P.sl + F.off;
-    uint32_t *anc = P.anc + F.off;
-    for (uint64_t k = k0 + threadIdx.x; k < k1; k += kBlock) {
-        const uint32_t to = zl[k], from = sl[k];
-        if (to < N && from < N)
-            anc[to] = from;
-        else
-            atomicOr(&ctl->err, (int)kErrBounds);
-    }
 }
 
 __global__ void k_tick_epilogue(const uint32_t *jobs, uint32_t njobs, const BatchFilter *filters,
@@ -222,11 +200,11 @@ __global__ __launch_bounds__(kBlock) void k_tick_monitor(const TickItem *items,
     double *part = P.tpart + 5 * (F.off / kFT + it.tile);
     if (F.model == kModelLg) {
         const Real *st = (cur ? (const Real *)P.st1 : (const Real *)P.st0) + F.soff;
-        lg_monitor_tile<Real>(ctl, st, lg.dim, P.anc + F.off, P.lw + F.off, F.n, it.tile, part);
+        lg_monitor_tile<Real>(ctl, st, lg.dim, P.anc + F.off, P.sl + F.off, P.lw + F.off, F.n, it.tile, part);
     } else {
         typedef typename StateOf<Real>::type S4;
         const S4 *st = (const S4 *)((cur ? (const Real *)P.st1 : (const Real *)P.st0) + F.soff);
-        cv_monitor_tile<Real>(ctl, st, P.anc + F.off, P.lw + F.off, F.n, it.tile, part);
+        cv_monitor_tile<Real>(ctl, st, P.anc + F.off, P.sl + F.off, P.lw + F.off, F.n, it.tile, part);
     }
 }
 
@@ -307,7 +285,6 @@ void launch_tick(bool fp64, int scheme, const TickItem *items, uint32_t nitems,
         hipLaunchKernelGGL(k_tick_apply<InvSystematic>, gi, b, 0, st, items, filters, P);
     hipLaunchKernelGGL(k_tick_scan_cnt, gj, bs, 0, st, jobs, filters, P);
     hipLaunchKernelGGL(k_tick_layout, gi, b, 0, st, items, filters, P);
-    hipLaunchKernelGGL(k_tick_zero_fill, gi, b, 0, st, items, filters, P);
     hipLaunchKernelGGL(k_tick_epilogue, gsmall, bsmall, 0, st, jobs, njobs, filters, P, scheme);
     if (fp64)
         hipLaunchKernelGGL(k_tick_monitor<double>, gi, b, 0, st, items, filters, P, lg);

@@ -28,9 +28,12 @@ struct ResampleBuffers {
 // kernels_fold.hip
 // lse_part: the move's tile partials of lw ({max, sum exp(lw - max), ...},
 // 5 doubles per 2048-particle tile), or nullptr when lw has none
+// encode: the ancestors of zero slots are written encoded (kAncZero | zero
+// rank, resolved through rb.sl by anc_source) and no zero-fill pass runs
 void launch_resample_from_log_weights(FilterCtl *ctl, const double *lw, uint64_t M, int scheme,
                                       ResampleBuffers &rb, StepRow *hist, hipStream_t st,
-                                      const double *lse_part = nullptr, bool pre_started = false);
+                                      const double *lse_part = nullptr, bool pre_started = false,
+                                      bool encode = false);
 void launch_resample_from_weights(FilterCtl *ctl, const double *w, uint64_t M, int scheme,
                                   ResampleBuffers &rb, StepRow *hist, hipStream_t st);
 void launch_fold_values(FilterCtl *ctl, const double *x, uint64_t M, ResampleBuffers &rb,
@@ -48,9 +51,12 @@ void launch_threefry4x32_fill(const uint32_t key[4], const uint32_t ctr0[4], uin
 
 // kernels_cv.hip
 // fused move (+ init when t == 0), partials per tile: {max, sum e, sum e^2, mean x, mean y}
+// anc / sl: the ancestors of the last resample and its surplus list (for
+// encoded zero slots, see anc_source)
 void launch_cv_move(bool fp64, FilterCtl *ctl, const void *st_in, void *st_out,
-                    const uint32_t *anc, double *lw, const double *obs, const CvParams &p,
-                    uint64_t seed, uint64_t t, uint64_t N, double *partials, hipStream_t st);
+                    const uint32_t *anc, const uint32_t *sl, double *lw, const double *obs,
+                    const CvParams &p, uint64_t seed, uint64_t t, uint64_t N, double *partials,
+                    hipStream_t st);
 // starts (optional, with prologue_scheme a non-residual scheme): on a resample
 // the reduce also runs the scheme's prologue and writes the fold's tile starts
 // of the filter's weights into starts->ts; the resample then runs with
@@ -59,11 +65,11 @@ void launch_cv_reduce(FilterCtl *ctl, const double *partials, uint64_t ntiles, S
                       uint64_t t, hipStream_t st, int prologue_scheme = -1,
                       const FoldScratch *starts = nullptr);
 void launch_cv_monitor(bool fp64, FilterCtl *ctl, const void *state, const uint32_t *anc,
-                       const double *lw, uint64_t N, double *partials, StepRow *hist,
-                       hipStream_t st);
+                       const uint32_t *sl, const double *lw, uint64_t N, double *partials,
+                       StepRow *hist, hipStream_t st);
 void launch_cv_weights(FilterCtl *ctl, const double *lw, uint64_t N, double *w, hipStream_t st);
 void launch_cv_gather(bool fp64, FilterCtl *ctl, const void *st_in, void *st_out,
-                      const uint32_t *anc, uint64_t N, hipStream_t st);
+                      const uint32_t *anc, const uint32_t *sl, uint64_t N, hipStream_t st);
 
 // kernels_cv.hip: device-resident particle values (the vSMC value-type adaptor)
 void launch_state_cv_move(bool fp64, void *st, double *inc, const CvParams &p, uint64_t seed,

@@ -15,6 +15,7 @@ namespace vsmcx {
 template <typename Real>
 __device__ __forceinline__ void lg_move_tile(const FilterCtl *ctl, const Real *__restrict__ in,
                                              Real *__restrict__ out, const uint32_t *__restrict__ anc,
+                                             const uint32_t *__restrict__ sl,
                                              double *__restrict__ lw, const double *__restrict__ y,
                                              const LgParams &lp, uint32_t k0, uint32_t k1,
                                              uint64_t t, uint64_t N, uint64_t tile, double *part)
@@ -39,7 +40,7 @@ __device__ __forceinline__ void lg_move_tile(const FilterCtl *ctl, const Real *_
         Real x[kLgMaxDim], xn[kLgMaxDim];
         double lwo = 0;
         if (t > 0) {
-            uint64_t src = prev ? (uint64_t)anc[i] : i;
+            uint64_t src = prev ? (uint64_t)anc_source(anc, sl, i) : i;
             src = src < N ? src : i;
 #pragma unroll
             for (int d = 0; d < kLgMaxDim; ++d) x[d] = d < D ? in[src * D + d] : (Real)0;
@@ -106,8 +107,9 @@ __device__ __forceinline__ void lg_move_tile(const FilterCtl *ctl, const Real *_
 // final monitor of a linear-Gaussian job: mean of the first two coordinates
 template <typename Real>
 __device__ __forceinline__ void lg_monitor_tile(const FilterCtl *ctl, const Real *st, int D,
-                                                const uint32_t *anc, const double *lw, uint64_t N,
-                                                uint64_t tile, double *part)
+                                                const uint32_t *anc, const uint32_t *sl,
+                                                const double *lw, uint64_t N, uint64_t tile,
+                                                double *part)
 {
     __shared__ Part5 s_p5[kBlock / kWave];
     const int prev = ctl->prev_resampled;
@@ -117,7 +119,7 @@ __device__ __forceinline__ void lg_monitor_tile(const FilterCtl *ctl, const Real
     for (int k = 0; k < kMoveItems; ++k) {
         const uint64_t i = base + (uint64_t)k * kBlock + threadIdx.x;
         if (i >= N) continue;
-        uint64_t src = prev ? (uint64_t)anc[i] : i;
+        uint64_t src = prev ? (uint64_t)anc_source(anc, sl, i) : i;
         src = src < N ? src : i;
         const double w = prev ? invN : exp(lw[i] - m) * c;
         v.mx += w * (double)st[src * D];
