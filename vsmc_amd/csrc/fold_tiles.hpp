@@ -125,6 +125,11 @@ struct InvSystematic {
         if (nn == 0) return 0;
         const uint64_t n0 = cnt_floor((c - u0) * cnt_d(nn), nn);
         const int64_t a = (int64_t)n0;
+        // nearly always v(n0) <= c < v(n0 + 1), which fixes F = n0 + 1 by
+        // monotonicity alone (two places instead of five)
+        const int at = le(a, c);
+        if (at && !le(a + 1, c)) return n0 + 1;
+        if (!at && le(a - 1, c)) return n0;
         if (le(a - 2, c) && !le(a + 2, c))
             return (uint64_t)(a - 1 + le(a - 1, c) + le(a, c) + le(a + 1, c));
         uint64_t n = n0;
@@ -865,6 +870,16 @@ __device__ void fold_tile_apply(FilterCtl *ctl, const XSrc &xs, const Inv &inv, 
 // anc[i] = i where rep_i > 0; the k-th zero slot (in index order) goes to the
 // zero list at rank k, the k-th surplus copy (index order, rep_i - 1 copies
 // of i) to the surplus list at rank k; k_zero_fill pairs them.
+//
+// The surplus list of a tile is written in one of two ways.  When it fits in
+// LDS and no thread holds more than kDirectCopies copies, each thread writes
+// its own copies at their tile-local ranks into LDS (a short loop per thread)
+// and the list goes out in striped stores.  Otherwise (a few elements with
+// many copies) each copy finds its element by a binary search over the
+// running copy counts, which balances any distribution over the threads.
+// Both write the same list.
+constexpr uint32_t kDirectCopies = 32;
+
 template <typename Lds>
 __device__ void fold_tile_layout(FilterCtl *ctl, uint64_t t, uint64_t M, const uint32_t *rep32,
                                  uint64_t zsx, const FoldOut &out, Lds &L)
@@ -880,12 +895,14 @@ __device__ void fold_tile_layout(FilterCtl *ctl, uint64_t t, uint64_t M, const u
     __syncthreads();  // the LDS of a previous tile is no longer read
     const int tid = threadIdx.x;
     const uint64_t base = t * (uint64_t)kFT;
+    const uint32_t b32 = (uint32_t)base;  // element indices stay below 2^32
     const int e0 = tid * kFI;
+    const int nv = M - base < (uint64_t)kFT ? (int)(M - base) : kFT;  // elements of the tile
+    const uint32_t *const rt = rep32 + base;
 #pragma unroll
     for (int k = 0; k < kFI; ++k) {
         const int e = k * kBlock + tid;
-        const uint64_t i = base + e;
-        s_rep[fidx(e)] = i < M ? rep32[i] : 1u;
+        s_rep[fidx(e)] = e < nv ? rt[e] : 1u;
     }
     __syncthreads();
     uint32_t rv[kFI];
@@ -903,20 +920,24 @@ __device__ void fold_tile_layout(FilterCtl *ctl, uint64_t t, uint64_t M, const u
     // (its barriers also end every read of s_rep before s_z is written)
     const uint64_t zs_excl =
         block_excl_scan<kBlock>(zs_thr, AddOp(), (uint64_t)0, s_u64, &zs_tile);
-    const uint64_t pre = zsx;
-    uint32_t zl_local = (uint32_t)(zs_excl >> 32);   // zero slots before this thread (tile)
-    uint64_t srank_local = zs_excl & 0xffffffffull;  // surplus before this thread (tile)
-    const uint64_t s_base = pre & 0xffffffffull;
+    const uint64_t z_base = zsx >> 32, s_base = zsx & 0xffffffffull;
+    const uint32_t S_tile = (uint32_t)(zs_tile & 0xffffffffull);
+    const bool direct = __syncthreads_and((uint32_t)(zs_thr & 0xffffffffull) <= kDirectCopies) &&
+                        S_tile <= (uint32_t)kFPad;
+    if (!out.encode) {
+        uint32_t zl_local = (uint32_t)(zs_excl >> 32);  // zero slots before this thread (tile)
 #pragma unroll
-    for (int k = 0; k < kFI; ++k) {
-        const int e = e0 + k;
-        const uint64_t i = base + e;
-        const uint32_t r = rv[k];
-        if (i < M && r == 0) s_z[zl_local++] = (uint32_t)e;
-        if (r >= 2) srank_local += r - 1;
-        s_sp[fidx(e)] = (uint32_t)srank_local;
+        for (int k = 0; k < kFI; ++k)
+            if (e0 + k < nv && rv[k] == 0) s_z[zl_local++] = (uint32_t)(e0 + k);
     }
-    const uint64_t z_base = pre >> 32;
+    if (!direct) {
+        uint32_t srank_local = (uint32_t)(zs_excl & 0xffffffffull);  // surplus before this thread
+#pragma unroll
+        for (int k = 0; k < kFI; ++k) {
+            if (rv[k] >= 2) srank_local += rv[k] - 1;
+            s_sp[fidx(e0 + k)] = srank_local;
+        }
+    }
     if (out.encode) {
         // every slot's ancestor word, staged for striped stores: i, or the
         // zero marker with the slot's global zero rank (s_z is not used)
@@ -925,8 +946,7 @@ __device__ void fold_tile_layout(FilterCtl *ctl, uint64_t t, uint64_t M, const u
 #pragma unroll
         for (int k = 0; k < kFI; ++k) {
             const int e = e0 + k;
-            const uint64_t i = base + e;
-            uint32_t a = (uint32_t)i;
+            uint32_t a = b32 + (uint32_t)e;
             if (rv[k] == 0) {
                 const uint64_t rank = z_base + zr++;
                 if (rank < M)
@@ -937,31 +957,48 @@ __device__ void fold_tile_layout(FilterCtl *ctl, uint64_t t, uint64_t M, const u
             s_z[fidx(e)] = a;
         }
         __syncthreads();
+        uint32_t *const at = out.anc + base;
 #pragma unroll
         for (int k = 0; k < kFI; ++k) {
             const int e = k * kBlock + tid;
-            const uint64_t i = base + e;
-            if (i < M) out.anc[i] = s_z[fidx(e)];
+            if (e < nv) at[e] = s_z[fidx(e)];
         }
     } else {
+        uint32_t *const at = out.anc + base;
 #pragma unroll
         for (int k = 0; k < kFI; ++k) {
             const int e = k * kBlock + tid;
-            const uint64_t i = base + e;
-            if (i < M) out.anc[i] = (uint32_t)i;
+            if (e < nv) at[e] = b32 + (uint32_t)e;
         }
         __syncthreads();
         // the zero list: ranks [z_base, z_base + Z_tile), striped
         const uint32_t Z_tile = (uint32_t)(zs_tile >> 32);
         for (uint32_t k = tid; k < Z_tile; k += kBlock) {
             if (z_base + k < M)
-                out.zl[z_base + k] = (uint32_t)(base + s_z[k]);
+                out.zl[z_base + k] = b32 + s_z[k];
             else
                 atomicOr(&ctl->err, (int)kErrBounds);
         }
     }
     // expand surplus copies: positions [s_base, s_base + S_tile)
-    const uint32_t S_tile = (uint32_t)(zs_tile & 0xffffffffull);
+    if (direct) {
+        // s_sp is not read in this mode: the thread's copies at their ranks
+        uint32_t pos = (uint32_t)(zs_excl & 0xffffffffull);
+#pragma unroll
+        for (int k = 0; k < kFI; ++k) {
+            const uint32_t src = b32 + (uint32_t)(e0 + k);
+#pragma unroll 1
+            for (uint32_t c = 1; c < rv[k]; ++c) s_sp[pos++] = src;
+        }
+        __syncthreads();
+        for (uint32_t k = tid; k < S_tile; k += kBlock) {
+            if (s_base + k < M)
+                out.sl[s_base + k] = s_sp[k];
+            else
+                atomicOr(&ctl->err, (int)kErrBounds);
+        }
+        return;
+    }
     for (uint32_t k = tid; k < S_tile; k += kBlock) {
         int lo = 0, hi = kFT - 1;  // first e with s_sp[e] > k
         while (lo < hi) {
@@ -972,7 +1009,7 @@ __device__ void fold_tile_layout(FilterCtl *ctl, uint64_t t, uint64_t M, const u
                 lo = mid + 1;
         }
         if (s_base + k < M)
-            out.sl[s_base + k] = (uint32_t)(base + lo);
+            out.sl[s_base + k] = b32 + (uint32_t)lo;
         else
             atomicOr(&ctl->err, (int)kErrBounds);
     }
