@@ -300,29 +300,34 @@ VSMCX_HD Seg elem_seg(double x, double sb, double sa, double eps_rel, double eps
 // its interval are monotone), and without ties the run adds the same number of
 // grid units whatever the parity.  fast_b receives b, or kNoBinade for the
 // element-by-element path.
+//
+// The two paths are also callable apart (run_seg_fast, run_seg_elems), so that
+// device code can run the fast path over values held in registers (its loops
+// unroll) and the rare element-by-element path over values in LDS.
 template <typename X>
-VSMCX_HD Seg run_seg(X x, int n, double s, double er, double ea, int *fast_b)
+VSMCX_HD bool run_seg_fast(X x, int n, double s, double er, double ea, int *fast_b, Seg *r)
 {
     double se = s;
     for (int k = 0; k < n; ++k) se += x(k);
     const int b0 = certain_binade(s, er, ea);
-    if (b0 < 1024 && b0 == certain_binade(se, er, ea)) {
-        uint64_t d = 0;
-        int tie = 0;
-        for (int k = 0; k < n; ++k) {
-            int kind;
-            d += grid_units(x(k), b0, &kind) + (kind == 1);
-            tie |= kind == 2;
-        }
-        if (!tie) {
-            *fast_b = b0;
-            Seg r;
-            r.tr = tr_units(d, b0);
-            r.cnt = 0;
-            return r;
-        }
+    if (!(b0 < 1024 && b0 == certain_binade(se, er, ea))) return false;
+    uint64_t d = 0;
+    int tie = 0;
+    for (int k = 0; k < n; ++k) {
+        int kind;
+        d += grid_units(x(k), b0, &kind) + (kind == 1);
+        tie |= kind == 2;
     }
-    *fast_b = kNoBinade;
+    if (tie) return false;
+    *fast_b = b0;
+    r->tr = tr_units(d, b0);
+    r->cnt = 0;
+    return true;
+}
+
+template <typename X>
+VSMCX_HD Seg run_seg_elems(X x, int n, double s, double er, double ea)
+{
     Seg g = seg_identity();
     // the element-by-element path is rare: kept rolled, so the common path
     // stays small in the instruction cache
@@ -338,33 +343,65 @@ VSMCX_HD Seg run_seg(X x, int n, double s, double er, double ea, int *fast_b)
     return g;
 }
 
+template <typename X>
+VSMCX_HD Seg run_seg(X x, int n, double s, double er, double ea, int *fast_b)
+{
+    Seg r;
+    if (run_seg_fast(x, n, s, er, ea, fast_b, &r)) return r;
+    *fast_b = kNoBinade;
+    return run_seg_elems(x, n, s, er, ea);
+}
+
 // Exact fold values of the same run: pre is the summary of everything before
 // it, vb(c) the exact value right after special c (fold order).  out(k, a_k)
 // receives each value.  Returns false if a transducer does not apply or an
 // element taken as negligible would change the value (a broken bound).
+//
+// In parts: the exact value before the run (run_start), then the fast path's
+// integer walk in one binade (run_values_units) or the element-by-element path
+// (run_values_elems).
+struct RunStart {
+    uint64_t vcnt;  // specials before the run
+    double vbase;   // exact value right after the last of them (0 if none)
+    double a_prev;  // exact value before the run's first element
+    bool ok;
+};
+
+template <typename VB>
+VSMCX_HD RunStart run_start(const Seg &pre, VB vb)
+{
+    RunStart r;
+    r.ok = true;
+    r.vcnt = pre.cnt;
+    r.vbase = r.vcnt ? vb(r.vcnt - 1) : 0.0;
+    if (!tr_apply(pre.tr, r.vbase, &r.a_prev)) {
+        r.ok = false;
+        r.a_prev = r.vbase;
+    }
+    return r;
+}
+
+template <typename X, typename OUT>
+VSMCX_HD bool run_values_units(X x, int n, int fast_b, double a_prev, OUT out)
+{
+    bool ok = binade_of(a_prev) == fast_b;
+    uint64_t A = units_of(a_prev);
+    for (int k = 0; k < n; ++k) {
+        int kind;
+        A += grid_units(x(k), fast_b, &kind) + (kind == 1);
+        if (A >= kTwo53) ok = false;
+        out(k, value_of(fast_b, A & (kTwo53 - 1)));
+    }
+    return ok;
+}
+
 template <typename X, typename VB, typename OUT>
-VSMCX_HD bool run_values(X x, int n, double s, double er, double ea, int fast_b, Seg pre, VB vb,
-                         OUT out)
+VSMCX_HD bool run_values_elems(X x, int n, double s, double er, double ea, Seg pre,
+                               const RunStart &st, VB vb, OUT out)
 {
     bool ok = true;
-    uint64_t vcnt = pre.cnt;
-    double vbase = vcnt ? vb(vcnt - 1) : 0.0;
-    double a_prev;
-    if (!tr_apply(pre.tr, vbase, &a_prev)) {
-        ok = false;
-        a_prev = vbase;
-    }
-    if (fast_b != kNoBinade) {
-        if (binade_of(a_prev) != fast_b) ok = false;
-        uint64_t A = units_of(a_prev);
-        for (int k = 0; k < n; ++k) {
-            int kind;
-            A += grid_units(x(k), fast_b, &kind) + (kind == 1);
-            if (A >= kTwo53) ok = false;
-            out(k, value_of(fast_b, A & (kTwo53 - 1)));
-        }
-        return ok;
-    }
+    uint64_t vcnt = st.vcnt;
+    double vbase = st.vbase, a_prev = st.a_prev;
 #if defined(__HIP_DEVICE_COMPILE__)
 #pragma unroll 1
 #endif
@@ -388,6 +425,16 @@ VSMCX_HD bool run_values(X x, int n, double s, double er, double ea, int fast_b,
         a_prev = a;
     }
     return ok;
+}
+
+template <typename X, typename VB, typename OUT>
+VSMCX_HD bool run_values(X x, int n, double s, double er, double ea, int fast_b, Seg pre, VB vb,
+                         OUT out)
+{
+    const RunStart st = run_start(pre, vb);
+    const bool ok = fast_b != kNoBinade ? run_values_units(x, n, fast_b, st.a_prev, out)
+                                        : run_values_elems(x, n, s, er, ea, pre, st, vb, out);
+    return st.ok && ok;
 }
 
 // One special element in fold order: the run transducer since the previous

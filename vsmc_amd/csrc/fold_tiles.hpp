@@ -116,17 +116,16 @@ struct InvSystematic {
         return v((uint64_t)n) <= c;
     }
     // F(c) is floor((c - u0) nn) + 1 up to the roundings of that estimate
-    // and of v (a few units of 2^-53 on values <= 1, with N < 2^31).  So F
-    // is counted without branches over the three places from the estimate
-    // - 1 once the two places outside them confirm the bracket; anything
-    // else takes the search.
+    // and of v (a few units of 2^-53 on values <= 1, with N < 2^31).  Nearly
+    // always v(n0) <= c < v(n0 + 1) at the estimate n0, which fixes F =
+    // n0 + 1 by monotonicity alone (two places); else F is counted over the
+    // three places from n0 - 1 once the two places outside them confirm the
+    // bracket; anything else takes the search.
     __device__ uint64_t F(double c) const
     {
         if (nn == 0) return 0;
         const uint64_t n0 = cnt_floor((c - u0) * cnt_d(nn), nn);
         const int64_t a = (int64_t)n0;
-        // nearly always v(n0) <= c < v(n0 + 1), which fixes F = n0 + 1 by
-        // monotonicity alone (two places instead of five)
         const int at = le(a, c);
         if (at && !le(a + 1, c)) return n0 + 1;
         if (!at && le(a - 1, c)) return n0;
@@ -403,15 +402,41 @@ __device__ __forceinline__ void load_tile(const XSrc &xs, uint64_t base, uint64_
 }
 
 // The segment pass and the apply pass derive every element's approximate
-// prefix from the tile's scanned start with the same code (thread_sum, the
-// block scan, run_seg), on the same data, so they classify every element
+// prefix from the tile's scanned start with the same code (the thread's run
+// read into registers once, thread_sum, the block scan, the run's
+// classification), on the same data, so they classify every element
 // identically.
-__device__ __forceinline__ double thread_sum(const double *s_x, int e0)
+__device__ __forceinline__ void thread_run(const double *s_x, int e0, double xv[kFI])
+{
+#pragma unroll
+    for (int k = 0; k < kFI; ++k) xv[k] = s_x[fidx(e0 + k)];
+}
+__device__ __forceinline__ double thread_sum(const double xv[kFI])
+{
+    double s = 0;
+#pragma unroll
+    for (int k = 0; k < kFI; ++k) s += xv[k];
+    return s;
+}
+// the same sum read from LDS one value at a time (fewer registers live)
+__device__ __forceinline__ double thread_sum_lds(const double *s_x, int e0)
 {
     double s = 0;
 #pragma unroll 1
     for (int k = 0; k < kFI; ++k) s += s_x[fidx(e0 + k)];
     return s;
+}
+
+// The run's segment summary: the fast path over the registers, the rare
+// element-by-element path over the LDS copy (a loop that does not unroll
+// would put a register array in scratch memory).
+__device__ __forceinline__ Seg thread_run_seg(const double xv[kFI], const double *s_x, int e0,
+                                              double s_thr, double er, double ea, int *fast_b)
+{
+    Seg g;
+    if (run_seg_fast([&](int k) { return xv[k]; }, kFI, s_thr, er, ea, fast_b, &g)) return g;
+    *fast_b = kNoBinade;
+    return run_seg_elems([&](int k) { return s_x[fidx(e0 + k)]; }, kFI, s_thr, er, ea);
 }
 
 // Exclusive scan of n per-tile values by one workgroup of kScanBlock threads,
@@ -524,13 +549,14 @@ __device__ void fold_tile_seg(const XSrc &xs, uint64_t base, uint64_t M, double 
     const int e0 = tid * kFI;
     load_tile(xs, base, M, s_x, wx_out);
     __syncthreads();
+    double xv[kFI];
+    thread_run(s_x, e0, xv);
     const double s_thr =
-        ts + block_excl_scan<kBlock>(thread_sum(s_x, e0), AddOp(), 0.0, L.red, (double *)nullptr);
+        ts + block_excl_scan<kBlock>(thread_sum(xv), AddOp(), 0.0, L.red, (double *)nullptr);
     double er, ea;
     fold_eps(M, &er, &ea);
-    const auto xat = [&](int k) { return s_x[fidx(e0 + k)]; };
     int fast_b;
-    const Seg g = run_seg(xat, kFI, s_thr, er, ea, &fast_b);
+    const Seg g = thread_run_seg(xv, s_x, e0, s_thr, er, ea, &fast_b);
     const Seg thr_pre = thread_seg_prefix(g, fast_b, L);
     // tile total from the last thread (a struct total through a pointer would
     // keep the struct in scratch memory)
@@ -652,8 +678,16 @@ __device__ void fold_tile_apply(FilterCtl *ctl, const XSrc &xs, const Inv &inv, 
     __syncthreads();
 
     // ---- exact fold values ----------------------------------------------------
+    // The closed-form inversions take the thread's run in registers; the
+    // sorted-array inversion and the plain fold read it from LDS a value at a
+    // time, which keeps their register count (and waves per SIMD) at what
+    // their latency-bound searches need.  Same values and sums either way.
+    constexpr bool kReg = Inv::kInv && !(Inv::kWindow && !Inv::kWindowTmp);
+    double xv[kFI];
+    if constexpr (kReg) thread_run(s_x, e0, xv);
+    const double tsum = kReg ? thread_sum(xv) : thread_sum_lds(s_x, e0);
     const double s_thr =
-        ts + block_excl_scan<kBlock>(thread_sum(s_x, e0), AddOp(), 0.0, s_red, (double *)nullptr);
+        ts + block_excl_scan<kBlock>(tsum, AddOp(), 0.0, s_red, (double *)nullptr);
     double er, ea;
     fold_eps(M, &er, &ea);
     const auto xat = [&](int k) { return s_x[fidx(e0 + k)]; };
@@ -663,12 +697,29 @@ __device__ void fold_tile_apply(FilterCtl *ctl, const XSrc &xs, const Inv &inv, 
         thr_pre = thr_in[tid];
         fast_b = fb_in[tid];
     } else {
-        const Seg g = run_seg(xat, kFI, s_thr, er, ea, &fast_b);
+        Seg g;
+        if constexpr (kReg)
+            g = thread_run_seg(xv, s_x, e0, s_thr, er, ea, &fast_b);
+        else
+            g = run_seg(xat, kFI, s_thr, er, ea, &fast_b);
         thr_pre = thread_seg_prefix(g, fast_b, L);
     }
-    int bad = !run_values(
-        xat, kFI, s_thr, er, ea, fast_b, seg_op(tpre, thr_pre),
-        [=](uint64_t c) { return gval[c]; }, [&](int k, double a) { s_a[fidx(e0 + k)] = a; });
+    const auto vb = [=](uint64_t c) { return gval[c]; };
+    const auto put = [&](int k, double a) { s_a[fidx(e0 + k)] = a; };
+    int bad;
+    if constexpr (kReg) {
+        const Seg run_pre = seg_op(tpre, thr_pre);
+        const RunStart rs = run_start(run_pre, vb);
+        bad = !rs.ok;
+        if (fast_b != kNoBinade) {
+            if (!run_values_units([&](int k) { return xv[k]; }, kFI, fast_b, rs.a_prev, put))
+                bad = 1;
+        } else if (!run_values_elems(xat, kFI, s_thr, er, ea, run_pre, rs, vb, put)) {
+            bad = 1;
+        }
+    } else {
+        bad = !run_values(xat, kFI, s_thr, er, ea, fast_b, seg_op(tpre, thr_pre), vb, put);
+    }
     double v_start = 0.0;  // exact a_{base-1}
     if (t > 0) {
         const double vb = tpre.cnt ? gval[tpre.cnt - 1] : 0.0;
