@@ -30,11 +30,13 @@ struct CtxAlloc {
     T *n(uint64_t count) { return c->alloc_n<T>(count); }
 };
 
-// Fold / resample scratch for up to M particles (and nuni uniforms).
-void alloc_resample(vsmcx_ctx *c, ResampleBuffers &rb, uint64_t M, uint64_t nuni, bool lists)
+// Fold / resample scratch for up to M particles (and nuni uniforms); own_ufs:
+// the uniforms' fold gets scratch of its own (it runs on a side stream).
+void alloc_resample(vsmcx_ctx *c, ResampleBuffers &rb, uint64_t M, uint64_t nuni, bool lists,
+                    bool own_ufs)
 {
     CtxAlloc a{c};
-    layout_resample(a, rb, M, nuni, lists);
+    layout_resample(a, rb, M, nuni, lists, own_ufs);
 }
 
 void free_resample(vsmcx_ctx *c, ResampleBuffers &rb)
@@ -44,7 +46,26 @@ void free_resample(vsmcx_ctx *c, ResampleBuffers &rb)
                   rb.fs.gval, rb.fs.rep32, rb.fs.zs, rb.fs.zsx, rb.fs.wx, rb.fs.thr,
                   rb.fs.thr_fb, rb.fs.cagg, rb.fs.cpre, rb.fs.tcnt};
     for (void *p : ps) c->release(p);
+    if (rb.ufs.agg != rb.fs.agg) {  // the uniforms' own fold scratch
+        void *us[] = {rb.ufs.agg, rb.ufs.pre, rb.ufs.cagg, rb.ufs.cpre, rb.ufs.tcnt,
+                      rb.ufs.spec, rb.ufs.gval, rb.ufs.thr, rb.ufs.thr_fb};
+        for (void *p : us) c->release(p);
+    }
     rb = ResampleBuffers();
+}
+
+// the side stream's objects (nothing is left running on it once the filter's
+// stream is done: every resample waits for the side stream's join)
+void free_side(vsmcx_ctx *c, SideFold &sd)
+{
+    if (sd.s) {
+        (void)hipStreamSynchronize(sd.s);
+        (void)hipStreamDestroy(sd.s);
+    }
+    if (sd.fork) (void)hipEventDestroy(sd.fork);
+    if (sd.join) (void)hipEventDestroy(sd.join);
+    c->release(sd.uctl);
+    sd = SideFold();
 }
 
 }  // namespace vsmcx
@@ -182,10 +203,13 @@ struct vsmcx_filter {
 
     size_t state_bytes() const { return fp64 ? 32 : 16; }
 
-    // the move kernel of step t (the dominant kernel of the pipeline)
+    // the move kernel of step t (the dominant kernel of the pipeline); a
+    // multinomial filter forks its step's sorted uniforms to the side stream
+    // first, so they overlap the move and the weight fold's first passes
     void launch_move()
     {
         hipStream_t s = ctx->stream;
+        if (rb.side.s && d.scheme == kMultinomial) launch_side_uniforms(ctl, rb, s);
         if (plain)
             launch_plain_move(st[0], inc, obs, p, d.seed, t, N, s);
         else
@@ -438,12 +462,23 @@ int vsmcx_filter_create(vsmcx_ctx *ctx, const vsmcx_filter_desc *desc, vsmcx_fil
         f->obs = ctx->alloc_n<double>(2 * f->T);
         f->ctl = ctx->alloc_n<FilterCtl>(1);
         f->hist = ctx->alloc_zero<StepRow>(f->T);
-        alloc_resample(ctx, f->rb, f->N, f->N, true);
+        // the fused pipelines run the sorted uniforms' fold on a side stream
+        const bool side = !f->plain && (desc->scheme == VSMCX_MULTINOMIAL ||
+                                        desc->scheme == VSMCX_RESIDUAL);
+        alloc_resample(ctx, f->rb, f->N, f->N, true, side);
+        if (side) {
+            SideFold &sd = f->rb.side;
+            sd.uctl = ctx->alloc_zero<FilterCtl>(1);
+            VSMCX_HIP(hipStreamCreateWithFlags(&sd.s, hipStreamNonBlocking));
+            VSMCX_HIP(hipEventCreateWithFlags(&sd.fork, hipEventDisableTiming));
+            VSMCX_HIP(hipEventCreateWithFlags(&sd.join, hipEventDisableTiming));
+        }
     } catch (...) {
         for (void *p : {f->st[0], f->st[1], (void *)f->lw, (void *)f->partials, (void *)f->obs,
                         (void *)f->ctl, (void *)f->hist, (void *)f->w, (void *)f->inc,
                         (void *)f->ppart, (void *)f->scal})
             ctx->release(p);
+        free_side(ctx, f->rb.side);
         free_resample(ctx, f->rb);
         throw;
     }
@@ -465,6 +500,7 @@ int vsmcx_filter_destroy(vsmcx_filter *f)
     if (!f) return VSMCX_OK;
     vsmcx_ctx *c = f->ctx;
     (void)hipStreamSynchronize(c->stream);
+    free_side(c, f->rb.side);
     for (void *p : {f->st[0], f->st[1], (void *)f->lw, (void *)f->partials, (void *)f->obs,
                     (void *)f->ctl, (void *)f->hist, (void *)f->w, (void *)f->inc,
                     (void *)f->ppart, (void *)f->scal})

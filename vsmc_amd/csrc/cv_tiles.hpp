@@ -41,12 +41,12 @@ __device__ __forceinline__ void cv_move_tile(
     // Loads first, for all of the thread's particles (ancestor indices, then
     // the gathered states and old log-weights), so their latencies overlap;
     // then the arithmetic and the stores.
-    uint64_t src[kMoveItems];
+    uint32_t src[kMoveItems];  // particle indices stay below 2^32
 #pragma unroll
     for (int k = 0; k < kMoveItems; ++k) {
         const uint64_t i = base + (uint64_t)k * kBlock + threadIdx.x;
-        uint64_t j = (t > 0 && prev && i < N) ? (uint64_t)anc_source(anc, sl, i) : i;
-        src[k] = j < N ? j : (i < N ? i : 0);  // never read outside the state buffer
+        const uint32_t j = (t > 0 && prev && i < N) ? anc_source(anc, sl, i) : (uint32_t)i;
+        src[k] = j < N ? j : (i < N ? (uint32_t)i : 0u);  // never read outside the state buffer
     }
     S4 s[kMoveItems];
     double lwo[kMoveItems];

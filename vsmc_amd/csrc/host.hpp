@@ -111,8 +111,10 @@ struct vsmcx_ctx {
 namespace vsmcx {
 struct ResampleBuffers;
 // Fold / resample scratch for up to M particles and nuni uniforms, with the
-// ancestor and cfrp lists when lists is set (capi.cpp)
-void alloc_resample(vsmcx_ctx *c, ResampleBuffers &rb, uint64_t M, uint64_t nuni, bool lists);
+// ancestor and cfrp lists when lists is set, and fold scratch of the uniforms'
+// own when own_ufs is set (capi.cpp)
+void alloc_resample(vsmcx_ctx *c, ResampleBuffers &rb, uint64_t M, uint64_t nuni, bool lists,
+                    bool own_ufs = false);
 void free_resample(vsmcx_ctx *c, ResampleBuffers &rb);
 }  // namespace vsmcx
 

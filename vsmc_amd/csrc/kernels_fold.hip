@@ -219,6 +219,7 @@ struct Epi {
     StepRow *hist;
     int on;
     int systematic_like;
+    const int *uerr = nullptr;  // error bits of the uniforms' fold on the side stream
 };
 
 __global__ __launch_bounds__(kScanBlock) void k_fold_scan_cnt(FilterCtl *ctl, FoldScratch fs,
@@ -229,6 +230,7 @@ __global__ __launch_bounds__(kScanBlock) void k_fold_scan_cnt(FilterCtl *ctl, Fo
     const uint64_t tot = wg_scan<uint64_t>(fs.zs, fs.zsx, fs.tiles, AddOp(), (uint64_t)0, s_red);
     if (threadIdx.x == 0) {
         ctl->zeros = tot >> 32;
+        if (epi.uerr && *epi.uerr) atomicOr(&ctl->err, *epi.uerr);
         if (epi.on) resample_epilogue(ctl, epi.hist, epi.systematic_like, M);
     }
 }
@@ -457,10 +459,13 @@ __global__ __launch_bounds__(1024) void k_scan_tiles_max(FilterCtl *ctl, const d
 // when the ancestor layout is.
 // lse_part (XWeight sources only): the move's tile partials of these
 // log-weights, from which the tile starts follow without a tile-sum pass.
+// join (optional): an event the inversion waits for (the sorted uniforms from
+// the side stream)
 template <typename XSrc, typename Inv>
 static void launch_fold(FilterCtl *ctl, XSrc xs, Inv inv, uint64_t M, FoldScratch &fs,
                         const FoldOut &out, hipStream_t st, const double *lse_part = nullptr,
-                        bool pre_started = false, Epi epi = Epi{nullptr, 0, 0})
+                        bool pre_started = false, Epi epi = Epi{nullptr, 0, 0},
+                        hipEvent_t join = nullptr)
 {
     fs.tiles = (M + kFT - 1) / kFT;
     const dim3 grid((unsigned)fs.tiles);
@@ -474,6 +479,7 @@ static void launch_fold(FilterCtl *ctl, XSrc xs, Inv inv, uint64_t M, FoldScratc
         launch_scan_seg(ctl, fs, st);
         if (!Inv::kInv && !out.a_out) return;
         const XCached<XSrc> xc{fs.wx, xs};
+        if (join) VSMCX_HIP(hipStreamWaitEvent(st, join, 0));
         hipLaunchKernelGGL((k_fold_apply<XCached<XSrc>, Inv>), grid, dim3(kBlock), 0, st, ctl, xc,
                            inv, M, fs, out);
         if (!Inv::kInv || !out.anc) return;
@@ -490,6 +496,7 @@ static void launch_fold(FilterCtl *ctl, XSrc xs, Inv inv, uint64_t M, FoldScratc
                            (double *)nullptr);
         launch_scan_seg(ctl, fs, st);
         if (!Inv::kInv && !out.a_out) return;
+        if (join) VSMCX_HIP(hipStreamWaitEvent(st, join, 0));
         hipLaunchKernelGGL((k_fold_apply<XCached<XSrc>, Inv>), grid, dim3(kBlock), 0, st, ctl, xc,
                            inv, M, fs, out);
     } else {
@@ -497,6 +504,7 @@ static void launch_fold(FilterCtl *ctl, XSrc xs, Inv inv, uint64_t M, FoldScratc
                            (double *)nullptr);
         launch_scan_seg(ctl, fs, st);
         if (!Inv::kInv && !out.a_out) return;
+        if (join) VSMCX_HIP(hipStreamWaitEvent(st, join, 0));
         hipLaunchKernelGGL((k_fold_apply<XSrc, Inv>), grid, dim3(kBlock), 0, st, ctl, xs, inv, M,
                            fs, out);
     }
@@ -509,18 +517,68 @@ static void launch_fold(FilterCtl *ctl, XSrc xs, Inv inv, uint64_t M, FoldScratc
 template <typename XSrc, typename Inv>
 static void launch_fold_from_starts(FilterCtl *ctl, XSrc xs, Inv inv, uint64_t M,
                                     FoldScratch &fs, const FoldOut &out, hipStream_t st,
-                                    Epi epi = Epi{nullptr, 0, 0})
+                                    Epi epi = Epi{nullptr, 0, 0}, hipEvent_t join = nullptr)
 {
     const dim3 grid((unsigned)fs.tiles);
     hipLaunchKernelGGL((k_fold_seg<XSrc>), grid, dim3(kBlock), 0, st, ctl, xs, M, fs,
                        (double *)nullptr);
     launch_scan_seg(ctl, fs, st);
     if (!Inv::kInv && !out.a_out) return;
+    if (join) VSMCX_HIP(hipStreamWaitEvent(st, join, 0));
     hipLaunchKernelGGL((k_fold_apply<XSrc, Inv>), grid, dim3(kBlock), 0, st, ctl, xs, inv, M, fs,
                        out);
     if (!Inv::kInv || !out.anc) return;
     hipLaunchKernelGGL(k_fold_scan_cnt, dim3(1), dim3(kScanBlock), 0, st, ctl, fs, epi, M);
     hipLaunchKernelGGL(k_fold_layout, grid, dim3(kBlock), 0, st, ctl, fs, out, M);
+}
+
+// The sorted uniforms (nn of them, nn <= rb.nuni): the exact fold of the x_n
+// over the capacity (zeros from nn on leave the fold unchanged), in place in
+// uscan, with its own tile sums and starts (the residual asum fold's tile sums
+// in fs.tsum are still to be read by the scaled fold).  c: the control block
+// the passes read (resample, nn, engine) and write (fold total, errors).
+static void launch_uniform_fold(FilterCtl *c, ResampleBuffers &rb, hipStream_t st)
+{
+    FoldScratch &ufs = rb.ufs;
+    const uint64_t nu = rb.nuni;
+    const dim3 fgrid((unsigned)ufs.tiles);
+    hipLaunchKernelGGL(k_sorted_terms, fgrid, dim3(kBlock), 0, st, c, rb.uscan, ufs.tsum, nu);
+    hipLaunchKernelGGL(k_fold_scan_sum, dim3(1), dim3(kScanBlock), 0, st, c, ufs);
+    // the apply pass leaves u = 1 - exp(-a) as a running max per fold tile
+    launch_fold_from_starts(c, XArray{rb.uscan}, InvNone(), nu, ufs,
+                            FoldOut{nullptr, nullptr, nullptr, nullptr, rb.uscan, rb.u_tmax}, st);
+    hipLaunchKernelGGL(k_scan_tiles_max, dim3(1), dim3(1024), 0, st, c, rb.u_tmax, rb.u_tmaxex,
+                       (uint64_t)ufs.tiles);
+}
+
+// The side stream's control block for one resample: the engine state and nn
+// (multinomial: n; residual: n - R), resampling on (multinomial computes its
+// uniforms before the step's decision; unused when it does not resample).
+__global__ void k_side_begin(FilterCtl *uctl, const FilterCtl *ctl, int residual)
+{
+    if (threadIdx.x != 0) return;
+    uctl->eng = ctl->eng;
+    uctl->n = ctl->n;
+    uctl->nn = residual ? ctl->n - ctl->r_int : ctl->n;
+    uctl->resample = residual ? ctl->resample : 1;
+    uctl->err = 0;
+}
+
+static void fork_uniforms(FilterCtl *ctl, ResampleBuffers &rb, hipStream_t st, int residual)
+{
+    SideFold &sd = rb.side;
+    VSMCX_HIP(hipEventRecord(sd.fork, st));
+    VSMCX_HIP(hipStreamWaitEvent(sd.s, sd.fork, 0));
+    hipLaunchKernelGGL(k_side_begin, dim3(1), dim3(64), 0, sd.s, sd.uctl, ctl, residual);
+    launch_uniform_fold(sd.uctl, rb, sd.s);
+    VSMCX_HIP(hipEventRecord(sd.join, sd.s));
+}
+
+void launch_side_uniforms(FilterCtl *ctl, ResampleBuffers &rb, hipStream_t st)
+{
+    if (!rb.side.s) return;
+    fork_uniforms(ctl, rb, st, 0);
+    rb.side.pending = true;
 }
 
 template <typename W>
@@ -532,6 +590,12 @@ static void resample_pipeline(FilterCtl *ctl, W w, uint64_t M, int scheme, Resam
     out.encode = encode && rb.anc;
     const bool residual = scheme == kResidual || scheme == kResidualStratified ||
                           scheme == kResidualSystematic;
+    const bool sorted = scheme == kMultinomial || scheme == kResidual;
+    // the uniforms on the side stream: multinomial's were forked at the start
+    // of the step (launch_side_uniforms), residual's fork once R is known
+    SideFold &sd = rb.side;
+    const bool side = sorted && sd.s && (scheme == kResidual || sd.pending);
+    sd.pending = false;
     FoldScratch &fs = rb.fs;
     fs.tiles = (M + kFT - 1) / kFT;
     if (residual) {
@@ -539,6 +603,7 @@ static void resample_pipeline(FilterCtl *ctl, W w, uint64_t M, int scheme, Resam
         hipLaunchKernelGGL((k_resid_tsum<W>), dim3((unsigned)fs.tiles), dim3(kBlock), 0, st, ctl,
                            w, M, fs);
         hipLaunchKernelGGL(k_fold_scan_sum_resid, dim3(1), dim3(kScanBlock), 0, st, ctl, fs);
+        if (side) fork_uniforms(ctl, rb, st, 1);
         const XCached<XResid<W>> xr{fs.wx, XResid<W>{w, 0.0}};
         launch_fold_from_starts(ctl, xr, InvNone(), M, fs,
                                 FoldOut{nullptr, nullptr, nullptr, nullptr, nullptr}, st);
@@ -548,24 +613,13 @@ static void resample_pipeline(FilterCtl *ctl, W w, uint64_t M, int scheme, Resam
     }
     // with the ancestor lists the count scan carries the engine bookkeeping
     const int sys_like = (scheme == kSystematic || scheme == kResidualSystematic) ? 1 : 0;
-    const Epi epi{hist, rb.anc ? 1 : 0, sys_like};
-    const bool sorted = scheme == kMultinomial || scheme == kResidual;
-    if (sorted) {
-        // uniforms: nn <= rb.nuni of them.  The exact fold of the x_n over the
-        // capacity (zeros from nn on leave the fold unchanged), in place in
-        // uscan, with its own tile sums and starts (the residual asum fold's
-        // tile sums in fs.tsum are still to be read by the scaled fold)
-        FoldScratch &ufs = rb.ufs;
-        const uint64_t nu = rb.nuni;
-        const dim3 fgrid((unsigned)ufs.tiles);
-        hipLaunchKernelGGL(k_sorted_terms, fgrid, dim3(kBlock), 0, st, ctl, rb.uscan, ufs.tsum, nu);
-        hipLaunchKernelGGL(k_fold_scan_sum, dim3(1), dim3(kScanBlock), 0, st, ctl, ufs);
-        // the apply pass leaves u = 1 - exp(-a) as a running max per fold tile
-        launch_fold_from_starts(ctl, XArray{rb.uscan}, InvNone(), nu, ufs,
-                                FoldOut{nullptr, nullptr, nullptr, nullptr, rb.uscan, rb.u_tmax},
-                                st);
-        hipLaunchKernelGGL(k_scan_tiles_max, dim3(1), dim3(1024), 0, st, ctl, rb.u_tmax,
-                           rb.u_tmaxex, (uint64_t)ufs.tiles);
+    Epi epi{hist, rb.anc ? 1 : 0, sys_like};
+    hipEvent_t join = nullptr;
+    if (side) {
+        join = sd.join;
+        epi.uerr = &sd.uctl->err;
+    } else if (sorted) {
+        launch_uniform_fold(ctl, rb, st);
     }
     InvArray ia{rb.uscan, rb.u_tmaxex, 0};
     if (residual) {
@@ -574,14 +628,14 @@ static void resample_pipeline(FilterCtl *ctl, W w, uint64_t M, int scheme, Resam
         hipLaunchKernelGGL(k_fold_scan_sum_scaled, dim3(1), dim3(kScanBlock), 0, st, ctl, fs);
         const XResidStored xs{fs.wx, fs.rep32, 0.0};
         if (scheme == kResidual)
-            launch_fold_from_starts(ctl, xs, ia, M, fs, out, st, epi);
+            launch_fold_from_starts(ctl, xs, ia, M, fs, out, st, epi, join);
         else if (scheme == kResidualStratified)
             launch_fold_from_starts(ctl, xs, InvStratified(), M, fs, out, st, epi);
         else
             launch_fold_from_starts(ctl, xs, InvSystematic(), M, fs, out, st, epi);
     } else {
         if (scheme == kMultinomial)
-            launch_fold(ctl, w, ia, M, rb.fs, out, st, lse_part, pre_started, epi);
+            launch_fold(ctl, w, ia, M, rb.fs, out, st, lse_part, pre_started, epi, join);
         else if (scheme == kStratified)
             launch_fold(ctl, w, InvStratified(), M, rb.fs, out, st, lse_part, pre_started, epi);
         else

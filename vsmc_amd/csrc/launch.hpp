@@ -8,6 +8,20 @@
 namespace vsmcx {
 
 // Buffers of one resampling pipeline sized for M particles.
+// The sorted uniforms' fold on a second stream of the filter, concurrent with
+// the filter's own passes: it reads only the engine state and nn, which are
+// fixed before the step's weights are (multinomial: from the start of the
+// step; residual: once R is known).  It has its own control block (the fold
+// passes write fold totals and error bits there) and its own fold scratch;
+// the fork and join events order it against the filter's stream, and the
+// weight fold's inversion waits for the join.  Owned by the filter.
+struct SideFold {
+    hipStream_t s = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr;
+    FilterCtl *uctl = nullptr;
+    bool pending = false;  // the uniforms of the coming resample are already launched
+};
+
 struct ResampleBuffers {
     uint32_t *anc = nullptr;     // [M] ancestors (read by the next move)
     uint32_t *zl = nullptr;      // [M] zero-slot list
@@ -19,9 +33,11 @@ struct ResampleBuffers {
     double *u_tmaxex = nullptr;  // [nuni / kFoldTile] exclusive running max of tiles
     uint64_t nuni = 0;           // capacity of the uniform arrays
     FoldScratch fs{};
-    // the sorted uniforms' own fold: tile sums and starts of its own, every
-    // other array shared with fs (the two folds run one after the other)
+    // the sorted uniforms' own fold: tile sums and starts of its own; the
+    // other arrays shared with fs when the two folds run one after the other,
+    // its own when it runs on side.s
     FoldScratch ufs{};
+    SideFold side{};
 };
 
 
@@ -38,6 +54,11 @@ void launch_resample_from_weights(FilterCtl *ctl, const double *w, uint64_t M, i
                                   ResampleBuffers &rb, StepRow *hist, hipStream_t st);
 void launch_fold_values(FilterCtl *ctl, const double *x, uint64_t M, ResampleBuffers &rb,
                         hipStream_t st);
+// Multinomial with rb.side.s: the sorted uniforms of the step's resample on
+// the side stream, forked from st now (before the move; the engine state is
+// final once the previous step's work on st is done).  The resample then only
+// waits for them.
+void launch_side_uniforms(FilterCtl *ctl, ResampleBuffers &rb, hipStream_t st);
 // out[i] = in[i] for i < n (uint32 -> uint64)
 void launch_widen_u32(const uint32_t *in, uint64_t *out, uint64_t n, hipStream_t st);
 

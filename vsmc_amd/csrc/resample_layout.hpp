@@ -18,9 +18,12 @@ constexpr uint64_t layout_round(uint64_t b)
 }
 
 // Fold / resample buffers for up to M particles and nuni sorted uniforms, with
-// the ancestor and cfrp lists when `lists` is set.
+// the ancestor and cfrp lists when `lists` is set, and with fold scratch of the
+// uniforms' own when `own_ufs` is set (their fold then runs concurrently with
+// the weight fold).
 template <class A>
-constexpr void layout_resample(A &a, ResampleBuffers &rb, uint64_t M, uint64_t nuni, bool lists)
+constexpr void layout_resample(A &a, ResampleBuffers &rb, uint64_t M, uint64_t nuni, bool lists,
+                               bool own_ufs = false)
 {
     // the fold scratch serves the weight fold (M elements) and the sorted
     // uniforms' fold (nuni elements)
@@ -60,6 +63,20 @@ constexpr void layout_resample(A &a, ResampleBuffers &rb, uint64_t M, uint64_t n
     u.ts = a.template n<double>(utiles + 1);
     u.wx = nullptr;
     u.tiles = utiles;
+    if (own_ufs) {
+        // every array the uniforms' fold writes (its inversion-free apply
+        // writes no counts, lists or replication)
+        const uint64_t uch = (utiles + kFoldChunk - 1) / kFoldChunk;
+        u.agg = a.template n<Seg>(utiles);
+        u.pre = a.template n<Seg>(utiles);
+        u.cagg = a.template n<Seg>(uch);
+        u.cpre = a.template n<Seg>(uch);
+        u.tcnt = a.template n<uint32_t>(utiles);
+        u.spec = a.template n<SpecRec>(utiles * kFoldTile);
+        u.gval = a.template n<double>(nuni + 1);
+        u.thr = a.template n<Seg>(utiles * kBlock);
+        u.thr_fb = a.template n<int32_t>(utiles * kBlock);
+    }
 }
 
 // Counts the bytes a layout takes (every buffer rounded up to kLayoutAlign).
