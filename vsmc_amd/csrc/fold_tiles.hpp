@@ -923,13 +923,11 @@ __device__ void fold_tile_apply(FilterCtl *ctl, const XSrc &xs, const Inv &inv, 
 // of i) to the surplus list at rank k; k_zero_fill pairs them.
 //
 // The surplus list of a tile is written in one of two ways.  When it fits in
-// LDS and no thread holds more than kDirectCopies copies, each thread writes
-// its own copies at their tile-local ranks into LDS (a short loop per thread)
-// and the list goes out in striped stores.  Otherwise (a few elements with
-// many copies) each copy finds its element by a binary search over the
-// running copy counts, which balances any distribution over the threads.
-// Both write the same list.
-constexpr uint32_t kDirectCopies = 32;
+// LDS, every element with copies marks the tile-local rank of its first copy
+// with its index, and an inclusive max-scan over the ranks carries each index
+// over its copies (indices grow with rank); the list then goes out in striped
+// stores.  Otherwise (more copies than LDS holds) each copy finds its element
+// by a binary search over the running copy counts.  Both write the same list.
 
 template <typename Lds>
 __device__ void fold_tile_layout(FilterCtl *ctl, uint64_t t, uint64_t M, const uint32_t *rep32,
@@ -973,8 +971,7 @@ __device__ void fold_tile_layout(FilterCtl *ctl, uint64_t t, uint64_t M, const u
         block_excl_scan<kBlock>(zs_thr, AddOp(), (uint64_t)0, s_u64, &zs_tile);
     const uint64_t z_base = zsx >> 32, s_base = zsx & 0xffffffffull;
     const uint32_t S_tile = (uint32_t)(zs_tile & 0xffffffffull);
-    const bool direct = __syncthreads_and((uint32_t)(zs_thr & 0xffffffffull) <= kDirectCopies) &&
-                        S_tile <= (uint32_t)kFPad;
+    const bool direct = S_tile <= (uint32_t)kFPad;
     if (!out.encode) {
         uint32_t zl_local = (uint32_t)(zs_excl >> 32);  // zero slots before this thread (tile)
 #pragma unroll
@@ -1033,18 +1030,38 @@ __device__ void fold_tile_layout(FilterCtl *ctl, uint64_t t, uint64_t M, const u
     }
     // expand surplus copies: positions [s_base, s_base + S_tile)
     if (direct) {
-        // s_sp is not read in this mode: the thread's copies at their ranks
+        // s_sp is not otherwise used in this mode: marks, then the list
+        for (uint32_t k = tid; k < S_tile; k += kBlock) s_sp[k] = 0;
+        __syncthreads();
         uint32_t pos = (uint32_t)(zs_excl & 0xffffffffull);
 #pragma unroll
         for (int k = 0; k < kFI; ++k) {
-            const uint32_t src = b32 + (uint32_t)(e0 + k);
-#pragma unroll 1
-            for (uint32_t c = 1; c < rv[k]; ++c) s_sp[pos++] = src;
+            if (rv[k] >= 2) {
+                s_sp[pos] = (uint32_t)(e0 + k) + 1;  // 0: no mark
+                pos += rv[k] - 1;
+            }
         }
+        __syncthreads();
+        // inclusive max-scan of the marks: each thread a run of per entries
+        constexpr int kPer = (kFPad + kBlock - 1) / kBlock;
+        const uint32_t k0 = (uint32_t)tid * kPer;
+        uint32_t mv[kPer];
+        uint32_t run = 0;
+#pragma unroll
+        for (int j = 0; j < kPer; ++j) {
+            const uint32_t v = k0 + j < S_tile ? s_sp[k0 + j] : 0u;
+            run = v > run ? v : run;
+            mv[j] = run;
+        }
+        const uint32_t carry = block_excl_scan<kBlock>(
+            run, MaxOp(), 0u, reinterpret_cast<uint32_t *>(s_u64), (uint32_t *)nullptr);
+#pragma unroll
+        for (int j = 0; j < kPer; ++j)
+            if (k0 + j < S_tile) s_sp[k0 + j] = mv[j] > carry ? mv[j] : carry;
         __syncthreads();
         for (uint32_t k = tid; k < S_tile; k += kBlock) {
             if (s_base + k < M)
-                out.sl[s_base + k] = s_sp[k];
+                out.sl[s_base + k] = b32 + s_sp[k] - 1;
             else
                 atomicOr(&ctl->err, (int)kErrBounds);
         }
