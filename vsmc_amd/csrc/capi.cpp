@@ -48,7 +48,7 @@ void free_resample(vsmcx_ctx *c, ResampleBuffers &rb)
     for (void *p : ps) c->release(p);
     if (rb.ufs.agg != rb.fs.agg) {  // the uniforms' own fold scratch
         void *us[] = {rb.ufs.agg, rb.ufs.pre, rb.ufs.cagg, rb.ufs.cpre, rb.ufs.tcnt,
-                      rb.ufs.spec, rb.ufs.gval, rb.ufs.thr, rb.ufs.thr_fb};
+                      rb.ufs.spec, rb.ufs.gval, rb.ufs.thr, rb.ufs.thr_fb, rb.uguess};
         for (void *p : us) c->release(p);
     }
     rb = ResampleBuffers();
@@ -64,6 +64,7 @@ void free_side(vsmcx_ctx *c, SideFold &sd)
     }
     if (sd.fork) (void)hipEventDestroy(sd.fork);
     if (sd.join) (void)hipEventDestroy(sd.join);
+    if (sd.starts) (void)hipEventDestroy(sd.starts);
     c->release(sd.uctl);
     sd = SideFold();
 }
@@ -472,6 +473,7 @@ int vsmcx_filter_create(vsmcx_ctx *ctx, const vsmcx_filter_desc *desc, vsmcx_fil
             VSMCX_HIP(hipStreamCreateWithFlags(&sd.s, hipStreamNonBlocking));
             VSMCX_HIP(hipEventCreateWithFlags(&sd.fork, hipEventDisableTiming));
             VSMCX_HIP(hipEventCreateWithFlags(&sd.join, hipEventDisableTiming));
+            VSMCX_HIP(hipEventCreateWithFlags(&sd.starts, hipEventDisableTiming));
         }
     } catch (...) {
         for (void *p : {f->st[0], f->st[1], (void *)f->lw, (void *)f->partials, (void *)f->obs,

@@ -9,11 +9,26 @@
 // one workgroup per filter.  Both run the same code on the same data, so a
 // filter's ancestors do not depend on which path resamples it.
 #pragma once
+#include <type_traits>
+
 #include "device_util.hpp"
 #include "fold_core.hpp"
 #include "internal.hpp"
 
 namespace vsmcx {
+
+// Tiles of 2048 elements, 8 per thread.  Per-element data lives in LDS (one
+// padding double per 8 keeps the thread-consecutive reads conflict-free) and
+// the per-element loops are not unrolled, which keeps every instantiation
+// small enough for the instruction cache.
+constexpr int kFI = kFoldItems;
+constexpr int kFT = kFoldTile;
+constexpr int kFPad = kFT + kFT / kFI;
+// Uniforms per window of the windowed inversions: the whole LDS array but one
+// entry (stratified staging holds len + 1 draws), so a tile whose uniforms
+// exceed its 2048 elements by up to 12% still takes one window.
+constexpr int kWin = kFPad - 1;
+__device__ __forceinline__ int fidx(int e) { return e + e / kFI; }
 
 // ------------------------------------------------------------------------
 // x sources: the sequence being folded
@@ -257,11 +272,34 @@ struct InvArray {
     const double *uscan;
     const double *tmax;  // exclusive running max of earlier tiles
     uint64_t nn;
+    // optional: per fold tile t a guess of F(start of the tile), from the
+    // tile's approximate start (k_side_windows); only a starting place, the
+    // answer is verified where it is used
+    const uint32_t *guess = nullptr;
     __device__ void bind(const FilterCtl *c) { nn = c->nn; }
     __device__ double v(uint64_t n) const
     {
         const double a = uscan[n], b = tmax[n / kFoldTile];
         return a < b ? b : a;
+    }
+    // F(c_lo) and F(c_hi) (first n with v_n > c, nn if none) from guesses:
+    // lanes 0-31 take c_lo, lanes 32-63 c_hi, lane j of a half tests n =
+    // g - 1 + j.  v is non-decreasing, so when v_{g-1} <= c (or g = 0) the
+    // first n with v_n > c among the 32 places is F(c), whatever the guess.
+    // Returns false (for the whole wave) when a half cannot conclude that way.
+    __device__ bool wave_F2_guess(double c_lo, double c_hi, uint64_t g_lo, uint64_t g_hi,
+                                  uint64_t *f_lo, uint64_t *f_hi) const
+    {
+        const int lane = threadIdx.x & 63, j = lane & 31;
+        const double c = lane < 32 ? c_lo : c_hi;
+        const int64_t n = (int64_t)(lane < 32 ? g_lo : g_hi) - 1 + j;
+        const bool gt = n < 0 ? false : ((uint64_t)n >= nn ? true : v((uint64_t)n) > c);
+        const uint64_t mask = __ballot(gt);
+        const uint32_t hl = (uint32_t)mask, hh = (uint32_t)(mask >> 32);
+        if (!hl || (hl & 1u) || !hh || (hh & 1u)) return false;
+        *f_lo = g_lo - 1 + (uint64_t)(__ffs(hl) - 1);
+        *f_hi = g_hi - 1 + (uint64_t)(__ffs(hh) - 1);
+        return true;
     }
     __device__ void fill_window(double *s_v, double *, uint64_t w0, int len) const
     {
@@ -295,18 +333,6 @@ struct FoldOut {
 };
 
 
-// Tiles of 2048 elements, 8 per thread.  Per-element data lives in LDS (one
-// padding double per 8 keeps the thread-consecutive reads conflict-free) and
-// the per-element loops are not unrolled, which keeps every instantiation
-// small enough for the instruction cache.
-constexpr int kFI = kFoldItems;
-constexpr int kFT = kFoldTile;
-constexpr int kFPad = kFT + kFT / kFI;
-// Uniforms per window of the windowed inversions: the whole LDS array but one
-// entry (stratified staging holds len + 1 draws), so a tile whose uniforms
-// exceed its 2048 elements by up to 12% still takes one window.
-constexpr int kWin = kFPad - 1;
-__device__ __forceinline__ int fidx(int e) { return e + e / kFI; }
 
 constexpr int kScanBlock = 1024;  // single-workgroup tile scans
 constexpr int kScanItems = 8;     // tiles per thread per round
@@ -792,6 +818,20 @@ __device__ void fold_tile_apply(FilterCtl *ctl, const XSrc &xs, const Inv &inv, 
                             lo = t == 0 ? 0 : flo;
                             hi = need_hi ? (fhi < lo ? lo : fhi) : lo;
                             done = true;
+                        }
+                    }
+                    if constexpr (std::is_same<Inv, InvArray>::value) {
+                        // guesses of the bounds: the starts of this tile and
+                        // of the next (the last tile has none for its end)
+                        const uint64_t T = (M + kFT - 1) / kFT;
+                        if (inv.guess && t + 1 < T) {
+                            uint64_t flo, fhi;
+                            if (inv.wave_F2_guess(v_start, c_hi, inv.guess[t], inv.guess[t + 1],
+                                                  &flo, &fhi)) {
+                                lo = t == 0 ? 0 : flo;
+                                hi = need_hi ? (fhi < lo ? lo : fhi) : lo;
+                                done = true;
+                            }
                         }
                     }
                     if (!done) {

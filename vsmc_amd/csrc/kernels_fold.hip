@@ -571,7 +571,54 @@ static void fork_uniforms(FilterCtl *ctl, ResampleBuffers &rb, hipStream_t st, i
     VSMCX_HIP(hipStreamWaitEvent(sd.s, sd.fork, 0));
     hipLaunchKernelGGL(k_side_begin, dim3(1), dim3(64), 0, sd.s, sd.uctl, ctl, residual);
     launch_uniform_fold(sd.uctl, rb, sd.s);
+}
+
+// Guesses of the inversion's uniform bounds for the weight fold's tiles: per
+// tile t, F(lower end of the error interval around its approximate start
+// ts[t]), by the 32-ary search of wave_first_greater2, two tiles per wave.
+// The inversion verifies each guess (InvArray::wave_F2_guess), so these only
+// spare it the search.
+__global__ __launch_bounds__(kBlock) void k_side_windows(const FilterCtl *uctl, const double *ts,
+                                                         uint64_t T, uint64_t M, InvArray inv,
+                                                         uint32_t *guess)
+{
+    if (!uctl->resample) return;
+    inv.bind(uctl);
+    const uint64_t w = (blockIdx.x * (uint64_t)kBlock + threadIdx.x) / kWave;
+    const uint64_t t0 = 2 * w, t1 = 2 * w + 1;
+    if (t0 >= T) return;  // uniform over the wave
+    double er, ea;
+    fold_eps(M, &er, &ea);
+    const double c0 = err_lo(ts[t0], er, ea);
+    const double c1 = t1 < T ? err_lo(ts[t1], er, ea) : c0;
+    uint64_t f0, f1;
+    wave_first_greater2(inv, c0, c1, inv.nn, &f0, &f1);
+    if ((threadIdx.x & (kWave - 1)) == 0) {
+        guess[t0] = (uint32_t)f0;
+        if (t1 < T) guess[t1] = (uint32_t)f1;
+    }
+}
+
+// The side stream's last step of a resample, and the join: with the weight
+// fold's approximate tile starts in fs.ts (ts_ready), the guesses for its
+// inversion after them.  Returns the guesses, or nullptr.
+static const uint32_t *join_side(ResampleBuffers &rb, hipStream_t st, uint64_t M, bool ts_ready)
+{
+    SideFold &sd = rb.side;
+    const uint32_t *g = nullptr;
+    if (ts_ready && rb.uguess) {
+        VSMCX_HIP(hipEventRecord(sd.starts, st));
+        VSMCX_HIP(hipStreamWaitEvent(sd.s, sd.starts, 0));
+        const uint64_t T = (M + kFT - 1) / kFT;
+        const uint64_t waves = (T + 1) / 2;
+        const unsigned blocks = (unsigned)((waves + kBlock / kWave - 1) / (kBlock / kWave));
+        const InvArray inv{rb.uscan, rb.u_tmaxex, 0};
+        hipLaunchKernelGGL(k_side_windows, dim3(blocks ? blocks : 1), dim3(kBlock), 0, sd.s,
+                           sd.uctl, rb.fs.ts, T, M, inv, rb.uguess);
+        g = rb.uguess;
+    }
     VSMCX_HIP(hipEventRecord(sd.join, sd.s));
+    return g;
 }
 
 void launch_side_uniforms(FilterCtl *ctl, ResampleBuffers &rb, hipStream_t st)
@@ -615,17 +662,20 @@ static void resample_pipeline(FilterCtl *ctl, W w, uint64_t M, int scheme, Resam
     const int sys_like = (scheme == kSystematic || scheme == kResidualSystematic) ? 1 : 0;
     Epi epi{hist, rb.anc ? 1 : 0, sys_like};
     hipEvent_t join = nullptr;
+    InvArray ia{rb.uscan, rb.u_tmaxex, 0};
     if (side) {
         join = sd.join;
         epi.uerr = &sd.uctl->err;
+        // multinomial: the reduce wrote the tile starts (pre_started)
+        if (!residual) ia.guess = join_side(rb, st, M, pre_started);
     } else if (sorted) {
         launch_uniform_fold(ctl, rb, st);
     }
-    InvArray ia{rb.uscan, rb.u_tmaxex, 0};
     if (residual) {
         // the scaled fractions: tile starts from the asum fold's tile sums, x
         // and integer parts from what its tile-sum pass stored
         hipLaunchKernelGGL(k_fold_scan_sum_scaled, dim3(1), dim3(kScanBlock), 0, st, ctl, fs);
+        if (side) ia.guess = join_side(rb, st, M, true);
         const XResidStored xs{fs.wx, fs.rep32, 0.0};
         if (scheme == kResidual)
             launch_fold_from_starts(ctl, xs, ia, M, fs, out, st, epi, join);

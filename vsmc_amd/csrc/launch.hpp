@@ -18,6 +18,7 @@ namespace vsmcx {
 struct SideFold {
     hipStream_t s = nullptr;
     hipEvent_t fork = nullptr, join = nullptr;
+    hipEvent_t starts = nullptr;  // the weight fold's approximate tile starts written
     FilterCtl *uctl = nullptr;
     bool pending = false;  // the uniforms of the coming resample are already launched
 };
@@ -31,6 +32,7 @@ struct ResampleBuffers {
     double *uscan = nullptr;     // [nuni] sorted uniforms: running max within fold tiles
     double *u_tmax = nullptr;    // [nuni / kFoldTile] tile maxima
     double *u_tmaxex = nullptr;  // [nuni / kFoldTile] exclusive running max of tiles
+    uint32_t *uguess = nullptr;  // [M / kFoldTile] guesses of the inversion's bounds (side fold)
     uint64_t nuni = 0;           // capacity of the uniform arrays
     FoldScratch fs{};
     // the sorted uniforms' own fold: tile sums and starts of its own; the

@@ -76,6 +76,7 @@ constexpr void layout_resample(A &a, ResampleBuffers &rb, uint64_t M, uint64_t n
         u.gval = a.template n<double>(nuni + 1);
         u.thr = a.template n<Seg>(utiles * kBlock);
         u.thr_fb = a.template n<int32_t>(utiles * kBlock);
+        rb.uguess = a.template n<uint32_t>(tiles);
     }
 }
 
