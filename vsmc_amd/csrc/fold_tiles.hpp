@@ -167,7 +167,7 @@ __device__ __noinline__ double strat_u(const Tf64State *eng, double delta, uint6
 struct InvStratified {
     static constexpr bool kInv = true;
     static constexpr bool kWindow = true;
-    static constexpr bool kWindowTmp = true;  // fill_window stages raw uniforms
+    static constexpr bool kWindowTmp = true;  // fill_window keeps a table in L.tmp
     static constexpr bool kPairF = true;
     const Tf64State *eng;  // the control block's resample engine (global memory)
     double delta;
@@ -181,13 +181,17 @@ struct InvStratified {
         const double b = u(n - 1);
         return a < b ? b : a;
     }
-    // v(w0 .. w0+len-1) into s_v (len < kFPad); tmp holds len + 1 doubles.
-    // u_n are generated a Threefry block (four consecutive draws) per thread.
-    __device__ void fill_window(double *s_v, double *tmp, uint64_t w0, int len) const
+    // v(w0 .. w0+len-1) into s_v (len <= kWin).  The u_n come a Threefry block
+    // (four consecutive draws) per thread; within a block v_n = max(u_n,
+    // u_{n-1}) is formed in registers, and the first draw of each block takes
+    // the previous block's last draw from s_last (kWinBlocks entries) after a
+    // barrier.  Every s_v entry is written by one thread in each phase.
+    __device__ void fill_window(double *s_v, double *s_last, uint64_t w0, int len) const
     {
         const uint64_t nlo = w0 ? w0 - 1 : 0, nhi = w0 + (uint64_t)len;
         const uint64_t index = eng->index;
         const uint64_t p0 = nlo + index, p1 = nhi + index;  // stream positions
+        const uint64_t b0 = p0 / 4, b1 = (p1 - 1) / 4;       // blocks
         u64x4 c0;
         uint64_t key[4];
 #pragma unroll
@@ -195,26 +199,30 @@ struct InvStratified {
             c0.v[j] = eng->ctr[j];
             key[j] = eng->key[j];
         }
-        for (uint64_t bk = p0 / 4 + threadIdx.x; bk <= (p1 - 1) / 4; bk += blockDim.x) {
+        for (uint64_t bk = b0 + threadIdx.x; bk <= b1; bk += blockDim.x) {
             const u64x4 r = threefry4x64_20(ctr_add(c0, bk), key);
+            double prev = -1.0;  // u of the block's previous draw in range
 #pragma unroll
             for (int w = 0; w < 4; ++w) {
                 const uint64_t p = 4 * bk + w;
                 if (p >= p0 && p < p1) {
                     const uint64_t n = p - index;
-                    tmp[n - nlo] = canonical64(r.v[w]) * delta + cnt_d(n) * delta;
+                    const double u = canonical64(r.v[w]) * delta + cnt_d(n) * delta;
+                    if (n >= w0) s_v[n - w0] = u < prev ? prev : u;
+                    prev = u;
                 }
             }
+            s_last[bk - b0] = prev;
         }
         __syncthreads();
-        for (int k = threadIdx.x; k < len; k += blockDim.x) {
-            const uint64_t n = w0 + k;
-            double a = tmp[n - nlo];
-            if (n > 0) {
-                const double b = tmp[n - 1 - nlo];
-                a = a < b ? b : a;
+        // the first draw of every later block: 4 bk is in range for b0 < bk <= b1
+        for (uint64_t bk = b0 + 1 + threadIdx.x; bk <= b1; bk += blockDim.x) {
+            const uint64_t n = 4 * bk - index;
+            if (n >= w0) {
+                const uint64_t k = n - w0;
+                const double b = s_last[bk - b0 - 1];
+                s_v[k] = s_v[k] < b ? b : s_v[k];
             }
-            s_v[k] = a;
         }
     }
     // u_n = (n + U_n) delta with U_n in [0, 1), so the first n with v_n > c
@@ -547,13 +555,13 @@ __device__ __forceinline__ double fold_tile_sum(const XSrc &xs, uint64_t base, u
     return block_reduce<kBlock>(s, AddOp(), s_red);
 }
 
-// LDS of the per-tile bodies; the second window buffer only where the
-// inversion stages raw uniforms in it (stratified), which halves the LDS of
-// the other windowed inversion and doubles its workgroups per CU.
+// LDS of the per-tile bodies; the table of Threefry blocks' last draws only
+// where the inversion generates its uniforms (stratified).
+constexpr int kWinBlocks = kWin / 4 + 3;  // Threefry blocks a window of kWin draws + 1 spans
 template <bool kWindowBuf>
 struct FoldLdsT {
     double x[kFPad];
-    double tmp[kWindowBuf ? kFPad : 1];
+    double tmp[kWindowBuf ? kWinBlocks : 1];
     double red[kBlock / kWave];
     Seg seg[kBlock / kWave];
     uint64_t u64[kBlock / kWave];
@@ -789,7 +797,7 @@ __device__ void fold_tile_apply(FilterCtl *ctl, const XSrc &xs, const Inv &inv, 
         // ---- inversion: c_i = F(a_i) for i <= M-2 -------------------------------
         const uint64_t nn = inv.nn;
         const uint64_t last_cmp = M >= 2 ? M - 2 : 0;  // highest index compared
-        uint64_t c[kFI];
+        uint32_t c[kFI];  // counts below 2^31
         double av[kFI];
 #pragma unroll
         for (int k = 0; k < kFI; ++k) av[k] = s_a[fidx(e0 + k)];
@@ -798,7 +806,7 @@ __device__ void fold_tile_apply(FilterCtl *ctl, const XSrc &xs, const Inv &inv, 
 #pragma unroll
             for (int k = 0; k < kFI; ++k) {
                 const uint64_t i = base + e0 + k;
-                c[k] = (M >= 2 && i <= last_cmp) ? inv.F(av[k]) : 0;
+                c[k] = (M >= 2 && i <= last_cmp) ? (uint32_t)inv.F(av[k]) : 0u;
             }
             if (t > 0) cstart = inv.F(v_start);
         } else {
@@ -850,7 +858,7 @@ __device__ void fold_tile_apply(FilterCtl *ctl, const XSrc &xs, const Inv &inv, 
             const uint64_t wlo = s_bu[0], whi = s_bu[1];
             cstart = wlo;
 #pragma unroll
-            for (int k = 0; k < kFI; ++k) c[k] = wlo;
+            for (int k = 0; k < kFI; ++k) c[k] = (uint32_t)wlo;
             for (uint64_t w0 = wlo; w0 < whi; w0 += kWin) {
                 const int len = (int)((whi - w0) < (uint64_t)kWin ? (whi - w0) : (uint64_t)kWin);
                 __syncthreads();
