@@ -41,11 +41,34 @@ __device__ __forceinline__ void cv_move_tile(
     // Loads first, for all of the thread's particles (ancestor indices, then
     // the gathered states and old log-weights), so their latencies overlap;
     // then the arithmetic and the stores.
+    // The ancestor words of all the thread's particles are loaded before any is
+    // looked at; then the zero slots' sources come from the surplus list
+    // (anc_source), those loads again issued together.
     uint32_t src[kMoveItems];  // particle indices stay below 2^32
+    if (t > 0 && prev) {  // uniform over the launch
+#pragma unroll
+        for (int k = 0; k < kMoveItems; ++k) {
+            const uint64_t i = base + (uint64_t)k * kBlock + threadIdx.x;
+            const uint32_t a = anc[i < N ? i : N - 1];  // in bounds, no branch
+            src[k] = i < N ? a : (uint32_t)i;
+        }
+#pragma unroll
+        for (int k = 0; k < kMoveItems; ++k) {
+            const uint32_t a = src[k];
+            if (a & kAncZero) {
+                const uint32_t r = a & ~kAncZero;
+                src[k] = r < N ? sl[r] : ~0u;
+            }
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < kMoveItems; ++k)
+            src[k] = (uint32_t)(base + (uint64_t)k * kBlock + threadIdx.x);
+    }
 #pragma unroll
     for (int k = 0; k < kMoveItems; ++k) {
         const uint64_t i = base + (uint64_t)k * kBlock + threadIdx.x;
-        const uint32_t j = (t > 0 && prev && i < N) ? anc_source(anc, sl, i) : (uint32_t)i;
+        const uint32_t j = src[k];
         src[k] = j < N ? j : (i < N ? (uint32_t)i : 0u);  // never read outside the state buffer
     }
     S4 s[kMoveItems];

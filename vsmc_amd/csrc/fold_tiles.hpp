@@ -419,19 +419,36 @@ __device__ void wave_first_greater2(const Inv &inv, double c_lo, double c_hi, ui
     *f_hi = __shfl(r, 32);
 }
 
-// x of one tile into LDS (striped, coalesced); zero past the end, which
-// classifies as negligible.
+// x of one tile, striped (element k * kBlock + tid of the tile in v[k]); zero
+// past the end, which classifies as negligible.  Every value is computed at an
+// index clamped into [0, M) and zeroed past the end afterwards, so the loop
+// has no branch and a thread's loads are issued together (a branch per
+// element would wait out one load before the next).
+template <typename XSrc>
+__device__ __forceinline__ void tile_values(const XSrc &xs, uint64_t base, uint64_t M,
+                                            double v[kFI])
+{
+#pragma unroll
+    for (int k = 0; k < kFI; ++k) {
+        const uint64_t i = base + k * kBlock + threadIdx.x;
+        const double t = xs.x(i < M ? i : M - 1);
+        v[k] = i < M ? t : 0.0;
+    }
+}
+
+// x of one tile into LDS (striped, coalesced)
 template <typename XSrc>
 __device__ __forceinline__ void load_tile(const XSrc &xs, uint64_t base, uint64_t M, double *s_x,
                                           double *wx_out = nullptr)
 {
+    double v[kFI];
+    tile_values(xs, base, M, v);
 #pragma unroll
     for (int k = 0; k < kFI; ++k) {
         const int e = k * kBlock + threadIdx.x;
         const uint64_t i = base + e;
-        const double v = i < M ? xs.x(i) : 0.0;
-        s_x[fidx(e)] = v;
-        if (wx_out && i < M) wx_out[i] = v;
+        s_x[fidx(e)] = v[k];
+        if (wx_out && i < M) wx_out[i] = v[k];
     }
 }
 
@@ -488,8 +505,12 @@ __device__ T wg_scan_b(In in, T *out, uint64_t n, Op op, T ident, T *lds, Visit 
     for (uint64_t c0 = 0; c0 < n; c0 += (uint64_t)BLOCK * kScanItems) {
         const uint64_t i0 = c0 + (uint64_t)threadIdx.x * kScanItems;
         T v[kScanItems];
+        // loads at indices clamped into [0, n): no branch between them
 #pragma unroll
-        for (int k = 0; k < kScanItems; ++k) v[k] = i0 + k < n ? in[i0 + k] : ident;
+        for (int k = 0; k < kScanItems; ++k) {
+            const T x = in[i0 + k < n ? i0 + k : n - 1];
+            v[k] = i0 + k < n ? x : ident;
+        }
         T loc = ident;
 #pragma unroll
         for (int k = 0; k < kScanItems; ++k) loc = op(loc, v[k]);
@@ -544,13 +565,14 @@ template <typename XSrc>
 __device__ __forceinline__ double fold_tile_sum(const XSrc &xs, uint64_t base, uint64_t M,
                                                 double *s_red, double *wx_out = nullptr)
 {
+    double v[kFI];
+    tile_values(xs, base, M, v);
     double s = 0;
 #pragma unroll
     for (int k = 0; k < kFI; ++k) {
         const uint64_t i = base + k * kBlock + threadIdx.x;
-        const double v = i < M ? xs.x(i) : 0.0;
-        if (wx_out && i < M) wx_out[i] = v;
-        s += v;
+        if (wx_out && i < M) wx_out[i] = v[k];
+        s += v[k];
     }
     return block_reduce<kBlock>(s, AddOp(), s_red);
 }
@@ -999,7 +1021,8 @@ __device__ void fold_tile_layout(FilterCtl *ctl, uint64_t t, uint64_t M, const u
 #pragma unroll
     for (int k = 0; k < kFI; ++k) {
         const int e = k * kBlock + tid;
-        s_rep[fidx(e)] = e < nv ? rt[e] : 1u;
+        const uint32_t r = rt[e < nv ? e : nv - 1];  // in bounds, no branch
+        s_rep[fidx(e)] = e < nv ? r : 1u;
     }
     __syncthreads();
     uint32_t rv[kFI];

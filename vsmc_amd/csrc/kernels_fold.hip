@@ -292,13 +292,17 @@ __global__ __launch_bounds__(kBlock) void k_resid_tsum(FilterCtl *ctl, W w, uint
     const uint64_t base = blockIdx.x * (uint64_t)kFT;
     double s = 0;
     uint64_t is = 0;
+    // the weights of all the thread's elements first (no branch between the
+    // loads), then the splits and stores
+    double wv[kFI];
+    tile_values(xr.w, base, M, wv);
 #pragma unroll
     for (int k = 0; k < kFI; ++k) {
         const uint64_t i = base + k * kBlock + threadIdx.x;
         double r = 0.0;
         if (i < M) {
             double ip;
-            r = modf(xr.nd * xr.w.x(i), &ip);
+            r = modf(xr.nd * wv[k], &ip);
             fs.wx[i] = r;
             fs.rep32[i] = (uint32_t)ip;
             is += (uint64_t)ip;

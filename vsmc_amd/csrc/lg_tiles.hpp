@@ -42,8 +42,13 @@ __device__ __forceinline__ void lg_move_tile(const FilterCtl *ctl, const Real *_
         if (t > 0) {
             uint64_t src = prev ? (uint64_t)anc_source(anc, sl, i) : i;
             src = src < N ? src : i;
+            // the state's loads at dimensions clamped into [0, D): no branch
+            // between them
 #pragma unroll
-            for (int d = 0; d < kLgMaxDim; ++d) x[d] = d < D ? in[src * D + d] : (Real)0;
+            for (int d = 0; d < kLgMaxDim; ++d) {
+                const Real xv = in[src * D + (d < D ? d : D - 1)];
+                x[d] = d < D ? xv : (Real)0;
+            }
             if (!prev) lwo = lw[i];
             const double w = prev ? invN : exp(lwo - m_old) * c_old;
             mx += w * (double)x[0];
@@ -122,8 +127,10 @@ __device__ __forceinline__ void lg_monitor_tile(const FilterCtl *ctl, const Real
         uint64_t src = prev ? (uint64_t)anc_source(anc, sl, i) : i;
         src = src < N ? src : i;
         const double w = prev ? invN : exp(lw[i] - m) * c;
+        // the second coordinate is 0 for a one-dimensional state, as in the move
+        const Real x1 = st[src * D + (D > 1 ? 1 : 0)];
         v.mx += w * (double)st[src * D];
-        v.my += w * (double)st[src * D + 1];
+        v.my += w * (D > 1 ? (double)x1 : 0.0);
     }
     v = block_reduce<kBlock>(v, Part5Add(), s_p5);
     if (threadIdx.x == 0) {
