@@ -71,15 +71,22 @@ __device__ __forceinline__ void cv_move_tile(
         const uint32_t j = src[k];
         src[k] = j < N ? j : (i < N ? (uint32_t)i : 0u);  // never read outside the state buffer
     }
+    // Every src[k] is in bounds, so the state loads need no branch (a load in
+    // a branch of its own waits for the loads before it); past N the values
+    // are not used.
     S4 s[kMoveItems];
     double lwo[kMoveItems];
 #pragma unroll
-    for (int k = 0; k < kMoveItems; ++k) {
-        const uint64_t i = base + (uint64_t)k * kBlock + threadIdx.x;
-        lwo[k] = 0;
-        if (t > 0 && i < N) {
-            s[k] = st_in[src[k]];
-            if (!prev) lwo[k] = lw[i];
+    for (int k = 0; k < kMoveItems; ++k) lwo[k] = 0;
+    if (t > 0) {  // uniform over the launch
+#pragma unroll
+        for (int k = 0; k < kMoveItems; ++k) s[k] = st_in[src[k]];
+        if (!prev) {
+#pragma unroll
+            for (int k = 0; k < kMoveItems; ++k) {
+                const uint64_t i = base + (uint64_t)k * kBlock + threadIdx.x;
+                lwo[k] = lw[i < N ? i : N - 1];
+            }
         }
     }
 
