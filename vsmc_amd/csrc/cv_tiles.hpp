@@ -17,35 +17,17 @@
 
 namespace vsmcx {
 
-// One move tile (kMoveTile particles): writes the tile's partials
-// (max, sum e, sum e^2, monitor x, monitor y) to part[0..4].
-template <typename Real>
-__device__ __forceinline__ void cv_move_tile(
-    const FilterCtl *ctl, const typename StateOf<Real>::type *__restrict__ st_in,
-    typename StateOf<Real>::type *__restrict__ st_out, const uint32_t *__restrict__ anc,
-    const uint32_t *__restrict__ sl, double *__restrict__ lw, const double *__restrict__ obs,
-    const CvParams &p, uint32_t k0, uint32_t k1, uint64_t t, uint64_t N, uint64_t tile,
-    double *__restrict__ part)
+// The source of each of the thread's particles in a move tile: with `gather`
+// (uniform over the launch) the ancestor of the last resample, a zero slot's
+// (kAncZero | its zero rank) taken from the surplus list at that rank
+// (anc_source); else the particle itself.  All the thread's ancestor words are
+// loaded before any is looked at, and the zero slots' list loads are again
+// issued together.  Every result is an index into the state buffer (< N).
+__device__ __forceinline__ void tile_sources(const uint32_t *__restrict__ anc,
+                                             const uint32_t *__restrict__ sl, bool gather,
+                                             uint64_t N, uint64_t base, uint32_t src[kMoveItems])
 {
-    typedef typename StateOf<Real>::type S4;
-    __shared__ double s_red[kBlock / kWave];
-    __shared__ Part5 s_p5[kBlock / kWave];
-    CvModel<Real> md;
-    md.load(p);
-    const int prev = t > 0 ? ctl->prev_resampled : 0;
-    const double m_old = ctl->m, c_old = ctl->coeff;
-    const double invN = 1.0 / (double)N;
-    const CvObs<Real> o(obs[2 * t], obs[2 * t + 1]);
-    const uint64_t base = tile * (uint64_t)kMoveTile;
-
-    // Loads first, for all of the thread's particles (ancestor indices, then
-    // the gathered states and old log-weights), so their latencies overlap;
-    // then the arithmetic and the stores.
-    // The ancestor words of all the thread's particles are loaded before any is
-    // looked at; then the zero slots' sources come from the surplus list
-    // (anc_source), those loads again issued together.
-    uint32_t src[kMoveItems];  // particle indices stay below 2^32
-    if (t > 0 && prev) {  // uniform over the launch
+    if (gather) {
 #pragma unroll
         for (int k = 0; k < kMoveItems; ++k) {
             const uint64_t i = base + (uint64_t)k * kBlock + threadIdx.x;
@@ -71,6 +53,34 @@ __device__ __forceinline__ void cv_move_tile(
         const uint32_t j = src[k];
         src[k] = j < N ? j : (i < N ? (uint32_t)i : 0u);  // never read outside the state buffer
     }
+}
+
+// One move tile (kMoveTile particles): writes the tile's partials
+// (max, sum e, sum e^2, monitor x, monitor y) to part[0..4].
+template <typename Real>
+__device__ __forceinline__ void cv_move_tile(
+    const FilterCtl *ctl, const typename StateOf<Real>::type *__restrict__ st_in,
+    typename StateOf<Real>::type *__restrict__ st_out, const uint32_t *__restrict__ anc,
+    const uint32_t *__restrict__ sl, double *__restrict__ lw, const double *__restrict__ obs,
+    const CvParams &p, uint32_t k0, uint32_t k1, uint64_t t, uint64_t N, uint64_t tile,
+    double *__restrict__ part)
+{
+    typedef typename StateOf<Real>::type S4;
+    __shared__ double s_red[kBlock / kWave];
+    __shared__ Part5 s_p5[kBlock / kWave];
+    CvModel<Real> md;
+    md.load(p);
+    const int prev = t > 0 ? ctl->prev_resampled : 0;
+    const double m_old = ctl->m, c_old = ctl->coeff;
+    const double invN = 1.0 / (double)N;
+    const CvObs<Real> o(obs[2 * t], obs[2 * t + 1]);
+    const uint64_t base = tile * (uint64_t)kMoveTile;
+
+    // Loads first, for all of the thread's particles (ancestor indices, then
+    // the gathered states and old log-weights), so their latencies overlap;
+    // then the arithmetic and the stores.
+    uint32_t src[kMoveItems];
+    tile_sources(anc, sl, t > 0 && prev, N, base, src);
     // Every src[k] is in bounds, so the state loads need no branch (a load in
     // a branch of its own waits for the loads before it); past N the values
     // are not used.
@@ -336,13 +346,24 @@ __device__ __forceinline__ void cv_monitor_tile(const FilterCtl *ctl,
     const double m = ctl->m, c = ctl->coeff, invN = 1.0 / (double)N;
     const uint64_t base = tile * (uint64_t)kMoveTile;
     Part5 v{0, 0, 0, 0};
+    // loads first (sources, states, log-weights), then the sums in particle order
+    uint32_t src[kMoveItems];
+    tile_sources(anc, sl, prev != 0, N, base, src);
+    typename StateOf<Real>::type s[kMoveItems];
+    double lwv[kMoveItems];
+#pragma unroll
+    for (int k = 0; k < kMoveItems; ++k) {
+        const uint64_t i = base + (uint64_t)k * kBlock + threadIdx.x;
+        s[k] = st[src[k]];
+        lwv[k] = prev ? 0.0 : lw[i < N ? i : N - 1];
+    }
+#pragma unroll
     for (int k = 0; k < kMoveItems; ++k) {
         const uint64_t i = base + (uint64_t)k * kBlock + threadIdx.x;
         if (i >= N) continue;
-        const auto s = st[prev ? (uint64_t)anc_source(anc, sl, i) : i];
-        const double w = prev ? invN : exp(lw[i] - m) * c;
-        v.mx += w * (double)s.x;
-        v.my += w * (double)s.y;
+        const double w = prev ? invN : exp(lwv[k] - m) * c;
+        v.mx += w * (double)s[k].x;
+        v.my += w * (double)s[k].y;
     }
     v = block_reduce<kBlock>(v, Part5Add(), s_p5);
     if (threadIdx.x == 0) {

@@ -546,8 +546,8 @@ struct TileSumsFromLse {
     double m, coeff;
     __device__ double operator[](uint64_t t) const
     {
-        const double mt = part[5 * t];
-        return mt == -INFINITY ? 0.0 : part[5 * t + 1] * exp(mt - m) * coeff;
+        const double mt = part[5 * t], se = part[5 * t + 1];  // both loads before the test
+        return mt == -INFINITY ? 0.0 : se * exp(mt - m) * coeff;
     }
 };
 

@@ -5,6 +5,7 @@
 // have the layout of the CV tiles (cv_tiles.hpp).
 #pragma once
 #include "cv_model.hpp"
+#include "cv_tiles.hpp"  // tile_sources
 #include "device_util.hpp"
 #include "launch.hpp"
 #include "lg_model.hpp"
@@ -121,16 +122,27 @@ __device__ __forceinline__ void lg_monitor_tile(const FilterCtl *ctl, const Real
     const double m = ctl->m, c = ctl->coeff, invN = 1.0 / (double)N;
     const uint64_t base = tile * (uint64_t)kMoveTile;
     Part5 v{0, 0, 0, 0};
+    // loads first (sources, the first two coordinates, log-weights), then the
+    // sums in particle order; the second coordinate is 0 for a one-dimensional
+    // state, as in the move
+    uint32_t src[kMoveItems];
+    tile_sources(anc, sl, prev != 0, N, base, src);
+    Real x0[kMoveItems], x1[kMoveItems];
+    double lwv[kMoveItems];
+#pragma unroll
+    for (int k = 0; k < kMoveItems; ++k) {
+        const uint64_t i = base + (uint64_t)k * kBlock + threadIdx.x;
+        x0[k] = st[(uint64_t)src[k] * D];
+        x1[k] = st[(uint64_t)src[k] * D + (D > 1 ? 1 : 0)];
+        lwv[k] = prev ? 0.0 : lw[i < N ? i : N - 1];
+    }
+#pragma unroll
     for (int k = 0; k < kMoveItems; ++k) {
         const uint64_t i = base + (uint64_t)k * kBlock + threadIdx.x;
         if (i >= N) continue;
-        uint64_t src = prev ? (uint64_t)anc_source(anc, sl, i) : i;
-        src = src < N ? src : i;
-        const double w = prev ? invN : exp(lw[i] - m) * c;
-        // the second coordinate is 0 for a one-dimensional state, as in the move
-        const Real x1 = st[src * D + (D > 1 ? 1 : 0)];
-        v.mx += w * (double)st[src * D];
-        v.my += w * (D > 1 ? (double)x1 : 0.0);
+        const double w = prev ? invN : exp(lwv[k] - m) * c;
+        v.mx += w * (double)x0[k];
+        v.my += w * (D > 1 ? (double)x1[k] : 0.0);
     }
     v = block_reduce<kBlock>(v, Part5Add(), s_p5);
     if (threadIdx.x == 0) {
