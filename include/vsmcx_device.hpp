@@ -54,6 +54,17 @@ VSMCX_HD uint32_t mulhi32(uint32_t a, uint32_t b)
     return (uint32_t)(((uint64_t)a * (uint64_t)b) >> 32);
 }
 
+// a ^ b ^ c: one v_bitop3_b32 on the device (gfx950's three-input bit
+// operation, truth table 0x96), two XORs on the host
+VSMCX_HD uint32_t xor3(uint32_t a, uint32_t b, uint32_t c)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_bitop3_b32(a, b, c, 0x96);
+#else
+    return a ^ b ^ c;
+#endif
+}
+
 VSMCX_HD u32x4 philox4x32_10(u32x4 c, uint32_t k0, uint32_t k1)
 {
     uint32_t s0 = c.v[0], s1 = c.v[1], s2 = c.v[2], s3 = c.v[3];
@@ -66,8 +77,8 @@ VSMCX_HD u32x4 philox4x32_10(u32x4 c, uint32_t k0, uint32_t k1)
         // one 32 x 32 -> 64 product gives both halves (v_mad_u64_u32 on gfx950)
         const uint64_t p0 = (uint64_t)0xCD9E8D57u * s2;
         const uint64_t p1 = (uint64_t)0xD2511F53u * s0;
-        const uint32_t n0 = (uint32_t)(p0 >> 32) ^ k0 ^ s1;
-        const uint32_t n2 = (uint32_t)(p1 >> 32) ^ k1 ^ s3;
+        const uint32_t n0 = xor3((uint32_t)(p0 >> 32), s1, k0);
+        const uint32_t n2 = xor3((uint32_t)(p1 >> 32), s3, k1);
         s0 = n0;
         s1 = (uint32_t)p0;
         s2 = n2;

@@ -23,16 +23,18 @@ namespace vsmcx {
 // (anc_source); else the particle itself.  All the thread's ancestor words are
 // loaded before any is looked at, and the zero slots' list loads are again
 // issued together.  Every result is an index into the state buffer (< N).
+template <bool kFull = false>
 __device__ __forceinline__ void tile_sources(const uint32_t *__restrict__ anc,
                                              const uint32_t *__restrict__ sl, bool gather,
                                              uint64_t N, uint64_t base, uint32_t src[kMoveItems])
 {
+    // kFull: the whole tile lies below N (every tile of a launch but the last)
     if (gather) {
 #pragma unroll
         for (int k = 0; k < kMoveItems; ++k) {
             const uint64_t i = base + (uint64_t)k * kBlock + threadIdx.x;
-            const uint32_t a = anc[i < N ? i : N - 1];  // in bounds, no branch
-            src[k] = i < N ? a : (uint32_t)i;
+            const uint32_t a = anc[kFull || i < N ? i : N - 1];  // in bounds, no branch
+            src[k] = kFull || i < N ? a : (uint32_t)i;
         }
 #pragma unroll
         for (int k = 0; k < kMoveItems; ++k) {
@@ -51,14 +53,16 @@ __device__ __forceinline__ void tile_sources(const uint32_t *__restrict__ anc,
     for (int k = 0; k < kMoveItems; ++k) {
         const uint64_t i = base + (uint64_t)k * kBlock + threadIdx.x;
         const uint32_t j = src[k];
-        src[k] = j < N ? j : (i < N ? (uint32_t)i : 0u);  // never read outside the state buffer
+        // never read outside the state buffer
+        src[k] = j < N ? j : (kFull || i < N ? (uint32_t)i : 0u);
     }
 }
 
 // One move tile (kMoveTile particles): writes the tile's partials
-// (max, sum e, sum e^2, monitor x, monitor y) to part[0..4].
-template <typename Real>
-__device__ __forceinline__ void cv_move_tile(
+// (max, sum e, sum e^2, monitor x, monitor y) to part[0..4].  kFull: the
+// tile lies below N, so no particle needs a bounds test.
+template <typename Real, bool kFull>
+__device__ __forceinline__ void cv_move_tile_body(
     const FilterCtl *ctl, const typename StateOf<Real>::type *__restrict__ st_in,
     typename StateOf<Real>::type *__restrict__ st_out, const uint32_t *__restrict__ anc,
     const uint32_t *__restrict__ sl, double *__restrict__ lw, const double *__restrict__ obs,
@@ -80,7 +84,7 @@ __device__ __forceinline__ void cv_move_tile(
     // the gathered states and old log-weights), so their latencies overlap;
     // then the arithmetic and the stores.
     uint32_t src[kMoveItems];
-    tile_sources(anc, sl, t > 0 && prev, N, base, src);
+    tile_sources<kFull>(anc, sl, t > 0 && prev, N, base, src);
     // Every src[k] is in bounds, so the state loads need no branch (a load in
     // a branch of its own waits for the loads before it); past N the values
     // are not used.
@@ -95,7 +99,7 @@ __device__ __forceinline__ void cv_move_tile(
 #pragma unroll
             for (int k = 0; k < kMoveItems; ++k) {
                 const uint64_t i = base + (uint64_t)k * kBlock + threadIdx.x;
-                lwo[k] = lw[i < N ? i : N - 1];
+                lwo[k] = lw[kFull || i < N ? i : N - 1];
             }
         }
     }
@@ -106,10 +110,10 @@ __device__ __forceinline__ void cv_move_tile(
     for (int k = 0; k < kMoveItems; ++k) {
         const uint64_t i = base + (uint64_t)k * kBlock + threadIdx.x;
         lwn[k] = -INFINITY;
-        if (i >= N) continue;
+        if (!kFull && i >= N) continue;
         S4 &sk = s[k];
         if (t > 0) {
-            const double w = prev ? invN : exp(lwo[k] - m_old) * c_old;
+            const double w = prev ? invN : exp_lse(lwo[k] - m_old) * c_old;
             mx += w * (double)sk.x;
             my += w * (double)sk.y;
         }
@@ -127,7 +131,7 @@ __device__ __forceinline__ void cv_move_tile(
     if (M != -INFINITY) {
 #pragma unroll
         for (int k = 0; k < kMoveItems; ++k) {
-            const double e = exp(lwn[k] - M);
+            const double e = exp_lse(lwn[k] - M);
             se += e;
             se2 += e * e;
         }
@@ -141,6 +145,22 @@ __device__ __forceinline__ void cv_move_tile(
         part[3] = v.mx;
         part[4] = v.my;
     }
+}
+
+template <typename Real>
+__device__ __forceinline__ void cv_move_tile(
+    const FilterCtl *ctl, const typename StateOf<Real>::type *__restrict__ st_in,
+    typename StateOf<Real>::type *__restrict__ st_out, const uint32_t *__restrict__ anc,
+    const uint32_t *__restrict__ sl, double *__restrict__ lw, const double *__restrict__ obs,
+    const CvParams &p, uint32_t k0, uint32_t k1, uint64_t t, uint64_t N, uint64_t tile,
+    double *__restrict__ part)
+{
+    if ((tile + 1) * (uint64_t)kMoveTile <= N)
+        cv_move_tile_body<Real, true>(ctl, st_in, st_out, anc, sl, lw, obs, p, k0, k1, t, N, tile,
+                                      part);
+    else
+        cv_move_tile_body<Real, false>(ctl, st_in, st_out, anc, sl, lw, obs, p, k0, k1, t, N,
+                                       tile, part);
 }
 
 // Tile partials {max, sum e, sum e^2, monitor x, monitor y} of ntiles tiles,
@@ -206,7 +226,7 @@ __device__ __forceinline__ Part5 vwave_sums(const double *partials, uint64_t nti
 #pragma unroll
             for (int c = 0; c < 5; ++c) q[u][c] = partials[5 * (j + (uint64_t)u * S) + c];
 #pragma unroll
-        for (int u = 0; u < A; ++u) sc[u] = q[u][0] == -INFINITY ? 0.0 : exp(q[u][0] - M);
+        for (int u = 0; u < A; ++u) sc[u] = q[u][0] == -INFINITY ? 0.0 : exp_lse(q[u][0] - M);
 #pragma unroll
         for (int u = 0; u < A; ++u) {
             a.se += q[u][1] * sc[u];
@@ -217,7 +237,7 @@ __device__ __forceinline__ Part5 vwave_sums(const double *partials, uint64_t nti
     }
     for (; j < ntiles; j += S) {
         const double *q = partials + 5 * j;
-        const double sc = q[0] == -INFINITY ? 0.0 : exp(q[0] - M);
+        const double sc = q[0] == -INFINITY ? 0.0 : exp_lse(q[0] - M);
         a.se += q[1] * sc;
         a.se2 += q[2] * sc * sc;
         a.mx += q[3];
@@ -361,7 +381,7 @@ __device__ __forceinline__ void cv_monitor_tile(const FilterCtl *ctl,
     for (int k = 0; k < kMoveItems; ++k) {
         const uint64_t i = base + (uint64_t)k * kBlock + threadIdx.x;
         if (i >= N) continue;
-        const double w = prev ? invN : exp(lwv[k] - m) * c;
+        const double w = prev ? invN : exp_lse(lwv[k] - m) * c;
         v.mx += w * (double)s[k].x;
         v.my += w * (double)s[k].y;
     }

@@ -13,6 +13,7 @@
 
 #include "device_util.hpp"
 #include "fold_core.hpp"
+#include "math_core.hpp"
 #include "internal.hpp"
 
 namespace vsmcx {
@@ -56,7 +57,7 @@ struct XWeight {  // W_i = exp(lw_i - m) * coeff, computed from the log-weights
     const double *lw;
     double m, coeff;
     __device__ void bind(const FilterCtl *c) { m = c->m; coeff = c->coeff; }
-    __device__ double x(uint64_t i) const { return exp(lw[i] - m) * coeff; }
+    __device__ double x(uint64_t i) const { return exp_lse(lw[i] - m) * coeff; }
     __device__ uint64_t ipart(uint64_t) const { return 0; }
 };
 
@@ -547,7 +548,7 @@ struct TileSumsFromLse {
     __device__ double operator[](uint64_t t) const
     {
         const double mt = part[5 * t], se = part[5 * t + 1];  // both loads before the test
-        return mt == -INFINITY ? 0.0 : se * exp(mt - m) * coeff;
+        return mt == -INFINITY ? 0.0 : se * exp_lse(mt - m) * coeff;
     }
 };
 

@@ -91,7 +91,7 @@ __global__ void k_cv_weights(const FilterCtl *ctl, const double *lw, uint64_t N,
     const double m = ctl->m, c = ctl->coeff;
     for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < N;
          i += (uint64_t)gridDim.x * blockDim.x)
-        w[i] = exp(lw[i] - m) * c;
+        w[i] = exp_lse(lw[i] - m) * c;
 }
 
 template <typename S4>

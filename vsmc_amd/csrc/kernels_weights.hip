@@ -36,7 +36,7 @@ __global__ __launch_bounds__(kBlock) void k_ws_add(const FilterCtl *ctl, double 
     if (M != -INFINITY) {
 #pragma unroll
         for (int k = 0; k < kMoveItems; ++k) {
-            const double e = exp(v[k] - M);
+            const double e = exp_lse(v[k] - M);
             se += e;
             se2 += e * e;
         }

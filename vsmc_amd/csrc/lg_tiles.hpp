@@ -51,7 +51,7 @@ __device__ __forceinline__ void lg_move_tile(const FilterCtl *ctl, const Real *_
                 x[d] = d < D ? xv : (Real)0;
             }
             if (!prev) lwo = lw[i];
-            const double w = prev ? invN : exp(lwo - m_old) * c_old;
+            const double w = prev ? invN : exp_lse(lwo - m_old) * c_old;
             mx += w * (double)x[0];
             my += w * (double)x[1];
         }
@@ -94,7 +94,7 @@ __device__ __forceinline__ void lg_move_tile(const FilterCtl *ctl, const Real *_
     if (M != -INFINITY) {
 #pragma unroll
         for (int k = 0; k < kMoveItems; ++k) {
-            const double e = exp(lwv[k] - M);
+            const double e = exp_lse(lwv[k] - M);
             se += e;
             se2 += e * e;
         }
@@ -140,7 +140,7 @@ __device__ __forceinline__ void lg_monitor_tile(const FilterCtl *ctl, const Real
     for (int k = 0; k < kMoveItems; ++k) {
         const uint64_t i = base + (uint64_t)k * kBlock + threadIdx.x;
         if (i >= N) continue;
-        const double w = prev ? invN : exp(lwv[k] - m) * c;
+        const double w = prev ? invN : exp_lse(lwv[k] - m) * c;
         v.mx += w * (double)x0[k];
         v.my += w * (D > 1 ? (double)x1[k] : 0.0);
     }

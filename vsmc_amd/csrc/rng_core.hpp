@@ -13,6 +13,7 @@
 // the block functions, counters and u01 conversions (also the public device
 // header for user kernels)
 #include "../../include/vsmcx_device.hpp"
+#include "math_core.hpp"
 
 namespace vsmcx {
 
@@ -84,7 +85,9 @@ VSMCX_HD double u01_oc_53(uint32_t u) { return u01_open_closed_32_53(u); }
 // uniform.  The open-closed 24-bit form (u01.h:112-113) takes the top of each
 // 2^-24 cell instead, which lowers E[-2 log u] by about 1e-6: every FP32
 // normal from it would be slightly short, a bias shared by all particles.
-VSMCX_HD float u01_oo_24(uint32_t u) { return u01_open_open_32_24(u); }
+// The kernels take the same value from its bits (u01_oo_24_bits: two
+// instructions instead of a shift, a conversion, an add and a multiply).
+VSMCX_HD float u01_oo_24(uint32_t u) { return u01_oo_24_bits(u); }
 
 #if defined(__HIPCC__)
 // Box-Muller pair in FP32 on the hardware transcendentals: v_log_f32 is log2,
