@@ -121,29 +121,32 @@ static int check_case(const std::vector<double> &x, int items, int tile, double 
         }
     double tot;
     if (!tr_apply(acc.tr, v, &tot)) return 6;
-    // pass 5: every value
+    // pass 5: every value, as the apply pass computes it: the exact value
+    // before each thread's run from its prefix summary, then the run's own
+    // fold steps; every run must end where the next run starts (the last run:
+    // at the fold total), or the pass reports a broken bound (rc 2)
+    double end_prev = 0.0;  // exact value before the first run
     for (size_t t = 0; t < ntiles; ++t) {
         Seg tp = pre[t];
         for (int j = 0; j < nthr; ++j) {
             const size_t i0 = t * tile + (size_t)j * items;
             const size_t id = t * nthr + j;
-            auto xat = [&](int k) { return X(i0 + k); };
-            std::vector<double> got(items);
-            const bool ok = run_values(
-                xat, items, s_thr[id], er, ea, fast[id], tp,
-                [&](uint64_t c) { return gval[c]; }, [&](int k, double a) { got[k] = a; });
-            if (!ok) return 2;
+            const RunStart st = run_start(tp, [&](uint64_t c) { return gval[c]; });
+            if (!st.ok || !same_value(st.a_prev, end_prev)) return 2;
+            double a = st.a_prev;
             for (int k = 0; k < items; ++k) {
+                a = a + X(i0 + k);
                 const size_t i = i0 + k;
-                if (i >= n) break;
-                if (got[k] != ref[i] && !(std::isnan(got[k]) && std::isnan(ref[i]))) {
-                    std::fprintf(stderr, "mismatch at %zu: %.17g vs %.17g\n", i, got[k], ref[i]);
+                if (i < n && !same_value(a, ref[i])) {
+                    std::fprintf(stderr, "mismatch at %zu: %.17g vs %.17g\n", i, a, ref[i]);
                     return 3;
                 }
             }
+            end_prev = a;
             tp = seg_op(tp, gthr[id]);
         }
     }
+    if (!same_value(end_prev, tot)) return 2;
     if (tot != ref[n - 1] && !(std::isnan(tot) && std::isnan(ref[n - 1]))) return 3;
     *nspec_out = (long)nspec;
     return 0;

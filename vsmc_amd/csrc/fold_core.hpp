@@ -214,6 +214,9 @@ VSMCX_HD bool tr_apply(const Tr &t, double v, double *out)
     return true;
 }
 
+// Equal values, NaN matching NaN (a fold that met a NaN stays NaN).
+VSMCX_HD bool same_value(double a, double b) { return a == b || (a != a && b != b); }
+
 // Lower end of the error interval around an approximate prefix S.
 VSMCX_HD double err_lo(double S, double eps_rel, double eps_abs)
 {
@@ -352,14 +355,11 @@ VSMCX_HD Seg run_seg(X x, int n, double s, double er, double ea, int *fast_b)
     return run_seg_elems(x, n, s, er, ea);
 }
 
-// Exact fold values of the same run: pre is the summary of everything before
-// it, vb(c) the exact value right after special c (fold order).  out(k, a_k)
-// receives each value.  Returns false if a transducer does not apply or an
-// element taken as negligible would change the value (a broken bound).
-//
-// In parts: the exact value before the run (run_start), then the fast path's
-// integer walk in one binade (run_values_units) or the element-by-element path
-// (run_values_elems).
+// The exact value before a run: pre is the summary of everything before it,
+// vb(c) the exact value right after special c (fold order).  From there the
+// run's own values are vSMC's fold steps a = fl(a + x), exact by construction;
+// a run that ends where the next run's start says it should confirms that
+// start (and a chain of such checks confirms every value).
 struct RunStart {
     uint64_t vcnt;  // specials before the run
     double vbase;   // exact value right after the last of them (0 if none)
@@ -379,62 +379,6 @@ VSMCX_HD RunStart run_start(const Seg &pre, VB vb)
         r.a_prev = r.vbase;
     }
     return r;
-}
-
-template <typename X, typename OUT>
-VSMCX_HD bool run_values_units(X x, int n, int fast_b, double a_prev, OUT out)
-{
-    bool ok = binade_of(a_prev) == fast_b;
-    uint64_t A = units_of(a_prev);
-    for (int k = 0; k < n; ++k) {
-        int kind;
-        A += grid_units(x(k), fast_b, &kind) + (kind == 1);
-        if (A >= kTwo53) ok = false;
-        out(k, value_of(fast_b, A & (kTwo53 - 1)));
-    }
-    return ok;
-}
-
-template <typename X, typename VB, typename OUT>
-VSMCX_HD bool run_values_elems(X x, int n, double s, double er, double ea, Seg pre,
-                               const RunStart &st, VB vb, OUT out)
-{
-    bool ok = true;
-    uint64_t vcnt = st.vcnt;
-    double vbase = st.vbase, a_prev = st.a_prev;
-#if defined(__HIP_DEVICE_COMPILE__)
-#pragma unroll 1
-#endif
-    for (int k = 0; k < n; ++k) {
-        const double xk = x(k);
-        const double sb = s;
-        s = sb + xk;
-        const Seg el = elem_seg(xk, sb, s, er, ea);
-        pre = seg_op(pre, el);
-        if (pre.cnt != vcnt) {
-            vcnt = pre.cnt;
-            vbase = vb(vcnt - 1);
-        }
-        double a;
-        if (!tr_apply(pre.tr, vbase, &a)) {
-            ok = false;
-            a = vbase;
-        }
-        if (el.cnt == 0 && el.tr.b == kNoBinade && xk != 0.0 && a_prev + xk != a_prev) ok = false;
-        out(k, a);
-        a_prev = a;
-    }
-    return ok;
-}
-
-template <typename X, typename VB, typename OUT>
-VSMCX_HD bool run_values(X x, int n, double s, double er, double ea, int fast_b, Seg pre, VB vb,
-                         OUT out)
-{
-    const RunStart st = run_start(pre, vb);
-    const bool ok = fast_b != kNoBinade ? run_values_units(x, n, fast_b, st.a_prev, out)
-                                        : run_values_elems(x, n, s, er, ea, pre, st, vb, out);
-    return st.ok && ok;
 }
 
 // One special element in fold order: the run transducer since the previous

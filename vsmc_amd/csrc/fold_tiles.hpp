@@ -589,6 +589,7 @@ struct FoldLdsT {
     Seg seg[kBlock / kWave];
     uint64_t u64[kBlock / kWave];
     uint64_t bu[2];
+    double wend[kBlock / kWave];  // exact value at the end of each wave's runs
 };
 
 // ---- classification, tile segment summary, special records ----------------------
@@ -712,11 +713,19 @@ __device__ double fold_chain(const Seg *pre, const uint32_t *cnt_lds, uint64_t T
 // special.  Then c_i = F(a_i) (uniforms at or below a_i) and the replication
 // counts as differences; per tile the number of zero slots and of surplus
 // copies for the layout pass.
+//
+// Only the exact value before each thread's run comes from the segment
+// machinery (the run's prefix transducer applied to the value after the last
+// special); the run's own values are vSMC's fold steps a = fl(a + x) from that
+// value, exact by construction.  Every run must then end where the next one
+// starts (the tile's last run: at the next tile's start, tpre_next, or at the
+// fold total), which checks every start and so every value: a broken
+// classification bound is reported, never returned.
 template <typename XSrc, typename Inv, typename Lds>
 __device__ void fold_tile_apply(FilterCtl *ctl, const XSrc &xs, const Inv &inv, uint64_t t,
-                                uint64_t M, double ts, const Seg tpre, const double *gval,
-                                const FoldOut &out, uint32_t *rep32, uint64_t *zs_out, Lds &L,
-                                const Seg *thr_in = nullptr, const int32_t *fb_in = nullptr)
+                                uint64_t M, double ts, const Seg tpre, const Seg *tpre_next,
+                                const double *gval, const FoldOut &out, uint32_t *rep32,
+                                uint64_t *zs_out, Lds &L, const Seg *thr_in = nullptr)
 {
     // x, overwritten in place (each thread its own elements) by the exact
     // values; later the uniform window and the staged replication counts
@@ -742,45 +751,62 @@ __device__ void fold_tile_apply(FilterCtl *ctl, const XSrc &xs, const Inv &inv, 
     constexpr bool kReg = Inv::kInv && !(Inv::kWindow && !Inv::kWindowTmp);
     double xv[kFI];
     if constexpr (kReg) thread_run(s_x, e0, xv);
-    const double tsum = kReg ? thread_sum(xv) : thread_sum_lds(s_x, e0);
-    const double s_thr =
-        ts + block_excl_scan<kBlock>(tsum, AddOp(), 0.0, s_red, (double *)nullptr);
-    double er, ea;
-    fold_eps(M, &er, &ea);
-    const auto xat = [&](int k) { return s_x[fidx(e0 + k)]; };
-    int fast_b;
     Seg thr_pre;
     if (thr_in) {  // the segment pass's own classification of this thread's run
         thr_pre = thr_in[tid];
-        fast_b = fb_in[tid];
     } else {
+        const double tsum = kReg ? thread_sum(xv) : thread_sum_lds(s_x, e0);
+        const double s_thr =
+            ts + block_excl_scan<kBlock>(tsum, AddOp(), 0.0, s_red, (double *)nullptr);
+        double er, ea;
+        fold_eps(M, &er, &ea);
+        int fast_b;
         Seg g;
         if constexpr (kReg)
             g = thread_run_seg(xv, s_x, e0, s_thr, er, ea, &fast_b);
         else
-            g = run_seg(xat, kFI, s_thr, er, ea, &fast_b);
+            g = run_seg([&](int k) { return s_x[fidx(e0 + k)]; }, kFI, s_thr, er, ea, &fast_b);
         thr_pre = thread_seg_prefix(g, fast_b, L);
     }
     const auto vb = [=](uint64_t c) { return gval[c]; };
-    const auto put = [&](int k, double a) { s_a[fidx(e0 + k)] = a; };
-    int bad;
+    const RunStart rs = run_start(seg_op(tpre, thr_pre), vb);
+    int bad = !rs.ok;
+    double a = rs.a_prev;
     if constexpr (kReg) {
-        const Seg run_pre = seg_op(tpre, thr_pre);
-        const RunStart rs = run_start(run_pre, vb);
-        bad = !rs.ok;
-        if (fast_b != kNoBinade) {
-            if (!run_values_units([&](int k) { return xv[k]; }, kFI, fast_b, rs.a_prev, put))
-                bad = 1;
-        } else if (!run_values_elems(xat, kFI, s_thr, er, ea, run_pre, rs, vb, put)) {
-            bad = 1;
+#pragma unroll
+        for (int k = 0; k < kFI; ++k) {
+            a = a + xv[k];
+            s_a[fidx(e0 + k)] = a;
         }
     } else {
-        bad = !run_values(xat, kFI, s_thr, er, ea, fast_b, seg_op(tpre, thr_pre), vb, put);
+#pragma unroll 1
+        for (int k = 0; k < kFI; ++k) {
+            const int e = fidx(e0 + k);
+            a = a + s_x[e];
+            s_a[e] = a;
+        }
     }
-    double v_start = 0.0;  // exact a_{base-1}
-    if (t > 0) {
-        const double vb = tpre.cnt ? gval[tpre.cnt - 1] : 0.0;
-        if (!tr_apply(tpre.tr, vb, &v_start)) bad = 1;
+    // exact a_{base-1}: thread 0's start, which only wave 0 reads below
+    const double v_start = __shfl(rs.a_prev, 0);
+    {
+        // run boundaries: the previous lane's end, the previous wave's end,
+        // and after the last run the next tile's start or the fold total
+        const int lane = tid & (kWave - 1), wid = tid / kWave;
+        const double up = __shfl_up(a, 1);
+        if (lane == kWave - 1) L.wend[wid] = a;
+        __syncthreads();
+        double want = up;
+        if (lane == 0) want = wid > 0 ? L.wend[wid - 1] : rs.a_prev;
+        bool ok = same_value(rs.a_prev, want);
+        if (tid == kBlock - 1) {
+            double nx = ctl->fold_total;
+            if (tpre_next) {
+                const Seg q = *tpre_next;
+                if (!tr_apply(q.tr, q.cnt ? gval[q.cnt - 1] : 0.0, &nx)) ok = false;
+            }
+            ok = ok && same_value(a, nx);
+        }
+        if (!ok) bad = 1;
     }
     if (bad) atomicOr(&ctl->err, (int)kErrFoldClass);
     __syncthreads();
@@ -831,7 +857,7 @@ __device__ void fold_tile_apply(FilterCtl *ctl, const XSrc &xs, const Inv &inv, 
                 const uint64_t i = base + e0 + k;
                 c[k] = (M >= 2 && i <= last_cmp) ? (uint32_t)inv.F(av[k]) : 0u;
             }
-            if (t > 0) cstart = inv.F(v_start);
+            if (t > 0 && tid == 0) cstart = inv.F(v_start);
         } else {
             // the tile holds uniforms [F(a_{base-1}), F(a_{last compared}))
             if (tid < kWave) {

@@ -142,8 +142,10 @@ __global__ __launch_bounds__(kBlock) void k_tick_apply(const TickItem *items,
     inv.bind(ctl);
     const uint64_t o = F.off / kFT + it.tile;
     const FoldOut fo{P.anc + F.off, P.zl + F.off, P.sl + F.off, nullptr, nullptr};
-    fold_tile_apply(ctl, xs, inv, it.tile, F.n, P.ts[o], P.tpre[o], P.gval + F.off, fo,
-                    P.rep32 + F.off, P.tzs + o, L, P.thr + o * kBlock, P.thr_fb + o * kBlock);
+    const bool has_next = it.tile + 1 < tiles_of_job(F);
+    fold_tile_apply(ctl, xs, inv, it.tile, F.n, P.ts[o], P.tpre[o],
+                    has_next ? P.tpre + o + 1 : nullptr, P.gval + F.off, fo, P.rep32 + F.off,
+                    P.tzs + o, L, P.thr + o * kBlock);
 }
 
 __global__ __launch_bounds__(kJobBlock) void k_tick_scan_cnt(const uint32_t *jobs,
