@@ -152,6 +152,48 @@ static int check_case(const std::vector<double> &x, int items, int tile, double 
     return 0;
 }
 
+// The fast path's units from its own fold steps started at 2^b, against the
+// units counted element by element on the grid (q + [fraction > 1/2], ties
+// refused), over runs that sit anywhere in their binade, with exact halves,
+// zeros, values below half a grid unit and binades down to the subnormals.
+static int check_fast_units(std::mt19937_64 &g)
+{
+    int bad = 0;
+    std::uniform_real_distribution<double> U(0, 1);
+    for (int rep = 0; rep < 400000; ++rep) {
+        const int b = (rep % 7 == 0) ? -1000 - (int)(g() % 23) : (int)(g() % 80) - 60;
+        const double lo = std::ldexp(1.0, b);
+        // a start inside the binade and eight increments that keep the run in it
+        const double s = lo * (1.0 + 0.5 * U(g));
+        const double room = std::ldexp(1.0, b + 1) - s;
+        double x[8];
+        for (int k = 0; k < 8; ++k) {
+            const int kind = (int)(g() % 6);
+            const double grid = std::ldexp(1.0, b - 52);
+            if (kind == 0) x[k] = 0.0;
+            else if (kind == 1) x[k] = grid * (0.5 + (double)(g() % 64));  // exact tie
+            else if (kind == 2) x[k] = grid * 0.25 * U(g);                 // rounds away
+            else x[k] = room / 9 * U(g);
+        }
+        int fb = kNoBinade;
+        Seg r;
+        const bool fast = run_seg_fast([&](int k) { return x[k]; }, 8, s, 0.0, 0.0, &fb, &r);
+        uint64_t d = 0;
+        int tie = 0;
+        for (int k = 0; k < 8; ++k) {
+            int kind;
+            d += grid_units(x[k], binade_of(s), &kind) + (kind == 1);
+            tie |= kind == 2;
+        }
+        double se = s;
+        for (int k = 0; k < 8; ++k) se += x[k];
+        const bool one_binade = binade_of(s) == binade_of(se) && binade_of(s) < 1024;
+        if (fast != (one_binade && !tie)) ++bad;
+        if (fast && (fb != binade_of(s) || r.tr.d0 != d || r.tr.d1 != d)) ++bad;
+    }
+    return bad;
+}
+
 int main()
 {
     std::mt19937_64 g(12345);
@@ -207,6 +249,9 @@ int main()
         if (check_case(x, 2, 4, 0.0, g, &ns)) ++failures;
         ++cases;
     }
+    const int unit_bad = check_fast_units(g);
+    if (unit_bad) std::fprintf(stderr, "fast-path units differ in %d runs\n", unit_bad);
+    failures += unit_bad;
     std::printf("cases %d failures %d specials total %ld max %ld\n", cases, failures, total_spec,
                 max_spec);
     return failures ? 1 : 0;

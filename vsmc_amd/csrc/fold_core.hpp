@@ -307,6 +307,14 @@ VSMCX_HD Seg elem_seg(double x, double sb, double sa, double eps_rel, double eps
 // The two paths are also callable apart (run_seg_fast, run_seg_elems), so that
 // device code can run the fast path over values held in registers (its loops
 // unroll) and the rare element-by-element path over values in LDS.
+//
+// The run's units d come from its own fold steps started at 2^b (b above the
+// subnormal binade): without ties each step adds q_k + [fraction > 1/2] grid
+// units whatever the running value in the binade, the run's increments total
+// less than 2^b (the exact run lies in [2^b, 2^(b+1))), so from 2^b the
+// steps stay in the binade and d is the significand field of the end value.
+// A step is a tie when its rounding error, x_k - (S_k - S_{k-1}) (exact: the
+// fast two-sum with S_{k-1} >= 2^b > x_k), is half a grid unit.
 template <typename X>
 VSMCX_HD bool run_seg_fast(X x, int n, double s, double er, double ea, int *fast_b, Seg *r)
 {
@@ -316,10 +324,22 @@ VSMCX_HD bool run_seg_fast(X x, int n, double s, double er, double ea, int *fast
     if (!(b0 < 1024 && b0 == certain_binade(se, er, ea))) return false;
     uint64_t d = 0;
     int tie = 0;
-    for (int k = 0; k < n; ++k) {
-        int kind;
-        d += grid_units(x(k), b0, &kind) + (kind == 1);
-        tie |= kind == 2;
+    if (b0 > -1022) {
+        const double half = ldexp(1.0, b0 - 53);
+        double S = value_of(b0, 1ull << 52);  // 2^b0
+        for (int k = 0; k < n; ++k) {
+            const double xk = x(k);
+            const double Sn = S + xk;
+            tie |= fabs(xk - (Sn - S)) == half;
+            S = Sn;
+        }
+        d = dbits(S) & ((1ull << 52) - 1);
+    } else {
+        for (int k = 0; k < n; ++k) {
+            int kind;
+            d += grid_units(x(k), b0, &kind) + (kind == 1);
+            tie |= kind == 2;
+        }
     }
     if (tie) return false;
     *fast_b = b0;
