@@ -89,7 +89,23 @@ VSMCX_HD u32x4 philox4x32_10(u32x4 c, uint32_t k0, uint32_t k1)
     return o;
 }
 
-VSMCX_HD uint64_t rotl64(uint64_t x, unsigned r) { return (x << r) | (x >> (64 - r)); }
+// Rotations by a constant in 1..63.  On the device the two 32-bit halves
+// come from one funnel shift each (v_alignbit_b32), with the halves swapped
+// for r >= 32.
+VSMCX_HD uint64_t rotl64(uint64_t x, unsigned r)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    const uint32_t lo = (uint32_t)x, hi = (uint32_t)(x >> 32);
+    const uint32_t a = r < 32 ? hi : lo, b = r < 32 ? lo : hi;  // {a, b} = x rotated by 0 or 32
+    const unsigned s = r & 31u;
+    if (s == 0) return ((uint64_t)a << 32) | b;
+    const uint32_t nh = __builtin_amdgcn_alignbit(a, b, 32u - s);
+    const uint32_t nl = __builtin_amdgcn_alignbit(b, a, 32u - s);
+    return ((uint64_t)nh << 32) | nl;
+#else
+    return (x << r) | (x >> (64 - r));
+#endif
+}
 VSMCX_HD uint32_t rotl32(uint32_t x, unsigned r) { return (x << r) | (x >> (32 - r)); }
 
 VSMCX_HD u64x4 threefry4x64_20(u64x4 c, const uint64_t key[4])
