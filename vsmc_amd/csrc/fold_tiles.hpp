@@ -718,13 +718,14 @@ __device__ double fold_chain(const Seg *pre, const uint32_t *cnt_lds, uint64_t T
 // machinery (the run's prefix transducer applied to the value after the last
 // special); the run's own values are vSMC's fold steps a = fl(a + x) from that
 // value, exact by construction.  Every run must then end where the next one
-// starts (the tile's last run: at the next tile's start, tpre_next, or at the
-// fold total), which checks every start and so every value: a broken
+// starts (the tile's last run: at the next tile's start, from tpre_next when
+// has_next, or at the fold total), which checks every start and so every value: a broken
 // classification bound is reported, never returned.
 template <typename XSrc, typename Inv, typename Lds>
 __device__ void fold_tile_apply(FilterCtl *ctl, const XSrc &xs, const Inv &inv, uint64_t t,
-                                uint64_t M, double ts, const Seg tpre, const Seg *tpre_next,
-                                const double *gval, const FoldOut &out, uint32_t *rep32,
+                                uint64_t M, double ts, const Seg tpre, const Seg tpre_next,
+                                bool has_next, const double *gval, const FoldOut &out,
+                                uint32_t *rep32,
                                 uint64_t *zs_out, Lds &L, const Seg *thr_in = nullptr)
 {
     // x, overwritten in place (each thread its own elements) by the exact
@@ -800,10 +801,9 @@ __device__ void fold_tile_apply(FilterCtl *ctl, const XSrc &xs, const Inv &inv, 
         bool ok = same_value(rs.a_prev, want);
         if (tid == kBlock - 1) {
             double nx = ctl->fold_total;
-            if (tpre_next) {
-                const Seg q = *tpre_next;
-                if (!tr_apply(q.tr, q.cnt ? gval[q.cnt - 1] : 0.0, &nx)) ok = false;
-            }
+            if (has_next &&
+                !tr_apply(tpre_next.tr, tpre_next.cnt ? gval[tpre_next.cnt - 1] : 0.0, &nx))
+                ok = false;
             ok = ok && same_value(a, nx);
         }
         if (!ok) bad = 1;

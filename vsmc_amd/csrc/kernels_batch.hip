@@ -229,7 +229,8 @@ __device__ __noinline__ void batch_fold_apply(const BatchFilter &F, FilterCtl *c
 #pragma unroll 1
     for (uint64_t tt = 0; tt < T; ++tt)
         fold_tile_apply(ctl, xs, inv, tt, N, P.ts[F.off / kFT + tt], P.tpre[F.off / kFT + tt],
-                        tt + 1 < T ? P.tpre + F.off / kFT + tt + 1 : nullptr, P.gval + F.off, fo,
+                        tt + 1 < T ? P.tpre[F.off / kFT + tt + 1] : seg_identity(), tt + 1 < T,
+                        P.gval + F.off, fo,
                         P.rep32 + F.off, P.tzs + F.off / kFT + tt, S.L);
     __syncthreads();
     uint64_t *zs = P.tzs + F.off / kFT;

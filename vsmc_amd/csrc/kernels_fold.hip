@@ -206,11 +206,11 @@ __global__ __launch_bounds__(kBlock) void k_fold_apply(FilterCtl *ctl, XSrc xs, 
     inv.bind(ctl);
     const uint64_t t = blockIdx.x;
     const Seg tpre = seg_op(fs.cpre[t / kFoldChunk], fs.pre[t]);
-    Seg next;
     const bool has_next = t + 1 < (M + kFT - 1) / kFT;
-    if (has_next) next = seg_op(fs.cpre[(t + 1) / kFoldChunk], fs.pre[t + 1]);
-    fold_tile_apply(ctl, xs, inv, t, M, fs.ts[t], tpre, has_next ? &next : nullptr, fs.gval, out,
-                    fs.rep32, fs.zs + t, L, fs.thr ? fs.thr + t * kBlock : nullptr);
+    const Seg next =
+        has_next ? seg_op(fs.cpre[(t + 1) / kFoldChunk], fs.pre[t + 1]) : seg_identity();
+    fold_tile_apply(ctl, xs, inv, t, M, fs.ts[t], tpre, next, has_next, fs.gval, out, fs.rep32,
+                    fs.zs + t, L, fs.thr ? fs.thr + t * kBlock : nullptr);
 }
 
 // ---- pass 6: zero / surplus ranks of the tiles (one workgroup) ---------------------

@@ -144,7 +144,8 @@ __global__ __launch_bounds__(kBlock) void k_tick_apply(const TickItem *items,
     const FoldOut fo{P.anc + F.off, P.zl + F.off, P.sl + F.off, nullptr, nullptr};
     const bool has_next = it.tile + 1 < tiles_of_job(F);
     fold_tile_apply(ctl, xs, inv, it.tile, F.n, P.ts[o], P.tpre[o],
-                    has_next ? P.tpre + o + 1 : nullptr, P.gval + F.off, fo, P.rep32 + F.off,
+                    has_next ? P.tpre[o + 1] : seg_identity(), has_next, P.gval + F.off, fo,
+                    P.rep32 + F.off,
                     P.tzs + o, L, P.thr + o * kBlock);
 }
 
