@@ -321,6 +321,66 @@ __device__ __forceinline__ void tile_partials_lse_block(const double *partials, 
     __syncthreads();  // L may be reused
 }
 
+// tile_partials_lse_block when every lane holds at most kStashPer tiles (up
+// to kStashTiles tiles, a filter of up to 2^24 particles): each lane loads all
+// of its partials in one round, takes the same maxima and the same additions
+// in the same order (the same bits), and leaves each tile's scaled sum
+// se_t e^(M_t - M) in stash[t], the product the fold's tile starts multiply
+// by 1 / sum (TileSumsFromLse).  blockDim.x >= 64 * part_vwaves(ntiles).
+constexpr int kStashPer = 8;
+constexpr int kStashTiles = kStashPer * kWave * kPartMaxWaves;
+__device__ __forceinline__ void tile_partials_lse_block_stash(const double *partials,
+                                                              uint64_t ntiles, double &M,
+                                                              Part5 &acc, PartLds &L,
+                                                              double *stash)
+{
+    const int nv = part_vwaves(ntiles);
+    const int pw = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
+    const uint64_t S = (uint64_t)kWave * nv, j0 = (uint64_t)pw * kWave + lane;
+    const bool act = pw < nv;
+    double q[kStashPer][5];
+    if (act) {
+#pragma unroll
+        for (int u = 0; u < kStashPer; ++u) {
+            const uint64_t j = j0 + (uint64_t)u * S;
+            const uint64_t jj = j < ntiles ? j : ntiles - 1;  // in bounds, no branch
+#pragma unroll
+            for (int c = 0; c < 5; ++c) q[u][c] = partials[5 * jj + c];
+        }
+        double m = -INFINITY;
+#pragma unroll
+        for (int u = 0; u < kStashPer; ++u)
+            if (j0 + (uint64_t)u * S < ntiles) m = m < q[u][0] ? q[u][0] : m;
+        m = wave_reduce(m, MaxOp());
+        if (lane == 0) L.m[pw] = m;
+    }
+    __syncthreads();
+    M = L.m[0];
+    for (int w = 1; w < nv; ++w) M = M < L.m[w] ? L.m[w] : M;
+    if (act) {
+        Part5 a{0, 0, 0, 0};
+#pragma unroll
+        for (int u = 0; u < kStashPer; ++u) {
+            const uint64_t j = j0 + (uint64_t)u * S;
+            if (j < ntiles) {
+                const double sc = q[u][0] == -INFINITY ? 0.0 : exp_lse(q[u][0] - M);
+                const double s1 = q[u][1] * sc;
+                a.se += s1;
+                a.se2 += q[u][2] * sc * sc;
+                a.mx += q[u][3];
+                a.my += q[u][4];
+                stash[j] = s1;
+            }
+        }
+        a = wave_reduce(a, Part5Add());
+        if (lane == 0) L.p[pw] = a;
+    }
+    __syncthreads();
+    acc = L.p[0];
+    for (int w = 1; w < nv; ++w) acc = Part5Add()(acc, L.p[w]);
+    __syncthreads();  // L may be reused
+}
+
 __device__ __forceinline__ void tile_partials_monitor_block(const double *partials,
                                                             uint64_t ntiles, double &mx,
                                                             double &my, PartLds &L)

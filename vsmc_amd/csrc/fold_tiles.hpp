@@ -555,6 +555,14 @@ struct TileSumsFromLse {
 // Tile sums of the scaled residual fractions r_i * rc from the tile sums of
 // r_i: each differs from a sum of the fold's own products by the rounding of
 // the products and of one more multiply, a few u relative, inside fold_eps.
+// The same tile sums from the products se_t e^(M_t - m) that the reduction
+// left in LDS (tile_partials_lse_block_stash): the same bits.
+struct TileSumsStashed {
+    const double *stash;
+    double coeff;
+    __device__ double operator[](uint64_t t) const { return stash[t] * coeff; }
+};
+
 struct TileSumsScaled {
     const double *tsum;
     double rc;
@@ -647,6 +655,28 @@ __device__ void fold_tile_seg(const XSrc &xs, uint64_t base, uint64_t M, double 
 }
 
 // ---- the chain over the specials in fold order -------------------------------------
+// Thread 0's part: n records from LDS in fold order, one transducer
+// application and one real FP64 add each; the next record is read while the
+// current one is applied.
+__device__ __forceinline__ void chain_run(const SpecRec *s_r, int n, double &a, double *s_v,
+                                          int *bad)
+{
+    if (n <= 0) return;
+    SpecRec cur = s_r[0];
+#pragma unroll 1
+    for (int k = 0; k < n; ++k) {
+        const SpecRec nxt = s_r[k + 1 < n ? k + 1 : k];
+        double v;
+        if (!tr_apply(cur.run, a, &v)) {
+            *bad = 1;
+            v = a;
+        }
+        a = v + cur.x;
+        s_v[k] = a;
+        cur = nxt;
+    }
+}
+
 // T tiles with exclusive segment summaries pre[] (their special counts also
 // in cnt_lds when that is not null), records at spec[tile * kFT + k].  BLOCK
 // threads take a chunk of specials each round: each finds its record (binary
@@ -683,19 +713,8 @@ __device__ double fold_chain(const Seg *pre, const uint32_t *cnt_lds, uint64_t T
             s_r[tid] = r;
         }
         __syncthreads();
-        if (tid == 0) {
-            const int n = (int)(S - c0 < (uint64_t)BLOCK ? S - c0 : (uint64_t)BLOCK);
-#pragma unroll 1
-            for (int k = 0; k < n; ++k) {
-                double v;
-                if (!tr_apply(s_r[k].run, a, &v)) {
-                    *bad = 1;
-                    v = a;
-                }
-                a = v + s_r[k].x;
-                s_v[k] = a;
-            }
-        }
+        if (tid == 0)
+            chain_run(s_r, (int)(S - c0 < (uint64_t)BLOCK ? S - c0 : (uint64_t)BLOCK), a, s_v, bad);
         __syncthreads();
         if (g < S) gval[g] = s_v[tid];
         __syncthreads();

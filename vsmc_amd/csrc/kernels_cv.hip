@@ -42,9 +42,14 @@ __global__ __launch_bounds__(kScanBlock) void k_cv_reduce(FilterCtl *ctl, const 
     __shared__ double s_red[kScanBlock / kWave];
     __shared__ double s_mc[2];
     __shared__ int s_res;
+    __shared__ double s_stash[kStashTiles];  // se_t e^(M_t - M) per tile
+    const bool stash = ntiles <= (uint64_t)kStashTiles;
     double M;
     Part5 acc;
-    tile_partials_lse_block(partials, ntiles, M, acc, L);
+    if (stash)
+        tile_partials_lse_block_stash(partials, ntiles, M, acc, L, s_stash);
+    else
+        tile_partials_lse_block(partials, ntiles, M, acc, L);
     if (threadIdx.x == 0) {
         cv_step_scalars(ctl, M, acc.se, acc.se2, acc.mx, acc.my, hist, t);
         const int res = ctl->resample;
@@ -56,8 +61,13 @@ __global__ __launch_bounds__(kScanBlock) void k_cv_reduce(FilterCtl *ctl, const 
     if (prologue_scheme < 0 || !fs.ts) return;
     __syncthreads();
     if (!s_res) return;
-    const TileSumsFromLse in{partials, s_mc[0], s_mc[1]};
-    wg_scan<double>(in, fs.ts, fs.tiles, AddOp(), 0.0, s_red);
+    if (stash) {
+        const TileSumsStashed in{s_stash, s_mc[1]};
+        wg_scan<double>(in, fs.ts, fs.tiles, AddOp(), 0.0, s_red);
+    } else {
+        const TileSumsFromLse in{partials, s_mc[0], s_mc[1]};
+        wg_scan<double>(in, fs.ts, fs.tiles, AddOp(), 0.0, s_red);
+    }
 }
 
 // Monitor of the last step: sum_i W_i (x_i, y_i) over the post-resample state.

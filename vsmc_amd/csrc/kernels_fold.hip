@@ -137,19 +137,8 @@ __device__ double fold_chain_chunked(const FoldScratch &fs, const uint32_t *ccnt
             s_r[tid] = rec;
         }
         __syncthreads();
-        if (tid == 0) {
-            const int n = (int)(S - c0 < (uint64_t)BLOCK ? S - c0 : (uint64_t)BLOCK);
-#pragma unroll 1
-            for (int k = 0; k < n; ++k) {
-                double v;
-                if (!tr_apply(s_r[k].run, a, &v)) {
-                    *bad = 1;
-                    v = a;
-                }
-                a = v + s_r[k].x;
-                s_v[k] = a;
-            }
-        }
+        if (tid == 0)
+            chain_run(s_r, (int)(S - c0 < (uint64_t)BLOCK ? S - c0 : (uint64_t)BLOCK), a, s_v, bad);
         __syncthreads();
         if (g < S) fs.gval[g] = s_v[tid];
         __syncthreads();
@@ -172,10 +161,22 @@ __global__ __launch_bounds__(kScanBlock) void k_fold_scan_seg(FilterCtl *ctl, Fo
     const uint64_t T = fs.tiles;
     const uint64_t C = (T + kFoldChunk - 1) / kFoldChunk;
     const bool in_lds = C <= (uint64_t)kLdsChunks;
-    const Seg tail = wg_scan<Seg>(fs.cagg, fs.cpre, C, SegOp(), seg_identity(), s_seg,
-                                  [&](uint64_t c, const Seg &p) {
-                                      if (in_lds) s_cnt[c] = (uint32_t)p.cnt;
-                                  });
+    Seg tail;
+    if (C <= (uint64_t)kScanBlock) {
+        // one chunk per thread: a single workgroup scan, no serial run per thread
+        const uint64_t c = threadIdx.x;
+        const Seg v = c < C ? fs.cagg[c] : seg_identity();
+        const Seg ex = block_excl_scan<kScanBlock>(v, SegOp(), seg_identity(), s_seg, &tail);
+        if (c < C) {
+            fs.cpre[c] = ex;
+            if (in_lds) s_cnt[c] = (uint32_t)ex.cnt;
+        }
+    } else {
+        tail = wg_scan<Seg>(fs.cagg, fs.cpre, C, SegOp(), seg_identity(), s_seg,
+                            [&](uint64_t c, const Seg &p) {
+                                if (in_lds) s_cnt[c] = (uint32_t)p.cnt;
+                            });
+    }
     __syncthreads();  // cpre[] and s_cnt written by other threads
     int bad;
     const double tot =
