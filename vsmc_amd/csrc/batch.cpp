@@ -56,15 +56,17 @@ struct vsmcx_batch {
     // before it reads.
     std::vector<std::pair<void *, size_t>> mutable_arrays() const
     {
-        return {{P.st0, total * state_bytes()},
-                {P.st1, total * state_bytes()},
-                {P.lw, 8 * total},
-                {P.anc, 4 * total},
-                {P.ctl, sizeof(FilterCtl) * nf},
-                {P.next_t, 8 * (size_t)nf},
-                {P.cur, 4 * (size_t)nf},
-                {P.err_any, sizeof(int)},
-                {P.hist, sizeof(StepRow) * total_obs}};
+        std::vector<std::pair<void *, size_t>> a = {{P.st0, total * state_bytes()},
+                                                    {P.st1, total * state_bytes()},
+                                                    {P.lw, 8 * total},
+                                                    {P.anc, 4 * total},
+                                                    {P.ctl, sizeof(FilterCtl) * nf},
+                                                    {P.next_t, 8 * (size_t)nf},
+                                                    {P.cur, 4 * (size_t)nf},
+                                                    {P.err_any, sizeof(int)},
+                                                    {P.hist, sizeof(StepRow) * total_obs}};
+        if (P.lwc) a.push_back({P.lwc, 4 * total});
+        return a;
     }
     void copy_arrays(const std::vector<void *> &saved, bool restore)
     {
@@ -78,7 +80,7 @@ struct vsmcx_batch {
 
     void release_all()
     {
-        void *ps[] = {df, order_work, order_given, P.st0, P.st1, P.lw, P.anc, P.zl, P.sl,
+        void *ps[] = {df, order_work, order_given, P.st0, P.st1, P.lw, P.lwc, P.anc, P.zl, P.sl,
                       P.rep32, P.spec, P.gval, P.ts, P.tagg, P.tpre, P.tzs, P.tsum, P.tzsx,
                       P.tpart, P.wx, P.thr, P.thr_fb, P.ctl, P.hist, (void *)P.obs, P.next_t,
                       (void *)P.avail, P.cur, P.err_any, queue, jobs_dev, items_dev};
@@ -210,6 +212,7 @@ int vsmcx_batch_create(vsmcx_ctx *ctx, const vsmcx_batch_desc *d, vsmcx_batch **
         P.st0 = ctx->alloc(off * b->state_bytes());
         P.st1 = ctx->alloc(off * b->state_bytes());
         P.lw = ctx->alloc_n<double>(off);
+        if (!b->fp64) P.lwc = ctx->alloc_n<float>(off);
         P.anc = ctx->alloc_n<uint32_t>(off);
         P.zl = ctx->alloc_n<uint32_t>(off);
         P.sl = ctx->alloc_n<uint32_t>(off);
@@ -247,6 +250,7 @@ int vsmcx_batch_create(vsmcx_ctx *ctx, const vsmcx_batch_desc *d, vsmcx_batch **
             h.scheme = d->scheme;
             h.eng.key[0] = d->resample_seed[f];
             h.eng.index = 4;
+            h.lwc = P.lwc ? P.lwc + b->hf[f].off : nullptr;
             hc[f] = h;
         }
         std::vector<uint32_t> given(b->nf), work(b->nf);
@@ -489,7 +493,8 @@ int vsmcx_batch_state(vsmcx_batch *b, uint64_t f, double *state, double *log_wei
         }
     }
     if (log_weight) {
-        VSMCX_HIP(hipMemcpyAsync(log_weight, b->P.lw + F.off, 8 * N, hipMemcpyDeviceToHost, s));
+        launch_lw_expand(b->P.ctl + f, b->P.lw + F.off, N, b->P.wx + F.off, s);
+        VSMCX_HIP(hipMemcpyAsync(log_weight, b->P.wx + F.off, 8 * N, hipMemcpyDeviceToHost, s));
         b->ctx->sync();
         for (uint64_t i = 0; i < N; ++i)
             log_weight[i] = h.prev_resampled ? 0.0 : log_weight[i] - h.m;

@@ -188,6 +188,7 @@ struct vsmcx_filter {
     void *st[2] = {nullptr, nullptr};
     int cur = 0;
     double *lw = nullptr;
+    float *lwc = nullptr;  // compact log-weights (mixed pipeline)
     double *partials = nullptr;
     double *obs = nullptr;
     FilterCtl *ctl = nullptr;
@@ -459,6 +460,7 @@ int vsmcx_filter_create(vsmcx_ctx *ctx, const vsmcx_filter_desc *desc, vsmcx_fil
             launch_plain_init_weights(f->w, f->N, ctx->stream);
         }
         f->lw = ctx->alloc_n<double>(f->N);
+        if (!f->fp64 && !f->plain) f->lwc = ctx->alloc_n<float>(f->N);
         f->partials = ctx->alloc_n<double>(5 * f->tiles);
         f->obs = ctx->alloc_n<double>(2 * f->T);
         f->ctl = ctx->alloc_n<FilterCtl>(1);
@@ -476,7 +478,7 @@ int vsmcx_filter_create(vsmcx_ctx *ctx, const vsmcx_filter_desc *desc, vsmcx_fil
             VSMCX_HIP(hipEventCreateWithFlags(&sd.starts, hipEventDisableTiming));
         }
     } catch (...) {
-        for (void *p : {f->st[0], f->st[1], (void *)f->lw, (void *)f->partials, (void *)f->obs,
+        for (void *p : {f->st[0], f->st[1], (void *)f->lw, (void *)f->lwc, (void *)f->partials, (void *)f->obs,
                         (void *)f->ctl, (void *)f->hist, (void *)f->w, (void *)f->inc,
                         (void *)f->ppart, (void *)f->scal})
             ctx->release(p);
@@ -490,6 +492,7 @@ int vsmcx_filter_create(vsmcx_ctx *ctx, const vsmcx_filter_desc *desc, vsmcx_fil
     h.scheme = desc->scheme;
     h.eng.key[0] = desc->resample_seed;
     h.eng.index = 4;
+    h.lwc = f->lwc;
     VSMCX_HIP(hipMemcpyAsync(f->ctl, &h, sizeof(h), hipMemcpyHostToDevice, ctx->stream));
     ctx->sync();
     *out = f.release();
@@ -503,7 +506,7 @@ int vsmcx_filter_destroy(vsmcx_filter *f)
     vsmcx_ctx *c = f->ctx;
     (void)hipStreamSynchronize(c->stream);
     free_side(c, f->rb.side);
-    for (void *p : {f->st[0], f->st[1], (void *)f->lw, (void *)f->partials, (void *)f->obs,
+    for (void *p : {f->st[0], f->st[1], (void *)f->lw, (void *)f->lwc, (void *)f->partials, (void *)f->obs,
                     (void *)f->ctl, (void *)f->hist, (void *)f->w, (void *)f->inc,
                     (void *)f->ppart, (void *)f->scal})
         c->release(p);
@@ -657,13 +660,18 @@ int vsmcx_filter_save(vsmcx_filter *f, const char *path)
     FilterCtl ctl;
     VSMCX_HIP(hipMemcpyAsync(&ctl, f->ctl, sizeof(ctl), hipMemcpyDeviceToHost, f->ctx->stream));
     auto st = to_host<unsigned char>(f->ctx, f->st[f->cur], N * f->state_bytes());
-    auto lw = to_host<double>(f->ctx, f->lw, N);
+    // the file holds FP64 log-weights whatever their form on the device
+    launch_lw_expand(f->ctl, f->lw, N, f->rb.fs.wx, f->ctx->stream);
+    auto lw = to_host<double>(f->ctx, f->rb.fs.wx, N);
     auto anc = to_host<uint32_t>(f->ctx, f->rb.anc, N);
     auto sl = to_host<uint32_t>(f->ctx, f->rb.sl, N);
     auto obs = to_host<double>(f->ctx, f->obs, 2 * f->nobs);
     auto hist = to_host<StepRow>(f->ctx, f->hist, f->T);
     f->ctx->sync();
     check_dev_err(ctl);
+    ctl.lw_kind = 0;  // the file holds FP64 log-weights and no device pointer
+    ctl.lw_store_kind = 0;
+    ctl.lwc = nullptr;
     // the file holds decoded ancestors (every slot's source)
     if (ctl.prev_resampled)
         for (uint64_t i = 0; i < N; ++i) anc[i] = anc_source(anc.data(), sl.data(), i);
@@ -739,6 +747,9 @@ int vsmcx_filter_load(vsmcx_ctx *ctx, const char *path, vsmcx_filter **out)
     VSMCX_HIP(hipMemcpyAsync(f->st[0], st.data(), st.size(), hipMemcpyHostToDevice, s));
     VSMCX_HIP(hipMemcpyAsync(f->lw, lw.data(), 8 * N, hipMemcpyHostToDevice, s));
     VSMCX_HIP(hipMemcpyAsync(f->rb.anc, anc.data(), 4 * N, hipMemcpyHostToDevice, s));
+    ctl.lw_kind = 0;
+    ctl.lw_store_kind = 0;
+    ctl.lwc = f->lwc;
     if (h.nobs)
         VSMCX_HIP(hipMemcpyAsync(f->obs, obs.data(), 8 * obs.size(), hipMemcpyHostToDevice, s));
     VSMCX_HIP(hipMemcpyAsync(f->hist, hist.data(), sizeof(StepRow) * hist.size(),
@@ -778,7 +789,9 @@ int vsmcx_filter_state(vsmcx_filter *f, double *state, double *log_weight)
     if (log_weight) {
         FilterCtl h;
         VSMCX_HIP(hipMemcpyAsync(&h, f->ctl, sizeof(h), hipMemcpyDeviceToHost, f->ctx->stream));
-        VSMCX_HIP(hipMemcpyAsync(log_weight, f->lw, 8 * N, hipMemcpyDeviceToHost, f->ctx->stream));
+        launch_lw_expand(f->ctl, f->lw, N, f->rb.fs.wx, f->ctx->stream);
+        VSMCX_HIP(hipMemcpyAsync(log_weight, f->rb.fs.wx, 8 * N, hipMemcpyDeviceToHost,
+                                 f->ctx->stream));
         f->ctx->sync();
         // vSMC keeps log-weights normalised to max 0 (core/weight_set.hpp:108-117);
         // after a resample they are all zero (set_equal_weight).  The plain

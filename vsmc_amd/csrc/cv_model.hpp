@@ -114,21 +114,27 @@ __device__ __forceinline__ double CvModel<double>::llh(double x, double y,
 // floating-point arithmetic", 1991, Theorem 4): the correction cancels the
 // rounding of u, which for particles near the observation (s ~ 1e-7) is as
 // large as s itself; u == 1 leaves log2(1 + s) = s / ln 2.
+// The FP32 model's log-likelihood keeps one FP32 term l2 per particle, and the
+// FP64 value is c2 - (nu + 1)/2 ln2 l2: a compact log-weight (lw_kind 1)
+// stores l2.
+__device__ __forceinline__ float cv_llh_l2(const CvModel<float> &md, float x, float y,
+                                           const CvObs<float> &o)
+{
+    const float dx = ((x - o.hx) - o.lx) * md.inv_scale;
+    const float dy = ((y - o.hy) - o.ly) * md.inv_scale;
+    const float a = dx * dx * md.inv_nu, b = dy * dy * md.inv_nu;
+    const float s = __builtin_fmaf(a, b, a + b);
+    const float u = 1.0f + s;
+    const float d = u - 1.0f;  // exact
+    return d == 0.0f ? s * 1.44269504f : __builtin_amdgcn_logf(u) * (s * __builtin_amdgcn_rcpf(d));
+}
+
 template <>
 __device__ __forceinline__ double CvModel<float>::llh(float x, float y,
                                                       const CvObs<float> &o) const
 {
-    const float dx = ((x - o.hx) - o.lx) * inv_scale;
-    const float dy = ((y - o.hy) - o.ly) * inv_scale;
-    const float a = dx * dx * inv_nu, b = dy * dy * inv_nu;
-    const float s = __builtin_fmaf(a, b, a + b);
-    const float u = 1.0f + s;
-    const float d = u - 1.0f;  // exact
-    const float l2 = d == 0.0f ? s * 1.44269504f
-                               : __builtin_amdgcn_logf(u) * (s * __builtin_amdgcn_rcpf(d));
-    return c2 - half_nu1_ln2 * (double)l2;
+    return c2 - half_nu1_ln2 * (double)cv_llh_l2(*this, x, y, o);
 }
-
 
 // x += sd_pos z0 + dt vx (old velocity), v += sd_vel z; the prior at t = 0
 // (example/pf/include/pf_smp.hpp:150-164, 207-238)
@@ -203,6 +209,7 @@ __device__ __forceinline__ void cv_step_scalars(FilterCtl *ctl, double M, double
     ctl->resample = res;
     ctl->t = t;
     ctl->prev_resampled = 0;
+    ctl->lw_kind = ctl->lw_store_kind;
     ctl->r_int = 0;
     ctl->draws = 0;
     if (hist) {

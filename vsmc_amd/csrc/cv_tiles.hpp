@@ -11,6 +11,8 @@
 // (core/weight_set.hpp:324-331), an online log-sum-exp (core/weight_set.hpp:
 // 108-125), and the previous step's MonitorMCMC mean (core/monitor.hpp:274-302).
 #pragma once
+#include <type_traits>
+
 #include "cv_model.hpp"
 #include "device_util.hpp"
 #include "internal.hpp"
@@ -63,7 +65,7 @@ __device__ __forceinline__ void tile_sources(const uint32_t *__restrict__ anc,
 // tile lies below N, so no particle needs a bounds test.
 template <typename Real, bool kFull>
 __device__ __forceinline__ void cv_move_tile_body(
-    const FilterCtl *ctl, const typename StateOf<Real>::type *__restrict__ st_in,
+    FilterCtl *ctl, const typename StateOf<Real>::type *__restrict__ st_in,
     typename StateOf<Real>::type *__restrict__ st_out, const uint32_t *__restrict__ anc,
     const uint32_t *__restrict__ sl, double *__restrict__ lw, const double *__restrict__ obs,
     const CvParams &p, uint32_t k0, uint32_t k1, uint64_t t, uint64_t N, uint64_t tile,
@@ -76,9 +78,19 @@ __device__ __forceinline__ void cv_move_tile_body(
     md.load(p);
     const int prev = t > 0 ? ctl->prev_resampled : 0;
     const double m_old = ctl->m, c_old = ctl->coeff;
+    const LwView lw_old = lw_view(ctl, lw);  // the previous step's log-weights
     const double invN = 1.0 / (double)N;
     const CvObs<Real> o(obs[2 * t], obs[2 * t + 1]);
     const uint64_t base = tile * (uint64_t)kMoveTile;
+    // With no old log-weights to add (first step, or after a resample) the FP32
+    // pipeline's new log-weight is the increment alone, kept in compact form.
+    float *const lwc = ctl->lwc;
+    const bool compact = std::is_same<Real, float>::value && (t == 0 || prev) && lwc != nullptr;
+    if (tile == 0 && threadIdx.x == 0) {  // read by the reduction after this launch
+        ctl->lw_store_kind = compact ? 1 : 0;
+        ctl->lw_c2 = md.c2;
+        ctl->lw_h = md.half_nu1_ln2;
+    }
 
     // Loads first, for all of the thread's particles (ancestor indices, then
     // the gathered states and old log-weights), so their latencies overlap;
@@ -99,7 +111,7 @@ __device__ __forceinline__ void cv_move_tile_body(
 #pragma unroll
             for (int k = 0; k < kMoveItems; ++k) {
                 const uint64_t i = base + (uint64_t)k * kBlock + threadIdx.x;
-                lwo[k] = lw[kFull || i < N ? i : N - 1];
+                lwo[k] = lw_old(kFull || i < N ? i : N - 1);
             }
         }
     }
@@ -120,9 +132,18 @@ __device__ __forceinline__ void cv_move_tile_body(
         Real z[4];
         cv_normals(z, k0, k1, i, t);
         cv_advance(md, sk, z, t);
-        lwn[k] = lwo[k] + md.llh(sk.x, sk.y, o);
         st_out[i] = sk;
-        lw[i] = lwn[k];
+        if constexpr (std::is_same<Real, float>::value) {
+            const float l2 = cv_llh_l2(md, sk.x, sk.y, o);
+            lwn[k] = lwo[k] + (md.c2 - md.half_nu1_ln2 * (double)l2);
+            if (compact)
+                lwc[i] = l2;
+            else
+                lw[i] = lwn[k];
+        } else {
+            lwn[k] = lwo[k] + md.llh(sk.x, sk.y, o);
+            lw[i] = lwn[k];
+        }
         m = lwn[k] > m ? lwn[k] : m;
     }
     // the tile's maximum first, then one exp per particle relative to it
@@ -149,7 +170,7 @@ __device__ __forceinline__ void cv_move_tile_body(
 
 template <typename Real>
 __device__ __forceinline__ void cv_move_tile(
-    const FilterCtl *ctl, const typename StateOf<Real>::type *__restrict__ st_in,
+    FilterCtl *ctl, const typename StateOf<Real>::type *__restrict__ st_in,
     typename StateOf<Real>::type *__restrict__ st_out, const uint32_t *__restrict__ anc,
     const uint32_t *__restrict__ sl, double *__restrict__ lw, const double *__restrict__ obs,
     const CvParams &p, uint32_t k0, uint32_t k1, uint64_t t, uint64_t N, uint64_t tile,
@@ -424,6 +445,7 @@ __device__ __forceinline__ void cv_monitor_tile(const FilterCtl *ctl,
     __shared__ Part5 s_p5[kBlock / kWave];
     const int prev = ctl->prev_resampled;
     const double m = ctl->m, c = ctl->coeff, invN = 1.0 / (double)N;
+    const LwView lwf = lw_view(ctl, lw);
     const uint64_t base = tile * (uint64_t)kMoveTile;
     Part5 v{0, 0, 0, 0};
     // loads first (sources, states, log-weights), then the sums in particle order
@@ -435,7 +457,7 @@ __device__ __forceinline__ void cv_monitor_tile(const FilterCtl *ctl,
     for (int k = 0; k < kMoveItems; ++k) {
         const uint64_t i = base + (uint64_t)k * kBlock + threadIdx.x;
         s[k] = st[src[k]];
-        lwv[k] = prev ? 0.0 : lw[i < N ? i : N - 1];
+        lwv[k] = prev ? 0.0 : lwf(i < N ? i : N - 1);
     }
 #pragma unroll
     for (int k = 0; k < kMoveItems; ++k) {

@@ -56,8 +56,14 @@ struct XCached {
 struct XWeight {  // W_i = exp(lw_i - m) * coeff, computed from the log-weights
     const double *lw;
     double m, coeff;
-    __device__ void bind(const FilterCtl *c) { m = c->m; coeff = c->coeff; }
-    __device__ double x(uint64_t i) const { return exp_lse(lw[i] - m) * coeff; }
+    LwView lwv;  // lw in the filter's form
+    __device__ void bind(const FilterCtl *c)
+    {
+        m = c->m;
+        coeff = c->coeff;
+        lwv = lw_view(c, lw);
+    }
+    __device__ double x(uint64_t i) const { return exp_lse(lwv(i) - m) * coeff; }
     __device__ uint64_t ipart(uint64_t) const { return 0; }
 };
 

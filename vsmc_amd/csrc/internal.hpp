@@ -86,7 +86,36 @@ struct FilterCtl {
     int scheme;
     int err;
     uint64_t specials; // special elements of the last fold
+    // Form of the current log-weights: 0 FP64 values in the lw buffer; 1
+    // compact, the FP32 pipeline's log2 terms l2 (4 bytes each) in lwc with
+    // lw = lw_c2 - lw_h l2, which the move stores on steps whose weights are the
+    // increments alone (the first step and every step after a resample) when
+    // the filter has an lwc buffer.  Two buffers, so a step that adds to
+    // compact log-weights never writes over values it has yet to read.  The
+    // move of the step in flight records its form in lw_store_kind; the
+    // reduction after it makes that lw_kind.
+    int lw_kind;
+    int lw_store_kind;
+    double lw_c2, lw_h;
+    float *lwc;  // the filter's compact buffer (device), or null
 };
+
+// Log-weight i of the lw buffer in either form (FilterCtl::lw_kind); the
+// compact form gives the very bits the move computed.
+struct LwView {
+    const double *lw;
+    const float *lwc;
+    int kind;
+    double c2, h;
+    __device__ __forceinline__ double operator()(uint64_t i) const
+    {
+        return kind ? c2 - h * (double)lwc[i] : lw[i];
+    }
+};
+__device__ __forceinline__ LwView lw_view(const FilterCtl *ctl, const double *lw)
+{
+    return LwView{lw, ctl->lwc, ctl->lw_kind, ctl->lw_c2, ctl->lw_h};
+}
 
 // Parameters of the constant-velocity model (example/pf).
 struct CvParams {

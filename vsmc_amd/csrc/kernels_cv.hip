@@ -99,9 +99,19 @@ __global__ __launch_bounds__(kScanBlock) void k_cv_monitor_sum(FilterCtl *ctl,
 __global__ void k_cv_weights(const FilterCtl *ctl, const double *lw, uint64_t N, double *w)
 {
     const double m = ctl->m, c = ctl->coeff;
+    const LwView lwv = lw_view(ctl, lw);
     for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < N;
          i += (uint64_t)gridDim.x * blockDim.x)
-        w[i] = exp_lse(lw[i] - m) * c;
+        w[i] = exp_lse(lwv(i) - m) * c;
+}
+
+// The log-weights in FP64 whatever their form (exports and checkpoints).
+__global__ void k_lw_expand(const FilterCtl *ctl, const double *lw, uint64_t N, double *out)
+{
+    const LwView lwv = lw_view(ctl, lw);
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < N;
+         i += (uint64_t)gridDim.x * blockDim.x)
+        out[i] = lwv(i);
 }
 
 template <typename S4>
@@ -153,6 +163,12 @@ void launch_cv_monitor(bool fp64, FilterCtl *ctl, const void *stp, const uint32_
                            (const S4f *)stp, anc, sl, lw, N, partials);
     hipLaunchKernelGGL(k_cv_monitor_sum, dim3(1), dim3(kScanBlock), 0, st, ctl, partials,
                        (uint64_t)tiles_of(N), hist);
+}
+
+void launch_lw_expand(const FilterCtl *ctl, const double *lw, uint64_t N, double *out,
+                      hipStream_t st)
+{
+    hipLaunchKernelGGL(k_lw_expand, dim3(tiles_of(N) * 16), dim3(256), 0, st, ctl, lw, N, out);
 }
 
 void launch_cv_weights(FilterCtl *ctl, const double *lw, uint64_t N, double *w, hipStream_t st)

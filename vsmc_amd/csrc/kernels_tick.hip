@@ -34,7 +34,7 @@ __global__ __launch_bounds__(kBlock) void k_tick_move(const TickItem *items,
 {
     const TickItem it = items[blockIdx.x];
     const BatchFilter F = filters[it.slot];
-    const FilterCtl *ctl = P.ctl + it.slot;
+    FilterCtl *ctl = P.ctl + it.slot;
     const uint64_t t = P.next_t[it.slot];
     const uint32_t cur = P.cur[it.slot];
     const uint32_t k0 = (uint32_t)F.seed, k1 = (uint32_t)(F.seed >> 32);

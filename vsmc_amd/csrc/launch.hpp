@@ -91,6 +91,9 @@ void launch_cv_monitor(bool fp64, FilterCtl *ctl, const void *state, const uint3
                        const uint32_t *sl, const double *lw, uint64_t N, double *partials,
                        StepRow *hist, hipStream_t st);
 void launch_cv_weights(FilterCtl *ctl, const double *lw, uint64_t N, double *w, hipStream_t st);
+// FP64 log-weights from the lw buffer in either form (FilterCtl::lw_kind)
+void launch_lw_expand(const FilterCtl *ctl, const double *lw, uint64_t N, double *out,
+                      hipStream_t st);
 void launch_cv_gather(bool fp64, FilterCtl *ctl, const void *st_in, void *st_out,
                       const uint32_t *anc, const uint32_t *sl, uint64_t N, hipStream_t st);
 
@@ -151,6 +154,7 @@ struct LgParams {
 struct BatchPool {
     void *st0, *st1;       // states (float or double elements), ping-pong
     double *lw;
+    float *lwc;            // compact log-weights (FP32 state only)
     uint32_t *anc, *zl, *sl, *rep32;
     SpecRec *spec;
     double *gval;

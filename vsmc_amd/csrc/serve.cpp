@@ -159,7 +159,7 @@ struct vsmcx_serve {
 
     void release_all()
     {
-        void *ps[] = {df, P.st0, P.st1, P.lw, P.anc, P.zl, P.sl, P.rep32,
+        void *ps[] = {df, P.st0, P.st1, P.lw, P.lwc, P.anc, P.zl, P.sl, P.rep32,
                       P.spec, P.gval, P.ts, P.tagg, P.tpre, P.tzs, P.tsum, P.tzsx, P.tpart,
                       P.wx, P.thr, P.thr_fb,
                       stage_d[0], stage_d[1], P.ctl, P.hist, (void *)P.obs,
@@ -236,6 +236,7 @@ struct vsmcx_serve {
             h.scheme = scheme;
             h.eng.key[0] = J.d.resample_seed;
             h.eng.index = 4;
+            h.lwc = P.lwc ? P.lwc + F.off : nullptr;
             st_init.push_back(r);
             J.uploaded = 0;
             mark_avail(J);
@@ -433,11 +434,12 @@ int vsmcx_serve_create(vsmcx_ctx *ctx, const vsmcx_serve_desc *d, vsmcx_serve **
     S->max_jobs = d->max_jobs;
     S->max_horizon = d->max_horizon;
     // The budget covers the particle arena: per particle 80 B of weights,
-    // ancestors, lists and fold scratch plus two states of the average model
-    // dimension ((4 + lg_dim) / 2, the CV/LG mix).
+    // ancestors, lists and fold scratch (84 B with FP32 state: the compact
+    // log-weights) plus two states of the average model dimension
+    // ((4 + lg_dim) / 2, the CV/LG mix).
     const double rb = S->real_bytes();
     const double avg_dim = 0.5 * (4 + d->lg_dim);
-    const double per_particle = 80 + 2 * rb * avg_dim;
+    const double per_particle = (rb == 4 ? 84 : 80) + 2 * rb * avg_dim;
     const uint64_t particles = (uint64_t)((double)d->hbm_budget / per_particle);
     S->cap_units = particles / kUnit;
     REQUIRE(S->cap_units >= 1, "HBM budget too small for one job");
@@ -452,6 +454,7 @@ int vsmcx_serve_create(vsmcx_ctx *ctx, const vsmcx_serve_desc *d, vsmcx_serve **
         P.st0 = ctx->alloc(S->cap_state * S->real_bytes());
         P.st1 = ctx->alloc(S->cap_state * S->real_bytes());
         P.lw = ctx->alloc_n<double>(P_);
+        if (S->real_bytes() == 4) P.lwc = ctx->alloc_n<float>(P_);
         P.anc = ctx->alloc_n<uint32_t>(P_);
         P.zl = ctx->alloc_n<uint32_t>(P_);
         P.sl = ctx->alloc_n<uint32_t>(P_);
