@@ -34,24 +34,30 @@ SCHEME_NAMES = ["multinomial", "residual", "stratified", "systematic", "residual
 
 def move_bytes(prec, prev_resampled):
     """Algorithmic HBM bytes per particle of one move-kernel launch.
-    mixed:  state 16 B read + 16 B write, log-weight 8 B write, plus 4 B ancestor
-            (after a resample) or 8 B log-weight read (otherwise);
-    fp64:   same with 32 B states."""
-    sb = 16 if prec == V.MIXED else 32
-    return 2 * sb + 8 + (4 if prev_resampled else 8)
+    mixed:  state 16 B read + 16 B write; after a resample 4 B ancestor read and
+            the log-weight in its compact form (the 4-byte FP32 log2 term it is
+            computed from), otherwise 8 B log-weight read and 8 B write;
+    fp64:   states 32 B read + 32 B write, log-weight 8 B write, plus 4 B
+            ancestor (after a resample) or 8 B log-weight read (otherwise)."""
+    if prec == V.MIXED:
+        return 32 + (4 + 4 if prev_resampled else 8 + 8)
+    return 2 * 32 + 8 + (4 if prev_resampled else 8)
 
 
-def step_bytes(dim, hist, first=0):
+def step_bytes(dim, hist, first=0, compact=False):
     """Minimal algorithmic HBM bytes per particle of the steps first.. of one
     filter's history (FP32 state of dim values): the state write (4 dim) and,
-    after step 0, its read (4 dim); the log-weight write (8); after step 0 the
-    ancestor index (4) when the previous step resampled, else the old
-    log-weight (8); on a resampling step one weight read (8) and one ancestor
-    write (4).  The extra passes of the exact fold are not counted."""
+    after step 0, its read (4 dim); the log-weight write (8, or 4 in compact
+    form on step 0 and after a resample when `compact`: the CV model's FP32
+    pipeline); after step 0 the ancestor index (4) when the previous step
+    resampled, else the old log-weight (8); on a resampling step one weight
+    read (8) and one ancestor write (4).  The extra passes of the exact fold
+    are not counted."""
     res = hist["resampled"].astype(bool)
     t = np.arange(len(res))
     prev = np.concatenate([[False], res[:-1]])
-    b = 4 * dim + 8 + np.where(t > 0, 4 * dim + np.where(prev, 4, 8), 0) + np.where(res, 12, 0)
+    lw_write = np.where(compact & ((t == 0) | prev), 4, 8)
+    b = 4 * dim + lw_write + np.where(t > 0, 4 * dim + np.where(prev, 4, 8), 0) + np.where(res, 12, 0)
     return float(b[first:].sum())
 
 
@@ -220,7 +226,8 @@ def run_batched(nfilters, seed, tune=True):
     updates = float(np.sum(ns * (Ts - done)))
     hists = [b.history(f) for f in range(nfilters)]
     rfrac = float(np.mean([h["resampled"].mean() for h in hists]))
-    nbytes = sum(float(ns[f]) * step_bytes(4, hists[f], int(done[f])) for f in range(nfilters))
+    nbytes = sum(float(ns[f]) * step_bytes(4, hists[f], int(done[f]), compact=True)
+                 for f in range(nfilters))
     b.close()
     ctx.close()
     return {"value": updates / (ms * 1e-3), "unit": "particle-updates/s", "ms": ms,
@@ -235,8 +242,9 @@ def run_batched(nfilters, seed, tune=True):
             "resample_frac": rfrac,
             "roofline": whole_step_roofline(
                 nbytes, ms * 1e-3, "all kernels of every timed step: minimal algorithmic bytes "
-                "(state read/write, log-weight write, ancestor or log-weight read; on "
-                "resampling steps one weight read and one ancestor write) over the timed run"),
+                "(state read/write, log-weight write (4 B compact after a resample, else 8 B), "
+                "ancestor or log-weight read; on resampling steps one weight read and one "
+                "ancestor write) over the timed run"),
             "config": "config-3: CV filters, N log-uniform [2^8, 2^16], T uniform [50, 500], "
                       "mixed precision, systematic, ESS<N/2; the tuner times one persistent "
                       "launch (several grids and orders) and one tick per step on the same first "
@@ -321,7 +329,7 @@ def run_serving(seconds, rate, seed, budget_gb=200.0, max_jobs=8192, pacing="clo
     nbytes = 0.0
     for j, q in ids.items():
         dim = 10 if models[q] == V.MODEL_LG else 4
-        nbytes += float(ns[q]) * step_bytes(dim, s.history(j, int(Ts[q])))
+        nbytes += float(ns[q]) * step_bytes(dim, s.history(j, int(Ts[q])), compact=dim == 4)
     s.close()
     ctx.close()
     pace = ("closed loop (next observation when the previous step completes)"
@@ -336,8 +344,9 @@ def run_serving(seconds, rate, seed, budget_gb=200.0, max_jobs=8192, pacing="clo
             "roofline": whole_step_roofline(
                 nbytes, st["elapsed_ms"] * 1e-3, "all kernels of every tick: minimal algorithmic "
                 "bytes of every job step (FP32 state of 4 or 10 values read/written, log-weight "
-                "write, ancestor or log-weight read; on resampling steps one weight read and one "
-                "ancestor write) over the serving interval"),
+                "write (CV: 4 B compact after a resample), ancestor or log-weight read; on "
+                "resampling steps one weight read and one ancestor write) over the serving "
+                "interval"),
             "config": f"config-4: Poisson arrivals {rate}/s for {seconds}s, 50/50 CV and 10-D "
                       f"linear-Gaussian (A spectral radius 0.9, q 0.1, r 0.2, s0 0.25), N "
                       f"log-uniform [2^10, 2^20], T uniform [20, 1000], {budget_gb:g} GB budget, "
