@@ -480,15 +480,23 @@ static void launch_fold(FilterCtl *ctl, XSrc xs, Inv inv, uint64_t M, FoldScratc
         if (!pre_started)  // else the reduce already wrote the starts
             hipLaunchKernelGGL(k_fold_scan_sum_lse, dim3(1), dim3(kScanBlock), 0, st, ctl,
                                lse_part, fs);
-        // the segment pass computes x and leaves it for the apply pass, which
-        // reads x more than once per element
-        hipLaunchKernelGGL((k_fold_seg<XSrc>), grid, dim3(kBlock), 0, st, ctl, xs, M, fs, fs.wx);
+        // The systematic apply computes x from the log-weights again (an exp
+        // per element is cheaper there than 8 more bytes per element, written
+        // and read); the heavier inversions read the x the segment pass left.
+        constexpr bool kCache = !std::is_same<Inv, InvSystematic>::value;
+        hipLaunchKernelGGL((k_fold_seg<XSrc>), grid, dim3(kBlock), 0, st, ctl, xs, M, fs,
+                           kCache ? fs.wx : (double *)nullptr);
         launch_scan_seg(ctl, fs, st);
         if (!Inv::kInv && !out.a_out) return;
-        const XCached<XSrc> xc{fs.wx, xs};
         if (join) VSMCX_HIP(hipStreamWaitEvent(st, join, 0));
-        hipLaunchKernelGGL((k_fold_apply<XCached<XSrc>, Inv>), grid, dim3(kBlock), 0, st, ctl, xc,
-                           inv, M, fs, out);
+        if constexpr (kCache) {
+            const XCached<XSrc> xc{fs.wx, xs};
+            hipLaunchKernelGGL((k_fold_apply<XCached<XSrc>, Inv>), grid, dim3(kBlock), 0, st, ctl,
+                               xc, inv, M, fs, out);
+        } else {
+            hipLaunchKernelGGL((k_fold_apply<XSrc, Inv>), grid, dim3(kBlock), 0, st, ctl, xs, inv,
+                               M, fs, out);
+        }
         if (!Inv::kInv || !out.anc) return;
         hipLaunchKernelGGL(k_fold_scan_cnt, dim3(1), dim3(kScanBlock), 0, st, ctl, fs, epi, M);
         hipLaunchKernelGGL(k_fold_layout, grid, dim3(kBlock), 0, st, ctl, fs, out, M);

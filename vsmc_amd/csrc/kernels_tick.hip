@@ -85,6 +85,9 @@ __global__ __launch_bounds__(kJobBlock) void k_tick_scan_sum(const uint32_t *job
     wg_scan_b<kJobBlock>(in, P.ts + o, tiles_of_job(F), AddOp(), 0.0, s_red);
 }
 
+// kCache: x left in P.wx for the apply pass (the stratified inversion); the
+// systematic apply computes x from the log-weights again.
+template <bool kCache>
 __global__ __launch_bounds__(kBlock) void k_tick_seg(const TickItem *items,
                                                      const BatchFilter *filters, BatchPool P)
 {
@@ -93,12 +96,12 @@ __global__ __launch_bounds__(kBlock) void k_tick_seg(const TickItem *items,
     if (!ctl->resample) return;
     __shared__ FoldLdsT<false> L;
     const BatchFilter F = filters[it.slot];
-    // x from the log-weights, left in P.wx for the apply pass
     XWeight xs{P.lw + F.off, 0.0, 0.0};
     xs.bind(ctl);
     const uint64_t o = F.off / kFT + it.tile;
     fold_tile_seg(xs, it.tile * (uint64_t)kFT, F.n, P.ts[o], P.spec + F.off + it.tile * (uint64_t)kFT,
-                  P.tagg + o, L, P.thr + o * kBlock, P.thr_fb + o * kBlock, P.wx + F.off);
+                  P.tagg + o, L, P.thr + o * kBlock, P.thr_fb + o * kBlock,
+                  kCache ? P.wx + F.off : nullptr);
 }
 
 __global__ __launch_bounds__(kJobBlock) void k_tick_scan_seg(const uint32_t *jobs,
@@ -136,8 +139,16 @@ __global__ __launch_bounds__(kBlock) void k_tick_apply(const TickItem *items,
     if (!ctl->resample) return;
     __shared__ FoldLdsT<Inv::kWindowTmp> L;
     const BatchFilter F = filters[it.slot];
-    // the cached weights and the segment pass's per-thread prefixes
-    const XCached<XWeight> xs{P.wx + F.off, XWeight{P.lw + F.off, 0.0, 0.0}};
+    // the weights (cached by the segment pass for the stratified inversion)
+    // and the segment pass's per-thread prefixes
+    typedef typename std::conditional<std::is_same<Inv, InvSystematic>::value, XWeight,
+                                      XCached<XWeight>>::type XS;
+    XS xs;
+    if constexpr (std::is_same<Inv, InvSystematic>::value)
+        xs = XWeight{P.lw + F.off, 0.0, 0.0};
+    else
+        xs = XCached<XWeight>{P.wx + F.off, XWeight{P.lw + F.off, 0.0, 0.0}};
+    xs.bind(ctl);
     Inv inv;
     inv.bind(ctl);
     const uint64_t o = F.off / kFT + it.tile;
@@ -280,7 +291,10 @@ void launch_tick(bool fp64, int scheme, const TickItem *items, uint32_t nitems,
         hipLaunchKernelGGL(k_tick_move<float>, gi, b, 0, st, items, filters, P, cv, lg);
     hipLaunchKernelGGL(k_tick_reduce, gj, bw, 0, st, jobs, filters, P, scheme);
     hipLaunchKernelGGL(k_tick_scan_sum, gj, bs, 0, st, jobs, filters, P);
-    hipLaunchKernelGGL(k_tick_seg, gi, b, 0, st, items, filters, P);
+    if (scheme == kStratified)
+        hipLaunchKernelGGL(k_tick_seg<true>, gi, b, 0, st, items, filters, P);
+    else
+        hipLaunchKernelGGL(k_tick_seg<false>, gi, b, 0, st, items, filters, P);
     hipLaunchKernelGGL(k_tick_scan_seg, gj, bs, 0, st, jobs, filters, P);
     if (scheme == kStratified)
         hipLaunchKernelGGL(k_tick_apply<InvStratified>, gi, b, 0, st, items, filters, P);
