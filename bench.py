@@ -79,12 +79,16 @@ def whole_step_roofline(total_bytes, seconds, scope):
             "scope": scope}
 
 
-def run_precision(ctx, prec, N, W, K, obs, schemes, profile=True, concurrent=True):
+def run_precision(ctx, prec, N, W, K, obs, schemes, profile=True, concurrent=True,
+                  per_filter=True):
     """The six filters of config 1, each advanced one step per bench step.
     concurrent: every filter has its own context (its own HIP stream, default
     priority), so one filter's single-workgroup scans overlap another's
     full-grid passes; the total is host wall-clock between full
-    synchronisations.  Otherwise the filters share ctx's stream in turn."""
+    synchronisations.  Otherwise the filters share ctx's stream in turn, with
+    a HIP event before every filter's step (per_filter) or only before every
+    bench step (each event on the stream costs a few microseconds of idle GPU,
+    so the headline run records one per bench step)."""
     T = W + K
     ctxs = [V.Context(0) for _ in schemes] if concurrent else [ctx] * len(schemes)
     filters = [V.Filter(c, V.cv_desc(N, T, precision=prec, scheme=s, seed=2024 + s,
@@ -106,14 +110,14 @@ def run_precision(ctx, prec, N, W, K, obs, schemes, profile=True, concurrent=Tru
     if concurrent:
         tms = [V.Timer(c, K + 1) for c in ctxs]
     else:
-        tm = V.Timer(ctx, K * len(filters) + 1)
+        tm = V.Timer(ctx, (K * len(filters) if per_filter else K) + 1)
     t0 = time.perf_counter()
     m = 0
     for k in range(K):
         for j, f in enumerate(filters):
             if concurrent:
                 tms[j].record(k)
-            else:
+            elif per_filter or j == 0:
                 tm.record(m)
                 m += 1
             f.run(1)
@@ -129,13 +133,21 @@ def run_precision(ctx, prec, N, W, K, obs, schemes, profile=True, concurrent=Tru
         total_ms = wall * 1e3
         for t in tms:
             t.close()
-    else:
+    elif per_filter:
         per = np.array([tm.elapsed(i, i + 1) for i in range(K * len(filters))]).reshape(
             K, len(filters))
         total_ms = tm.elapsed(0, K * len(filters))
         tm.close()
+    else:
+        per = None
+        step_ms = np.array([tm.elapsed(k, k + 1) for k in range(K)])
+        total_ms = tm.elapsed(0, K)
+        tm.close()
     out = {"total_ms": total_ms, "wall_s": wall, "per_filter_step_ms": per,
            "concurrent": concurrent}
+    if per is not None:
+        step_ms = per.sum(axis=1) if not concurrent else per.max(axis=1)
+    out["step_ms"] = step_ms
     moves = []
     hist = []
     for f in filters:
@@ -517,13 +529,19 @@ def main():
     for prec in precs:
         print(f"[bench] precision {prec}: {len(schemes)} filters x N={N}, W={W} K={K}",
               file=sys.stderr, flush=True)
-        # the six filters in turn on one stream: C1's workload
+        # the six filters in turn on one stream: C1's workload, one HIP event
+        # per bench step
         res[prec] = run_precision(ctx, prec, N, W, K, obs[:W + K], schemes, profile=False,
-                                  concurrent=False)
+                                  concurrent=False, per_filter=False)
         # the move kernel's roofline from a short run with every move launch
         # bracketed by HIP events
         serial[prec] = run_precision(ctx, prec, N, 2, ks, obs[:2 + ks], schemes, profile=True,
                                      concurrent=False)
+    # each scheme's step time: a shorter in-turn run with an event before every
+    # filter's step
+    kp = min(K, 24)
+    per_run = run_precision(ctx, V.MIXED, N, W, kp, obs[:W + kp], schemes, profile=False,
+                            concurrent=False, per_filter=True)
     conc = None
     if not a.serial_filters:
         print(f"[bench] concurrent streams: {len(schemes)} filters x N={N}", file=sys.stderr,
@@ -536,7 +554,7 @@ def main():
     mixed = res[V.MIXED]
     value = nf * N * K / (mixed["total_ms"] * 1e-3)
     # a bench step: every filter one step, in turn
-    steps_ms = mixed["per_filter_step_ms"].sum(axis=1)
+    steps_ms = mixed["step_ms"]
 
     def step_roofline(prec, r):
         b = sum(N * c1_step_bytes(prec, h, W) for h in r["hist"])
@@ -561,9 +579,9 @@ def main():
         "roofline": roofline(V.MIXED, serial[V.MIXED], N, 2, schemes),
         "whole_step_roofline": step_roofline(V.MIXED, mixed),
     }
-    per_scheme = {}  # each scheme's step within the in-turn run
+    per_scheme = {}  # each scheme's step within an in-turn run (events per filter step)
     for j, s in enumerate(schemes):
-        ms = mixed["per_filter_step_ms"][:, j]
+        ms = per_run["per_filter_step_ms"][:, j]
         h = mixed["hist"][j]
         per_scheme[SCHEME_NAMES[s]] = {
             "updates_per_s": N * len(ms) / (ms.sum() * 1e-3),
