@@ -191,6 +191,52 @@ def test_filter_fp64_matches_oracle(ctx, oracle, scheme):
     f.close()
 
 
+def _tile_surplus(rep, tile=2048):
+    """Surplus copies (rep - 1 over elements with rep >= 2) of every 2048-slot tile."""
+    rep = np.asarray(rep, np.int64)
+    s = np.maximum(rep - 1, 0)
+    n = rep.size
+    return np.add.reduceat(s, np.arange(0, n, tile))
+
+
+@pytest.mark.parametrize("scheme", [V.SYSTEMATIC, V.STRATIFIED, V.MULTINOMIAL, V.RESIDUAL])
+def test_filter_fp64_tiles_with_more_copies_than_slots(ctx, oracle, scheme):
+    """A first observation at the edge of the prior makes a few particles
+    carry almost all the weight (ESS about 1), so their tiles hold more
+    surplus copies than the layout pass can stage in LDS: each copy then
+    finds its element by binary search over the tile's running copy counts.
+    The FP64 fused filter still equals the oracle's run of vSMC's algorithm
+    (decisions and draws equal, values to 1e-9); the test first checks, with
+    the oracle's own resampler on the oracle's first weights, that such tiles
+    occur."""
+    N, T = 1 << 16, 5
+    obs = _obs(T).copy()
+    obs[0] = (6.0, -6.0)  # three prior sds away in both coordinates
+    _, lw0, _ = oracle.cv_run(N, obs[:1], 42, 4242, scheme, 0.0)
+    w0 = np.exp(lw0 - lw0.max())
+    w0 /= w0.sum()
+    rep, _, _ = oracle.resample(V.SYSTEMATIC, w0, 4242)
+    assert _tile_surplus(rep).max() > 2304  # the LDS staging's capacity
+    say("filter fp64 overflow tiles", scheme)
+    f = V.Filter(ctx, V.cv_desc(N, T, precision=V.FP64_FUSED, scheme=scheme, seed=42,
+                                resample_seed=4242))
+    f.set_obs(obs)
+    f.run(T)
+    f.finish()
+    h = f.history()
+    st, lw = f.state()
+    ost, olw, rec = oracle.cv_run(N, obs, 42, 4242, scheme, 0.5)
+    assert h["resampled"][0] == 1
+    assert (h["resampled"] == rec["resampled"]).all()
+    assert (h["draws"] == rec["draws"]).all()
+    np.testing.assert_allclose(h["ess"], rec["ess"], rtol=1e-9)
+    np.testing.assert_allclose(h["log_z"], rec["log_z"], rtol=1e-9, atol=1e-9)
+    np.testing.assert_allclose(h["mean_x"], rec["mean_x"], rtol=1e-9, atol=1e-9)
+    np.testing.assert_allclose(st, ost, rtol=1e-9, atol=1e-9)
+    np.testing.assert_allclose(lw, olw, rtol=1e-9, atol=1e-9)
+    f.close()
+
+
 @pytest.mark.parametrize("scheme", range(6))
 def test_filter_debug_step_ancestors(ctx, oracle, scheme):
     """Given the weights the device resampled from (and, for the sorted-uniform

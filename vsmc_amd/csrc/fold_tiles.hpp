@@ -366,8 +366,12 @@ struct SatAddOp {
 // gives the same prefixes as the transducer scan at a fraction of its cost.
 // That is the case in every tile away from the few binade crossings of the
 // running sum.
+// uni_b (optional) receives the uniform binade, or kNoBinade when the tile
+// took the general scan; with a uniform binade the prefix is tr_units(*ex_d,
+// uni_b) (identity for thread 0), all the apply pass needs to store.
 template <typename Lds>
-__device__ __forceinline__ Seg thread_seg_prefix(const Seg &g, int fast_b, Lds &L)
+__device__ __forceinline__ Seg thread_seg_prefix(const Seg &g, int fast_b, Lds &L,
+                                                 int *uni_b = nullptr, uint64_t *ex_d = nullptr)
 {
     if (threadIdx.x == 0) L.bu[0] = (uint64_t)(uint32_t)fast_b;
     __syncthreads();
@@ -377,9 +381,41 @@ __device__ __forceinline__ Seg thread_seg_prefix(const Seg &g, int fast_b, Lds &
             block_excl_scan<kBlock>(g.tr.d0, SatAddOp(), (uint64_t)0, L.u64, (uint64_t *)nullptr);
         Seg r = seg_identity();
         if (threadIdx.x != 0) r.tr = tr_units(ex, fb0);
+        if (uni_b) {
+            *uni_b = fb0;
+            *ex_d = ex;
+        }
         return r;
     }
+    if (uni_b) {
+        *uni_b = kNoBinade;
+        *ex_d = 0;
+    }
     return block_excl_scan<kBlock>(g, SegOp(), seg_identity(), L.seg, (Seg *)nullptr);
+}
+
+// The per-thread prefixes the segment pass leaves for the apply pass, per
+// tile: thr[0..255] as Segs, or, when the whole tile took the fast path in
+// one binade (fb[0] = that binade, else kNoBinade), only each thread's unit
+// count as a uint64 in the first 2 KB of the tile's thr slots.
+__device__ __forceinline__ void thr_store(Seg *thr, int32_t *fb, int uni_b, uint64_t ex_d,
+                                          const Seg &pre)
+{
+    const int tid = threadIdx.x;
+    if (tid == 0) fb[0] = uni_b;
+    if (uni_b != kNoBinade)
+        reinterpret_cast<uint64_t *>(thr)[tid] = ex_d;
+    else
+        thr[tid] = pre;
+}
+__device__ __forceinline__ Seg thr_load(const Seg *thr, const int32_t *fb)
+{
+    const int tid = threadIdx.x;
+    const int b = fb[0];
+    if (b == kNoBinade) return thr[tid];
+    Seg r = seg_identity();
+    if (tid != 0) r.tr = tr_units(reinterpret_cast<const uint64_t *>(thr)[tid], b);
+    return r;
 }
 
 // F(c_lo) and F(c_hi) (first n in [0, nn) with v(n) > c, nn if none) by one
@@ -629,15 +665,14 @@ __device__ void fold_tile_seg(const XSrc &xs, uint64_t base, uint64_t M, double 
     fold_eps(M, &er, &ea);
     int fast_b;
     const Seg g = thread_run_seg(xv, s_x, e0, s_thr, er, ea, &fast_b);
-    const Seg thr_pre = thread_seg_prefix(g, fast_b, L);
+    int uni_b;
+    uint64_t ex_d;
+    const Seg thr_pre = thread_seg_prefix(g, fast_b, L, &uni_b, &ex_d);
     // tile total from the last thread (a struct total through a pointer would
     // keep the struct in scratch memory)
     if (tid == kBlock - 1) *agg_out = seg_op(thr_pre, g);
     // the apply pass can start from these instead of classifying again
-    if (thr_out) {
-        thr_out[tid] = thr_pre;
-        fb_out[tid] = fast_b;
-    }
+    if (thr_out) thr_store(thr_out, fb_out, uni_b, ex_d, thr_pre);
     if (g.cnt) {
         Seg pre = thr_pre;  // tile-local
         double s = s_thr;
@@ -751,7 +786,8 @@ __device__ void fold_tile_apply(FilterCtl *ctl, const XSrc &xs, const Inv &inv, 
                                 uint64_t M, double ts, const Seg tpre, const Seg tpre_next,
                                 bool has_next, const double *gval, const FoldOut &out,
                                 uint32_t *rep32,
-                                uint64_t *zs_out, Lds &L, const Seg *thr_in = nullptr)
+                                uint64_t *zs_out, Lds &L, const Seg *thr_in = nullptr,
+                                const int32_t *fb_in = nullptr)
 {
     // x, overwritten in place (each thread its own elements) by the exact
     // values; later the uniform window and the staged replication counts
@@ -779,7 +815,7 @@ __device__ void fold_tile_apply(FilterCtl *ctl, const XSrc &xs, const Inv &inv, 
     if constexpr (kReg) thread_run(s_x, e0, xv);
     Seg thr_pre;
     if (thr_in) {  // the segment pass's own classification of this thread's run
-        thr_pre = thr_in[tid];
+        thr_pre = thr_load(thr_in, fb_in);
     } else {
         const double tsum = kReg ? thread_sum(xv) : thread_sum_lds(s_x, e0);
         const double s_thr =

@@ -147,7 +147,7 @@ struct FoldScratch {
     uint64_t *zsx;     // [tiles] exclusive prefix of zs
     double *wx;        // [M] the values x_i, written by the tile-sum pass (optional)
     Seg *thr;          // [tiles * 256] each thread's exclusive segment prefix in its tile (optional)
-    int32_t *thr_fb;   // [tiles * 256] each thread's fast-path binade (with thr)
+    int32_t *thr_fb;   // [tiles * 256] per tile: the binade of a uniform tile (thr_store)
     size_t tiles;
 };
 

@@ -211,7 +211,8 @@ __global__ __launch_bounds__(kBlock) void k_fold_apply(FilterCtl *ctl, XSrc xs, 
     const Seg next =
         has_next ? seg_op(fs.cpre[(t + 1) / kFoldChunk], fs.pre[t + 1]) : seg_identity();
     fold_tile_apply(ctl, xs, inv, t, M, fs.ts[t], tpre, next, has_next, fs.gval, out, fs.rep32,
-                    fs.zs + t, L, fs.thr ? fs.thr + t * kBlock : nullptr);
+                    fs.zs + t, L, fs.thr ? fs.thr + t * kBlock : nullptr,
+                    fs.thr ? fs.thr_fb + t * kBlock : nullptr);
 }
 
 // ---- pass 6: zero / surplus ranks of the tiles (one workgroup) ---------------------

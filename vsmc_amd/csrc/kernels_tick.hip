@@ -157,7 +157,7 @@ __global__ __launch_bounds__(kBlock) void k_tick_apply(const TickItem *items,
     fold_tile_apply(ctl, xs, inv, it.tile, F.n, P.ts[o], P.tpre[o],
                     has_next ? P.tpre[o + 1] : seg_identity(), has_next, P.gval + F.off, fo,
                     P.rep32 + F.off,
-                    P.tzs + o, L, P.thr + o * kBlock);
+                    P.tzs + o, L, P.thr + o * kBlock, P.thr_fb + o * kBlock);
 }
 
 __global__ __launch_bounds__(kJobBlock) void k_tick_scan_cnt(const uint32_t *jobs,

@@ -166,7 +166,7 @@ struct BatchPool {
     double *tpart;         // per tile (serving tick): 5 move partials
     double *wx;            // per particle (ticks): the weights the tile-sum pass wrote
     Seg *thr;              // per tile x 256 (ticks): each thread's segment prefix in its tile
-    int32_t *thr_fb;       // per tile x 256 (ticks): each thread's fast-path binade
+    int32_t *thr_fb;       // per tile x 256 (ticks): the binade of a uniform tile (thr_store)
     FilterCtl *ctl;        // [filters]
     StepRow *hist;         // history rows
     const double *obs;     // observations (CV: 2 per step, LG: dim per step)
