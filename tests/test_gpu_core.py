@@ -333,6 +333,98 @@ def test_filter_ancestors_from_seed_full_size(ctx, oracle, scheme):
     f.close()
 
 
+def _rep_from_prefix(acc, v, nn):
+    """vSMC's inversion (resample/common.hpp:74-83) from a given array of
+    cumulative weights acc and the running max v of the uniforms: element i <
+    M-1 takes the uniforms with acc_{i-1} < v_n <= acc_i, the last one the rest."""
+    M = acc.size
+    c = np.searchsorted(v, acc[:M - 1], side="right").astype(np.int64)
+    c = np.maximum.accumulate(c)  # a scan that is not monotone takes nothing where it falls
+    rep = np.empty(M, np.int64)
+    rep[0] = c[0]
+    rep[1:M - 1] = np.diff(c)
+    rep[M - 1] = nn - c[-1]
+    return rep.astype(np.uint64)
+
+
+def _blocked_scan(w, dtype):
+    """Inclusive prefix sums of w in dtype, rounded in the order of a one-pass
+    blocked GPU scan: tiles of 2048 as 256 threads of 8 consecutive values; a
+    sequential sum within each thread; a Hillis-Steele scan of the 256 thread
+    sums; the tile totals chained in tile order (a decoupled look-back adds the
+    previous tiles' prefix to each tile's total); then value i is (tile prefix
+    + thread prefix) + the thread's own running sum."""
+    n = w.size
+    T = -(-n // 2048)
+    x = np.zeros(T * 2048, dtype)
+    x[:n] = w.astype(dtype)
+    run = np.cumsum(x.reshape(T, 256, 8), axis=2, dtype=dtype)
+    s = run[:, :, -1].copy()
+    d = 1
+    while d < 256:
+        t = s.copy()
+        t[:, d:] = s[:, d:] + s[:, :-d]
+        s = t
+        d *= 2
+    thr = np.zeros_like(s)
+    thr[:, 1:] = s[:, :-1]
+    tile = np.zeros(T, dtype)
+    tile[1:] = np.cumsum(s[:-1, -1], dtype=dtype)
+    acc = (tile[:, None, None] + thr[:, :, None]) + run
+    return acc.reshape(-1)[:n]
+
+
+@pytest.mark.parametrize("scheme", [V.SYSTEMATIC, V.STRATIFIED, V.MULTINOMIAL])
+def test_scan_precision_mismatch_full_size(ctx, oracle, scheme):
+    """How many ancestors a cumulative-weight scan other than vSMC's sequential
+    FP64 left fold misplaces at N = 2^24, on the filter's own weights and
+    vSMC's uniforms from the engine state before the resample, over two
+    resampling steps.  Scans: the rounding order of a blocked GPU scan
+    (_blocked_scan) in FP64 and in FP32, and the sequential FP32 fold.  The
+    exact fold (the device's own replication counts) misplaces none; an FP32
+    scan misplaces a large share of them: at 2^24 one FP32 unit at the top of
+    [0.5, 1) is the grid spacing 1/N itself."""
+    N, T = 1 << 24, 2
+    kind = {V.SYSTEMATIC: 2, V.STRATIFIED: 1, V.MULTINOMIAL: 0}[scheme]
+    obs = _obs(T, 11)
+    f = V.Filter(ctx, V.cv_desc(N, T, precision=V.MIXED, scheme=scheme, threshold=1.01))
+    f.set_obs(obs)
+    from oracle_ctypes import TF64Engine
+    for t in range(T):
+        say("scan precision", scheme, t)
+        d = f.debug_step()
+        assert d["resampled"]
+        e0 = d["engine"]
+        oe = TF64Engine()
+        for j in range(4):
+            oe.ctr[j] = e0.ctr[j]
+            oe.key[j] = e0.key[j]
+            oe.buf[j] = e0.buffer[j]
+        oe.index = e0.index
+        u, _ = oracle.u01seq(kind, N, engine=oe)
+        v = np.maximum.accumulate(u)
+        w = d["weight"]
+        acc = np.cumsum(w)  # numpy accumulates in order: vSMC's accw += w[i]
+        rep = _rep_from_prefix(acc, v, N)
+        assert (rep == d["rep"]).all(), (scheme, t)
+        cf = oracle.cfrp_trans(rep)
+        scans = {
+            "fp64 blocked scan": _blocked_scan(w, np.float64),
+            "fp32 blocked scan": _blocked_scan(w, np.float32).astype(np.float64),
+            "fp32 sequential": np.cumsum(w.astype(np.float32), dtype=np.float32).astype(np.float64),
+        }
+        counts = {}
+        for name, a in scans.items():
+            r = _rep_from_prefix(a, v, N)
+            counts[name] = (int((r != rep).sum()), int((oracle.cfrp_trans(r) != cf).sum()))
+        say("scan precision", scheme, t, "N", N, "exact fold: 0 replication counts, 0 ancestors",
+            "; ".join("%s: %d replication counts, %d ancestors" % (k, c[0], c[1])
+                      for k, c in counts.items()))
+        assert counts["fp32 blocked scan"][1] > N // 1000
+        assert counts["fp32 sequential"][1] > N // 1000
+    f.close()
+
+
 def test_filter_mixed_first_step_coupled(ctx, oracle):
     """Before any resample has moved particles between slots the mixed pipeline
     sees the same particles as FP64: ESS within 1e-5 relative and log-evidence
