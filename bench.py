@@ -498,6 +498,8 @@ def main():
                     help="also run the GMM workload in FP64 at the same size")
     ap.add_argument("--gmm-cpu-n", type=int, default=1 << 16)
     ap.add_argument("--gmm-cpu-iters", type=int, default=3)
+    ap.add_argument("--no-probe", action="store_true",
+                    help="skip the streaming-rate probe (copy, read, write of 1 GiB)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a device-to-device copy of 1 GiB five times (calibrates the "
                          "FETCH_SIZE / WRITE_SIZE counters against known bytes) and exit")
@@ -523,6 +525,8 @@ def main():
     ks = min(K, 8)  # steps of the short serial run that gives the move kernel's roofline
     obs = V.cv_simulate(max(W + K, 2 + ks), 11)
     ctx = V.Context(0)
+    # the streaming rates this GPU reaches, next to the published peak
+    probe = None if a.no_probe else ctx.stream_probe(1 << 30, 5)
     res = {}
     precs = [V.MIXED] + ([] if a.no_fp64 else [V.FP64_FUSED, V.FP64_PLAIN])
     serial = {}
@@ -579,6 +583,11 @@ def main():
         "roofline": roofline(V.MIXED, serial[V.MIXED], N, 2, schemes),
         "whole_step_roofline": step_roofline(V.MIXED, mixed),
     }
+    if probe is not None:
+        probe.update({"bytes": 1 << 30, "kernels": "vsmcx_stream_probe: grid-stride 16-B "
+                      "loads/stores over 1 GiB, best of 5 launches at 1024 and 4096 workgroups",
+                      "move_achieved_vs_copy": out["roofline"]["achieved"] / probe["copy_GBps"]})
+        out["hbm_probe"] = probe
     per_scheme = {}  # each scheme's step within an in-turn run (events per filter step)
     for j, s in enumerate(schemes):
         ms = per_run["per_filter_step_ms"][:, j]

@@ -53,6 +53,13 @@ int vsmcx_ctx_destroy(vsmcx_ctx *ctx);
 int vsmcx_ctx_sync(vsmcx_ctx *ctx);
 /* bytes currently allocated by the context */
 int vsmcx_ctx_allocated(vsmcx_ctx *ctx, uint64_t *bytes);
+/* Streaming rates of the device, a measurement aid with no counterpart in the
+ * reference: a copy of `bytes` bytes from one buffer to another, a read of
+ * them and a write of them (16 B per lane, grid-stride), each the best of
+ * `reps` timed launches; gbps[0..2] = copy (read + write bytes), read, write
+ * in 10^9 bytes/s.  Allocates 2 x bytes from the context's budget for the
+ * call.  bytes in [2^20, 2^36], reps in [1, 100]. */
+int vsmcx_stream_probe(vsmcx_ctx *ctx, uint64_t bytes, int reps, double gbps[3]);
 
 /* ---------------------------------------------------------------- RNG
  * Bulk block generation: block k (k = 0 .. nblocks-1) is the engine's

@@ -65,6 +65,20 @@ def test_threefry_fills(ctx, oracle):
         assert (out[k] == oracle.threefry4x64_block(c, [5, 6, 7, 8])).all()
 
 
+def test_stream_probe(ctx):
+    """The streaming probe reports rates between a floor no HBM part is below
+    and the published 8 TB/s peak, and gives its buffers back."""
+    say("stream probe")
+    before = ctx.allocated()
+    r = ctx.stream_probe(1 << 28, 3)
+    say("stream probe", r)
+    for k in ("copy_GBps", "read_GBps", "write_GBps"):
+        assert 500.0 < r[k] < 8000.0, (k, r[k])
+    assert ctx.allocated() == before
+    with pytest.raises(V.VsmcxError):
+        ctx.stream_probe(1 << 10, 3)
+
+
 # ---------------------------------------------------------------- fold
 def _weights(kind, n, rng):
     if kind == "flat":

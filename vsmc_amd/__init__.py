@@ -127,6 +127,7 @@ def lib():
         "vsmcx_ctx_destroy": ([vp], i32),
         "vsmcx_ctx_sync": ([vp], i32),
         "vsmcx_ctx_allocated": ([vp, P(u64)], i32),
+        "vsmcx_stream_probe": ([vp, u64, i32, vp], i32),
         "vsmcx_philox4x32_fill": ([vp, vp, vp, u64, vp], i32),
         "vsmcx_threefry4x64_fill": ([vp, vp, vp, u64, vp], i32),
         "vsmcx_threefry4x32_fill": ([vp, vp, vp, u64, vp], i32),
@@ -258,6 +259,13 @@ class Context:
         v = C.c_uint64()
         _check(lib().vsmcx_ctx_allocated(self.h, C.byref(v)))
         return v.value
+
+    def stream_probe(self, nbytes=1 << 30, reps=5):
+        """Streaming rates of the device in 10^9 bytes/s: a copy of nbytes
+        (read + write bytes), a read and a write, each the best of reps."""
+        g = np.zeros(3)
+        _check(lib().vsmcx_stream_probe(self.h, nbytes, reps, g.ctypes.data))
+        return {"copy_GBps": float(g[0]), "read_GBps": float(g[1]), "write_GBps": float(g[2])}
 
     # ---- RNG ----
     def philox4x32_fill(self, key, ctr0, nblocks):

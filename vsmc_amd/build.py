@@ -19,7 +19,7 @@ FLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17", "-ffp-contract=o
 SOURCES = ["kernels_rng.hip", "kernels_fold.hip", "kernels_cv.hip", "kernels_plain.hip",
            "kernels_batch.hip", "kernels_tick.hip", "kernels_gmm.hip", "capi.cpp",
            "batch.cpp", "serve.cpp", "gmm.cpp", "state.cpp",
-           "kernels_weights.hip", "weights.cpp", "noalloc.cpp"]
+           "kernels_weights.hip", "weights.cpp", "noalloc.cpp", "kernels_probe.hip"]
 
 
 def _compile(src):

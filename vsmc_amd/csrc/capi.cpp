@@ -317,6 +317,39 @@ int vsmcx_ctx_allocated(vsmcx_ctx *ctx, uint64_t *bytes)
     API_END
 }
 
+int vsmcx_stream_probe(vsmcx_ctx *ctx, uint64_t bytes, int reps, double gbps[3])
+{
+    API_BEGIN
+    REQUIRE(ctx && gbps, "NULL argument");
+    REQUIRE(bytes >= (1ull << 20) && bytes <= (1ull << 36), "bytes outside [2^20, 2^36]");
+    REQUIRE(reps >= 1 && reps <= 100, "reps outside [1, 100]");
+    const uint64_t n = bytes / 16;
+    float4 *a = ctx->alloc_n<float4>(n);
+    float4 *b = nullptr;
+    try {
+        b = ctx->alloc_n<float4>(n);
+    } catch (...) {
+        ctx->release(a);
+        throw;
+    }
+    hipError_t e = hipMemsetAsync(a, 0, n * 16, ctx->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(b, 0, n * 16, ctx->stream);
+    float ms[3] = {-1.f, -1.f, -1.f};
+    for (int k = 0; k < 3 && e == hipSuccess; ++k) {
+        ms[k] = launch_stream_probe(k, a, b, n, reps, ctx->stream);
+        if (ms[k] <= 0.f) e = hipErrorUnknown;
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    ctx->release(a);
+    ctx->release(b);
+    VSMCX_HIP(e);
+    const double nb = (double)(n * 16);
+    gbps[0] = 2 * nb / (ms[0] * 1e-3) / 1e9;
+    gbps[1] = nb / (ms[1] * 1e-3) / 1e9;
+    gbps[2] = nb / (ms[2] * 1e-3) / 1e9;
+    API_END
+}
+
 // ---------------------------------------------------------------- RNG
 int vsmcx_philox4x32_fill(vsmcx_ctx *ctx, const uint32_t key[2], const uint32_t ctr0[4],
                           uint64_t nblocks, uint32_t *out)

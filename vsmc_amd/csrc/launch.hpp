@@ -64,6 +64,11 @@ void launch_side_uniforms(FilterCtl *ctl, ResampleBuffers &rb, hipStream_t st);
 // out[i] = in[i] for i < n (uint32 -> uint64)
 void launch_widen_u32(const uint32_t *in, uint64_t *out, uint64_t n, hipStream_t st);
 
+// kernels_probe.hip: best ms of reps launches of a streaming probe (kind 0
+// copy a -> b, 1 read a, 2 write b) over n float4 elements; negative on error
+float launch_stream_probe(int kind, const float4 *a, float4 *b, uint64_t n, int reps,
+                          hipStream_t st);
+
 // kernels_rng.hip
 void launch_philox4x32_fill(uint32_t k0, uint32_t k1, const uint32_t ctr0[4], uint64_t nblocks,
                             uint32_t *out, hipStream_t st);
