@@ -987,18 +987,22 @@ __device__ void fold_tile_apply(FilterCtl *ctl, const XSrc &xs, const Inv &inv, 
                         pos = len;
                         continue;
                     }
-                    uint64_t g0;
-                    if (inv.bracket(a, &g0)) {
-                        // answer in [b, b + 6) iff v[b-1] <= a and v[b+6] > a
-                        const int64_t b = (int64_t)g0 - (int64_t)w0;
-                        const int bl = b < 0 ? 0 : (b > len ? len : (int)b);
-                        const int bh = b + 6 > len ? len : (b + 6 < 0 ? 0 : (int)(b + 6));
-                        if ((bl == 0 || s_x[bl - 1] <= a) && (bh == len || s_x[bh] > a)) {
-                            int q = bl;
-                            while (q < bh && s_x[q] <= a) ++q;
-                            c[k] += q;
-                            pos = q;
-                            continue;
+                    if constexpr (Inv::kPairF) {
+                        // u_n = (n + U_n) delta lies within a few units 2^-52 of
+                        // [n, n + 1) / nn, so for b = floor(a nn) - w0 the window
+                        // holds v_{b-2} <= a < v_{b+2}; both ends are checked, and
+                        // the count is then b - 1 plus the places b-1, b, b+1 at
+                        // or below a (v is non-decreasing), without a loop
+                        const int b = (int)floor(a * cnt_d(nn) - cnt_d(w0));
+                        if (b >= 2 && b + 2 < len) {
+                            const double m2 = s_x[b - 2], m1 = s_x[b - 1], z0 = s_x[b],
+                                         p1 = s_x[b + 1], p2 = s_x[b + 2];
+                            if (m2 <= a && p2 > a) {
+                                const int q = b - 1 + (m1 <= a) + (z0 <= a) + (p1 <= a);
+                                c[k] += q;
+                                pos = q;
+                                continue;
+                            }
                         }
                     }
                     int lo = pos, step = 1, hi = pos;
