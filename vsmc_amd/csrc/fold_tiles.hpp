@@ -103,6 +103,7 @@ struct XResidStored {
 // ------------------------------------------------------------------------
 struct InvNone {
     static constexpr bool kInv = false;
+    static constexpr int kMinWaves = 1;  // waves per SIMD the apply kernel is compiled for
     static constexpr bool kWindow = false;
     static constexpr bool kWindowTmp = false;
     static constexpr bool kPairF = false;
@@ -122,6 +123,7 @@ __device__ __forceinline__ uint64_t cnt_floor(double est, uint64_t nn)
 // U01SequenceSystematic (include/vsmc/rng/u01.hpp:253-287): u_n = u0 + n*delta
 struct InvSystematic {
     static constexpr bool kInv = true;
+    static constexpr int kMinWaves = 6;  // waves per SIMD the apply kernel is compiled for
     static constexpr bool kWindow = false;
     static constexpr bool kWindowTmp = false;
     static constexpr bool kPairF = false;
@@ -173,6 +175,7 @@ __device__ __noinline__ double strat_u(const Tf64State *eng, double delta, uint6
 // inversion's running-max semantics (u_{n-2} < u_n always).
 struct InvStratified {
     static constexpr bool kInv = true;
+    static constexpr int kMinWaves = 1;  // waves per SIMD the apply kernel is compiled for
     static constexpr bool kWindow = true;
     static constexpr bool kWindowTmp = true;  // fill_window keeps a table in L.tmp
     static constexpr bool kPairF = true;
@@ -281,6 +284,7 @@ struct InvStratified {
 // max(tile_prefix_max[n / kFoldTile], uscan[n]).
 struct InvArray {
     static constexpr bool kInv = true;
+    static constexpr int kMinWaves = 1;  // waves per SIMD the apply kernel is compiled for
     static constexpr bool kWindow = true;
     static constexpr bool kWindowTmp = false;  // fill_window reads the array
     static constexpr bool kPairF = false;

@@ -198,8 +198,8 @@ static void launch_scan_seg(FilterCtl *ctl, FoldScratch &fs, hipStream_t st)
 
 // ---- pass 5: exact values, inversion, replication --------------------------------
 template <typename XSrc, typename Inv>
-__global__ __launch_bounds__(kBlock) void k_fold_apply(FilterCtl *ctl, XSrc xs, Inv inv,
-                                                       uint64_t M, FoldScratch fs, FoldOut out)
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(Inv::kMinWaves)))
+void k_fold_apply(FilterCtl *ctl, XSrc xs, Inv inv, uint64_t M, FoldScratch fs, FoldOut out)
 {
     if (!ctl->resample) return;
     __shared__ FoldLdsT<Inv::kWindowTmp> L;

@@ -27,8 +27,11 @@ __device__ __forceinline__ uint64_t tiles_of_job(const BatchFilter &F)
 }
 
 // ---- move (all jobs' tiles) ---------------------------------------------------------
+// FP32: compiled for five waves per SIMD (the CV and LG bodies together)
 template <typename Real>
-__global__ __launch_bounds__(kBlock) void k_tick_move(const TickItem *items,
+__global__ __launch_bounds__(kBlock)
+__attribute__((amdgpu_waves_per_eu(std::is_same<Real, float>::value ? 5 : 1)))
+void k_tick_move(const TickItem *items,
                                                       const BatchFilter *filters, BatchPool P,
                                                       CvParams cv, LgParams lg)
 {
