@@ -925,12 +925,55 @@ __device__ void fold_tile_apply(FilterCtl *ctl, const XSrc &xs, const Inv &inv, 
             if (t > 0 && tid == 0) cstart = inv.F(v_start);
         } else {
             // the tile holds uniforms [F(a_{base-1}), F(a_{last compared}))
-            if (tid < kWave) {
+            const bool need_hi = M >= 2 && base <= last_cmp;
+            // the superset window: the stratified closed form puts F(v_start)
+            // and F(c_hi) within [e - 1, e + 2] of e = floor(c nn), so [lo, hi)
+            // from e(v_start) - 2 to e(c_hi) + 3 holds the tile's uniforms.
+            // Checked once filled: v_lo <= v_start (every place below lo
+            // counts for every element) and v_{hi-1} > c_hi (none from there
+            // on does); otherwise the exact bounds are searched as below.
+            bool superset = false;
+            if constexpr (Inv::kPairF) {
+                if (tid == 0) {
+                    uint64_t mode = 0;
+                    if (need_hi && nn > 0) {
+                        const uint64_t il =
+                            (base + kFT - 1 < last_cmp) ? base + kFT - 1 : last_cmp;
+                        const double c_hi = s_a[fidx((int)(il - base))];
+                        uint64_t lo, hi;
+                        inv.bracket(v_start, &lo);
+                        inv.bracket(c_hi, &hi);
+                        if (t == 0) lo = 0;
+                        hi = hi + 5 < nn ? hi + 5 : nn;
+                        if (lo < hi && hi - lo <= (uint64_t)kWin) {
+                            mode = 1;
+                            s_bu[0] = lo;
+                            s_bu[1] = hi;
+                            L.wend[0] = v_start;
+                            L.wend[1] = c_hi;
+                        }
+                    }
+                    s_u64[0] = mode;
+                }
+                __syncthreads();
+                if (s_u64[0]) {
+                    const uint64_t lo = s_bu[0], hi = s_bu[1];
+                    const int len = (int)(hi - lo);
+                    inv.fill_window(s_x, s_tmp, lo, len);
+                    __syncthreads();
+                    superset = (lo == 0 || s_x[0] <= L.wend[0]) &&
+                               (hi == nn || s_x[len - 1] > L.wend[1]);
+                    if (!superset) __syncthreads();  // the window is read before the refill
+                }
+            }
+            if (!superset && tid < kWave) {
                 uint64_t lo = 0, hi = 0;
                 if (M >= 2) {
-                    const bool need_hi = base <= last_cmp;
                     const uint64_t il = (base + kFT - 1 < last_cmp) ? base + kFT - 1 : last_cmp;
-                    const double c_hi = need_hi ? s_a[fidx((int)(il - base))] : v_start;
+                    // (c_hi saved before a superset window took the LDS)
+                    const double c_hi = !need_hi ? v_start
+                                        : (Inv::kPairF && s_u64[0]) ? L.wend[1]
+                                                                    : s_a[fidx((int)(il - base))];
                     bool done = false;
                     // F(c_hi) >= F(v_start) = lo for sorted values; the max
                     // keeps the lower bound lo for any others
@@ -968,16 +1011,20 @@ __device__ void fold_tile_apply(FilterCtl *ctl, const XSrc &xs, const Inv &inv, 
                     s_bu[1] = hi;
                 }
             }
-            __syncthreads();
+            if (!superset) __syncthreads();
             const uint64_t wlo = s_bu[0], whi = s_bu[1];
             cstart = wlo;
 #pragma unroll
             for (int k = 0; k < kFI; ++k) c[k] = (uint32_t)wlo;
+            bool filled = superset;  // the superset window is already in LDS
             for (uint64_t w0 = wlo; w0 < whi; w0 += kWin) {
                 const int len = (int)((whi - w0) < (uint64_t)kWin ? (whi - w0) : (uint64_t)kWin);
-                __syncthreads();
-                inv.fill_window(s_x, s_tmp, w0, len);
-                __syncthreads();
+                if (!filled) {
+                    __syncthreads();
+                    inv.fill_window(s_x, s_tmp, w0, len);
+                    __syncthreads();
+                }
+                filled = false;
                 const double first = s_x[0], lastv = s_x[len - 1];
                 // the thread's values are sorted: gallop from the previous
                 // position, then bisect (first index with value > a)
@@ -1026,6 +1073,13 @@ __device__ void fold_tile_apply(FilterCtl *ctl, const XSrc &xs, const Inv &inv, 
                     c[k] += lo;
                     pos = lo;
                 }
+            }
+            if (superset && tid == 0) {
+                // F(v_start): the places of the single window at or below it
+                int q = 0;
+                const int len = (int)(whi - wlo);
+                while (q < len && s_x[q] <= L.wend[0]) ++q;
+                cstart = t == 0 ? 0 : wlo + (uint64_t)q;
             }
 #pragma unroll
             for (int k = 0; k < kFI; ++k) {
