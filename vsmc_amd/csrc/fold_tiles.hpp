@@ -1026,8 +1026,11 @@ __device__ void fold_tile_apply(FilterCtl *ctl, const XSrc &xs, const Inv &inv, 
                 }
                 filled = false;
                 const double first = s_x[0], lastv = s_x[len - 1];
-                // the thread's values are sorted: gallop from the previous
-                // position, then bisect (first index with value > a)
+                // the thread's values are sorted: the first one by a binary
+                // search over the window (the same rounds in every lane), each
+                // later one from the previous position: four places in one
+                // round (most values take fewer), then gallop and bisect
+                // (first index with value > a)
                 int pos = 0;
 #pragma unroll
                 for (int k = 0; k < kFI; ++k) {
@@ -1057,10 +1060,30 @@ __device__ void fold_tile_apply(FilterCtl *ctl, const XSrc &xs, const Inv &inv, 
                         }
                     }
                     int lo = pos, step = 1, hi = pos;
-                    while (hi < len && s_x[hi] <= a) {
-                        lo = hi + 1;
-                        hi += step;
-                        step <<= 1;
+                    if (k == 0) {
+                        hi = len - 1;  // v_{len-1} > a here
+                    } else {
+                        // places pos .. pos+3 (those past the window read as
+                        // above a); v is non-decreasing, so the ones at or
+                        // below a come first
+                        const int j1 = pos + 1 < len ? pos + 1 : len - 1;
+                        const int j2 = pos + 2 < len ? pos + 2 : len - 1;
+                        const int j3 = pos + 3 < len ? pos + 3 : len - 1;
+                        const double v0 = s_x[pos < len ? pos : len - 1], v1 = s_x[j1],
+                                     v2 = s_x[j2], v3 = s_x[j3];
+                        const int n = (pos < len && v0 <= a) + (pos + 1 < len && v1 <= a) +
+                                      (pos + 2 < len && v2 <= a) + (pos + 3 < len && v3 <= a);
+                        if (n < 4) {
+                            c[k] += pos + n;
+                            pos += n;
+                            continue;
+                        }
+                        lo = hi = pos + 4;
+                        while (hi < len && s_x[hi] <= a) {
+                            lo = hi + 1;
+                            hi += step;
+                            step <<= 1;
+                        }
                     }
                     if (hi > len) hi = len;
                     while (lo < hi) {
