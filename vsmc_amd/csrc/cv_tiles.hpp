@@ -200,6 +200,8 @@ __device__ __forceinline__ void cv_move_tile(
 // that of the plain loop, but a wave does not wait a memory latency per tile.
 constexpr int kPartAhead = 8;
 constexpr int kPartMaxWaves = 16;
+// the reduce after the move: one workgroup of kPartMaxWaves waves
+constexpr int kReduceBlock = kPartMaxWaves * kWave;
 
 __device__ __forceinline__ int part_vwaves(uint64_t ntiles)
 {

@@ -33,13 +33,13 @@ __global__ __launch_bounds__(kBlock) void k_cv_move(
 // uniform-grid prologue and the fold's approximate tile starts of the
 // filter's weights (TileSumsFromLse) into fs.ts, which the fold would
 // otherwise compute in two more launches.
-__global__ __launch_bounds__(kScanBlock) void k_cv_reduce(FilterCtl *ctl, const double *partials,
+__global__ __launch_bounds__(kReduceBlock) void k_cv_reduce(FilterCtl *ctl, const double *partials,
                                                          uint64_t ntiles, StepRow *hist,
                                                          uint64_t t, int prologue_scheme,
                                                          FoldScratch fs)
 {
     __shared__ PartLds L;
-    __shared__ double s_red[kScanBlock / kWave];
+    __shared__ double s_red[kReduceBlock / kWave];
     __shared__ double s_mc[2];
     __shared__ int s_res;
     __shared__ double s_stash[kStashTiles];  // se_t e^(M_t - M) per tile
@@ -82,7 +82,7 @@ __global__ __launch_bounds__(kBlock) void k_cv_monitor(FilterCtl *ctl,
                           partials + 5 * (uint64_t)blockIdx.x);
 }
 
-__global__ __launch_bounds__(kScanBlock) void k_cv_monitor_sum(FilterCtl *ctl,
+__global__ __launch_bounds__(kReduceBlock) void k_cv_monitor_sum(FilterCtl *ctl,
                                                               const double *partials,
                                                               uint64_t ntiles, StepRow *hist)
 {
@@ -147,7 +147,7 @@ void launch_cv_reduce(FilterCtl *ctl, const double *partials, uint64_t ntiles, S
 {
     FoldScratch fs{};
     if (starts) fs = *starts;
-    hipLaunchKernelGGL(k_cv_reduce, dim3(1), dim3(kScanBlock), 0, st, ctl, partials, ntiles, hist,
+    hipLaunchKernelGGL(k_cv_reduce, dim3(1), dim3(kReduceBlock), 0, st, ctl, partials, ntiles, hist,
                        t, starts ? prologue_scheme : -1, fs);
 }
 
@@ -161,7 +161,7 @@ void launch_cv_monitor(bool fp64, FilterCtl *ctl, const void *stp, const uint32_
     else
         hipLaunchKernelGGL(k_cv_monitor<float>, dim3(tiles_of(N)), dim3(kBlock), 0, st, ctl,
                            (const S4f *)stp, anc, sl, lw, N, partials);
-    hipLaunchKernelGGL(k_cv_monitor_sum, dim3(1), dim3(kScanBlock), 0, st, ctl, partials,
+    hipLaunchKernelGGL(k_cv_monitor_sum, dim3(1), dim3(kReduceBlock), 0, st, ctl, partials,
                        (uint64_t)tiles_of(N), hist);
 }
 
