@@ -175,7 +175,7 @@ __device__ __noinline__ double strat_u(const Tf64State *eng, double delta, uint6
 // inversion's running-max semantics (u_{n-2} < u_n always).
 struct InvStratified {
     static constexpr bool kInv = true;
-    static constexpr int kMinWaves = 1;  // waves per SIMD the apply kernel is compiled for
+    static constexpr int kMinWaves = 5;  // waves per SIMD the apply kernel is compiled for
     static constexpr bool kWindow = true;
     static constexpr bool kWindowTmp = true;  // fill_window keeps a table in L.tmp
     static constexpr bool kPairF = true;
@@ -284,7 +284,7 @@ struct InvStratified {
 // max(tile_prefix_max[n / kFoldTile], uscan[n]).
 struct InvArray {
     static constexpr bool kInv = true;
-    static constexpr int kMinWaves = 1;  // waves per SIMD the apply kernel is compiled for
+    static constexpr int kMinWaves = 8;  // waves per SIMD the apply kernel is compiled for
     static constexpr bool kWindow = true;
     static constexpr bool kWindowTmp = false;  // fill_window reads the array
     static constexpr bool kPairF = false;
