@@ -81,7 +81,7 @@ struct vsmcx_batch {
     void release_all()
     {
         void *ps[] = {df, order_work, order_given, P.st0, P.st1, P.lw, P.lwc, P.anc, P.zl, P.sl,
-                      P.rep32, P.spec, P.gval, P.ts, P.tagg, P.tpre, P.tzs, P.tsum, P.tzsx,
+                      P.rep32, P.rep8, P.spec, P.gval, P.ts, P.tagg, P.tpre, P.tzs, P.tsum, P.tzsx,
                       P.tpart, P.wx, P.thr, P.thr_fb, P.ctl, P.hist, (void *)P.obs, P.next_t,
                       (void *)P.avail, P.cur, P.err_any, queue, jobs_dev, items_dev};
         for (void *q : ps) ctx->release(q);
@@ -217,6 +217,7 @@ int vsmcx_batch_create(vsmcx_ctx *ctx, const vsmcx_batch_desc *d, vsmcx_batch **
         P.zl = ctx->alloc_n<uint32_t>(off);
         P.sl = ctx->alloc_n<uint32_t>(off);
         P.rep32 = ctx->alloc_n<uint32_t>(off);
+        P.rep8 = ctx->alloc_n<uint8_t>(off);
         P.spec = ctx->alloc_n<SpecRec>(off);
         P.gval = ctx->alloc_n<double>(off);
         P.ts = ctx->alloc_n<double>(off / kFoldTile);

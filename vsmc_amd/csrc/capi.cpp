@@ -43,7 +43,7 @@ void free_resample(vsmcx_ctx *c, ResampleBuffers &rb)
 {
     void *ps[] = {rb.anc, rb.zl, rb.sl, rb.rep_out, rb.a_out, rb.ufs.tsum, rb.ufs.ts, rb.uscan,
                   rb.u_tmax, rb.u_tmaxex, rb.fs.tsum, rb.fs.ts, rb.fs.agg, rb.fs.pre, rb.fs.spec,
-                  rb.fs.gval, rb.fs.rep32, rb.fs.zs, rb.fs.zsx, rb.fs.wx, rb.fs.thr,
+                  rb.fs.gval, rb.fs.rep32, rb.fs.rep8, rb.fs.zs, rb.fs.zsx, rb.fs.wx, rb.fs.thr,
                   rb.fs.thr_fb, rb.fs.cagg, rb.fs.cpre, rb.fs.tcnt};
     for (void *p : ps) c->release(p);
     if (rb.ufs.agg != rb.fs.agg) {  // the uniforms' own fold scratch

@@ -789,7 +789,7 @@ template <typename XSrc, typename Inv, typename Lds>
 __device__ void fold_tile_apply(FilterCtl *ctl, const XSrc &xs, const Inv &inv, uint64_t t,
                                 uint64_t M, double ts, const Seg tpre, const Seg tpre_next,
                                 bool has_next, const double *gval, const FoldOut &out,
-                                uint32_t *rep32,
+                                uint32_t *rep32, uint8_t *rep8,
                                 uint64_t *zs_out, Lds &L, const Seg *thr_in = nullptr,
                                 const int32_t *fb_in = nullptr)
 {
@@ -1150,11 +1150,22 @@ __device__ void fold_tile_apply(FilterCtl *ctl, const XSrc &xs, const Inv &inv, 
         if (!out.anc) return;  // replication only (M != N): no cfrp layout
         const uint64_t zs_tile = block_reduce<kBlock>(zs_thr, AddOp(), s_u64);
         if (tid == 0) *zs_out = zs_tile;
+        // the counts as bytes for the layout pass, four elements a word (the
+        // byte array spans whole tiles): a count of 255 or more goes in as
+        // 255 with the count itself in rep32 (after every read of rep32 in
+        // this tile: the residual integer parts, before the reduce above)
+        uint32_t *const r8 = reinterpret_cast<uint32_t *>(rep8 + base);
 #pragma unroll
-        for (int k = 0; k < kFI; ++k) {
-            const int e = k * kBlock + tid;
-            const uint64_t i = base + e;
-            if (i < M) rep32[i] = s_rep[e];
+        for (int w = 0; w < kFI / 4; ++w) {
+            const int q = w * kBlock + tid;  // word q: elements 4q .. 4q+3
+            uint32_t pk = 0;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const uint32_t r = s_rep[4 * q + j];
+                pk |= (r < 255u ? r : 255u) << (8 * j);
+                if (r >= 255u && base + 4 * q + j < M) rep32[base + 4 * q + j] = r;
+            }
+            r8[q] = pk;
         }
     }
 }
@@ -1174,7 +1185,7 @@ __device__ void fold_tile_apply(FilterCtl *ctl, const XSrc &xs, const Inv &inv, 
 
 template <typename Lds>
 __device__ void fold_tile_layout(FilterCtl *ctl, uint64_t t, uint64_t M, const uint32_t *rep32,
-                                 uint64_t zsx, const FoldOut &out, Lds &L)
+                                 const uint8_t *rep8, uint64_t zsx, const FoldOut &out, Lds &L)
 {
     // kFPad entries each, element e at fidx(e): the per-thread loop reads
     // consecutive entries, which the padding spreads over the LDS banks.  Once
@@ -1190,12 +1201,19 @@ __device__ void fold_tile_layout(FilterCtl *ctl, uint64_t t, uint64_t M, const u
     const uint32_t b32 = (uint32_t)base;  // element indices stay below 2^32
     const int e0 = tid * kFI;
     const int nv = M - base < (uint64_t)kFT ? (int)(M - base) : kFT;  // elements of the tile
-    const uint32_t *const rt = rep32 + base;
+    // the apply pass's byte counts, four a word (255: the count is in rep32)
+    const uint32_t *const r8 = reinterpret_cast<const uint32_t *>(rep8 + base);
 #pragma unroll
-    for (int k = 0; k < kFI; ++k) {
-        const int e = k * kBlock + tid;
-        const uint32_t r = rt[e < nv ? e : nv - 1];  // in bounds, no branch
-        s_rep[fidx(e)] = e < nv ? r : 1u;
+    for (int w = 0; w < kFI / 4; ++w) {
+        const int q = w * kBlock + tid;
+        const uint32_t pk = r8[q];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int e = 4 * q + j;
+            uint32_t r = (pk >> (8 * j)) & 255u;
+            if (r == 255u && e < nv) r = rep32[base + e];
+            s_rep[fidx(e)] = e < nv ? r : 1u;
+        }
     }
     __syncthreads();
     uint32_t rv[kFI];

@@ -142,7 +142,9 @@ struct FoldScratch {
     uint32_t *tcnt;    // [tiles] pre[t].cnt: specials before tile t within its chunk
     SpecRec *spec;     // [tiles * kFoldTile] special elements, tile-local order
     double *gval;      // [M + 1] exact fold value after each special, fold order
-    uint32_t *rep32;   // [M] replication counts for the layout pass
+    uint32_t *rep32;   // [M] residual integer parts; counts of 255 or more for the layout pass
+    uint8_t *rep8;     // [M rounded up to whole tiles] replication counts for the layout
+                       // pass, one byte each; 255: the count is in rep32
     uint64_t *zs;      // [tiles] zero slots << 32 | surplus copies
     uint64_t *zsx;     // [tiles] exclusive prefix of zs
     double *wx;        // [M] the values x_i, written by the tile-sum pass (optional)

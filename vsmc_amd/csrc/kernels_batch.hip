@@ -231,7 +231,7 @@ __device__ __noinline__ void batch_fold_apply(const BatchFilter &F, FilterCtl *c
         fold_tile_apply(ctl, xs, inv, tt, N, P.ts[F.off / kFT + tt], P.tpre[F.off / kFT + tt],
                         tt + 1 < T ? P.tpre[F.off / kFT + tt + 1] : seg_identity(), tt + 1 < T,
                         P.gval + F.off, fo,
-                        P.rep32 + F.off, P.tzs + F.off / kFT + tt, S.L);
+                        P.rep32 + F.off, P.rep8 + F.off, P.tzs + F.off / kFT + tt, S.L);
     __syncthreads();
     uint64_t *zs = P.tzs + F.off / kFT;
     uint64_t carry = 0;
@@ -264,7 +264,8 @@ __device__ __noinline__ void batch_fold_layout(const BatchFilter &F, FilterCtl *
     const FoldOut fo{anc, P.zl + F.off, P.sl + F.off, nullptr, nullptr, nullptr, true};
 #pragma unroll 1
     for (uint64_t tt = 0; tt < T; ++tt)
-        fold_tile_layout(ctl, tt, N, P.rep32 + F.off, P.tzs[F.off / kFT + tt], fo, S.L);
+        fold_tile_layout(ctl, tt, N, P.rep32 + F.off, P.rep8 + F.off, P.tzs[F.off / kFT + tt],
+                         fo, S.L);
     __syncthreads();
     if (tid == 0)
         resample_epilogue(ctl, P.hist + F.hist_off, SchemeOf<Inv>::value == kSystematic, N);

@@ -211,6 +211,7 @@ void k_fold_apply(FilterCtl *ctl, XSrc xs, Inv inv, uint64_t M, FoldScratch fs, 
     const Seg next =
         has_next ? seg_op(fs.cpre[(t + 1) / kFoldChunk], fs.pre[t + 1]) : seg_identity();
     fold_tile_apply(ctl, xs, inv, t, M, fs.ts[t], tpre, next, has_next, fs.gval, out, fs.rep32,
+                    fs.rep8,
                     fs.zs + t, L, fs.thr ? fs.thr + t * kBlock : nullptr,
                     fs.thr ? fs.thr_fb + t * kBlock : nullptr);
 }
@@ -246,7 +247,7 @@ __global__ __launch_bounds__(kBlock) void k_fold_layout(FilterCtl *ctl, FoldScra
     if (!ctl->resample) return;
     __shared__ FoldLdsT<false> L;
     const uint64_t t = blockIdx.x;
-    fold_tile_layout(ctl, t, M, fs.rep32, fs.zsx[t], out, L);
+    fold_tile_layout(ctl, t, M, fs.rep32, fs.rep8, fs.zsx[t], out, L);
 }
 
 // ------------------------------------------------------------------------

@@ -159,7 +159,7 @@ __global__ __launch_bounds__(kBlock) void k_tick_apply(const TickItem *items,
     const bool has_next = it.tile + 1 < tiles_of_job(F);
     fold_tile_apply(ctl, xs, inv, it.tile, F.n, P.ts[o], P.tpre[o],
                     has_next ? P.tpre[o + 1] : seg_identity(), has_next, P.gval + F.off, fo,
-                    P.rep32 + F.off,
+                    P.rep32 + F.off, P.rep8 + F.off,
                     P.tzs + o, L, P.thr + o * kBlock, P.thr_fb + o * kBlock);
 }
 
@@ -189,7 +189,8 @@ __global__ __launch_bounds__(kBlock) void k_tick_layout(const TickItem *items,
     // encoded ancestors: the move of the next step reads zero slots' sources
     // from the surplus list, so there is no zero-fill pass
     const FoldOut fo{P.anc + F.off, P.zl + F.off, P.sl + F.off, nullptr, nullptr, nullptr, true};
-    fold_tile_layout(ctl, it.tile, F.n, P.rep32 + F.off, P.tzsx[F.off / kFT + it.tile], fo, L);
+    fold_tile_layout(ctl, it.tile, F.n, P.rep32 + F.off, P.rep8 + F.off,
+                     P.tzsx[F.off / kFT + it.tile], fo, L);
 }
 
 __global__ void k_tick_epilogue(const uint32_t *jobs, uint32_t njobs, const BatchFilter *filters,

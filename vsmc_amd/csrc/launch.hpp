@@ -161,6 +161,7 @@ struct BatchPool {
     double *lw;
     float *lwc;            // compact log-weights (FP32 state only)
     uint32_t *anc, *zl, *sl, *rep32;
+    uint8_t *rep8;         // replication counts as bytes (FoldScratch::rep8)
     SpecRec *spec;
     double *gval;
     double *ts;            // per tile (index off / kFoldTile + tile): approximate start

@@ -51,6 +51,7 @@ constexpr void layout_resample(A &a, ResampleBuffers &rb, uint64_t M, uint64_t n
     fs.spec = a.template n<SpecRec>(tiles * kFoldTile);
     fs.gval = a.template n<double>(F + 1);
     fs.rep32 = a.template n<uint32_t>(M);
+    fs.rep8 = a.template n<uint8_t>((M + kFoldTile - 1) / kFoldTile * kFoldTile);
     fs.zs = a.template n<uint64_t>(tiles);
     fs.zsx = a.template n<uint64_t>(tiles);
     fs.wx = a.template n<double>(M);

@@ -159,7 +159,7 @@ struct vsmcx_serve {
 
     void release_all()
     {
-        void *ps[] = {df, P.st0, P.st1, P.lw, P.lwc, P.anc, P.zl, P.sl, P.rep32,
+        void *ps[] = {df, P.st0, P.st1, P.lw, P.lwc, P.anc, P.zl, P.sl, P.rep32, P.rep8,
                       P.spec, P.gval, P.ts, P.tagg, P.tpre, P.tzs, P.tsum, P.tzsx, P.tpart,
                       P.wx, P.thr, P.thr_fb,
                       stage_d[0], stage_d[1], P.ctl, P.hist, (void *)P.obs,
@@ -459,6 +459,7 @@ int vsmcx_serve_create(vsmcx_ctx *ctx, const vsmcx_serve_desc *d, vsmcx_serve **
         P.zl = ctx->alloc_n<uint32_t>(P_);
         P.sl = ctx->alloc_n<uint32_t>(P_);
         P.rep32 = ctx->alloc_n<uint32_t>(P_);
+        P.rep8 = ctx->alloc_n<uint8_t>(P_);
         P.spec = ctx->alloc_n<SpecRec>(P_);
         P.gval = ctx->alloc_n<double>(P_);
         P.ts = ctx->alloc_n<double>(S->cap_units);
