@@ -91,11 +91,16 @@ struct XResid {
 // so the scaled fold takes no exp per element.
 struct XResidStored {
     const double *r;
-    const uint32_t *ip;
+    const uint8_t *ip8;    // integer parts as bytes, 255: the part is in ip32
+    const uint32_t *ip32;
     double rc;
     __device__ void bind(const FilterCtl *c) { rc = c->rcoeff; }
     __device__ double x(uint64_t i) const { return r[i] * rc; }
-    __device__ uint64_t ipart(uint64_t i) const { return ip[i]; }
+    __device__ uint64_t ipart(uint64_t i) const
+    {
+        const uint32_t b = ip8[i];
+        return b < 255u ? b : ip32[i];
+    }
 };
 
 // ------------------------------------------------------------------------

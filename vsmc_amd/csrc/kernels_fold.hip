@@ -281,10 +281,11 @@ __global__ void k_resample_epilogue(FilterCtl *ctl, StepRow *hist, int systemati
 
 // Residual, the asum fold's tile-sum pass: r_i = modf(N W_i, &i_i)
 // (residual.hpp:57-61) once per element.  The fractions r_i go to wx and the
-// integer parts to rep32, where the scaled fold reads them back (its
-// inversion pass reads every i_i of a tile before it writes the tile's
-// replication counts there); the tile sums of r_i go to tsum and those of i_i
-// to zs, which the start scan consumes before the inversion pass reuses it.
+// integer parts to rep8 as bytes (255: the part itself in rep32), where the
+// scaled fold reads them back (its inversion pass reads every i_i of a tile
+// before it writes the tile's replication counts there); the tile sums of r_i
+// go to tsum and those of i_i to zs, which the start scan consumes before
+// the inversion pass reuses it.
 template <typename W>
 __global__ __launch_bounds__(kBlock) void k_resid_tsum(FilterCtl *ctl, W w, uint64_t M,
                                                        FoldScratch fs)
@@ -308,8 +309,10 @@ __global__ __launch_bounds__(kBlock) void k_resid_tsum(FilterCtl *ctl, W w, uint
         if (i < M) {
             double ip;
             r = modf(xr.nd * wv[k], &ip);
+            const uint32_t iu = (uint32_t)ip;
             fs.wx[i] = r;
-            fs.rep32[i] = (uint32_t)ip;
+            fs.rep8[i] = (uint8_t)(iu < 255u ? iu : 255u);
+            if (iu >= 255u) fs.rep32[i] = iu;
             is += (uint64_t)ip;
         }
         s += r;
@@ -693,7 +696,7 @@ static void resample_pipeline(FilterCtl *ctl, W w, uint64_t M, int scheme, Resam
         // and integer parts from what its tile-sum pass stored
         hipLaunchKernelGGL(k_fold_scan_sum_scaled, dim3(1), dim3(kScanBlock), 0, st, ctl, fs);
         if (side) ia.guess = join_side(rb, st, M, true);
-        const XResidStored xs{fs.wx, fs.rep32, 0.0};
+        const XResidStored xs{fs.wx, fs.rep8, fs.rep32, 0.0};
         if (scheme == kResidual)
             launch_fold_from_starts(ctl, xs, ia, M, fs, out, st, epi, join);
         else if (scheme == kResidualStratified)
