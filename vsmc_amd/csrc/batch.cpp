@@ -136,12 +136,20 @@ struct vsmcx_batch {
                             const std::pair<uint64_t, uint32_t> &c) { return a.first > c.first; });
         jobs_h.clear();
         items_h.clear();
+        // the jobs' first tiles, then their other tiles: the first j jobs'
+        // items are the first j of the one and a prefix of the other
         std::vector<uint64_t> item_end;  // items of the first j jobs
         for (const auto &a : act) {
             jobs_h.push_back(a.second);
+            items_h.push_back(TickItem{a.second, 0});
+        }
+        const size_t nfirst = items_h.size();
+        uint64_t nall = 0;
+        for (const auto &a : act) {
             const uint64_t T = (hf[a.second].n + kFoldTile - 1) / kFoldTile;
-            for (uint64_t tt = 0; tt < T; ++tt) items_h.push_back(TickItem{a.second, (uint32_t)tt});
-            item_end.push_back(items_h.size());
+            for (uint64_t tt = 1; tt < T; ++tt) items_h.push_back(TickItem{a.second, (uint32_t)tt});
+            nall += T;
+            item_end.push_back(nall);
         }
         hipStream_t s = ctx->stream;
         VSMCX_HIP(hipMemcpyAsync(jobs_dev, jobs_h.data(), 4 * jobs_h.size(), hipMemcpyHostToDevice, s));
@@ -150,7 +158,8 @@ struct vsmcx_batch {
         size_t nj = act.size();
         for (uint64_t k = 0; k < act.front().first; ++k) {
             while (nj > 0 && act[nj - 1].first <= k) --nj;  // filters done before tick k
-            launch_tick(fp64, scheme, items_dev, (uint32_t)item_end[nj - 1], jobs_dev, (uint32_t)nj,
+            const TickItems items{items_dev, items_dev + nfirst, (uint32_t)nj};
+            launch_tick(fp64, scheme, items, (uint32_t)item_end[nj - 1], jobs_dev, (uint32_t)nj,
                         df, P, p, lg, s);
         }
         VSMCX_HIP(hipGetLastError());

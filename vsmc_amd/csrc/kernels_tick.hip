@@ -31,7 +31,7 @@ __device__ __forceinline__ uint64_t tiles_of_job(const BatchFilter &F)
 template <typename Real>
 __global__ __launch_bounds__(kBlock)
 __attribute__((amdgpu_waves_per_eu(std::is_same<Real, float>::value ? 5 : 1)))
-void k_tick_move(const TickItem *items,
+void k_tick_move(const TickItems items,
                                                       const BatchFilter *filters, BatchPool P,
                                                       CvParams cv, LgParams lg)
 {
@@ -91,7 +91,7 @@ __global__ __launch_bounds__(kJobBlock) void k_tick_scan_sum(const uint32_t *job
 // kCache: x left in P.wx for the apply pass (the stratified inversion); the
 // systematic apply computes x from the log-weights again.
 template <bool kCache>
-__global__ __launch_bounds__(kBlock) void k_tick_seg(const TickItem *items,
+__global__ __launch_bounds__(kBlock) void k_tick_seg(const TickItems items,
                                                      const BatchFilter *filters, BatchPool P)
 {
     const TickItem it = items[blockIdx.x];
@@ -134,7 +134,7 @@ __global__ __launch_bounds__(kJobBlock) void k_tick_scan_seg(const uint32_t *job
 }
 
 template <typename Inv>
-__global__ __launch_bounds__(kBlock) void k_tick_apply(const TickItem *items,
+__global__ __launch_bounds__(kBlock) void k_tick_apply(const TickItems items,
                                                        const BatchFilter *filters, BatchPool P)
 {
     const TickItem it = items[blockIdx.x];
@@ -178,7 +178,7 @@ __global__ __launch_bounds__(kJobBlock) void k_tick_scan_cnt(const uint32_t *job
     if (threadIdx.x == 0) ctl->zeros = tot >> 32;
 }
 
-__global__ __launch_bounds__(kBlock) void k_tick_layout(const TickItem *items,
+__global__ __launch_bounds__(kBlock) void k_tick_layout(const TickItems items,
                                                         const BatchFilter *filters, BatchPool P)
 {
     const TickItem it = items[blockIdx.x];
@@ -206,7 +206,7 @@ __global__ void k_tick_epilogue(const uint32_t *jobs, uint32_t njobs, const Batc
 
 // ---- the last step's monitor, for jobs that finish in this tick ----------------------
 template <typename Real>
-__global__ __launch_bounds__(kBlock) void k_tick_monitor(const TickItem *items,
+__global__ __launch_bounds__(kBlock) void k_tick_monitor(const TickItems items,
                                                          const BatchFilter *filters, BatchPool P,
                                                          LgParams lg)
 {
@@ -282,7 +282,7 @@ void launch_tick_upload(const TickUpload &u, BatchFilter *filters, const BatchPo
     hipLaunchKernelGGL(k_tick_upload, dim3(g), dim3(64), 0, st, u, filters, P);
 }
 
-void launch_tick(bool fp64, int scheme, const TickItem *items, uint32_t nitems,
+void launch_tick(bool fp64, int scheme, const TickItems &items, uint32_t nitems,
                  const uint32_t *jobs, uint32_t njobs, const BatchFilter *filters,
                  const BatchPool &P, const CvParams &cv, const LgParams &lg, hipStream_t st)
 {

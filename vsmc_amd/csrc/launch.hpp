@@ -188,6 +188,21 @@ struct TickItem {
     uint32_t tile;
 };
 
+// A tick's work items in launch order: the first tile of every job, then the
+// jobs' other tiles job by job.  A job's first tile holds most of its fold's
+// binade crossings (the running sum starts near zero), so its workgroup takes
+// the element-by-element path and runs longest; started first, these
+// workgroups finish inside the launch instead of setting its tail.
+struct TickItems {
+    const TickItem *first;  // [nfirst] tile 0 of each job, in job order
+    const TickItem *rest;   // the other tiles, job by job
+    uint32_t nfirst;
+    __device__ TickItem operator[](uint32_t b) const
+    {
+        return b < nfirst ? first[b] : rest[b - nfirst];
+    }
+};
+
 // One tick's host-to-device traffic, packed into one pinned buffer and sent
 // with one copy: slots (re)initialised for admitted jobs, runs of new
 // observations, and the observation count now on the device per slot.
@@ -220,7 +235,7 @@ void launch_tick_upload(const TickUpload &u, BatchFilter *filters, const BatchPo
                         hipStream_t st);
 
 // kernels_tick.hip: one step of every listed job, one grid launch per phase
-void launch_tick(bool fp64, int scheme, const TickItem *items, uint32_t nitems,
+void launch_tick(bool fp64, int scheme, const TickItems &items, uint32_t nitems,
                  const uint32_t *jobs, uint32_t njobs, const BatchFilter *filters,
                  const BatchPool &P, const CvParams &cv, const LgParams &lg, hipStream_t st);
 

@@ -333,10 +333,11 @@ struct vsmcx_serve {
             const Job &J = jobs[slot_job[slot]];
             if (J.state == 1 && J.uploaded > J.issued) tick.push_back((uint32_t)slot);
         }
-        items_h.clear();
+        items_h.clear();  // the jobs' first tiles, then their other tiles (TickItems)
+        for (uint32_t slot : tick) items_h.push_back(TickItem{slot, 0});
         for (uint32_t slot : tick) {
             const uint64_t T = (jobs[slot_job[slot]].d.n + kUnit - 1) / kUnit;
-            for (uint64_t tt = 0; tt < T; ++tt) items_h.push_back(TickItem{slot, (uint32_t)tt});
+            for (uint64_t tt = 1; tt < T; ++tt) items_h.push_back(TickItem{slot, (uint32_t)tt});
         }
         if (tick.empty() && st_init.empty() && st_runs.empty() && st_avail.empty()) return 0;
         const int b = next_buf;
@@ -368,7 +369,9 @@ struct vsmcx_serve {
         u.n_runs = (uint32_t)st_runs.size();
         u.n_avail = (uint32_t)st_avail.size();
         launch_tick_upload(u, df, P, s);
-        launch_tick(fp64, scheme, (const TickItem *)(sd + o_items), (uint32_t)items_h.size(),
+        const TickItem *items_d = (const TickItem *)(sd + o_items);
+        const TickItems items{items_d, items_d + tick.size(), (uint32_t)tick.size()};
+        launch_tick(fp64, scheme, items, (uint32_t)items_h.size(),
                     (const uint32_t *)(sd + o_jobs), (uint32_t)tick.size(), df, P, cv, lg, s);
         VSMCX_HIP(hipGetLastError());
         // the history rows of the jobs this tick finishes, copied back behind it
