@@ -160,7 +160,7 @@ struct vsmcx_batch {
             while (nj > 0 && act[nj - 1].first <= k) --nj;  // filters done before tick k
             const TickItems items{items_dev, items_dev + nfirst, (uint32_t)nj};
             launch_tick(fp64, scheme, items, (uint32_t)item_end[nj - 1], jobs_dev, (uint32_t)nj,
-                        df, P, p, lg, s);
+                        df, P, p, lg, s, true);
         }
         VSMCX_HIP(hipGetLastError());
     }

@@ -27,10 +27,12 @@ __device__ __forceinline__ uint64_t tiles_of_job(const BatchFilter &F)
 }
 
 // ---- move (all jobs' tiles) ---------------------------------------------------------
-// FP32: compiled for five waves per SIMD (the CV and LG bodies together)
-template <typename Real>
+// kCvOnly: every job of the launch runs the CV model (the batch), so the LG
+// body is not compiled in.  FP32 with both bodies: compiled for five waves per
+// SIMD.
+template <typename Real, bool kCvOnly>
 __global__ __launch_bounds__(kBlock)
-__attribute__((amdgpu_waves_per_eu(std::is_same<Real, float>::value ? 5 : 1)))
+__attribute__((amdgpu_waves_per_eu(std::is_same<Real, float>::value && !kCvOnly ? 5 : 1)))
 void k_tick_move(const TickItems items,
                                                       const BatchFilter *filters, BatchPool P,
                                                       CvParams cv, LgParams lg)
@@ -42,7 +44,7 @@ void k_tick_move(const TickItems items,
     const uint32_t cur = P.cur[it.slot];
     const uint32_t k0 = (uint32_t)F.seed, k1 = (uint32_t)(F.seed >> 32);
     double *part = P.tpart + 5 * (F.off / kFT + it.tile);
-    if (F.model == kModelLg) {
+    if (!kCvOnly && F.model == kModelLg) {
         Real *sa = (Real *)P.st0 + F.soff, *sb = (Real *)P.st1 + F.soff;
         lg_move_tile<Real>(ctl, cur ? sb : sa, cur ? sa : sb, P.anc + F.off, P.sl + F.off, P.lw + F.off,
                            P.obs + F.obs_off + (uint64_t)lg.dim * t, lg, k0, k1, t, F.n, it.tile,
@@ -284,15 +286,20 @@ void launch_tick_upload(const TickUpload &u, BatchFilter *filters, const BatchPo
 
 void launch_tick(bool fp64, int scheme, const TickItems &items, uint32_t nitems,
                  const uint32_t *jobs, uint32_t njobs, const BatchFilter *filters,
-                 const BatchPool &P, const CvParams &cv, const LgParams &lg, hipStream_t st)
+                 const BatchPool &P, const CvParams &cv, const LgParams &lg, hipStream_t st,
+                 bool cv_only)
 {
     if (!nitems || !njobs) return;
     const dim3 gi(nitems), gj(njobs), b(kBlock), bs(kJobBlock), bw(kWave);
     const dim3 gsmall((njobs + 255) / 256), bsmall(256);
-    if (fp64)
-        hipLaunchKernelGGL(k_tick_move<double>, gi, b, 0, st, items, filters, P, cv, lg);
+    if (fp64 && cv_only)
+        hipLaunchKernelGGL((k_tick_move<double, true>), gi, b, 0, st, items, filters, P, cv, lg);
+    else if (fp64)
+        hipLaunchKernelGGL((k_tick_move<double, false>), gi, b, 0, st, items, filters, P, cv, lg);
+    else if (cv_only)
+        hipLaunchKernelGGL((k_tick_move<float, true>), gi, b, 0, st, items, filters, P, cv, lg);
     else
-        hipLaunchKernelGGL(k_tick_move<float>, gi, b, 0, st, items, filters, P, cv, lg);
+        hipLaunchKernelGGL((k_tick_move<float, false>), gi, b, 0, st, items, filters, P, cv, lg);
     hipLaunchKernelGGL(k_tick_reduce, gj, bw, 0, st, jobs, filters, P, scheme);
     hipLaunchKernelGGL(k_tick_scan_sum, gj, bs, 0, st, jobs, filters, P);
     if (scheme == kStratified)

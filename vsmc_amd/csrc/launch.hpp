@@ -235,9 +235,11 @@ void launch_tick_upload(const TickUpload &u, BatchFilter *filters, const BatchPo
                         hipStream_t st);
 
 // kernels_tick.hip: one step of every listed job, one grid launch per phase
+// cv_only: every listed job runs the CV model (the move then has no LG body)
 void launch_tick(bool fp64, int scheme, const TickItems &items, uint32_t nitems,
                  const uint32_t *jobs, uint32_t njobs, const BatchFilter *filters,
-                 const BatchPool &P, const CvParams &cv, const LgParams &lg, hipStream_t st);
+                 const BatchPool &P, const CvParams &cv, const LgParams &lg, hipStream_t st,
+                 bool cv_only = false);
 
 void launch_batch_run(bool fp64, int scheme, const BatchFilter *filters, const uint32_t *order,
                       uint32_t nf, const BatchPool &P, const CvParams &p, const LgParams &lg,
