@@ -48,25 +48,27 @@ float launch_stream_probe(int kind, const float4 *a, float4 *b, uint64_t n, int 
         return -1.f;
     }
     float best = -1.f;
+    bool ok = true;
     for (unsigned grid : {1024u, 4096u}) {
-        for (int r = 0; r <= reps; ++r) {  // the first launch of a shape is not timed
-            if (hipEventRecord(e0, st) != hipSuccess) return -1.f;
+        for (int r = 0; ok && r <= reps; ++r) {  // the first launch of a shape is not timed
+            ok = hipEventRecord(e0, st) == hipSuccess;
+            if (!ok) break;
             if (kind == 0)
                 hipLaunchKernelGGL(k_probe_copy, dim3(grid), dim3(256), 0, st, a, b, n);
             else if (kind == 1)
                 hipLaunchKernelGGL(k_probe_read, dim3(grid), dim3(256), 0, st, a, b, n);
             else
                 hipLaunchKernelGGL(k_probe_write, dim3(grid), dim3(256), 0, st, b, n);
-            if (hipEventRecord(e1, st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess)
-                return -1.f;
             float ms = 0.f;
-            if (hipEventElapsedTime(&ms, e0, e1) != hipSuccess) return -1.f;
-            if (r > 0 && (best < 0.f || ms < best)) best = ms;
+            ok = hipGetLastError() == hipSuccess && hipEventRecord(e1, st) == hipSuccess &&
+                 hipEventSynchronize(e1) == hipSuccess &&
+                 hipEventElapsedTime(&ms, e0, e1) == hipSuccess;
+            if (ok && r > 0 && (best < 0.f || ms < best)) best = ms;
         }
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    return best;
+    return ok ? best : -1.f;
 }
 
 }  // namespace vsmcx
