@@ -24,8 +24,12 @@
  *     rational check on tiny inputs
  *   - CV particle filter run              : golden run driven through
  *     vsmc::Particle<T>, WeightSet and Resample<Systematic>
- *   - GMM tempered SMC                    : parity unpinned beyond its pieces
- *     (log-likelihood closed form, CESS definition); see DESIGN.md
+ *   - GMM tempered SMC                    : the tempering search (CESS and
+ *     alpha_ess bisection) equals the compiled move_alpha.hpp and WeightSet
+ *     (golden fixtures and the live reference harness); log-likelihood by
+ *     closed form; log evidence by quadrature for K = 1.  Initial draws and
+ *     MH normals come from other engines than the reference's per-particle
+ *     std distributions, so the whole run is pinned statistically only.
  *   - Kalman filter                       : closed form, checked against a
  *     brute-force Gaussian marginal on tiny T
  */
