@@ -339,7 +339,9 @@ int vsmcx_stream_probe(vsmcx_ctx *ctx, uint64_t bytes, int reps, double gbps[3])
         ms[k] = launch_stream_probe(k, a, b, n, reps, ctx->stream);
         if (ms[k] <= 0.f) e = hipErrorUnknown;
     }
-    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    // nothing may still use the buffers when they are released, error or not
+    const hipError_t es = hipStreamSynchronize(ctx->stream);
+    if (e == hipSuccess) e = es;
     ctx->release(a);
     ctx->release(b);
     VSMCX_HIP(e);
